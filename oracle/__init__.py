@@ -1,0 +1,289 @@
+"""ctypes wrapper of the CPU FP64 oracle (test infrastructure only).
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg import
+this package; it is the checker, never the product path.  It loads
+``oracle/liboracle.so`` (the restatement in mdoracle.c) and, where it was
+built, ``oracle/_ref/libmdbas_ref.so`` (the reference's own sources).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libmdbas_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OBox(C.Structure):
+    _fields_ = [("type", C.c_int)] + [(k, C.c_double) for k in (
+        "a1 a2 a3 b1 b2 b3 c1 c2 c3 u1 u2 u3 v1 v2 v3 w1 w2 w3 pa pb pc u v w det vol".split())]
+
+
+class OParam(C.Structure):
+    _fields_ = [("nAtom", C.c_int), ("cutOff", C.c_double), ("cutOn", C.c_double),
+                ("delr", C.c_double), ("chargeConst", C.c_double), ("alpha", C.c_double),
+                ("m1", C.c_int), ("m2", C.c_int), ("m3", C.c_int), ("nbsp", C.c_int),
+                ("timeStep", C.c_double), ("tolShake", C.c_double),
+                ("maxCycle", C.c_int), ("update", C.c_int), ("linkRatio", C.c_int),
+                ("cutOff2", C.c_double), ("cutOn2", C.c_double), ("switch2", C.c_double),
+                ("rcutOff", C.c_double), ("rcutOff2", C.c_double), ("rTimeStep", C.c_double)]
+
+
+class OEnergy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                "eDir eCorr eRec eVdw virDir virCorr virRec virVdw".split()] + [
+                ("stressRec", C.c_double * 6), ("nInCut", C.c_int), ("nListed", C.c_int)]
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so (and the reference library when its sources are
+    present) with the committed Makefile."""
+    target = "all" if ref else "oracle"
+    subprocess.run(["make", "-s", "-C", HERE, target], check=True)
+
+
+_LIB = None
+_REF = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build(ref=False)
+        L = C.CDLL(LIB_PATH)
+        L.oc_spme.restype = C.c_double
+        L.oc_ewald_classic.restype = C.c_double
+        L.oc_dist.restype = C.c_double
+        L.oc_ewald_dir.restype = C.c_double
+        L.oc_ewald_corr.restype = C.c_double
+        L.oc_vdw_switch.restype = C.c_double
+        L.oc_verlet_list.restype = C.c_long
+        L.oc_link_cell_list.restype = C.c_long
+        _LIB = L
+    return _LIB
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _REF
+    if _REF is None:
+        R = C.CDLL(REF_PATH)
+        for f in ("ref_ewald_dir", "ref_ewald_corr", "ref_vdw_switch", "ref_coulomb",
+                  "ref_ewald_rec"):
+            getattr(R, f).restype = C.c_double
+        R.ref_create.restype = C.c_void_p
+        R.ref_list_count.restype = C.c_long
+        for f in ("ref_makelist", "ref_list_count", "ref_list_copy", "ref_excl_copy",
+                  "ref_vdw_switch", "ref_coulomb", "ref_nonbond", "ref_ewald_rec", "ref_vv",
+                  "ref_constvel"):
+            getattr(R, f).argtypes = None
+        _REF = R
+    return _REF
+
+
+def P(a):
+    """ctypes pointer to a contiguous numpy array (double or int32)."""
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    raise TypeError(a.dtype)
+
+
+def make_box(L: float) -> OBox:
+    b = OBox()
+    b.type = 1
+    b.a1 = b.b2 = b.c3 = float(L)
+    lib().oc_init_box(C.byref(b))
+    return b
+
+
+def make_param(cfg: dict, n: int) -> OParam:
+    p = OParam()
+    p.nAtom = n
+    p.cutOff, p.cutOn, p.delr = cfg["cutOff"], cfg["cutOn"], cfg["delr"]
+    p.chargeConst = 332.0716 * 418.4
+    p.alpha = cfg["alpha"]
+    p.m1 = p.m2 = p.m3 = cfg["grid"]
+    p.nbsp = cfg["nbsp"]
+    p.timeStep = cfg["timeStep"]
+    p.tolShake = cfg["tolShake"]
+    p.maxCycle, p.update, p.linkRatio = cfg["maxCycle"], cfg["update"], cfg["linkRatio"]
+    lib().oc_param_derive(C.byref(p))
+    return p
+
+
+class OracleSystem:
+    """One water box with oracle-side exclusions and list state."""
+
+    def __init__(self, wb, cfg: dict | None = None):
+        self.wb = wb
+        self.cfg = dict(wb.params) if cfg is None else cfg
+        self.p = make_param(self.cfg, wb.n)
+        self.b = make_box(wb.L)
+        self.keyLink = lib().oc_choose_link(C.byref(self.p), C.byref(self.b))
+        self.exOff = np.zeros(wb.n + 1, np.int32)
+        cap = 8 * len(wb.ca) + 16
+        self.exLst = np.zeros(cap, np.int32)
+        ne = lib().oc_exclusions(wb.n, 0, None, None, len(wb.ca), P(wb.ca), P(wb.cb), 0, None,
+                                 None, None, int(self.keyLink), P(self.exOff), P(self.exLst), cap)
+        if ne < 0:
+            raise RuntimeError(f"oracle exclusions failed: {ne}")
+        self.exLst = self.exLst[:ne].copy()
+        self.neighList = None
+
+    def build_list(self, x, y, z, build: bool = True):
+        """Reference-format list; x,y,z may be wrapped in place (link cells,
+        first build) exactly as list.c does."""
+        n = self.wb.n
+        np_ = np.zeros(n, np.int32)
+        no = np.zeros(n, np.int32)
+        cap = 1
+        nl = np.zeros(cap, np.int32)
+        for _ in range(2):
+            if self.keyLink:
+                cnt = lib().oc_link_cell_list(C.byref(self.p), C.byref(self.b), int(build),
+                                              P(x), P(y), P(z), P(self.wb.frozen), P(self.exOff),
+                                              P(self.exLst), P(nl), C.c_long(cap), P(np_), P(no))
+            else:
+                cnt = lib().oc_verlet_list(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z),
+                                           P(self.wb.frozen), P(self.exOff), P(self.exLst),
+                                           P(nl), C.c_long(cap), P(np_), P(no))
+            if cnt < 0:
+                raise RuntimeError(f"oracle list failed: {cnt}")
+            if cnt <= cap:
+                break
+            cap = int(cnt)
+            nl = np.zeros(cap, np.int32)
+            build = False
+        self.neighList, self.neighPair, self.neighOrder = nl[:cnt], np_, no
+        return self.neighList, self.neighPair, self.neighOrder
+
+    def ewald_real(self, x, y, z, f=None):
+        n = self.wb.n
+        f = np.zeros((3, n)) if f is None else f
+        e = OEnergy()
+        lib().oc_ewald_real(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z),
+                            P(f[0]), P(f[1]), P(f[2]), P(self.wb.q), P(self.wb.eps),
+                            P(self.wb.sig), P(self.neighList), P(self.neighPair),
+                            P(self.neighOrder), P(self.exOff), P(self.exLst), C.byref(e))
+        return f, e
+
+    def spme(self, x, y, z, f=None):
+        n = self.wb.n
+        f = np.zeros((3, n)) if f is None else f
+        st = np.zeros(6)
+        vir = C.c_double(0)
+        e = lib().oc_spme(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z),
+                          P(f[0]), P(f[1]), P(f[2]), P(self.wb.q), P(st), C.byref(vir))
+        return e, f, st, vir.value
+
+    def energy(self, x, y, z):
+        """energy() (energy.c:103-205) for the Ewald path: zero forces, SPME,
+        real-space pairs, correction.  Returns forces (3,N) and a dict."""
+        eRec, f, st, vr = self.spme(x, y, z)
+        f, e = self.ewald_real(x, y, z, f)
+        elec = e.eDir + eRec + e.eCorr
+        return f, dict(elec=elec, vdw=e.eVdw, pot=elec + e.eVdw, eDir=e.eDir, eRec=eRec,
+                       eCorr=e.eCorr, virelec=e.virDir + vr + e.virCorr, virvdw=e.virVdw,
+                       nInCut=e.nInCut, nListed=e.nListed, stressRec=st)
+
+    def vv(self, stage, x, y, z, vx, vy, vz, f):
+        ek = C.c_double(0)
+        vs = C.c_double(0)
+        nc = C.c_int(0)
+        wb = self.wb
+        err = lib().oc_vv_nve(C.byref(self.p), C.byref(self.b), int(stage), len(wb.ca), P(wb.ca),
+                              P(wb.cb), P(wb.rc2), P(x), P(y), P(z), P(vx), P(vy), P(vz),
+                              P(f[0]), P(f[1]), P(f[2]), P(wb.mass), P(wb.rmass),
+                              P(wb.nAtConst), C.byref(ek), C.byref(vs), C.byref(nc))
+        return err, ek.value, vs.value, nc.value
+
+    def md_run(self, nsteps, x, y, z, vx, vy, vz):
+        n = self.wb.n
+        f = np.zeros((3, n))
+        et, ep, ek = (np.zeros(nsteps + 1) for _ in range(3))
+        sec = C.c_double(0)
+        wb = self.wb
+        err = lib().oc_md_run(C.byref(self.p), C.byref(self.b), int(self.keyLink), int(nsteps),
+                              P(x), P(y), P(z), P(vx), P(vy), P(vz), P(f[0]), P(f[1]), P(f[2]),
+                              P(wb.q), P(wb.eps), P(wb.sig), P(wb.mass), P(wb.rmass),
+                              P(wb.frozen), P(wb.nAtConst), len(wb.ca), P(wb.ca), P(wb.cb),
+                              P(wb.rc2), P(et), P(ep), P(ek), C.byref(sec))
+        return err, f, et, ep, ek, sec.value
+
+
+class RefSystem:
+    """The compiled reference (oracle/_ref) driven on the same arrays."""
+
+    def __init__(self, wb, cfg: dict | None = None, noLink: int = 0):
+        cfg = dict(wb.params) if cfg is None else cfg
+        self.wb, self.cfg = wb, cfg
+        R = ref()
+        R.ref_create.argtypes = [C.c_int, C.c_int, _dp, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_int, C.c_int, _ip, _ip, _dp, _ip, _ip]
+        self.box9 = wb.box9
+        self.h = C.c_void_p(R.ref_create(wb.n, 1, P(self.box9), cfg["cutOff"], cfg["cutOn"],
+                                         cfg["delr"], cfg["alpha"], cfg["linkRatio"],
+                                         cfg["update"], noLink, cfg["timeStep"],
+                                         cfg["tolShake"], cfg["maxCycle"], len(wb.ca), P(wb.ca),
+                                         P(wb.cb), P(wb.rc2), P(wb.frozen), P(wb.nAtConst)))
+
+    def makelist(self, x, y, z, step=0):
+        R = ref()
+        keylink = R.ref_makelist(self.h, P(x), P(y), P(z), C.c_int(step))
+        cnt = R.ref_list_count(self.h)
+        n = self.wb.n
+        nl = np.zeros(max(cnt, 1), np.int32)
+        np_ = np.zeros(n, np.int32)
+        no = np.zeros(n, np.int32)
+        R.ref_list_copy(self.h, P(nl), P(np_), P(no))
+        return keylink, nl[:cnt], np_, no
+
+    def exclusions(self):
+        n = self.wb.n
+        cap = 8 * len(self.wb.ca) + 16
+        off = np.zeros(n + 1, np.int32)
+        lst = np.zeros(cap, np.int32)
+        ne = ref().ref_excl_copy(self.h, P(off), P(lst), C.c_int(cap))
+        return off, lst[:ne]
+
+    def nonbond(self, elecType, x, y, z):
+        n = self.wb.n
+        f = np.zeros((3, n))
+        out = np.zeros(4)
+        ref().ref_nonbond(self.h, C.c_int(elecType), P(x), P(y), P(z), P(f[0]), P(f[1]),
+                          P(f[2]), P(self.wb.q), P(self.wb.eps), P(self.wb.sig), P(out))
+        return f, out
+
+    def ewald_rec(self, kmax, x, y, z):
+        n = self.wb.n
+        f = np.zeros((3, n))
+        st = np.zeros(6)
+        vir = C.c_double(0)
+        e = ref().ref_ewald_rec(self.h, C.c_int(kmax), P(x), P(y), P(z), P(f[0]), P(f[1]),
+                                P(f[2]), P(self.wb.q), P(st), C.byref(vir))
+        return e, f, st, vir.value
+
+    def vv(self, stage, x, y, z, vx, vy, vz, f):
+        out = np.zeros(2)
+        ref().ref_vv(self.h, C.c_int(stage), P(x), P(y), P(z), P(vx), P(vy), P(vz),
+                     P(f[0]), P(f[1]), P(f[2]), P(self.wb.mass), P(self.wb.rmass), P(out))
+        return out
+
+    def constvel(self, x, y, z, vx, vy, vz):
+        ref().ref_constvel(self.h, P(x), P(y), P(z), P(vx), P(vy), P(vz), P(self.wb.mass))

@@ -1,0 +1,1130 @@
+/*
+ * mdoracle.c -- CPU FP64 oracle (test infrastructure only; see mdoracle.h).
+ *
+ * A from-scratch restatement of the MDBas nonbonded path.  Citations are
+ * to the reference sources (src/*.c, include/*.h).  Compile with
+ * -ffp-contract=off: the list predicates must round like the reference.
+ */
+#include "mdoracle.h"
+
+#include <complex.h>
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define SQ(x) ((x) * (x))
+#define OC_PI 3.141592653589793       /* global.h:52 */
+#define OC_TWOPI 6.283185307179586    /* global.h:53 */
+#define OC_SQRTPI 1.772453850905516   /* global.h:54 */
+#define OC_TOLLIST 0.01               /* global.h:59 */
+
+typedef double complex ocplx;
+
+/* io.c:714-723 */
+void oc_param_derive(oc_param *p)
+{
+    p->rcutOff = 1. / p->cutOff;
+    p->cutOff2 = SQ(p->cutOff);
+    p->rcutOff2 = 1. / p->cutOff2;
+    p->cutOn2 = SQ(p->cutOn);
+    p->switch2 = 1. / (SQ(p->cutOff2 - p->cutOn2) * (p->cutOff2 - p->cutOn2));
+    p->rTimeStep = 1. / p->timeStep;
+}
+
+/* init.c:840-908: norms, reciprocal vectors (divided by det), perpendicular
+ * widths pa/pb/pc and the volume. */
+void oc_init_box(oc_box *b)
+{
+    b->u1 = b->b2 * b->c3 - b->c2 * b->b3;
+    b->u2 = b->c1 * b->b3 - b->b1 * b->c3;
+    b->u3 = b->b1 * b->c2 - b->c1 * b->b2;
+    b->v1 = b->c2 * b->a3 - b->a2 * b->c3;
+    b->v2 = b->a1 * b->c3 - b->c1 * b->a3;
+    b->v3 = b->c1 * b->a2 - b->a1 * b->c2;
+    b->w1 = b->a2 * b->b3 - b->b2 * b->a3;
+    b->w2 = b->b1 * b->a3 - b->a1 * b->b3;
+    b->w3 = b->a1 * b->b2 - b->b1 * b->a2;
+    b->det = b->a1 * b->u1 + b->a2 * b->u2 + b->a3 * b->u3;
+    b->vol = fabs(b->det);
+    b->pa = b->vol / sqrt(SQ(b->u1) + SQ(b->u2) + SQ(b->u3));
+    b->pb = b->vol / sqrt(SQ(b->v1) + SQ(b->v2) + SQ(b->v3));
+    b->pc = b->vol / sqrt(SQ(b->w1) + SQ(b->w2) + SQ(b->w3));
+    b->u = 1.0 / b->pa;
+    b->v = 1.0 / b->pb;
+    b->w = 1.0 / b->pc;
+    if (fabs(b->det) > 0.) {
+        b->u1 /= b->det; b->u2 /= b->det; b->u3 /= b->det;
+        b->v1 /= b->det; b->v2 /= b->det; b->v3 /= b->det;
+        b->w1 /= b->det; b->w2 /= b->det; b->w3 /= b->det;
+    } else {
+        b->u1 = b->u2 = b->u3 = b->v1 = b->v2 = b->v3 = b->w1 = b->w2 = b->w3 = 0.;
+    }
+}
+
+/* utils.c:606-609: round half away from zero through a C truncating cast. */
+int oc_nint(double x)
+{
+    return x >= 0. ? (int)(x + 0.5) : (int)(x - 0.5);
+}
+
+/* utils.c:38-80: minimum image of d in place, returns |d|^2 summed as
+ * (dx^2 + dy^2) + dz^2.  The cubic branch uses a1/u1 on all three axes. */
+double oc_dist(const oc_box *b, double d[3])
+{
+    double xt, yt, zt;
+    switch (b->type) {
+    case 1:
+        d[0] -= b->a1 * oc_nint(d[0] * b->u1);
+        d[1] -= b->a1 * oc_nint(d[1] * b->u1);
+        d[2] -= b->a1 * oc_nint(d[2] * b->u1);
+        break;
+    case 2:
+        d[0] -= b->a1 * oc_nint(d[0] * b->u1);
+        d[1] -= b->b2 * oc_nint(d[1] * b->v2);
+        d[2] -= b->c3 * oc_nint(d[2] * b->w3);
+        break;
+    case 3:
+        xt = d[0] * b->u1 + d[1] * b->u2 + d[2] * b->u3;
+        yt = d[0] * b->v1 + d[1] * b->v2 + d[2] * b->v3;
+        zt = d[0] * b->w1 + d[1] * b->w2 + d[2] * b->w3;
+        xt -= oc_nint(xt);
+        yt -= oc_nint(yt);
+        zt -= oc_nint(zt);
+        d[0] = xt * b->a1 + yt * b->b1 + zt * b->c1;
+        d[1] = xt * b->a2 + yt * b->b2 + zt * b->c2;
+        d[2] = xt * b->a3 + yt * b->b3 + zt * b->c3;
+        break;
+    default:
+        break;
+    }
+    return SQ(d[0]) + SQ(d[1]) + SQ(d[2]);
+}
+
+/* utils.c:162-218: wrap coordinates in place. */
+void oc_image_update(const oc_box *b, int n, double *x, double *y, double *z)
+{
+    for (int i = 0; i < n; i++) {
+        double d[3] = {x[i], y[i], z[i]};
+        switch (b->type) {
+        case 1:
+            x[i] = x[i] - b->a1 * oc_nint(x[i] * b->u1);
+            y[i] = y[i] - b->a1 * oc_nint(y[i] * b->u1);
+            z[i] = z[i] - b->a1 * oc_nint(z[i] * b->u1);
+            break;
+        case 2:
+            x[i] = x[i] - b->a1 * oc_nint(x[i] * b->u1);
+            y[i] = y[i] - b->b2 * oc_nint(y[i] * b->v2);
+            z[i] = z[i] - b->c3 * oc_nint(z[i] * b->w3);
+            break;
+        case 3: {
+            double xt = d[0] * b->u1 + d[1] * b->u2 + d[2] * b->u3;
+            double yt = d[0] * b->v1 + d[1] * b->v2 + d[2] * b->v3;
+            double zt = d[0] * b->w1 + d[1] * b->w2 + d[2] * b->w3;
+            xt -= oc_nint(xt);
+            yt -= oc_nint(yt);
+            zt -= oc_nint(zt);
+            x[i] = xt * b->a1 + yt * b->b1 + zt * b->c1;
+            y[i] = xt * b->a2 + yt * b->b2 + zt * b->c2;
+            z[i] = xt * b->a3 + yt * b->b3 + zt * b->c3;
+            break;
+        }
+        default:
+            break;
+        }
+    }
+}
+
+/* list.c:38-57: link cells only when every axis has >= 3 cells and there
+ * are more than 27 cells. */
+int oc_choose_link(const oc_param *p, const oc_box *b)
+{
+    int ratio = p->linkRatio < 1 ? 1 : (p->linkRatio > 5 ? 5 : p->linkRatio);
+    double cutnb = p->cutOff + p->delr;
+    int nx = (int)(b->pa * (double)ratio / cutnb);
+    int ny = (int)(b->pb * (double)ratio / cutnb);
+    int nz = (int)(b->pc * (double)ratio / cutnb);
+    if (nx < 3 || ny < 3 || nz < 3) return 0;
+    if (nx * ny * nz <= 27) return 0;
+    return 1;
+}
+
+/* ---------------------------------------------------------------- */
+/* Exclusions: list.c:108-894 (ordered) and 1125-1791 (symmetric).   */
+/* ---------------------------------------------------------------- */
+
+typedef struct { int *v; int n, cap; } ivec;
+
+static void iv_push(ivec *a, int x)
+{
+    if (a->n == a->cap) {
+        a->cap = a->cap ? 2 * a->cap : 8;
+        a->v = (int *)realloc(a->v, (size_t)a->cap * sizeof(int));
+    }
+    a->v[a->n++] = x;
+}
+
+static int iv_has(const ivec *a, int x)
+{
+    for (int k = 0; k < a->n; k++)
+        if (a->v[k] == x) return 1;
+    return 0;
+}
+
+/* Adds pair (i,j) to the exclusion table.  In ordered mode the pair is kept
+ * at the smaller index (list.c:141-150); in symmetric mode at both
+ * (list.c:1159-1181).  dedupe mirrors the reference: bonds and constraints
+ * are appended unconditionally, angle and walk pairs only if absent. */
+static void ex_add(ivec *ex, int i, int j, int sym, int dedupe)
+{
+    int ii = i, jj = j;
+    if (!sym && jj < ii) { ii = j; jj = i; }
+    if (dedupe && iv_has(&ex[ii], jj)) return;
+    iv_push(&ex[ii], jj);
+    if (sym) iv_push(&ex[jj], ii);
+}
+
+int oc_exclusions(int n, int nBond, const int *ba, const int *bb,
+                  int nConst, const int *ca, const int *cb,
+                  int nAngle, const int *aa, const int *ab, const int *ac,
+                  int sym, int *off, int *lst, int cap)
+{
+    ivec *ex = (ivec *)calloc((size_t)n, sizeof(ivec));
+    ivec *con = (ivec *)calloc((size_t)n, sizeof(ivec));
+    int err = 0;
+
+    for (int i = 0; i < nBond; i++) {                  /* list.c:137-174 */
+        ex_add(ex, ba[i], bb[i], sym, 0);
+        iv_push(&con[ba[i]], bb[i]);
+        iv_push(&con[bb[i]], ba[i]);
+    }
+    for (int i = 0; i < nConst; i++) {                 /* list.c:176-216 */
+        ex_add(ex, ca[i], cb[i], sym, 0);
+        iv_push(&con[ca[i]], cb[i]);
+        iv_push(&con[cb[i]], ca[i]);
+    }
+    for (int i = 0; i < nAngle; i++) {                 /* list.c:218-326 */
+        ex_add(ex, aa[i], ab[i], sym, 1);
+        ex_add(ex, aa[i], ac[i], sym, 1);
+        ex_add(ex, ab[i], ac[i], sym, 1);
+    }
+    /* Connectivity walk, list.c:764-862.  A 1-4 path with no dihedral
+     * parameters is DIHE_NONPARAM_ERROR (list.c:835-836). */
+    for (int ia = 0; ia < n && !err; ia++) {
+        for (int j = 0; j < con[ia].n && !err; j++) {
+            int ib = con[ia].v[j];
+            for (int k = 0; k < con[ib].n && !err; k++) {
+                int ic = con[ib].v[k];
+                if (ic == ia) continue;
+                ex_add(ex, ia, ic, sym, 1);
+                for (int l = 0; l < con[ic].n; l++) {
+                    int id = con[ic].v[l];
+                    if (id == ia || id == ib) continue;
+                    err = 110;
+                    break;
+                }
+            }
+        }
+    }
+
+    int total = 0;
+    for (int i = 0; i < n; i++) total += ex[i].n;
+    if (!err && !sym && n > 0 && ex[n - 1].n != 0) err = 111;   /* list.c:877 */
+    int ret;
+    if (err) {
+        ret = -err;
+    } else if (total > cap) {
+        ret = -total - 1000;
+    } else {
+        int o = 0;
+        for (int i = 0; i < n; i++) {
+            off[i] = o;
+            for (int k = 0; k < ex[i].n; k++) lst[o++] = ex[i].v[k];
+        }
+        off[n] = o;
+        ret = total;
+    }
+    for (int i = 0; i < n; i++) { free(ex[i].v); free(con[i].v); }
+    free(ex);
+    free(con);
+    return ret;
+}
+
+static int is_excluded(const int *exOff, const int *exLst, int i, int j)
+{
+    for (int k = exOff[i]; k < exOff[i + 1]; k++)
+        if (exLst[k] == j) return 1;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- */
+/* O(N^2) Verlet list, list.c:896-1123.                              */
+/* ---------------------------------------------------------------- */
+long oc_verlet_list(const oc_param *p, const oc_box *b,
+                    const double *x, const double *y, const double *z,
+                    const int *frozen, const int *exOff, const int *exLst,
+                    int *neighList, long cap, int *neighPair, int *neighOrder)
+{
+    int n = p->nAtom;
+    double cutnb2 = p->cutOff + p->delr;
+    cutnb2 = SQ(cutnb2);
+    long offset = 0;
+    for (int i = 0; i < n; i++) { neighPair[i] = 0; neighOrder[i] = i; }
+    for (int i = 0; i < n - 1; i++) {
+        for (int j = i + 1; j < n; j++) {
+            double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+            double r2 = oc_dist(b, d);
+            if (r2 <= cutnb2) {
+                int excl = (frozen[i] * frozen[j]) ? 1 : is_excluded(exOff, exLst, i, j);
+                if (!excl) {
+                    if (offset < cap) neighList[offset] = j;
+                    neighPair[i]++;
+                    offset++;
+                }
+            }
+        }
+    }
+    return offset;
+}
+
+/* ---------------------------------------------------------------- */
+/* Link-cell list, list.c:1793-2619.                                 */
+/* ---------------------------------------------------------------- */
+
+/* The first 14 entries of the transx/transy/transz tables
+ * (list.c:1797-1855): the half shell searched when linkRatio is 1. */
+static const int LC_TX[14] = {0, 1, 0, 0, -1, 1, 0, -1, 1, 0, -1, 1, -1, 1};
+static const int LC_TY[14] = {0, 0, 1, 0, 1, 1, -1, 0, 0, 1, -1, -1, 1, 1};
+static const int LC_TZ[14] = {0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1};
+
+long oc_link_cell_list(const oc_param *p, const oc_box *b, int build,
+                       double *x, double *y, double *z,
+                       const int *frozen, const int *exOff, const int *exLst,
+                       int *neighList, long cap, int *neighPair, int *neighOrder)
+{
+    int n = p->nAtom;
+    double cutnb = p->cutOff + p->delr;
+    double cutnb2 = SQ(cutnb);
+    if (p->linkRatio != 1) return -412;   /* only the 14-cell half shell */
+    int nlcx = (int)(b->pa * 1.0 / cutnb);
+    int nlcy = (int)(b->pb * 1.0 / cutnb);
+    int nlcz = (int)(b->pc * 1.0 / cutnb);
+    if (nlcx < 3 || nlcy < 3 || nlcz < 3) return -412;   /* list.c:1897-1905 */
+    double dnx = (double)nlcx, dny = (double)nlcy, dnz = (double)nlcz;
+    int ncell = nlcx * nlcy * nlcz;
+
+    if (build) oc_image_update(b, n, x, y, z);           /* list.c:1927 */
+
+    double *xu = (double *)malloc((size_t)n * sizeof(double));
+    double *yu = (double *)malloc((size_t)n * sizeof(double));
+    double *zu = (double *)malloc((size_t)n * sizeof(double));
+    int *link = (int *)malloc((size_t)n * sizeof(int));
+    int *head = (int *)malloc((size_t)ncell * sizeof(int));
+    for (int k = 0; k < ncell; k++) head[k] = -1;
+
+    for (int i = 0; i < n; i++) {                          /* list.c:1932-1937 */
+        xu[i] = (x[i] * b->u1 + y[i] * b->u2 + z[i] * b->u3) + 0.5;
+        yu[i] = (x[i] * b->v1 + y[i] * b->v2 + z[i] * b->v3) + 0.5;
+        zu[i] = (x[i] * b->w1 + y[i] * b->w2 + z[i] * b->w3) + 0.5;
+    }
+    for (int i = 0; i < n; i++) {                          /* list.c:1942-1967 */
+        /* (int)dnlcx * xu: the cast binds to dnlcx, the product truncates
+         * on assignment. */
+        int ix = (int)(dnx * xu[i]);
+        int iy = (int)(dny * yu[i]);
+        int iz = (int)(dnz * zu[i]);
+        if (ix > nlcx - 1) ix = nlcx - 1;
+        if (iy > nlcy - 1) iy = nlcy - 1;
+        if (iz > nlcz - 1) iz = nlcz - 1;
+        int ic = ix + nlcx * (iy + nlcy * iz);
+        link[i] = head[ic];
+        head[ic] = i;
+    }
+    int io = 0;                                            /* list.c:1969-1980 */
+    for (int k = 0; k < ncell; k++)
+        for (int i = head[k]; i != -1; i = link[i]) neighOrder[io++] = i;
+    for (int i = 0; i < n; i++) neighPair[i] = 0;
+
+    long offset = 0;
+    int ix = 0, iy = 0, iz = 0;
+    for (int k = 0; k < ncell; k++) {                      /* list.c:2130-2261 */
+        for (int i = head[k]; i != -1; i = link[i]) {
+            for (int l = 0; l < 14; l++) {
+                double cx = 0., cy = 0., cz = 0.;
+                int jx = ix + LC_TX[l], jy = iy + LC_TY[l], jz = iz + LC_TZ[l];
+                if (jx > nlcx - 1) { jx -= nlcx; cx = 1.; } else if (jx < 0) { jx += nlcx; cx = -1.; }
+                if (jy > nlcy - 1) { jy -= nlcy; cy = 1.; } else if (jy < 0) { jy += nlcy; cy = -1.; }
+                if (jz > nlcz - 1) { jz -= nlcz; cz = 1.; } else if (jz < 0) { jz += nlcz; cz = -1.; }
+                int ll = jx + nlcx * (jy + nlcy * jz);
+                int j = (k == ll) ? link[i] : head[ll];
+                for (; j != -1; j = link[j]) {
+                    double xt = xu[j] - xu[i] + cx;
+                    double yt = yu[j] - yu[i] + cy;
+                    double zt = zu[j] - zu[i] + cz;
+                    double xd = xt * b->a1 + yt * b->b1 + zt * b->c1;
+                    double yd = xt * b->a2 + yt * b->b2 + zt * b->c2;
+                    double zd = xt * b->a3 + yt * b->b3 + zt * b->c3;
+                    double r2 = SQ(xd) + SQ(yd) + SQ(zd);
+                    if (r2 <= cutnb2) {
+                        int excl = (frozen[i] * frozen[j]) ? 1 : is_excluded(exOff, exLst, i, j);
+                        if (!excl) {
+                            if (offset < cap) neighList[offset] = j;
+                            neighPair[i]++;
+                            offset++;
+                        }
+                    }
+                }
+            }
+        }
+        if (++ix > nlcx - 1) {
+            ix = 0;
+            if (++iy > nlcy - 1) { iy = 0; iz++; }
+        }
+    }
+    free(xu); free(yu); free(zu); free(link); free(head);
+    return offset;
+}
+
+/* ---------------------------------------------------------------- */
+/* Pair functors.                                                    */
+/* ---------------------------------------------------------------- */
+
+/* ewald.c:356-369 */
+double oc_ewald_dir(double alpha, double qel, double r, double rt, double *dE)
+{
+    double ar = alpha * r;
+    double e = qel * erfc(ar) * rt;
+    double d = 2.0 * qel * alpha * exp(-SQ(ar)) / OC_SQRTPI;
+    *dE = -(e + d) * rt;
+    return e;
+}
+
+/* ewald.c:371-384 */
+double oc_ewald_corr(double alpha, double qel, double r, double rt, double *dE)
+{
+    double ar = alpha * r;
+    double e = -qel * erf(ar) * rt;
+    double d = 2.0 * qel * alpha * exp(-SQ(ar)) / OC_SQRTPI;
+    *dE = -(e + d) * rt;
+    return e;
+}
+
+/* vdw.c:144-184: 4 eps ((s/r)^12 - (s/r)^6) with the CHARMM switch on
+ * (ron, rc]; no rc test inside. */
+double oc_vdw_switch(const oc_param *p, double veps, double vsig, double r2,
+                     double rt, double *dE)
+{
+    double s6 = vsig * rt;
+    s6 = SQ(s6) * s6 * (SQ(s6) * s6);
+    double s12 = SQ(s6);
+    if (r2 <= p->cutOn2) {
+        *dE = 24. * veps * rt * (s6 - 2. * s12);
+        return 4. * veps * (s12 - s6);
+    }
+    double sw1 = p->cutOff2 - r2;
+    double S = SQ(sw1) * (p->cutOff2 + 2. * r2 - 3. * p->cutOn2) * p->switch2;
+    double dS = 12. * r2 * sw1 * (p->cutOn2 - r2) * p->switch2;
+    double pv = 4. * veps * (s12 - s6);
+    double dpv = 24. * veps * (s6 - 2. * s12);
+    *dE = (pv * dS + dpv * S) * rt;
+    return pv * S;
+}
+
+/* elec.c:137-263: SHIFT1, SHIFT2, SWITCH Coulomb functors. */
+static double oc_coulomb(const oc_param *p, int type, double qel, double r2,
+                         double rt, double *dE)
+{
+    double pe = p->chargeConst * qel * rt;
+    if (type == 2) {
+        double s1 = rt * r2 * p->rcutOff, s2 = r2 * p->rcutOff2;
+        double f = 1. - 2. * s1 + s2, df = 2. * (s2 - s1);
+        *dE = pe * rt * (df - f);
+        return pe * f;
+    }
+    if (type == 3) {
+        double s1 = r2 * p->rcutOff2, s2 = SQ(s1);
+        double f = 1. - 2. * s1 + s2, df = 4. * (s2 - s1);
+        *dE = pe * rt * (df - f);
+        return pe * f;
+    }
+    if (type == 4) {
+        if (r2 <= p->cutOn2) {
+            *dE = -pe * rt;
+            return pe;
+        }
+        double sw1 = p->cutOff2 - r2;
+        double f = SQ(sw1) * (p->cutOff2 + 2. * r2 - 3. * p->cutOn2) * p->switch2;
+        double df = 12. * r2 * sw1 * (p->cutOn2 - r2) * p->switch2;
+        *dE = pe * rt * (df - f);
+        return pe * f;
+    }
+    *dE = 0.;
+    return 0.;
+}
+
+/* energy.c:480-627: real-space loop over the list in neighOrder order,
+ * then the erf correction over excluded pairs with j > i. */
+void oc_ewald_real(const oc_param *p, const oc_box *b,
+                   const double *x, const double *y, const double *z,
+                   double *fx, double *fy, double *fz,
+                   const double *q, const double *eps, const double *sig,
+                   const int *neighList, const int *neighPair, const int *neighOrder,
+                   const int *exOff, const int *exLst, oc_energy *e)
+{
+    double eDir = 0., virDir = 0., eVdw = 0., virVdw = 0., eCorr = 0., virCorr = 0.;
+    long offset = 0;
+    int nin = 0, nlist = 0;
+    for (int l = 0; l < p->nAtom; l++) {
+        int i = neighOrder[l];
+        double fxi = 0., fyi = 0., fzi = 0.;
+        for (int k = 0; k < neighPair[i]; k++) {
+            int j = neighList[offset++];
+            double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+            double r2 = oc_dist(b, d);
+            double r = sqrt(r2);
+            double rt = 1. / r;
+            nlist++;
+            if (r2 <= p->cutOff2) {
+                double dEl, dVw;
+                double qel = p->chargeConst * q[i] * q[j];
+                eDir += oc_ewald_dir(p->alpha, qel, r, rt, &dEl);
+                double veps = eps[i] * eps[j];
+                double vsig = sig[i] + sig[j];
+                eVdw += oc_vdw_switch(p, veps, vsig, r2, rt, &dVw);
+                virDir += dEl * r;
+                virVdw += dVw * r;
+                double fxj = (dEl + dVw) * d[0] * rt;
+                double fyj = (dEl + dVw) * d[1] * rt;
+                double fzj = (dEl + dVw) * d[2] * rt;
+                fxi += fxj; fyi += fyj; fzi += fzj;
+                fx[j] += -fxj; fy[j] += -fyj; fz[j] += -fzj;
+                nin++;
+            }
+        }
+        fx[i] += fxi; fy[i] += fyi; fz[i] += fzi;
+    }
+    for (int i = 0; i < p->nAtom; i++) {
+        double fxi = 0., fyi = 0., fzi = 0.;
+        for (int k = exOff[i]; k < exOff[i + 1]; k++) {
+            int j = exLst[k];
+            if (j > i) {
+                double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+                double r2 = oc_dist(b, d);
+                double r = sqrt(r2);
+                double rt = 1. / r;
+                double dC;
+                double qel = p->chargeConst * q[i] * q[j];
+                eCorr += oc_ewald_corr(p->alpha, qel, r, rt, &dC);
+                virCorr += dC * r;
+                double fxj = dC * d[0] * rt, fyj = dC * d[1] * rt, fzj = dC * d[2] * rt;
+                fxi += fxj; fyi += fyj; fzi += fzj;
+                fx[j] += -fxj; fy[j] += -fyj; fz[j] += -fzj;
+            }
+        }
+        fx[i] += fxi; fy[i] += fyi; fz[i] += fzi;
+    }
+    e->eDir = eDir; e->virDir = virDir;
+    e->eVdw = eVdw; e->virVdw = virVdw;
+    e->eCorr = eCorr; e->virCorr = virCorr;
+    e->nInCut = nin; e->nListed = nlist;
+}
+
+/* energy.c:216-345 */
+void oc_nonbond(const oc_param *p, const oc_box *b, int elecType,
+                const double *x, const double *y, const double *z,
+                double *fx, double *fy, double *fz,
+                const double *q, const double *eps, const double *sig,
+                const int *neighList, const int *neighPair, const int *neighOrder,
+                double *eElec, double *eVdw, double *virElec, double *virVdw)
+{
+    double el = 0., ev = 0., vel = 0., vvw = 0.;
+    long offset = 0;
+    for (int l = 0; l < p->nAtom; l++) {
+        int i = neighOrder[l];
+        double fxi = 0., fyi = 0., fzi = 0.;
+        for (int k = 0; k < neighPair[i]; k++) {
+            int j = neighList[offset++];
+            double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+            double r2 = oc_dist(b, d);
+            double r = sqrt(r2);
+            double rt = 1. / r;
+            if (r2 <= p->cutOff2) {
+                double dEl, dVw;
+                el += oc_coulomb(p, elecType, q[i] * q[j], r2, rt, &dEl);
+                ev += oc_vdw_switch(p, eps[i] * eps[j], sig[i] + sig[j], r2, rt, &dVw);
+                vel += dEl * r;
+                vvw += dVw * r;
+                double fxj = (dEl + dVw) * d[0] * rt;
+                double fyj = (dEl + dVw) * d[1] * rt;
+                double fzj = (dEl + dVw) * d[2] * rt;
+                fxi += fxj; fyi += fyj; fzi += fzj;
+                fx[j] += -fxj; fy[j] += -fyj; fz[j] += -fzj;
+            }
+        }
+        fx[i] += fxi; fy[i] += fyi; fz[i] += fzi;
+    }
+    *eElec = el; *eVdw = ev; *virElec = vel; *virVdw = vvw;
+}
+
+/* ---------------------------------------------------------------- */
+/* FFT: mixed-radix recursive Cooley-Tukey (any length).             */
+/* ---------------------------------------------------------------- */
+
+static int small_factor(int n)
+{
+    for (int f = 2; f * f <= n; f++)
+        if (n % f == 0) return f;
+    return n;
+}
+
+/* out[k] = sum_t in[t*s] w^(t k), w = exp(sign 2 pi i / n); tw is the
+ * table exp(sign 2 pi i j / n0) with n dividing n0. */
+static void fft_rec(const ocplx *in, ocplx *out, int n, int s, const ocplx *tw,
+                    int n0, ocplx *tmp)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = small_factor(n), m = n / p;
+    for (int r = 0; r < p; r++) fft_rec(in + r * s, out + r * m, m, s * p, tw, n0, tmp);
+    int step = n0 / n;
+    for (int k = 0; k < m; k++) {
+        for (int qd = 0; qd < p; qd++) {
+            ocplx acc = 0;
+            int kk = k + qd * m;
+            for (int r = 0; r < p; r++) {
+                long e = ((long)r * kk) % n;
+                acc += out[r * m + k] * tw[e * step];
+            }
+            tmp[qd] = acc;
+        }
+        for (int qd = 0; qd < p; qd++) out[k + qd * m] = tmp[qd];
+    }
+}
+
+static void fft1d_lines(ocplx *a, int n, int count, int stride, int dist, int sign)
+{
+    ocplx *tw = (ocplx *)malloc((size_t)n * sizeof(ocplx));
+    ocplx *in = (ocplx *)malloc((size_t)n * sizeof(ocplx));
+    ocplx *out = (ocplx *)malloc((size_t)n * sizeof(ocplx));
+    ocplx *tmp = (ocplx *)malloc((size_t)n * sizeof(ocplx));
+    for (int j = 0; j < n; j++) {
+        double ang = (double)sign * OC_TWOPI * (double)j / (double)n;
+        tw[j] = cos(ang) + I * sin(ang);
+    }
+    for (int c = 0; c < count; c++) {
+        ocplx *base = a + (long)c * dist;
+        for (int t = 0; t < n; t++) in[t] = base[(long)t * stride];
+        fft_rec(in, out, n, 1, tw, n, tmp);
+        for (int t = 0; t < n; t++) base[(long)t * stride] = out[t];
+    }
+    free(tw); free(in); free(out); free(tmp);
+}
+
+int oc_fft3d(double *data, int n1, int n2, int n3, int sign)
+{
+    ocplx *a = (ocplx *)data;
+    /* axis 3 (contiguous) */
+    fft1d_lines(a, n3, n1 * n2, 1, n3, sign);
+    /* axis 2 */
+    for (int i = 0; i < n1; i++) fft1d_lines(a + (long)i * n2 * n3, n2, n3, n3, 1, sign);
+    /* axis 1 */
+    fft1d_lines(a, n1, n2 * n3, n2 * n3, 1, sign);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- */
+/* SPME, spme.c:170-699.                                             */
+/* ---------------------------------------------------------------- */
+
+/* spme.c:269-355 for one coordinate, same operation order. */
+void oc_bspline(double s, int p, double *bsp, double *bsd)
+{
+    int is = (int)s;
+    bsd[0] = 1.0;
+    bsd[1] = -1.0;
+    bsp[0] = s - is;
+    bsp[1] = 1.0 - bsp[0];
+    for (int k = 2; k < p; k++) {
+        bsp[k] = 0.0;
+        for (int j = k; j > 0; j--) {
+            if (k == p - 1) bsd[j] = bsp[j] - bsp[j - 1];
+            double ts = s + (double)j - is;
+            bsp[j] = (ts * bsp[j] + ((double)(k + 1) - ts) * bsp[j - 1]) / ((double)k);
+        }
+        if (k == p - 1) bsd[0] = bsp[0];
+        double ts = s - is;
+        bsp[0] = ts * bsp[0] / ((double)k);
+    }
+}
+
+/* spme.c:170-267: |bspc(i)|^2 with bspc(i) = e^{2 pi i i (p-1)/m} /
+ * sum_{k=0}^{p-2} M_p(k+1) e^{2 pi i i k/m}. */
+void oc_bspmod(int m, int p, double *mod2)
+{
+    double *bsp = (double *)calloc((size_t)(p + 1), sizeof(double));
+    ocplx *epl = (ocplx *)malloc((size_t)m * sizeof(ocplx));
+    bsp[0] = 0.0;
+    bsp[1] = 1.0;
+    for (int k = 2; k < p; k++) {
+        bsp[k] = 0.0;
+        for (int j = k; j > 0; j--)
+            bsp[j] = ((double)j * bsp[j] + (double)(k + 1 - j) * bsp[j - 1]) / ((double)k);
+    }
+    epl[0] = 1.0 + I * 0.0;
+    for (int i = 1; i <= m / 2; i++) {
+        epl[i] = cexp(I * OC_TWOPI * (double)i / (double)m);
+        epl[m - i] = conj(epl[i]);
+    }
+    for (int i = 0; i < m; i++) {
+        ocplx coeff = 0. + I * 0.0;
+        for (int k = 0; k < p - 1; k++) coeff += bsp[k + 1] * epl[(i * k) % m];
+        ocplx c = epl[(i * (p - 1)) % m] / coeff;
+        mod2[i] = creal(c * conj(c));
+    }
+    free(bsp);
+    free(epl);
+}
+
+double oc_spme(const oc_param *p, const oc_box *b,
+               const double *x, const double *y, const double *z,
+               double *fx, double *fy, double *fz, const double *q,
+               double stress[6], double *virRec)
+{
+    int n = p->nAtom, m1 = p->m1, m2 = p->m2, m3 = p->m3, P = p->nbsp;
+    long G = (long)m1 * m2 * m3;
+    double *sx = (double *)malloc((size_t)n * 3 * sizeof(double));
+    double *bp = (double *)malloc((size_t)n * 3 * P * sizeof(double));
+    double *bd = (double *)malloc((size_t)n * 3 * P * sizeof(double));
+    double *qsp = (double *)calloc((size_t)G, sizeof(double));
+    ocplx *ft = (ocplx *)malloc((size_t)G * sizeof(ocplx));
+    double *b1 = (double *)malloc((size_t)m1 * sizeof(double));
+    double *b2 = (double *)malloc((size_t)m2 * sizeof(double));
+    double *b3 = (double *)malloc((size_t)m3 * sizeof(double));
+    oc_bspmod(m1, P, b1);
+    oc_bspmod(m2, P, b2);
+    oc_bspmod(m3, P, b3);
+
+    /* self and net-charge terms every call (spme.c:376-389, 680) */
+    double systq = 0., eself = 0.;
+    for (int i = 0; i < n; i++) { systq += q[i]; eself += SQ(q[i]); }
+    eself = -p->chargeConst * p->alpha * eself / OC_SQRTPI;
+
+    double rVol = OC_TWOPI / b->vol;
+    double rAlpha2 = -0.25 / SQ(p->alpha);
+    double rc = fmin((double)m1 * b->u, (double)m2 * b->v);
+    rc = fmin(rc, (double)m3 * b->w);
+    rc = rc * 1.05 * OC_TWOPI;
+    double rc2 = SQ(rc);
+
+    for (int i = 0; i < n; i++) {                              /* spme.c:434-439 */
+        sx[3 * i + 0] = (double)m1 * (x[i] * b->u1 + y[i] * b->u2 + z[i] * b->u3 + 0.5);
+        sx[3 * i + 1] = (double)m2 * (x[i] * b->v1 + y[i] * b->v2 + z[i] * b->v3 + 0.5);
+        sx[3 * i + 2] = (double)m3 * (x[i] * b->w1 + y[i] * b->w2 + z[i] * b->w3 + 0.5);
+        for (int a = 0; a < 3; a++)
+            oc_bspline(sx[3 * i + a], P, bp + (3 * i + a) * P, bd + (3 * i + a) * P);
+    }
+    for (int l = 0; l < n; l++) {                              /* spme.c:449-491 */
+        const double *B1 = bp + (3 * l) * P, *B2 = B1 + P, *B3 = B2 + P;
+        for (int i = 0; i < P; i++) {
+            int ii = (int)sx[3 * l] - i;
+            if (ii >= m1) ii = 0;
+            if (ii < 0) ii += m1;
+            int o1 = m2 * ii;
+            for (int j = 0; j < P; j++) {
+                int jj = (int)sx[3 * l + 1] - j;
+                if (jj >= m2) jj = 0;
+                if (jj < 0) jj += m2;
+                int o2 = m3 * (jj + o1);
+                for (int k = 0; k < P; k++) {
+                    int kk = (int)sx[3 * l + 2] - k;
+                    if (kk >= m3) kk = 0;
+                    if (kk < 0) kk += m3;
+                    qsp[kk + o2] += q[l] * B1[i] * B2[j] * B3[k];
+                }
+            }
+        }
+    }
+    for (long g = 0; g < G; g++) ft[g] = qsp[g] + I * 0.0;
+    oc_fft3d((double *)ft, m1, m2, m3, +1);                    /* FFTW_BACKWARD */
+
+    double st[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < m1; i++) {                             /* spme.c:505-583 */
+        int ii = i > m1 / 2 ? i - m1 : i;
+        double tt = OC_TWOPI * (double)ii;
+        double r1x = tt * b->u1, r1y = tt * b->u2, r1z = tt * b->u3;
+        double bm1 = b1[i];
+        for (int j = 0; j < m2; j++) {
+            int jj = j > m2 / 2 ? j - m2 : j;
+            tt = OC_TWOPI * (double)jj;
+            double r2x = r1x + (tt * b->v1), r2y = r1y + (tt * b->v2), r2z = r1z + (tt * b->v3);
+            double bm2 = bm1 * b2[j];
+            for (int k = 0; k < m3; k++) {
+                int kk = k > m3 / 2 ? k - m3 : k;
+                long g = k + (long)m3 * (j + (long)m2 * i);
+                tt = OC_TWOPI * (double)kk;
+                double rx = r2x + (tt * b->w1), ry = r2y + (tt * b->w2), rz = r2z + (tt * b->w3);
+                double bm3 = bm2 * b3[k];
+                double rm = SQ(rx) + SQ(ry) + SQ(rz);
+                if (rm > DBL_EPSILON && rm <= rc2) {
+                    double rrm = 1.0 / rm;
+                    ocplx cam = bm3 * exp(rAlpha2 * rm) * rrm * ft[g];
+                    double vam = 2.0 * (rrm - rAlpha2) * creal(cam * conj(ft[g]));
+                    st[0] -= vam * rx * rx;
+                    st[1] -= vam * rx * ry;
+                    st[2] -= vam * rx * rz;
+                    st[3] -= vam * ry * ry;
+                    st[4] -= vam * ry * rz;
+                    st[5] -= vam * rz * rz;
+                    ft[g] = cam;
+                } else {
+                    ft[g] = 0.0 + I * 0.0;
+                }
+            }
+        }
+    }
+    oc_fft3d((double *)ft, m1, m2, m3, -1);                    /* FFTW_FORWARD */
+
+    double fact1 = -2.0 * rVol * p->chargeConst;
+    for (int l = 0; l < n; l++) {                              /* spme.c:593-646 */
+        const double *B1 = bp + (3 * l) * P, *B2 = B1 + P, *B3 = B2 + P;
+        const double *D1 = bd + (3 * l) * P, *D2 = D1 + P, *D3 = D2 + P;
+        for (int i = 0; i < P; i++) {
+            int ii = (int)sx[3 * l] - i;
+            if (ii >= m1) ii = 0;
+            if (ii < 0) ii += m1;
+            int o1 = m2 * ii;
+            for (int j = 0; j < P; j++) {
+                int jj = (int)sx[3 * l + 1] - j;
+                if (jj >= m2) jj = 0;
+                if (jj < 0) jj += m2;
+                int o2 = m3 * (jj + o1);
+                for (int k = 0; k < P; k++) {
+                    int kk = (int)sx[3 * l + 2] - k;
+                    if (kk >= m3) kk = 0;
+                    if (kk < 0) kk += m3;
+                    double qt = creal(ft[kk + o2]);
+                    double fbx = qt * D1[i] * B2[j] * B3[k] * (double)m1;
+                    double fby = qt * B1[i] * D2[j] * B3[k] * (double)m2;
+                    double fbz = qt * B1[i] * B2[j] * D3[k] * (double)m3;
+                    fx[l] += fact1 * q[l] * (fbx * b->u1 + fby * b->v1 + fbz * b->w1);
+                    fy[l] += fact1 * q[l] * (fbx * b->u2 + fby * b->v2 + fbz * b->w2);
+                    fz[l] += fact1 * q[l] * (fbx * b->u3 + fby * b->v3 + fbz * b->w3);
+                }
+            }
+        }
+    }
+    double fxm = 0., fym = 0., fzm = 0.;                       /* spme.c:648-670 */
+    for (int l = 0; l < n; l++) { fxm += fx[l]; fym += fy[l]; fzm += fz[l]; }
+    fxm /= (double)n; fym /= (double)n; fzm /= (double)n;
+    for (int l = 0; l < n; l++) { fx[l] -= fxm; fy[l] -= fym; fz[l] -= fzm; }
+
+    ocplx et = 0.0 + I * 0.0;                                  /* spme.c:674-697 */
+    for (long g = 0; g < G; g++) et += ft[g] * qsp[g];
+    double eNN = -(0.5 * OC_PI * p->chargeConst) * (SQ(systq / p->alpha) / b->vol);
+    fact1 = rVol * p->chargeConst;
+    double eRec = creal(et);
+    stress[0] = fact1 * (st[0] + eRec) + eNN;
+    stress[1] = fact1 * st[1];
+    stress[2] = fact1 * st[2];
+    stress[3] = fact1 * (st[3] + eRec) + eNN;
+    stress[4] = fact1 * st[4];
+    stress[5] = fact1 * (st[5] + eRec) + eNN;
+    eRec = fact1 * eRec + eself + eNN;
+    *virRec = -(stress[0] + stress[3] + stress[5]);
+
+    free(sx); free(bp); free(bd); free(qsp); free(ft); free(b1); free(b2); free(b3);
+    return eRec;
+}
+
+/* Classic Ewald reciprocal sum, E = (2pi/V) C sum_{m != 0} exp(-k^2/4a^2)/k^2
+ * |S(k)|^2 with k = 2pi (m1 u + m2 v + m3 w), over the full cube |m_i| <=
+ * kmax, plus the same self and net-charge terms as SPME (Essmann et al.
+ * 1995, eqs. 2.4-2.5).  Forces are added to fx,fy,fz (no mean removal). */
+double oc_ewald_classic(const oc_param *p, const oc_box *b, int kmax,
+                        const double *x, const double *y, const double *z,
+                        double *fx, double *fy, double *fz, const double *q)
+{
+    int n = p->nAtom;
+    double fac = OC_TWOPI / b->vol * p->chargeConst;
+    double e = 0.;
+    for (int a = -kmax; a <= kmax; a++)
+        for (int c = -kmax; c <= kmax; c++)
+            for (int d = -kmax; d <= kmax; d++) {
+                if (!a && !c && !d) continue;
+                double kx = OC_TWOPI * (a * b->u1 + c * b->v1 + d * b->w1);
+                double ky = OC_TWOPI * (a * b->u2 + c * b->v2 + d * b->w2);
+                double kz = OC_TWOPI * (a * b->u3 + c * b->v3 + d * b->w3);
+                double k2 = kx * kx + ky * ky + kz * kz;
+                double f = exp(-k2 / (4. * SQ(p->alpha))) / k2;
+                double sr = 0., si = 0.;
+                for (int j = 0; j < n; j++) {
+                    double ph = kx * x[j] + ky * y[j] + kz * z[j];
+                    sr += q[j] * cos(ph);
+                    si += q[j] * sin(ph);
+                }
+                e += f * (sr * sr + si * si);
+                for (int j = 0; j < n; j++) {
+                    double ph = kx * x[j] + ky * y[j] + kz * z[j];
+                    /* Im(conj(S) e^{i k r_j}) = sr sin - si cos */
+                    double im = sr * sin(ph) - si * cos(ph);
+                    double g = fac * f * 2. * q[j] * im;
+                    fx[j] += g * kx;
+                    fy[j] += g * ky;
+                    fz[j] += g * kz;
+                }
+            }
+    double systq = 0., q2 = 0.;
+    for (int j = 0; j < n; j++) { systq += q[j]; q2 += q[j] * q[j]; }
+    double eself = -p->chargeConst * p->alpha * q2 / OC_SQRTPI;
+    double eNN = -(0.5 * OC_PI * p->chargeConst) * (SQ(systq / p->alpha) / b->vol);
+    return fac * e + eself + eNN;
+}
+
+/* ---------------------------------------------------------------- */
+/* Velocity Verlet NVE + SHAKE, integrate.c:1518-1634, shake.c.      */
+/* ---------------------------------------------------------------- */
+
+static double oc_dist2(const oc_box *b, double *dx, double *dy, double *dz)
+{
+    double d[3] = {*dx, *dy, *dz};
+    double r2 = oc_dist(b, d);     /* utils.c:82-122 is dist() on three scalars */
+    *dx = d[0]; *dy = d[1]; *dz = d[2];
+    return r2;
+}
+
+/* shake.c:209-357 */
+static int oc_shake_r(const oc_param *p, const oc_box *b, int nc, const int *ca,
+                      const int *cb, const double *rc2, double *x, double *y,
+                      double *z, double *vx, double *vy, double *vz,
+                      const double *ddx, const double *ddy, const double *ddz,
+                      const double *rmass, const int *nAtConst, double *virshake,
+                      int *ncycle)
+{
+    int n = p->nAtom;
+    double *xt = (double *)malloc((size_t)n * 3 * sizeof(double));
+    double *yt = xt + n, *zt = yt + n;
+    double *rt2 = (double *)malloc((size_t)nc * 4 * sizeof(double));
+    double *dtx = rt2 + nc, *dty = dtx + nc, *dtz = dty + nc;
+    int icycle = 0, conv = 0;
+    double dt = p->timeStep;
+    *virshake = 0.;
+    do {
+        double maxdist = 0.;
+        for (int i = 0; i < nc; i++) {
+            int ia = ca[i], ib = cb[i];
+            dtx[i] = x[ib] - x[ia];
+            dty[i] = y[ib] - y[ia];
+            dtz[i] = z[ib] - z[ia];
+            rt2[i] = oc_dist2(b, &dtx[i], &dty[i], &dtz[i]);
+            double dd = fabs(rt2[i] - rc2[i]) / sqrt(rc2[i]);
+            maxdist = maxdist >= dd ? maxdist : dd;
+        }
+        maxdist = 0.5 * maxdist;
+        if (maxdist < p->tolShake) conv = 1;
+        if (!conv) {
+            for (int i = 0; i < n; i++) { xt[i] = 0.; yt[i] = 0.; zt[i] = 0.; }
+            for (int i = 0; i < nc; i++) {
+                int ia = ca[i], ib = cb[i];
+                double trmi = dt * rmass[ia], trmj = dt * rmass[ib];
+                double lambda = -(rc2[i] - rt2[i]) /
+                    (dt * (trmi + trmj) * ((ddx[i] * dtx[i]) + (ddy[i] * dty[i]) + (ddz[i] * dtz[i])));
+                *virshake += lambda * (SQ(ddx[i]) + SQ(ddy[i]) + SQ(ddz[i]));
+                double li = 0.5 * lambda * trmi;
+                xt[ia] += ddx[i] * li; yt[ia] += ddy[i] * li; zt[ia] += ddz[i] * li;
+                double lj = -0.5 * lambda * trmj;
+                xt[ib] += ddx[i] * lj; yt[ib] += ddy[i] * lj; zt[ib] += ddz[i] * lj;
+            }
+            for (int i = 0; i < nc; i++) {
+                int ia = ca[i], ib = cb[i];
+                double nia = 1.0 / (double)nAtConst[ia], nib = 1.0 / (double)nAtConst[ib];
+                x[ia] += dt * xt[ia] * nia; y[ia] += dt * yt[ia] * nia; z[ia] += dt * zt[ia] * nia;
+                x[ib] += dt * xt[ib] * nib; y[ib] += dt * yt[ib] * nib; z[ib] += dt * zt[ib] * nib;
+                vx[ia] += xt[ia] * nia; vy[ia] += yt[ia] * nia; vz[ia] += zt[ia] * nia;
+                vx[ib] += xt[ib] * nib; vy[ib] += yt[ib] * nib; vz[ib] += zt[ib] * nib;
+            }
+        }
+        icycle++;
+    } while (!conv && icycle < p->maxCycle);
+    free(xt);
+    free(rt2);
+    if (ncycle) *ncycle = icycle;
+    return conv ? 0 : 311;
+}
+
+/* shake.c:359-457 */
+static int oc_shake_v(const oc_param *p, int nc, const int *ca, const int *cb,
+                      double *vx, double *vy, double *vz, const double *ddx,
+                      const double *ddy, const double *ddz, const double *rmass,
+                      const int *nAtConst, int *ncycle)
+{
+    int n = p->nAtom;
+    double *xt = (double *)malloc((size_t)n * 3 * sizeof(double));
+    double *yt = xt + n, *zt = yt + n;
+    int icycle = 0, conv = 0;
+    double tolvel = p->tolShake * p->rTimeStep;
+    do {
+        for (int i = 0; i < n; i++) { xt[i] = 0.; yt[i] = 0.; zt[i] = 0.; }
+        double maxdist = 0.;
+        for (int i = 0; i < nc; i++) {
+            int ia = ca[i], ib = cb[i];
+            double trmi = 0.5 * p->timeStep * rmass[ia];
+            double trmj = 0.5 * p->timeStep * rmass[ib];
+            double lambda = (ddx[i] * (vx[ib] - vx[ia]) + ddy[i] * (vy[ib] - vy[ia]) +
+                             ddz[i] * (vz[ib] - vz[ia])) /
+                            ((trmi + trmj) * (SQ(ddx[i]) + SQ(ddy[i]) + SQ(ddz[i])));
+            double al = fabs(lambda);
+            maxdist = maxdist >= al ? maxdist : al;
+            double li = lambda * trmi;
+            xt[ia] += ddx[i] * li; yt[ia] += ddy[i] * li; zt[ia] += ddz[i] * li;
+            double lj = -lambda * trmj;
+            xt[ib] += ddx[i] * lj; yt[ib] += ddy[i] * lj; zt[ib] += ddz[i] * lj;
+        }
+        if (maxdist < tolvel) conv = 1;
+        if (!conv) {
+            for (int i = 0; i < nc; i++) {
+                int ia = ca[i], ib = cb[i];
+                double nia = 1.0 / (double)nAtConst[ia], nib = 1.0 / (double)nAtConst[ib];
+                vx[ia] += xt[ia] * nia; vy[ia] += yt[ia] * nia; vz[ia] += zt[ia] * nia;
+                vx[ib] += xt[ib] * nib; vy[ib] += yt[ib] * nib; vz[ib] += zt[ib] * nib;
+            }
+        }
+        icycle++;
+    } while (!conv && icycle < p->maxCycle);
+    free(xt);
+    if (ncycle) *ncycle = icycle;
+    return conv ? 0 : 311;
+}
+
+int oc_vv_nve(const oc_param *p, const oc_box *b, int stage,
+              int nConst, const int *ca, const int *cb, const double *rc2,
+              double *x, double *y, double *z,
+              double *vx, double *vy, double *vz,
+              const double *fx, const double *fy, const double *fz,
+              const double *mass, const double *rmass, const int *nAtConst,
+              double *ekin, double *virshake, int *ncycle)
+{
+    int n = p->nAtom, err = 0;
+    double dt = p->timeStep;
+    double *dd = NULL;
+    if (nConst > 0) {                                     /* integrate.c:1526-1545 */
+        dd = (double *)malloc((size_t)nConst * 3 * sizeof(double));
+        for (int i = 0; i < nConst; i++) {
+            dd[i] = x[cb[i]] - x[ca[i]];
+            dd[nConst + i] = y[cb[i]] - y[ca[i]];
+            dd[2 * nConst + i] = z[cb[i]] - z[ca[i]];
+            oc_dist2(b, &dd[i], &dd[nConst + i], &dd[2 * nConst + i]);
+        }
+    }
+    for (int i = 0; i < n; i++) {                         /* integrate.c:1554-1561 */
+        vx[i] += 0.5 * dt * fx[i] * rmass[i];
+        vy[i] += 0.5 * dt * fy[i] * rmass[i];
+        vz[i] += 0.5 * dt * fz[i] * rmass[i];
+    }
+    if (stage == 1) {
+        for (int i = 0; i < n; i++) {                     /* integrate.c:1568-1576 */
+            x[i] += dt * vx[i];
+            y[i] += dt * vy[i];
+            z[i] += dt * vz[i];
+        }
+        if (nConst > 0) {
+            double vs = 0.;
+            err = oc_shake_r(p, b, nConst, ca, cb, rc2, x, y, z, vx, vy, vz, dd,
+                             dd + nConst, dd + 2 * nConst, rmass, nAtConst, &vs, ncycle);
+            if (virshake) *virshake = vs;
+        }
+    } else {
+        if (nConst > 0)
+            err = oc_shake_v(p, nConst, ca, cb, vx, vy, vz, dd, dd + nConst, dd + 2 * nConst,
+                             rmass, nAtConst, ncycle);
+        double ek = 0.;                                   /* utils.c:502-515 */
+        for (int i = 0; i < n; i++) ek += mass[i] * (SQ(vx[i]) + SQ(vy[i]) + SQ(vz[i]));
+        if (ekin) *ekin = ek * 0.5;
+        oc_image_update(b, n, x, y, z);                   /* integrate.c:1620-1627 */
+    }
+    free(dd);
+    return err;
+}
+
+/* ---------------------------------------------------------------- */
+/* Whole MD run (mdbas.c:160-300).                                   */
+/* ---------------------------------------------------------------- */
+
+typedef struct {
+    int *nl; long cap; int *np; int *no;
+} oc_list;
+
+static int md_list(const oc_param *p, const oc_box *b, int keyLink, int build,
+                   double *x, double *y, double *z, const int *frozen,
+                   const int *exOff, const int *exLst, oc_list *L)
+{
+    for (;;) {
+        long cnt = keyLink
+            ? oc_link_cell_list(p, b, build, x, y, z, frozen, exOff, exLst, L->nl, L->cap, L->np, L->no)
+            : oc_verlet_list(p, b, x, y, z, frozen, exOff, exLst, L->nl, L->cap, L->np, L->no);
+        if (cnt < 0) return (int)(-cnt);
+        if (cnt <= L->cap) return 0;
+        /* list.c:2110 sizing rule; on overflow the reference reallocs */
+        L->cap = (long)((double)cnt * (1. + 2. * OC_TOLLIST)) + 1;
+        L->nl = (int *)realloc(L->nl, (size_t)L->cap * sizeof(int));
+        build = 0;   /* positions were already wrapped by the first attempt */
+    }
+}
+
+int oc_md_run(const oc_param *p, const oc_box *b, int keyLink, int nsteps,
+              double *x, double *y, double *z,
+              double *vx, double *vy, double *vz,
+              double *fx, double *fy, double *fz,
+              const double *q, const double *eps, const double *sig,
+              const double *mass, const double *rmass, const int *frozen,
+              const int *nAtConst, int nConst, const int *ca, const int *cb,
+              const double *rc2, double *etot, double *epot, double *ekin,
+              double *seconds)
+{
+    int n = p->nAtom, err = 0;
+    int *exOff = (int *)malloc((size_t)(n + 1) * sizeof(int));
+    int excap = 4 * nConst + 16;
+    int *exLst = (int *)malloc((size_t)excap * sizeof(int));
+    int ne = oc_exclusions(n, 0, NULL, NULL, nConst, ca, cb, 0, NULL, NULL, NULL,
+                           keyLink ? 1 : 0, exOff, exLst, excap);
+    if (ne < 0) { free(exOff); free(exLst); return ne > -1000 ? -ne : 1; }
+    oc_list L;
+    L.cap = 1024;
+    L.nl = (int *)malloc((size_t)L.cap * sizeof(int));
+    L.np = (int *)malloc((size_t)n * sizeof(int));
+    L.no = (int *)malloc((size_t)n * sizeof(int));
+
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    err = md_list(p, b, keyLink, 1, x, y, z, frozen, exOff, exLst, &L);
+    double ek = 0.;
+    for (int i = 0; i < n; i++) ek += mass[i] * (SQ(vx[i]) + SQ(vy[i]) + SQ(vz[i]));
+    ek *= 0.5;
+    for (int step = 0; step <= nsteps && !err; step++) {
+        if (step > 0) {
+            double vs;
+            err = oc_vv_nve(p, b, 1, nConst, ca, cb, rc2, x, y, z, vx, vy, vz, fx, fy, fz,
+                            mass, rmass, nAtConst, NULL, &vs, NULL);
+            if (err) break;
+            if (step % p->update == 0)
+                err = md_list(p, b, keyLink, 0, x, y, z, frozen, exOff, exLst, &L);
+            if (err) break;
+        }
+        for (int i = 0; i < n; i++) { fx[i] = 0.; fy[i] = 0.; fz[i] = 0.; }
+        double st[6], vr;
+        double eRec = oc_spme(p, b, x, y, z, fx, fy, fz, q, st, &vr);
+        oc_energy e;
+        oc_ewald_real(p, b, x, y, z, fx, fy, fz, q, eps, sig, L.nl, L.np, L.no, exOff, exLst, &e);
+        double pot = (e.eDir + eRec + e.eCorr) + e.eVdw;
+        if (step > 0) {
+            err = oc_vv_nve(p, b, 2, nConst, ca, cb, rc2, x, y, z, vx, vy, vz, fx, fy, fz,
+                            mass, rmass, nAtConst, &ek, NULL, NULL);
+        }
+        if (epot) epot[step] = pot;
+        if (ekin) ekin[step] = ek;
+        if (etot) etot[step] = pot + ek;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (seconds) *seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    free(L.nl); free(L.np); free(L.no); free(exOff); free(exLst);
+    return err;
+}

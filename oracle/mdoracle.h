@@ -1,0 +1,186 @@
+/*
+ * mdoracle.h -- CPU FP64 oracle for the MDBas nonbonded force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a from-scratch C restatement of the
+ * reference semantics, used as the checker for the HIP library.  Only the
+ * test suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it; the product library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Build with
+ * -O2 -ffp-contract=off (no FMA contraction), which is what the reference's
+ * x86-64 SSE2 build does, so that the neighbour-list distance tests round
+ * exactly as the reference's do.
+ *
+ * Pinning (what checks each function beyond retyping it):
+ *   oc_init_box, oc_nint, oc_dist        : compiled reference (oracle/_ref),
+ *                                          special cases at +-0.5.
+ *   oc_exclusions                        : compiled reference exclude_list /
+ *                                          link_cell_exclude_list (as sets).
+ *   oc_verlet_list, oc_link_cell_list    : compiled reference list builders,
+ *                                          element-by-element; brute force.
+ *   oc_ewald_real                        : compiled reference ewald_dir /
+ *                                          ewald_corr / vdw_switch; finite
+ *                                          differences; Newton's third law.
+ *   oc_spme                              : classic Ewald (oc_ewald_classic and
+ *                                          the compiled reference ewald_rec) at
+ *                                          convergence; B-spline identities;
+ *                                          FD of the energy.
+ *   oc_fft3d                             : direct O(n^2) DFT on small grids.
+ *   oc_vv_nve / oc_shake_*               : compiled reference vv_integrate /
+ *                                          vv_shake_r / vv_shake_v.
+ */
+#ifndef MDORACLE_H
+#define MDORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Box: the nine box vectors in, everything else derived by oc_init_box.
+ * type follows the reference BOX_TYPE: 0 none, 1 cubic, 2 orthorhombic,
+ * 3 triclinic (global.h enum BOX_TYPE). */
+typedef struct {
+    int type;
+    double a1, a2, a3, b1, b2, b3, c1, c2, c3;
+    double u1, u2, u3, v1, v2, v3, w1, w2, w3;
+    double pa, pb, pc, u, v, w, det, vol;
+} oc_box;
+
+/* Run parameters (the subset of the reference PARAM/EWALD/NEIGH used on
+ * the nonbonded path) plus the derived constants of io.c:714-723. */
+typedef struct {
+    int nAtom;
+    double cutOff, cutOn, delr, chargeConst;
+    double alpha;
+    int m1, m2, m3, nbsp;
+    double timeStep, tolShake;
+    int maxCycle, update, linkRatio;
+    /* derived by oc_param_derive */
+    double cutOff2, cutOn2, switch2, rcutOff, rcutOff2, rTimeStep;
+} oc_param;
+
+/* Energies / virials of one force evaluation (subset of ENERGY). */
+typedef struct {
+    double eDir, eCorr, eRec, eVdw;
+    double virDir, virCorr, virRec, virVdw;
+    double stressRec[6];
+    int nInCut;      /* listed pairs with r2 <= cutOff2 */
+    int nListed;     /* listed pairs */
+} oc_energy;
+
+void   oc_param_derive(oc_param *p);
+void   oc_init_box(oc_box *b);
+int    oc_nint(double x);
+double oc_dist(const oc_box *b, double d[3]);
+void   oc_image_update(const oc_box *b, int n, double *x, double *y, double *z);
+
+/* Decide the list builder the way makelist does on its first call:
+ * 1 = link cells, 0 = O(N^2) Verlet list. */
+int    oc_choose_link(const oc_param *p, const oc_box *b);
+
+/* Exclusion sets from bonds, constraints and angles plus the 1-3
+ * connectivity walk.  Output CSR (off[n+1], lst).  sym=0 stores each pair
+ * once at the smaller index (exclude_list); sym=1 stores it at both atoms
+ * (link_cell_exclude_list).  Returns the number of entries, or -110 when a
+ * 1-4 path exists (the reference raises DIHE_NONPARAM_ERROR without
+ * dihedrals), or -(needed) when cap is too small. */
+int oc_exclusions(int n, int nBond, const int *ba, const int *bb,
+                  int nConst, const int *ca, const int *cb,
+                  int nAngle, const int *aa, const int *ab, const int *ac,
+                  int sym, int *off, int *lst, int cap);
+
+/* O(N^2) Verlet list (list.c:896-1123).  Writes neighPair[n],
+ * neighOrder[n] and up to cap entries of neighList; returns the pair count
+ * (which may exceed cap: call again with a larger buffer). */
+long oc_verlet_list(const oc_param *p, const oc_box *b,
+                    const double *x, const double *y, const double *z,
+                    const int *frozen, const int *exOff, const int *exLst,
+                    int *neighList, long cap, int *neighPair, int *neighOrder);
+
+/* Link-cell list (list.c:1793-2619).  build=1 wraps x,y,z first (the
+ * first-call builder); build=0 is the update builder, which does not.
+ * Returns the pair count, or -412 when there are too few cells. */
+long oc_link_cell_list(const oc_param *p, const oc_box *b, int build,
+                       double *x, double *y, double *z,
+                       const int *frozen, const int *exOff, const int *exLst,
+                       int *neighList, long cap, int *neighPair, int *neighOrder);
+
+/* Real-space Ewald + switched LJ over the list plus the excluded-pair erf
+ * correction (energy.c:480-627 with ewald.c:356-384, vdw.c:144-184).
+ * Forces accumulate (+=) into fx,fy,fz. */
+void oc_ewald_real(const oc_param *p, const oc_box *b,
+                   const double *x, const double *y, const double *z,
+                   double *fx, double *fy, double *fz,
+                   const double *q, const double *eps, const double *sig,
+                   const int *neighList, const int *neighPair, const int *neighOrder,
+                   const int *exOff, const int *exLst, oc_energy *e);
+
+/* Non-Ewald "plain" electrostatics with the SHIFT1 (2), SHIFT2 (3) or
+ * SWITCH (4) functor plus switched LJ (energy.c:216-345, elec.c:137-263). */
+void oc_nonbond(const oc_param *p, const oc_box *b, int elecType,
+                const double *x, const double *y, const double *z,
+                double *fx, double *fy, double *fz,
+                const double *q, const double *eps, const double *sig,
+                const int *neighList, const int *neighPair, const int *neighOrder,
+                double *eElec, double *eVdw, double *virElec, double *virVdw);
+
+/* One pair: the functors themselves, for table tests. */
+double oc_ewald_dir(double alpha, double qel, double r, double rt, double *dE);
+double oc_ewald_corr(double alpha, double qel, double r, double rt, double *dE);
+double oc_vdw_switch(const oc_param *p, double veps, double vsig, double r2,
+                     double rt, double *dE);
+
+/* SPME reciprocal energy, forces (+=, then the mean of the whole force
+ * array is removed as spme.c:648-670 does), stress[6] and virial.  The
+ * self and net-charge terms are computed on every call. */
+double oc_spme(const oc_param *p, const oc_box *b,
+               const double *x, const double *y, const double *z,
+               double *fx, double *fy, double *fz, const double *q,
+               double stress[6], double *virRec);
+
+/* B-spline values/derivatives for one fractional coordinate s, as bspgen
+ * does (spme.c:269-355): bsp[i] = M_p(u+i), bsd[i] = M_p'(u+i). */
+void oc_bspline(double s, int p, double *bsp, double *bsd);
+
+/* |b(m)|^2 moduli for one axis (spme.c:170-267). */
+void oc_bspmod(int m, int p, double *mod2);
+
+/* Classic Ewald reciprocal sum over |m_i| <= kmax, for pinning SPME. */
+double oc_ewald_classic(const oc_param *p, const oc_box *b, int kmax,
+                        const double *x, const double *y, const double *z,
+                        double *fx, double *fy, double *fz, const double *q);
+
+/* In-place 3-D complex FFT (interleaved re,im doubles, index 3 fastest),
+ * unnormalised, exponent sign `sign` (+1 = FFTW_BACKWARD). */
+int oc_fft3d(double *data, int n1, int n2, int n3, int sign);
+
+/* Velocity-Verlet NVE stage 1 or 2 (integrate.c:1518-1634) with SHAKE
+ * (shake.c:209-457).  Returns 0, or 311 when SHAKE does not converge.
+ * ekin/virshake receive the kinetic energy (stage 2) and SHAKE virial
+ * (stage 1). */
+int oc_vv_nve(const oc_param *p, const oc_box *b, int stage,
+              int nConst, const int *ca, const int *cb, const double *rc2,
+              double *x, double *y, double *z,
+              double *vx, double *vy, double *vz,
+              const double *fx, const double *fy, const double *fz,
+              const double *mass, const double *rmass, const int *nAtConst,
+              double *ekin, double *virshake, int *ncycle);
+
+/* Whole MD run: energy at step 0, then nsteps of VV stage 1, list update
+ * every p->update steps, energy, VV stage 2 (mdbas.c:160-300).
+ * etot/epot/ekin (length nsteps+1, may be NULL) receive per-step energies.
+ * keyLink: 1 link cells, 0 O(N^2).  Returns 0 or an error code. */
+int oc_md_run(const oc_param *p, const oc_box *b, int keyLink, int nsteps,
+              double *x, double *y, double *z,
+              double *vx, double *vy, double *vz,
+              double *fx, double *fy, double *fz,
+              const double *q, const double *eps, const double *sig,
+              const double *mass, const double *rmass, const int *frozen,
+              const int *nAtConst, int nConst, const int *ca, const int *cb,
+              const double *rc2, double *etot, double *epot, double *ekin,
+              double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,265 @@
+/*
+ * ref_driver.c -- flat-array entry points around the compiled reference
+ * (test infrastructure only).
+ *
+ * oracle/Makefile compiles the reference's own src/list.c, ewald.c, vdw.c,
+ * elec.c, utils.c, shake.c, integrate.c, init.c, energy.c, memory.c and
+ * errors.c from where they lie under the reference tree, links them with
+ * this file into oracle/_ref/libmdbas_ref.so, exports only the ref_*
+ * symbols and lets the linker drop every function nothing here reaches.
+ * That drops energy.c's ewald_energy (it calls spme_energy, which lives
+ * in spme.c, which needs the absent fftw3.h) and init.c's file readers,
+ * so no stand-in for any missing header or library is written.
+ *
+ * This file plays the role of the reference's mdbas.c for the objects it
+ * links: it owns outFile and the structs the reference functions take.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "global.h"
+#include "list.h"
+#include "ewald.h"
+#include "vdw.h"
+#include "elec.h"
+#include "utils.h"
+#include "shake.h"
+#include "integrate.h"
+#include "energy.h"
+
+FILE *outFile;   /* defined by mdbas.c in the reference */
+
+void init_box(PBC *box);   /* init.c:840 */
+
+typedef struct {
+    CTRL ctrl;
+    PARAM param;
+    PBC box;
+    NEIGH neigh;
+    EWALD ewald;
+    ENERGY ener;
+    BATH bath;
+    CONSTRAINT *cons;
+    int *frozen, *nAtConst;
+    int *neighList, *neighPair, *neighOrder, *neighList14, **exclList, *exclPair;
+    int listed;
+    int shakeAlloc;
+} ref_sys;
+
+static void ensure_out(void)
+{
+    if (!outFile) outFile = fopen("/dev/null", "w");
+}
+
+/* box9 = a1 a2 a3 b1 b2 b3 c1 c2 c3; out[24] = u1..w3 (9), pa pb pc,
+ * u v w, det vol, a b c (norms), 3 spare. */
+void ref_init_box(int type, const double *box9, double *out)
+{
+    PBC b;
+    memset(&b, 0, sizeof b);
+    b.type = (enum BOX_TYPE)type;
+    b.a1 = box9[0]; b.a2 = box9[1]; b.a3 = box9[2];
+    b.b1 = box9[3]; b.b2 = box9[4]; b.b3 = box9[5];
+    b.c1 = box9[6]; b.c2 = box9[7]; b.c3 = box9[8];
+    init_box(&b);
+    double o[24] = {b.u1, b.u2, b.u3, b.v1, b.v2, b.v3, b.w1, b.w2, b.w3,
+                    b.pa, b.pb, b.pc, b.u, b.v, b.w, b.det, b.vol, b.a, b.b, b.c, 0, 0, 0, 0};
+    memcpy(out, o, sizeof o);
+}
+
+int ref_nint(double x) { return nint(x); }
+
+ref_sys *ref_create(int n, int boxType, const double *box9,
+                    double cutOff, double cutOn, double delr, double alpha,
+                    int linkRatio, int update, int noLink,
+                    double timeStep, double tolShake, int maxCycle,
+                    int nConst, const int *ca, const int *cb, const double *rc2,
+                    const int *frozen, const int *nAtConst)
+{
+    ensure_out();
+    ref_sys *s = (ref_sys *)calloc(1, sizeof(ref_sys));
+    s->box.type = (enum BOX_TYPE)boxType;
+    s->box.a1 = box9[0]; s->box.a2 = box9[1]; s->box.a3 = box9[2];
+    s->box.b1 = box9[3]; s->box.b2 = box9[4]; s->box.b3 = box9[5];
+    s->box.c1 = box9[6]; s->box.c2 = box9[7]; s->box.c3 = box9[8];
+    init_box(&s->box);
+
+    s->ctrl.newjob = 1;
+    s->ctrl.noLink = noLink;
+    s->ctrl.keyConstH = 1;
+    s->ctrl.elecType = SWITCH;
+    s->ctrl.vdwType = VSWITCH;
+    s->ctrl.integrator = VELOCITY;
+    s->ctrl.ens = NVE;
+    s->ctrl.keyEwald = 2;
+
+    s->param.nAtom = n;
+    s->param.nConst = nConst;
+    s->param.cutOff = cutOff;
+    s->param.cutOn = cutOn;
+    s->param.delr = delr;
+    s->param.chargeConst = chgcharmm * kcaltoiu;   /* init.c:102-103 */
+    s->param.timeStep = timeStep;
+    s->param.tolShake = tolShake;
+    s->param.maxCycle = maxCycle;
+    /* io.c:714-723 */
+    s->param.rcutOff = 1. / s->param.cutOff;
+    s->param.cutOff2 = X2(s->param.cutOff);
+    s->param.rcutOff2 = 1. / s->param.cutOff2;
+    s->param.cutOn2 = X2(s->param.cutOn);
+    s->param.switch2 = 1. / X3(s->param.cutOff2 - s->param.cutOn2);
+    s->param.rTimeStep = 1. / s->param.timeStep;
+
+    s->neigh.update = update;
+    s->neigh.linkRatio = linkRatio;
+    s->ewald.alpha = alpha;
+
+    s->cons = (CONSTRAINT *)malloc((size_t)(nConst > 0 ? nConst : 1) * sizeof(CONSTRAINT));
+    for (int i = 0; i < nConst; i++) {
+        s->cons[i].a = ca[i];
+        s->cons[i].b = cb[i];
+        s->cons[i].rc2 = rc2[i];
+    }
+    s->frozen = (int *)malloc((size_t)n * sizeof(int));
+    s->nAtConst = (int *)malloc((size_t)n * sizeof(int));
+    memcpy(s->frozen, frozen, (size_t)n * sizeof(int));
+    memcpy(s->nAtConst, nAtConst, (size_t)n * sizeof(int));
+    return s;
+}
+
+/* makelist (list.c:29-106): first call builds exclusions and the list
+ * (link cells or O(N^2)); later calls rebuild when step % update == 0.
+ * x,y,z may be wrapped in place, as the reference does. */
+int ref_makelist(ref_sys *s, double *x, double *y, double *z, int step)
+{
+    s->param.step = step;
+    makelist(&s->ctrl, &s->param, &s->box, &s->neigh, s->cons, NULL, NULL, NULL, NULL,
+             x, y, z, s->frozen, &s->neighList, &s->neighPair, &s->neighOrder,
+             &s->neighList14, &s->exclList, &s->exclPair);
+    s->listed = 1;
+    return s->ctrl.keyLink;
+}
+
+long ref_list_count(ref_sys *s)
+{
+    long t = 0;
+    for (int i = 0; i < s->param.nAtom; i++) t += s->neighPair[i];
+    return t;
+}
+
+void ref_list_copy(ref_sys *s, int *neighList, int *neighPair, int *neighOrder)
+{
+    long t = ref_list_count(s);
+    memcpy(neighList, s->neighList, (size_t)t * sizeof(int));
+    memcpy(neighPair, s->neighPair, (size_t)s->param.nAtom * sizeof(int));
+    memcpy(neighOrder, s->neighOrder, (size_t)s->param.nAtom * sizeof(int));
+}
+
+/* Exclusions as CSR; rows beyond the table (the ordered table has N-1
+ * rows) are empty. */
+int ref_excl_copy(ref_sys *s, int *off, int *lst, int cap)
+{
+    int n = s->param.nAtom, o = 0;
+    int rows = s->ctrl.keyLink ? n : n - 1;
+    for (int i = 0; i < n; i++) {
+        off[i] = o;
+        if (i < rows)
+            for (int k = 0; k < s->exclPair[i]; k++) {
+                if (o < cap) lst[o] = s->exclList[i][k];
+                o++;
+            }
+    }
+    off[n] = o;
+    return o;
+}
+
+double ref_ewald_dir(double alpha, double qel, double r, double rt, double *dE)
+{
+    EWALD e;
+    memset(&e, 0, sizeof e);
+    e.alpha = alpha;
+    return ewald_dir(&e, dE, qel, r, rt);
+}
+
+double ref_ewald_corr(double alpha, double qel, double r, double rt, double *dE)
+{
+    EWALD e;
+    memset(&e, 0, sizeof e);
+    e.alpha = alpha;
+    return ewald_corr(&e, dE, qel, r, rt);
+}
+
+double ref_vdw_switch(ref_sys *s, double veps, double vsig, double r2, double rt, double *dE)
+{
+    return vdw_switch(&s->param, dE, veps, vsig, r2, rt);
+}
+
+double ref_coulomb(ref_sys *s, int type, double qel, double r2, double rt, double *dE)
+{
+    if (type == SHIFT1) return coulomb_shift1(&s->param, dE, qel, r2, rt);
+    if (type == SHIFT2) return coulomb_shift2(&s->param, dE, qel, r2, rt);
+    if (type == SWITCH) return coulomb_switch(&s->param, dE, qel, r2, rt);
+    *dE = 0.;
+    return 0.;
+}
+
+/* energy.c:216-345 over the current list; out = elec, vdw, virelec, virvdw. */
+void ref_nonbond(ref_sys *s, int elecType, const double *x, const double *y, const double *z,
+                 double *fx, double *fy, double *fz, const double *q,
+                 const double *eps, const double *sig, double *out)
+{
+    s->ctrl.elecType = (enum ELEC_TYPE)elecType;
+    s->ctrl.vdwType = VSWITCH;
+    init_energy_ptrs(&s->ctrl);
+    memset(&s->ener, 0, sizeof s->ener);
+    nonbond_energy(&s->param, &s->ener, &s->box, x, y, z, fx, fy, fz, q, eps, sig,
+                   s->neighList, s->neighPair, s->neighOrder);
+    out[0] = s->ener.elec;
+    out[1] = s->ener.vdw;
+    out[2] = s->ener.virelec;
+    out[3] = s->ener.virvdw;
+}
+
+/* Classic Ewald (ewald.c:10-354) with explicit kmax on each axis. */
+double ref_ewald_rec(ref_sys *s, int kmax, const double *x, const double *y, const double *z,
+                     double *fx, double *fy, double *fz, const double *q,
+                     double *stress, double *vir)
+{
+    s->ctrl.keyAlpha = 1;
+    s->ctrl.keyMmax = 1;
+    s->ewald.prec = 1e-6;
+    s->ewald.m1max = s->ewald.m2max = s->ewald.m3max = kmax;
+    init_ewald(&s->ctrl, &s->param, &s->ewald, &s->box);
+    double e = ewald_rec(&s->param, &s->ewald, &s->box, x, y, z, fx, fy, fz, q, stress, vir);
+    ewald_free(&s->ewald);
+    return e;
+}
+
+/* vv_integrate (integrate.c:1479-1634) for NVE.  out = kin, virshake. */
+void ref_vv(ref_sys *s, int stage, double *x, double *y, double *z,
+            double *vx, double *vy, double *vz, double *fx, double *fy, double *fz,
+            double *mass, double *rmass, double *out)
+{
+    if (!s->shakeAlloc) {
+        integrators_allocate_arrays(&s->ctrl, &s->param);
+        shake_allocate_arrays(&s->param);
+        s->shakeAlloc = 1;
+    }
+    vv_integrate(&s->ctrl, &s->param, &s->ener, &s->box, &s->bath, s->cons, x, y, z,
+                 vx, vy, vz, fx, fy, fz, mass, rmass, s->nAtConst, stage);
+    out[0] = s->ener.kin;
+    out[1] = s->ener.virshake;
+}
+
+/* init_constvel (init.c:724-833): projects velocities onto the
+ * constraint manifold. */
+void init_constvel(PARAM *param, PBC *box, CONSTRAINT constList[], double x[],
+                   double y[], double z[], double vx[], double vy[], double vz[],
+                   double mass[], int nAtConst[]);
+
+void ref_constvel(ref_sys *s, double *x, double *y, double *z,
+                  double *vx, double *vy, double *vz, double *mass)
+{
+    init_constvel(&s->param, &s->box, s->cons, x, y, z, vx, vy, vz, mass, s->nAtConst);
+}
