@@ -1,0 +1,207 @@
+"""Benchmark: ns/day and pair-interactions/s of the 30,000-atom SPME water
+box in mixed precision on one MI355X, with the accuracy against FP64, the
+pair kernel's roofline and the FP64 CPU oracle timed on the host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one MD step of the whole system (VV stage 1 with SHAKE, list
+rebuild every 20 steps, SPME + real-space Ewald/LJ forces, VV stage 2
+with SHAKE), all on the device.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "ns/day & pair-interactions/s, 30k-atom SPME water; mixed vs FP64 @ stated err"
+FP32_PEAK = 157.3e12          # MI355X dense FP32 vector, FLOP/s
+HBM_PEAK = 8.0e12             # MI355X HBM3E, B/s
+FLOP_IN, FLOP_OUT = 64, 8     # per in-cutoff pair / per listed pair beyond rc
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def cpu_baseline(wb, cfg, budget_s=25.0):
+    """The FP64 oracle (a from-scratch port of the reference's path) on one
+    host core: one link-cell list build, step energies and VV stages,
+    combined as step + build/update."""
+    import oracle as O
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    o = O.OracleSystem(wb, cfg)
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    vx, vy, vz = wb.vx.copy(), wb.vy.copy(), wb.vz.copy()
+    t0 = time.perf_counter()
+    o.build_list(x, y, z, build=True)
+    t_list = time.perf_counter() - t0
+    f, e = o.energy(x, y, z)
+    steps, t_step = 0, 0.0
+    t_begin = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        o.vv(1, x, y, z, vx, vy, vz, f)
+        f, e = o.energy(x, y, z)
+        o.vv(2, x, y, z, vx, vy, vz, f)
+        t_step += time.perf_counter() - t0
+        steps += 1
+        if time.perf_counter() - t_begin > budget_s * 0.6 or steps >= 10:
+            break
+    per_step = t_step / steps + t_list / cfg["update"]
+    ns_day = cfg["timeStep"] * 1e-3 * 86400.0 / per_step
+    return dict(value=ns_day, unit="ns/day", cores=1, kind="port",
+                pair_interactions_per_s=e["nInCut"] / per_step,
+                s_per_step=per_step,
+                sample=(f"FP64 oracle on 1 host core ({os.uname().machine}, nproc {os.cpu_count()}): "
+                        f"1 link-cell list build ({t_list:.2f} s) + {steps} MD steps of the same "
+                        f"30k-atom box ({t_step / steps:.2f} s/step), per step = step + build/20"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sharded "
+              "multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import mdbas_amd as M
+    from mdbas_amd import configs
+    cfg = configs.CONFIGS["c1"]
+    wb = configs.build(cfg)
+    N, K, W = wb.n, args.steps, args.warmup
+    log(f"workload {cfg['name']}: {N} atoms, K={K}, W={W}")
+
+    # accuracy at the initial coordinates: mixed vs FP64 forces
+    forces = {}
+    for prec in (M.MDG_FP64, M.MDG_MIXED):
+        c = M.Context(wb, precision=prec)
+        c.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        c.initial_energy()
+        forces[prec] = np.stack(c.download()[6:9])
+        c.close()
+    f64, fmx = forces[M.MDG_FP64], forces[M.MDG_MIXED]
+    eps_rms = rel(fmx, f64)
+    eps_max = float((np.linalg.norm(fmx - f64, axis=0) / np.linalg.norm(f64, axis=0)).max())
+    log(f"force error mixed vs FP64: rms {eps_rms:.3e} max {eps_max:.3e}")
+
+    ctx = M.Context(wb, precision=M.MDG_MIXED)
+    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    ctx.initial_energy()
+    if W:
+        ctx.run(W, use_graph=True)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+    sync()
+    t0 = time.perf_counter()
+    recs = ctx.run(K, use_graph=True)                  # returns after a stream sync
+    sync()
+    wall = time.perf_counter() - t0
+    R = M.records_to_dict(recs)
+    ms_step = wall / K * 1e3
+    ns_day = cfg["timeStep"] * 1e-3 * K * 86400.0 / wall
+    pin = float(R["nInCut"].mean())
+    pis = float(R["nInCut"].sum()) / wall
+    stats = ctx.list_stats()
+    log(f"timed {K} steps: {wall:.3f} s, {ms_step:.4f} ms/step, {ns_day:.1f} ns/day")
+
+    # kernel durations: the same K steps again as one captured graph with
+    # HIP events around every launch on the context stream
+    ctx.set_profiling(True)
+    Kp = min(K, 200)
+    recs2 = M.records_to_dict(ctx.run(Kp, use_graph=2))
+    ph = ctx.phase_times()
+    ctx.set_profiling(False)
+    log(f"phases (ms): {ph}")
+    plist = float(stats["atom_pairs"])
+    pin2 = float(recs2["nInCut"].mean())
+    flop = FLOP_IN * pin2 + FLOP_OUT * (plist - pin2)
+    achieved = flop / (ph["pair_ms"] * 1e-3)
+    G = cfg["grid"] ** 3
+    spme_bytes = 56 * N + 32 * G
+
+    # energy drift of the mixed run against an FP64 run of the same steps
+    e_tot = R["tot"]
+    t_ns = np.arange(K) * cfg["timeStep"] * 1e-3
+    drift_mixed = float(np.polyfit(t_ns, e_tot / 418.4, 1)[0]) / wb.n_waters
+    c64 = M.Context(wb, precision=M.MDG_FP64)
+    c64.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    c64.initial_energy()
+    if W:
+        c64.run(W, use_graph=True)
+    r64 = M.records_to_dict(c64.run(K, use_graph=True))
+    c64.close()
+    drift_64 = float(np.polyfit(t_ns, r64["tot"] / 418.4, 1)[0]) / wb.n_waters
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (FP64 oracle, 1 core)...")
+        cpu = cpu_baseline(wb, cfg)
+        log(f"cpu: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": ns_day,
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("tip3p_10k_mixed: 10,000 rigid TIP3P waters (30,000 atoms), 67.0 A "
+                         "cubic box, seed 1; SPME 64^3 order 4, alpha 0.32; cutoff 12 A, LJ "
+                         "switch from 10 A, skin 2 A, list every 20 steps; SHAKE; dt 2 fs; "
+                         "FP32 pair/spline math, int64 fixed-point forces"),
+            "atoms": N, "dt_fs": cfg["timeStep"] * 1e3, "precision": "mixed"},
+        "pair_interactions_per_s": pis,
+        "pair_interactions_per_step": pin,
+        "listed_pairs": plist,
+        "accuracy": {"force_rms_rel_err_vs_fp64": eps_rms, "force_max_rel_err_vs_fp64": eps_max,
+                     "energy_drift_kcal_mol_ns_per_water_mixed": drift_mixed,
+                     "energy_drift_kcal_mol_ns_per_water_fp64": drift_64,
+                     "drift_window_steps": K},
+        "phases_ms": ph,
+        "roofline": {"kernel": "k_pair_cluster<MIXED>", "bound": "valu", "achieved": achieved / 1e12,
+                     "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK,
+                     "traffic": None,
+                     "flop_per_launch": flop, "launch_ms": ph["pair_ms"]},
+        "roofline_spme": {"bound": "hbm", "achieved": spme_bytes / (ph["spme_ms"] * 1e-3) / 1e9,
+                          "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": spme_bytes / (ph["spme_ms"] * 1e-3) / HBM_PEAK,
+                          "bytes_per_step": spme_bytes},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (ns_day / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

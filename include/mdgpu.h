@@ -1,0 +1,253 @@
+/*
+ * mdgpu.h -- C ABI of libmdgpu, the MI355X (gfx950) nonbonded force path
+ * for MDBas-style molecular dynamics.
+ *
+ * The structs below have the same member order and types as the
+ * reference's include/global.h (enums are int-sized), so a reference host
+ * can pass its own CTRL / PARAM / PBC / EWALD / NEIGH / ENERGY / BATH /
+ * CONSTRAINT / BOND / ANGLE / DIHE pointers by cast.  Every drop-in entry
+ * point takes the reference function's parameter list with a leading
+ * context pointer and returns an int status instead of calling exit():
+ * 0 on success, otherwise the reference's ERROR_TYPE number (errors.h)
+ * where one applies, or one of the MDG_E* codes below.
+ *
+ * Threading: one host thread per context; every call is ordered on the
+ * context's single HIP stream.  Host arrays stay owned by the caller; the
+ * context owns all device memory.
+ */
+#ifndef MDGPU_H
+#define MDGPU_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference struct mirrors (global.h:165-316) ---------------------- */
+
+typedef struct {                       /* PBC, global.h:175-184 */
+    int type;                          /* 0 none, 1 cubic, 2 orthorhombic, 3 triclinic */
+    double a, a1, a2, a3, b, b1, b2, b3, c, c1, c2, c3;
+    double u, u1, u2, u3, v, v1, v2, v3, w, w1, w2, w3;
+    double pa, pb, pc, det, vol, vol0;
+    double stress1, stress2, stress3;
+    double stress4, stress5, stress6;
+    double stress7, stress8, stress9;
+} MDG_PBC;
+
+typedef struct {                       /* CTRL, global.h:195-217 */
+    int newjob;
+    int keyRand, keyTraj, keyProp, keyForF, keyRest;
+    int printOut, printProp, printTraj, printRest;
+    int keyMd, mdType;
+    int seed;
+    int keyMinim;
+    int keyLink, noLink;
+    int keyConstH;
+    int keyNb14, keyNumForce;
+    int keyEwald, keyAlpha, keyMmax;
+    int elecType;                      /* 0 none, 1 full, 2 shift1, 3 shift2, 4 switch */
+    int vdwType;                       /* 0 none, 1 full, 2 switch */
+    int integrator;                    /* 0 leap-frog, 1 velocity Verlet */
+    int ens;                           /* 0 NVE */
+} MDG_CTRL;
+
+typedef struct {                       /* PARAM, global.h:219-238 */
+    int step, nSteps;
+    double timeStep, rTimeStep;
+    int nAtom, nDegFree, nFrozen;
+    int nBond, nAngle, nDihedral, nImproper, nUb;
+    int nConst, maxCycle;
+    double tolShake;
+    double chargeConst, delr, scal14;
+    double cutOff, rcutOff, cutOff2, rcutOff2;
+    double cutOn, cutOn2, switch2;
+    double temp0, press0, kinTemp0;
+    double tolMinim, maxminsiz, maxminst;
+} MDG_PARAM;
+
+typedef struct { double tauT, chiT, tauP, chiP, compress; } MDG_BATH;   /* global.h:240-245 */
+
+typedef struct { int type; int a, b; double k, r0, beta; } MDG_BOND;     /* global.h:247-252 */
+typedef struct { int type; int a, b, c; double k, theta0; } MDG_ANGLE;  /* global.h:254-259 */
+typedef struct { int type; int order; int a, b, c, d; double k, phi0, mult; } MDG_DIHE;
+
+typedef struct {                       /* NEIGH, global.h:269-276 */
+    int update, linkRatio, nCells, sizeList, nPair14;
+} MDG_NEIGH;
+
+typedef struct {                       /* ENERGY, global.h:282-290 */
+    double tot, pot, kin;
+    double elec, vdw;
+    double bond, ang, ub, dihe, impr;
+    double conint, consv;
+    double virelec, virvdw, virbond, virub;
+    double virshake, virpot, virtot;
+} MDG_ENERGY;
+
+typedef struct { int a, b; double rc2; } MDG_CONSTRAINT;                /* global.h:296-300 */
+
+typedef struct {                       /* EWALD, global.h:311-316 */
+    int nbsp;
+    int mmax, m1max, m2max, m3max;
+    double prec, tol, tol1, alpha;
+} MDG_EWALD;
+
+/* ---- library codes ---------------------------------------------------- */
+
+enum {
+    MDG_FP64 = 0,                      /* FP64 everywhere */
+    MDG_MIXED = 1                      /* FP32 pair/spline math, int64 fixed-point forces */
+};
+
+enum {
+    MDG_OK = 0,
+    MDG_EHIP = 1001,                   /* a HIP / rocFFT call failed */
+    MDG_EUNSUPPORTED = 1002,           /* a setting this path does not implement */
+    MDG_ELIST = 1003,                  /* device list capacity exceeded */
+    MDG_EARG = 1004,                   /* bad argument (NULL, size mismatch) */
+    MDG_ESHAKE_ORDER = 1005            /* a constraint group re-diverged after converging */
+};
+
+typedef struct mdg_ctx mdg_ctx;
+
+/* Per-step energies written by mdg_run_steps (internal units, 10 J/mol). */
+typedef struct {
+    int step;
+    int listRebuilt;
+    double pot, kin, tot;
+    double elec, vdw;                  /* elec = direct + reciprocal + correction */
+    double eDir, eRec, eCorr;
+    double virelec, virvdw, virshake;
+    double nInCut;                     /* listed pairs with r <= rc this step */
+} MDG_STEP;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Creates a context for nAtom = param->nAtom atoms: copies the run
+ * settings, picks the list builder the way makelist does on its first call
+ * (list.c:38-57), sizes the SPME mesh from ewald->m1max..m3max as given
+ * (the reference's init_spme has already rounded them, spme.c:54-73) and
+ * builds the rocFFT plans.  precision is MDG_FP64 or MDG_MIXED. */
+mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, const MDG_PBC *box,
+                    const MDG_EWALD *ewald, const MDG_NEIGH *neigh, int precision,
+                    int *status);
+void mdg_destroy(mdg_ctx *ctx);
+const char *mdg_last_error(const mdg_ctx *ctx);
+int mdg_precision(const mdg_ctx *ctx);
+int mdg_key_link(const mdg_ctx *ctx);  /* 1 link cells, 0 O(N^2) */
+
+/* Per-atom parameters and the constraint list after setup() (init.c:248-616):
+ * charges, eps (sqrt(eps_kcal*418.4), io.c:1174), sig, masses, inverse
+ * masses, frozen flags, constraint counts per atom.  Builds the exclusion
+ * table from the constraints (consth; list.c:176-216 and the connectivity
+ * walk 764-862). */
+int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps[], const double sig[],
+                     const double mass[], const double rmass[], const int frozen[],
+                     const int nAtConst[], const MDG_CONSTRAINT constList[], int nConst);
+
+/* ---- drop-in mirrors (host arrays in, host arrays out) ------------------ */
+
+/* makelist, list.c:29-106 / list.h:28-31.  First call (ctrl->newjob == 1):
+ * exclusions plus the O(N^2) or link-cell list, built on the GPU, returned
+ * in the reference format through malloc'd out-parameters; the link-cell
+ * build wraps x,y,z in place as list.c:1927 does.  Later calls rebuild when
+ * param->step % neigh->update == 0 (realloc'ing *neighList if it grows).
+ * Only bonds, constraints and angles feed the exclusions; dihedrals or
+ * impropers give MDG_EUNSUPPORTED. */
+int mdg_makelist(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_PBC *box, MDG_NEIGH *neigh,
+                 MDG_CONSTRAINT constList[], MDG_BOND bond[], MDG_ANGLE angle[],
+                 MDG_DIHE dihe[], MDG_DIHE impr[], double x[], double y[], double z[],
+                 int frozen[], int **neighList, int **neighPair, int **neighOrder,
+                 int **neighList14, int ***exclList, int **exclPair);
+
+/* ewald_energy, energy.c:450-643 / energy.h:49-53: SPME reciprocal part
+ * (with its mean-force removal), real-space Ewald + switched LJ over the
+ * given list, excluded-pair correction.  Forces and ENERGY fields
+ * accumulate (+=) exactly as the reference's do; fx,fy,fz should hold
+ * zeros on entry, as energy() guarantees. */
+int mdg_ewald_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+                     MDG_EWALD *ewald, MDG_PBC *box, const double x[], const double y[],
+                     const double z[], double fx[], double fy[], double fz[], const double q[],
+                     const double eps[], const double sig[], const int neighList[],
+                     const int neighPair[], const int neighOrder[], int **exclList,
+                     const int exclPair[]);
+
+/* spme_energy, spme.c:357-699 / spme.h:44-45: adds reciprocal forces to
+ * fx,fy,fz, then removes the mean of the whole force arrays; writes
+ * stress[6], *virEwaldRec and the energy (self and net-charge terms
+ * included, computed on every call). */
+int mdg_spme_energy(mdg_ctx *ctx, MDG_PARAM *param, MDG_EWALD *ewald, MDG_PBC *box,
+                    const double x[], const double y[], const double z[], double fx[],
+                    double fy[], double fz[], const double q[], double stress[6],
+                    double *virEwaldRec, double *eEwaldRec);
+
+/* nonbond_energy, energy.c:216-345 / energy.h:39-42: cutoff Coulomb with
+ * the SHIFT1 / SHIFT2 / SWITCH functor of ctrl->elecType (elec.c:137-263)
+ * plus switched LJ over the given list; accumulates into ener and f. */
+int mdg_nonbond_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+                       MDG_PBC *box, const double x[], const double y[], const double z[],
+                       double fx[], double fy[], double fz[], const double q[],
+                       const double eps[], const double sig[], const int neighList[],
+                       const int neighPair[], const int neighOrder[]);
+
+/* energy, energy.c:103-205 / energy.h:30-37, for the nonbonded-only
+ * systems this path covers: zeroes energies, stress and forces, calls the
+ * Ewald or plain nonbonded path, sums pot / virpot, zeroes frozen atoms. */
+int mdg_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+               MDG_EWALD *ewald, MDG_PBC *box, MDG_NEIGH *neigh, const double x[],
+               const double y[], const double z[], double vx[], double vy[], double vz[],
+               double fx[], double fy[], double fz[], const double q[], const double eps[],
+               const double sig[], const int frozen[], const int neighList[],
+               const int neighPair[], const int neighOrder[], int **exclList,
+               const int exclPair[]);
+
+/* vv_integrate, integrate.c:1479-1634 / integrate.h:72-76, NVE: stage 1
+ * = half kick, drift, SHAKE-R; stage 2 = half kick, SHAKE-V, kinetic
+ * energy, kinetic stress, wrap.  SHAKE keeps the reference's global
+ * stopping rule (shake.c:243-338) without a host round trip. */
+int mdg_vv_integrate(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+                     MDG_PBC *box, MDG_BATH *bath, MDG_CONSTRAINT constList[], double *x,
+                     double *y, double *z, double *vx, double *vy, double *vz, double *fx,
+                     double *fy, double *fz, double *mass, double *rmass, int *nAtConst,
+                     int stage);
+
+/* ---- device-resident MD ------------------------------------------------ */
+
+int mdg_upload_state(mdg_ctx *ctx, const double x[], const double y[], const double z[],
+                     const double vx[], const double vy[], const double vz[]);
+int mdg_download_state(mdg_ctx *ctx, double x[], double y[], double z[], double vx[],
+                       double vy[], double vz[], double fx[], double fy[], double fz[]);
+
+/* Force and energy of the uploaded state (the step-0 energy() call of
+ * mdbas.c:174), building the device list first. */
+int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec);
+
+/* nsteps MD steps in the order of mdbas.c:244-301 (VV stage 1, list
+ * rebuild every neigh->update steps, energy, VV stage 2), entirely on the
+ * device; with use_graph the step sequence is captured once in a hipGraph
+ * and replayed.  records (may be NULL) receives nsteps entries, copied
+ * after the last step: there is no per-step host round trip. */
+int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *records);
+
+/* Average kernel durations (ms) of the last mdg_run_steps, measured with
+ * HIP events on the context stream: [0] pair kernel, [1] SPME (spline to
+ * gather), [2] integrator, [3] list rebuild (per rebuild), [4] whole step.
+ * Events are recorded only when profiling is switched on. */
+int mdg_set_profiling(mdg_ctx *ctx, int on);
+int mdg_phase_times(mdg_ctx *ctx, double out[5]);
+
+/* Counters of the device cluster-pair list: [0] cluster pairs,
+ * [1] listed atom pairs (mask bits), [2] clusters. */
+int mdg_list_stats(mdg_ctx *ctx, long out[3]);
+
+/* The device cluster-pair list as atom pairs (i < j), sorted, for the
+ * bit-exact comparison with the reference list: writes up to cap pairs,
+ * returns the count through *npairs. */
+int mdg_list_pairs(mdg_ctx *ctx, int *pairs_i, int *pairs_j, long cap, long *npairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,363 @@
+"""mdbas_amd -- MI355X (gfx950) nonbonded force path with MDBas semantics.
+
+The product is ``libmdgpu.so`` (HIP kernels behind the C ABI declared in
+``include/mdgpu.h``).  This module is the thin Python side of that ABI:
+ctypes mirrors of the reference structs (global.h) and a ``Context`` that
+forwards to the ``mdg_*`` entry points.  There is no CPU fallback: loading
+fails loudly if the library is missing, and every compute call runs on
+the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdgpu.so")
+
+MDG_FP64, MDG_MIXED = 0, 1
+ERRORS = {1001: "HIP/rocFFT error", 1002: "unsupported setting", 1003: "list capacity",
+          1004: "bad argument", 1005: "SHAKE group re-diverged", 311: "SHAKE not converged",
+          110: "1-4 pair without dihedral", 111: "exclusion list last atom", 412: "link cells"}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class MDG_PBC(C.Structure):
+    _fields_ = [("type", C.c_int)] + [(k, C.c_double) for k in (
+        "a a1 a2 a3 b b1 b2 b3 c c1 c2 c3 u u1 u2 u3 v v1 v2 v3 w w1 w2 w3 "
+        "pa pb pc det vol vol0 stress1 stress2 stress3 stress4 stress5 stress6 "
+        "stress7 stress8 stress9").split()]
+
+
+class MDG_CTRL(C.Structure):
+    _fields_ = [(k, C.c_int) for k in (
+        "newjob keyRand keyTraj keyProp keyForF keyRest printOut printProp printTraj printRest "
+        "keyMd mdType seed keyMinim keyLink noLink keyConstH keyNb14 keyNumForce keyEwald "
+        "keyAlpha keyMmax elecType vdwType integrator ens").split()]
+
+
+class MDG_PARAM(C.Structure):
+    _fields_ = [("step", C.c_int), ("nSteps", C.c_int), ("timeStep", C.c_double),
+                ("rTimeStep", C.c_double), ("nAtom", C.c_int), ("nDegFree", C.c_int),
+                ("nFrozen", C.c_int), ("nBond", C.c_int), ("nAngle", C.c_int),
+                ("nDihedral", C.c_int), ("nImproper", C.c_int), ("nUb", C.c_int),
+                ("nConst", C.c_int), ("maxCycle", C.c_int), ("tolShake", C.c_double)] + [
+                (k, C.c_double) for k in (
+                    "chargeConst delr scal14 cutOff rcutOff cutOff2 rcutOff2 cutOn cutOn2 "
+                    "switch2 temp0 press0 kinTemp0 tolMinim maxminsiz maxminst").split()]
+
+
+class MDG_BATH(C.Structure):
+    _fields_ = [(k, C.c_double) for k in "tauT chiT tauP chiP compress".split()]
+
+
+class MDG_NEIGH(C.Structure):
+    _fields_ = [(k, C.c_int) for k in "update linkRatio nCells sizeList nPair14".split()]
+
+
+class MDG_ENERGY(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "tot pot kin elec vdw bond ang ub dihe impr conint consv virelec virvdw virbond virub "
+        "virshake virpot virtot").split()]
+
+
+class MDG_CONSTRAINT(C.Structure):
+    _fields_ = [("a", C.c_int), ("b", C.c_int), ("rc2", C.c_double)]
+
+
+class MDG_EWALD(C.Structure):
+    _fields_ = [("nbsp", C.c_int), ("mmax", C.c_int), ("m1max", C.c_int), ("m2max", C.c_int),
+                ("m3max", C.c_int), ("prec", C.c_double), ("tol", C.c_double),
+                ("tol1", C.c_double), ("alpha", C.c_double)]
+
+
+class MDG_STEP(C.Structure):
+    _fields_ = [("step", C.c_int), ("listRebuilt", C.c_int)] + [(k, C.c_double) for k in (
+        "pot kin tot elec vdw eDir eRec eCorr virelec virvdw virshake nInCut").split()]
+
+
+# Every entry point include/mdgpu.h declares (checked by the ABI test).
+EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg_set_topology "
+           "mdg_makelist mdg_ewald_energy mdg_spme_energy mdg_nonbond_energy mdg_energy "
+           "mdg_vv_integrate mdg_upload_state mdg_download_state mdg_initial_energy "
+           "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
+           "mdg_list_pairs").split()
+
+_LIB = None
+
+
+class MdgError(RuntimeError):
+    pass
+
+
+def lib_path() -> str:
+    return LIB_PATH
+
+
+def load():
+    """Load libmdgpu.so; raises if it was not built (no fallback path)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise MdgError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or "
+                       "python mdbas_amd/build.py) first")
+    L = C.CDLL(LIB_PATH)
+    L.mdg_create.restype = C.c_void_p
+    L.mdg_create.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                             C.POINTER(C.c_int)]
+    L.mdg_destroy.argtypes = [C.c_void_p]
+    L.mdg_last_error.restype = C.c_char_p
+    L.mdg_last_error.argtypes = [C.c_void_p]
+    for f in EXPORTS:
+        fn = getattr(L, f)
+        if f not in ("mdg_create", "mdg_destroy", "mdg_last_error"):
+            fn.restype = C.c_int
+    _LIB = L
+    return L
+
+
+def P(a):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    raise TypeError(a.dtype)
+
+
+def reference_structs(wb, cfg: dict):
+    """The CTRL/PARAM/PBC/EWALD/NEIGH values a reference SIMU deck for this
+    workload produces (io.c:29-723, init.c:102-109, setup's consth)."""
+    ctrl = MDG_CTRL()
+    ctrl.newjob = 1
+    ctrl.keyMd = 1
+    ctrl.mdType = 1
+    ctrl.keyConstH = 1
+    ctrl.keyEwald = 2
+    ctrl.keyAlpha = 1
+    ctrl.keyMmax = 1
+    ctrl.elecType = 0
+    ctrl.vdwType = 2
+    ctrl.integrator = 1
+    ctrl.ens = 0
+    p = MDG_PARAM()
+    p.nAtom = wb.n
+    p.timeStep = cfg["timeStep"]
+    p.rTimeStep = 1.0 / cfg["timeStep"]
+    p.nConst = len(wb.ca)
+    p.maxCycle = cfg["maxCycle"]
+    p.tolShake = cfg["tolShake"]
+    p.chargeConst = 332.0716 * 418.4
+    p.delr = cfg["delr"]
+    p.cutOff = cfg["cutOff"]
+    p.cutOn = cfg["cutOn"]
+    p.rcutOff = 1.0 / p.cutOff
+    p.cutOff2 = p.cutOff * p.cutOff
+    p.rcutOff2 = 1.0 / p.cutOff2
+    p.cutOn2 = p.cutOn * p.cutOn
+    p.switch2 = 1.0 / (p.cutOff2 - p.cutOn2) ** 3
+    p.temp0 = cfg.get("temp", 300.0)
+    p.nDegFree = 3 * wb.n - 3 - len(wb.ca)
+    b = MDG_PBC()
+    b.type = 1
+    b.a1 = b.b2 = b.c3 = float(wb.L)
+    e = MDG_EWALD()
+    e.nbsp = cfg["nbsp"]
+    e.m1max = e.m2max = e.m3max = cfg["grid"]
+    e.mmax = cfg["grid"] ** 3
+    e.alpha = cfg["alpha"]
+    e.prec = 1e-6
+    ng = MDG_NEIGH()
+    ng.update = cfg["update"]
+    ng.linkRatio = cfg["linkRatio"]
+    return ctrl, p, b, e, ng
+
+
+class Context:
+    """One device context (one HIP stream) for one water box."""
+
+    def __init__(self, wb, cfg: dict | None = None, precision: int = MDG_FP64):
+        self.L = load()
+        self.wb = wb
+        self.cfg = dict(wb.params) if cfg is None else cfg
+        self.ctrl, self.param, self.pbc, self.ewald, self.neigh = reference_structs(wb, self.cfg)
+        st = C.c_int(0)
+        h = self.L.mdg_create(C.byref(self.ctrl), C.byref(self.param), C.byref(self.pbc),
+                              C.byref(self.ewald), C.byref(self.neigh), int(precision),
+                              C.byref(st))
+        if not h:
+            raise MdgError(f"mdg_create failed: {st.value} {ERRORS.get(st.value, '')}")
+        self.h = C.c_void_p(h)
+        self.precision = precision
+        n = wb.n
+        self.cons = (MDG_CONSTRAINT * max(1, len(wb.ca)))()
+        for k in range(len(wb.ca)):
+            self.cons[k].a, self.cons[k].b, self.cons[k].rc2 = int(wb.ca[k]), int(wb.cb[k]), float(wb.rc2[k])
+        self._chk(self.L.mdg_set_topology(self.h, P(wb.q), P(wb.eps), P(wb.sig), P(wb.mass),
+                                          P(wb.rmass), P(wb.frozen), P(wb.nAtConst), self.cons,
+                                          len(wb.ca)), "mdg_set_topology")
+        self.n = n
+        self._lists = None
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.mdg_last_error(self.h).decode() if getattr(self, "h", None) else ""
+            raise MdgError(f"{what} -> {rc} ({ERRORS.get(rc, '')}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mdg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def key_link(self) -> int:
+        return self.L.mdg_key_link(self.h)
+
+    # ---- drop-in mirrors -------------------------------------------------
+    def makelist(self, x, y, z, step: int = 0):
+        """mdg_makelist; returns (neighList, neighPair, neighOrder, exclOff,
+        exclLst) as numpy arrays (copies of the malloc'd reference-format
+        arrays, which are then freed)."""
+        if self._lists is None:
+            self._lists = dict(nl=C.POINTER(C.c_int)(), np=C.POINTER(C.c_int)(),
+                               no=C.POINTER(C.c_int)(), n14=C.POINTER(C.c_int)(),
+                               el=C.POINTER(C.POINTER(C.c_int))(), ep=C.POINTER(C.c_int)())
+        d = self._lists
+        self.param.step = step
+        self._chk(self.L.mdg_makelist(self.h, C.byref(self.ctrl), C.byref(self.param),
+                                      C.byref(self.pbc), C.byref(self.neigh), self.cons, None,
+                                      None, None, None, P(x), P(y), P(z), P(self.wb.frozen),
+                                      C.byref(d["nl"]), C.byref(d["np"]), C.byref(d["no"]),
+                                      C.byref(d["n14"]), C.byref(d["el"]), C.byref(d["ep"])),
+                  "mdg_makelist")
+        n = self.n
+        np_ = np.ctypeslib.as_array(d["np"], shape=(n,)).copy()
+        no = np.ctypeslib.as_array(d["no"], shape=(n,)).copy()
+        tot = int(np_.sum())
+        nl = np.ctypeslib.as_array(d["nl"], shape=(max(tot, 1),))[:tot].copy()
+        ep = np.ctypeslib.as_array(d["ep"], shape=(n,)).copy()
+        rows = n if self.key_link else n - 1
+        off = np.zeros(n + 1, np.int32)
+        lst = []
+        for i in range(n):
+            off[i + 1] = off[i] + ep[i]
+            if i < rows and ep[i]:
+                lst.extend(d["el"][i][k] for k in range(ep[i]))
+        self._excl = (d["el"], d["ep"])
+        return nl, np_, no, off, np.array(lst, np.int32)
+
+    def ewald_energy(self, x, y, z, nl, np_, no, f=None):
+        """mdg_ewald_energy over a reference-format list; returns forces (3,N)
+        and the accumulated ENERGY struct."""
+        n = self.n
+        f = np.zeros((3, n)) if f is None else f
+        e = MDG_ENERGY()
+        el, ep = self._excl
+        self._chk(self.L.mdg_ewald_energy(self.h, C.byref(self.ctrl), C.byref(self.param),
+                                          C.byref(e), C.byref(self.ewald), C.byref(self.pbc),
+                                          P(x), P(y), P(z), P(f[0]), P(f[1]), P(f[2]),
+                                          P(self.wb.q), P(self.wb.eps), P(self.wb.sig), P(nl),
+                                          P(np_), P(no), el, ep), "mdg_ewald_energy")
+        return f, e
+
+    def nonbond_energy(self, elec_type, x, y, z, nl, np_, no):
+        n = self.n
+        f = np.zeros((3, n))
+        e = MDG_ENERGY()
+        saved = self.ctrl.elecType
+        self.ctrl.elecType = elec_type
+        try:
+            self._chk(self.L.mdg_nonbond_energy(self.h, C.byref(self.ctrl), C.byref(self.param),
+                                                C.byref(e), C.byref(self.pbc), P(x), P(y), P(z),
+                                                P(f[0]), P(f[1]), P(f[2]), P(self.wb.q),
+                                                P(self.wb.eps), P(self.wb.sig), P(nl), P(np_),
+                                                P(no)), "mdg_nonbond_energy")
+        finally:
+            self.ctrl.elecType = saved
+        return f, e
+
+    def spme_energy(self, x, y, z, f=None):
+        n = self.n
+        f = np.zeros((3, n)) if f is None else f
+        st = np.zeros(6)
+        vir = C.c_double(0)
+        en = C.c_double(0)
+        self._chk(self.L.mdg_spme_energy(self.h, C.byref(self.param), C.byref(self.ewald),
+                                         C.byref(self.pbc), P(x), P(y), P(z), P(f[0]), P(f[1]),
+                                         P(f[2]), P(self.wb.q), P(st), C.byref(vir),
+                                         C.byref(en)), "mdg_spme_energy")
+        return en.value, f, st, vir.value
+
+    def vv_integrate(self, stage, x, y, z, vx, vy, vz, f):
+        e = MDG_ENERGY()
+        nat = self.wb.nAtConst
+        self._chk(self.L.mdg_vv_integrate(self.h, C.byref(self.ctrl), C.byref(self.param),
+                                          C.byref(e), C.byref(self.pbc), None, self.cons, P(x),
+                                          P(y), P(z), P(vx), P(vy), P(vz), P(f[0]), P(f[1]),
+                                          P(f[2]), P(self.wb.mass), P(self.wb.rmass), P(nat),
+                                          int(stage)), "mdg_vv_integrate")
+        return e
+
+    # ---- device-resident ------------------------------------------------
+    def upload(self, x, y, z, vx, vy, vz):
+        self._chk(self.L.mdg_upload_state(self.h, P(x), P(y), P(z), P(vx), P(vy), P(vz)),
+                  "mdg_upload_state")
+
+    def download(self):
+        n = self.n
+        a = [np.zeros(n) for _ in range(9)]
+        self._chk(self.L.mdg_download_state(self.h, *[P(v) for v in a]), "mdg_download_state")
+        return a
+
+    def initial_energy(self) -> MDG_STEP:
+        r = MDG_STEP()
+        self._chk(self.L.mdg_initial_energy(self.h, C.byref(r)), "mdg_initial_energy")
+        return r
+
+    def run(self, nsteps: int, use_graph: bool = True):
+        recs = (MDG_STEP * max(1, nsteps))()
+        self._chk(self.L.mdg_run_steps(self.h, int(nsteps), int(bool(use_graph)), recs),
+                  "mdg_run_steps")
+        return recs
+
+    def set_profiling(self, on: bool):
+        self._chk(self.L.mdg_set_profiling(self.h, int(bool(on))), "mdg_set_profiling")
+
+    def phase_times(self):
+        out = np.zeros(5)
+        self._chk(self.L.mdg_phase_times(self.h, P(out)), "mdg_phase_times")
+        return dict(pair_ms=out[0], spme_ms=out[1], integrate_ms=out[2], rebuild_ms=out[3],
+                    step_ms=out[4])
+
+    def list_stats(self):
+        out = (C.c_long * 3)()
+        self._chk(self.L.mdg_list_stats(self.h, out), "mdg_list_stats")
+        return dict(cluster_pairs=out[0], atom_pairs=out[1], clusters=out[2])
+
+    def list_pairs(self):
+        cnt = C.c_long(0)
+        self._chk(self.L.mdg_list_pairs(self.h, None, None, C.c_long(0), C.byref(cnt)),
+                  "mdg_list_pairs")
+        k = cnt.value
+        pi = np.zeros(max(k, 1), np.int32)
+        pj = np.zeros(max(k, 1), np.int32)
+        self._chk(self.L.mdg_list_pairs(self.h, P(pi), P(pj), C.c_long(k), C.byref(cnt)),
+                  "mdg_list_pairs")
+        return pi[:k], pj[:k]
+
+
+def records_to_dict(recs):
+    keys = [f[0] for f in MDG_STEP._fields_]
+    return {k: np.array([getattr(r, k) for r in recs]) for k in keys}

@@ -1,0 +1,719 @@
+// mdg_context.hip -- the C ABI of libmdgpu (include/mdgpu.h): context,
+// topology, the drop-in mirrors of the reference entry points and the
+// device-resident MD loop.
+#include "mdg_internal.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
+#define TOLLIST_ 0.01            // global.h:59
+#define SQRTPI_C 1.772453850905516
+#define PI_C 3.141592653589793
+#define TWOPI_C 6.283185307179586
+
+// ---------------------------------------------------------------- helpers
+
+// init.c:840-908, restated: reciprocal vectors divided by det, widths,
+// volume.  The reference computes the same expressions, so the result is
+// bit-identical to its init_box.
+static void box_from_mirror(const MDG_PBC *in, MdgBox &b)
+{
+#pragma clang fp contract(off)
+    b.type = in->type;
+    b.a1 = in->a1; b.a2 = in->a2; b.a3 = in->a3;
+    b.b1 = in->b1; b.b2 = in->b2; b.b3 = in->b3;
+    b.c1 = in->c1; b.c2 = in->c2; b.c3 = in->c3;
+    b.u1 = b.b2 * b.c3 - b.c2 * b.b3;
+    b.u2 = b.c1 * b.b3 - b.b1 * b.c3;
+    b.u3 = b.b1 * b.c2 - b.c1 * b.b2;
+    b.v1 = b.c2 * b.a3 - b.a2 * b.c3;
+    b.v2 = b.a1 * b.c3 - b.c1 * b.a3;
+    b.v3 = b.c1 * b.a2 - b.a1 * b.c2;
+    b.w1 = b.a2 * b.b3 - b.b2 * b.a3;
+    b.w2 = b.b1 * b.a3 - b.a1 * b.b3;
+    b.w3 = b.a1 * b.b2 - b.b1 * b.a2;
+    b.det = b.a1 * b.u1 + b.a2 * b.u2 + b.a3 * b.u3;
+    b.vol = fabs(b.det);
+    b.pa = b.vol / sqrt(b.u1 * b.u1 + b.u2 * b.u2 + b.u3 * b.u3);
+    b.pb = b.vol / sqrt(b.v1 * b.v1 + b.v2 * b.v2 + b.v3 * b.v3);
+    b.pc = b.vol / sqrt(b.w1 * b.w1 + b.w2 * b.w2 + b.w3 * b.w3);
+    b.u = 1.0 / b.pa;
+    b.v = 1.0 / b.pb;
+    b.w = 1.0 / b.pc;
+    if (fabs(b.det) > 0.) {
+        b.u1 /= b.det; b.u2 /= b.det; b.u3 /= b.det;
+        b.v1 /= b.det; b.v2 /= b.det; b.v3 /= b.det;
+        b.w1 /= b.det; b.w2 /= b.det; b.w3 /= b.det;
+    } else {
+        b.u1 = b.u2 = b.u3 = b.v1 = b.v2 = b.v3 = b.w1 = b.w2 = b.w3 = 0.;
+    }
+}
+
+// io.c:714-723 derived constants plus the rest of the run settings.
+static void params_from_mirror(const MDG_CTRL *ctrl, const MDG_PARAM *p, const MDG_EWALD *ew,
+                               const MDG_NEIGH *ng, MdgParams &P)
+{
+#pragma clang fp contract(off)
+    P.n = p->nAtom;
+    P.cutOff = p->cutOff;
+    P.cutOn = p->cutOn;
+    P.delr = p->delr;
+    P.chargeConst = p->chargeConst;
+    P.alpha = ew ? ew->alpha : 0.;
+    P.rcutOff = 1. / P.cutOff;
+    P.cutOff2 = P.cutOff * P.cutOff;
+    P.rcutOff2 = 1. / P.cutOff2;
+    P.cutOn2 = P.cutOn * P.cutOn;
+    double d = P.cutOff2 - P.cutOn2;
+    P.switch2 = 1. / (d * d * d);
+    double cutnb = P.cutOff + P.delr;        // list.c:906-907
+    P.cutnb2 = cutnb * cutnb;
+    P.timeStep = p->timeStep;
+    P.rTimeStep = 1. / p->timeStep;
+    P.tolShake = p->tolShake;
+    P.maxCycle = p->maxCycle;
+    P.update = ng ? ng->update : 20;
+    P.linkRatio = ng ? std::min(std::max(ng->linkRatio, 1), 5) : 1;
+    P.m1 = ew ? ew->m1max : 0;
+    P.m2 = ew ? ew->m2max : 0;
+    P.m3 = ew ? ew->m3max : 0;
+    P.nbsp = ew ? ew->nbsp : 4;
+    P.elecType = ctrl ? ctrl->elecType : 0;
+    P.vdwOn = ctrl ? (ctrl->vdwType == 2) : 1;
+}
+
+template <typename T>
+static int dmalloc(mdg_ctx *ctx, T **p, size_t n)
+{
+    HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+    return MDG_OK;
+}
+
+template <typename T>
+static int h2d(mdg_ctx *ctx, T *d, const T *h, size_t n)
+{
+    if (n) HIPCHK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return MDG_OK;
+}
+
+template <typename T>
+static int d2h(mdg_ctx *ctx, T *h, const T *d, size_t n)
+{
+    if (n) HIPCHK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    return MDG_OK;
+}
+
+static int sync(mdg_ctx *ctx)
+{
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDG_OK;
+}
+
+int mdg_alloc_state(mdg_ctx *ctx)
+{
+    const int n = ctx->n;
+    double **arr[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->fx,
+                      &ctx->fy, &ctx->fz, &ctx->q, &ctx->eps, &ctx->sig, &ctx->mass, &ctx->rmass,
+                      &ctx->frx64, &ctx->fry64, &ctx->frz64};
+    for (double **a : arr) RETCHK(dmalloc(ctx, a, n));
+    RETCHK(dmalloc(ctx, &ctx->frx, n));
+    RETCHK(dmalloc(ctx, &ctx->fry, n));
+    RETCHK(dmalloc(ctx, &ctx->frz, n));
+    RETCHK(dmalloc(ctx, &ctx->frozen, n));
+    RETCHK(dmalloc(ctx, &ctx->nAtConst, n));
+    RETCHK(dmalloc(ctx, &ctx->exOff, n + 1));
+    RETCHK(dmalloc(ctx, &ctx->epart, PART_TOTAL));
+    RETCHK(dmalloc(ctx, &ctx->scal, S_COUNT));
+    RETCHK(dmalloc(ctx, &ctx->ctrl, CTRL_WORDS));
+    RETCHK(mdg_fill_i32(ctx, ctx->ctrl, CTRL_WORDS, 0));
+    RETCHK(mdg_fill_f64(ctx, ctx->scal, S_COUNT, 0.0));
+    RETCHK(mdg_fill_i32(ctx, ctx->frozen, n, 0));
+    RETCHK(mdg_fill_i32(ctx, ctx->nAtConst, n, 0));
+    return MDG_OK;
+}
+
+// ---------------------------------------------------------------- exclusions
+
+// list.c:108-894 (ordered, pair kept at the smaller index) and
+// list.c:1125-1791 (symmetric).  Bonds and constraints are appended as
+// given; angle pairs and the 1-3 connectivity walk only when absent.  A
+// 1-4 path is DIHE_NONPARAM_ERROR (110) since dihedrals are not handled.
+int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nConst,
+                         const MDG_CONSTRAINT *cons, int nAngle, const MDG_ANGLE *angle)
+{
+    const int n = ctx->n;
+    const bool sym = ctx->keyLink != 0;
+    std::vector<std::vector<int>> ex(n), con(n);
+    auto add = [&](int i, int j, bool dedupe) {
+        int ii = i, jj = j;
+        if (!sym && jj < ii) std::swap(ii, jj);
+        if (dedupe)
+            for (int v : ex[ii])
+                if (v == jj) return;
+        ex[ii].push_back(jj);
+        if (sym) ex[jj].push_back(ii);
+    };
+    for (int i = 0; i < nBond; i++) {
+        add(bond[i].a, bond[i].b, false);
+        con[bond[i].a].push_back(bond[i].b);
+        con[bond[i].b].push_back(bond[i].a);
+    }
+    for (int i = 0; i < nConst; i++) {
+        add(cons[i].a, cons[i].b, false);
+        con[cons[i].a].push_back(cons[i].b);
+        con[cons[i].b].push_back(cons[i].a);
+    }
+    for (int i = 0; i < nAngle; i++) {
+        add(angle[i].a, angle[i].b, true);
+        add(angle[i].a, angle[i].c, true);
+        add(angle[i].b, angle[i].c, true);
+    }
+    for (int ia = 0; ia < n; ia++)
+        for (int ib : con[ia])
+            for (int ic : con[ib]) {
+                if (ic == ia) continue;
+                add(ia, ic, true);
+                for (int id : con[ic]) {
+                    if (id == ia || id == ib) continue;
+                    ctx->err = "1-4 pair without dihedral parameters";
+                    return 110;
+                }
+            }
+    if (!sym && n > 0 && !ex[n - 1].empty()) return 111;
+    ctx->hExOff.assign(n + 1, 0);
+    ctx->hExLst.clear();
+    for (int i = 0; i < n; i++) {
+        ctx->hExOff[i] = (int)ctx->hExLst.size();
+        ctx->hExLst.insert(ctx->hExLst.end(), ex[i].begin(), ex[i].end());
+    }
+    ctx->hExOff[n] = (int)ctx->hExLst.size();
+    ctx->exclBuilt = true;
+    return mdg_upload_exclusions(ctx);
+}
+
+int mdg_upload_exclusions(mdg_ctx *ctx)
+{
+    const int ne = (int)ctx->hExLst.size();
+    if (ne > ctx->nExcl || !ctx->exLst) {
+        if (ctx->exLst) HIPCHK(hipFree(ctx->exLst));
+        RETCHK(dmalloc(ctx, &ctx->exLst, ne + 1));
+        ctx->nExcl = ne;
+    }
+    RETCHK(h2d(ctx, ctx->exOff, ctx->hExOff.data(), ctx->n + 1));
+    RETCHK(h2d(ctx, ctx->exLst, ctx->hExLst.data(), ne));
+    return sync(ctx);
+}
+
+// Exclusion table from the reference's (exclList, exclPair) arguments.
+static int exclusions_from_args(mdg_ctx *ctx, int **exclList, const int *exclPair)
+{
+    const int n = ctx->n;
+    ctx->hExOff.assign(n + 1, 0);
+    ctx->hExLst.clear();
+    for (int i = 0; i < n; i++) {
+        ctx->hExOff[i] = (int)ctx->hExLst.size();
+        for (int k = 0; k < exclPair[i]; k++) ctx->hExLst.push_back(exclList[i][k]);
+    }
+    ctx->hExOff[n] = (int)ctx->hExLst.size();
+    return mdg_upload_exclusions(ctx);
+}
+
+// ---------------------------------------------------------------- create
+
+extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, const MDG_PBC *box,
+                               const MDG_EWALD *ewald, const MDG_NEIGH *neigh, int precision,
+                               int *status)
+{
+    auto fail = [&](mdg_ctx *ctx, int code) -> mdg_ctx * {
+        if (status) *status = code;
+        if (ctx) {
+            fprintf(stderr, "mdg_create: %s\n", ctx->err.c_str());
+            mdg_destroy(ctx);
+        }
+        return nullptr;
+    };
+    if (!param || !box || param->nAtom <= 0) return fail(nullptr, MDG_EARG);
+    mdg_ctx *ctx = new mdg_ctx();
+    ctx->precision = precision == MDG_MIXED ? MDG_MIXED : MDG_FP64;
+    ctx->n = param->nAtom;
+    params_from_mirror(ctrl, param, ewald, neigh, ctx->P);
+    box_from_mirror(box, ctx->B);
+    if (ctx->B.type < 1 || ctx->B.type > 3) {
+        ctx->err = "a periodic box is required";
+        return fail(ctx, MDG_EUNSUPPORTED);
+    }
+    if (hipGetDevice(&ctx->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->err = "no HIP device / stream";
+        return fail(ctx, MDG_EHIP);
+    }
+    // makelist's first-call builder choice (list.c:38-57)
+    double cutnb = ctx->P.cutOff + ctx->P.delr;
+    int r = ctx->P.linkRatio;
+    ctx->nlc[0] = (int)(ctx->B.pa * (double)r / cutnb);
+    ctx->nlc[1] = (int)(ctx->B.pb * (double)r / cutnb);
+    ctx->nlc[2] = (int)(ctx->B.pc * (double)r / cutnb);
+    ctx->keyLink = 1;
+    if (ctx->nlc[0] < 3 || ctx->nlc[1] < 3 || ctx->nlc[2] < 3) ctx->keyLink = 0;
+    if (ctx->nlc[0] * ctx->nlc[1] * ctx->nlc[2] <= 27) ctx->keyLink = 0;
+    if (ctrl && ctrl->noLink) ctx->keyLink = 0;
+    if (ctx->keyLink && r != 1) {
+        ctx->err = "link cells are implemented for linkRatio 1 (the 14-cell half shell)";
+        return fail(ctx, MDG_EUNSUPPORTED);
+    }
+    ctx->ncell = ctx->nlc[0] * ctx->nlc[1] * ctx->nlc[2];
+    if (int e = mdg_alloc_state(ctx)) return fail(ctx, e);
+    ctx->m1 = ctx->P.m1;
+    ctx->m2 = ctx->P.m2;
+    ctx->m3 = ctx->P.m3;
+    if (ctx->m1 > 0 && ctx->m2 > 0 && ctx->m3 > 0)
+        if (int e = mdg_spme_setup(ctx)) return fail(ctx, e);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, MDG_EHIP);
+    if (status) *status = MDG_OK;
+    return ctx;
+}
+
+extern "C" void mdg_destroy(mdg_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (int k = 0; k < 2; k++)
+        if (ctx->graphExec[k]) hipGraphExecDestroy(ctx->graphExec[k]);
+    mdg_spme_free(ctx);
+    void *ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->vx, ctx->vy, ctx->vz, ctx->fx, ctx->fy, ctx->fz,
+                    ctx->q, ctx->eps, ctx->sig, ctx->mass, ctx->rmass, ctx->frozen, ctx->nAtConst,
+                    ctx->ca, ctx->cb, ctx->rc2, ctx->dd, ctx->grpAtomOff, ctx->grpAtoms,
+                    ctx->grpConOff, ctx->grpCons, ctx->grpCycle, ctx->grpVir, ctx->ctrl,
+                    ctx->exOff, ctx->exLst, ctx->xu, ctx->yu, ctx->zu, ctx->cellOf, ctx->cellCnt,
+                    ctx->cellStart, ctx->lcKeys, ctx->lcS3, ctx->rl.neighList, ctx->rl.neighPair,
+                    ctx->rl.neighOrder, ctx->rl.cnt, ctx->rl.off, ctx->cl.slotAtom,
+                    ctx->cl.atomSlot, ctx->cl.bb, ctx->cl.bbh, ctx->cl.cpOff, ctx->cl.cpJ,
+                    ctx->cl.cpMask, ctx->cl.cpI, ctx->cl.tmpCnt, ctx->cl.tmpJ, ctx->cl.tmpMask,
+                    ctx->cl.colCnt, ctx->cl.colStart, ctx->cl.colPad, ctx->cl.colSlot,
+                    ctx->cl.sortTmp, ctx->cl.sortKey, ctx->sx, ctx->sy, ctx->sz, ctx->sp32,
+                    ctx->sq, ctx->seps, ctx->ssig, ctx->fdx, ctx->fdy, ctx->fdz, ctx->scal,
+                    ctx->fix, ctx->fiy, ctx->fiz, ctx->frx, ctx->fry, ctx->frz, ctx->frx64,
+                    ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,
+                    ctx->grid64, ctx->cgrid64, ctx->grid32, ctx->cgrid32, ctx->gridq,
+                    ctx->fftWork, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
+                    ctx->shkMask};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (int k = 0; k < 8; k++)
+        if (ctx->scanBuf[k]) hipFree(ctx->scanBuf[k]);
+    for (hipEvent_t e : ctx->ev)
+        if (e) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *mdg_last_error(const mdg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+extern "C" int mdg_precision(const mdg_ctx *ctx) { return ctx ? ctx->precision : -1; }
+extern "C" int mdg_key_link(const mdg_ctx *ctx) { return ctx ? ctx->keyLink : -1; }
+
+extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps[],
+                                const double sig[], const double mass[], const double rmass[],
+                                const int frozen[], const int nAtConst[],
+                                const MDG_CONSTRAINT constList[], int nConst)
+{
+    if (!ctx || !q || !eps || !sig || !mass || !rmass) return MDG_EARG;
+    const int n = ctx->n;
+    RETCHK(h2d(ctx, ctx->q, q, n));
+    RETCHK(h2d(ctx, ctx->eps, eps, n));
+    RETCHK(h2d(ctx, ctx->sig, sig, n));
+    RETCHK(h2d(ctx, ctx->mass, mass, n));
+    RETCHK(h2d(ctx, ctx->rmass, rmass, n));
+    ctx->hasFrozen = false;
+    if (frozen) {
+        RETCHK(h2d(ctx, ctx->frozen, frozen, n));
+        for (int i = 0; i < n; i++) ctx->hasFrozen |= frozen[i] != 0;
+    }
+    if (nAtConst) RETCHK(h2d(ctx, ctx->nAtConst, nAtConst, n));
+    // spme.c:376-389 and 680, on every evaluation of the same charges
+    double systq = 0., q2 = 0.;
+    for (int i = 0; i < n; i++) {
+        systq += q[i];
+        q2 += q[i] * q[i];
+    }
+    ctx->systq = systq;
+    ctx->eSelf = -ctx->P.chargeConst * ctx->P.alpha * q2 / SQRTPI_C;
+    ctx->eNet = ctx->B.vol > 0 && ctx->P.alpha > 0
+        ? -(0.5 * PI_C * ctx->P.chargeConst) * ((systq / ctx->P.alpha) * (systq / ctx->P.alpha) / ctx->B.vol)
+        : 0.;
+    // constraints
+    if (ctx->ca) { hipFree(ctx->ca); hipFree(ctx->cb); hipFree(ctx->rc2); hipFree(ctx->dd); }
+    ctx->nConst = nConst;
+    std::vector<int> a(std::max(nConst, 1)), b(std::max(nConst, 1));
+    std::vector<double> r2(std::max(nConst, 1));
+    for (int c = 0; c < nConst; c++) {
+        a[c] = constList[c].a;
+        b[c] = constList[c].b;
+        r2[c] = constList[c].rc2;
+    }
+    RETCHK(dmalloc(ctx, &ctx->ca, nConst));
+    RETCHK(dmalloc(ctx, &ctx->cb, nConst));
+    RETCHK(dmalloc(ctx, &ctx->rc2, nConst));
+    RETCHK(dmalloc(ctx, &ctx->dd, 3 * (size_t)nConst));
+    RETCHK(h2d(ctx, ctx->ca, a.data(), nConst));
+    RETCHK(h2d(ctx, ctx->cb, b.data(), nConst));
+    RETCHK(h2d(ctx, ctx->rc2, r2.data(), nConst));
+    if (ctx->grpAtomOff) {
+        hipFree(ctx->grpAtomOff); hipFree(ctx->grpAtoms); hipFree(ctx->grpConOff);
+        hipFree(ctx->grpCons); hipFree(ctx->grpCycle); hipFree(ctx->grpVir);
+    }
+    RETCHK(mdg_build_groups(ctx, constList, nConst));
+    RETCHK(mdg_build_exclusions(ctx, 0, nullptr, nConst, constList, 0, nullptr));
+    return sync(ctx);
+}
+
+// ---------------------------------------------------------------- drop-in: makelist
+
+static int download_ref_list(mdg_ctx *ctx, MDG_NEIGH *neigh, int **neighList, int **neighPair,
+                             int **neighOrder, bool first)
+{
+    const int n = ctx->n;
+    RefList &R = ctx->rl;
+    if (first) {
+        *neighPair = (int *)malloc(n * sizeof(int));
+        *neighOrder = (int *)malloc(n * sizeof(int));
+        // list.c:2110 / 953 sizing
+        neigh->sizeList = (int)((double)R.count * (1. + 2. * TOLLIST_)) + 1;
+        *neighList = (int *)malloc((size_t)neigh->sizeList * sizeof(int));
+    } else if (R.count > neigh->sizeList) {
+        while (neigh->sizeList < R.count)
+            neigh->sizeList = (int)((double)neigh->sizeList * (1. + TOLLIST_)) + 1;
+        *neighList = (int *)realloc(*neighList, (size_t)neigh->sizeList * sizeof(int));
+    }
+    if (!*neighList || !*neighPair || !*neighOrder) return MDG_EARG;
+    RETCHK(d2h(ctx, *neighList, R.neighList, R.count));
+    RETCHK(d2h(ctx, *neighPair, R.neighPair, n));
+    RETCHK(d2h(ctx, *neighOrder, R.neighOrder, n));
+    return sync(ctx);
+}
+
+extern "C" int mdg_makelist(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_PBC *box,
+                            MDG_NEIGH *neigh, MDG_CONSTRAINT constList[], MDG_BOND bond[],
+                            MDG_ANGLE angle[], MDG_DIHE dihe[], MDG_DIHE impr[], double x[],
+                            double y[], double z[], int frozen[], int **neighList,
+                            int **neighPair, int **neighOrder, int **neighList14,
+                            int ***exclList, int **exclPair)
+{
+    (void)box; (void)dihe; (void)impr;
+    if (!ctx || !ctrl || !param || !neigh || !x || !y || !z) return MDG_EARG;
+    const int n = ctx->n;
+    if (param->nAtom != n) return MDG_EARG;
+    if (ctrl->newjob == 1) {
+        if (param->nDihedral > 0 || param->nImproper > 0) {
+            ctx->err = "dihedral and improper exclusions are not implemented";
+            return MDG_EUNSUPPORTED;
+        }
+        neigh->linkRatio = std::min(std::max(neigh->linkRatio, 1), 5);
+        ctrl->keyLink = ctx->keyLink;
+        if (frozen) {
+            RETCHK(h2d(ctx, ctx->frozen, (const int *)frozen, n));
+        }
+        RETCHK(mdg_build_exclusions(ctx, param->nBond, bond, ctrl->keyConstH ? param->nConst : 0,
+                                    constList, param->nAngle, angle));
+        // exclusion table in the reference layout (N-1 rows ordered, N rows symmetric)
+        const int rows = ctx->keyLink ? n : n - 1;
+        *exclPair = (int *)malloc(n * sizeof(int));
+        *exclList = (int **)malloc(std::max(rows, 1) * sizeof(int *));
+        for (int i = 0; i < n; i++) (*exclPair)[i] = ctx->hExOff[i + 1] - ctx->hExOff[i];
+        for (int i = 0; i < rows; i++) {
+            (*exclList)[i] = (int *)malloc(std::max((*exclPair)[i], 1) * sizeof(int));
+            for (int k = 0; k < (*exclPair)[i]; k++)
+                (*exclList)[i][k] = ctx->hExLst[ctx->hExOff[i] + k];
+        }
+        neigh->nPair14 = 0;
+        if (neighList14) *neighList14 = (int *)malloc(sizeof(int));
+        RETCHK(h2d(ctx, ctx->x, (const double *)x, n));
+        RETCHK(h2d(ctx, ctx->y, (const double *)y, n));
+        RETCHK(h2d(ctx, ctx->z, (const double *)z, n));
+        RETCHK(mdg_build_ref_list(ctx, true));
+        if (ctx->keyLink) {
+            // list.c:1927: the first link-cell build wraps the coordinates
+            RETCHK(d2h(ctx, x, ctx->x, n));
+            RETCHK(d2h(ctx, y, ctx->y, n));
+            RETCHK(d2h(ctx, z, ctx->z, n));
+            neigh->nCells = ctx->ncell;
+        }
+        RETCHK(download_ref_list(ctx, neigh, neighList, neighPair, neighOrder, true));
+        ctrl->newjob = 0;
+    } else if (param->step % neigh->update == 0) {
+        RETCHK(h2d(ctx, ctx->x, (const double *)x, n));
+        RETCHK(h2d(ctx, ctx->y, (const double *)y, n));
+        RETCHK(h2d(ctx, ctx->z, (const double *)z, n));
+        RETCHK(mdg_build_ref_list(ctx, false));
+        RETCHK(download_ref_list(ctx, neigh, neighList, neighPair, neighOrder, false));
+    }
+    int c[CTRL_WORDS];
+    RETCHK(d2h(ctx, c, ctx->ctrl, CTRL_WORDS));
+    RETCHK(sync(ctx));
+    if (c[CTRL_SEG_OVF]) {
+        ctx->err = "a link cell holds more atoms than one LDS sort segment";
+        return MDG_EUNSUPPORTED;
+    }
+    return MDG_OK;
+}
+
+// ---------------------------------------------------------------- drop-in: energies
+
+// Upload the list in the reference format with per-position offsets.
+static int upload_ref_list(mdg_ctx *ctx, const int *neighList, const int *neighPair,
+                           const int *neighOrder)
+{
+    const int n = ctx->n;
+    RefList &R = ctx->rl;
+    std::vector<long> off(n + 1, 0);
+    for (int l = 0; l < n; l++) off[l + 1] = off[l] + neighPair[neighOrder[l]];
+    const long total = off[n];
+    if (!R.off) RETCHK(dmalloc(ctx, &R.off, n + 1));
+    if (!R.neighOrder) RETCHK(dmalloc(ctx, &R.neighOrder, n));
+    if (!R.neighPair) RETCHK(dmalloc(ctx, &R.neighPair, n));
+    if (!R.cnt) RETCHK(dmalloc(ctx, &R.cnt, n));
+    if (total > R.cap) {
+        if (R.neighList) HIPCHK(hipFree(R.neighList));
+        R.cap = total + 1024;
+        RETCHK(dmalloc(ctx, &R.neighList, R.cap));
+    }
+    R.count = total;
+    RETCHK(h2d(ctx, R.off, off.data(), n + 1));
+    RETCHK(h2d(ctx, R.neighOrder, neighOrder, n));
+    RETCHK(h2d(ctx, R.neighList, neighList, total));
+    return sync(ctx);
+}
+
+static int upload_xyzq(mdg_ctx *ctx, const double *x, const double *y, const double *z,
+                       const double *q, const double *eps, const double *sig)
+{
+    const int n = ctx->n;
+    RETCHK(h2d(ctx, ctx->x, x, n));
+    RETCHK(h2d(ctx, ctx->y, y, n));
+    RETCHK(h2d(ctx, ctx->z, z, n));
+    if (q) RETCHK(h2d(ctx, ctx->q, q, n));
+    if (eps) RETCHK(h2d(ctx, ctx->eps, eps, n));
+    if (sig) RETCHK(h2d(ctx, ctx->sig, sig, n));
+    return MDG_OK;
+}
+
+static int refresh_charge_terms(mdg_ctx *ctx, const double *q)
+{
+    double systq = 0., q2 = 0.;
+    for (int i = 0; i < ctx->n; i++) {
+        systq += q[i];
+        q2 += q[i] * q[i];
+    }
+    ctx->systq = systq;
+    ctx->eSelf = -ctx->P.chargeConst * ctx->P.alpha * q2 / SQRTPI_C;
+    ctx->eNet = -(0.5 * PI_C * ctx->P.chargeConst) *
+                ((systq / ctx->P.alpha) * (systq / ctx->P.alpha) / ctx->B.vol);
+    return MDG_OK;
+}
+
+extern "C" int mdg_spme_energy(mdg_ctx *ctx, MDG_PARAM *param, MDG_EWALD *ewald, MDG_PBC *box,
+                               const double x[], const double y[], const double z[], double fx[],
+                               double fy[], double fz[], const double q[], double stress[6],
+                               double *virEwaldRec, double *eEwaldRec)
+{
+    (void)param; (void)ewald; (void)box;
+    if (!ctx || !x || !fx || !q || !ctx->theta) return MDG_EARG;
+    const int n = ctx->n;
+    RETCHK(upload_xyzq(ctx, x, y, z, q, nullptr, nullptr));
+    RETCHK(refresh_charge_terms(ctx, q));
+    RETCHK(h2d(ctx, ctx->fx, (const double *)fx, n));
+    RETCHK(h2d(ctx, ctx->fy, (const double *)fy, n));
+    RETCHK(h2d(ctx, ctx->fz, (const double *)fz, n));
+    double e[8];
+    RETCHK(mdg_spme64(ctx, ctx->fx, ctx->fy, ctx->fz, e));
+    RETCHK(d2h(ctx, fx, ctx->fx, n));
+    RETCHK(d2h(ctx, fy, ctx->fy, n));
+    RETCHK(d2h(ctx, fz, ctx->fz, n));
+    RETCHK(sync(ctx));
+    if (eEwaldRec) *eEwaldRec = e[0];
+    if (virEwaldRec) *virEwaldRec = e[1];
+    if (stress)
+        for (int k = 0; k < 6; k++) stress[k] = e[2 + k];
+    return MDG_OK;
+}
+
+extern "C" int mdg_ewald_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+                                MDG_EWALD *ewald, MDG_PBC *box, const double x[], const double y[],
+                                const double z[], double fx[], double fy[], double fz[],
+                                const double q[], const double eps[], const double sig[],
+                                const int neighList[], const int neighPair[],
+                                const int neighOrder[], int **exclList, const int exclPair[])
+{
+    if (!ctx || !ctrl || !ener || !x || !fx || !q || !neighPair || !neighOrder) return MDG_EARG;
+    const int n = ctx->n;
+    if (ctrl->keyEwald == 1) {
+        ctx->err = "classic k-space Ewald (keyEwald 1) is not on this path";
+        return MDG_EUNSUPPORTED;
+    }
+    ctx->P.vdwOn = ctrl->vdwType == 2;
+    RETCHK(upload_xyzq(ctx, x, y, z, q, eps, sig));
+    RETCHK(upload_ref_list(ctx, neighList, neighPair, neighOrder));
+    RETCHK(exclusions_from_args(ctx, exclList, exclPair));
+    RETCHK(h2d(ctx, ctx->fx, (const double *)fx, n));
+    RETCHK(h2d(ctx, ctx->fy, (const double *)fy, n));
+    RETCHK(h2d(ctx, ctx->fz, (const double *)fz, n));
+    double erec[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (ctrl->keyEwald == 2) {
+        if (!ctx->theta) return MDG_EARG;
+        RETCHK(refresh_charge_terms(ctx, q));
+        RETCHK(mdg_spme64(ctx, ctx->fx, ctx->fy, ctx->fz, erec));
+    }
+    RETCHK(mdg_pair_ref64(ctx, 0, ctx->fx, ctx->fy, ctx->fz, ctx->rl.neighList, ctx->rl.off,
+                          ctx->rl.neighOrder, ctx->scal + S_REF));
+    RETCHK(mdg_excl_corr64(ctx, ctx->fx, ctx->fy, ctx->fz, ctx->scal + S_REFCORR));
+    double s[8];
+    RETCHK(d2h(ctx, s, ctx->scal + S_REF, 8));
+    RETCHK(d2h(ctx, fx, ctx->fx, n));
+    RETCHK(d2h(ctx, fy, ctx->fy, n));
+    RETCHK(d2h(ctx, fz, ctx->fz, n));
+    RETCHK(sync(ctx));
+    // energy.c:623-627
+    ener->elec += s[0] + erec[0] + s[6];
+    ener->vdw += s[1];
+    ener->virelec += s[2] + erec[1] + s[7];
+    ener->virvdw += s[3];
+    return MDG_OK;
+}
+
+extern "C" int mdg_nonbond_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param,
+                                  MDG_ENERGY *ener, MDG_PBC *box, const double x[],
+                                  const double y[], const double z[], double fx[], double fy[],
+                                  double fz[], const double q[], const double eps[],
+                                  const double sig[], const int neighList[],
+                                  const int neighPair[], const int neighOrder[])
+{
+    (void)param; (void)box;
+    if (!ctx || !ctrl || !ener || !x || !fx || !q || !neighPair || !neighOrder) return MDG_EARG;
+    const int n = ctx->n;
+    // energy.c:45-65: FULL and NOELEC evaluate no electrostatics
+    int et = ctrl->elecType;
+    if (et == 0) et = 1;
+    if (et < 1 || et > 4) return MDG_EUNSUPPORTED;
+    ctx->P.vdwOn = ctrl->vdwType == 2;
+    RETCHK(upload_xyzq(ctx, x, y, z, q, eps, sig));
+    RETCHK(upload_ref_list(ctx, neighList, neighPair, neighOrder));
+    RETCHK(h2d(ctx, ctx->fx, (const double *)fx, n));
+    RETCHK(h2d(ctx, ctx->fy, (const double *)fy, n));
+    RETCHK(h2d(ctx, ctx->fz, (const double *)fz, n));
+    RETCHK(mdg_pair_ref64(ctx, et, ctx->fx, ctx->fy, ctx->fz, ctx->rl.neighList, ctx->rl.off,
+                          ctx->rl.neighOrder, ctx->scal + S_REF));
+    double s[6];
+    RETCHK(d2h(ctx, s, ctx->scal + S_REF, 6));
+    RETCHK(d2h(ctx, fx, ctx->fx, n));
+    RETCHK(d2h(ctx, fy, ctx->fy, n));
+    RETCHK(d2h(ctx, fz, ctx->fz, n));
+    RETCHK(sync(ctx));
+    ener->elec += s[0];
+    ener->vdw += s[1];
+    ener->virelec += s[2];
+    ener->virvdw += s[3];
+    return MDG_OK;
+}
+
+extern "C" int mdg_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+                          MDG_EWALD *ewald, MDG_PBC *box, MDG_NEIGH *neigh, const double x[],
+                          const double y[], const double z[], double vx[], double vy[],
+                          double vz[], double fx[], double fy[], double fz[], const double q[],
+                          const double eps[], const double sig[], const int frozen[],
+                          const int neighList[], const int neighPair[], const int neighOrder[],
+                          int **exclList, const int exclPair[])
+{
+    (void)neigh;
+    if (!ctx || !ctrl || !param || !ener || !box) return MDG_EARG;
+    if (param->nBond > 0 || param->nAngle > 0 || param->nUb > 0 || param->nDihedral > 0 ||
+        param->nImproper > 0 || ctrl->keyNb14) {
+        ctx->err = "bonded and 1-4 terms are not on this path";
+        return MDG_EUNSUPPORTED;
+    }
+    // energy.c:115-144
+    ener->elec = ener->vdw = ener->bond = ener->ub = ener->ang = ener->dihe = ener->impr = 0.;
+    ener->virelec = ener->virvdw = ener->virbond = ener->virub = ener->virpot = 0.;
+    box->stress1 = box->stress2 = box->stress3 = box->stress4 = box->stress5 = 0.;
+    box->stress6 = box->stress7 = box->stress8 = box->stress9 = 0.;
+    for (int i = 0; i < ctx->n; i++) fx[i] = fy[i] = fz[i] = 0.;
+    int st;
+    if (ctrl->keyEwald != 0)
+        st = mdg_ewald_energy(ctx, ctrl, param, ener, ewald, box, x, y, z, fx, fy, fz, q, eps, sig,
+                              neighList, neighPair, neighOrder, exclList, exclPair);
+    else
+        st = mdg_nonbond_energy(ctx, ctrl, param, ener, box, x, y, z, fx, fy, fz, q, eps, sig,
+                                neighList, neighPair, neighOrder);
+    if (st) return st;
+    ener->pot = ener->elec + ener->vdw + ener->bond + ener->ang + ener->ub + ener->dihe + ener->impr;
+    ener->virpot = ener->virbond + ener->virub + ener->virelec + ener->virvdw;
+    if (param->nFrozen > 0 && frozen)   // utils.c:131-153
+        for (int i = 0; i < ctx->n; i++)
+            if (frozen[i]) vx[i] = vy[i] = vz[i] = fx[i] = fy[i] = fz[i] = 0.;
+    return MDG_OK;
+}
+
+// ---------------------------------------------------------------- drop-in: integrator
+
+extern "C" int mdg_vv_integrate(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+                                MDG_PBC *box, MDG_BATH *bath, MDG_CONSTRAINT constList[],
+                                double *x, double *y, double *z, double *vx, double *vy,
+                                double *vz, double *fx, double *fy, double *fz, double *mass,
+                                double *rmass, int *nAtConst, int stage)
+{
+    (void)bath;
+    if (!ctx || !param || !ener || !x || !vx || !fx || !mass || !rmass) return MDG_EARG;
+    if (ctrl && ctrl->ens != 0) {
+        ctx->err = "only the NVE ensemble is on this path";
+        return MDG_EUNSUPPORTED;
+    }
+    const int n = ctx->n;
+    ctx->P.timeStep = param->timeStep;
+    ctx->P.rTimeStep = 1. / param->timeStep;
+    ctx->P.tolShake = param->tolShake;
+    ctx->P.maxCycle = param->maxCycle;
+    if (!ctx->grpAtomOff) {
+        if (nAtConst) RETCHK(h2d(ctx, ctx->nAtConst, (const int *)nAtConst, n));
+        RETCHK(mdg_set_topology(ctx, std::vector<double>(n, 0.).data(),
+                                std::vector<double>(n, 0.).data(),
+                                std::vector<double>(n, 0.).data(), mass, rmass, nullptr, nAtConst,
+                                constList, param->nConst));
+    }
+    RETCHK(h2d(ctx, ctx->mass, (const double *)mass, n));
+    RETCHK(h2d(ctx, ctx->rmass, (const double *)rmass, n));
+    if (nAtConst) RETCHK(h2d(ctx, ctx->nAtConst, (const int *)nAtConst, n));
+    RETCHK(upload_xyzq(ctx, x, y, z, nullptr, nullptr, nullptr));
+    RETCHK(h2d(ctx, ctx->vx, (const double *)vx, n));
+    RETCHK(h2d(ctx, ctx->vy, (const double *)vy, n));
+    RETCHK(h2d(ctx, ctx->vz, (const double *)vz, n));
+    RETCHK(h2d(ctx, ctx->fx, (const double *)fx, n));
+    RETCHK(h2d(ctx, ctx->fy, (const double *)fy, n));
+    RETCHK(h2d(ctx, ctx->fz, (const double *)fz, n));
+    double kin[8] = {0}, vs = 0.;
+    int st = mdg_vv_stage(ctx, stage, ctx->fx, ctx->fy, ctx->fz, kin, &vs, true);
+    if (st) return st;
+    RETCHK(d2h(ctx, x, ctx->x, n));
+    RETCHK(d2h(ctx, y, ctx->y, n));
+    RETCHK(d2h(ctx, z, ctx->z, n));
+    RETCHK(d2h(ctx, vx, ctx->vx, n));
+    RETCHK(d2h(ctx, vy, ctx->vy, n));
+    RETCHK(d2h(ctx, vz, ctx->vz, n));
+    RETCHK(sync(ctx));
+    if (stage == 1) {
+        if (param->nConst > 0) ener->virshake = vs;
+    } else {
+        ener->kin = kin[0];
+        if (box) {   // integrate.c:1603-1611 (the SHAKE stress is zero in stage 2)
+            box->stress1 += kin[1];
+            box->stress2 += kin[2];
+            box->stress3 += kin[3];
+            box->stress4 += kin[2];
+            box->stress5 += kin[4];
+            box->stress6 += kin[5];
+            box->stress7 += kin[3];
+            box->stress8 += kin[5];
+            box->stress9 += kin[6];
+        }
+    }
+    return MDG_OK;
+}

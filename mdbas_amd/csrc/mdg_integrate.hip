@@ -1,0 +1,676 @@
+// mdg_integrate.hip -- velocity Verlet NVE with SHAKE on the GPU
+// (integrate.c:1518-1634, shake.c:209-457).
+//
+// The reference SHAKE is a Jacobi sweep over all constraints that stops
+// at the first sweep index C at which the largest residual of the whole
+// system is below tol.  Constraint groups (connected components of the
+// constraint graph; a water molecule is one group) never share atoms, so a
+// group's state after k sweeps does not depend on other groups; only C is
+// global.  A group's residual can dip below tol and rise above it again
+// (the averaged Jacobi iteration oscillates with period two), so C is not
+// simply the largest per-group first-convergence index.  Kernel A runs
+// each group (one thread) until it has been converged at two consecutive
+// checks, records the converged / not-converged bit of every sweep index
+// it checked, ANDs those bits into a device-wide mask and keeps its states
+// for the first SHK_W sweeps from its first convergence on.  C is the first
+// set bit of the mask; kernel B gives every group its state after exactly
+// C sweeps (from the kept window, or by sweeping on), as the reference's
+// global loop would, with no host round trip.  A group settled at two
+// consecutive checks is taken as converged from then on; an oscillation
+// wider than that would be a divergence from the reference in its SHAKE
+// stopping sweep only.  Arithmetic follows the reference's operation order
+// with contraction off.
+#include "mdg_internal.h"
+
+struct GrpArgs {
+    MdgParams P;
+    MdgBox B;
+    int nGrp;
+    const int *atomOff, *atoms, *conOff, *cons;
+    const int *ca, *cb;
+    const double *rc2, *rmass, *mass;
+    const int *nAtConst;
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double *fx, *fy, *fz;
+    double *dd;           // [3][nConst]
+    int nConst;
+    int *cycle;
+    double *vir;
+    int *ctrl;
+    double *part;         // stage 2: kin, stresk[6]
+    double *scr;          // [SHK_SLOTS][6][n] saved states
+    double *virScr;       // [SHK_SLOTS][nGrp] SHAKE virial at those states
+    int *first, *kend;    // per group: first converged sweep, last checked sweep
+    unsigned long long *mask;   // device-wide AND of converged-sweep bits
+};
+
+__device__ inline int local_of(const int *atoms, int na, int g)
+{
+    for (int k = 0; k < na; k++)
+        if (atoms[k] == g) return k;
+    return 0;
+}
+
+struct Grp {
+    int na, nc;
+    int at[MDG_GRP_ATOMS];
+    int ia[MDG_GRP_CONS], ib[MDG_GRP_CONS], cid[MDG_GRP_CONS];
+    double x[MDG_GRP_ATOMS], y[MDG_GRP_ATOMS], z[MDG_GRP_ATOMS];
+    double vx[MDG_GRP_ATOMS], vy[MDG_GRP_ATOMS], vz[MDG_GRP_ATOMS];
+    double rm[MDG_GRP_ATOMS], nat[MDG_GRP_ATOMS];
+    double ddx[MDG_GRP_CONS], ddy[MDG_GRP_CONS], ddz[MDG_GRP_CONS], rc2[MDG_GRP_CONS];
+};
+
+__device__ inline void grp_load(const GrpArgs &A, int g, Grp &G)
+{
+    const int a0 = A.atomOff[g], c0 = A.conOff[g];
+    G.na = A.atomOff[g + 1] - a0;
+    G.nc = A.conOff[g + 1] - c0;
+    for (int k = 0; k < G.na; k++) {
+        int i = A.atoms[a0 + k];
+        G.at[k] = i;
+        G.x[k] = A.x[i];
+        G.y[k] = A.y[i];
+        G.z[k] = A.z[i];
+        G.vx[k] = A.vx[i];
+        G.vy[k] = A.vy[i];
+        G.vz[k] = A.vz[i];
+        G.rm[k] = A.rmass[i];
+        int nc = A.nAtConst[i];
+        G.nat[k] = nc > 0 ? 1.0 / (double)nc : 0.0;
+    }
+    for (int c = 0; c < G.nc; c++) {
+        int ci = A.cons[c0 + c];
+        G.cid[c] = ci;
+        G.ia[c] = local_of(G.at, G.na, A.ca[ci]);
+        G.ib[c] = local_of(G.at, G.na, A.cb[ci]);
+        G.rc2[c] = A.rc2[ci];
+    }
+}
+
+__device__ inline void grp_store(const GrpArgs &A, const Grp &G, bool pos)
+{
+    for (int k = 0; k < G.na; k++) {
+        int i = G.at[k];
+        if (pos) {
+            A.x[i] = G.x[k];
+            A.y[i] = G.y[k];
+            A.z[i] = G.z[k];
+        }
+        A.vx[i] = G.vx[k];
+        A.vy[i] = G.vy[k];
+        A.vz[i] = G.vz[k];
+    }
+}
+
+// shake.c:243-258: residual check over the group; fills dt / rt2.
+__device__ inline double shake_r_check(const GrpArgs &A, const Grp &G, double *dtx, double *dty,
+                                       double *dtz, double *rt2)
+{
+#pragma clang fp contract(off)
+    double maxd = 0.;
+    for (int c = 0; c < G.nc; c++) {
+        dtx[c] = G.x[G.ib[c]] - G.x[G.ia[c]];
+        dty[c] = G.y[G.ib[c]] - G.y[G.ia[c]];
+        dtz[c] = G.z[G.ib[c]] - G.z[G.ia[c]];
+        rt2[c] = mdg_dist(A.B, dtx[c], dty[c], dtz[c]);
+        double d = fabs(rt2[c] - G.rc2[c]) / sqrt(G.rc2[c]);
+        maxd = maxd >= d ? maxd : d;
+    }
+    return 0.5 * maxd;
+}
+
+// shake.c:262-332: one correction sweep over the group.
+__device__ inline void shake_r_apply(const GrpArgs &A, Grp &G, const double *dtx,
+                                     const double *dty, const double *dtz, const double *rt2,
+                                     double &vir)
+{
+#pragma clang fp contract(off)
+    const double dt = A.P.timeStep;
+    double xt[MDG_GRP_ATOMS], yt[MDG_GRP_ATOMS], zt[MDG_GRP_ATOMS];
+    for (int k = 0; k < G.na; k++) xt[k] = yt[k] = zt[k] = 0.;
+    for (int c = 0; c < G.nc; c++) {
+        const int a = G.ia[c], b = G.ib[c];
+        double trmi = dt * G.rm[a], trmj = dt * G.rm[b];
+        double lambda = -(G.rc2[c] - rt2[c]) /
+            (dt * (trmi + trmj) * ((G.ddx[c] * dtx[c]) + (G.ddy[c] * dty[c]) + (G.ddz[c] * dtz[c])));
+        vir += lambda * (G.ddx[c] * G.ddx[c] + G.ddy[c] * G.ddy[c] + G.ddz[c] * G.ddz[c]);
+        double li = 0.5 * lambda * trmi;
+        xt[a] += G.ddx[c] * li;
+        yt[a] += G.ddy[c] * li;
+        zt[a] += G.ddz[c] * li;
+        double lj = -0.5 * lambda * trmj;
+        xt[b] += G.ddx[c] * lj;
+        yt[b] += G.ddy[c] * lj;
+        zt[b] += G.ddz[c] * lj;
+    }
+    for (int c = 0; c < G.nc; c++) {
+        const int a = G.ia[c], b = G.ib[c];
+        double nia = G.nat[a], nib = G.nat[b];
+        G.x[a] += dt * xt[a] * nia;
+        G.y[a] += dt * yt[a] * nia;
+        G.z[a] += dt * zt[a] * nia;
+        G.x[b] += dt * xt[b] * nib;
+        G.y[b] += dt * yt[b] * nib;
+        G.z[b] += dt * zt[b] * nib;
+        G.vx[a] += xt[a] * nia;
+        G.vy[a] += yt[a] * nia;
+        G.vz[a] += zt[a] * nia;
+        G.vx[b] += xt[b] * nib;
+        G.vy[b] += yt[b] * nib;
+        G.vz[b] += zt[b] * nib;
+    }
+}
+
+// shake.c:378-430: velocity residual (max |lambda|) and the corrections.
+__device__ inline double shake_v_sweep(const GrpArgs &A, Grp &G, bool apply_if_needed,
+                                       double tolvel, bool force_apply)
+{
+#pragma clang fp contract(off)
+    const double dt = A.P.timeStep;
+    double xt[MDG_GRP_ATOMS], yt[MDG_GRP_ATOMS], zt[MDG_GRP_ATOMS];
+    for (int k = 0; k < G.na; k++) xt[k] = yt[k] = zt[k] = 0.;
+    double maxd = 0.;
+    for (int c = 0; c < G.nc; c++) {
+        const int a = G.ia[c], b = G.ib[c];
+        double trmi = 0.5 * dt * G.rm[a], trmj = 0.5 * dt * G.rm[b];
+        double lambda = (G.ddx[c] * (G.vx[b] - G.vx[a]) + G.ddy[c] * (G.vy[b] - G.vy[a]) +
+                         G.ddz[c] * (G.vz[b] - G.vz[a])) /
+                        ((trmi + trmj) *
+                         (G.ddx[c] * G.ddx[c] + G.ddy[c] * G.ddy[c] + G.ddz[c] * G.ddz[c]));
+        double al = fabs(lambda);
+        maxd = maxd >= al ? maxd : al;
+        double li = lambda * trmi;
+        xt[a] += G.ddx[c] * li;
+        yt[a] += G.ddy[c] * li;
+        zt[a] += G.ddz[c] * li;
+        double lj = -lambda * trmj;
+        xt[b] += G.ddx[c] * lj;
+        yt[b] += G.ddy[c] * lj;
+        zt[b] += G.ddz[c] * lj;
+    }
+    bool apply = force_apply || (apply_if_needed && !(maxd < tolvel));
+    if (apply) {
+        for (int c = 0; c < G.nc; c++) {
+            const int a = G.ia[c], b = G.ib[c];
+            double nia = G.nat[a], nib = G.nat[b];
+            G.vx[a] += xt[a] * nia;
+            G.vy[a] += yt[a] * nia;
+            G.vz[a] += zt[a] * nia;
+            G.vx[b] += xt[b] * nib;
+            G.vy[b] += yt[b] * nib;
+            G.vz[b] += zt[b] * nib;
+        }
+    }
+    return maxd;
+}
+
+// Window of SHAKE states kept per group around its first convergence.
+#define SHK_W 8
+#define SHK_SLOTS (SHK_W + 2)     // [0] start, [1..W] window, [W+1] end
+#define SHK_MASKW 4               // 256 sweep indices
+
+__device__ inline void scr_store(const GrpArgs &A, const Grp &G, int slot, bool pos)
+{
+    const long n = A.P.n;
+    double *s = A.scr + (long)slot * 6 * n;
+    for (int k = 0; k < G.na; k++) {
+        const int i = G.at[k];
+        if (pos) {
+            s[i] = G.x[k];
+            s[n + i] = G.y[k];
+            s[2 * n + i] = G.z[k];
+        }
+        s[3 * n + i] = G.vx[k];
+        s[4 * n + i] = G.vy[k];
+        s[5 * n + i] = G.vz[k];
+    }
+}
+
+__device__ inline void scr_load(const GrpArgs &A, Grp &G, int slot, bool pos)
+{
+    const long n = A.P.n;
+    const double *s = A.scr + (long)slot * 6 * n;
+    for (int k = 0; k < G.na; k++) {
+        const int i = G.at[k];
+        if (pos) {
+            G.x[k] = s[i];
+            G.y[k] = s[n + i];
+            G.z[k] = s[2 * n + i];
+        }
+        G.vx[k] = s[3 * n + i];
+        G.vy[k] = s[4 * n + i];
+        G.vz[k] = s[5 * n + i];
+    }
+}
+
+__device__ inline void dd_load(const GrpArgs &A, Grp &G)
+{
+    for (int c = 0; c < G.nc; c++) {
+        G.ddx[c] = A.dd[G.cid[c]];
+        G.ddy[c] = A.dd[A.nConst + G.cid[c]];
+        G.ddz[c] = A.dd[2 * A.nConst + G.cid[c]];
+    }
+}
+
+__device__ inline void dd_compute(const GrpArgs &A, Grp &G)
+{
+    for (int c = 0; c < G.nc; c++) {
+        double dx = G.x[G.ib[c]] - G.x[G.ia[c]];
+        double dy = G.y[G.ib[c]] - G.y[G.ia[c]];
+        double dz = G.z[G.ib[c]] - G.z[G.ia[c]];
+        mdg_dist(A.B, dx, dy, dz);
+        G.ddx[c] = dx;
+        G.ddy[c] = dy;
+        G.ddz[c] = dz;
+        A.dd[G.cid[c]] = dx;
+        A.dd[A.nConst + G.cid[c]] = dy;
+        A.dd[2 * A.nConst + G.cid[c]] = dz;
+    }
+}
+
+// Publishes the group's converged-at-sweep-k bits into the device-wide
+// AND mask: bits for k <= kend are as checked; later sweeps count as
+// converged when the group was converged at its last two checks.
+__device__ inline void publish_mask(const GrpArgs &A, const unsigned long long (&bits)[SHK_MASKW],
+                                    int kend, bool settled)
+{
+    for (int w = 0; w < SHK_MASKW; w++) {
+        unsigned long long m = bits[w];
+        if (settled)
+            for (int b = 0; b < 64; b++)
+                if (w * 64 + b > kend && w * 64 + b < A.P.maxCycle) m |= 1ull << b;
+        atomicAnd(&A.mask[w], m);
+    }
+}
+
+// First sweep index at which every group was converged (-1 if none).
+__device__ inline int global_stop(const GrpArgs &A)
+{
+    for (int w = 0; w < SHK_MASKW; w++) {
+        unsigned long long m = A.mask[w];
+        if (m) {
+            int k = w * 64 + __ffsll((long long)m) - 1;
+            return k < A.P.maxCycle ? k : -1;
+        }
+    }
+    return -1;
+}
+
+// Stage 1, part A: constraint vectors, half kick, drift, then SHAKE-R
+// sweeps until the group has been converged at two consecutive checks
+// (integrate.c:1526-1590, shake.c:243-338), recording which sweep indices
+// were converged and keeping the states of the first SHK_W sweeps from
+// the first convergence on.
+__global__ void __launch_bounds__(128) k_vv1_a(GrpArgs A)
+{
+#pragma clang fp contract(off)
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= A.nGrp) return;
+    Grp G;
+    grp_load(A, g, G);
+    const double dt = A.P.timeStep;
+    dd_compute(A, G);
+    for (int k = 0; k < G.na; k++) {
+        int i = G.at[k];
+        G.vx[k] += 0.5 * dt * A.fx[i] * G.rm[k];
+        G.vy[k] += 0.5 * dt * A.fy[i] * G.rm[k];
+        G.vz[k] += 0.5 * dt * A.fz[i] * G.rm[k];
+    }
+    for (int k = 0; k < G.na; k++) {
+        G.x[k] += dt * G.vx[k];
+        G.y[k] += dt * G.vy[k];
+        G.z[k] += dt * G.vz[k];
+    }
+    if (G.nc == 0) {
+        grp_store(A, G, true);
+        return;
+    }
+    scr_store(A, G, 0, true);
+    unsigned long long bits[SHK_MASKW] = {0, 0, 0, 0};
+    double dtx[MDG_GRP_CONS], dty[MDG_GRP_CONS], dtz[MDG_GRP_CONS], rt2[MDG_GRP_CONS];
+    double vir = 0.;
+    int first = -1, k = 0;
+    bool prev = false, settled = false;
+    for (;;) {
+        const bool cv = shake_r_check(A, G, dtx, dty, dtz, rt2) < A.P.tolShake;
+        if (cv) bits[k >> 6] |= 1ull << (k & 63);
+        if (cv && first < 0) first = k;
+        if (first >= 0 && k - first < SHK_W) {
+            scr_store(A, G, 1 + k - first, true);
+            A.virScr[(long)(1 + k - first) * A.nGrp + g] = vir;
+        }
+        if (cv && prev) {
+            settled = true;
+            break;
+        }
+        prev = cv;
+        if (k + 1 >= A.P.maxCycle) break;
+        shake_r_apply(A, G, dtx, dty, dtz, rt2, vir);
+        k++;
+    }
+    scr_store(A, G, SHK_W + 1, true);
+    A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vir;
+    A.first[g] = first;
+    A.kend[g] = k;
+    publish_mask(A, bits, k, settled);
+}
+
+// Stage 1, part B: every group takes its state at the global stop sweep C
+// (from the window, or by sweeping on from the last or the first state).
+__global__ void __launch_bounds__(128) k_vv1_b(GrpArgs A)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= A.nGrp) return;
+    if (A.conOff[g + 1] == A.conOff[g]) {
+        A.vir[g] = 0.;
+        return;
+    }
+    const int C = global_stop(A);
+    if (C < 0) {
+        atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 1);
+        return;
+    }
+    Grp G;
+    grp_load(A, g, G);
+    dd_load(A, G);
+    const int first = A.first[g], kend = A.kend[g];
+    double vir;
+    int at, slot;
+    if (first >= 0 && C >= first && C - first < SHK_W && C <= kend) {
+        slot = 1 + C - first;
+        at = C;
+    } else if (C >= kend) {
+        slot = SHK_W + 1;
+        at = kend;
+    } else {
+        slot = 0;
+        at = 0;
+    }
+    scr_load(A, G, slot, true);
+    vir = slot == 0 ? 0. : A.virScr[(long)slot * A.nGrp + g];
+    double dtx[MDG_GRP_CONS], dty[MDG_GRP_CONS], dtz[MDG_GRP_CONS], rt2[MDG_GRP_CONS];
+    for (int k = at; k < C; k++) {
+        shake_r_check(A, G, dtx, dty, dtz, rt2);
+        shake_r_apply(A, G, dtx, dty, dtz, rt2, vir);
+    }
+    A.vir[g] = vir;
+    grp_store(A, G, true);
+}
+
+// Stage 2, part A: constraint vectors, half kick, SHAKE-V sweeps with the
+// same bookkeeping (integrate.c:1526-1561, shake.c:359-457).
+__global__ void __launch_bounds__(128) k_vv2_a(GrpArgs A)
+{
+#pragma clang fp contract(off)
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= A.nGrp) return;
+    Grp G;
+    grp_load(A, g, G);
+    const double dt = A.P.timeStep;
+    dd_compute(A, G);
+    for (int k = 0; k < G.na; k++) {
+        int i = G.at[k];
+        G.vx[k] += 0.5 * dt * A.fx[i] * G.rm[k];
+        G.vy[k] += 0.5 * dt * A.fy[i] * G.rm[k];
+        G.vz[k] += 0.5 * dt * A.fz[i] * G.rm[k];
+    }
+    if (G.nc == 0) {
+        grp_store(A, G, false);
+        return;
+    }
+    scr_store(A, G, 0, false);
+    const double tolvel = A.P.tolShake * A.P.rTimeStep;
+    unsigned long long bits[SHK_MASKW] = {0, 0, 0, 0};
+    int first = -1, k = 0;
+    bool prev = false, settled = false;
+    for (;;) {
+        Grp T = G;
+        const bool cv = shake_v_sweep(A, T, false, tolvel, true) < tolvel;   // T = G after a sweep
+        if (cv) bits[k >> 6] |= 1ull << (k & 63);
+        if (cv && first < 0) first = k;
+        if (first >= 0 && k - first < SHK_W) scr_store(A, G, 1 + k - first, false);
+        if (cv && prev) {
+            settled = true;
+            break;
+        }
+        prev = cv;
+        if (k + 1 >= A.P.maxCycle) break;
+        G = T;
+        k++;
+    }
+    scr_store(A, G, SHK_W + 1, false);
+    A.first[g] = first;
+    A.kend[g] = k;
+    publish_mask(A, bits, k, settled);
+}
+
+// Stage 2, part B: state at the global stop sweep, kinetic energy and
+// kinetic stress partials (utils.c:502-542), wrap (integrate.c:1620-1627).
+__global__ void __launch_bounds__(128) k_vv2_b(GrpArgs A)
+{
+#pragma clang fp contract(off)
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (g < A.nGrp) {
+        Grp G;
+        grp_load(A, g, G);
+        if (A.conOff[g + 1] != A.conOff[g]) {
+            const int C = global_stop(A);
+            if (C < 0) {
+                atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 2);
+            } else {
+                dd_load(A, G);
+                const int first = A.first[g], kend = A.kend[g];
+                int at, slot;
+                if (first >= 0 && C >= first && C - first < SHK_W && C <= kend) {
+                    slot = 1 + C - first;
+                    at = C;
+                } else if (C >= kend) {
+                    slot = SHK_W + 1;
+                    at = kend;
+                } else {
+                    slot = 0;
+                    at = 0;
+                }
+                scr_load(A, G, slot, false);
+                const double tolvel = A.P.tolShake * A.P.rTimeStep;
+                for (int k = at; k < C; k++) shake_v_sweep(A, G, false, tolvel, true);
+            }
+        }
+        for (int k = 0; k < G.na; k++) {
+            double m = A.mass[G.at[k]];
+            acc[0] += m * (G.vx[k] * G.vx[k] + G.vy[k] * G.vy[k] + G.vz[k] * G.vz[k]);
+            acc[1] += m * G.vx[k] * G.vx[k];
+            acc[2] += m * G.vx[k] * G.vy[k];
+            acc[3] += m * G.vx[k] * G.vz[k];
+            acc[4] += m * G.vy[k] * G.vy[k];
+            acc[5] += m * G.vy[k] * G.vz[k];
+            acc[6] += m * G.vz[k] * G.vz[k];
+            // image_update, utils.c:162-218
+            double xs = G.x[k], ys = G.y[k], zs = G.z[k];
+            if (A.B.type == 1) {
+                xs = xs - A.B.a1 * mdg_nint(xs * A.B.u1);
+                ys = ys - A.B.a1 * mdg_nint(ys * A.B.u1);
+                zs = zs - A.B.a1 * mdg_nint(zs * A.B.u1);
+            } else if (A.B.type == 2) {
+                xs = xs - A.B.a1 * mdg_nint(xs * A.B.u1);
+                ys = ys - A.B.b2 * mdg_nint(ys * A.B.v2);
+                zs = zs - A.B.c3 * mdg_nint(zs * A.B.w3);
+            } else if (A.B.type == 3) {
+                double xt = xs * A.B.u1 + ys * A.B.u2 + zs * A.B.u3;
+                double yt = xs * A.B.v1 + ys * A.B.v2 + zs * A.B.v3;
+                double zt = xs * A.B.w1 + ys * A.B.w2 + zs * A.B.w3;
+                xt -= mdg_nint(xt);
+                yt -= mdg_nint(yt);
+                zt -= mdg_nint(zt);
+                xs = xt * A.B.a1 + yt * A.B.b1 + zt * A.B.c1;
+                ys = xt * A.B.a2 + yt * A.B.b2 + zt * A.B.c2;
+                zs = xt * A.B.a3 + yt * A.B.b3 + zt * A.B.c3;
+            }
+            G.x[k] = xs;
+            G.y[k] = ys;
+            G.z[k] = zs;
+        }
+        grp_store(A, G, true);
+    }
+    mdg_block_partials<7>(acc, A.part);
+}
+
+__global__ void k_ctrl_reset(int *ctrl, unsigned long long *mask)
+{
+    if (threadIdx.x == 0) ctrl[CTRL_SHAKE_MAX] = 0;
+    if (threadIdx.x < SHK_MASKW) mask[threadIdx.x] = ~0ull;
+}
+
+// Connected components of the constraint graph (host, once).
+int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
+{
+    const int n = ctx->n;
+    std::vector<int> parent(n);
+    for (int i = 0; i < n; i++) parent[i] = i;
+    auto find = [&](int a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    for (int c = 0; c < nConst; c++) {
+        int a = find(cons[c].a), b = find(cons[c].b);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+    std::vector<int> gid(n, -1), root2g(n, -1);
+    int ng = 0;
+    for (int i = 0; i < n; i++) {
+        int r = find(i);
+        if (root2g[r] < 0) root2g[r] = ng++;
+        gid[i] = root2g[r];
+    }
+    std::vector<int> aoff(ng + 1, 0), coff(ng + 1, 0);
+    for (int i = 0; i < n; i++) aoff[gid[i] + 1]++;
+    for (int c = 0; c < nConst; c++) coff[gid[cons[c].a] + 1]++;
+    for (int g = 0; g < ng; g++) {
+        aoff[g + 1] += aoff[g];
+        coff[g + 1] += coff[g];
+        if (aoff[g + 1] - aoff[g] > MDG_GRP_ATOMS || coff[g + 1] - coff[g] > MDG_GRP_CONS) {
+            ctx->err = "constraint group larger than 8 atoms / 16 constraints";
+            return MDG_EUNSUPPORTED;
+        }
+    }
+    std::vector<int> atoms(n), conl(std::max(nConst, 1));
+    std::vector<int> fa(aoff.begin(), aoff.end() - 1), fc(coff.begin(), coff.end() - 1);
+    for (int i = 0; i < n; i++) atoms[fa[gid[i]]++] = i;       // ascending within a group
+    for (int c = 0; c < nConst; c++) conl[fc[gid[cons[c].a]]++] = c;   // global order
+    ctx->nGrp = ng;
+    if (ctx->shkScr) {
+        hipFree(ctx->shkScr); hipFree(ctx->shkVir); hipFree(ctx->shkFirst); hipFree(ctx->shkKend);
+        hipFree(ctx->shkMask);
+    }
+    HIPCHK(hipMalloc(&ctx->shkScr, (size_t)SHK_SLOTS * 6 * n * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->shkVir, (size_t)SHK_SLOTS * std::max(ng, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->shkFirst, std::max(ng, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->shkKend, std::max(ng, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->shkMask, SHK_MASKW * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&ctx->grpAtomOff, (ng + 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->grpConOff, (ng + 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->grpAtoms, n * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->grpCons, std::max(nConst, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->grpCycle, ng * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->grpVir, ng * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->grpAtomOff, aoff.data(), (ng + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->grpConOff, coff.data(), (ng + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->grpAtoms, atoms.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    if (nConst > 0)
+        HIPCHK(hipMemcpy(ctx->grpCons, conl.data(), nConst * sizeof(int), hipMemcpyHostToDevice));
+    return MDG_OK;
+}
+
+// Runs one VV stage on the device state with forces fx.. (device, atom
+// order).  Stage 1 leaves the SHAKE virial sum in scal[S_VIRSHAKE]; stage 2
+// leaves kin and the kinetic stress in scal[S_KIN..S_KIN+6].
+int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
+                 double *virshake_out, bool sync)
+{
+    GrpArgs A;
+    A.P = ctx->P;
+    A.B = ctx->B;
+    A.nGrp = ctx->nGrp;
+    A.atomOff = ctx->grpAtomOff;
+    A.atoms = ctx->grpAtoms;
+    A.conOff = ctx->grpConOff;
+    A.cons = ctx->grpCons;
+    A.ca = ctx->ca;
+    A.cb = ctx->cb;
+    A.rc2 = ctx->rc2;
+    A.rmass = ctx->rmass;
+    A.mass = ctx->mass;
+    A.nAtConst = ctx->nAtConst;
+    A.x = ctx->x;
+    A.y = ctx->y;
+    A.z = ctx->z;
+    A.vx = ctx->vx;
+    A.vy = ctx->vy;
+    A.vz = ctx->vz;
+    A.fx = fx;
+    A.fy = fy;
+    A.fz = fz;
+    A.dd = ctx->dd;
+    A.nConst = ctx->nConst;
+    A.cycle = ctx->grpCycle;
+    A.vir = ctx->grpVir;
+    A.ctrl = ctx->ctrl;
+    A.part = ctx->epart + PART_KIN;
+    if (!ctx->shkScr) {
+        ctx->err = "constraint groups are not set up";
+        return MDG_EARG;
+    }
+    if (ctx->P.maxCycle > 64 * SHK_MASKW) {
+        ctx->err = "maxCycle above 256 is not supported";
+        return MDG_EUNSUPPORTED;
+    }
+    A.scr = ctx->shkScr;
+    A.virScr = ctx->shkVir;
+    A.first = ctx->shkFirst;
+    A.kend = ctx->shkKend;
+    A.mask = ctx->shkMask;
+    const int nb = nblk(ctx->nGrp, 128);
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl, ctx->shkMask);
+    LAUNCHCHK();
+    if (stage == 1) {
+        hipLaunchKernelGGL(k_vv1_a, dim3(nb), dim3(128), 0, ctx->stream, A);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_vv1_b, dim3(nb), dim3(128), 0, ctx->stream, A);
+        LAUNCHCHK();
+        // SHAKE virial: sum of the per-group sums (fixed order)
+        RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, ctx->nGrp, 1, ctx->scal + S_VIRSHAKE));
+    } else {
+        hipLaunchKernelGGL(k_vv2_a, dim3(nb), dim3(128), 0, ctx->stream, A);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_vv2_b, dim3(nb), dim3(128), 0, ctx->stream, A);
+        LAUNCHCHK();
+        RETCHK(mdg_reduce_partials(ctx, A.part, nb, 7, ctx->scal + S_KIN));
+    }
+    if (sync) {
+        int c[CTRL_WORDS];
+        double s[8];
+        HIPCHK(hipMemcpyAsync(c, ctx->ctrl, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(s, ctx->scal + S_KIN, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        double vs = 0.;
+        HIPCHK(hipMemcpyAsync(&vs, ctx->scal + S_VIRSHAKE, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (kin_out) {
+            for (int k = 0; k < 7; k++) kin_out[k] = s[k];
+            kin_out[0] *= 0.5;
+        }
+        if (virshake_out) *virshake_out = vs;
+        if (c[CTRL_SHAKE_FAIL]) {
+            ctx->err = "SHAKE did not converge within maxCycle";
+            return 311;
+        }
+        if (c[CTRL_SHAKE_ORDER]) {
+            ctx->err = "a constraint group re-diverged after converging";
+            return MDG_ESHAKE_ORDER;
+        }
+    }
+    return MDG_OK;
+}

@@ -1,0 +1,355 @@
+// mdg_internal.h -- shared state and device helpers of libmdgpu (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mdgpu.h"
+
+#define MDG_WAVE 64
+#define MDG_CL 8                 // atoms per cluster (8 x 8 = one wavefront per cluster pair)
+#define MDG_GRP_ATOMS 8          // max atoms in one constraint group
+#define MDG_GRP_CONS 16          // max constraints in one constraint group
+#define MDG_SEG_MAX 4096         // max atoms in one cell segment sorted in LDS
+
+// Fixed-point scales: forces in internal units (10 J/mol/A) carry 2^32
+// fractional steps in an int64 (|F| < 2^31); energies and virials 2^24
+// (|E| < 2^39); spread charges 2^40 (|Q| < 2^23).
+#define MDG_FSCALE 4294967296.0
+#define MDG_ESCALE 16777216.0
+#define MDG_QSCALE 1099511627776.0
+
+// Host/device copy of the run settings (PARAM/EWALD/NEIGH subset with the
+// derived constants of io.c:714-723) and the box (PBC after init_box).
+struct MdgBox {
+    int type;
+    double a1, a2, a3, b1, b2, b3, c1, c2, c3;
+    double u1, u2, u3, v1, v2, v3, w1, w2, w3;
+    double pa, pb, pc, u, v, w, det, vol;
+};
+
+struct MdgParams {
+    int n;
+    double cutOff, cutOn, delr, chargeConst, alpha;
+    double cutOff2, cutOn2, switch2, rcutOff, rcutOff2, cutnb2;
+    double timeStep, rTimeStep, tolShake;
+    int maxCycle, update, linkRatio;
+    int m1, m2, m3, nbsp;
+    int elecType;
+    int vdwOn;                   // vdwType == VSWITCH (vdw full / none evaluate nothing)
+};
+
+// Reference-format list on the device.
+struct RefList {
+    int *neighList = nullptr;    // flat j indices, in neighOrder order of i
+    long cap = 0;
+    long count = 0;
+    int *neighPair = nullptr;    // per atom
+    int *neighOrder = nullptr;   // position -> atom
+    int *cnt = nullptr;          // per position
+    long *off = nullptr;         // per position exclusive offsets (+1)
+};
+
+// Cluster-pair list: atoms live in "slots" (cluster*8 + k), ordered by
+// spatial columns; each entry (I, J, mask) covers the 64 atom pairs of
+// clusters I <= J, bit a*8+b set iff (slot I*8+a, slot J*8+b) is a listed
+// pair of the reference list.
+struct ClusterList {
+    int nclu = 0, nslot = 0;
+    int *slotAtom = nullptr;     // slot -> atom (-1 padding)
+    int *atomSlot = nullptr;     // atom -> slot
+    float4 *bb = nullptr;        // per cluster: centre xyz, unused
+    float4 *bbh = nullptr;       // per cluster: half extents xyz, unused
+    int *cpOff = nullptr;        // per I cluster: first entry (nclu+1)
+    int *cpJ = nullptr;          // entry -> J
+    unsigned long long *cpMask = nullptr;
+    int *cpI = nullptr;          // entry -> I
+    long cap = 0;
+    int *tmpCnt = nullptr;       // per I: entries found
+    int *tmpJ = nullptr;         // per I scratch [nclu][kmax]
+    unsigned long long *tmpMask = nullptr;
+    int kmax = 0;
+    // spatial sort scratch
+    int ncol = 0, nfx = 0, nfy = 0;
+    int *colCnt = nullptr, *colStart = nullptr, *colPad = nullptr, *colSlot = nullptr;
+    int *sortTmp = nullptr;
+    float *sortKey = nullptr;
+};
+
+struct mdg_ctx {
+    int precision = MDG_FP64;
+    int n = 0;
+    int keyLink = 0;
+    int nlc[3] = {0, 0, 0};
+    MdgParams P{};
+    MdgBox B{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // state (FP64, reference atom order)
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    // topology
+    double *q = nullptr, *eps = nullptr, *sig = nullptr, *mass = nullptr, *rmass = nullptr;
+    int *frozen = nullptr, *nAtConst = nullptr;
+    bool hasFrozen = false;
+    int nConst = 0;
+    int *ca = nullptr, *cb = nullptr;
+    double *rc2 = nullptr;
+    double *dd = nullptr;        // constraint vectors of the current VV stage [3][nConst]
+    // constraint groups (connected components; singletons for free atoms)
+    int nGrp = 0;
+    int *grpAtomOff = nullptr, *grpAtoms = nullptr, *grpConOff = nullptr, *grpCons = nullptr;
+    int *grpCycle = nullptr;     // per group: cycles to own convergence
+    double *grpVir = nullptr;    // per group: SHAKE virial
+    int *ctrl = nullptr;         // device control words (see CTRL_*)
+    double *shkScr = nullptr, *shkVir = nullptr;   // SHAKE state window (mdg_integrate.hip)
+    int *shkFirst = nullptr, *shkKend = nullptr;
+    unsigned long long *shkMask = nullptr;
+    // exclusions (CSR; symmetric for link cells, ordered otherwise)
+    int *exOff = nullptr, *exLst = nullptr;
+    int nExcl = 0;
+    std::vector<int> hExOff, hExLst;
+    bool exclBuilt = false;
+
+    // link-cell / Verlet scratch
+    double *xu = nullptr, *yu = nullptr, *zu = nullptr;
+    int *cellOf = nullptr, *cellCnt = nullptr, *cellStart = nullptr;
+    int ncell = 0;
+    unsigned long long *lcKeys = nullptr;
+    double *lcS3 = nullptr;
+
+    RefList rl;
+    ClusterList cl;
+
+    // forces / energies of the device-resident path (slot order)
+    double *sx = nullptr, *sy = nullptr, *sz = nullptr;        // slot positions
+    float4 *sp32 = nullptr;                                     // slot q, eps, sig
+    double *sq = nullptr, *seps = nullptr, *ssig = nullptr;
+    double *fdx = nullptr, *fdy = nullptr, *fdz = nullptr;     // FP64 slot forces
+    double *scal = nullptr;                                     // reduced scalars (S_*)
+    long long *fix = nullptr, *fiy = nullptr, *fiz = nullptr;  // int64 fixed point
+    float *frx = nullptr, *fry = nullptr, *frz = nullptr;      // SPME forces (mixed)
+    double *frx64 = nullptr, *fry64 = nullptr, *frz64 = nullptr;
+    double *epart = nullptr;     // per-block FP64 partials
+    int epartCap = 0;
+    MDG_STEP *recDev = nullptr;  // per-step records
+    int recCap = 0;
+
+    // SPME
+    int m1 = 0, m2 = 0, m3 = 0, mh = 0;   // mh = m3/2+1
+    double *theta = nullptr;     // [m1][m2][mh]
+    float *theta32 = nullptr;
+    double *grid64 = nullptr;    // real mesh (FP64)
+    double *cgrid64 = nullptr;   // complex half spectrum (FP64, interleaved)
+    float *grid32 = nullptr;
+    float *cgrid32 = nullptr;
+    long long *gridq = nullptr;  // fixed-point spread mesh (mixed)
+    rocfft_plan fwd = nullptr, bwd = nullptr, fwd32 = nullptr, bwd32 = nullptr;
+    rocfft_execution_info info = nullptr;
+    void *fftWork = nullptr;
+    size_t fftWorkSize = 0;
+    double eSelf = 0., eNet = 0., systq = 0.;
+
+    // timing
+    bool profiling = false;
+    hipEvent_t ev[12] = {};
+    hipEvent_t *evCur = ev;      // the event set the next step records into
+    double phase[5] = {0, 0, 0, 0, 0};
+
+    // scan scratch (mdg_util.hip)
+    void *scanBuf[8] = {};
+    size_t scanSize[8] = {};
+
+    // graph
+    hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    int graphStepsPerLaunch = 0;
+    int step = 0;                // last completed MD step
+    int stepsSinceList = 0;
+};
+
+// control words in ctx->ctrl
+enum {
+    CTRL_SHAKE_MAX = 0,          // max cycles of any group (stage A -> B)
+    CTRL_SHAKE_FAIL = 1,         // a group did not converge within maxCycle
+    CTRL_SHAKE_ORDER = 2,        // a group re-diverged after converging
+    CTRL_LIST_OVF = 3,           // cluster scratch overflow
+    CTRL_STEP = 4,               // device-side step counter
+    CTRL_REBUILT = 5,
+    CTRL_SEG_OVF = 6,
+    CTRL_WORDS = 8
+};
+
+// reduced scalars in ctx->scal
+enum {
+    S_EDIR = 0, S_EVDW, S_VIRDIR, S_VIRVDW, S_NIN,   // pair kernel (5)
+    S_ECORR = 5, S_VIRCORR,                          // exclusion correction (2)
+    S_EREC = 7,                                      // sum theta|F|^2, then st0..st5 (7)
+    S_FRX = 14, S_FRY, S_FRZ,                        // sums of reciprocal forces (3)
+    S_KIN = 17,                                      // sum m v^2, kinetic stress (7)
+    S_VIRSHAKE = 24,
+    S_REF = 25,                                      // drop-in pair loop (6)
+    S_REFCORR = 31,                                  // drop-in correction (2)
+    S_COUNT = 40
+};
+
+// regions of ctx->epart (per-block partial sums), each PART_R doubles
+#define PART_R (16384 * 8)
+enum {
+    PART_PAIR = 0, PART_EXCL = PART_R, PART_INFL = 2 * PART_R, PART_GATHER = 3 * PART_R,
+    PART_KIN = 4 * PART_R, PART_MISC = 5 * PART_R, PART_REF = 6 * PART_R, PART_TOTAL = 8 * PART_R
+};
+
+// ---------------------------------------------------------------- helpers
+#define HIPCHK(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);     \
+            return MDG_EHIP;                                                  \
+        }                                                                     \
+    } while (0)
+
+#define LAUNCHCHK()                                                           \
+    do {                                                                      \
+        hipError_t e_ = hipGetLastError();                                    \
+        if (e_ != hipSuccess) {                                               \
+            ctx->err = std::string("kernel launch: ") + hipGetErrorString(e_);\
+            return MDG_EHIP;                                                  \
+        }                                                                     \
+    } while (0)
+
+#define RETCHK(call)                                                          \
+    do {                                                                      \
+        int r_ = (call);                                                      \
+        if (r_ != 0) return r_;                                               \
+    } while (0)
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+// utils.c:606-609: round half away from zero through a truncating cast.
+__host__ __device__ inline int mdg_nint(double x)
+{
+    return x >= 0. ? (int)(x + 0.5) : (int)(x - 0.5);
+}
+
+// utils.c:38-80 (dist): minimum image of d in place, |d|^2 as
+// (dx^2 + dy^2) + dz^2 without contraction.
+__host__ __device__ inline double mdg_dist(const MdgBox &b, double &dx, double &dy, double &dz)
+{
+#pragma clang fp contract(off)
+    double xt, yt, zt;
+    switch (b.type) {
+    case 1:
+        dx -= b.a1 * mdg_nint(dx * b.u1);
+        dy -= b.a1 * mdg_nint(dy * b.u1);
+        dz -= b.a1 * mdg_nint(dz * b.u1);
+        break;
+    case 2:
+        dx -= b.a1 * mdg_nint(dx * b.u1);
+        dy -= b.b2 * mdg_nint(dy * b.v2);
+        dz -= b.c3 * mdg_nint(dz * b.w3);
+        break;
+    case 3:
+        xt = dx * b.u1 + dy * b.u2 + dz * b.u3;
+        yt = dx * b.v1 + dy * b.v2 + dz * b.v3;
+        zt = dx * b.w1 + dy * b.w2 + dz * b.w3;
+        xt -= mdg_nint(xt);
+        yt -= mdg_nint(yt);
+        zt -= mdg_nint(zt);
+        dx = xt * b.a1 + yt * b.b1 + zt * b.c1;
+        dy = xt * b.a2 + yt * b.b2 + zt * b.c2;
+        dz = xt * b.a3 + yt * b.b3 + zt * b.c3;
+        break;
+    default:
+        break;
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Fixed-point conversions (round to nearest).
+__device__ inline long long mdg_tofix(double v, double scale)
+{
+    return (long long)__double2ll_rn(v * scale);
+}
+
+__device__ inline void mdg_atomic_add_fix(long long *p, double v, double scale)
+{
+    long long iv = mdg_tofix(v, scale);
+    if (iv != 0) atomicAdd((unsigned long long *)p, (unsigned long long)iv);
+}
+
+// Wave-level reductions (64 lanes).
+__device__ inline double mdg_wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline float mdg_wave_sumf(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block reduction of NV doubles into out[blockIdx.x*NV + k] (blockDim <= 1024).
+template <int NV>
+__device__ inline void mdg_block_partials(double (&v)[NV], double *out)
+{
+    __shared__ double red[16][NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = mdg_wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; w++) s += red[w][threadIdx.x];
+        out[(long)blockIdx.x * NV + threadIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------- internal API
+int mdg_alloc_state(mdg_ctx *ctx);
+int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nConst,
+                         const MDG_CONSTRAINT *cons, int nAngle, const MDG_ANGLE *angle);
+int mdg_upload_exclusions(mdg_ctx *ctx);
+
+// scan / sort utilities (mdg_util.hip)
+int mdg_scan_int_to_long(mdg_ctx *ctx, const int *in, long *out, int n);  // out[0]=0, out[n]=total
+int mdg_scan_int(mdg_ctx *ctx, const int *in, int *out, int n);           // exclusive, out[n]=total
+int mdg_fill_i64(mdg_ctx *ctx, long long *p, long n);
+int mdg_fill_f64(mdg_ctx *ctx, double *p, long n, double v);
+int mdg_fill_i32(mdg_ctx *ctx, int *p, long n, int v);
+int mdg_reduce_partials(mdg_ctx *ctx, const double *part, int nblocks, int nv, double *out);
+
+// list (mdg_list.hip)
+int mdg_build_ref_list(mdg_ctx *ctx, bool wrap);
+int mdg_build_cluster_list(mdg_ctx *ctx);
+
+// pair (mdg_pair.hip)
+int mdg_pair_ref64(mdg_ctx *ctx, int elecType, double *fx, double *fy, double *fz,
+                   const int *neighList, const long *off, const int *order, double *eout);
+int mdg_excl_corr64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double *eout);
+int mdg_pair_cluster(mdg_ctx *ctx);
+int mdg_slot_params(mdg_ctx *ctx);
+
+// spme (mdg_spme.hip)
+int mdg_spme_setup(mdg_ctx *ctx);
+int mdg_spme_free(mdg_ctx *ctx);
+int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8]);
+int mdg_spme_device(mdg_ctx *ctx);   // device-resident: frx.. per atom, sums into scal
+
+// integrate (mdg_integrate.hip)
+int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst);
+int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
+                 double *virshake_out, bool sync);

@@ -1,0 +1,606 @@
+// mdg_pair.hip -- real-space nonbonded kernels.
+//
+//  * k_pair_ref64: the reference pair loop (energy.c:480-561 for Ewald,
+//    energy.c:216-345 for the cutoff Coulomb functors) over a list in the
+//    reference format, FP64, one thread per i in neighOrder order.
+//  * k_excl64: the excluded-pair erf correction (energy.c:567-621).
+//  * k_pair_cluster<MIXED>: the same physics over the cluster-pair list,
+//    one 64-lane wavefront per i-cluster, lane (a, b) = pair (i_a, j_b) of
+//    each 8x8 cluster pair.  i forces stay in registers across the whole
+//    j list and are reduced once over b; j forces are reduced over a with
+//    cross-lane shuffles after each cluster pair and added with one
+//    coalesced 64-byte atomic row per component.  MIXED = FP32 pair
+//    arithmetic (displacements formed in FP64 first) with int64
+//    fixed-point force accumulation; otherwise FP64 throughout.
+#include "mdg_internal.h"
+
+#define SQRTPI 1.772453850905516     // global.h:54
+
+// ewald.c:356-369
+__device__ inline double ewald_dir64(double alpha, double qel, double r, double rt, double &dE)
+{
+    double ar = alpha * r;
+    double e = qel * erfc(ar) * rt;
+    double d = 2.0 * qel * alpha * exp(-(ar * ar)) / SQRTPI;
+    dE = -(e + d) * rt;
+    return e;
+}
+
+// ewald.c:371-384
+__device__ inline double ewald_corr64(double alpha, double qel, double r, double rt, double &dE)
+{
+    double ar = alpha * r;
+    double e = -qel * erf(ar) * rt;
+    double d = 2.0 * qel * alpha * exp(-(ar * ar)) / SQRTPI;
+    dE = -(e + d) * rt;
+    return e;
+}
+
+// vdw.c:144-184
+__device__ inline double vdw_switch64(const MdgParams &P, double veps, double vsig, double r2,
+                                      double rt, double &dE)
+{
+    double s = vsig * rt;
+    double s3 = s * s * s;
+    double s6 = s3 * s3;
+    double s12 = s6 * s6;
+    if (r2 <= P.cutOn2) {
+        dE = 24. * veps * rt * (s6 - 2. * s12);
+        return 4. * veps * (s12 - s6);
+    }
+    double sw1 = P.cutOff2 - r2;
+    double S = sw1 * sw1 * (P.cutOff2 + 2. * r2 - 3. * P.cutOn2) * P.switch2;
+    double dS = 12. * r2 * sw1 * (P.cutOn2 - r2) * P.switch2;
+    double pv = 4. * veps * (s12 - s6);
+    double dpv = 24. * veps * (s6 - 2. * s12);
+    dE = (pv * dS + dpv * S) * rt;
+    return pv * S;
+}
+
+// elec.c:137-263: SHIFT1 (2), SHIFT2 (3), SWITCH (4); qel = q_i q_j.
+__device__ inline double coulomb64(const MdgParams &P, int type, double qel, double r2, double rt,
+                                   double &dE)
+{
+    double pe = P.chargeConst * qel * rt;
+    if (type == 2) {
+        double s1 = rt * r2 * P.rcutOff, s2 = r2 * P.rcutOff2;
+        double f = 1. - 2. * s1 + s2, df = 2. * (s2 - s1);
+        dE = pe * rt * (df - f);
+        return pe * f;
+    }
+    if (type == 3) {
+        double s1 = r2 * P.rcutOff2, s2 = s1 * s1;
+        double f = 1. - 2. * s1 + s2, df = 4. * (s2 - s1);
+        dE = pe * rt * (df - f);
+        return pe * f;
+    }
+    if (type == 4) {
+        if (r2 <= P.cutOn2) {
+            dE = -pe * rt;
+            return pe;
+        }
+        double sw1 = P.cutOff2 - r2;
+        double f = sw1 * sw1 * (P.cutOff2 + 2. * r2 - 3. * P.cutOn2) * P.switch2;
+        double df = 12. * r2 * sw1 * (P.cutOn2 - r2) * P.switch2;
+        dE = pe * rt * (df - f);
+        return pe * f;
+    }
+    dE = 0.;
+    return 0.;
+}
+
+// ---------------------------------------------------------------- reference list
+
+// energies out per block: [eElec, eVdw, virElec, virVdw, nIn, nListed]
+template <int ELEC>
+__global__ void __launch_bounds__(256)
+k_pair_ref64(MdgParams P, MdgBox B, const double *__restrict__ x, const double *__restrict__ y,
+             const double *__restrict__ z, const double *__restrict__ q,
+             const double *__restrict__ eps, const double *__restrict__ sig,
+             const int *__restrict__ neighList, const long *__restrict__ off,
+             const int *__restrict__ order, double *fx, double *fy, double *fz,
+             double *__restrict__ part)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (l < P.n) {
+        const int i = order[l];
+        const double xi = x[i], yi = y[i], zi = z[i], qi = q[i], ei = eps[i], si = sig[i];
+        double fxi = 0., fyi = 0., fzi = 0.;
+        for (long k = off[l]; k < off[l + 1]; k++) {
+            const int j = neighList[k];
+            double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+            double r2 = mdg_dist(B, dx, dy, dz);
+            acc[5] += 1.;
+            if (r2 <= P.cutOff2) {
+                double r = sqrt(r2), rt = 1. / r;
+                double dEl, dVw, el;
+                if (ELEC == 0) el = ewald_dir64(P.alpha, P.chargeConst * qi * q[j], r, rt, dEl);
+                else el = coulomb64(P, ELEC, qi * q[j], r2, rt, dEl);
+                double ev = 0.;
+                dVw = 0.;
+                if (P.vdwOn) ev = vdw_switch64(P, ei * eps[j], si + sig[j], r2, rt, dVw);
+                acc[0] += el;
+                acc[1] += ev;
+                acc[2] += dEl * r;
+                acc[3] += dVw * r;
+                acc[4] += 1.;
+                double fs = (dEl + dVw);
+                double fxj = fs * dx * rt, fyj = fs * dy * rt, fzj = fs * dz * rt;
+                fxi += fxj;
+                fyi += fyj;
+                fzi += fzj;
+                atomicAdd(&fx[j], -fxj);
+                atomicAdd(&fy[j], -fyj);
+                atomicAdd(&fz[j], -fzj);
+            }
+        }
+        atomicAdd(&fx[i], fxi);
+        atomicAdd(&fy[i], fyi);
+        atomicAdd(&fz[i], fzi);
+    }
+    mdg_block_partials<6>(acc, part);
+}
+
+// energy.c:567-621 -- per block: [eCorr, virCorr]
+__global__ void __launch_bounds__(256)
+k_excl64(MdgParams P, MdgBox B, const double *__restrict__ x, const double *__restrict__ y,
+         const double *__restrict__ z, const double *__restrict__ q, const int *__restrict__ exOff,
+         const int *__restrict__ exLst, double *fx, double *fy, double *fz,
+         double *__restrict__ part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < P.n) {
+        double fxi = 0., fyi = 0., fzi = 0.;
+        for (int k = exOff[i]; k < exOff[i + 1]; k++) {
+            int j = exLst[k];
+            if (j <= i) continue;
+            double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+            double r2 = mdg_dist(B, dx, dy, dz);
+            double r = sqrt(r2), rt = 1. / r, dC;
+            acc[0] += ewald_corr64(P.alpha, P.chargeConst * q[i] * q[j], r, rt, dC);
+            acc[1] += dC * r;
+            double fxj = dC * dx * rt, fyj = dC * dy * rt, fzj = dC * dz * rt;
+            fxi += fxj;
+            fyi += fyj;
+            fzi += fzj;
+            atomicAdd(&fx[j], -fxj);
+            atomicAdd(&fy[j], -fyj);
+            atomicAdd(&fz[j], -fzj);
+        }
+        if (fxi != 0. || fyi != 0. || fzi != 0.) {
+            atomicAdd(&fx[i], fxi);
+            atomicAdd(&fy[i], fyi);
+            atomicAdd(&fz[i], fzi);
+        }
+    }
+    mdg_block_partials<2>(acc, part);
+}
+
+int mdg_pair_ref64(mdg_ctx *ctx, int elecType, double *fx, double *fy, double *fz,
+                   const int *neighList, const long *off, const int *order, double *eout)
+{
+    const int n = ctx->n, nb = nblk(n, 256);
+    double *part = ctx->epart + PART_REF;
+    switch (elecType) {
+    case 0:
+        hipLaunchKernelGGL(k_pair_ref64<0>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
+                           order, fx, fy, fz, part);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_pair_ref64<2>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
+                           order, fx, fy, fz, part);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_pair_ref64<3>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
+                           order, fx, fy, fz, part);
+        break;
+    case 4:
+        hipLaunchKernelGGL(k_pair_ref64<4>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
+                           order, fx, fy, fz, part);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_pair_ref64<1>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
+                           order, fx, fy, fz, part);
+        break;
+    default:
+        ctx->err = "unsupported elecType";
+        return MDG_EUNSUPPORTED;
+    }
+    LAUNCHCHK();
+    RETCHK(mdg_reduce_partials(ctx, part, nb, 6, eout));
+    return MDG_OK;
+}
+
+int mdg_excl_corr64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double *eout)
+{
+    const int n = ctx->n, nb = nblk(n, 256);
+    double *part = ctx->epart + PART_EXCL;
+    hipLaunchKernelGGL(k_excl64, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B, ctx->x,
+                       ctx->y, ctx->z, ctx->q, ctx->exOff, ctx->exLst, fx, fy, fz, part);
+    LAUNCHCHK();
+    RETCHK(mdg_reduce_partials(ctx, part, nb, 2, eout));
+    return MDG_OK;
+}
+
+// ---------------------------------------------------------------- cluster kernels
+
+// FP32 constants of the mixed pair kernel.
+struct PairF {
+    float alpha, c2a, qc, cutOn2, cutOff2, switch2, threeOn2;
+    double cutOff2d, L0, L1, L2, iL0, iL1, iL2;
+};
+
+// erfc(x) for x >= 0 by the Chebyshev-fitted form of Numerical Recipes
+// (Press et al., 2nd ed., sec. 6.2, "erfcc"): fractional error < 1.2e-7
+// over the whole range.  Returns erfc; *ez receives exp(-x^2).
+__device__ inline float mdg_erfc_f32(float x, float *ez)
+{
+    float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+    float p = fmaf(t, 0.17087277f, -0.82215223f);
+    p = fmaf(t, p, 1.48851587f);
+    p = fmaf(t, p, -1.13520398f);
+    p = fmaf(t, p, 0.27886807f);
+    p = fmaf(t, p, -0.18628806f);
+    p = fmaf(t, p, 0.09678418f);
+    p = fmaf(t, p, 0.37409196f);
+    p = fmaf(t, p, 1.00002368f);
+    p = fmaf(t, p, -1.26551223f);
+    float x2 = x * x;
+    *ez = expf(-x2);
+    return t * expf(p - x2);
+}
+
+struct ClusterArgs {
+    MdgParams P;
+    MdgBox B;
+    PairF F;
+    int nclu;
+    const double *sx, *sy, *sz;      // slot positions
+    const float4 *sp32;              // slot (q, eps, sig, 0) for MIXED
+    const double *sq, *seps, *ssig;  // slot parameters for FP64
+    const int *cpOff, *cpJ;
+    const unsigned long long *cpMask;
+    long long *fix, *fiy, *fiz;      // MIXED: int64 fixed-point slot forces
+    double *fdx, *fdy, *fdz;         // FP64: slot forces
+    double *part;                    // per block: eDir, eVdw, virDir, virVdw, nIn
+};
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
+{
+    const int lane = threadIdx.x & 63;
+    const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int a = lane >> 3, b = lane & 7;
+    double e_acc[5] = {0, 0, 0, 0, 0};
+    if (I < A.nclu) {
+        const int si = I * MDG_CL + a;
+        const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
+        float qi, ei, gi;
+        double qid, eid, gid;
+        if (MIXED) {
+            float4 p = A.sp32[si];
+            qi = p.x; ei = p.y; gi = p.z;
+        } else {
+            qid = A.sq[si]; eid = A.seps[si]; gid = A.ssig[si];
+        }
+        float fxi = 0.f, fyi = 0.f, fzi = 0.f;
+        double fxid = 0., fyid = 0., fzid = 0.;
+        double ed = 0., ev = 0., vd = 0., vv = 0.;
+        int nin = 0;
+        const int e0 = A.cpOff[I], e1 = A.cpOff[I + 1];
+        for (int e = e0; e < e1; e++) {
+            const int J = A.cpJ[e];
+            const unsigned long long m = A.cpMask[e];
+            const int sj = J * MDG_CL + b;
+            double dx = A.sx[sj] - xi, dy = A.sy[sj] - yi, dz = A.sz[sj] - zi;
+            double r2;
+            if (MIXED) {
+                dx = fma(-A.F.L0, rint(dx * A.F.iL0), dx);
+                dy = fma(-A.F.L1, rint(dy * A.F.iL1), dy);
+                dz = fma(-A.F.L2, rint(dz * A.F.iL2), dz);
+                r2 = dx * dx + dy * dy + dz * dz;
+            } else {
+                r2 = mdg_dist(A.B, dx, dy, dz);
+            }
+            const bool in = ((m >> lane) & 1ull) && r2 <= A.P.cutOff2;
+            if (MIXED) {
+                float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+                if (in) {
+                    const float4 pj = A.sp32[sj];
+                    const float r2f = (float)r2;
+                    const float rinv = __builtin_amdgcn_rsqf(r2f);
+                    const float r = r2f * rinv;
+                    const float qq = A.F.qc * qi * pj.x;
+                    float ez;
+                    const float erfcv = mdg_erfc_f32(A.F.alpha * r, &ez);
+                    const float eel = qq * erfcv * rinv;
+                    const float del = -(eel + qq * A.F.c2a * ez) * rinv;
+                    const float veps = ei * pj.y;
+                    const float s = (gi + pj.z) * rinv;
+                    const float s2 = s * s;
+                    const float s6 = s2 * s2 * s2;
+                    const float s12 = s6 * s6;
+                    const float pv = 4.f * veps * (s12 - s6);
+                    const float dpv = 24.f * veps * (s6 - 2.f * s12);
+                    float S = 1.f, dS = 0.f;
+                    if (r2f > A.F.cutOn2) {
+                        const float sw1 = A.F.cutOff2 - r2f;
+                        S = sw1 * sw1 * (A.F.cutOff2 + 2.f * r2f - A.F.threeOn2) * A.F.switch2;
+                        dS = 12.f * r2f * sw1 * (A.F.cutOn2 - r2f) * A.F.switch2;
+                    }
+                    const float evdw = pv * S;
+                    const float dvdw = (pv * dS + dpv * S) * rinv;
+                    const float fs = (del + dvdw) * rinv;
+                    fjx = fs * (float)dx;
+                    fjy = fs * (float)dy;
+                    fjz = fs * (float)dz;
+                    ed += eel;
+                    ev += evdw;
+                    vd += del * r;
+                    vv += dvdw * r;
+                    nin++;
+                }
+                fxi += fjx;
+                fyi += fjy;
+                fzi += fjz;
+                // j side: sum over the 8 i lanes (a), negate
+                fjx += __shfl_xor(fjx, 8, 64);
+                fjy += __shfl_xor(fjy, 8, 64);
+                fjz += __shfl_xor(fjz, 8, 64);
+                fjx += __shfl_xor(fjx, 16, 64);
+                fjy += __shfl_xor(fjy, 16, 64);
+                fjz += __shfl_xor(fjz, 16, 64);
+                fjx += __shfl_xor(fjx, 32, 64);
+                fjy += __shfl_xor(fjy, 32, 64);
+                fjz += __shfl_xor(fjz, 32, 64);
+                if (a == 0 && (m >> 0) != 0ull) {
+                    mdg_atomic_add_fix(&A.fix[sj], -(double)fjx, MDG_FSCALE);
+                    mdg_atomic_add_fix(&A.fiy[sj], -(double)fjy, MDG_FSCALE);
+                    mdg_atomic_add_fix(&A.fiz[sj], -(double)fjz, MDG_FSCALE);
+                }
+            } else {
+                double fjx = 0., fjy = 0., fjz = 0.;
+                if (in) {
+                    const double r = sqrt(r2), rt = 1. / r;
+                    double dEl, dVw;
+                    const double el = ewald_dir64(A.P.alpha, A.P.chargeConst * qid * A.sq[sj], r,
+                                                  rt, dEl);
+                    const double evw = vdw_switch64(A.P, eid * A.seps[sj], gid + A.ssig[sj], r2,
+                                                    rt, dVw);
+                    const double fs = dEl + dVw;
+                    fjx = fs * dx * rt;
+                    fjy = fs * dy * rt;
+                    fjz = fs * dz * rt;
+                    e_acc[0] += el;
+                    e_acc[1] += evw;
+                    e_acc[2] += dEl * r;
+                    e_acc[3] += dVw * r;
+                    e_acc[4] += 1.;
+                }
+                fxid += fjx;
+                fyid += fjy;
+                fzid += fjz;
+                fjx += __shfl_xor(fjx, 8, 64);
+                fjy += __shfl_xor(fjy, 8, 64);
+                fjz += __shfl_xor(fjz, 8, 64);
+                fjx += __shfl_xor(fjx, 16, 64);
+                fjy += __shfl_xor(fjy, 16, 64);
+                fjz += __shfl_xor(fjz, 16, 64);
+                fjx += __shfl_xor(fjx, 32, 64);
+                fjy += __shfl_xor(fjy, 32, 64);
+                fjz += __shfl_xor(fjz, 32, 64);
+                if (a == 0) {
+                    if (fjx != 0.) atomicAdd(&A.fdx[sj], -fjx);
+                    if (fjy != 0.) atomicAdd(&A.fdy[sj], -fjy);
+                    if (fjz != 0.) atomicAdd(&A.fdz[sj], -fjz);
+                }
+            }
+        }
+        if (MIXED) {
+            double gx = fxi, gy = fyi, gz = fzi;
+            gx += __shfl_xor(gx, 1, 64);
+            gy += __shfl_xor(gy, 1, 64);
+            gz += __shfl_xor(gz, 1, 64);
+            gx += __shfl_xor(gx, 2, 64);
+            gy += __shfl_xor(gy, 2, 64);
+            gz += __shfl_xor(gz, 2, 64);
+            gx += __shfl_xor(gx, 4, 64);
+            gy += __shfl_xor(gy, 4, 64);
+            gz += __shfl_xor(gz, 4, 64);
+            if (b == 0) {
+                mdg_atomic_add_fix(&A.fix[si], gx, MDG_FSCALE);
+                mdg_atomic_add_fix(&A.fiy[si], gy, MDG_FSCALE);
+                mdg_atomic_add_fix(&A.fiz[si], gz, MDG_FSCALE);
+            }
+            e_acc[0] = ed;
+            e_acc[1] = ev;
+            e_acc[2] = vd;
+            e_acc[3] = vv;
+            e_acc[4] = nin;
+        } else {
+            fxid += __shfl_xor(fxid, 1, 64);
+            fyid += __shfl_xor(fyid, 1, 64);
+            fzid += __shfl_xor(fzid, 1, 64);
+            fxid += __shfl_xor(fxid, 2, 64);
+            fyid += __shfl_xor(fyid, 2, 64);
+            fzid += __shfl_xor(fzid, 2, 64);
+            fxid += __shfl_xor(fxid, 4, 64);
+            fyid += __shfl_xor(fyid, 4, 64);
+            fzid += __shfl_xor(fzid, 4, 64);
+            if (b == 0) {
+                atomicAdd(&A.fdx[si], fxid);
+                atomicAdd(&A.fdy[si], fyid);
+                atomicAdd(&A.fdz[si], fzid);
+            }
+        }
+    }
+    mdg_block_partials<5>(e_acc, A.part);
+}
+
+// Excluded-pair erf correction on the device-resident path (FP64 math in
+// both precisions), forces into the slot accumulators.  Per block:
+// [eCorr, virCorr].
+template <bool MIXED>
+__global__ void __launch_bounds__(256)
+k_excl_dev(MdgParams P, MdgBox B, const double *__restrict__ x, const double *__restrict__ y,
+           const double *__restrict__ z, const double *__restrict__ q, const int *__restrict__ exOff,
+           const int *__restrict__ exLst, const int *__restrict__ atomSlot, long long *fix,
+           long long *fiy, long long *fiz, double *fdx, double *fdy, double *fdz,
+           double *__restrict__ part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < P.n) {
+        double fxi = 0., fyi = 0., fzi = 0.;
+        for (int k = exOff[i]; k < exOff[i + 1]; k++) {
+            int j = exLst[k];
+            if (j <= i) continue;
+            double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+            double r2 = mdg_dist(B, dx, dy, dz);
+            double r = sqrt(r2), rt = 1. / r, dC;
+            acc[0] += ewald_corr64(P.alpha, P.chargeConst * q[i] * q[j], r, rt, dC);
+            acc[1] += dC * r;
+            double fxj = dC * dx * rt, fyj = dC * dy * rt, fzj = dC * dz * rt;
+            fxi += fxj;
+            fyi += fyj;
+            fzi += fzj;
+            int sj = atomSlot[j];
+            if (MIXED) {
+                mdg_atomic_add_fix(&fix[sj], -fxj, MDG_FSCALE);
+                mdg_atomic_add_fix(&fiy[sj], -fyj, MDG_FSCALE);
+                mdg_atomic_add_fix(&fiz[sj], -fzj, MDG_FSCALE);
+            } else {
+                atomicAdd(&fdx[sj], -fxj);
+                atomicAdd(&fdy[sj], -fyj);
+                atomicAdd(&fdz[sj], -fzj);
+            }
+        }
+        int si = atomSlot[i];
+        if (MIXED) {
+            mdg_atomic_add_fix(&fix[si], fxi, MDG_FSCALE);
+            mdg_atomic_add_fix(&fiy[si], fyi, MDG_FSCALE);
+            mdg_atomic_add_fix(&fiz[si], fzi, MDG_FSCALE);
+        } else if (fxi != 0. || fyi != 0. || fzi != 0.) {
+            atomicAdd(&fdx[si], fxi);
+            atomicAdd(&fdy[si], fyi);
+            atomicAdd(&fdz[si], fzi);
+        }
+    }
+    mdg_block_partials<2>(acc, part);
+}
+
+// Slot-ordered positions for the pair kernel (padding slots get copies of
+// a far point; their mask bits are never set).
+__global__ void k_slot_pos(int nslot, const int *__restrict__ slotAtom, const double *__restrict__ x,
+                           const double *__restrict__ y, const double *__restrict__ z,
+                           double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    int a = slotAtom[s];
+    sx[s] = a >= 0 ? x[a] : 1e7;
+    sy[s] = a >= 0 ? y[a] : 1e7;
+    sz[s] = a >= 0 ? z[a] : 1e7;
+}
+
+__global__ void k_slot_param(int nslot, const int *__restrict__ slotAtom,
+                             const double *__restrict__ q, const double *__restrict__ eps,
+                             const double *__restrict__ sig, float4 *__restrict__ sp32,
+                             double *__restrict__ sq, double *__restrict__ seps,
+                             double *__restrict__ ssig)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    int a = slotAtom[s];
+    double qa = a >= 0 ? q[a] : 0., ea = a >= 0 ? eps[a] : 0., ga = a >= 0 ? sig[a] : 0.;
+    sp32[s] = make_float4((float)qa, (float)ea, (float)ga, 0.f);
+    sq[s] = qa;
+    seps[s] = ea;
+    ssig[s] = ga;
+}
+
+int mdg_slot_params(mdg_ctx *ctx)
+{
+    ClusterList &C = ctx->cl;
+    const int nslot = C.nclu * MDG_CL;
+    hipLaunchKernelGGL(k_slot_param, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream, nslot,
+                       C.slotAtom, ctx->q, ctx->eps, ctx->sig, ctx->sp32, ctx->sq, ctx->seps,
+                       ctx->ssig);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+int mdg_pair_cluster(mdg_ctx *ctx)
+{
+    ClusterList &C = ctx->cl;
+    const int nslot = C.nclu * MDG_CL;
+    hipLaunchKernelGGL(k_slot_pos, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream, nslot,
+                       C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz);
+    LAUNCHCHK();
+    ClusterArgs A;
+    A.P = ctx->P;
+    A.B = ctx->B;
+    A.F.alpha = (float)ctx->P.alpha;
+    A.F.c2a = (float)(2.0 * ctx->P.alpha / SQRTPI);
+    A.F.qc = (float)ctx->P.chargeConst;
+    A.F.cutOn2 = (float)ctx->P.cutOn2;
+    A.F.cutOff2 = (float)ctx->P.cutOff2;
+    A.F.switch2 = (float)ctx->P.switch2;
+    A.F.threeOn2 = (float)(3.0 * ctx->P.cutOn2);
+    A.F.cutOff2d = ctx->P.cutOff2;
+    A.F.L0 = ctx->B.a1;
+    A.F.L1 = ctx->B.type == 1 ? ctx->B.a1 : ctx->B.b2;
+    A.F.L2 = ctx->B.type == 1 ? ctx->B.a1 : ctx->B.c3;
+    A.F.iL0 = 1.0 / A.F.L0;
+    A.F.iL1 = 1.0 / A.F.L1;
+    A.F.iL2 = 1.0 / A.F.L2;
+    A.nclu = C.nclu;
+    A.sx = ctx->sx;
+    A.sy = ctx->sy;
+    A.sz = ctx->sz;
+    A.sp32 = ctx->sp32;
+    A.sq = ctx->sq;
+    A.seps = ctx->seps;
+    A.ssig = ctx->ssig;
+    A.cpOff = C.cpOff;
+    A.cpJ = C.cpJ;
+    A.cpMask = C.cpMask;
+    A.fix = ctx->fix;
+    A.fiy = ctx->fiy;
+    A.fiz = ctx->fiz;
+    A.fdx = ctx->fdx;
+    A.fdy = ctx->fdy;
+    A.fdz = ctx->fdz;
+    A.part = ctx->epart + PART_PAIR;
+    const int nb = nblk(C.nclu, 4);
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
+    if (ctx->precision == MDG_MIXED)
+        hipLaunchKernelGGL(k_pair_cluster<true>, dim3(nb), dim3(256), 0, ctx->stream, A);
+    else
+        hipLaunchKernelGGL(k_pair_cluster<false>, dim3(nb), dim3(256), 0, ctx->stream, A);
+    LAUNCHCHK();
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
+    RETCHK(mdg_reduce_partials(ctx, A.part, nb, 5, ctx->scal + S_EDIR));
+    const int n = ctx->n;
+    const int nbe = nblk(n, 256);
+    double *pe = ctx->epart + PART_EXCL;
+    if (ctx->precision == MDG_MIXED)
+        hipLaunchKernelGGL(k_excl_dev<true>, dim3(nbe), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->exOff, ctx->exLst, C.atomSlot,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, pe);
+    else
+        hipLaunchKernelGGL(k_excl_dev<false>, dim3(nbe), dim3(256), 0, ctx->stream, ctx->P,
+                           ctx->B, ctx->x, ctx->y, ctx->z, ctx->q, ctx->exOff, ctx->exLst,
+                           C.atomSlot, ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz,
+                           pe);
+    LAUNCHCHK();
+    RETCHK(mdg_reduce_partials(ctx, pe, nbe, 2, ctx->scal + S_ECORR));
+    return MDG_OK;
+}

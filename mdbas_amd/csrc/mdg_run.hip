@@ -1,0 +1,474 @@
+// mdg_run.hip -- device-resident MD: the step order of mdbas.c:244-301
+// (VV stage 1, list rebuild every `update` steps, energy, VV stage 2) with
+// every array on the device and no host round trip per step.  A step is
+// captured once per variant (plain / with list rebuild) in a hipGraph and
+// replayed.
+#include "mdg_internal.h"
+
+#include <cstring>
+
+#define TWOPI_R 6.283185307179586
+
+// F = pair/correction slot forces + reciprocal forces - their mean
+// (spme.c:648-670 removes the mean of the reciprocal part only, because
+// energy() zeroes the arrays before SPME runs).  Frozen atoms get zero
+// force and velocity (utils.c:131-153).  Slot accumulators are cleared
+// for the next step.
+template <bool MIXED>
+__global__ void k_finalize(int n, const int *__restrict__ atomSlot, long long *fix, long long *fiy,
+                           long long *fiz, double *fdx, double *fdy, double *fdz,
+                           const float *__restrict__ frx, const float *__restrict__ fry,
+                           const float *__restrict__ frz, const double *__restrict__ frx64,
+                           const double *__restrict__ fry64, const double *__restrict__ frz64,
+                           const double *__restrict__ scal, const int *__restrict__ frozen,
+                           int hasFrozen, double *fx, double *fy, double *fz, double *vx,
+                           double *vy, double *vz)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = atomSlot[i];
+    const double mx = scal[S_FRX] / (double)n, my = scal[S_FRY] / (double)n,
+                 mz = scal[S_FRZ] / (double)n;
+    double gx, gy, gz;
+    if (MIXED) {
+        gx = (double)fix[s] * (1.0 / MDG_FSCALE) + ((double)frx[i] - mx);
+        gy = (double)fiy[s] * (1.0 / MDG_FSCALE) + ((double)fry[i] - my);
+        gz = (double)fiz[s] * (1.0 / MDG_FSCALE) + ((double)frz[i] - mz);
+    } else {
+        gx = (frx64[i] - mx) + fdx[s];
+        gy = (fry64[i] - my) + fdy[s];
+        gz = (frz64[i] - mz) + fdz[s];
+    }
+    if (hasFrozen && frozen[i]) {
+        gx = gy = gz = 0.;
+        vx[i] = vy[i] = vz[i] = 0.;
+    }
+    fx[i] = gx;
+    fy[i] = gy;
+    fz[i] = gz;
+}
+
+__global__ void k_clear_slots(int nslot, long long *fix, long long *fiy, long long *fiz,
+                              double *fdx, double *fdy, double *fdz, int mixed)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    if (mixed) {
+        fix[s] = 0;
+        fiy[s] = 0;
+        fiz[s] = 0;
+    } else {
+        fdx[s] = 0.;
+        fdy[s] = 0.;
+        fdz[s] = 0.;
+    }
+}
+
+__global__ void k_kinetic(int n, const double *__restrict__ mass, const double *__restrict__ vx,
+                          const double *__restrict__ vy, const double *__restrict__ vz,
+                          double *__restrict__ part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (i < n) acc[0] = mass[i] * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    mdg_block_partials<7>(acc, part);
+}
+
+struct RecArgs {
+    double fact, eSelf, eNet;
+    int recCap;
+};
+
+// One MDG_STEP record from the reduced scalars (spme.c:674-697 for the
+// reciprocal energy and virial, energy.c:623-627 and 193-196 for sums).
+__global__ void k_record(RecArgs R, const double *__restrict__ scal, int *ctrl, MDG_STEP *rec,
+                         int stepNo, int rebuilt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int idx = ctrl[CTRL_STEP];
+    const double er = scal[S_EREC];
+    const double st0 = R.fact * (scal[S_EREC + 1] + er) + R.eNet;
+    const double st3 = R.fact * (scal[S_EREC + 4] + er) + R.eNet;
+    const double st5 = R.fact * (scal[S_EREC + 6] + er) + R.eNet;
+    const double eRec = R.fact * er + R.eSelf + R.eNet;
+    const double virRec = -(st0 + st3 + st5);
+    MDG_STEP s;
+    s.step = stepNo + idx;
+    s.listRebuilt = rebuilt;
+    s.eDir = scal[S_EDIR];
+    s.eRec = eRec;
+    s.eCorr = scal[S_ECORR];
+    s.elec = s.eDir + eRec + s.eCorr;
+    s.vdw = scal[S_EVDW];
+    s.pot = s.elec + s.vdw;
+    s.kin = 0.5 * scal[S_KIN];
+    s.tot = s.pot + s.kin;
+    s.virelec = scal[S_VIRDIR] + virRec + scal[S_VIRCORR];
+    s.virvdw = scal[S_VIRVDW];
+    s.virshake = scal[S_VIRSHAKE];
+    s.nInCut = scal[S_NIN];
+    if (idx < R.recCap) rec[idx] = s;
+    ctrl[CTRL_STEP] = idx + 1;
+}
+
+static int ensure_slots(mdg_ctx *ctx)
+{
+    if (ctx->sx) return MDG_OK;
+    const int ns = ctx->cl.nclu * MDG_CL + 64;
+    HIPCHK(hipMalloc(&ctx->sx, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->sy, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->sz, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->sp32, ns * sizeof(float4)));
+    HIPCHK(hipMalloc(&ctx->sq, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->seps, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->ssig, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->fdx, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->fdy, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->fdz, ns * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->fix, ns * sizeof(long long)));
+    HIPCHK(hipMalloc(&ctx->fiy, ns * sizeof(long long)));
+    HIPCHK(hipMalloc(&ctx->fiz, ns * sizeof(long long)));
+    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, ctx->stream, ns, ctx->fix,
+                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, 1);
+    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, ctx->stream, ns, ctx->fix,
+                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, 0);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+static int ensure_records(mdg_ctx *ctx, int n)
+{
+    if (ctx->recCap >= n) return MDG_OK;
+    if (ctx->recDev) HIPCHK(hipFree(ctx->recDev));
+    ctx->recCap = std::max(n, 64);
+    HIPCHK(hipMalloc(&ctx->recDev, ctx->recCap * sizeof(MDG_STEP)));
+    // the record kernel holds recCap by value: recapture the graphs
+    for (int k = 0; k < 2; k++)
+        if (ctx->graphExec[k]) {
+            hipGraphExecDestroy(ctx->graphExec[k]);
+            ctx->graphExec[k] = nullptr;
+        }
+    return MDG_OK;
+}
+
+// Forces of the current positions on the device (energy() of
+// energy.c:103-205), optionally rebuilding the cluster list first.
+static int dev_energy(mdg_ctx *ctx, bool rebuild)
+{
+    const int n = ctx->n;
+    if (rebuild) {
+        if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
+        RETCHK(mdg_build_cluster_list(ctx));
+        RETCHK(ensure_slots(ctx));
+        RETCHK(mdg_slot_params(ctx));
+        if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
+    }
+    RETCHK(mdg_spme_device(ctx));
+    RETCHK(mdg_pair_cluster(ctx));
+    const bool mixed = ctx->precision == MDG_MIXED;
+    if (mixed)
+        hipLaunchKernelGGL(k_finalize<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
+                           ctx->cl.atomSlot, ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy,
+                           ctx->fdz, ctx->frx, ctx->fry, ctx->frz, ctx->frx64, ctx->fry64,
+                           ctx->frz64, ctx->scal, ctx->frozen, ctx->hasFrozen ? 1 : 0, ctx->fx,
+                           ctx->fy, ctx->fz, ctx->vx, ctx->vy, ctx->vz);
+    else
+        hipLaunchKernelGGL(k_finalize<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
+                           ctx->cl.atomSlot, ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy,
+                           ctx->fdz, ctx->frx, ctx->fry, ctx->frz, ctx->frx64, ctx->fry64,
+                           ctx->frz64, ctx->scal, ctx->frozen, ctx->hasFrozen ? 1 : 0, ctx->fx,
+                           ctx->fy, ctx->fz, ctx->vx, ctx->vy, ctx->vz);
+    LAUNCHCHK();
+    const int ns = ctx->cl.nclu * MDG_CL;
+    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, ctx->stream, ns, ctx->fix,
+                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, mixed ? 1 : 0);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+static RecArgs rec_args(mdg_ctx *ctx)
+{
+    RecArgs R;
+    R.fact = (TWOPI_R / ctx->B.vol) * ctx->P.chargeConst;
+    R.eSelf = ctx->eSelf;
+    R.eNet = ctx->eNet;
+    R.recCap = ctx->recCap;
+    return R;
+}
+
+// One MD step (mdbas.c:244-301).
+static int dev_step(mdg_ctx *ctx, bool rebuild, int stepBase)
+{
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[8], ctx->stream));
+    RETCHK(mdg_vv_stage(ctx, 1, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[9], ctx->stream));
+    RETCHK(dev_energy(ctx, rebuild));
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[10], ctx->stream));
+    RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, ctx->stream, rec_args(ctx), ctx->scal,
+                       ctx->ctrl, ctx->recDev, stepBase, rebuild ? 1 : 0);
+    LAUNCHCHK();
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
+    return MDG_OK;
+}
+
+static int check_flags(mdg_ctx *ctx)
+{
+    int c[CTRL_WORDS];
+    HIPCHK(hipMemcpyAsync(c, ctx->ctrl, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (c[CTRL_SHAKE_FAIL]) {
+        ctx->err = "SHAKE did not converge within maxCycle";
+        return 311;
+    }
+    if (c[CTRL_SHAKE_ORDER]) {
+        ctx->err = "a constraint group re-diverged after converging";
+        return MDG_ESHAKE_ORDER;
+    }
+    if (c[CTRL_LIST_OVF]) {
+        ctx->err = "cluster-pair list capacity exceeded";
+        return MDG_ELIST;
+    }
+    if (c[CTRL_SEG_OVF]) {
+        ctx->err = "a cell holds more atoms than one LDS sort segment";
+        return MDG_EUNSUPPORTED;
+    }
+    return MDG_OK;
+}
+
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+extern "C" int mdg_upload_state(mdg_ctx *ctx, const double x[], const double y[],
+                                const double z[], const double vx[], const double vy[],
+                                const double vz[])
+{
+    if (!ctx || !x || !vx) return MDG_EARG;
+    const size_t b = ctx->n * sizeof(double);
+    HIPCHK(hipMemcpyAsync(ctx->x, x, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->y, y, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->z, z, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->vx, vx, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->vy, vy, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->vz, vz, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->step = 0;
+    return MDG_OK;
+}
+
+extern "C" int mdg_download_state(mdg_ctx *ctx, double x[], double y[], double z[], double vx[],
+                                  double vy[], double vz[], double fx[], double fy[], double fz[])
+{
+    if (!ctx) return MDG_EARG;
+    const size_t b = ctx->n * sizeof(double);
+    double *h[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
+    double *d[9] = {ctx->x, ctx->y, ctx->z, ctx->vx, ctx->vy, ctx->vz, ctx->fx, ctx->fy, ctx->fz};
+    for (int k = 0; k < 9; k++)
+        if (h[k]) HIPCHK(hipMemcpyAsync(h[k], d[k], b, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDG_OK;
+}
+
+extern "C" int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec)
+{
+    if (!ctx || !ctx->theta || !ctx->grpAtomOff) return MDG_EARG;
+    RETCHK(ensure_records(ctx, 64));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
+    RETCHK(dev_energy(ctx, true));
+    hipLaunchKernelGGL(k_kinetic, dim3(nblk(ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n,
+                       ctx->mass, ctx->vx, ctx->vy, ctx->vz, ctx->epart + PART_KIN);
+    LAUNCHCHK();
+    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_KIN, nblk(ctx->n, 256), 7, ctx->scal + S_KIN));
+    RETCHK(mdg_fill_f64(ctx, ctx->scal + S_VIRSHAKE, 1, 0.0));
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, ctx->stream, rec_args(ctx), ctx->scal,
+                       ctx->ctrl, ctx->recDev, ctx->step, 1);
+    LAUNCHCHK();
+    RETCHK(check_flags(ctx));
+    if (rec) HIPCHK(hipMemcpy(rec, ctx->recDev, sizeof(MDG_STEP), hipMemcpyDeviceToHost));
+    ctx->stepsSinceList = 0;
+    return MDG_OK;
+}
+
+static int capture(mdg_ctx *ctx, int variant)
+{
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int st = dev_step(ctx, variant == 1, 0);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (st) return st;
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return MDG_EHIP;
+    }
+    HIPCHK(hipGraphInstantiate(&ctx->graphExec[variant], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return MDG_OK;
+}
+
+static void accumulate_phase(mdg_ctx *ctx, const hipEvent_t *e, bool rebuild, int &nReb)
+{
+    float t, t1, t2;
+    hipEventElapsedTime(&t, e[0], e[1]);
+    ctx->phase[0] += t;
+    hipEventElapsedTime(&t, e[2], e[3]);
+    ctx->phase[1] += t;
+    hipEventElapsedTime(&t1, e[8], e[9]);
+    hipEventElapsedTime(&t2, e[10], e[11]);
+    ctx->phase[2] += t1 + t2;
+    if (rebuild) {
+        hipEventElapsedTime(&t, e[6], e[7]);
+        ctx->phase[3] += t;
+        nReb++;
+    }
+    hipEventElapsedTime(&t, e[8], e[11]);
+    ctx->phase[4] += t;
+}
+
+// use_graph: 0 plain launches, 1 one graph per step variant (replayed),
+// 2 the whole run captured as one graph.  With profiling on, HIP events
+// bracket the pair kernel, SPME, the integrator, the list rebuild and the
+// step on the context stream (in mode 2 they are nodes of the graph, one
+// set per step, read after the run; in modes 0/1 the run syncs per step).
+extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *records)
+{
+    if (!ctx || nsteps < 0) return MDG_EARG;
+    if (!ctx->cl.cpJ) {
+        ctx->err = "call mdg_initial_energy first";
+        return MDG_EARG;
+    }
+    RETCHK(ensure_records(ctx, nsteps));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
+    LAUNCHCHK();
+    const bool prof = ctx->profiling;
+    for (int k = 0; k < 5; k++) ctx->phase[k] = 0.;
+    const int stepBase = ctx->step + 1;
+    int nReb = 0;
+    if (use_graph == 2) {
+        std::vector<hipEvent_t> evs;
+        if (prof) {
+            evs.resize((size_t)12 * nsteps);
+            for (auto &e : evs) HIPCHK(hipEventCreate(&e));
+        }
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int st = MDG_OK;
+        for (int s = 0; s < nsteps && st == MDG_OK; s++) {
+            if (prof) ctx->evCur = &evs[(size_t)12 * s];
+            st = dev_step(ctx, ((ctx->step + 1 + s) % ctx->P.update) == 0, stepBase);
+        }
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->evCur = ctx->ev;
+        if (st) return st;
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+            return MDG_EHIP;
+        }
+        hipGraphExec_t ge;
+        HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        HIPCHK(hipGraphLaunch(ge, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipGraphExecDestroy(ge));
+        for (int s = 0; s < nsteps; s++) {
+            const int step = ctx->step + 1;
+            if (prof) accumulate_phase(ctx, &evs[(size_t)12 * s], (step % ctx->P.update) == 0, nReb);
+            ctx->step = step;
+        }
+        for (auto &ev : evs) hipEventDestroy(ev);
+    } else {
+        for (int s = 0; s < nsteps; s++) {
+            const int step = ctx->step + 1;
+            const bool rebuild = (step % ctx->P.update) == 0;
+            if (use_graph && !prof) {
+                const int v = rebuild ? 1 : 0;
+                if (!ctx->graphExec[v]) RETCHK(capture(ctx, v));
+                HIPCHK(hipGraphLaunch(ctx->graphExec[v], ctx->stream));
+            } else {
+                ctx->evCur = ctx->ev;
+                RETCHK(dev_step(ctx, rebuild, stepBase));
+                if (prof) {
+                    HIPCHK(hipEventSynchronize(ctx->ev[11]));
+                    accumulate_phase(ctx, ctx->ev, rebuild, nReb);
+                }
+            }
+            ctx->step = step;
+        }
+    }
+    if (prof && nsteps > 0) {
+        for (int k = 0; k < 5; k++)
+            if (k != 3) ctx->phase[k] /= nsteps;
+        ctx->phase[3] = nReb ? ctx->phase[3] / nReb : 0.;
+    }
+    RETCHK(check_flags(ctx));
+    if (records && nsteps > 0) {
+        HIPCHK(hipMemcpy(records, ctx->recDev, nsteps * sizeof(MDG_STEP), hipMemcpyDeviceToHost));
+        for (int s = 0; s < nsteps; s++) records[s].step = stepBase + s;
+    }
+    return MDG_OK;
+}
+
+extern "C" int mdg_set_profiling(mdg_ctx *ctx, int on)
+{
+    if (!ctx) return MDG_EARG;
+    ctx->profiling = on != 0;
+    if (ctx->profiling)
+        for (int k = 0; k < 12; k++)
+            if (!ctx->ev[k]) HIPCHK(hipEventCreate(&ctx->ev[k]));
+    return MDG_OK;
+}
+
+extern "C" int mdg_phase_times(mdg_ctx *ctx, double out[5])
+{
+    if (!ctx || !out) return MDG_EARG;
+    for (int k = 0; k < 5; k++) out[k] = ctx->phase[k];
+    return MDG_OK;
+}
+
+extern "C" int mdg_list_stats(mdg_ctx *ctx, long out[3])
+{
+    if (!ctx || !out || !ctx->cl.cpOff) return MDG_EARG;
+    const int nclu = ctx->cl.nclu;
+    int ncp = 0;
+    HIPCHK(hipMemcpy(&ncp, ctx->cl.cpOff + nclu, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> m(ncp);
+    if (ncp) HIPCHK(hipMemcpy(m.data(), ctx->cl.cpMask, ncp * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost));
+    long bits = 0;
+    for (auto v : m) bits += __builtin_popcountll(v);
+    out[0] = ncp;
+    out[1] = bits;
+    out[2] = nclu;
+    return MDG_OK;
+}
+
+extern "C" int mdg_list_pairs(mdg_ctx *ctx, int *pi, int *pj, long cap, long *npairs)
+{
+    if (!ctx || !npairs || !ctx->cl.cpOff) return MDG_EARG;
+    const int nclu = ctx->cl.nclu;
+    int ncp = 0;
+    HIPCHK(hipMemcpy(&ncp, ctx->cl.cpOff + nclu, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> I(ncp), J(ncp), slot(nclu * MDG_CL);
+    std::vector<unsigned long long> m(ncp);
+    if (ncp) {
+        HIPCHK(hipMemcpy(I.data(), ctx->cl.cpI, ncp * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(J.data(), ctx->cl.cpJ, ncp * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(m.data(), ctx->cl.cpMask, ncp * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+    }
+    HIPCHK(hipMemcpy(slot.data(), ctx->cl.slotAtom, slot.size() * sizeof(int), hipMemcpyDeviceToHost));
+    long k = 0;
+    for (int e = 0; e < ncp; e++)
+        for (int bit = 0; bit < 64; bit++)
+            if ((m[e] >> bit) & 1ull) {
+                int a = slot[I[e] * MDG_CL + (bit >> 3)], b = slot[J[e] * MDG_CL + (bit & 7)];
+                if (k < cap && pi && pj) {
+                    pi[k] = std::min(a, b);
+                    pj[k] = std::max(a, b);
+                }
+                k++;
+            }
+    *npairs = k;
+    return MDG_OK;
+}

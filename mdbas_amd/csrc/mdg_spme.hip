@@ -1,0 +1,497 @@
+// mdg_spme.hip -- smooth particle-mesh Ewald (spme.c:357-699) on the GPU.
+//
+// Per step: scaled coordinates and order-p cardinal B-splines per atom,
+// charge spreading onto the m1 x m2 x m3 mesh (row-major, index 3
+// fastest, as spme.c:449-491), a real-to-complex 3-D FFT (rocFFT), the
+// influence function theta(k) = |b1 b2 b3|^2 exp(-|k|^2/4a^2)/|k|^2
+// applied on the half spectrum with the energy and stress sums taken
+// there in FP64, a complex-to-real FFT back, and the force gather.
+// The reference runs full complex FFTs with the opposite exponent sign
+// first (FFTW_BACKWARD then FORWARD); for a real mesh and an even theta
+// the convolution is the same real field.
+//
+// FP64 mode spreads with FP64 atomics.  MIXED mode evaluates splines in
+// FP32, spreads q*M1*M2*M3 as int64 fixed point (2^40 steps, so the mesh
+// is bitwise reproducible), runs the FFTs in FP32 and gathers in FP32.
+#include "mdg_internal.h"
+
+#include <complex>
+#include <cfloat>
+
+#define TWOPI 6.283185307179586     // global.h:53
+#define PI_ 3.141592653589793
+#define SQRTPI_ 1.772453850905516
+#define MAXP 12
+
+// spme.c:269-355 for one coordinate in the reference's operation order.
+template <typename T>
+__device__ inline void bspline_dev(double s, int p, T *bsp, T *bsd)
+{
+#pragma clang fp contract(off)
+    const int is = (int)s;
+    const T u = (T)(s - (double)is);
+    bsd[0] = (T)1;
+    bsd[1] = (T)-1;
+    bsp[0] = u;
+    bsp[1] = (T)1 - u;
+    for (int k = 2; k < p; k++) {
+        bsp[k] = (T)0;
+        for (int j = k; j > 0; j--) {
+            if (k == p - 1) bsd[j] = bsp[j] - bsp[j - 1];
+            T ts;
+            if (sizeof(T) == 8) ts = (T)(s + (double)j - (double)is);
+            else ts = u + (T)j;
+            bsp[j] = (ts * bsp[j] + ((T)(k + 1) - ts) * bsp[j - 1]) / (T)k;
+        }
+        if (k == p - 1) bsd[0] = bsp[0];
+        bsp[0] = u * bsp[0] / (T)k;
+    }
+}
+
+__device__ inline void scaled_coords(const MdgBox &B, int m1, int m2, int m3, double x, double y,
+                                     double z, double &s1, double &s2, double &s3)
+{
+#pragma clang fp contract(off)
+    // spme.c:434-439
+    s1 = (double)m1 * (x * B.u1 + y * B.u2 + z * B.u3 + 0.5);
+    s2 = (double)m2 * (x * B.v1 + y * B.v2 + z * B.v3 + 0.5);
+    s3 = (double)m3 * (x * B.w1 + y * B.w2 + z * B.w3 + 0.5);
+}
+
+__device__ inline int wrap_index(int ii, int m)
+{
+    // spme.c:453-459
+    if (ii >= m) ii = 0;
+    if (ii < 0) ii += m;
+    // the reference rule covers atoms within one box length of the cell;
+    // anything farther is folded in rather than read out of bounds
+    if (ii < 0 || ii >= m) ii = ((ii % m) + m) % m;
+    return ii;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256)
+k_spme_spread(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
+              const double *__restrict__ y, const double *__restrict__ z,
+              const double *__restrict__ q, double *grid64, long long *gridq)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int p = P.nbsp, m1 = P.m1, m2 = P.m2, m3 = P.m3;
+    double s1, s2, s3;
+    scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+    typedef typename std::conditional<MIXED, float, double>::type T;
+    T b1[MAXP], b2[MAXP], b3[MAXP], d[MAXP];
+    bspline_dev<T>(s1, p, b1, d);
+    bspline_dev<T>(s2, p, b2, d);
+    bspline_dev<T>(s3, p, b3, d);
+    const T ql = (T)q[l];
+    const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+    for (int i = 0; i < p; i++) {
+        const int ii = wrap_index(i1 - i, m1);
+        const int o1 = m2 * ii;
+        const T qa = ql * b1[i];
+        for (int j = 0; j < p; j++) {
+            const int jj = wrap_index(i2 - j, m2);
+            const long o2 = (long)m3 * (jj + o1);
+            const T qb = qa * b2[j];
+            for (int k = 0; k < p; k++) {
+                const int kk = wrap_index(i3 - k, m3);
+                if (MIXED) {
+                    long long v = __double2ll_rn((double)(qb * b3[k]) * MDG_QSCALE);
+                    atomicAdd((unsigned long long *)&gridq[kk + o2], (unsigned long long)v);
+                } else {
+                    atomicAdd(&grid64[kk + o2], (double)qb * (double)b3[k]);
+                }
+            }
+        }
+    }
+}
+
+// fixed-point mesh -> FP32 mesh, clearing the fixed-point mesh for the next
+// step (no separate memset).
+__global__ void k_grid_fix2f(long G, long long *__restrict__ gq, float *__restrict__ g32)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < G; i += (long)gridDim.x * blockDim.x) {
+        g32[i] = (float)((double)gq[i] * (1.0 / MDG_QSCALE));
+        gq[i] = 0;
+    }
+}
+
+// Influence function on the half spectrum, energy and stress partials.
+// Per block: [sum theta|F|^2, st0..st5] with full-spectrum weights.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_spme_influence(MdgBox B, int m1, int m2, int m3, int mh, double rAlpha2,
+                 const T *__restrict__ theta, T *cg, double *__restrict__ part)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const long G = (long)m1 * m2 * mh;
+    if (g < G) {
+        const int k = (int)(g % mh);
+        const int j = (int)((g / mh) % m2);
+        const int i = (int)(g / ((long)mh * m2));
+        const T th = theta[g];
+        const T re = cg[2 * g], im = cg[2 * g + 1];
+        if (th != (T)0) {
+            const double w = (k == 0 || (2 * k == m3)) ? 1.0 : 2.0;
+            const double e = (double)th * ((double)re * (double)re + (double)im * (double)im);
+            // spme.c:509-547 reciprocal vector for (i, j, k)
+            const int ii = i > m1 / 2 ? i - m1 : i;
+            const int jj = j > m2 / 2 ? j - m2 : j;
+            const int kk = k > m3 / 2 ? k - m3 : k;
+            const double t1 = TWOPI * ii, t2 = TWOPI * jj, t3 = TWOPI * kk;
+            const double rx = t1 * B.u1 + t2 * B.v1 + t3 * B.w1;
+            const double ry = t1 * B.u2 + t2 * B.v2 + t3 * B.w2;
+            const double rz = t1 * B.u3 + t2 * B.v3 + t3 * B.w3;
+            const double rm = rx * rx + ry * ry + rz * rz;
+            const double vam = 2.0 * (1.0 / rm - rAlpha2) * e;
+            acc[0] = w * e;
+            acc[1] = -w * vam * rx * rx;
+            acc[2] = -w * vam * rx * ry;
+            acc[3] = -w * vam * rx * rz;
+            acc[4] = -w * vam * ry * ry;
+            acc[5] = -w * vam * ry * rz;
+            acc[6] = -w * vam * rz * rz;
+        }
+        cg[2 * g] = re * th;
+        cg[2 * g + 1] = im * th;
+    }
+    mdg_block_partials<7>(acc, part);
+}
+
+// spme.c:593-646: force gather; writes the reciprocal force per atom and
+// per-block sums of it (for the mean removal of spme.c:648-670).
+template <bool MIXED, typename TG, typename TF>
+__global__ void __launch_bounds__(256)
+k_spme_gather(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
+              const double *__restrict__ y, const double *__restrict__ z,
+              const double *__restrict__ q, const TG *__restrict__ grid, double fact1,
+              TF *__restrict__ frx, TF *__restrict__ fry, TF *__restrict__ frz,
+              double *__restrict__ part)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[3] = {0, 0, 0};
+    if (l < n) {
+        const int p = P.nbsp, m1 = P.m1, m2 = P.m2, m3 = P.m3;
+        double s1, s2, s3;
+        scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+        typedef typename std::conditional<MIXED, float, double>::type T;
+        T b1[MAXP], b2[MAXP], b3[MAXP], d1[MAXP], d2[MAXP], d3[MAXP];
+        bspline_dev<T>(s1, p, b1, d1);
+        bspline_dev<T>(s2, p, b2, d2);
+        bspline_dev<T>(s3, p, b3, d3);
+        const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+        T fbx = 0, fby = 0, fbz = 0;
+        for (int i = 0; i < p; i++) {
+            const int ii = wrap_index(i1 - i, m1);
+            const int o1 = m2 * ii;
+            for (int j = 0; j < p; j++) {
+                const int jj = wrap_index(i2 - j, m2);
+                const long o2 = (long)m3 * (jj + o1);
+                const T c1 = d1[i] * b2[j], c2 = b1[i] * d2[j], c3 = b1[i] * b2[j];
+                for (int k = 0; k < p; k++) {
+                    const int kk = wrap_index(i3 - k, m3);
+                    const T qt = (T)grid[kk + o2];
+                    fbx += qt * c1 * b3[k];
+                    fby += qt * c2 * b3[k];
+                    fbz += qt * c3 * d3[k];
+                }
+            }
+        }
+        const double gx = (double)fbx * m1, gy = (double)fby * m2, gz = (double)fbz * m3;
+        const double c = fact1 * q[l];
+        const double fxl = c * (gx * B.u1 + gy * B.v1 + gz * B.w1);
+        const double fyl = c * (gx * B.u2 + gy * B.v2 + gz * B.w2);
+        const double fzl = c * (gx * B.u3 + gy * B.v3 + gz * B.w3);
+        frx[l] = (TF)fxl;
+        fry[l] = (TF)fyl;
+        frz[l] = (TF)fzl;
+        acc[0] = (double)(TF)fxl;
+        acc[1] = (double)(TF)fyl;
+        acc[2] = (double)(TF)fzl;
+    }
+    mdg_block_partials<3>(acc, part);
+}
+
+// ---------------------------------------------------------------- host side
+
+// spme.c:170-267: |bspc(i)|^2 moduli.
+static void bsp_moduli(int m, int p, std::vector<double> &mod2)
+{
+    std::vector<double> bsp(p + 1, 0.0);
+    bsp[0] = 0.0;
+    bsp[1] = 1.0;
+    for (int k = 2; k < p; k++) {
+        bsp[k] = 0.0;
+        for (int j = k; j > 0; j--)
+            bsp[j] = ((double)j * bsp[j] + (double)(k + 1 - j) * bsp[j - 1]) / ((double)k);
+    }
+    std::vector<std::complex<double>> epl(m);
+    epl[0] = std::complex<double>(1.0, 0.0);
+    for (int i = 1; i <= m / 2; i++) {
+        double ang = TWOPI * (double)i / (double)m;
+        epl[i] = std::complex<double>(cos(ang), sin(ang));
+        epl[m - i] = std::conj(epl[i]);
+    }
+    mod2.assign(m, 0.0);
+    for (int i = 0; i < m; i++) {
+        std::complex<double> coeff(0.0, 0.0);
+        for (int k = 0; k < p - 1; k++) coeff += bsp[k + 1] * epl[((long)i * k) % m];
+        std::complex<double> c = epl[((long)i * (p - 1)) % m] / coeff;
+        mod2[i] = std::real(c * std::conj(c));
+    }
+}
+
+static int make_plans(mdg_ctx *ctx, bool dbl, rocfft_plan *fwd, rocfft_plan *bwd)
+{
+    size_t len[3] = {(size_t)ctx->m3, (size_t)ctx->m2, (size_t)ctx->m1};
+    rocfft_precision prec = dbl ? rocfft_precision_double : rocfft_precision_single;
+    if (rocfft_plan_create(fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                           prec, 3, len, 1, nullptr) != rocfft_status_success ||
+        rocfft_plan_create(bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                           prec, 3, len, 1, nullptr) != rocfft_status_success) {
+        ctx->err = "rocfft_plan_create failed";
+        return MDG_EHIP;
+    }
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(*fwd, &w1);
+    rocfft_plan_get_work_buffer_size(*bwd, &w2);
+    size_t w = std::max(w1, w2);
+    if (w > ctx->fftWorkSize) {
+        if (ctx->fftWork) HIPCHK(hipFree(ctx->fftWork));
+        HIPCHK(hipMalloc(&ctx->fftWork, w));
+        ctx->fftWorkSize = w;
+        rocfft_execution_info_set_work_buffer(ctx->info, ctx->fftWork, ctx->fftWorkSize);
+    }
+    return MDG_OK;
+}
+
+int mdg_spme_setup(mdg_ctx *ctx)
+{
+    static bool s_setup = false;
+    if (!s_setup) {
+        if (rocfft_setup() != rocfft_status_success) {
+            ctx->err = "rocfft_setup failed";
+            return MDG_EHIP;
+        }
+        s_setup = true;
+    }
+    const int m1 = ctx->m1, m2 = ctx->m2, m3 = ctx->m3, p = ctx->P.nbsp;
+    if (p < 2 || p > MAXP) {
+        ctx->err = "spline order outside 2..12";
+        return MDG_EUNSUPPORTED;
+    }
+    ctx->mh = m3 / 2 + 1;
+    const long G = (long)m1 * m2 * m3, H = (long)m1 * m2 * ctx->mh;
+    std::vector<double> b1, b2, b3;
+    bsp_moduli(m1, p, b1);
+    bsp_moduli(m2, p, b2);
+    bsp_moduli(m3, p, b3);
+    const MdgBox &B = ctx->B;
+    double rAlpha2 = -0.25 / (ctx->P.alpha * ctx->P.alpha);
+    double rc = std::min((double)m1 * B.u, (double)m2 * B.v);
+    rc = std::min(rc, (double)m3 * B.w);
+    rc = rc * 1.05 * TWOPI;
+    const double rc2 = rc * rc;
+    std::vector<double> th(H);
+    std::vector<float> thf(H);
+    for (int i = 0; i < m1; i++) {
+        int ii = i > m1 / 2 ? i - m1 : i;
+        double t1 = TWOPI * (double)ii;
+        for (int j = 0; j < m2; j++) {
+            int jj = j > m2 / 2 ? j - m2 : j;
+            double t2 = TWOPI * (double)jj;
+            for (int k = 0; k < ctx->mh; k++) {
+                int kk = k > m3 / 2 ? k - m3 : k;
+                double t3 = TWOPI * (double)kk;
+                double rx = t1 * B.u1 + t2 * B.v1 + t3 * B.w1;
+                double ry = t1 * B.u2 + t2 * B.v2 + t3 * B.w2;
+                double rz = t1 * B.u3 + t2 * B.v3 + t3 * B.w3;
+                double rm = rx * rx + ry * ry + rz * rz;
+                double v = 0.0;
+                if (rm > DBL_EPSILON && rm <= rc2) v = b1[i] * b2[j] * b3[k] * exp(rAlpha2 * rm) / rm;
+                long g = ((long)i * m2 + j) * ctx->mh + k;
+                th[g] = v;
+                thf[g] = (float)v;
+            }
+        }
+    }
+    HIPCHK(hipMalloc(&ctx->theta, H * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->theta32, H * sizeof(float)));
+    HIPCHK(hipMemcpy(ctx->theta, th.data(), H * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->theta32, thf.data(), H * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&ctx->grid64, G * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->cgrid64, 2 * H * sizeof(double)));
+    if (rocfft_execution_info_create(&ctx->info) != rocfft_status_success) {
+        ctx->err = "rocfft_execution_info_create failed";
+        return MDG_EHIP;
+    }
+    rocfft_execution_info_set_stream(ctx->info, ctx->stream);
+    RETCHK(make_plans(ctx, true, &ctx->fwd, &ctx->bwd));
+    if (ctx->precision == MDG_MIXED) {
+        HIPCHK(hipMalloc(&ctx->grid32, G * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->cgrid32, 2 * H * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->gridq, G * sizeof(long long)));
+        RETCHK(mdg_fill_i64(ctx, ctx->gridq, G));
+        RETCHK(make_plans(ctx, false, &ctx->fwd32, &ctx->bwd32));
+    }
+    return MDG_OK;
+}
+
+int mdg_spme_free(mdg_ctx *ctx)
+{
+    if (ctx->fwd) rocfft_plan_destroy(ctx->fwd);
+    if (ctx->bwd) rocfft_plan_destroy(ctx->bwd);
+    if (ctx->fwd32) rocfft_plan_destroy(ctx->fwd32);
+    if (ctx->bwd32) rocfft_plan_destroy(ctx->bwd32);
+    if (ctx->info) rocfft_execution_info_destroy(ctx->info);
+    ctx->fwd = ctx->bwd = ctx->fwd32 = ctx->bwd32 = nullptr;
+    ctx->info = nullptr;
+    return MDG_OK;
+}
+
+static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGather, void *frx,
+                    void *fry, void *frz)
+{
+    const int n = ctx->n, m1 = ctx->m1, m2 = ctx->m2, m3 = ctx->m3, mh = ctx->mh;
+    const long G = (long)m1 * m2 * m3, H = (long)m1 * m2 * mh;
+    const double rAlpha2 = -0.25 / (ctx->P.alpha * ctx->P.alpha);
+    const double fact1 = -2.0 * (TWOPI / ctx->B.vol) * ctx->P.chargeConst;
+    if (mixed) {
+        hipLaunchKernelGGL(k_spme_spread<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
+                           ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, nullptr, ctx->gridq);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(G, 256), 8192)), dim3(256),
+                           0, ctx->stream, G, ctx->gridq, ctx->grid32);
+        LAUNCHCHK();
+        void *in[1] = {ctx->grid32}, *out[1] = {ctx->cgrid32};
+        if (rocfft_execute(ctx->fwd32, in, out, ctx->info) != rocfft_status_success) {
+            ctx->err = "rocfft_execute (forward) failed";
+            return MDG_EHIP;
+        }
+        hipLaunchKernelGGL(k_spme_influence<float>, dim3(nblk(H, 256)), dim3(256), 0, ctx->stream,
+                           ctx->B, m1, m2, m3, mh, rAlpha2, ctx->theta32, ctx->cgrid32, partInfl);
+        LAUNCHCHK();
+        void *in2[1] = {ctx->cgrid32}, *out2[1] = {ctx->grid32};
+        if (rocfft_execute(ctx->bwd32, in2, out2, ctx->info) != rocfft_status_success) {
+            ctx->err = "rocfft_execute (inverse) failed";
+            return MDG_EHIP;
+        }
+        hipLaunchKernelGGL((k_spme_gather<true, float, float>), dim3(nblk(n, 256)), dim3(256), 0,
+                           ctx->stream, ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q,
+                           ctx->grid32, fact1, (float *)frx, (float *)fry, (float *)frz,
+                           partGather);
+        LAUNCHCHK();
+    } else {
+        RETCHK(mdg_fill_f64(ctx, ctx->grid64, G, 0.0));
+        hipLaunchKernelGGL(k_spme_spread<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
+                           ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, ctx->grid64, nullptr);
+        LAUNCHCHK();
+        void *in[1] = {ctx->grid64}, *out[1] = {ctx->cgrid64};
+        if (rocfft_execute(ctx->fwd, in, out, ctx->info) != rocfft_status_success) {
+            ctx->err = "rocfft_execute (forward) failed";
+            return MDG_EHIP;
+        }
+        hipLaunchKernelGGL(k_spme_influence<double>, dim3(nblk(H, 256)), dim3(256), 0,
+                           ctx->stream, ctx->B, m1, m2, m3, mh, rAlpha2, ctx->theta, ctx->cgrid64,
+                           partInfl);
+        LAUNCHCHK();
+        void *in2[1] = {ctx->cgrid64}, *out2[1] = {ctx->grid64};
+        if (rocfft_execute(ctx->bwd, in2, out2, ctx->info) != rocfft_status_success) {
+            ctx->err = "rocfft_execute (inverse) failed";
+            return MDG_EHIP;
+        }
+        hipLaunchKernelGGL((k_spme_gather<false, double, double>), dim3(nblk(n, 256)), dim3(256),
+                           0, ctx->stream, ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q,
+                           ctx->grid64, fact1, (double *)frx, (double *)fry, (double *)frz,
+                           partGather);
+        LAUNCHCHK();
+    }
+    return MDG_OK;
+}
+
+// SPME energy pieces into ctx->scal: S_EREC.. (sum theta|F|^2, st0..5),
+// S_FRX.. (sums of the reciprocal forces).  Reciprocal forces per atom go
+// to frx64.. (FP64 mode) or frx.. (MIXED).
+int mdg_spme_device(mdg_ctx *ctx)
+{
+    const int n = ctx->n;
+    const long H = (long)ctx->m1 * ctx->m2 * ctx->mh;
+    const bool mixed = ctx->precision == MDG_MIXED;
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[2], ctx->stream));
+    if (mixed)
+        RETCHK(spme_run(ctx, true, ctx->epart + PART_INFL, ctx->epart + PART_GATHER, ctx->frx,
+                        ctx->fry, ctx->frz));
+    else
+        RETCHK(spme_run(ctx, false, ctx->epart + PART_INFL, ctx->epart + PART_GATHER, ctx->frx64,
+                        ctx->fry64, ctx->frz64));
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[3], ctx->stream));
+    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_INFL, nblk(H, 256), 7, ctx->scal + S_EREC));
+    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_GATHER, nblk(n, 256), 3, ctx->scal + S_FRX));
+    return MDG_OK;
+}
+
+// Drop-in spme_energy: FP64 always.  f (device, 3 arrays) += reciprocal
+// forces, then the mean of the whole arrays is removed.  out[0..7]:
+// energy, virial, stress[6].
+__global__ void k_add_rec(int n, double *fx, double *fy, double *fz, const double *frx,
+                          const double *fry, const double *frz, double *part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[3] = {0, 0, 0};
+    if (i < n) {
+        fx[i] += frx[i];
+        fy[i] += fry[i];
+        fz[i] += frz[i];
+        acc[0] = fx[i];
+        acc[1] = fy[i];
+        acc[2] = fz[i];
+    }
+    mdg_block_partials<3>(acc, part);
+}
+
+__global__ void k_sub_mean(int n, double *fx, double *fy, double *fz, const double *sums)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fx[i] -= sums[0] / (double)n;
+    fy[i] -= sums[1] / (double)n;
+    fz[i] -= sums[2] / (double)n;
+}
+
+int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8])
+{
+    const int n = ctx->n;
+    const long H = (long)ctx->m1 * ctx->m2 * ctx->mh;
+    RETCHK(spme_run(ctx, false, ctx->epart + PART_INFL, ctx->epart + PART_GATHER, ctx->frx64,
+                    ctx->fry64, ctx->frz64));
+    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_INFL, nblk(H, 256), 7, ctx->scal + S_EREC));
+    hipLaunchKernelGGL(k_add_rec, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, fx, fy, fz,
+                       ctx->frx64, ctx->fry64, ctx->frz64, ctx->epart + PART_MISC);
+    LAUNCHCHK();
+    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_MISC, nblk(n, 256), 3, ctx->scal + S_FRX));
+    hipLaunchKernelGGL(k_sub_mean, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, fx, fy, fz,
+                       ctx->scal + S_FRX);
+    LAUNCHCHK();
+    double s[7];
+    HIPCHK(hipMemcpyAsync(s, ctx->scal + S_EREC, 7 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // spme.c:674-697
+    const double fact = (TWOPI / ctx->B.vol) * ctx->P.chargeConst;
+    const double eRec = s[0];
+    double st[6];
+    st[0] = fact * (s[1] + eRec) + ctx->eNet;
+    st[1] = fact * s[2];
+    st[2] = fact * s[3];
+    st[3] = fact * (s[4] + eRec) + ctx->eNet;
+    st[4] = fact * s[5];
+    st[5] = fact * (s[6] + eRec) + ctx->eNet;
+    eout[0] = fact * eRec + ctx->eSelf + ctx->eNet;
+    eout[1] = -(st[0] + st[3] + st[5]);
+    for (int k = 0; k < 6; k++) eout[2 + k] = st[k];
+    return MDG_OK;
+}

@@ -1,0 +1,219 @@
+"""GPU parity tests: the HIP library (through its C ABI) against the FP64
+oracle on the same seeded inputs.
+
+Bars: the neighbour pair set and list order are bit-exact; FP64 energies
+and forces agree to 1e-12 relative (summation order differs, nothing
+else); the mixed-precision path's RMS relative force error vs FP64 is at
+most 1e-5 (the target the mixed path is built to)."""
+import numpy as np
+import pytest
+
+import mdbas_amd as M
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def pairset(nl, np_, no, n):
+    a = np.repeat(no, np_[no]).astype(np.int64)
+    return np.sort(np.minimum(a, nl) * n + np.maximum(a, nl))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c0(c0_box):
+    o = O.OracleSystem(c0_box)
+    x, y, z = c0_box.x.copy(), c0_box.y.copy(), c0_box.z.copy()
+    o.build_list(x, y, z)
+    f, e = o.energy(x, y, z)
+    return dict(wb=c0_box, o=o, x=(x, y, z), f=f, e=e)
+
+
+# ------------------------------------------------------------------ drop-in list
+
+@pytest.mark.parametrize("which", ["c0", "link"])
+def test_makelist_bit_exact(which, c0_box, link_box):
+    wb = c0_box if which == "c0" else link_box
+    say("makelist", which, wb.n)
+    o = O.OracleSystem(wb)
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    nl, np_, no = o.build_list(x, y, z)
+    ctx = M.Context(wb)
+    assert ctx.key_link == o.keyLink == (1 if which == "link" else 0)
+    gx, gy, gz = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    gnl, gnp, gno, eoff, elst = ctx.makelist(gx, gy, gz)
+    assert np.array_equal(gnl, nl) and np.array_equal(gnp, np_) and np.array_equal(gno, no)
+    assert np.array_equal(gx, x) and np.array_equal(gy, y) and np.array_equal(gz, z)
+    assert np.array_equal(eoff, o.exOff) and np.array_equal(elst, o.exLst)
+    # update build (no wrap) after a seeded displacement
+    rng = np.random.default_rng(77)
+    for a, b in ((x, gx), (y, gy), (z, gz)):
+        d = rng.normal(0.0, 0.05, len(a))
+        a += d
+        b += d
+    nl, np_, no = o.build_list(x, y, z, build=False)
+    gnl, gnp, gno, _, _ = ctx.makelist(gx, gy, gz, step=wb.params["update"])
+    assert np.array_equal(gnl, nl) and np.array_equal(gnp, np_) and np.array_equal(gno, no)
+    ctx.close()
+
+
+# ------------------------------------------------------------------ drop-in energies
+
+def test_ewald_energy_parity(c0):
+    wb = c0["wb"]
+    ctx = M.Context(wb)
+    gx, gy, gz = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    gnl, gnp, gno, _, _ = ctx.makelist(gx, gy, gz)
+    f, e = ctx.ewald_energy(gx, gy, gz, gnl, gnp, gno)
+    eo = c0["e"]
+    say("ewald", e.elec, eo["elec"], e.vdw, eo["vdw"])
+    assert e.elec == pytest.approx(eo["elec"], rel=1e-12)
+    assert e.vdw == pytest.approx(eo["vdw"], rel=1e-11)
+    assert e.virelec == pytest.approx(eo["virelec"], rel=1e-11)
+    assert e.virvdw == pytest.approx(eo["virvdw"], rel=1e-11)
+    assert relerr(f, c0["f"]) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("which", ["c0", "small"])
+def test_spme_energy_parity(which, c0_box, small_box):
+    wb = c0_box if which == "c0" else small_box   # 32^3 and a 20^3 (not a power of two) mesh
+    o = O.OracleSystem(wb)
+    er, fr, st, vr = o.spme(wb.x.copy(), wb.y.copy(), wb.z.copy())
+    ctx = M.Context(wb)
+    eg, fg, sg, vg = ctx.spme_energy(wb.x.copy(), wb.y.copy(), wb.z.copy())
+    say("spme", which, eg, er)
+    assert eg == pytest.approx(er, rel=1e-12)
+    assert vg == pytest.approx(vr, rel=1e-10)
+    assert np.allclose(sg, st, rtol=1e-10, atol=1e-8 * np.abs(st).max())
+    assert relerr(fg, fr) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("elec", [2, 3, 4])
+def test_nonbond_energy_parity(elec, c0):
+    import ctypes as C
+    wb, o = c0["wb"], c0["o"]
+    x, y, z = c0["x"]
+    fo = np.zeros((3, wb.n))
+    vals = [C.c_double(0) for _ in range(4)]
+    O.lib().oc_nonbond(C.byref(o.p), C.byref(o.b), elec, O.P(x), O.P(y), O.P(z), O.P(fo[0]),
+                       O.P(fo[1]), O.P(fo[2]), O.P(wb.q), O.P(wb.eps), O.P(wb.sig),
+                       O.P(o.neighList), O.P(o.neighPair), O.P(o.neighOrder),
+                       *[C.byref(v) for v in vals])
+    ctx = M.Context(wb)
+    fg, eg = ctx.nonbond_energy(elec, x, y, z, o.neighList, o.neighPair, o.neighOrder)
+    assert eg.elec == pytest.approx(vals[0].value, rel=1e-12)
+    assert eg.vdw == pytest.approx(vals[1].value, rel=1e-11)
+    assert relerr(fg, fo) < 1e-12
+    ctx.close()
+
+
+def test_vv_integrate_parity(c0):
+    wb, o = c0["wb"], c0["o"]
+    s_o = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    s_g = [a.copy() for a in s_o]
+    ctx = M.Context(wb)
+    f = c0["f"]
+    err, _, vs, ncyc = o.vv(1, *s_o, f)
+    eg = ctx.vv_integrate(1, *s_g, f.copy())
+    say("vv1 shake cycles", ncyc)
+    assert err == 0
+    assert max(np.abs(a - b).max() for a, b in zip(s_o, s_g)) < 1e-12
+    assert eg.virshake == pytest.approx(vs, rel=1e-10)
+    err, ek, _, _ = o.vv(2, *s_o, f)
+    eg = ctx.vv_integrate(2, *s_g, f.copy())
+    assert max(np.abs(a - b).max() for a, b in zip(s_o, s_g)) < 1e-12
+    assert eg.kin == pytest.approx(ek, rel=1e-12)
+    ctx.close()
+
+
+# ------------------------------------------------------------------ device-resident
+
+@pytest.mark.parametrize("which", ["c0", "link"])
+def test_cluster_pair_set_bit_exact(which, c0_box, link_box):
+    wb = c0_box if which == "c0" else link_box
+    o = O.OracleSystem(wb)
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    nl, np_, no = o.build_list(x, y, z)
+    ctx = M.Context(wb)
+    ctx.upload(x, y, z, wb.vx, wb.vy, wb.vz)
+    ctx.initial_energy()
+    pi, pj = ctx.list_pairs()
+    got = np.sort(pi.astype(np.int64) * wb.n + pj)
+    want = pairset(nl, np_, no, wb.n)
+    say("cluster list", which, ctx.list_stats(), len(want))
+    assert np.array_equal(got, want)
+    ctx.close()
+
+
+def test_device_fp64_energy_and_trajectory(c0):
+    wb = c0["wb"]
+    ctx = M.Context(wb, precision=M.MDG_FP64)
+    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    r = ctx.initial_energy()
+    eo = c0["e"]
+    assert r.pot == pytest.approx(eo["pot"], rel=1e-12)
+    assert r.nInCut == eo["nInCut"]
+    a = ctx.download()
+    assert relerr(np.stack(a[6:9]), c0["f"]) < 1e-12
+    nsteps = 25          # crosses the list rebuild at step 20
+    recs = M.records_to_dict(ctx.run(nsteps, use_graph=True))
+    o = O.OracleSystem(wb)
+    s = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    err, fo, et, ep, ek, _ = o.md_run(nsteps, *s)
+    assert err == 0
+    g = ctx.download()
+    say("traj tot", recs["tot"][-1], et[-1], "pos", max(np.abs(g[k] - s[k]).max() for k in range(3)))
+    assert np.abs(recs["tot"] - et[1:]).max() < 1e-10 * abs(et[0])
+    assert np.abs(recs["pot"] - ep[1:]).max() < 1e-10 * abs(ep[0])
+    assert max(np.abs(g[k] - s[k]).max() for k in range(3)) < 1e-9
+    assert relerr(np.stack(g[6:9]), fo) < 1e-9
+    ctx.close()
+
+
+@pytest.mark.parametrize("which", ["c0", "c1"])
+def test_mixed_force_error(which, c0_box, c1_box):
+    wb = c0_box if which == "c0" else c1_box
+    out = {}
+    for prec in (M.MDG_FP64, M.MDG_MIXED):
+        ctx = M.Context(wb, precision=prec)
+        ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        r = ctx.initial_energy()
+        out[prec] = (r, np.stack(ctx.download()[6:9]))
+        ctx.close()
+    f64, fm = out[M.MDG_FP64][1], out[M.MDG_MIXED][1]
+    rms = relerr(fm, f64)
+    mx = float((np.linalg.norm(fm - f64, axis=0) / np.linalg.norm(f64, axis=0)).max())
+    say("mixed", which, "rms rel", rms, "max rel", mx)
+    assert rms <= 1e-5
+    assert out[M.MDG_MIXED][0].pot == pytest.approx(out[M.MDG_FP64][0].pot, rel=5e-6)
+
+
+def test_mixed_graph_replay_is_bitwise_deterministic(c0_box):
+    """The mixed path has no floating-point atomics: a graph-replayed run
+    and a run of plain launches from the same state agree bit for bit,
+    replay after replay (three rebuild-free graphs plus a rebuild graph)."""
+    wb = c0_box
+    res = []
+    for graph in (True, False):
+        ctx = M.Context(wb, precision=M.MDG_MIXED)
+        ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        ctx.initial_energy()
+        r1 = M.records_to_dict(ctx.run(12, use_graph=graph))
+        _ = ctx.download()               # host work between replays
+        r2 = M.records_to_dict(ctx.run(12, use_graph=graph))
+        res.append((r1, r2, ctx.download()))
+        ctx.close()
+    (a1, a2, sa), (b1, b2, sb) = res
+    for k in ("tot", "pot", "kin", "eRec"):
+        assert np.array_equal(a1[k], b1[k]) and np.array_equal(a2[k], b2[k]), k
+    for u, v in zip(sa, sb):
+        assert np.array_equal(u, v)

@@ -1,0 +1,427 @@
+"""CPU tests that pin the FP64 oracle.
+
+Three kinds of check, so that a plausible slip in the restatement (a dropped
+term, a wrong index or sign) fails at least one:
+  * golden vectors written by the reference's own compiled code
+    (tests/golden/make_golden.py), compared bit for bit where the
+    arithmetic is the same;
+  * the compiled reference itself (oracle/_ref), when present, on more
+    inputs;
+  * mathematics: brute force, closed forms, identities, convergence to
+    classic Ewald, finite differences.
+"""
+import ctypes as C
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mdbas_amd import configs
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference tree not built here")
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+@pytest.fixture(scope="module")
+def gf():
+    return np.load(os.path.join(GOLD, "functors.npz"))
+
+
+@pytest.fixture(scope="module")
+def gl():
+    return np.load(os.path.join(GOLD, "lists.npz"))
+
+
+@pytest.fixture(scope="module")
+def gd():
+    return np.load(os.path.join(GOLD, "dynamics.npz"))
+
+
+def _p(cfg, n):
+    return O.make_param(cfg, n)
+
+
+# ------------------------------------------------------------------ functors
+
+def test_functors_match_reference_golden(gf, c0_box):
+    L = O.lib()
+    p = _p(configs.CONFIGS["c0"], c0_box.n)
+    alpha, qel = float(gf["alpha"]), float(gf["qel"])
+    veps, vsig = gf["vdw_par"]
+    for k, r in enumerate(gf["r"]):
+        d = C.c_double(0)
+        e = L.oc_ewald_dir(C.c_double(alpha), C.c_double(qel), C.c_double(r), C.c_double(1 / r), C.byref(d))
+        assert e == gf["dir_E"][k] and d.value == gf["dir_dE"][k]
+        e = L.oc_ewald_corr(C.c_double(alpha), C.c_double(qel), C.c_double(r), C.c_double(1 / r), C.byref(d))
+        assert e == gf["corr_E"][k] and d.value == gf["corr_dE"][k]
+        e = L.oc_vdw_switch(C.byref(p), C.c_double(veps), C.c_double(vsig), C.c_double(r * r),
+                            C.c_double(1 / r), C.byref(d))
+        assert e == gf["vdw_E"][k] and d.value == gf["vdw_dE"][k]
+
+
+def test_nint_and_box_match_reference(gf):
+    L = O.lib()
+    L.oc_nint.argtypes = [C.c_double]
+    for x, v in zip(gf["nint_x"], gf["nint_v"]):
+        assert L.oc_nint(x) == v
+    for b9, out in zip(gf["boxes"], gf["box_out"]):
+        b = O.OBox()
+        b.type = 1
+        (b.a1, b.a2, b.a3, b.b1, b.b2, b.b3, b.c1, b.c2, b.c3) = [float(v) for v in b9]
+        L.oc_init_box(C.byref(b))
+        got = [b.u1, b.u2, b.u3, b.v1, b.v2, b.v3, b.w1, b.w2, b.w3, b.pa, b.pb, b.pc, b.u, b.v,
+               b.w, b.det, b.vol]
+        assert np.array_equal(np.array(got), out[:17])
+
+
+def test_switch_closed_forms(c0_box):
+    """S(ron) = 1, S(rc) = 0 and dE/dr -> 0 at rc; E(r) continuous at ron."""
+    L = O.lib()
+    p = _p(configs.CONFIGS["c0"], c0_box.n)
+    veps, vsig = c0_box.eps[0] ** 2, 2 * c0_box.sig[0]
+    d = C.c_double(0)
+
+    def vdw(r):
+        e = L.oc_vdw_switch(C.byref(p), C.c_double(veps), C.c_double(vsig), C.c_double(r * r),
+                            C.c_double(1 / r), C.byref(d))
+        return e, d.value
+    e_in, d_in = vdw(10.0 - 1e-9)
+    e_out, _ = vdw(10.0 + 1e-9)
+    assert abs(e_in - e_out) < 1.01 * abs(d_in) * 2e-9 + 1e-14   # continuous at ron
+    e_rc, de_rc = vdw(12.0)
+    assert e_rc == 0.0 and de_rc == 0.0
+    lj = 4 * veps * ((vsig / 10.0) ** 12 - (vsig / 10.0) ** 6)
+    assert vdw(10.0)[0] == pytest.approx(lj, rel=1e-14)
+
+
+@pytest.mark.parametrize("which", ["dir", "corr", "vdw"])
+def test_functor_derivative_by_finite_differences(which, c0_box):
+    L = O.lib()
+    p = _p(configs.CONFIGS["c0"], c0_box.n)
+    d = C.c_double(0)
+    veps, vsig = c0_box.eps[0] ** 2, 2 * c0_box.sig[0]
+
+    def f(r):
+        if which == "dir":
+            return L.oc_ewald_dir(C.c_double(0.32), C.c_double(-5.0e4), C.c_double(r), C.c_double(1 / r), C.byref(d))
+        if which == "corr":
+            return L.oc_ewald_corr(C.c_double(0.32), C.c_double(-5.0e4), C.c_double(r), C.c_double(1 / r), C.byref(d))
+        return L.oc_vdw_switch(C.byref(p), C.c_double(veps), C.c_double(vsig), C.c_double(r * r),
+                               C.c_double(1 / r), C.byref(d))
+    for r in (1.0, 2.9, 5.0, 9.5, 10.7, 11.6):
+        h = 1e-5
+        num = (f(r + h) - f(r - h)) / (2 * h)
+        f(r)
+        assert d.value == pytest.approx(num, rel=2e-6, abs=1e-7)
+
+
+# ------------------------------------------------------------------ lists
+
+@pytest.mark.parametrize("name", ["c0", "link3k", "w64"])
+def test_lists_match_reference_golden(name, gl):
+    from tests.conftest import _small_link_cfg
+    cfg = {"c0": configs.CONFIGS["c0"], "link3k": _small_link_cfg(),
+           "w64": dict(configs.COMMON, n_waters=64, L=(64 / 0.0334) ** (1 / 3), seed=9, grid=16,
+                       cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35)}[name]
+    wb = configs.build(cfg)
+    o = O.OracleSystem(wb, cfg)
+    assert o.keyLink == int(gl[f"{name}_keylink"])
+    assert digest(o.exOff, o.exLst) == str(gl[f"{name}_excl_digest"])
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    nl, np_, no = o.build_list(x, y, z, build=True)
+    assert len(nl) == int(gl[f"{name}_count"])
+    assert digest(nl, np_, no) == str(gl[f"{name}_digest"])
+    assert digest(x, y, z) == str(gl[f"{name}_wrapped_digest"])
+    rng = np.random.default_rng(1234)
+    for a in (x, y, z):
+        a += rng.normal(0.0, 0.05, len(a))
+    assert digest(x, y, z) == str(gl[f"{name}_upd_pos_digest"])
+    nl, np_, no = o.build_list(x, y, z, build=False)
+    assert digest(nl, np_, no) == str(gl[f"{name}_upd_digest"])
+
+
+def test_verlet_list_equals_brute_force(small_box):
+    """The O(N^2) list is every non-excluded pair within rc+skin under the
+    minimum image, each once at its smaller index."""
+    wb = small_box
+    o = O.OracleSystem(wb)
+    nl, np_, no = o.build_list(wb.x.copy(), wb.y.copy(), wb.z.copy())
+    X = np.stack([wb.x, wb.y, wb.z], 1)
+    d = X[None, :, :] - X[:, None, :]
+    d -= wb.L * np.trunc(d / wb.L + np.where(d >= 0, 0.5, -0.5))
+    r2 = (d * d).sum(-1)
+    cut = (wb.params["cutOff"] + wb.params["delr"]) ** 2
+    iu = np.triu_indices(wb.n, 1)
+    ok = r2[iu] <= cut
+    excl = set(zip(wb.ca.tolist(), wb.cb.tolist()))
+    pairs = {(i, j) for i, j, k in zip(iu[0], iu[1], ok) if k and (i, j) not in excl}
+    got = set(zip(np.repeat(no, np_[no]).tolist(), nl.tolist()))
+    assert got == pairs
+
+
+def test_link_cells_equal_verlet_set(link_box):
+    """Link cells and the O(N^2) builder give the same pair set here (no
+    distance sits within rounding of rc+skin)."""
+    wb = link_box
+    o = O.OracleSystem(wb)
+    assert o.keyLink == 1
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    nl, np_, no = o.build_list(x, y, z)
+    a = np.repeat(no, np_[no]).astype(np.int64)
+    s1 = np.sort(np.minimum(a, nl) * wb.n + np.maximum(a, nl))
+    p2 = O.make_param(wb.params, wb.n)
+    v = O.OracleSystem(wb)
+    v.keyLink = 0
+    ex = O.lib()
+    exOff = np.zeros(wb.n + 1, np.int32)
+    exLst = np.zeros(8 * len(wb.ca), np.int32)
+    ne = ex.oc_exclusions(wb.n, 0, None, None, len(wb.ca), O.P(wb.ca), O.P(wb.cb), 0, None, None,
+                          None, 0, O.P(exOff), O.P(exLst), len(exLst))
+    v.exOff, v.exLst = exOff, exLst[:ne].copy()
+    nl2, np2, no2 = v.build_list(x.copy(), y.copy(), z.copy())
+    a2 = np.repeat(no2, np2[no2]).astype(np.int64)
+    s2 = np.sort(np.minimum(a2, nl2) * wb.n + np.maximum(a2, nl2))
+    assert np.array_equal(s1, s2)
+    del p2
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [21, 22])
+def test_lists_match_compiled_reference(seed):
+    from tests.conftest import _small_link_cfg
+    for cfg in (dict(configs.CONFIGS["c0"], seed=seed), dict(_small_link_cfg(), seed=seed)):
+        wb = configs.build(cfg)
+        o = O.OracleSystem(wb, cfg)
+        r = O.RefSystem(wb, cfg)
+        xo, yo, zo = wb.x.copy(), wb.y.copy(), wb.z.copy()
+        xr, yr, zr = wb.x.copy(), wb.y.copy(), wb.z.copy()
+        a = o.build_list(xo, yo, zo)
+        kl, *b = r.makelist(xr, yr, zr, 0)
+        assert kl == o.keyLink
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+        assert np.array_equal(xo, xr)
+
+
+# ------------------------------------------------------------------ pair loop
+
+def test_pair_loop_equals_brute_force(small_box):
+    """Real-space energy over the list = direct sum over every non-excluded
+    pair within rc (the list is fresh), plus the exclusion correction;
+    forces obey Newton's third law."""
+    wb = small_box
+    o = O.OracleSystem(wb)
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    o.build_list(x, y, z)
+    f, e = o.ewald_real(x, y, z)
+    from scipy.special import erfc, erf
+    X = np.stack([x, y, z], 1)
+    iu = np.triu_indices(wb.n, 1)
+    d = X[iu[1]] - X[iu[0]]
+    d -= wb.L * np.trunc(d / wb.L + np.where(d >= 0, 0.5, -0.5))
+    r = np.sqrt((d * d).sum(1))
+    excl = np.zeros(len(r), bool)
+    key = {(a, b) for a, b in zip(wb.ca.tolist(), wb.cb.tolist())}
+    excl[[k for k, (a, b) in enumerate(zip(iu[0], iu[1])) if (a, b) in key]] = True
+    p = wb.params
+    inc = (r <= p["cutOff"]) & ~excl
+    qq = 332.0716 * 418.4 * wb.q[iu[0]] * wb.q[iu[1]]
+    edir = (qq * erfc(p["alpha"] * r) / r)[inc].sum()
+    ecorr = (-qq * erf(p["alpha"] * r) / r)[excl].sum()
+    assert e.eDir == pytest.approx(edir, rel=1e-11)
+    assert e.eCorr == pytest.approx(ecorr, rel=1e-12)
+    assert e.nInCut == int(inc.sum())
+    assert np.abs(f.sum(1)).max() < 1e-7 * np.abs(f).max()
+
+
+def test_nonbond_loop_matches_compiled_reference_golden(gd):
+    """10 VV+SHAKE steps with SWITCH-Coulomb forces: oracle vs the
+    reference's own integrator, SHAKE and pair loop."""
+    cfg = dict(configs.COMMON, n_waters=64, L=(64 / 0.0334) ** (1 / 3), seed=9, grid=16,
+               cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35, update=5)
+    wb = configs.build(cfg)
+    o = O.OracleSystem(wb, cfg)
+    st = [a.copy() for a in gd["x0"]]
+    L = O.lib()
+    eE, eV, vE, vV = (C.c_double(0) for _ in range(4))
+
+    def forces():
+        f = np.zeros((3, wb.n))
+        L.oc_nonbond(C.byref(o.p), C.byref(o.b), 4, O.P(st[0]), O.P(st[1]), O.P(st[2]),
+                     O.P(f[0]), O.P(f[1]), O.P(f[2]), O.P(wb.q), O.P(wb.eps), O.P(wb.sig),
+                     O.P(o.neighList), O.P(o.neighPair), O.P(o.neighOrder), C.byref(eE),
+                     C.byref(eV), C.byref(vE), C.byref(vV))
+        return f
+    o.build_list(st[0], st[1], st[2], build=True)
+    f = forces()
+    kin, pot, vsh = [], [], []
+    for step in range(1, 11):
+        err, _, vs, _ = o.vv(1, *st, f)
+        assert err == 0
+        if step % 5 == 0:
+            o.build_list(st[0], st[1], st[2], build=False)
+        f = forces()
+        err, ek, _, _ = o.vv(2, *st, f)
+        kin.append(ek)
+        vsh.append(vs)
+        pot.append(eE.value + eV.value)
+    assert np.array_equal(np.stack(st), gd["x10"])
+    assert np.array_equal(np.array(kin), gd["kin"])
+    assert np.array_equal(np.array(vsh), gd["virshake"])
+    assert np.array_equal(np.array(pot), gd["pot"])
+
+
+# ------------------------------------------------------------------ SPME
+
+def test_bspline_identities():
+    L = O.lib()
+    bsp, bsd = np.zeros(8), np.zeros(8)
+    for p in (4, 6, 8):
+        for s in (3.0, 3.25, 7.999, 11.5, 0.3):
+            L.oc_bspline(C.c_double(s), p, O.P(bsp), O.P(bsd))
+            assert bsp[:p].sum() == pytest.approx(1.0, abs=1e-14)      # partition of unity
+            assert bsd[:p].sum() == pytest.approx(0.0, abs=1e-14)
+            h = 1e-6
+            bp, bm, t = np.zeros(8), np.zeros(8), np.zeros(8)
+            L.oc_bspline(C.c_double(s + h), p, O.P(bp), O.P(t))
+            L.oc_bspline(C.c_double(s - h), p, O.P(bm), O.P(t))
+            if int(s + h) == int(s - h):
+                assert np.allclose((bp - bm)[:p] / (2 * h), bsd[:p], atol=1e-8)
+
+
+def test_bspline_moduli_closed_form():
+    """|b(m)|^2 of Essmann et al. (1995) eq. 4.4 for order 4 (cubic): the
+    denominator sum_k M_4(k+1) e^{2 pi i m k/K} with M_4(1,2,3) = 1/6,
+    2/3, 1/6."""
+    L = O.lib()
+    for K in (16, 20, 32):
+        mod = np.zeros(K)
+        L.oc_bspmod(K, 4, O.P(mod))
+        m = np.arange(K)
+        den = 1 / 6 + 2 / 3 * np.exp(2j * np.pi * m / K) + 1 / 6 * np.exp(4j * np.pi * m / K)
+        assert np.allclose(mod, 1 / np.abs(den) ** 2, rtol=1e-13)
+
+
+@pytest.mark.parametrize("shape", [(4, 6, 5), (8, 8, 8), (6, 10, 12)])
+def test_fft_against_direct_dft(shape):
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    for sign in (+1, -1):
+        buf = np.ascontiguousarray(a).view(np.float64).copy()
+        O.lib().oc_fft3d(O.P(buf), *shape, sign)
+        got = buf.view(np.complex128).reshape(shape)
+        want = np.fft.ifftn(a) * a.size if sign > 0 else np.fft.fftn(a)
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-10)
+
+
+def test_spme_converges_to_classic_ewald(gd):
+    """SPME reciprocal energy -> the reference's classic Ewald sum (golden)
+    as the mesh and spline order grow; forces too (mean removed)."""
+    cfg = dict(configs.COMMON, n_waters=64, L=(64 / 0.0334) ** (1 / 3), seed=9, grid=16,
+               cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35)
+    wb = configs.build(cfg)
+    exact = float(gd["ewald_E"][-1])
+    errs = []
+    for grid, order in ((16, 4), (32, 6), (48, 8)):
+        o = O.OracleSystem(wb, dict(cfg, grid=grid, nbsp=order))
+        e, f, st, vir = o.spme(*gd["ewald_pos"])
+        errs.append(abs(e - exact) / abs(exact))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[-1] < 1e-9
+    fe = gd["ewald_f10"] - gd["ewald_f10"].mean(1, keepdims=True)
+    assert np.sqrt(((f - fe) ** 2).sum() / (fe ** 2).sum()) < 1e-6
+    # the oracle's own classic sum agrees with the reference's
+    fo = np.zeros((3, wb.n))
+    pr = O.make_param(cfg, wb.n)
+    ec = O.lib().oc_ewald_classic(C.byref(pr), C.byref(o.b), 8, *[O.P(v) for v in gd["ewald_pos"]],
+                                  O.P(fo[0]), O.P(fo[1]), O.P(fo[2]), O.P(wb.q))
+    assert ec == pytest.approx(exact, rel=1e-12)
+
+
+def test_spme_energy_force_consistency(small_box):
+    """SPME force = -dE/dx by central differences (spline order 6 is
+    differentiable enough)."""
+    wb = small_box
+    cfg = dict(wb.params, nbsp=6, grid=24)
+    o = O.OracleSystem(wb, cfg)
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    e0, f, _, _ = o.spme(x, y, z)
+    fm = f.mean(1)
+    for i in (0, 5, 100):
+        h = 1e-5
+        x[i] += h
+        ep = o.spme(x, y, z)[0]
+        x[i] -= 2 * h
+        em = o.spme(x, y, z)[0]
+        x[i] += h
+        assert f[0, i] + fm[0] == pytest.approx(-(ep - em) / (2 * h), rel=1e-5, abs=1e-3)
+
+
+# ------------------------------------------------------------------ integrator
+
+def test_constraint_velocity_projection_matches_reference(gd):
+    """The generator's init_constvel restatement (numpy) vs the reference's."""
+    from mdbas_amd import water
+    cfg = dict(configs.COMMON, n_waters=64, L=(64 / 0.0334) ** (1 / 3), seed=9, grid=16,
+               cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35)
+    wb = configs.build(cfg)
+    v = gd["cv_in"]
+    vx, vy, vz = water._init_constvel(wb.x, wb.y, wb.z, v[0].copy(), v[1].copy(), v[2].copy(),
+                                      wb.mass, wb.nAtConst, wb.ca, wb.cb, wb.L, 1e-8, 150)
+    assert np.allclose(np.stack([vx, vy, vz]), gd["cv_out"], rtol=0, atol=1e-9)
+
+
+@needs_ref
+def test_vv_shake_match_compiled_reference(c0_box):
+    wb = c0_box
+    o = O.OracleSystem(wb)
+    r = O.RefSystem(wb)
+    s1 = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    s2 = [a.copy() for a in s1]
+    o.build_list(*s1[:3])
+    f, _ = o.energy(*s1[:3])
+    for stage in (1, 2):
+        res = o.vv(stage, *s1, f)
+        out = r.vv(stage, *s2, f.copy())
+        assert res[0] == 0
+        for a, b in zip(s1, s2):
+            assert np.array_equal(a, b)
+        if stage == 1:
+            assert res[2] == out[1]
+        else:
+            assert res[1] == out[0]
+
+
+def test_md_run_conserves_constraints(c0_box):
+    wb = c0_box
+    o = O.OracleSystem(wb)
+    a = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    err, f, et, ep, ek, sec = o.md_run(5, *a)
+    assert err == 0
+    X = np.stack(a[:3], 1)
+    d = X[wb.cb] - X[wb.ca]
+    d -= wb.L * np.round(d / wb.L)
+    r2 = (d * d).sum(1)
+    assert np.abs(r2 - wb.rc2).max() / np.sqrt(wb.rc2).min() < 4e-8
+    assert np.all(np.isfinite(et))
+
+
+def test_shake_failure_code():
+    """maxCycle 1 cannot converge: error 311 (shake.c:340-341)."""
+    wb = configs.build(configs.CONFIGS["c0"])
+    cfg = dict(configs.CONFIGS["c0"], maxCycle=1)
+    o = O.OracleSystem(wb, cfg)
+    s = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    o.build_list(*s[:3])
+    f, _ = o.energy(*s[:3])
+    err, *_ = o.vv(1, *s, f)
+    assert err == 311
