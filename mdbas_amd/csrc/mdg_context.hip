@@ -297,7 +297,7 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->fix, ctx->fiy, ctx->fiz, ctx->frx, ctx->fry, ctx->frz, ctx->frx64,
                     ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,
                     ctx->grid64, ctx->cgrid64, ctx->grid32, ctx->cgrid32, ctx->gridq,
-                    ctx->fftWork, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
+                    ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
                     ctx->shkMask};
     for (void *p : ptrs)
         if (p) hipFree(p);

@@ -20,29 +20,9 @@
 // wider than that would be a divergence from the reference in its SHAKE
 // stopping sweep only.  Arithmetic follows the reference's operation order
 // with contraction off.
-#include "mdg_internal.h"
+#include "mdg_shake.h"
 
-struct GrpArgs {
-    MdgParams P;
-    MdgBox B;
-    int nGrp;
-    const int *atomOff, *atoms, *conOff, *cons;
-    const int *ca, *cb;
-    const double *rc2, *rmass, *mass;
-    const int *nAtConst;
-    double *x, *y, *z, *vx, *vy, *vz;
-    const double *fx, *fy, *fz;
-    double *dd;           // [3][nConst]
-    int nConst;
-    int *cycle;
-    double *vir;
-    int *ctrl;
-    double *part;         // stage 2: kin, stresk[6]
-    double *scr;          // [SHK_SLOTS][6][n] saved states
-    double *virScr;       // [SHK_SLOTS][nGrp] SHAKE virial at those states
-    int *first, *kend;    // per group: first converged sweep, last checked sweep
-    unsigned long long *mask;   // device-wide AND of converged-sweep bits
-};
+
 
 __device__ inline int local_of(const int *atoms, int na, int g)
 {
@@ -61,7 +41,7 @@ struct Grp {
     double ddx[MDG_GRP_CONS], ddy[MDG_GRP_CONS], ddz[MDG_GRP_CONS], rc2[MDG_GRP_CONS];
 };
 
-__device__ inline void grp_load(const GrpArgs &A, int g, Grp &G)
+__device__ inline void grp_load(const ShakeArgs &A, int g, Grp &G)
 {
     const int a0 = A.atomOff[g], c0 = A.conOff[g];
     G.na = A.atomOff[g + 1] - a0;
@@ -88,7 +68,7 @@ __device__ inline void grp_load(const GrpArgs &A, int g, Grp &G)
     }
 }
 
-__device__ inline void grp_store(const GrpArgs &A, const Grp &G, bool pos)
+__device__ inline void grp_store(const ShakeArgs &A, const Grp &G, bool pos)
 {
     for (int k = 0; k < G.na; k++) {
         int i = G.at[k];
@@ -104,7 +84,7 @@ __device__ inline void grp_store(const GrpArgs &A, const Grp &G, bool pos)
 }
 
 // shake.c:243-258: residual check over the group; fills dt / rt2.
-__device__ inline double shake_r_check(const GrpArgs &A, const Grp &G, double *dtx, double *dty,
+__device__ inline double shake_r_check(const ShakeArgs &A, const Grp &G, double *dtx, double *dty,
                                        double *dtz, double *rt2)
 {
 #pragma clang fp contract(off)
@@ -121,7 +101,7 @@ __device__ inline double shake_r_check(const GrpArgs &A, const Grp &G, double *d
 }
 
 // shake.c:262-332: one correction sweep over the group.
-__device__ inline void shake_r_apply(const GrpArgs &A, Grp &G, const double *dtx,
+__device__ inline void shake_r_apply(const ShakeArgs &A, Grp &G, const double *dtx,
                                      const double *dty, const double *dtz, const double *rt2,
                                      double &vir)
 {
@@ -163,7 +143,7 @@ __device__ inline void shake_r_apply(const GrpArgs &A, Grp &G, const double *dtx
 }
 
 // shake.c:378-430: velocity residual (max |lambda|) and the corrections.
-__device__ inline double shake_v_sweep(const GrpArgs &A, Grp &G, bool apply_if_needed,
+__device__ inline double shake_v_sweep(const ShakeArgs &A, Grp &G, bool apply_if_needed,
                                        double tolvel, bool force_apply)
 {
 #pragma clang fp contract(off)
@@ -205,12 +185,7 @@ __device__ inline double shake_v_sweep(const GrpArgs &A, Grp &G, bool apply_if_n
     return maxd;
 }
 
-// Window of SHAKE states kept per group around its first convergence.
-#define SHK_W 8
-#define SHK_SLOTS (SHK_W + 2)     // [0] start, [1..W] window, [W+1] end
-#define SHK_MASKW 4               // 256 sweep indices
-
-__device__ inline void scr_store(const GrpArgs &A, const Grp &G, int slot, bool pos)
+__device__ inline void scr_store(const ShakeArgs &A, const Grp &G, int slot, bool pos)
 {
     const long n = A.P.n;
     double *s = A.scr + (long)slot * 6 * n;
@@ -227,7 +202,7 @@ __device__ inline void scr_store(const GrpArgs &A, const Grp &G, int slot, bool 
     }
 }
 
-__device__ inline void scr_load(const GrpArgs &A, Grp &G, int slot, bool pos)
+__device__ inline void scr_load(const ShakeArgs &A, Grp &G, int slot, bool pos)
 {
     const long n = A.P.n;
     const double *s = A.scr + (long)slot * 6 * n;
@@ -244,7 +219,7 @@ __device__ inline void scr_load(const GrpArgs &A, Grp &G, int slot, bool pos)
     }
 }
 
-__device__ inline void dd_load(const GrpArgs &A, Grp &G)
+__device__ inline void dd_load(const ShakeArgs &A, Grp &G)
 {
     for (int c = 0; c < G.nc; c++) {
         G.ddx[c] = A.dd[G.cid[c]];
@@ -253,7 +228,7 @@ __device__ inline void dd_load(const GrpArgs &A, Grp &G)
     }
 }
 
-__device__ inline void dd_compute(const GrpArgs &A, Grp &G)
+__device__ inline void dd_compute(const ShakeArgs &A, Grp &G)
 {
     for (int c = 0; c < G.nc; c++) {
         double dx = G.x[G.ib[c]] - G.x[G.ia[c]];
@@ -269,40 +244,12 @@ __device__ inline void dd_compute(const GrpArgs &A, Grp &G)
     }
 }
 
-// Publishes the group's converged-at-sweep-k bits into the device-wide
-// AND mask: bits for k <= kend are as checked; later sweeps count as
-// converged when the group was converged at its last two checks.
-__device__ inline void publish_mask(const GrpArgs &A, const unsigned long long (&bits)[SHK_MASKW],
-                                    int kend, bool settled)
-{
-    for (int w = 0; w < SHK_MASKW; w++) {
-        unsigned long long m = bits[w];
-        if (settled)
-            for (int b = 0; b < 64; b++)
-                if (w * 64 + b > kend && w * 64 + b < A.P.maxCycle) m |= 1ull << b;
-        atomicAnd(&A.mask[w], m);
-    }
-}
-
-// First sweep index at which every group was converged (-1 if none).
-__device__ inline int global_stop(const GrpArgs &A)
-{
-    for (int w = 0; w < SHK_MASKW; w++) {
-        unsigned long long m = A.mask[w];
-        if (m) {
-            int k = w * 64 + __ffsll((long long)m) - 1;
-            return k < A.P.maxCycle ? k : -1;
-        }
-    }
-    return -1;
-}
-
 // Stage 1, part A: constraint vectors, half kick, drift, then SHAKE-R
 // sweeps until the group has been converged at two consecutive checks
 // (integrate.c:1526-1590, shake.c:243-338), recording which sweep indices
 // were converged and keeping the states of the first SHK_W sweeps from
 // the first convergence on.
-__global__ void __launch_bounds__(128) k_vv1_a(GrpArgs A)
+__global__ void __launch_bounds__(128) k_vv1_a(ShakeArgs A)
 {
 #pragma clang fp contract(off)
     int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -353,12 +300,12 @@ __global__ void __launch_bounds__(128) k_vv1_a(GrpArgs A)
     A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vir;
     A.first[g] = first;
     A.kend[g] = k;
-    publish_mask(A, bits, k, settled);
+    shake_publish_mask(A, bits, k, settled);
 }
 
 // Stage 1, part B: every group takes its state at the global stop sweep C
 // (from the window, or by sweeping on from the last or the first state).
-__global__ void __launch_bounds__(128) k_vv1_b(GrpArgs A)
+__global__ void __launch_bounds__(128) k_vv1_b(ShakeArgs A)
 {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.nGrp) return;
@@ -366,7 +313,7 @@ __global__ void __launch_bounds__(128) k_vv1_b(GrpArgs A)
         A.vir[g] = 0.;
         return;
     }
-    const int C = global_stop(A);
+    const int C = shake_global_stop(A);
     if (C < 0) {
         atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 1);
         return;
@@ -400,7 +347,7 @@ __global__ void __launch_bounds__(128) k_vv1_b(GrpArgs A)
 
 // Stage 2, part A: constraint vectors, half kick, SHAKE-V sweeps with the
 // same bookkeeping (integrate.c:1526-1561, shake.c:359-457).
-__global__ void __launch_bounds__(128) k_vv2_a(GrpArgs A)
+__global__ void __launch_bounds__(128) k_vv2_a(ShakeArgs A)
 {
 #pragma clang fp contract(off)
     int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -442,12 +389,12 @@ __global__ void __launch_bounds__(128) k_vv2_a(GrpArgs A)
     scr_store(A, G, SHK_W + 1, false);
     A.first[g] = first;
     A.kend[g] = k;
-    publish_mask(A, bits, k, settled);
+    shake_publish_mask(A, bits, k, settled);
 }
 
 // Stage 2, part B: state at the global stop sweep, kinetic energy and
 // kinetic stress partials (utils.c:502-542), wrap (integrate.c:1620-1627).
-__global__ void __launch_bounds__(128) k_vv2_b(GrpArgs A)
+__global__ void __launch_bounds__(128) k_vv2_b(ShakeArgs A)
 {
 #pragma clang fp contract(off)
     int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -456,7 +403,7 @@ __global__ void __launch_bounds__(128) k_vv2_b(GrpArgs A)
         Grp G;
         grp_load(A, g, G);
         if (A.conOff[g + 1] != A.conOff[g]) {
-            const int C = global_stop(A);
+            const int C = shake_global_stop(A);
             if (C < 0) {
                 atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 2);
             } else {
@@ -559,6 +506,18 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
     std::vector<int> fa(aoff.begin(), aoff.end() - 1), fc(coff.begin(), coff.end() - 1);
     for (int i = 0; i < n; i++) atoms[fa[gid[i]]++] = i;       // ascending within a group
     for (int c = 0; c < nConst; c++) conl[fc[gid[cons[c].a]]++] = c;   // global order
+    // every group a three-site molecule with constraints (a0,a1), (a0,a2),
+    // (a1,a2) in that order: use the register-resident kernels
+    bool w3 = ng > 0;
+    for (int g = 0; g < ng && w3; g++) {
+        if (aoff[g + 1] - aoff[g] != 3 || coff[g + 1] - coff[g] != 3) { w3 = false; break; }
+        const int *a = &atoms[aoff[g]];
+        const int *c = &conl[coff[g]];
+        const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+        for (int k = 0; k < 3; k++)
+            if (cons[c[k]].a != a[pa[k]] || cons[c[k]].b != a[pb[k]]) w3 = false;
+    }
+    ctx->shakeW3 = w3;
     ctx->nGrp = ng;
     if (ctx->shkScr) {
         hipFree(ctx->shkScr); hipFree(ctx->shkVir); hipFree(ctx->shkFirst); hipFree(ctx->shkKend);
@@ -589,7 +548,7 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
                  double *virshake_out, bool sync)
 {
-    GrpArgs A;
+    ShakeArgs A;
     A.P = ctx->P;
     A.B = ctx->B;
     A.nGrp = ctx->nGrp;
@@ -631,20 +590,22 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.first = ctx->shkFirst;
     A.kend = ctx->shkKend;
     A.mask = ctx->shkMask;
-    const int nb = nblk(ctx->nGrp, 128);
+    const bool w3 = ctx->shakeW3;
+    const int bs = w3 ? 64 : 128;
+    const int nb = nblk(ctx->nGrp, bs);
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl, ctx->shkMask);
     LAUNCHCHK();
     if (stage == 1) {
-        hipLaunchKernelGGL(k_vv1_a, dim3(nb), dim3(128), 0, ctx->stream, A);
+        hipLaunchKernelGGL(w3 ? k_w3_vv1_a : k_vv1_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
-        hipLaunchKernelGGL(k_vv1_b, dim3(nb), dim3(128), 0, ctx->stream, A);
+        hipLaunchKernelGGL(w3 ? k_w3_vv1_b : k_vv1_b, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         // SHAKE virial: sum of the per-group sums (fixed order)
         RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, ctx->nGrp, 1, ctx->scal + S_VIRSHAKE));
     } else {
-        hipLaunchKernelGGL(k_vv2_a, dim3(nb), dim3(128), 0, ctx->stream, A);
+        hipLaunchKernelGGL(w3 ? k_w3_vv2_a : k_vv2_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
-        hipLaunchKernelGGL(k_vv2_b, dim3(nb), dim3(128), 0, ctx->stream, A);
+        hipLaunchKernelGGL(w3 ? k_w3_vv2_b : k_vv2_b, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         RETCHK(mdg_reduce_partials(ctx, A.part, nb, 7, ctx->scal + S_KIN));
     }

@@ -15,6 +15,7 @@
 #define MDG_GRP_ATOMS 8          // max atoms in one constraint group
 #define MDG_GRP_CONS 16          // max constraints in one constraint group
 #define MDG_SEG_MAX 4096         // max atoms in one cell segment sorted in LDS
+#define MDG_CHUNK 32             // j-clusters per wavefront in the pair kernel
 
 // Fixed-point scales: forces in internal units (10 J/mol/A) carry 2^32
 // fractional steps in an int64 (|F| < 2^31); energies and virials 2^24
@@ -69,6 +70,9 @@ struct ClusterList {
     unsigned long long *cpMask = nullptr;
     int *cpI = nullptr;          // entry -> I
     long cap = 0;
+    int *chOff = nullptr;        // per I: first chunk (nclu+1)
+    int2 *chunk = nullptr;       // chunk -> (I, first entry)
+    int maxChunks = 0;
     int *tmpCnt = nullptr;       // per I: entries found
     int *tmpJ = nullptr;         // per I scratch [nclu][kmax]
     unsigned long long *tmpMask = nullptr;
@@ -112,6 +116,7 @@ struct mdg_ctx {
     double *shkScr = nullptr, *shkVir = nullptr;   // SHAKE state window (mdg_integrate.hip)
     int *shkFirst = nullptr, *shkKend = nullptr;
     unsigned long long *shkMask = nullptr;
+    bool shakeW3 = false;        // all groups are three-site molecules
     // exclusions (CSR; symmetric for link cells, ordered otherwise)
     int *exOff = nullptr, *exLst = nullptr;
     int nExcl = 0;

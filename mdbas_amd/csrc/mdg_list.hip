@@ -579,6 +579,23 @@ __global__ void k_cluster_compact(int nclu, int kmax, const int *__restrict__ tm
     }
 }
 
+__global__ void k_chunk_count(int nclu, const int *__restrict__ tmpCnt, int *__restrict__ nch)
+{
+    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < nclu) nch[I] = (tmpCnt[I] + MDG_CHUNK - 1) / MDG_CHUNK;
+}
+
+__global__ void k_chunk_fill(int nclu, const int *__restrict__ cpOff, const int *__restrict__ chOff,
+                             int maxChunks, int2 *__restrict__ chunk, int *__restrict__ ctrl)
+{
+    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nclu) return;
+    for (int c = chOff[I]; c < chOff[I + 1]; c++) {
+        if (c < maxChunks) chunk[c] = make_int2(I, cpOff[I] + (c - chOff[I]) * MDG_CHUNK);
+        else atomicOr(&ctrl[CTRL_LIST_OVF], 4);
+    }
+}
+
 // Builds the cluster-pair list from ctx->x,y,z (device).  Every call is
 // graph-capturable once the buffers exist (first call sizes them).
 int mdg_build_cluster_list(mdg_ctx *ctx)
@@ -674,10 +691,23 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&C.cpJ, C.cap * sizeof(int)));
         HIPCHK(hipMalloc(&C.cpI, C.cap * sizeof(int)));
         HIPCHK(hipMalloc(&C.cpMask, C.cap * sizeof(unsigned long long)));
+        C.maxChunks = (int)(C.cap / MDG_CHUNK) + nclu + 64;
+        HIPCHK(hipMalloc(&C.chunk, (size_t)C.maxChunks * sizeof(int2)));
+        HIPCHK(hipMalloc(&C.chOff, (nclu + 1) * sizeof(int)));
     }
     hipLaunchKernelGGL(k_cluster_compact, dim3(nclu), dim3(64), 0, ctx->stream, nclu, C.kmax,
                        C.tmpCnt, C.tmpJ, C.tmpMask, C.cpOff, C.cap, C.cpJ, C.cpMask, C.cpI,
                        ctx->ctrl);
+    LAUNCHCHK();
+    // work chunks of at most MDG_CHUNK entries for the pair kernel; the
+    // per-I chunk counts reuse tmpCnt's space after the compaction
+    int *nch = C.sortTmp;   // n ints, free again after the column sort (n >= nclu)
+    hipLaunchKernelGGL(k_chunk_count, dim3(nblk(nclu, 256)), dim3(256), 0, ctx->stream, nclu,
+                       C.tmpCnt, nch);
+    LAUNCHCHK();
+    RETCHK(mdg_scan_int(ctx, nch, C.chOff, nclu));
+    hipLaunchKernelGGL(k_chunk_fill, dim3(nblk(nclu, 256)), dim3(256), 0, ctx->stream, nclu,
+                       C.cpOff, C.chOff, C.maxChunks, C.chunk, ctx->ctrl);
     LAUNCHCHK();
     return MDG_OK;
 }

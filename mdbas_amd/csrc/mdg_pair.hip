@@ -267,6 +267,8 @@ struct ClusterArgs {
     const double *sq, *seps, *ssig;  // slot parameters for FP64
     const int *cpOff, *cpJ;
     const unsigned long long *cpMask;
+    const int *chOff;                // per i-cluster chunk offsets (nclu + 1)
+    const int2 *chunk;               // chunk -> (I, first entry)
     long long *fix, *fiy, *fiz;      // MIXED: int64 fixed-point slot forces
     double *fdx, *fdy, *fdz;         // FP64: slot forces
     double *part;                    // per block: eDir, eVdw, virDir, virVdw, nIn
@@ -276,10 +278,13 @@ template <bool MIXED>
 __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
 {
     const int lane = threadIdx.x & 63;
-    const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int W = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int a = lane >> 3, b = lane & 7;
     double e_acc[5] = {0, 0, 0, 0, 0};
-    if (I < A.nclu) {
+    if (W < A.chOff[A.nclu]) {
+        // chunk W: entries [e0, e1) of i-cluster I (at most MDG_CHUNK of them)
+        const int2 ch = A.chunk[W];
+        const int I = ch.x, e0 = ch.y, e1 = min(e0 + MDG_CHUNK, A.cpOff[I + 1]);
         const int si = I * MDG_CL + a;
         const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
         float qi, ei, gi;
@@ -294,12 +299,28 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
         double fxid = 0., fyid = 0., fzid = 0.;
         double ed = 0., ev = 0., vd = 0., vv = 0.;
         int nin = 0;
-        const int e0 = A.cpOff[I], e1 = A.cpOff[I + 1];
+        // software pipeline: the next j-cluster's data is in flight while
+        // the current one is computed
+        int Jn = A.cpJ[e0];
+        unsigned long long mn = A.cpMask[e0];
+        double xn = A.sx[Jn * MDG_CL + b], yn = A.sy[Jn * MDG_CL + b], zn = A.sz[Jn * MDG_CL + b];
+        float4 pn = MIXED ? A.sp32[Jn * MDG_CL + b] : make_float4(0.f, 0.f, 0.f, 0.f);
         for (int e = e0; e < e1; e++) {
-            const int J = A.cpJ[e];
-            const unsigned long long m = A.cpMask[e];
+            const int J = Jn;
+            const unsigned long long m = mn;
+            const double xj = xn, yj = yn, zj = zn;
+            const float4 pjc = pn;
+            if (e + 1 < e1) {
+                Jn = A.cpJ[e + 1];
+                mn = A.cpMask[e + 1];
+                const int sn = Jn * MDG_CL + b;
+                xn = A.sx[sn];
+                yn = A.sy[sn];
+                zn = A.sz[sn];
+                if (MIXED) pn = A.sp32[sn];
+            }
             const int sj = J * MDG_CL + b;
-            double dx = A.sx[sj] - xi, dy = A.sy[sj] - yi, dz = A.sz[sj] - zi;
+            double dx = xj - xi, dy = yj - yi, dz = zj - zi;
             double r2;
             if (MIXED) {
                 dx = fma(-A.F.L0, rint(dx * A.F.iL0), dx);
@@ -313,7 +334,7 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
             if (MIXED) {
                 float fjx = 0.f, fjy = 0.f, fjz = 0.f;
                 if (in) {
-                    const float4 pj = A.sp32[sj];
+                    const float4 pj = pjc;
                     const float r2f = (float)r2;
                     const float rinv = __builtin_amdgcn_rsqf(r2f);
                     const float r = r2f * rinv;
@@ -572,6 +593,8 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     A.cpOff = C.cpOff;
     A.cpJ = C.cpJ;
     A.cpMask = C.cpMask;
+    A.chOff = C.chOff;
+    A.chunk = C.chunk;
     A.fix = ctx->fix;
     A.fiy = ctx->fiy;
     A.fiz = ctx->fiz;
@@ -579,7 +602,7 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     A.fdy = ctx->fdy;
     A.fdz = ctx->fdz;
     A.part = ctx->epart + PART_PAIR;
-    const int nb = nblk(C.nclu, 4);
+    const int nb = nblk(C.maxChunks, 4);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     if (ctx->precision == MDG_MIXED)
         hipLaunchKernelGGL(k_pair_cluster<true>, dim3(nb), dim3(256), 0, ctx->stream, A);

@@ -1,0 +1,106 @@
+// mdg_shake.h -- shared state of the SHAKE kernels (generic groups in
+// mdg_integrate.hip, three-site molecules in mdg_shake_w3.hip).
+#pragma once
+#include "mdg_internal.h"
+
+// Window of SHAKE states kept per group around its first convergence.
+#define SHK_W 8
+#define SHK_SLOTS (SHK_W + 2)     // [0] start, [1..W] window, [W+1] end
+#define SHK_MASKW 4               // 256 sweep indices
+
+struct ShakeArgs {
+    MdgParams P;
+    MdgBox B;
+    int nGrp;
+    const int *atomOff, *atoms, *conOff, *cons;
+    const int *ca, *cb;
+    const double *rc2, *rmass, *mass;
+    const int *nAtConst;
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double *fx, *fy, *fz;
+    double *dd;           // [3][nConst]
+    int nConst;
+    int *cycle;
+    double *vir;
+    int *ctrl;
+    double *part;         // stage 2: kin, stresk[6]
+    double *scr;          // [SHK_SLOTS][6][n] saved states
+    double *virScr;       // [SHK_SLOTS][nGrp] SHAKE virial at those states
+    int *first, *kend;    // per group: first converged sweep, last checked sweep
+    unsigned long long *mask;   // device-wide AND of converged-sweep bits
+};
+
+// Publishes the group's converged-at-sweep-k bits into the device-wide
+// AND mask: bits for k <= kend are as checked; later sweeps count as
+// converged when the group was converged at its last two checks.
+__device__ inline void shake_publish_mask(const ShakeArgs &A, const unsigned long long (&bits)[SHK_MASKW],
+                                    int kend, bool settled)
+{
+    for (int w = 0; w < SHK_MASKW; w++) {
+        unsigned long long m = bits[w];
+        if (settled)
+            for (int b = 0; b < 64; b++)
+                if (w * 64 + b > kend && w * 64 + b < A.P.maxCycle) m |= 1ull << b;
+        atomicAnd(&A.mask[w], m);
+    }
+}
+
+// First sweep index at which every group was converged (-1 if none).
+__device__ inline int shake_global_stop(const ShakeArgs &A)
+{
+    for (int w = 0; w < SHK_MASKW; w++) {
+        unsigned long long m = A.mask[w];
+        if (m) {
+            int k = w * 64 + __ffsll((long long)m) - 1;
+            return k < A.P.maxCycle ? k : -1;
+        }
+    }
+    return -1;
+}
+
+
+// Which saved state leads to sweep C: the window slot, the end state, or
+// the start (when C lies beyond the window but before the end).
+__device__ inline void shake_pick_slot(int C, int first, int kend, int &slot, int &at)
+{
+    if (first >= 0 && C >= first && C - first < SHK_W && C <= kend) {
+        slot = 1 + C - first;
+        at = C;
+    } else if (C >= kend) {
+        slot = SHK_W + 1;
+        at = kend;
+    } else {
+        slot = 0;
+        at = 0;
+    }
+}
+
+// utils.c:162-218 (image_update) for one atom.
+__device__ inline void shake_image(const MdgBox &B, double &xs, double &ys, double &zs)
+{
+#pragma clang fp contract(off)
+    if (B.type == 1) {
+        xs = xs - B.a1 * mdg_nint(xs * B.u1);
+        ys = ys - B.a1 * mdg_nint(ys * B.u1);
+        zs = zs - B.a1 * mdg_nint(zs * B.u1);
+    } else if (B.type == 2) {
+        xs = xs - B.a1 * mdg_nint(xs * B.u1);
+        ys = ys - B.b2 * mdg_nint(ys * B.v2);
+        zs = zs - B.c3 * mdg_nint(zs * B.w3);
+    } else if (B.type == 3) {
+        double xt = xs * B.u1 + ys * B.u2 + zs * B.u3;
+        double yt = xs * B.v1 + ys * B.v2 + zs * B.v3;
+        double zt = xs * B.w1 + ys * B.w2 + zs * B.w3;
+        xt -= mdg_nint(xt);
+        yt -= mdg_nint(yt);
+        zt -= mdg_nint(zt);
+        xs = xt * B.a1 + yt * B.b1 + zt * B.c1;
+        ys = xt * B.a2 + yt * B.b2 + zt * B.c2;
+        zs = xt * B.a3 + yt * B.b3 + zt * B.c3;
+    }
+}
+
+__global__ void k_w3_vv1_a(ShakeArgs A);
+__global__ void k_w3_vv1_b(ShakeArgs A);
+__global__ void k_w3_vv2_a(ShakeArgs A);
+__global__ void k_w3_vv2_b(ShakeArgs A);
