@@ -203,6 +203,37 @@ int mdg_upload_exclusions(mdg_ctx *ctx)
     }
     RETCHK(h2d(ctx, ctx->exOff, ctx->hExOff.data(), ctx->n + 1));
     RETCHK(h2d(ctx, ctx->exLst, ctx->hExLst.data(), ne));
+    // Symmetric copy for the per-atom correction in the device-resident
+    // step: every pair (i, j > i) of row i, with its multiplicity, is listed
+    // under both atoms (the same pairs the reference's correction loop
+    // visits, whichever orientation the table has).
+    const int n = ctx->n;
+    std::vector<int> cnt(n + 1, 0);
+    for (int i = 0; i < n; i++)
+        for (int k = ctx->hExOff[i]; k < ctx->hExOff[i + 1]; k++) {
+            const int j = ctx->hExLst[k];
+            if (j <= i) continue;
+            cnt[i]++;
+            cnt[j]++;
+        }
+    std::vector<int> off(n + 1, 0);
+    for (int i = 0; i < n; i++) off[i + 1] = off[i] + cnt[i];
+    std::vector<int> lst(off[n] + 1), pos(off.begin(), off.end() - 1);
+    for (int i = 0; i < n; i++)
+        for (int k = ctx->hExOff[i]; k < ctx->hExOff[i + 1]; k++) {
+            const int j = ctx->hExLst[k];
+            if (j <= i) continue;
+            lst[pos[i]++] = j;
+            lst[pos[j]++] = i;
+        }
+    if (off[n] > ctx->nExSym || !ctx->exSymLst) {
+        if (ctx->exSymLst) HIPCHK(hipFree(ctx->exSymLst));
+        RETCHK(dmalloc(ctx, &ctx->exSymLst, off[n] + 1));
+        ctx->nExSym = off[n];
+    }
+    if (!ctx->exSymOff) RETCHK(dmalloc(ctx, &ctx->exSymOff, n + 1));
+    RETCHK(h2d(ctx, ctx->exSymOff, off.data(), n + 1));
+    RETCHK(h2d(ctx, ctx->exSymLst, lst.data(), off[n]));
     return sync(ctx);
 }
 
@@ -298,7 +329,7 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,
                     ctx->grid64, ctx->cgrid64, ctx->grid32, ctx->cgrid32, ctx->gridq,
                     ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
-                    ctx->shkMask};
+                    ctx->shkMask, ctx->exSymOff, ctx->exSymLst};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int k = 0; k < 8; k++)

@@ -249,11 +249,10 @@ __device__ inline void dd_compute(const ShakeArgs &A, Grp &G)
 // (integrate.c:1526-1590, shake.c:243-338), recording which sweep indices
 // were converged and keeping the states of the first SHK_W sweeps from
 // the first convergence on.
-__global__ void __launch_bounds__(128) k_vv1_a(ShakeArgs A)
+__device__ inline void vv1_a_body(const ShakeArgs &A, int g, unsigned long long (&bits)[SHK_MASKW],
+                                  int &kend, bool &settled)
 {
 #pragma clang fp contract(off)
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= A.nGrp) return;
     Grp G;
     grp_load(A, g, G);
     const double dt = A.P.timeStep;
@@ -271,14 +270,18 @@ __global__ void __launch_bounds__(128) k_vv1_a(ShakeArgs A)
     }
     if (G.nc == 0) {
         grp_store(A, G, true);
+        for (int w = 0; w < SHK_MASKW; w++) bits[w] = ~0ull;
+        kend = -1;
+        settled = true;
         return;
     }
     scr_store(A, G, 0, true);
-    unsigned long long bits[SHK_MASKW] = {0, 0, 0, 0};
+    for (int w = 0; w < SHK_MASKW; w++) bits[w] = 0;
     double dtx[MDG_GRP_CONS], dty[MDG_GRP_CONS], dtz[MDG_GRP_CONS], rt2[MDG_GRP_CONS];
     double vir = 0.;
     int first = -1, k = 0;
-    bool prev = false, settled = false;
+    bool prev = false;
+    settled = false;
     for (;;) {
         const bool cv = shake_r_check(A, G, dtx, dty, dtz, rt2) < A.P.tolShake;
         if (cv) bits[k >> 6] |= 1ull << (k & 63);
@@ -300,7 +303,18 @@ __global__ void __launch_bounds__(128) k_vv1_a(ShakeArgs A)
     A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vir;
     A.first[g] = first;
     A.kend[g] = k;
-    shake_publish_mask(A, bits, k, settled);
+    kend = k;
+}
+
+__global__ void __launch_bounds__(128) k_vv1_a(ShakeArgs A)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane reaches the wave-wide AND; idle lanes contribute all ones
+    unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
+    int kend = -1;
+    bool settled = true;
+    if (g < A.nGrp) vv1_a_body(A, g, bits, kend, settled);
+    shake_publish_mask(A, bits, kend, settled);
 }
 
 // Stage 1, part B: every group takes its state at the global stop sweep C
@@ -347,11 +361,10 @@ __global__ void __launch_bounds__(128) k_vv1_b(ShakeArgs A)
 
 // Stage 2, part A: constraint vectors, half kick, SHAKE-V sweeps with the
 // same bookkeeping (integrate.c:1526-1561, shake.c:359-457).
-__global__ void __launch_bounds__(128) k_vv2_a(ShakeArgs A)
+__device__ inline void vv2_a_body(const ShakeArgs &A, int g, unsigned long long (&bits)[SHK_MASKW],
+                                  int &kend, bool &settled)
 {
 #pragma clang fp contract(off)
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= A.nGrp) return;
     Grp G;
     grp_load(A, g, G);
     const double dt = A.P.timeStep;
@@ -364,13 +377,17 @@ __global__ void __launch_bounds__(128) k_vv2_a(ShakeArgs A)
     }
     if (G.nc == 0) {
         grp_store(A, G, false);
+        for (int w = 0; w < SHK_MASKW; w++) bits[w] = ~0ull;
+        kend = -1;
+        settled = true;
         return;
     }
     scr_store(A, G, 0, false);
     const double tolvel = A.P.tolShake * A.P.rTimeStep;
-    unsigned long long bits[SHK_MASKW] = {0, 0, 0, 0};
+    for (int w = 0; w < SHK_MASKW; w++) bits[w] = 0;
     int first = -1, k = 0;
-    bool prev = false, settled = false;
+    bool prev = false;
+    settled = false;
     for (;;) {
         Grp T = G;
         const bool cv = shake_v_sweep(A, T, false, tolvel, true) < tolvel;   // T = G after a sweep
@@ -389,7 +406,18 @@ __global__ void __launch_bounds__(128) k_vv2_a(ShakeArgs A)
     scr_store(A, G, SHK_W + 1, false);
     A.first[g] = first;
     A.kend[g] = k;
-    shake_publish_mask(A, bits, k, settled);
+    kend = k;
+}
+
+__global__ void __launch_bounds__(128) k_vv2_a(ShakeArgs A)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane reaches the wave-wide AND; idle lanes contribute all ones
+    unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
+    int kend = -1;
+    bool settled = true;
+    if (g < A.nGrp) vv2_a_body(A, g, bits, kend, settled);
+    shake_publish_mask(A, bits, kend, settled);
 }
 
 // Stage 2, part B: state at the global stop sweep, kinetic energy and
@@ -466,7 +494,6 @@ __global__ void __launch_bounds__(128) k_vv2_b(ShakeArgs A)
 
 __global__ void k_ctrl_reset(int *ctrl, unsigned long long *mask)
 {
-    if (threadIdx.x == 0) ctrl[CTRL_SHAKE_MAX] = 0;
     if (threadIdx.x < SHK_MASKW) mask[threadIdx.x] = ~0ull;
 }
 
@@ -543,8 +570,9 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
 }
 
 // Runs one VV stage on the device state with forces fx.. (device, atom
-// order).  Stage 1 leaves the SHAKE virial sum in scal[S_VIRSHAKE]; stage 2
-// leaves kin and the kinetic stress in scal[S_KIN..S_KIN+6].
+// order).  Per-group SHAKE virials go to grpVir (stage 1) and per-block
+// kinetic partials to epart + PART_KIN (stage 2); with sync the sums land
+// in scal[S_VIRSHAKE] and scal[S_KIN..S_KIN+6] and are returned.
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
                  double *virshake_out, bool sync)
 {
@@ -593,21 +621,26 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     const bool w3 = ctx->shakeW3;
     const int bs = w3 ? 64 : 128;
     const int nb = nblk(ctx->nGrp, bs);
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl, ctx->shkMask);
-    LAUNCHCHK();
+    // The device-resident step resets the mask in the kernels that run
+    // before each stage and reduces the partials in its record kernel; the
+    // drop-in call does both here.
+    if (sync) {
+        hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl, ctx->shkMask);
+        LAUNCHCHK();
+    }
     if (stage == 1) {
         hipLaunchKernelGGL(w3 ? k_w3_vv1_a : k_vv1_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         hipLaunchKernelGGL(w3 ? k_w3_vv1_b : k_vv1_b, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         // SHAKE virial: sum of the per-group sums (fixed order)
-        RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, ctx->nGrp, 1, ctx->scal + S_VIRSHAKE));
+        if (sync) RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, ctx->nGrp, 1, ctx->scal + S_VIRSHAKE));
     } else {
         hipLaunchKernelGGL(w3 ? k_w3_vv2_a : k_vv2_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         hipLaunchKernelGGL(w3 ? k_w3_vv2_b : k_vv2_b, dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
-        RETCHK(mdg_reduce_partials(ctx, A.part, nb, 7, ctx->scal + S_KIN));
+        if (sync) RETCHK(mdg_reduce_partials(ctx, A.part, nb, 7, ctx->scal + S_KIN));
     }
     if (sync) {
         int c[CTRL_WORDS];

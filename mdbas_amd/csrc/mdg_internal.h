@@ -120,6 +120,8 @@ struct mdg_ctx {
     // exclusions (CSR; symmetric for link cells, ordered otherwise)
     int *exOff = nullptr, *exLst = nullptr;
     int nExcl = 0;
+    int *exSymOff = nullptr, *exSymLst = nullptr;   // both orientations of every pair
+    int nExSym = 0;
     std::vector<int> hExOff, hExLst;
     bool exclBuilt = false;
 
@@ -205,7 +207,9 @@ enum {
 };
 
 // regions of ctx->epart (per-block partial sums), each PART_R doubles
-#define PART_R (16384 * 8)
+#define PART_R (1 << 18)
+// the cluster pair kernel loops over chunks with at most this many blocks
+#define MDG_PAIR_MAX_BLOCKS 16384
 enum {
     PART_PAIR = 0, PART_EXCL = PART_R, PART_INFL = 2 * PART_R, PART_GATHER = 3 * PART_R,
     PART_KIN = 4 * PART_R, PART_MISC = 5 * PART_R, PART_REF = 6 * PART_R, PART_TOTAL = 8 * PART_R
@@ -237,6 +241,10 @@ enum {
     } while (0)
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+static inline int mdg_pair_blocks(const mdg_ctx *ctx)
+{
+    return std::min(nblk(ctx->cl.maxChunks, 4), MDG_PAIR_MAX_BLOCKS);
+}
 
 // utils.c:606-609: round half away from zero through a truncating cast.
 __host__ __device__ inline int mdg_nint(double x)
@@ -276,6 +284,19 @@ __host__ __device__ inline double mdg_dist(const MdgBox &b, double &dx, double &
         break;
     }
     return dx * dx + dy * dy + dz * dz;
+}
+
+#define SQRTPI 1.772453850905516     // global.h:54
+
+// ewald.c:371-384: erf correction of an excluded pair; returns the energy,
+// dE receives dE/dr.
+__device__ inline double ewald_corr64(double alpha, double qel, double r, double rt, double &dE)
+{
+    double ar = alpha * r;
+    double e = -qel * erf(ar) * rt;
+    double d = 2.0 * qel * alpha * exp(-(ar * ar)) / SQRTPI;
+    dE = -(e + d) * rt;
+    return e;
 }
 
 // Fixed-point conversions (round to nearest).

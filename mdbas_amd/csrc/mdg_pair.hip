@@ -5,32 +5,20 @@
 //    reference format, FP64, one thread per i in neighOrder order.
 //  * k_excl64: the excluded-pair erf correction (energy.c:567-621).
 //  * k_pair_cluster<MIXED>: the same physics over the cluster-pair list,
-//    one 64-lane wavefront per i-cluster, lane (a, b) = pair (i_a, j_b) of
-//    each 8x8 cluster pair.  i forces stay in registers across the whole
-//    j list and are reduced once over b; j forces are reduced over a with
-//    cross-lane shuffles after each cluster pair and added with one
-//    coalesced 64-byte atomic row per component.  MIXED = FP32 pair
+//    one 64-lane wavefront per chunk (up to 32 j-clusters of one
+//    i-cluster), lane (a, b) = pair (i_a, j_b) of each 8x8 cluster pair.
+//    i forces stay in registers across the chunk and are reduced once over
+//    b; j forces are reduced over a with DPP moves after each cluster pair
+//    and added by one lane per j atom.  MIXED = FP32 pair
 //    arithmetic (displacements formed in FP64 first) with int64
 //    fixed-point force accumulation; otherwise FP64 throughout.
 #include "mdg_internal.h"
-
-#define SQRTPI 1.772453850905516     // global.h:54
 
 // ewald.c:356-369
 __device__ inline double ewald_dir64(double alpha, double qel, double r, double rt, double &dE)
 {
     double ar = alpha * r;
     double e = qel * erfc(ar) * rt;
-    double d = 2.0 * qel * alpha * exp(-(ar * ar)) / SQRTPI;
-    dE = -(e + d) * rt;
-    return e;
-}
-
-// ewald.c:371-384
-__device__ inline double ewald_corr64(double alpha, double qel, double r, double rt, double &dE)
-{
-    double ar = alpha * r;
-    double e = -qel * erf(ar) * rt;
     double d = 2.0 * qel * alpha * exp(-(ar * ar)) / SQRTPI;
     dE = -(e + d) * rt;
     return e;
@@ -253,8 +241,8 @@ __device__ inline float mdg_erfc_f32(float x, float *ez)
     p = fmaf(t, p, 1.00002368f);
     p = fmaf(t, p, -1.26551223f);
     float x2 = x * x;
-    *ez = expf(-x2);
-    return t * expf(p - x2);
+    *ez = __expf(-x2);
+    return t * __expf(p - x2);
 }
 
 struct ClusterArgs {
@@ -274,14 +262,51 @@ struct ClusterArgs {
     double *part;                    // per block: eDir, eVdw, virDir, virVdw, nIn
 };
 
+// Sum over the 8 lanes of each aligned group of 8 with DPP moves (plain
+// VALU ops, no LDS): quad_perm [1,0,3,2], quad_perm [2,3,0,1], then
+// row_half_mirror (lane i <- lane 7-i within the half row).
+__device__ inline float dpp_sum8(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));
+    return v;
+}
+
+__device__ inline double dpp_sum8(double v)
+{
+    auto mv = [](double x, int ctrl) {
+        long long b = __double_as_longlong(x);
+        int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+        switch (ctrl) {
+        case 0: lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, true);
+                hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, true); break;
+        case 1: lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, true);
+                hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, true); break;
+        default: lo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, true);
+                 hi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, true); break;
+        }
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    };
+    v += mv(v, 0);
+    v += mv(v, 1);
+    v += mv(v, 2);
+    return v;
+}
+
+// Lane mapping: a = lane & 7 is the i atom, b = lane >> 3 the j atom, so
+// each j-cluster pair's j-force sums run over 8 adjacent lanes (DPP) and
+// the i-force sums over lanes a, a+8, ..., a+56 happen once per chunk.
+// Mask bit a*8+b marks the listed pair (I_a, J_b).
 template <bool MIXED>
 __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
 {
     const int lane = threadIdx.x & 63;
-    const int W = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int a = lane >> 3, b = lane & 7;
+    const int a = lane & 7, b = lane >> 3;
+    const int bit = (a << 3) | b;
     double e_acc[5] = {0, 0, 0, 0, 0};
-    if (W < A.chOff[A.nclu]) {
+    const int nch = A.chOff[A.nclu];
+    for (int W = blockIdx.x * 4 + (threadIdx.x >> 6); W < nch; W += gridDim.x * 4) {
         // chunk W: entries [e0, e1) of i-cluster I (at most MDG_CHUNK of them)
         const int2 ch = A.chunk[W];
         const int I = ch.x, e0 = ch.y, e1 = min(e0 + MDG_CHUNK, A.cpOff[I + 1]);
@@ -330,21 +355,20 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
             } else {
                 r2 = mdg_dist(A.B, dx, dy, dz);
             }
-            const bool in = ((m >> lane) & 1ull) && r2 <= A.P.cutOff2;
+            const bool in = ((m >> bit) & 1ull) && r2 <= A.P.cutOff2;
             if (MIXED) {
                 float fjx = 0.f, fjy = 0.f, fjz = 0.f;
                 if (in) {
-                    const float4 pj = pjc;
                     const float r2f = (float)r2;
                     const float rinv = __builtin_amdgcn_rsqf(r2f);
                     const float r = r2f * rinv;
-                    const float qq = A.F.qc * qi * pj.x;
+                    const float qq = A.F.qc * qi * pjc.x;
                     float ez;
                     const float erfcv = mdg_erfc_f32(A.F.alpha * r, &ez);
                     const float eel = qq * erfcv * rinv;
                     const float del = -(eel + qq * A.F.c2a * ez) * rinv;
-                    const float veps = ei * pj.y;
-                    const float s = (gi + pj.z) * rinv;
+                    const float veps = ei * pjc.y;
+                    const float s = (gi + pjc.z) * rinv;
                     const float s2 = s * s;
                     const float s6 = s2 * s2 * s2;
                     const float s12 = s6 * s6;
@@ -371,17 +395,10 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
                 fxi += fjx;
                 fyi += fjy;
                 fzi += fjz;
-                // j side: sum over the 8 i lanes (a), negate
-                fjx += __shfl_xor(fjx, 8, 64);
-                fjy += __shfl_xor(fjy, 8, 64);
-                fjz += __shfl_xor(fjz, 8, 64);
-                fjx += __shfl_xor(fjx, 16, 64);
-                fjy += __shfl_xor(fjy, 16, 64);
-                fjz += __shfl_xor(fjz, 16, 64);
-                fjx += __shfl_xor(fjx, 32, 64);
-                fjy += __shfl_xor(fjy, 32, 64);
-                fjz += __shfl_xor(fjz, 32, 64);
-                if (a == 0 && (m >> 0) != 0ull) {
+                fjx = dpp_sum8(fjx);
+                fjy = dpp_sum8(fjy);
+                fjz = dpp_sum8(fjz);
+                if (a == 0) {
                     mdg_atomic_add_fix(&A.fix[sj], -(double)fjx, MDG_FSCALE);
                     mdg_atomic_add_fix(&A.fiy[sj], -(double)fjy, MDG_FSCALE);
                     mdg_atomic_add_fix(&A.fiz[sj], -(double)fjz, MDG_FSCALE);
@@ -408,15 +425,9 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
                 fxid += fjx;
                 fyid += fjy;
                 fzid += fjz;
-                fjx += __shfl_xor(fjx, 8, 64);
-                fjy += __shfl_xor(fjy, 8, 64);
-                fjz += __shfl_xor(fjz, 8, 64);
-                fjx += __shfl_xor(fjx, 16, 64);
-                fjy += __shfl_xor(fjy, 16, 64);
-                fjz += __shfl_xor(fjz, 16, 64);
-                fjx += __shfl_xor(fjx, 32, 64);
-                fjy += __shfl_xor(fjy, 32, 64);
-                fjz += __shfl_xor(fjz, 32, 64);
+                fjx = dpp_sum8(fjx);
+                fjy = dpp_sum8(fjy);
+                fjz = dpp_sum8(fjz);
                 if (a == 0) {
                     if (fjx != 0.) atomicAdd(&A.fdx[sj], -fjx);
                     if (fjy != 0.) atomicAdd(&A.fdy[sj], -fjy);
@@ -426,35 +437,27 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
         }
         if (MIXED) {
             double gx = fxi, gy = fyi, gz = fzi;
-            gx += __shfl_xor(gx, 1, 64);
-            gy += __shfl_xor(gy, 1, 64);
-            gz += __shfl_xor(gz, 1, 64);
-            gx += __shfl_xor(gx, 2, 64);
-            gy += __shfl_xor(gy, 2, 64);
-            gz += __shfl_xor(gz, 2, 64);
-            gx += __shfl_xor(gx, 4, 64);
-            gy += __shfl_xor(gy, 4, 64);
-            gz += __shfl_xor(gz, 4, 64);
+            for (int o = 8; o < 64; o <<= 1) {
+                gx += __shfl_xor(gx, o, 64);
+                gy += __shfl_xor(gy, o, 64);
+                gz += __shfl_xor(gz, o, 64);
+            }
             if (b == 0) {
                 mdg_atomic_add_fix(&A.fix[si], gx, MDG_FSCALE);
                 mdg_atomic_add_fix(&A.fiy[si], gy, MDG_FSCALE);
                 mdg_atomic_add_fix(&A.fiz[si], gz, MDG_FSCALE);
             }
-            e_acc[0] = ed;
-            e_acc[1] = ev;
-            e_acc[2] = vd;
-            e_acc[3] = vv;
-            e_acc[4] = nin;
+            e_acc[0] += ed;
+            e_acc[1] += ev;
+            e_acc[2] += vd;
+            e_acc[3] += vv;
+            e_acc[4] += nin;
         } else {
-            fxid += __shfl_xor(fxid, 1, 64);
-            fyid += __shfl_xor(fyid, 1, 64);
-            fzid += __shfl_xor(fzid, 1, 64);
-            fxid += __shfl_xor(fxid, 2, 64);
-            fyid += __shfl_xor(fyid, 2, 64);
-            fzid += __shfl_xor(fzid, 2, 64);
-            fxid += __shfl_xor(fxid, 4, 64);
-            fyid += __shfl_xor(fyid, 4, 64);
-            fzid += __shfl_xor(fzid, 4, 64);
+            for (int o = 8; o < 64; o <<= 1) {
+                fxid += __shfl_xor(fxid, o, 64);
+                fyid += __shfl_xor(fyid, o, 64);
+                fzid += __shfl_xor(fzid, o, 64);
+            }
             if (b == 0) {
                 atomicAdd(&A.fdx[si], fxid);
                 atomicAdd(&A.fdy[si], fyid);
@@ -465,63 +468,15 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
     mdg_block_partials<5>(e_acc, A.part);
 }
 
-// Excluded-pair erf correction on the device-resident path (FP64 math in
-// both precisions), forces into the slot accumulators.  Per block:
-// [eCorr, virCorr].
-template <bool MIXED>
-__global__ void __launch_bounds__(256)
-k_excl_dev(MdgParams P, MdgBox B, const double *__restrict__ x, const double *__restrict__ y,
-           const double *__restrict__ z, const double *__restrict__ q, const int *__restrict__ exOff,
-           const int *__restrict__ exLst, const int *__restrict__ atomSlot, long long *fix,
-           long long *fiy, long long *fiz, double *fdx, double *fdy, double *fdz,
-           double *__restrict__ part)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[2] = {0, 0};
-    if (i < P.n) {
-        double fxi = 0., fyi = 0., fzi = 0.;
-        for (int k = exOff[i]; k < exOff[i + 1]; k++) {
-            int j = exLst[k];
-            if (j <= i) continue;
-            double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
-            double r2 = mdg_dist(B, dx, dy, dz);
-            double r = sqrt(r2), rt = 1. / r, dC;
-            acc[0] += ewald_corr64(P.alpha, P.chargeConst * q[i] * q[j], r, rt, dC);
-            acc[1] += dC * r;
-            double fxj = dC * dx * rt, fyj = dC * dy * rt, fzj = dC * dz * rt;
-            fxi += fxj;
-            fyi += fyj;
-            fzi += fzj;
-            int sj = atomSlot[j];
-            if (MIXED) {
-                mdg_atomic_add_fix(&fix[sj], -fxj, MDG_FSCALE);
-                mdg_atomic_add_fix(&fiy[sj], -fyj, MDG_FSCALE);
-                mdg_atomic_add_fix(&fiz[sj], -fzj, MDG_FSCALE);
-            } else {
-                atomicAdd(&fdx[sj], -fxj);
-                atomicAdd(&fdy[sj], -fyj);
-                atomicAdd(&fdz[sj], -fzj);
-            }
-        }
-        int si = atomSlot[i];
-        if (MIXED) {
-            mdg_atomic_add_fix(&fix[si], fxi, MDG_FSCALE);
-            mdg_atomic_add_fix(&fiy[si], fyi, MDG_FSCALE);
-            mdg_atomic_add_fix(&fiz[si], fzi, MDG_FSCALE);
-        } else if (fxi != 0. || fyi != 0. || fzi != 0.) {
-            atomicAdd(&fdx[si], fxi);
-            atomicAdd(&fdy[si], fyi);
-            atomicAdd(&fdz[si], fzi);
-        }
-    }
-    mdg_block_partials<2>(acc, part);
-}
-
 // Slot-ordered positions for the pair kernel (padding slots get copies of
-// a far point; their mask bits are never set).
+// a far point; their mask bits are never set), and zeroed slot force
+// accumulators of the active precision.
+template <bool MIXED>
 __global__ void k_slot_pos(int nslot, const int *__restrict__ slotAtom, const double *__restrict__ x,
                            const double *__restrict__ y, const double *__restrict__ z,
-                           double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz)
+                           double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz,
+                           long long *fix, long long *fiy, long long *fiz, double *fdx, double *fdy,
+                           double *fdz)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot) return;
@@ -529,6 +484,15 @@ __global__ void k_slot_pos(int nslot, const int *__restrict__ slotAtom, const do
     sx[s] = a >= 0 ? x[a] : 1e7;
     sy[s] = a >= 0 ? y[a] : 1e7;
     sz[s] = a >= 0 ? z[a] : 1e7;
+    if (MIXED) {
+        fix[s] = 0;
+        fiy[s] = 0;
+        fiz[s] = 0;
+    } else {
+        fdx[s] = 0.;
+        fdy[s] = 0.;
+        fdz[s] = 0.;
+    }
 }
 
 __global__ void k_slot_param(int nslot, const int *__restrict__ slotAtom,
@@ -562,8 +526,14 @@ int mdg_pair_cluster(mdg_ctx *ctx)
 {
     ClusterList &C = ctx->cl;
     const int nslot = C.nclu * MDG_CL;
-    hipLaunchKernelGGL(k_slot_pos, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream, nslot,
-                       C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz);
+    if (ctx->precision == MDG_MIXED)
+        hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
+    else
+        hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
     LAUNCHCHK();
     ClusterArgs A;
     A.P = ctx->P;
@@ -602,7 +572,7 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     A.fdy = ctx->fdy;
     A.fdz = ctx->fdz;
     A.part = ctx->epart + PART_PAIR;
-    const int nb = nblk(C.maxChunks, 4);
+    const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     if (ctx->precision == MDG_MIXED)
         hipLaunchKernelGGL(k_pair_cluster<true>, dim3(nb), dim3(256), 0, ctx->stream, A);
@@ -610,20 +580,5 @@ int mdg_pair_cluster(mdg_ctx *ctx)
         hipLaunchKernelGGL(k_pair_cluster<false>, dim3(nb), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
-    RETCHK(mdg_reduce_partials(ctx, A.part, nb, 5, ctx->scal + S_EDIR));
-    const int n = ctx->n;
-    const int nbe = nblk(n, 256);
-    double *pe = ctx->epart + PART_EXCL;
-    if (ctx->precision == MDG_MIXED)
-        hipLaunchKernelGGL(k_excl_dev<true>, dim3(nbe), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
-                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->exOff, ctx->exLst, C.atomSlot,
-                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, pe);
-    else
-        hipLaunchKernelGGL(k_excl_dev<false>, dim3(nbe), dim3(256), 0, ctx->stream, ctx->P,
-                           ctx->B, ctx->x, ctx->y, ctx->z, ctx->q, ctx->exOff, ctx->exLst,
-                           C.atomSlot, ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz,
-                           pe);
-    LAUNCHCHK();
-    RETCHK(mdg_reduce_partials(ctx, pe, nbe, 2, ctx->scal + S_ECORR));
     return MDG_OK;
 }

@@ -9,59 +9,83 @@
 
 #define TWOPI_R 6.283185307179586
 
-// F = pair/correction slot forces + reciprocal forces - their mean
-// (spme.c:648-670 removes the mean of the reciprocal part only, because
-// energy() zeroes the arrays before SPME runs).  Frozen atoms get zero
-// force and velocity (utils.c:131-153).  Slot accumulators are cleared
-// for the next step.
-template <bool MIXED>
-__global__ void k_finalize(int n, const int *__restrict__ atomSlot, long long *fix, long long *fiy,
-                           long long *fiz, double *fdx, double *fdy, double *fdz,
-                           const float *__restrict__ frx, const float *__restrict__ fry,
-                           const float *__restrict__ frz, const double *__restrict__ frx64,
-                           const double *__restrict__ fry64, const double *__restrict__ frz64,
-                           const double *__restrict__ scal, const int *__restrict__ frozen,
-                           int hasFrozen, double *fx, double *fy, double *fz, double *vx,
-                           double *vy, double *vz)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = atomSlot[i];
-    const double mx = scal[S_FRX] / (double)n, my = scal[S_FRY] / (double)n,
-                 mz = scal[S_FRZ] / (double)n;
-    double gx, gy, gz;
-    if (MIXED) {
-        gx = (double)fix[s] * (1.0 / MDG_FSCALE) + ((double)frx[i] - mx);
-        gy = (double)fiy[s] * (1.0 / MDG_FSCALE) + ((double)fry[i] - my);
-        gz = (double)fiz[s] * (1.0 / MDG_FSCALE) + ((double)frz[i] - mz);
-    } else {
-        gx = (frx64[i] - mx) + fdx[s];
-        gy = (fry64[i] - my) + fdy[s];
-        gz = (frz64[i] - mz) + fdz[s];
-    }
-    if (hasFrozen && frozen[i]) {
-        gx = gy = gz = 0.;
-        vx[i] = vy[i] = vz[i] = 0.;
-    }
-    fx[i] = gx;
-    fy[i] = gy;
-    fz[i] = gz;
-}
+struct FinArgs {
+    MdgParams P;
+    MdgBox B;
+    int n, hasFrozen, nbGather;
+    const int *atomSlot;
+    const long long *fix, *fiy, *fiz;
+    const double *fdx, *fdy, *fdz;
+    const float *frx, *fry, *frz;
+    const double *frx64, *fry64, *frz64;
+    const double *gpart;                 // per-block sums of the reciprocal forces
+    const int *exOff, *exLst;            // symmetric exclusion table
+    const double *x, *y, *z, *q;
+    const int *frozen;
+    double *fx, *fy, *fz, *vx, *vy, *vz;
+    double *part;                        // per block: eCorr, virCorr
+    unsigned long long *mask;            // SHAKE mask, reset for VV stage 2
+};
 
-__global__ void k_clear_slots(int nslot, long long *fix, long long *fiy, long long *fiz,
-                              double *fdx, double *fdy, double *fdz, int mixed)
+// Per atom: F = slot pair force + (reciprocal force - mean reciprocal
+// force) + excluded-pair corrections.  The mean is over the reciprocal
+// part only (spme.c:648-670 runs on arrays energy() has just zeroed).
+// The correction of each excluded pair (energy.c:567-621) is evaluated
+// from both of its atoms with the reference's orientation (delta = x_hi -
+// x_lo), so every atom sums its own terms without atomics; energy and
+// virial count each pair once.  Frozen atoms get zero force and velocity
+// (utils.c:131-153).
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_finalize(FinArgs A)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslot) return;
-    if (mixed) {
-        fix[s] = 0;
-        fiy[s] = 0;
-        fiz[s] = 0;
-    } else {
-        fdx[s] = 0.;
-        fdy[s] = 0.;
-        fdz[s] = 0.;
+    __shared__ double mean[3];
+    if (threadIdx.x < 3) {
+        double s = 0.;
+        for (int b = 0; b < A.nbGather; b++) s += A.gpart[(long)b * 3 + threadIdx.x];
+        mean[threadIdx.x] = s / (double)A.n;
     }
+    if (blockIdx.x == 0 && threadIdx.x < 4) A.mask[threadIdx.x] = ~0ull;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < A.n) {
+        const int s = A.atomSlot[i];
+        double gx, gy, gz;
+        if (MIXED) {
+            gx = (double)A.fix[s] * (1.0 / MDG_FSCALE) + ((double)A.frx[i] - mean[0]);
+            gy = (double)A.fiy[s] * (1.0 / MDG_FSCALE) + ((double)A.fry[i] - mean[1]);
+            gz = (double)A.fiz[s] * (1.0 / MDG_FSCALE) + ((double)A.frz[i] - mean[2]);
+        } else {
+            gx = (A.frx64[i] - mean[0]) + A.fdx[s];
+            gy = (A.fry64[i] - mean[1]) + A.fdy[s];
+            gz = (A.frz64[i] - mean[2]) + A.fdz[s];
+        }
+        for (int k = A.exOff[i]; k < A.exOff[i + 1]; k++) {
+            const int j = A.exLst[k];
+            const int lo = min(i, j), hi = max(i, j);
+            double dx = A.x[hi] - A.x[lo], dy = A.y[hi] - A.y[lo], dz = A.z[hi] - A.z[lo];
+            const double r2 = mdg_dist(A.B, dx, dy, dz);
+            const double r = sqrt(r2), rt = 1. / r;
+            double dC;
+            const double e = ewald_corr64(A.P.alpha, A.P.chargeConst * A.q[lo] * A.q[hi], r, rt, dC);
+            const double sgn = (i == lo) ? 1.0 : -1.0;
+            gx += sgn * (dC * dx * rt);
+            gy += sgn * (dC * dy * rt);
+            gz += sgn * (dC * dz * rt);
+            if (i == lo) {
+                acc[0] += e;
+                acc[1] += dC * r;
+            }
+        }
+        if (A.hasFrozen && A.frozen[i]) {
+            gx = gy = gz = 0.;
+            A.vx[i] = A.vy[i] = A.vz[i] = 0.;
+        }
+        A.fx[i] = gx;
+        A.fy[i] = gy;
+        A.fz[i] = gz;
+    }
+    mdg_block_partials<2>(acc, A.part);
 }
 
 __global__ void k_kinetic(int n, const double *__restrict__ mass, const double *__restrict__ vx,
@@ -77,36 +101,65 @@ __global__ void k_kinetic(int n, const double *__restrict__ mass, const double *
 struct RecArgs {
     double fact, eSelf, eNet;
     int recCap;
+    // per-block partial counts of this step's producers
+    int nbPair, nbExcl, nbInfl, nbKin, nGrpVir;
+    const double *pPair, *pExcl, *pInfl, *pKin, *grpVir;
+    unsigned long long *mask;
 };
 
-// One MDG_STEP record from the reduced scalars (spme.c:674-697 for the
-// reciprocal energy and virial, energy.c:623-627 and 193-196 for sums).
-__global__ void k_record(RecArgs R, const double *__restrict__ scal, int *ctrl, MDG_STEP *rec,
-                         int stepNo, int rebuilt)
+// Sums rows [0, nrow) of column c of a row-major partial table with one
+// block (fixed order, so deterministic).
+__device__ inline double block_col_sum(const double *p, int nrow, int ncol, int c, double *red)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.;
+    for (int r = threadIdx.x; r < nrow; r += blockDim.x) s += p[(long)r * ncol + c];
+    s = mdg_wave_sum(s);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = 0.;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+    return t;
+}
+
+// One MDG_STEP record: reduces the step's partials (pair, correction,
+// influence function, kinetic energy, SHAKE virial) and assembles the
+// energies (spme.c:674-697 for the reciprocal energy and virial,
+// energy.c:623-627 and 193-196 for the sums); resets the SHAKE mask for
+// the next step's VV stage 1.
+__global__ void __launch_bounds__(256) k_record(RecArgs R, int *ctrl, MDG_STEP *rec, int rebuilt)
+{
+    __shared__ double red[4];
+    double v[22];
+    for (int c = 0; c < 5; c++) v[c] = block_col_sum(R.pPair, R.nbPair, 5, c, red);
+    for (int c = 0; c < 2; c++) v[5 + c] = block_col_sum(R.pExcl, R.nbExcl, 2, c, red);
+    for (int c = 0; c < 7; c++) v[7 + c] = block_col_sum(R.pInfl, R.nbInfl, 7, c, red);
+    v[14] = block_col_sum(R.pKin, R.nbKin, 7, 0, red);
+    v[15] = R.nGrpVir ? block_col_sum(R.grpVir, R.nGrpVir, 1, 0, red) : 0.;
+    if (threadIdx.x < 4) R.mask[threadIdx.x] = ~0ull;
+    if (threadIdx.x != 0) return;
     const int idx = ctrl[CTRL_STEP];
-    const double er = scal[S_EREC];
-    const double st0 = R.fact * (scal[S_EREC + 1] + er) + R.eNet;
-    const double st3 = R.fact * (scal[S_EREC + 4] + er) + R.eNet;
-    const double st5 = R.fact * (scal[S_EREC + 6] + er) + R.eNet;
+    const double er = v[7];
+    const double st0 = R.fact * (v[8] + er) + R.eNet;
+    const double st3 = R.fact * (v[11] + er) + R.eNet;
+    const double st5 = R.fact * (v[13] + er) + R.eNet;
     const double eRec = R.fact * er + R.eSelf + R.eNet;
     const double virRec = -(st0 + st3 + st5);
     MDG_STEP s;
-    s.step = stepNo + idx;
+    s.step = idx;
     s.listRebuilt = rebuilt;
-    s.eDir = scal[S_EDIR];
+    s.eDir = v[0];
     s.eRec = eRec;
-    s.eCorr = scal[S_ECORR];
+    s.eCorr = v[5];
     s.elec = s.eDir + eRec + s.eCorr;
-    s.vdw = scal[S_EVDW];
+    s.vdw = v[1];
     s.pot = s.elec + s.vdw;
-    s.kin = 0.5 * scal[S_KIN];
+    s.kin = 0.5 * v[14];
     s.tot = s.pot + s.kin;
-    s.virelec = scal[S_VIRDIR] + virRec + scal[S_VIRCORR];
-    s.virvdw = scal[S_VIRVDW];
-    s.virshake = scal[S_VIRSHAKE];
-    s.nInCut = scal[S_NIN];
+    s.virelec = v[2] + virRec + v[6];
+    s.virvdw = v[3];
+    s.virshake = v[15];
+    s.nInCut = v[4];
     if (idx < R.recCap) rec[idx] = s;
     ctrl[CTRL_STEP] = idx + 1;
 }
@@ -128,11 +181,6 @@ static int ensure_slots(mdg_ctx *ctx)
     HIPCHK(hipMalloc(&ctx->fix, ns * sizeof(long long)));
     HIPCHK(hipMalloc(&ctx->fiy, ns * sizeof(long long)));
     HIPCHK(hipMalloc(&ctx->fiz, ns * sizeof(long long)));
-    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, ctx->stream, ns, ctx->fix,
-                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, 1);
-    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, ctx->stream, ns, ctx->fix,
-                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, 0);
-    LAUNCHCHK();
     return MDG_OK;
 }
 
@@ -166,33 +214,67 @@ static int dev_energy(mdg_ctx *ctx, bool rebuild)
     RETCHK(mdg_spme_device(ctx));
     RETCHK(mdg_pair_cluster(ctx));
     const bool mixed = ctx->precision == MDG_MIXED;
+    FinArgs F;
+    F.P = ctx->P;
+    F.B = ctx->B;
+    F.n = n;
+    F.hasFrozen = ctx->hasFrozen ? 1 : 0;
+    F.nbGather = nblk(n, 256);
+    F.atomSlot = ctx->cl.atomSlot;
+    F.fix = ctx->fix;
+    F.fiy = ctx->fiy;
+    F.fiz = ctx->fiz;
+    F.fdx = ctx->fdx;
+    F.fdy = ctx->fdy;
+    F.fdz = ctx->fdz;
+    F.frx = ctx->frx;
+    F.fry = ctx->fry;
+    F.frz = ctx->frz;
+    F.frx64 = ctx->frx64;
+    F.fry64 = ctx->fry64;
+    F.frz64 = ctx->frz64;
+    F.gpart = ctx->epart + PART_GATHER;
+    F.exOff = ctx->exSymOff;
+    F.exLst = ctx->exSymLst;
+    F.x = ctx->x;
+    F.y = ctx->y;
+    F.z = ctx->z;
+    F.q = ctx->q;
+    F.frozen = ctx->frozen;
+    F.fx = ctx->fx;
+    F.fy = ctx->fy;
+    F.fz = ctx->fz;
+    F.vx = ctx->vx;
+    F.vy = ctx->vy;
+    F.vz = ctx->vz;
+    F.part = ctx->epart + PART_EXCL;
+    F.mask = ctx->shkMask;
     if (mixed)
-        hipLaunchKernelGGL(k_finalize<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
-                           ctx->cl.atomSlot, ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy,
-                           ctx->fdz, ctx->frx, ctx->fry, ctx->frz, ctx->frx64, ctx->fry64,
-                           ctx->frz64, ctx->scal, ctx->frozen, ctx->hasFrozen ? 1 : 0, ctx->fx,
-                           ctx->fy, ctx->fz, ctx->vx, ctx->vy, ctx->vz);
+        hipLaunchKernelGGL(k_finalize<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
     else
-        hipLaunchKernelGGL(k_finalize<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
-                           ctx->cl.atomSlot, ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy,
-                           ctx->fdz, ctx->frx, ctx->fry, ctx->frz, ctx->frx64, ctx->fry64,
-                           ctx->frz64, ctx->scal, ctx->frozen, ctx->hasFrozen ? 1 : 0, ctx->fx,
-                           ctx->fy, ctx->fz, ctx->vx, ctx->vy, ctx->vz);
-    LAUNCHCHK();
-    const int ns = ctx->cl.nclu * MDG_CL;
-    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, ctx->stream, ns, ctx->fix,
-                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, mixed ? 1 : 0);
+        hipLaunchKernelGGL(k_finalize<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
     LAUNCHCHK();
     return MDG_OK;
 }
 
-static RecArgs rec_args(mdg_ctx *ctx)
+static RecArgs rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
 {
     RecArgs R;
     R.fact = (TWOPI_R / ctx->B.vol) * ctx->P.chargeConst;
     R.eSelf = ctx->eSelf;
     R.eNet = ctx->eNet;
     R.recCap = ctx->recCap;
+    R.nbPair = mdg_pair_blocks(ctx);
+    R.nbExcl = nblk(ctx->n, 256);
+    R.nbInfl = nblk((long)ctx->m1 * ctx->m2 * ctx->mh, 256);
+    R.nbKin = nbKin;
+    R.nGrpVir = nGrpVir;
+    R.pPair = ctx->epart + PART_PAIR;
+    R.pExcl = ctx->epart + PART_EXCL;
+    R.pInfl = ctx->epart + PART_INFL;
+    R.pKin = ctx->epart + PART_KIN;
+    R.grpVir = ctx->grpVir;
+    R.mask = ctx->shkMask;
     return R;
 }
 
@@ -205,8 +287,9 @@ static int dev_step(mdg_ctx *ctx, bool rebuild, int stepBase)
     RETCHK(dev_energy(ctx, rebuild));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[10], ctx->stream));
     RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, ctx->stream, rec_args(ctx), ctx->scal,
-                       ctx->ctrl, ctx->recDev, stepBase, rebuild ? 1 : 0);
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, ctx->stream,
+                       rec_args(ctx, nblk(ctx->nGrp, ctx->shakeW3 ? 64 : 128), ctx->nGrp),
+                       ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
     return MDG_OK;
@@ -241,6 +324,11 @@ __global__ void k_set_int(int *p, int v)
     if (threadIdx.x == 0) *p = v;
 }
 
+__global__ void k_reset_mask(unsigned long long *mask)
+{
+    if (threadIdx.x < 4) mask[threadIdx.x] = ~0ull;
+}
+
 extern "C" int mdg_upload_state(mdg_ctx *ctx, const double x[], const double y[],
                                 const double z[], const double vx[], const double vy[],
                                 const double vz[])
@@ -273,17 +361,15 @@ extern "C" int mdg_download_state(mdg_ctx *ctx, double x[], double y[], double z
 
 extern "C" int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec)
 {
-    if (!ctx || !ctx->theta || !ctx->grpAtomOff) return MDG_EARG;
+    if (!ctx || !ctx->theta || !ctx->grpAtomOff || !ctx->exSymOff) return MDG_EARG;
     RETCHK(ensure_records(ctx, 64));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
     RETCHK(dev_energy(ctx, true));
     hipLaunchKernelGGL(k_kinetic, dim3(nblk(ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n,
                        ctx->mass, ctx->vx, ctx->vy, ctx->vz, ctx->epart + PART_KIN);
     LAUNCHCHK();
-    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_KIN, nblk(ctx->n, 256), 7, ctx->scal + S_KIN));
-    RETCHK(mdg_fill_f64(ctx, ctx->scal + S_VIRSHAKE, 1, 0.0));
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, ctx->stream, rec_args(ctx), ctx->scal,
-                       ctx->ctrl, ctx->recDev, ctx->step, 1);
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, ctx->stream,
+                       rec_args(ctx, nblk(ctx->n, 256), 0), ctx->ctrl, ctx->recDev, 1);
     LAUNCHCHK();
     RETCHK(check_flags(ctx));
     if (rec) HIPCHK(hipMemcpy(rec, ctx->recDev, sizeof(MDG_STEP), hipMemcpyDeviceToHost));
@@ -340,6 +426,10 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
     }
     RETCHK(ensure_records(ctx, nsteps));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
+    LAUNCHCHK();
+    // the step sequence expects a fresh SHAKE mask for its first stage 1
+    // (each step's record kernel resets it for the next)
+    hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, ctx->stream, ctx->shkMask);
     LAUNCHCHK();
     const bool prof = ctx->profiling;
     for (int k = 0; k < 5; k++) ctx->phase[k] = 0.;

@@ -34,14 +34,17 @@ struct ShakeArgs {
 // AND mask: bits for k <= kend are as checked; later sweeps count as
 // converged when the group was converged at its last two checks.
 __device__ inline void shake_publish_mask(const ShakeArgs &A, const unsigned long long (&bits)[SHK_MASKW],
-                                    int kend, bool settled)
+                                          int kend, bool settled)
 {
+    // AND within the wavefront first, then one atomic per wave and word
+    // (all-ones words change nothing and are skipped)
     for (int w = 0; w < SHK_MASKW; w++) {
         unsigned long long m = bits[w];
         if (settled)
             for (int b = 0; b < 64; b++)
                 if (w * 64 + b > kend && w * 64 + b < A.P.maxCycle) m |= 1ull << b;
-        atomicAnd(&A.mask[w], m);
+        for (int o = 32; o > 0; o >>= 1) m &= __shfl_xor(m, o, 64);
+        if ((threadIdx.x & 63) == 0 && m != ~0ull) atomicAnd(&A.mask[w], m);
     }
 }
 

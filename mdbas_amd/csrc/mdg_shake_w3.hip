@@ -266,11 +266,10 @@ __device__ inline void w3_kick(const ShakeArgs &A, W3 &G)
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) k_w3_vv1_a(ShakeArgs A)
+__device__ inline void w3_vv1_a_body(const ShakeArgs &A, int g, unsigned long long (&bits)[SHK_MASKW],
+                                  int &kend, bool &settled)
 {
 #pragma clang fp contract(off)
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= A.nGrp) return;
     W3 G;
     w3_load(A, g, G);
     const double dt = A.P.timeStep;
@@ -283,11 +282,12 @@ __global__ void __launch_bounds__(64) k_w3_vv1_a(ShakeArgs A)
         G.z[k] += dt * G.vz[k];
     }
     w3_scr_store(A, G, 0, true);
-    unsigned long long bits[SHK_MASKW] = {0, 0, 0, 0};
+    for (int w = 0; w < SHK_MASKW; w++) bits[w] = 0;
     double dtx[3], dty[3], dtz[3], rt2[3];
     double vir = 0.;
     int first = -1, k = 0;
-    bool prev = false, settled = false;
+    bool prev = false;
+    settled = false;
     for (;;) {
         const bool cv = w3_check(A, G, dtx, dty, dtz, rt2) < A.P.tolShake;
         if (cv) bits[k >> 6] |= 1ull << (k & 63);
@@ -309,7 +309,18 @@ __global__ void __launch_bounds__(64) k_w3_vv1_a(ShakeArgs A)
     A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vir;
     A.first[g] = first;
     A.kend[g] = k;
-    shake_publish_mask(A, bits, k, settled);
+    kend = k;
+}
+
+__global__ void __launch_bounds__(64) k_w3_vv1_a(ShakeArgs A)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane reaches the wave-wide AND; idle lanes contribute all ones
+    unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
+    int kend = -1;
+    bool settled = true;
+    if (g < A.nGrp) w3_vv1_a_body(A, g, bits, kend, settled);
+    shake_publish_mask(A, bits, kend, settled);
 }
 
 __global__ void __launch_bounds__(64) k_w3_vv1_b(ShakeArgs A)
@@ -338,10 +349,9 @@ __global__ void __launch_bounds__(64) k_w3_vv1_b(ShakeArgs A)
     w3_store(A, G, true);
 }
 
-__global__ void __launch_bounds__(64) k_w3_vv2_a(ShakeArgs A)
+__device__ inline void w3_vv2_a_body(const ShakeArgs &A, int g, unsigned long long (&bits)[SHK_MASKW],
+                                  int &kend, bool &settled)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= A.nGrp) return;
     W3 G;
     w3_load(A, g, G);
     w3_dd(A, G);
@@ -350,9 +360,10 @@ __global__ void __launch_bounds__(64) k_w3_vv2_a(ShakeArgs A)
     double den[3];
     w3_den(A, G, den);
     const double tolvel = A.P.tolShake * A.P.rTimeStep;
-    unsigned long long bits[SHK_MASKW] = {0, 0, 0, 0};
+    for (int w = 0; w < SHK_MASKW; w++) bits[w] = 0;
     int first = -1, k = 0;
-    bool prev = false, settled = false;
+    bool prev = false;
+    settled = false;
     double xt[3], yt[3], zt[3];
     for (;;) {
         // the check is the sweep's own max |lambda| (shake.c:421); every
@@ -374,7 +385,18 @@ __global__ void __launch_bounds__(64) k_w3_vv2_a(ShakeArgs A)
     w3_scr_store(A, G, SHK_W + 1, false);
     A.first[g] = first;
     A.kend[g] = k;
-    shake_publish_mask(A, bits, k, settled);
+    kend = k;
+}
+
+__global__ void __launch_bounds__(64) k_w3_vv2_a(ShakeArgs A)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane reaches the wave-wide AND; idle lanes contribute all ones
+    unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
+    int kend = -1;
+    bool settled = true;
+    if (g < A.nGrp) w3_vv2_a_body(A, g, bits, kend, settled);
+    shake_publish_mask(A, bits, kend, settled);
 }
 
 __global__ void __launch_bounds__(64) k_w3_vv2_b(ShakeArgs A)

@@ -286,6 +286,10 @@ int mdg_spme_setup(mdg_ctx *ctx)
     }
     ctx->mh = m3 / 2 + 1;
     const long G = (long)m1 * m2 * m3, H = (long)m1 * m2 * ctx->mh;
+    if (nblk(H, 256) * 7 > PART_R || nblk(ctx->n, 256) * 7 > PART_R) {
+        ctx->err = "mesh or atom count beyond the partial-sum buffers";
+        return MDG_EUNSUPPORTED;
+    }
     std::vector<double> b1, b2, b3;
     bsp_moduli(m1, p, b1);
     bsp_moduli(m2, p, b2);
@@ -413,13 +417,12 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
     return MDG_OK;
 }
 
-// SPME energy pieces into ctx->scal: S_EREC.. (sum theta|F|^2, st0..5),
-// S_FRX.. (sums of the reciprocal forces).  Reciprocal forces per atom go
-// to frx64.. (FP64 mode) or frx.. (MIXED).
+// Device-resident SPME: per-block partials of sum theta|F|^2 and st0..5 at
+// epart + PART_INFL, of the reciprocal forces at epart + PART_GATHER (the
+// step's record and finalize kernels reduce them).  Reciprocal forces per
+// atom go to frx64.. (FP64 mode) or frx.. (MIXED).
 int mdg_spme_device(mdg_ctx *ctx)
 {
-    const int n = ctx->n;
-    const long H = (long)ctx->m1 * ctx->m2 * ctx->mh;
     const bool mixed = ctx->precision == MDG_MIXED;
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[2], ctx->stream));
     if (mixed)
@@ -429,8 +432,6 @@ int mdg_spme_device(mdg_ctx *ctx)
         RETCHK(spme_run(ctx, false, ctx->epart + PART_INFL, ctx->epart + PART_GATHER, ctx->frx64,
                         ctx->fry64, ctx->frz64));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[3], ctx->stream));
-    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_INFL, nblk(H, 256), 7, ctx->scal + S_EREC));
-    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_GATHER, nblk(n, 256), 3, ctx->scal + S_FRX));
     return MDG_OK;
 }
 
