@@ -127,11 +127,12 @@ def main():
     stats = ctx.list_stats()
     log(f"timed {K} steps: {wall:.3f} s, {ms_step:.4f} ms/step, {ns_day:.1f} ns/day")
 
-    # kernel durations: the same K steps again as one captured graph with
-    # HIP events around every launch on the context stream
+    # kernel durations: up to 200 more steps as plain launches with HIP
+    # events around the pair kernel, SPME, the integrator, the list rebuild
+    # and the whole step, on the context stream
     ctx.set_profiling(True)
     Kp = min(K, 200)
-    recs2 = M.records_to_dict(ctx.run(Kp, use_graph=2))
+    recs2 = M.records_to_dict(ctx.run(Kp, use_graph=0))
     ph = ctx.phase_times()
     ctx.set_profiling(False)
     log(f"phases (ms): {ph}")

@@ -231,6 +231,32 @@ int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec);
  * after the last step: there is no per-step host round trip. */
 int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *records);
 
+/* ---- adjoint ------------------------------------------------------------ */
+
+/* New per-atom charges and LJ parameters (eps = sqrt(eps_kcal*418.4),
+ * sig as io.c:1174-1175 store them) for the current topology: the
+ * parameters the adjoint differentiates.  No reference counterpart (the
+ * reference reads them once, io.c:1150-1187). */
+int mdg_set_nonbond_params(mdg_ctx *ctx, const double q[], const double eps[],
+                           const double sig[]);
+
+/* dL/dq, dL/deps, dL/dsig = dL_dE * dE_pot/d(q, eps, sig) per atom, FP64,
+ * for the Ewald-path potential of the current device state (energy.c:450-643
+ * with spme.c:357-699: real-space pairs over the device list, SPME, the
+ * excluded-pair correction, the self and net-charge terms).  Call after
+ * mdg_initial_energy or mdg_run_steps.  No reference counterpart: it is the
+ * analytic form of what numderiv.c:39-190 estimates by finite differences. */
+int mdg_backward(mdg_ctx *ctx, double dL_dE, double dL_dq[], double dL_deps[], double dL_dsig[]);
+
+/* Runs nsteps MD steps from the current device state (as mdg_run_steps),
+ * writes J = the mean potential energy over steps 1..nsteps, and returns
+ * dJ/d(q, eps, sig) per atom through the whole trajectory: checkpoints
+ * every ckpt steps, recomputation of each segment, reverse sweep over the
+ * VV + SHAKE steps.  FP64 contexts of rigid three-site molecules.  The
+ * device state is left at step nsteps.  No reference counterpart. */
+int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, double dJ_dq[],
+                      double dJ_deps[], double dJ_dsig[]);
+
 /* Average kernel durations (ms) of the last mdg_run_steps, measured with
  * HIP events on the context stream: [0] pair kernel, [1] SPME (spline to
  * gather), [2] integrator, [3] list rebuild (per rebuild), [4] whole step.

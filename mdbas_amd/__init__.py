@@ -85,7 +85,7 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_makelist mdg_ewald_energy mdg_spme_energy mdg_nonbond_energy mdg_energy "
            "mdg_vv_integrate mdg_upload_state mdg_download_state mdg_initial_energy "
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
-           "mdg_list_pairs").split()
+           "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj").split()
 
 _LIB = None
 
@@ -117,6 +117,9 @@ def load():
         fn = getattr(L, f)
         if f not in ("mdg_create", "mdg_destroy", "mdg_last_error"):
             fn.restype = C.c_int
+    dp = C.POINTER(C.c_double)
+    L.mdg_backward.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
+    L.mdg_backward_traj.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp]
     _LIB = L
     return L
 
@@ -328,9 +331,32 @@ class Context:
 
     def run(self, nsteps: int, use_graph: bool = True):
         recs = (MDG_STEP * max(1, nsteps))()
-        self._chk(self.L.mdg_run_steps(self.h, int(nsteps), int(bool(use_graph)), recs),
+        self._chk(self.L.mdg_run_steps(self.h, int(nsteps), int(use_graph), recs),
                   "mdg_run_steps")
         return recs
+
+    # ---- adjoint --------------------------------------------------------
+    def set_params(self, q, eps, sig):
+        """New per-atom charges and LJ eps / sig (FP64 arrays of length n)."""
+        q, eps, sig = (np.ascontiguousarray(a, np.float64) for a in (q, eps, sig))
+        self._chk(self.L.mdg_set_nonbond_params(self.h, P(q), P(eps), P(sig)),
+                  "mdg_set_nonbond_params")
+
+    def backward(self, dL_dE: float = 1.0):
+        """dL/d(q, eps, sig) per atom at the current device state."""
+        g = np.zeros((3, self.n))
+        self._chk(self.L.mdg_backward(self.h, float(dL_dE), P(g[0]), P(g[1]), P(g[2])),
+                  "mdg_backward")
+        return g
+
+    def backward_traj(self, nsteps: int, ckpt: int):
+        """(J, dJ/d(q, eps, sig)) for J = mean potential energy over the next
+        nsteps steps (the device state advances by nsteps)."""
+        g = np.zeros((3, self.n))
+        J = C.c_double(0)
+        self._chk(self.L.mdg_backward_traj(self.h, int(nsteps), int(ckpt), C.byref(J), P(g[0]),
+                                           P(g[1]), P(g[2])), "mdg_backward_traj")
+        return J.value, g
 
     def set_profiling(self, on: bool):
         self._chk(self.L.mdg_set_profiling(self.h, int(bool(on))), "mdg_set_profiling")

@@ -275,6 +275,11 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
         ctx->err = "a periodic box is required";
         return fail(ctx, MDG_EUNSUPPORTED);
     }
+    if (ctx->precision == MDG_MIXED && ctx->B.type == 3) {
+        // the mixed pair kernel's minimum image is per axis
+        ctx->err = "mixed precision needs a cubic or orthorhombic box";
+        return fail(ctx, MDG_EUNSUPPORTED);
+    }
     if (hipGetDevice(&ctx->device) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         ctx->err = "no HIP device / stream";
@@ -329,7 +334,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,
                     ctx->grid64, ctx->cgrid64, ctx->grid32, ctx->cgrid32, ctx->gridq,
                     ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
-                    ctx->shkMask, ctx->exSymOff, ctx->exSymLst};
+                    ctx->shkMask, ctx->exSymOff, ctx->exSymLst, ctx->adjQ, ctx->adjE,
+                    ctx->adjS, ctx->adjPot};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int k = 0; k < 8; k++)
@@ -343,6 +349,22 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
 extern "C" const char *mdg_last_error(const mdg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 extern "C" int mdg_precision(const mdg_ctx *ctx) { return ctx ? ctx->precision : -1; }
 extern "C" int mdg_key_link(const mdg_ctx *ctx) { return ctx ? ctx->keyLink : -1; }
+
+// spme.c:376-389 and 680 (self and net-charge energies), computed for
+// every new set of charges.
+static void charge_sums(mdg_ctx *ctx, const double q[])
+{
+    double systq = 0., q2 = 0.;
+    for (int i = 0; i < ctx->n; i++) {
+        systq += q[i];
+        q2 += q[i] * q[i];
+    }
+    ctx->systq = systq;
+    ctx->eSelf = -ctx->P.chargeConst * ctx->P.alpha * q2 / SQRTPI_C;
+    ctx->eNet = ctx->B.vol > 0 && ctx->P.alpha > 0
+        ? -(0.5 * PI_C * ctx->P.chargeConst) * ((systq / ctx->P.alpha) * (systq / ctx->P.alpha) / ctx->B.vol)
+        : 0.;
+}
 
 extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps[],
                                 const double sig[], const double mass[], const double rmass[],
@@ -362,17 +384,7 @@ extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps
         for (int i = 0; i < n; i++) ctx->hasFrozen |= frozen[i] != 0;
     }
     if (nAtConst) RETCHK(h2d(ctx, ctx->nAtConst, nAtConst, n));
-    // spme.c:376-389 and 680, on every evaluation of the same charges
-    double systq = 0., q2 = 0.;
-    for (int i = 0; i < n; i++) {
-        systq += q[i];
-        q2 += q[i] * q[i];
-    }
-    ctx->systq = systq;
-    ctx->eSelf = -ctx->P.chargeConst * ctx->P.alpha * q2 / SQRTPI_C;
-    ctx->eNet = ctx->B.vol > 0 && ctx->P.alpha > 0
-        ? -(0.5 * PI_C * ctx->P.chargeConst) * ((systq / ctx->P.alpha) * (systq / ctx->P.alpha) / ctx->B.vol)
-        : 0.;
+    charge_sums(ctx, q);
     // constraints
     if (ctx->ca) { hipFree(ctx->ca); hipFree(ctx->cb); hipFree(ctx->rc2); hipFree(ctx->dd); }
     ctx->nConst = nConst;
@@ -396,6 +408,24 @@ extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps
     }
     RETCHK(mdg_build_groups(ctx, constList, nConst));
     RETCHK(mdg_build_exclusions(ctx, 0, nullptr, nConst, constList, 0, nullptr));
+    mdg_invalidate_graphs(ctx);
+    return sync(ctx);
+}
+
+// New per-atom charges and LJ parameters for an existing topology (the
+// parameters an adjoint differentiates); the device list's slot copies
+// follow at once.
+extern "C" int mdg_set_nonbond_params(mdg_ctx *ctx, const double q[], const double eps[],
+                                      const double sig[])
+{
+    if (!ctx || !q || !eps || !sig) return MDG_EARG;
+    const int n = ctx->n;
+    RETCHK(h2d(ctx, ctx->q, q, n));
+    RETCHK(h2d(ctx, ctx->eps, eps, n));
+    RETCHK(h2d(ctx, ctx->sig, sig, n));
+    charge_sums(ctx, q);
+    mdg_invalidate_graphs(ctx);
+    if (ctx->cl.cpJ && ctx->sx) RETCHK(mdg_slot_params(ctx));
     return sync(ctx);
 }
 
@@ -531,15 +561,7 @@ static int upload_xyzq(mdg_ctx *ctx, const double *x, const double *y, const dou
 
 static int refresh_charge_terms(mdg_ctx *ctx, const double *q)
 {
-    double systq = 0., q2 = 0.;
-    for (int i = 0; i < ctx->n; i++) {
-        systq += q[i];
-        q2 += q[i] * q[i];
-    }
-    ctx->systq = systq;
-    ctx->eSelf = -ctx->P.chargeConst * ctx->P.alpha * q2 / SQRTPI_C;
-    ctx->eNet = -(0.5 * PI_C * ctx->P.chargeConst) *
-                ((systq / ctx->P.alpha) * (systq / ctx->P.alpha) / ctx->B.vol);
+    charge_sums(ctx, q);
     return MDG_OK;
 }
 

@@ -122,6 +122,8 @@ struct mdg_ctx {
     int nExcl = 0;
     int *exSymOff = nullptr, *exSymLst = nullptr;   // both orientations of every pair
     int nExSym = 0;
+    // parameter adjoints dE/d(q, eps, sig) and the SPME mesh potential (mdg_adjoint.hip)
+    double *adjQ = nullptr, *adjE = nullptr, *adjS = nullptr, *adjPot = nullptr;
     std::vector<int> hExOff, hExLst;
     bool exclBuilt = false;
 
@@ -373,7 +375,16 @@ int mdg_slot_params(mdg_ctx *ctx);
 int mdg_spme_setup(mdg_ctx *ctx);
 int mdg_spme_free(mdg_ctx *ctx);
 int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8]);
-int mdg_spme_device(mdg_ctx *ctx);   // device-resident: frx.. per atom, sums into scal
+int mdg_spme_device(mdg_ctx *ctx);   // device-resident: frx.. per atom, partials in epart
+int mdg_spme_pot(mdg_ctx *ctx, double *pot);   // FP64 dE_rec/dq per atom
+int mdg_adj_pair(mdg_ctx *ctx);      // real-space dE/d(q, eps, sig) into fdx/fdy/fdz slots
+
+// device-resident step pieces (mdg_run.hip)
+int mdg_device_energy(mdg_ctx *ctx, bool rebuild);
+int mdg_device_step(mdg_ctx *ctx, bool rebuild);
+int mdg_reset_step_state(mdg_ctx *ctx);
+int mdg_check_flags(mdg_ctx *ctx);
+void mdg_invalidate_graphs(mdg_ctx *ctx);
 
 // integrate (mdg_integrate.hip)
 int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst);

@@ -522,19 +522,9 @@ int mdg_slot_params(mdg_ctx *ctx)
     return MDG_OK;
 }
 
-int mdg_pair_cluster(mdg_ctx *ctx)
+static ClusterArgs cluster_args(mdg_ctx *ctx)
 {
     ClusterList &C = ctx->cl;
-    const int nslot = C.nclu * MDG_CL;
-    if (ctx->precision == MDG_MIXED)
-        hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
-                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
-                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
-    else
-        hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
-                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
-                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
-    LAUNCHCHK();
     ClusterArgs A;
     A.P = ctx->P;
     A.B = ctx->B;
@@ -572,6 +562,23 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     A.fdy = ctx->fdy;
     A.fdz = ctx->fdz;
     A.part = ctx->epart + PART_PAIR;
+    return A;
+}
+
+int mdg_pair_cluster(mdg_ctx *ctx)
+{
+    ClusterList &C = ctx->cl;
+    const int nslot = C.nclu * MDG_CL;
+    if (ctx->precision == MDG_MIXED)
+        hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
+    else
+        hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
+    LAUNCHCHK();
+    ClusterArgs A = cluster_args(ctx);
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     if (ctx->precision == MDG_MIXED)
@@ -580,5 +587,91 @@ int mdg_pair_cluster(mdg_ctx *ctx)
         hipLaunchKernelGGL(k_pair_cluster<false>, dim3(nb), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
+    return MDG_OK;
+}
+
+// Adjoint of the real-space pair energy with respect to the per-atom
+// parameters, FP64 in both precisions.  For every listed pair inside the
+// cutoff (the pairs energy.c:496-541 sums):
+//   dE/dq_i   += C q_j erfc(a r) / r                      (and i <-> j)
+//   dE/deps_i += eps_j 4 (s^12 - s^6) S(r)                (and i <-> j)
+//   dE/dsig_i += 4 eps_i eps_j (12 s^11 - 6 s^5) S(r) / r (both atoms)
+// with s = (sig_i + sig_j) / r and S the switch of vdw.c:144-184 (1 below
+// cutOn).  Same wavefront layout as k_pair_cluster; slot sums go to
+// fdx (q), fdy (eps), fdz (sig).
+__global__ void __launch_bounds__(256) k_adj_cluster(ClusterArgs A)
+{
+    const int lane = threadIdx.x & 63;
+    const int a = lane & 7, b = lane >> 3;
+    const int bit = (a << 3) | b;
+    const int nch = A.chOff[A.nclu];
+    for (int W = blockIdx.x * 4 + (threadIdx.x >> 6); W < nch; W += gridDim.x * 4) {
+        const int2 ch = A.chunk[W];
+        const int I = ch.x, e0 = ch.y, e1 = min(e0 + MDG_CHUNK, A.cpOff[I + 1]);
+        const int si = I * MDG_CL + a;
+        const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
+        const double qi = A.sq[si], ei = A.seps[si], gi = A.ssig[si];
+        double gqi = 0., gei = 0., gsi = 0.;
+        for (int e = e0; e < e1; e++) {
+            const int J = A.cpJ[e];
+            const unsigned long long m = A.cpMask[e];
+            const int sj = J * MDG_CL + b;
+            double dx = A.sx[sj] - xi, dy = A.sy[sj] - yi, dz = A.sz[sj] - zi;
+            const double r2 = mdg_dist(A.B, dx, dy, dz);
+            double gqj = 0., gej = 0., gsj = 0.;
+            if (((m >> bit) & 1ull) && r2 <= A.P.cutOff2) {
+                const double r = sqrt(r2), rt = 1. / r;
+                const double qj = A.sq[sj], ej = A.seps[sj];
+                const double ec = A.P.chargeConst * erfc(A.P.alpha * r) * rt;
+                gqi += qj * ec;
+                gqj = qi * ec;
+                const double s = (gi + A.ssig[sj]) * rt, s2 = s * s, s6 = s2 * s2 * s2;
+                const double s12 = s6 * s6, s5 = s2 * s2 * s, s11 = s5 * s6;
+                double S = 1.;
+                if (r2 > A.P.cutOn2) {
+                    const double sw = A.P.cutOff2 - r2;
+                    S = sw * sw * (A.P.cutOff2 + 2. * r2 - 3. * A.P.cutOn2) * A.P.switch2;
+                }
+                const double lj = 4. * (s12 - s6) * S;
+                gei += ej * lj;
+                gej = ei * lj;
+                const double gs = 4. * ei * ej * (12. * s11 - 6. * s5) * rt * S;
+                gsi += gs;
+                gsj = gs;
+            }
+            gqj = dpp_sum8(gqj);
+            gej = dpp_sum8(gej);
+            gsj = dpp_sum8(gsj);
+            if (a == 0) {
+                if (gqj != 0.) atomicAdd(&A.fdx[sj], gqj);
+                if (gej != 0.) atomicAdd(&A.fdy[sj], gej);
+                if (gsj != 0.) atomicAdd(&A.fdz[sj], gsj);
+            }
+        }
+        for (int o = 8; o < 64; o <<= 1) {
+            gqi += __shfl_xor(gqi, o, 64);
+            gei += __shfl_xor(gei, o, 64);
+            gsi += __shfl_xor(gsi, o, 64);
+        }
+        if (b == 0) {
+            atomicAdd(&A.fdx[si], gqi);
+            atomicAdd(&A.fdy[si], gei);
+            atomicAdd(&A.fdz[si], gsi);
+        }
+    }
+}
+
+// Real-space parameter gradients into the FP64 slot accumulators.
+int mdg_adj_pair(mdg_ctx *ctx)
+{
+    ClusterList &C = ctx->cl;
+    const int nslot = C.nclu * MDG_CL;
+    hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream, nslot,
+                       C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz, ctx->fix,
+                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
+    LAUNCHCHK();
+    ClusterArgs A = cluster_args(ctx);
+    hipLaunchKernelGGL(k_adj_cluster, dim3(mdg_pair_blocks(ctx)), dim3(256), 0, ctx->stream, A);
+    LAUNCHCHK();
     return MDG_OK;
 }

@@ -107,37 +107,48 @@ struct RecArgs {
     unsigned long long *mask;
 };
 
-// Sums rows [0, nrow) of column c of a row-major partial table with one
-// block (fixed order, so deterministic).
-__device__ inline double block_col_sum(const double *p, int nrow, int ncol, int c, double *red)
+// Adds rows [0, nrow) of a row-major partial table with NC columns into
+// acc[0..NC) (this thread's rows only).
+template <int NC>
+__device__ inline void rows_add(const double *__restrict__ p, int nrow, double *acc)
 {
-    double s = 0.;
-    for (int r = threadIdx.x; r < nrow; r += blockDim.x) s += p[(long)r * ncol + c];
-    s = mdg_wave_sum(s);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double t = 0.;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
-    return t;
+    for (int r = threadIdx.x; r < nrow; r += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c] += p[(long)r * NC + c];
 }
 
 // One MDG_STEP record: reduces the step's partials (pair, correction,
-// influence function, kinetic energy, SHAKE virial) and assembles the
-// energies (spme.c:674-697 for the reciprocal energy and virial,
-// energy.c:623-627 and 193-196 for the sums); resets the SHAKE mask for
-// the next step's VV stage 1.
-__global__ void __launch_bounds__(256) k_record(RecArgs R, int *ctrl, MDG_STEP *rec, int rebuilt)
+// influence function, kinetic energy, SHAKE virial) in one block, in a
+// fixed order, and assembles the energies (spme.c:674-697 for the
+// reciprocal energy and virial, energy.c:623-627 and 193-196 for the
+// sums); resets the SHAKE mask for the next step's VV stage 1.
+#define REC_NV 16
+__global__ void __launch_bounds__(1024) k_record(RecArgs R, int *ctrl, MDG_STEP *rec, int rebuilt)
 {
-    __shared__ double red[4];
-    double v[22];
-    for (int c = 0; c < 5; c++) v[c] = block_col_sum(R.pPair, R.nbPair, 5, c, red);
-    for (int c = 0; c < 2; c++) v[5 + c] = block_col_sum(R.pExcl, R.nbExcl, 2, c, red);
-    for (int c = 0; c < 7; c++) v[7 + c] = block_col_sum(R.pInfl, R.nbInfl, 7, c, red);
-    v[14] = block_col_sum(R.pKin, R.nbKin, 7, 0, red);
-    v[15] = R.nGrpVir ? block_col_sum(R.grpVir, R.nGrpVir, 1, 0, red) : 0.;
+    __shared__ double red[16][REC_NV];
+    double v[REC_NV];
+#pragma unroll
+    for (int k = 0; k < REC_NV; k++) v[k] = 0.;
+    rows_add<5>(R.pPair, R.nbPair, v);
+    rows_add<2>(R.pExcl, R.nbExcl, v + 5);
+    rows_add<7>(R.pInfl, R.nbInfl, v + 7);
+    for (int r = threadIdx.x; r < R.nbKin; r += blockDim.x) v[14] += R.pKin[(long)r * 7];
+    rows_add<1>(R.grpVir, R.nGrpVir, v + 15);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < REC_NV; k++) {
+        const double s = mdg_wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
     if (threadIdx.x < 4) R.mask[threadIdx.x] = ~0ull;
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < REC_NV; k++) {
+        double s = 0.;
+        for (int w = 0; w < nw; w++) s += red[w][k];
+        v[k] = s;
+    }
     const int idx = ctrl[CTRL_STEP];
     const double er = v[7];
     const double st0 = R.fact * (v[8] + er) + R.eNet;
@@ -184,24 +195,30 @@ static int ensure_slots(mdg_ctx *ctx)
     return MDG_OK;
 }
 
+// Captured step graphs hold kernel arguments by value (record capacity,
+// self and net-charge energies, ...): drop them when those change.
+void mdg_invalidate_graphs(mdg_ctx *ctx)
+{
+    for (int k = 0; k < 2; k++)
+        if (ctx->graphExec[k]) {
+            (void)hipGraphExecDestroy(ctx->graphExec[k]);
+            ctx->graphExec[k] = nullptr;
+        }
+}
+
 static int ensure_records(mdg_ctx *ctx, int n)
 {
     if (ctx->recCap >= n) return MDG_OK;
     if (ctx->recDev) HIPCHK(hipFree(ctx->recDev));
     ctx->recCap = std::max(n, 64);
     HIPCHK(hipMalloc(&ctx->recDev, ctx->recCap * sizeof(MDG_STEP)));
-    // the record kernel holds recCap by value: recapture the graphs
-    for (int k = 0; k < 2; k++)
-        if (ctx->graphExec[k]) {
-            hipGraphExecDestroy(ctx->graphExec[k]);
-            ctx->graphExec[k] = nullptr;
-        }
+    mdg_invalidate_graphs(ctx);
     return MDG_OK;
 }
 
 // Forces of the current positions on the device (energy() of
 // energy.c:103-205), optionally rebuilding the cluster list first.
-static int dev_energy(mdg_ctx *ctx, bool rebuild)
+int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
 {
     const int n = ctx->n;
     if (rebuild) {
@@ -279,15 +296,15 @@ static RecArgs rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
 }
 
 // One MD step (mdbas.c:244-301).
-static int dev_step(mdg_ctx *ctx, bool rebuild, int stepBase)
+int mdg_device_step(mdg_ctx *ctx, bool rebuild)
 {
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[8], ctx->stream));
     RETCHK(mdg_vv_stage(ctx, 1, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[9], ctx->stream));
-    RETCHK(dev_energy(ctx, rebuild));
+    RETCHK(mdg_device_energy(ctx, rebuild));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[10], ctx->stream));
     RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
                        rec_args(ctx, nblk(ctx->nGrp, ctx->shakeW3 ? 64 : 128), ctx->nGrp),
                        ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     LAUNCHCHK();
@@ -295,7 +312,7 @@ static int dev_step(mdg_ctx *ctx, bool rebuild, int stepBase)
     return MDG_OK;
 }
 
-static int check_flags(mdg_ctx *ctx)
+int mdg_check_flags(mdg_ctx *ctx)
 {
     int c[CTRL_WORDS];
     HIPCHK(hipMemcpyAsync(c, ctx->ctrl, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
@@ -327,6 +344,16 @@ __global__ void k_set_int(int *p, int v)
 __global__ void k_reset_mask(unsigned long long *mask)
 {
     if (threadIdx.x < 4) mask[threadIdx.x] = ~0ull;
+}
+
+// Record index 0 and a fresh SHAKE mask, before a sequence of steps.
+int mdg_reset_step_state(mdg_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, ctx->stream, ctx->shkMask);
+    LAUNCHCHK();
+    return MDG_OK;
 }
 
 extern "C" int mdg_upload_state(mdg_ctx *ctx, const double x[], const double y[],
@@ -362,16 +389,21 @@ extern "C" int mdg_download_state(mdg_ctx *ctx, double x[], double y[], double z
 extern "C" int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec)
 {
     if (!ctx || !ctx->theta || !ctx->grpAtomOff || !ctx->exSymOff) return MDG_EARG;
+    if (ctx->B.type == 3) {
+        // the cluster list bins atoms in Cartesian columns
+        ctx->err = "the device-resident path needs a cubic or orthorhombic box";
+        return MDG_EUNSUPPORTED;
+    }
     RETCHK(ensure_records(ctx, 64));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
-    RETCHK(dev_energy(ctx, true));
+    RETCHK(mdg_device_energy(ctx, true));
     hipLaunchKernelGGL(k_kinetic, dim3(nblk(ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n,
                        ctx->mass, ctx->vx, ctx->vy, ctx->vz, ctx->epart + PART_KIN);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
                        rec_args(ctx, nblk(ctx->n, 256), 0), ctx->ctrl, ctx->recDev, 1);
     LAUNCHCHK();
-    RETCHK(check_flags(ctx));
+    RETCHK(mdg_check_flags(ctx));
     if (rec) HIPCHK(hipMemcpy(rec, ctx->recDev, sizeof(MDG_STEP), hipMemcpyDeviceToHost));
     ctx->stepsSinceList = 0;
     return MDG_OK;
@@ -381,7 +413,7 @@ static int capture(mdg_ctx *ctx, int variant)
 {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int st = dev_step(ctx, variant == 1, 0);
+    int st = mdg_device_step(ctx, variant == 1);
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (st) return st;
     if (e != hipSuccess) {
@@ -413,10 +445,10 @@ static void accumulate_phase(mdg_ctx *ctx, const hipEvent_t *e, bool rebuild, in
 }
 
 // use_graph: 0 plain launches, 1 one graph per step variant (replayed),
-// 2 the whole run captured as one graph.  With profiling on, HIP events
-// bracket the pair kernel, SPME, the integrator, the list rebuild and the
-// step on the context stream (in mode 2 they are nodes of the graph, one
-// set per step, read after the run; in modes 0/1 the run syncs per step).
+// 2 the whole run captured as one graph.  With profiling on, every mode
+// runs plain launches with HIP events bracketing the pair kernel, SPME,
+// the integrator, the list rebuild and the step on the context stream,
+// read after each step.
 extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *records)
 {
     if (!ctx || nsteps < 0) return MDG_EARG;
@@ -425,31 +457,20 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
         return MDG_EARG;
     }
     RETCHK(ensure_records(ctx, nsteps));
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
-    LAUNCHCHK();
     // the step sequence expects a fresh SHAKE mask for its first stage 1
     // (each step's record kernel resets it for the next)
-    hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, ctx->stream, ctx->shkMask);
-    LAUNCHCHK();
+    RETCHK(mdg_reset_step_state(ctx));
     const bool prof = ctx->profiling;
     for (int k = 0; k < 5; k++) ctx->phase[k] = 0.;
     const int stepBase = ctx->step + 1;
     int nReb = 0;
-    if (use_graph == 2) {
-        std::vector<hipEvent_t> evs;
-        if (prof) {
-            evs.resize((size_t)12 * nsteps);
-            for (auto &e : evs) HIPCHK(hipEventCreate(&e));
-        }
+    if (use_graph == 2 && !prof) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int st = MDG_OK;
-        for (int s = 0; s < nsteps && st == MDG_OK; s++) {
-            if (prof) ctx->evCur = &evs[(size_t)12 * s];
-            st = dev_step(ctx, ((ctx->step + 1 + s) % ctx->P.update) == 0, stepBase);
-        }
+        for (int s = 0; s < nsteps && st == MDG_OK; s++)
+            st = mdg_device_step(ctx, ((ctx->step + 1 + s) % ctx->P.update) == 0);
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        ctx->evCur = ctx->ev;
         if (st) return st;
         if (e != hipSuccess) {
             ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
@@ -461,12 +482,7 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
         HIPCHK(hipGraphLaunch(ge, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(hipGraphExecDestroy(ge));
-        for (int s = 0; s < nsteps; s++) {
-            const int step = ctx->step + 1;
-            if (prof) accumulate_phase(ctx, &evs[(size_t)12 * s], (step % ctx->P.update) == 0, nReb);
-            ctx->step = step;
-        }
-        for (auto &ev : evs) hipEventDestroy(ev);
+        ctx->step += nsteps;
     } else {
         for (int s = 0; s < nsteps; s++) {
             const int step = ctx->step + 1;
@@ -477,7 +493,7 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
                 HIPCHK(hipGraphLaunch(ctx->graphExec[v], ctx->stream));
             } else {
                 ctx->evCur = ctx->ev;
-                RETCHK(dev_step(ctx, rebuild, stepBase));
+                RETCHK(mdg_device_step(ctx, rebuild));
                 if (prof) {
                     HIPCHK(hipEventSynchronize(ctx->ev[11]));
                     accumulate_phase(ctx, ctx->ev, rebuild, nReb);
@@ -491,7 +507,7 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
             if (k != 3) ctx->phase[k] /= nsteps;
         ctx->phase[3] = nReb ? ctx->phase[3] / nReb : 0.;
     }
-    RETCHK(check_flags(ctx));
+    RETCHK(mdg_check_flags(ctx));
     if (records && nsteps > 0) {
         HIPCHK(hipMemcpy(records, ctx->recDev, nsteps * sizeof(MDG_STEP), hipMemcpyDeviceToHost));
         for (int s = 0; s < nsteps; s++) records[s].step = stepBase + s;

@@ -216,6 +216,38 @@ k_spme_gather(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
     mdg_block_partials<3>(acc, part);
 }
 
+// dE_rec/dq_l = 2 (2 pi / V) C sum_g (theta*Q)(g) M_l(g): the convolved
+// FP64 mesh gathered with the spline weights (the gradient of the
+// quadratic form of spme.c:674-680 with respect to one charge).
+__global__ void __launch_bounds__(256)
+k_spme_potential(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
+                 const double *__restrict__ y, const double *__restrict__ z,
+                 const double *__restrict__ grid, double scale, double *__restrict__ pot)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int p = P.nbsp, m1 = P.m1, m2 = P.m2, m3 = P.m3;
+    double s1, s2, s3;
+    scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+    double b1[MAXP], b2[MAXP], b3[MAXP], d[MAXP];
+    bspline_dev<double>(s1, p, b1, d);
+    bspline_dev<double>(s2, p, b2, d);
+    bspline_dev<double>(s3, p, b3, d);
+    const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+    double phi = 0.;
+    for (int i = 0; i < p; i++) {
+        const int ii = wrap_index(i1 - i, m1);
+        const int o1 = m2 * ii;
+        for (int j = 0; j < p; j++) {
+            const int jj = wrap_index(i2 - j, m2);
+            const long o2 = (long)m3 * (jj + o1);
+            const double c = b1[i] * b2[j];
+            for (int k = 0; k < p; k++) phi += grid[wrap_index(i3 - k, m3) + o2] * c * b3[k];
+        }
+    }
+    pot[l] = scale * phi;
+}
+
 // ---------------------------------------------------------------- host side
 
 // spme.c:170-267: |bspc(i)|^2 moduli.
@@ -494,5 +526,19 @@ int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8])
     eout[0] = fact * eRec + ctx->eSelf + ctx->eNet;
     eout[1] = -(st[0] + st[3] + st[5]);
     for (int k = 0; k < 6; k++) eout[2 + k] = st[k];
+    return MDG_OK;
+}
+
+// dE_rec/dq per atom (FP64, self and net-charge terms not included) into
+// pot (device, n); the FP64 reciprocal forces land in frx64.. as a side
+// effect.
+int mdg_spme_pot(mdg_ctx *ctx, double *pot)
+{
+    RETCHK(spme_run(ctx, false, ctx->epart + PART_INFL, ctx->epart + PART_MISC, ctx->frx64,
+                    ctx->fry64, ctx->frz64));
+    const double scale = 2.0 * (TWOPI / ctx->B.vol) * ctx->P.chargeConst;
+    hipLaunchKernelGGL(k_spme_potential, dim3(nblk(ctx->n, 256)), dim3(256), 0, ctx->stream,
+                       ctx->P, ctx->B, ctx->n, ctx->x, ctx->y, ctx->z, ctx->grid64, scale, pot);
+    LAUNCHCHK();
     return MDG_OK;
 }

@@ -144,6 +144,14 @@ class OracleSystem:
             raise RuntimeError(f"oracle exclusions failed: {ne}")
         self.exLst = self.exLst[:ne].copy()
         self.neighList = None
+        self.q, self.eps, self.sig = wb.q, wb.eps, wb.sig
+
+    def set_params(self, q=None, eps=None, sig=None):
+        """Per-atom nonbonded parameters used by energy() and energy_grad()
+        (the water box's own when None)."""
+        self.q = self.wb.q if q is None else np.ascontiguousarray(q, np.float64)
+        self.eps = self.wb.eps if eps is None else np.ascontiguousarray(eps, np.float64)
+        self.sig = self.wb.sig if sig is None else np.ascontiguousarray(sig, np.float64)
 
     def build_list(self, x, y, z, build: bool = True):
         """Reference-format list; x,y,z may be wrapped in place (link cells,
@@ -177,8 +185,8 @@ class OracleSystem:
         f = np.zeros((3, n)) if f is None else f
         e = OEnergy()
         lib().oc_ewald_real(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z),
-                            P(f[0]), P(f[1]), P(f[2]), P(self.wb.q), P(self.wb.eps),
-                            P(self.wb.sig), P(self.neighList), P(self.neighPair),
+                            P(f[0]), P(f[1]), P(f[2]), P(self.q), P(self.eps),
+                            P(self.sig), P(self.neighList), P(self.neighPair),
                             P(self.neighOrder), P(self.exOff), P(self.exLst), C.byref(e))
         return f, e
 
@@ -188,8 +196,18 @@ class OracleSystem:
         st = np.zeros(6)
         vir = C.c_double(0)
         e = lib().oc_spme(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z),
-                          P(f[0]), P(f[1]), P(f[2]), P(self.wb.q), P(st), C.byref(vir))
+                          P(f[0]), P(f[1]), P(f[2]), P(self.q), P(st), C.byref(vir))
         return e, f, st, vir.value
+
+    def energy_grad(self, x, y, z):
+        """dE_pot/d(q, eps, sig) per atom at x,y,z over the current list."""
+        n = self.wb.n
+        g = np.zeros((3, n))
+        lib().oc_energy_grad(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z), P(self.q),
+                             P(self.eps), P(self.sig), P(self.neighList), P(self.neighPair),
+                             P(self.neighOrder), P(self.exOff), P(self.exLst), P(g[0]),
+                             P(g[1]), P(g[2]))
+        return g
 
     def energy(self, x, y, z):
         """energy() (energy.c:103-205) for the Ewald path: zero forces, SPME,

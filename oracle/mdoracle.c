@@ -685,10 +685,38 @@ void oc_bspmod(int m, int p, double *mod2)
     free(epl);
 }
 
+static double spme_core(const oc_param *p, const oc_box *b,
+                        const double *x, const double *y, const double *z,
+                        double *fx, double *fy, double *fz, const double *q,
+                        double stress[6], double *virRec, double *pot);
+
 double oc_spme(const oc_param *p, const oc_box *b,
                const double *x, const double *y, const double *z,
                double *fx, double *fy, double *fz, const double *q,
                double stress[6], double *virRec)
+{
+    return spme_core(p, b, x, y, z, fx, fy, fz, q, stress, virRec, NULL);
+}
+
+/* dE_rec/dq_l of the SPME energy (self and net-charge terms excluded):
+ * E_rec = (2pi/V) C sum_g Q(g) (theta*Q)(g) is quadratic in the mesh
+ * charges Q, and Q(g) = sum_l q_l M_l(g), so dE/dq_l = 2 (2pi/V) C
+ * sum_g (theta*Q)(g) M_l(g): the convolved mesh gathered with the spline
+ * weights instead of their derivatives. */
+void oc_spme_pot(const oc_param *p, const oc_box *b, const double *x, const double *y,
+                 const double *z, const double *q, double *dq)
+{
+    int n = p->nAtom;
+    double *f = (double *)calloc((size_t)3 * n, sizeof(double));
+    double st[6], vr;
+    spme_core(p, b, x, y, z, f, f + n, f + 2 * n, q, st, &vr, dq);
+    free(f);
+}
+
+static double spme_core(const oc_param *p, const oc_box *b,
+                        const double *x, const double *y, const double *z,
+                        double *fx, double *fy, double *fz, const double *q,
+                        double stress[6], double *virRec, double *pot)
 {
     int n = p->nAtom, m1 = p->m1, m2 = p->m2, m3 = p->m3, P = p->nbsp;
     long G = (long)m1 * m2 * m3;
@@ -788,6 +816,7 @@ double oc_spme(const oc_param *p, const oc_box *b,
     for (int l = 0; l < n; l++) {                              /* spme.c:593-646 */
         const double *B1 = bp + (3 * l) * P, *B2 = B1 + P, *B3 = B2 + P;
         const double *D1 = bd + (3 * l) * P, *D2 = D1 + P, *D3 = D2 + P;
+        double phi = 0.;
         for (int i = 0; i < P; i++) {
             int ii = (int)sx[3 * l] - i;
             if (ii >= m1) ii = 0;
@@ -809,9 +838,11 @@ double oc_spme(const oc_param *p, const oc_box *b,
                     fx[l] += fact1 * q[l] * (fbx * b->u1 + fby * b->v1 + fbz * b->w1);
                     fy[l] += fact1 * q[l] * (fbx * b->u2 + fby * b->v2 + fbz * b->w2);
                     fz[l] += fact1 * q[l] * (fbx * b->u3 + fby * b->v3 + fbz * b->w3);
+                    phi += qt * B1[i] * B2[j] * B3[k];
                 }
             }
         }
+        if (pot) pot[l] = 2.0 * rVol * p->chargeConst * phi;
     }
     double fxm = 0., fym = 0., fzm = 0.;                       /* spme.c:648-670 */
     for (int l = 0; l < n; l++) { fxm += fx[l]; fym += fy[l]; fzm += fz[l]; }
@@ -1127,4 +1158,72 @@ int oc_md_run(const oc_param *p, const oc_box *b, int keyLink, int nsteps,
     if (seconds) *seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     free(L.nl); free(L.np); free(L.no); free(exOff); free(exLst);
     return err;
+}
+
+/* Gradient of the Ewald-path potential energy (energy.c:450-627 with
+ * spme.c:357-699) with respect to the per-atom parameters q, eps, sig,
+ * over the same list and exclusions:
+ *   real space   E = C q_i q_j erfc(a r)/r
+ *                  + 4 eps_i eps_j ((s^12 - s^6) S(r)),  s = (sig_i+sig_j)/r
+ *   correction   E = -C q_i q_j erf(a r)/r                    (j > i)
+ *   reciprocal   oc_spme_pot
+ *   self         E = -C a sum q^2 / sqrt(pi)
+ *   net charge   E = -(pi C / 2) (sum q / a)^2 / V.
+ * dq, deps, dsig are overwritten. */
+void oc_energy_grad(const oc_param *p, const oc_box *b,
+                    const double *x, const double *y, const double *z,
+                    const double *q, const double *eps, const double *sig,
+                    const int *neighList, const int *neighPair, const int *neighOrder,
+                    const int *exOff, const int *exLst,
+                    double *dq, double *deps, double *dsig)
+{
+    int n = p->nAtom;
+    double C = p->chargeConst, a = p->alpha;
+    oc_spme_pot(p, b, x, y, z, q, dq);
+    double systq = 0.;
+    for (int i = 0; i < n; i++) systq += q[i];
+    for (int i = 0; i < n; i++) {
+        dq[i] += -2.0 * C * a * q[i] / OC_SQRTPI;
+        dq[i] += -(OC_PI * C) * systq / (a * a * b->vol);
+        deps[i] = 0.;
+        dsig[i] = 0.;
+    }
+    long offset = 0;
+    for (int l = 0; l < n; l++) {
+        int i = neighOrder[l];
+        for (int k = 0; k < neighPair[i]; k++) {
+            int j = neighList[offset++];
+            double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+            double r2 = oc_dist(b, d);
+            if (r2 > p->cutOff2) continue;
+            double r = sqrt(r2), rt = 1. / r;
+            double ec = C * erfc(a * r) * rt;
+            dq[i] += q[j] * ec;
+            dq[j] += q[i] * ec;
+            double vsig = sig[i] + sig[j];
+            double s = vsig * rt, s2 = s * s, s6 = s2 * s2 * s2, s12 = s6 * s6;
+            double s5 = s2 * s2 * s, s11 = s5 * s6;
+            double S = 1.;
+            if (r2 > p->cutOn2) {
+                double sw = p->cutOff2 - r2;
+                S = sw * sw * (p->cutOff2 + 2. * r2 - 3. * p->cutOn2) * p->switch2;
+            }
+            double lj = 4. * (s12 - s6) * S;
+            deps[i] += eps[j] * lj;
+            deps[j] += eps[i] * lj;
+            double gs = 4. * eps[i] * eps[j] * (12. * s11 - 6. * s5) * rt * S;
+            dsig[i] += gs;
+            dsig[j] += gs;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int k = exOff[i]; k < exOff[i + 1]; k++) {
+            int j = exLst[k];
+            if (j <= i) continue;
+            double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+            double r = sqrt(oc_dist(b, d));
+            double ec = -C * erf(a * r) / r;
+            dq[i] += q[j] * ec;
+            dq[j] += q[i] * ec;
+        }
 }

@@ -26,6 +26,11 @@
  *                                          convergence; B-spline identities;
  *                                          FD of the energy.
  *   oc_fft3d                             : direct O(n^2) DFT on small grids.
+ *   oc_spme_pot, oc_energy_grad          : central finite differences of the
+ *                                          oracle energy (numderiv.c:39, 102,
+ *                                          190 stencils); Euler's theorem for
+ *                                          the degree-2 homogeneity of E_elec
+ *                                          in q and E_vdw in eps.
  *   oc_vv_nve / oc_shake_*               : compiled reference vv_integrate /
  *                                          vv_shake_r / vv_shake_v.
  */
@@ -137,6 +142,21 @@ double oc_spme(const oc_param *p, const oc_box *b,
                const double *x, const double *y, const double *z,
                double *fx, double *fy, double *fz, const double *q,
                double stress[6], double *virRec);
+
+/* dE_rec/dq per atom of the SPME energy (self / net-charge terms not
+ * included). */
+void oc_spme_pot(const oc_param *p, const oc_box *b, const double *x, const double *y,
+                 const double *z, const double *q, double *dq);
+
+/* Gradient of the Ewald-path potential (real space over the list,
+ * excluded-pair correction, SPME, self and net-charge terms) with respect
+ * to the per-atom q, eps, sig.  Overwrites dq, deps, dsig. */
+void oc_energy_grad(const oc_param *p, const oc_box *b,
+                    const double *x, const double *y, const double *z,
+                    const double *q, const double *eps, const double *sig,
+                    const int *neighList, const int *neighPair, const int *neighOrder,
+                    const int *exOff, const int *exLst,
+                    double *dq, double *deps, double *dsig);
 
 /* B-spline values/derivatives for one fractional coordinate s, as bspgen
  * does (spme.c:269-355): bsp[i] = M_p(u+i), bsd[i] = M_p'(u+i). */

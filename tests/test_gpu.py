@@ -198,12 +198,12 @@ def test_mixed_force_error(which, c0_box, c1_box):
 
 
 def test_mixed_graph_replay_is_bitwise_deterministic(c0_box):
-    """The mixed path has no floating-point atomics: a graph-replayed run
-    and a run of plain launches from the same state agree bit for bit,
-    replay after replay (three rebuild-free graphs plus a rebuild graph)."""
+    """The mixed path has no floating-point atomics: graph-replayed runs
+    (per-step graphs, and the whole run as one graph) and a run of plain
+    launches from the same state agree bit for bit, replay after replay."""
     wb = c0_box
     res = []
-    for graph in (True, False):
+    for graph in (1, 0, 2):
         ctx = M.Context(wb, precision=M.MDG_MIXED)
         ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
         ctx.initial_energy()
@@ -212,8 +212,128 @@ def test_mixed_graph_replay_is_bitwise_deterministic(c0_box):
         r2 = M.records_to_dict(ctx.run(12, use_graph=graph))
         res.append((r1, r2, ctx.download()))
         ctx.close()
-    (a1, a2, sa), (b1, b2, sb) = res
-    for k in ("tot", "pot", "kin", "eRec"):
-        assert np.array_equal(a1[k], b1[k]) and np.array_equal(a2[k], b2[k]), k
-    for u, v in zip(sa, sb):
-        assert np.array_equal(u, v)
+    (a1, a2, sa) = res[0]
+    for (b1, b2, sb) in res[1:]:
+        for k in ("tot", "pot", "kin", "eRec"):
+            assert np.array_equal(a1[k], b1[k]) and np.array_equal(a2[k], b2[k]), k
+        for u, v in zip(sa, sb):
+            assert np.array_equal(u, v)
+
+
+# ------------------------------------------------------------------ adjoint
+
+@pytest.mark.parametrize("prec", [M.MDG_FP64, M.MDG_MIXED])
+def test_backward_matches_oracle_gradient(prec, c0):
+    """mdg_backward (FP64 in either context precision) against the oracle's
+    dE/d(q, eps, sig) on the same list and coordinates."""
+    wb, o = c0["wb"], c0["o"]
+    g_o = o.energy_grad(*c0["x"])
+    ctx = M.Context(wb, precision=prec)
+    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    ctx.initial_energy()
+    g = ctx.backward(2.5)
+    ctx.close()
+    for k in range(3):
+        say("backward", prec, k, relerr(g[k], 2.5 * g_o[k]))
+        assert relerr(g[k], 2.5 * g_o[k]) < 1e-11
+
+
+def _c3_water(wb):
+    import torch
+    from mdbas_amd import autograd as AG
+    water = AG.Water(wb)
+    q = torch.tensor(wb.q, dtype=torch.float64, requires_grad=True)
+    eps_o = torch.tensor([0.1521], dtype=torch.float64, requires_grad=True)
+    rmin_o = torch.tensor([1.7682], dtype=torch.float64, requires_grad=True)
+    zero = torch.zeros(1, dtype=torch.float64)
+    return torch, AG, water, q, eps_o, rmin_o, zero
+
+
+def test_backward_torch_harness_against_finite_differences(c0_box):
+    """Instantaneous potential: per-type LJ (eps kcal/mol, Rmin/2) and
+    per-atom charges as FP64 leaves through torch.autograd, against the
+    4-point stencil of numderiv.c:102 (h = 1e-6) on GPU FP64 energies."""
+    from mdbas_amd.numderiv import derivative
+    torch, AG, water, q, eps_o, rmin_o, zero = _c3_water(c0_box)
+    E = AG.potential_energy(water, q, torch.cat([eps_o, zero]), torch.cat([rmin_o, zero]))
+    E.backward()
+    with torch.no_grad():
+        def fe(qv, ev, rv):
+            return float(AG.potential_energy(water, qv, torch.cat([ev, zero]),
+                                             torch.cat([rv, zero])))
+        fd_e = derivative(lambda d: fe(q, eps_o + d, rmin_o), 1e-6, 4)
+        fd_r = derivative(lambda d: fe(q, eps_o, rmin_o + d), 1e-6, 4)
+        say("adjoint eps", float(eps_o.grad), fd_e, "rmin", float(rmin_o.grad), fd_r)
+        assert float(eps_o.grad) == pytest.approx(fd_e, rel=1e-7)
+        assert float(rmin_o.grad) == pytest.approx(fd_r, rel=1e-7)
+        # E is quadratic in each charge, so the 4-point stencil is exact for
+        # any h; h = 1e-4 keeps the rounding of E (1e-16 of the sum of
+        # |pair terms|, about 1e-8) small against h
+        for i in (0, 1, 2, 1500):
+            dq = torch.zeros_like(q)
+            dq[i] = 1.0
+            fd = derivative(lambda d: fe(q + d * dq, eps_o, rmin_o), 1e-4, 4)
+            assert float(q.grad[i]) == pytest.approx(fd, rel=1e-8, abs=1e-4), i
+    water.close()
+
+
+def test_backward_traj_checkpointing_is_exact(c0_box):
+    """The same 30-step trajectory gradient with one segment and with
+    segments of 20 + 10 steps (recomputed from checkpoints)."""
+    wb = c0_box
+    out = []
+    for ckpt in (30, 20, 7):
+        ctx = M.Context(wb, precision=M.MDG_FP64)
+        ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        ctx.initial_energy()
+        out.append(ctx.backward_traj(30, ckpt))
+        ctx.close()
+    (J0, g0) = out[0]
+    for J, g in out[1:]:
+        assert J == pytest.approx(J0, rel=1e-13)
+        for k in range(3):
+            assert relerr(g[k], g0[k]) < 1e-9
+
+
+def _traj_fd_checks(c0_box, T, K, h):
+    from mdbas_amd.numderiv import derivative
+    torch, AG, water, q, eps_o, rmin_o, zero = _c3_water(c0_box)
+    J = AG.time_averaged_potential(water, q, torch.cat([eps_o, zero]), torch.cat([rmin_o, zero]),
+                                   T, K)
+    J.backward()
+    rng = np.random.default_rng(8)
+    dq = torch.tensor(rng.normal(size=len(q)), dtype=torch.float64)
+    with torch.no_grad():
+        def fj(qv, ev, rv):
+            water._set(qv, torch.sqrt(torch.cat([ev, zero])[water.types] * 418.4),
+                       torch.cat([rv, zero])[water.types] / AG.SIXTH_ROOT_2)
+            water.ctx.initial_energy()
+            R = M.records_to_dict(water.ctx.run(T, use_graph=True))
+            return float(R["pot"].mean())
+        assert fj(q, eps_o, rmin_o) == pytest.approx(float(J), rel=1e-12)
+        checks = [("eps_O", float(eps_o.grad), derivative(lambda d: fj(q, eps_o + d, rmin_o), h, 4)),
+                  ("rmin_O", float(rmin_o.grad), derivative(lambda d: fj(q, eps_o, rmin_o + d), h, 4)),
+                  ("q.dir", float(q.grad @ dq), derivative(lambda d: fj(q + d * dq, eps_o, rmin_o), h, 4))]
+    water.close()
+    for name, g, fd in checks:
+        say("traj adjoint T", T, name, g, fd, (g - fd) / abs(fd))
+    return checks
+
+
+def test_backward_traj_against_finite_differences(c0_box):
+    """Mean potential over 50 steps (dt = 1 fs, checkpoints every 20):
+    the reverse-sweep gradient against the 4-point stencil of
+    numderiv.c:102 (h = 1e-5) on whole FP64 GPU trajectories, for the O
+    eps and Rmin/2 leaves and a random direction in the charges."""
+    for name, g, fd in _traj_fd_checks(c0_box, 50, 20, 1e-5):
+        assert g == pytest.approx(fd, rel=1e-5), name
+
+
+def test_backward_traj_200_steps_against_finite_differences(c0_box):
+    """The 200-step average (checkpoints every 20) with h = 1e-6.  Over
+    200 fs the trajectory's sensitivity grows (the stencils at h = 1e-4,
+    1e-5 and 1e-6 themselves spread by about 3e-4 of the derivative: SHAKE
+    stops on a tolerance, so J(theta) has small steps that the dynamics
+    amplifies), so the bar here is 1e-3; the 50-step test holds 1e-5."""
+    for name, g, fd in _traj_fd_checks(c0_box, 200, 20, 1e-6):
+        assert g == pytest.approx(fd, rel=1e-3), name

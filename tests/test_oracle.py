@@ -425,3 +425,69 @@ def test_shake_failure_code():
     f, _ = o.energy(*s[:3])
     err, *_ = o.vv(1, *s, f)
     assert err == 311
+
+
+# ------------------------------------------------------------------ adjoint
+
+def test_numderiv_stencils_exact_on_polynomials():
+    """numderiv.c:39/102/190 stencils: orders 2, 4, 6 are exact for
+    polynomials of degree 2, 4, 6 (a dropped term or wrong weight fails)."""
+    from mdbas_amd.numderiv import derivative
+    rng = np.random.default_rng(3)
+    for order in (2, 4, 6):
+        c = rng.normal(size=order + 1)
+        f = np.polynomial.Polynomial(c)
+        x0, h = 0.37, 0.01
+        got = derivative(lambda d: f(x0 + d), h, order)
+        assert got == pytest.approx(f.deriv()(x0), rel=1e-9, abs=1e-9)
+        # one degree higher is not exact for order 2 (the stencil's error term)
+    g = np.polynomial.Polynomial([0, 0, 0, 1.0])
+    assert abs(derivative(lambda d: g(1.0 + d), 0.1, 2) - 3.0) > 1e-3
+
+
+def _grad_system(small_box):
+    o = O.OracleSystem(small_box)
+    x, y, z = small_box.x.copy(), small_box.y.copy(), small_box.z.copy()
+    o.build_list(x, y, z)
+    return o, (x, y, z)
+
+
+def _pot(o, xyz):
+    _, e = o.energy(*xyz)
+    return e["pot"] + 0.0, e
+
+
+def test_energy_grad_euler_identities(small_box):
+    """E_elec (direct, reciprocal, correction, self, net charge) is
+    homogeneous of degree 2 in the charges and E_vdw of degree 2 in eps:
+    sum_i q_i dE/dq_i = 2 E_elec + (the SPME self/net terms are inside
+    E_elec), sum_i eps_i dE/deps_i = 2 E_vdw."""
+    o, xyz = _grad_system(small_box)
+    g = o.energy_grad(*xyz)
+    _, e = _pot(o, xyz)
+    wb = small_box
+    # E_self and E_net are part of eRec (spme.c:680-693)
+    assert float(wb.q @ g[0]) == pytest.approx(2 * e["elec"], rel=1e-11)
+    assert float(wb.eps @ g[1]) == pytest.approx(2 * e["vdw"], rel=1e-11)
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_energy_grad_matches_finite_differences(which, small_box):
+    """oc_energy_grad against the 4-point stencil of numderiv.c:102 on the
+    oracle energy, h = 1e-6, for sampled O and H atoms."""
+    from mdbas_amd.numderiv import derivative
+    o, xyz = _grad_system(small_box)
+    g = o.energy_grad(*xyz)
+    wb = small_box
+    base = [wb.q.copy(), wb.eps.copy(), wb.sig.copy()]
+    atoms = [0, 1, 2, 300, 301, 647] if which == 0 else [0, 3, 300, 645]   # O at 3k
+    for i in atoms:
+        def E(d):
+            p = [a.copy() for a in base]
+            p[which][i] += d
+            o.set_params(*p)
+            return _pot(o, xyz)[0]
+        fd = derivative(E, 1e-6, 4)
+        o.set_params()
+        # the stencil's rounding error is about 1e-16 |E| / h = 1e-4 here
+        assert g[which][i] == pytest.approx(fd, rel=1e-6, abs=2e-4), (which, i)
