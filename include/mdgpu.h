@@ -231,6 +231,24 @@ int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec);
  * after the last step: there is no per-step host round trip. */
 int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *records);
 
+/* ---- replica batch ------------------------------------------------------ */
+
+/* nrep independent periodic boxes advanced together in one context: atoms
+ * (and constraints, by global index) of replica r are nAtom[r] consecutive
+ * entries of the concatenated arrays; box[r] is its cubic or orthorhombic
+ * box and mesh[r] its cubic SPME mesh edge; ctrl / param / ewald / neigh
+ * hold the settings every replica shares (param->nAtom is ignored).  Every
+ * replica must be one the reference lists with verlet_list (makelist's
+ * O(N^2) choice, list.c:38-57).  The context then takes mdg_set_topology,
+ * mdg_upload_state, mdg_download_state, mdg_initial_energy (nrep records)
+ * and mdg_run_steps (records[step * nrep + r]); each step is one captured
+ * graph whose kernels cover all replicas.  No reference counterpart (the
+ * reference runs one system per process). */
+mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, const MDG_EWALD *ewald,
+                          const MDG_NEIGH *neigh, int nrep, const int nAtom[], const MDG_PBC box[],
+                          const int mesh[], int precision, int *status);
+int mdg_batch_size(const mdg_ctx *ctx);   /* replicas, 0 for a single system */
+
 /* ---- adjoint ------------------------------------------------------------ */
 
 /* New per-atom charges and LJ parameters (eps = sqrt(eps_kcal*418.4),

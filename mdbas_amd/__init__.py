@@ -85,7 +85,8 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_makelist mdg_ewald_energy mdg_spme_energy mdg_nonbond_energy mdg_energy "
            "mdg_vv_integrate mdg_upload_state mdg_download_state mdg_initial_energy "
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
-           "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj").split()
+           "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj "
+           "mdg_batch_create mdg_batch_size").split()
 
 _LIB = None
 
@@ -110,12 +111,15 @@ def load():
     L.mdg_create.restype = C.c_void_p
     L.mdg_create.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                              C.POINTER(C.c_int)]
+    L.mdg_batch_create.restype = C.c_void_p
+    L.mdg_batch_create.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     L.mdg_destroy.argtypes = [C.c_void_p]
     L.mdg_last_error.restype = C.c_char_p
     L.mdg_last_error.argtypes = [C.c_void_p]
     for f in EXPORTS:
         fn = getattr(L, f)
-        if f not in ("mdg_create", "mdg_destroy", "mdg_last_error"):
+        if f not in ("mdg_create", "mdg_batch_create", "mdg_destroy", "mdg_last_error"):
             fn.restype = C.c_int
     dp = C.POINTER(C.c_double)
     L.mdg_backward.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
@@ -382,6 +386,78 @@ class Context:
         self._chk(self.L.mdg_list_pairs(self.h, P(pi), P(pj), C.c_long(k), C.byref(cnt)),
                   "mdg_list_pairs")
         return pi[:k], pj[:k]
+
+
+class BatchContext(Context):
+    """Many independent water boxes (each small enough for the O(N^2)
+    list) advanced together: one context, every kernel covering all
+    replicas.  Arrays are the replicas' concatenated; records come per step
+    and replica."""
+
+    def __init__(self, boxes, precision: int = MDG_MIXED):
+        self.L = load()
+        self.boxes = boxes
+        self.R = len(boxes)
+        cfg0 = dict(boxes[0].params)
+        self.cfg = cfg0
+        self.ctrl, self.param, _, self.ewald, self.neigh = reference_structs(boxes[0], cfg0)
+        pbcs = (MDG_PBC * self.R)()
+        for k, b in enumerate(boxes):
+            pbcs[k].type = 1
+            pbcs[k].a1 = pbcs[k].b2 = pbcs[k].c3 = float(b.L)
+        nat = np.array([b.n for b in boxes], np.int32)
+        mesh = np.array([b.params["grid"] for b in boxes], np.int32)
+        st = C.c_int(0)
+        h = self.L.mdg_batch_create(C.byref(self.ctrl), C.byref(self.param), C.byref(self.ewald),
+                                    C.byref(self.neigh), self.R, nat.ctypes.data, pbcs,
+                                    mesh.ctypes.data, int(precision), C.byref(st))
+        if not h:
+            raise MdgError(f"mdg_batch_create failed: {st.value} {ERRORS.get(st.value, '')}")
+        self.h = C.c_void_p(h)
+        self.precision = precision
+        self.aoff = np.concatenate([[0], np.cumsum(nat)]).astype(np.int64)
+        self.n = int(self.aoff[-1])
+        cat = lambda k: np.ascontiguousarray(np.concatenate([getattr(b, k) for b in boxes]))  # noqa: E731
+        self.q, self.eps, self.sig = cat("q"), cat("eps"), cat("sig")
+        self.mass, self.rmass = cat("mass"), cat("rmass")
+        self.frozen, self.nAtConst = cat("frozen"), cat("nAtConst")
+        ca = np.concatenate([b.ca + o for b, o in zip(boxes, self.aoff[:-1])])
+        cb = np.concatenate([b.cb + o for b, o in zip(boxes, self.aoff[:-1])])
+        rc2 = np.concatenate([b.rc2 for b in boxes])
+        self.cons = (MDG_CONSTRAINT * max(1, len(ca)))()
+        for k in range(len(ca)):
+            self.cons[k].a, self.cons[k].b, self.cons[k].rc2 = int(ca[k]), int(cb[k]), float(rc2[k])
+        self._chk(self.L.mdg_set_topology(self.h, P(self.q), P(self.eps), P(self.sig),
+                                          P(self.mass), P(self.rmass), P(self.frozen),
+                                          P(self.nAtConst), self.cons, len(ca)),
+                  "mdg_set_topology")
+        self.wb = None
+        self._lists = None
+
+    def upload_boxes(self):
+        cat = lambda k: np.ascontiguousarray(np.concatenate([getattr(b, k) for b in self.boxes]))  # noqa: E731
+        self.upload(*[cat(k) for k in ("x", "y", "z", "vx", "vy", "vz")])
+
+    def split(self, a):
+        """Per-replica views of a concatenated per-atom array."""
+        return [a[self.aoff[r]:self.aoff[r + 1]] for r in range(self.R)]
+
+    def initial_energy(self):
+        recs = (MDG_STEP * self.R)()
+        self._chk(self.L.mdg_initial_energy(self.h, recs), "mdg_initial_energy")
+        return recs
+
+    def run(self, nsteps: int, use_graph: bool = True):
+        """Records as an (nsteps, R) array of MDG_STEP."""
+        recs = (MDG_STEP * max(1, nsteps * self.R))()
+        self._chk(self.L.mdg_run_steps(self.h, int(nsteps), int(use_graph), recs), "mdg_run_steps")
+        return recs
+
+
+def batch_records(recs, R: int):
+    """(steps, R) arrays of every MDG_STEP field."""
+    d = records_to_dict(recs)
+    return {k: v.reshape(-1, R) for k, v in d.items()}
 
 
 def records_to_dict(recs):

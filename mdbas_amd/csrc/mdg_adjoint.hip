@@ -19,7 +19,7 @@
 //        solution that the reference's iteration converges to);
 //      - (dF/dx)^T w = -H w and (dF/dtheta)^T w, as central differences of
 //        the exact forces and of the exact parameter gradient along w
-//        (directional derivatives of first derivatives, step 2e-5 A at the
+//        (directional derivatives of first derivatives, step 5e-6 A at the
 //        largest component), with the terms of the mean-force removal of
 //        spme.c:648-670 kept exact.
 #include "mdg_internal.h"
@@ -551,13 +551,13 @@ struct VjpScratch {
 //   (dF/dtheta)^T u = -d/dh dE/dtheta (x + h u) + (1/N) d/dh dE_rec/dq (x + h s)
 //   (dm/dx)^T s     = (1/N) (dFr_rec/dx) (s, s, ..., s)
 // (the Hessian of E is symmetric); each directional derivative is a
-// central difference with the largest displacement 2e-5 A.
+// central difference with the largest displacement 5e-6 A.
 static int force_vjp(mdg_ctx *ctx, const double *base, const double *u, double umax, double *xb,
                      const VjpScratch &W)
 {
     if (umax <= 0.) return MDG_OK;
     const int n = ctx->n, nbA = nblk(n, 256);
-    const double h = 2e-5 / umax;
+    const double h = 5e-6 / umax;
     // along u
     RETCHK(set_positions(ctx, base, h, u));
     if (xb) {
@@ -592,7 +592,7 @@ static int force_vjp(mdg_ctx *ctx, const double *base, const double *u, double u
     const double sx = sm[0] * n, sy = sm[1] * n, sz = sm[2] * n;
     const double smax = std::max(fabs(sx), std::max(fabs(sy), fabs(sz)));
     if (smax <= 0.) return MDG_OK;
-    const double hs = 2e-5 / smax;
+    const double hs = 5e-6 / smax;
     for (int side = 0; side < 2; side++) {
         const double d = side == 0 ? hs : -hs;
         hipLaunchKernelGGL(k_shift, dim3(nbA), dim3(256), 0, ctx->stream, n, base, d * sx, d * sy,

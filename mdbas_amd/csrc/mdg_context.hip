@@ -311,6 +311,100 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
     return ctx;
 }
 
+// Replica batch: nrep independent boxes (cubic or orthorhombic, each
+// small enough for the O(N^2) list of list.c:896-1040) with their own SPME
+// meshes, concatenated atom by atom.  ctrl / param / ewald / neigh carry
+// the settings all replicas share (param->nAtom is ignored).
+extern "C" mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *param,
+                                     const MDG_EWALD *ewald, const MDG_NEIGH *neigh, int nrep,
+                                     const int nAtom[], const MDG_PBC box[], const int mesh[],
+                                     int precision, int *status)
+{
+    auto fail = [&](mdg_ctx *ctx, int code) -> mdg_ctx * {
+        if (status) *status = code;
+        if (ctx) {
+            fprintf(stderr, "mdg_batch_create: %s\n", ctx->err.c_str());
+            mdg_destroy(ctx);
+        }
+        return nullptr;
+    };
+    if (!param || !ewald || !box || !nAtom || !mesh || nrep <= 0) return fail(nullptr, MDG_EARG);
+    mdg_ctx *ctx = new mdg_ctx();
+    ctx->bt = new MdgBatch();
+    MdgBatch &T = *ctx->bt;
+    T.R = nrep;
+    ctx->precision = precision == MDG_MIXED ? MDG_MIXED : MDG_FP64;
+    params_from_mirror(ctrl, param, ewald, neigh, ctx->P);
+    T.hAoff.assign(nrep + 1, 0);
+    T.hBox.resize(nrep);
+    T.hMesh.assign(mesh, mesh + nrep);
+    T.hColOff.assign(nrep + 1, 0);
+    T.hNfx.resize(nrep);
+    T.hNfy.resize(nrep);
+    const double cutnb = ctx->P.cutOff + ctx->P.delr;
+    for (int r = 0; r < nrep; r++) {
+        if (nAtom[r] <= 0 || mesh[r] < 4) {
+            ctx->err = "every replica needs atoms and a mesh of at least 4";
+            return fail(ctx, MDG_EARG);
+        }
+        T.hAoff[r + 1] = T.hAoff[r] + nAtom[r];
+        box_from_mirror(&box[r], T.hBox[r]);
+        const MdgBox &B = T.hBox[r];
+        if (B.type != 1 && B.type != 2) {
+            ctx->err = "batch replicas need cubic or orthorhombic boxes";
+            return fail(ctx, MDG_EUNSUPPORTED);
+        }
+        // makelist's builder choice (list.c:38-57): this path is the O(N^2) one
+        const int nx = (int)(B.pa / cutnb), ny = (int)(B.pb / cutnb), nz = (int)(B.pc / cutnb);
+        const bool link = !(nx < 3 || ny < 3 || nz < 3 || nx * ny * nz <= 27 || (ctrl && ctrl->noLink));
+        if (link) {
+            ctx->err = "a replica large enough for link cells: create it as its own context";
+            return fail(ctx, MDG_EUNSUPPORTED);
+        }
+        T.hNfx[r] = std::max(1, (int)(B.pa / 4.5));
+        T.hNfy[r] = std::max(1, (int)(B.pb / 4.5));
+        T.hColOff[r + 1] = T.hColOff[r] + T.hNfx[r] * T.hNfy[r];
+    }
+    ctx->n = T.hAoff[nrep];
+    ctx->P.n = ctx->n;
+    ctx->B = T.hBox[0];
+    ctx->keyLink = 0;
+    if (hipGetDevice(&ctx->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->err = "no HIP device / stream";
+        return fail(ctx, MDG_EHIP);
+    }
+    if (int e = mdg_alloc_state(ctx)) return fail(ctx, e);
+    std::vector<int> rep(ctx->n);
+    for (int r = 0; r < nrep; r++)
+        for (int i = T.hAoff[r]; i < T.hAoff[r + 1]; i++) rep[i] = r;
+    int e = MDG_OK;
+    auto up = [&](auto **d, const auto *h, size_t cnt) {
+        if (e) return;
+        if ((e = dmalloc(ctx, d, cnt))) return;
+        e = h2d(ctx, *d, h, cnt);
+    };
+    up(&T.aoff, T.hAoff.data(), nrep + 1);
+    up(&T.arep, rep.data(), ctx->n);
+    up(&T.boxes, T.hBox.data(), nrep);
+    up(&T.mesh, T.hMesh.data(), nrep);
+    up(&T.colOff, T.hColOff.data(), nrep + 1);
+    up(&T.nfx, T.hNfx.data(), nrep);
+    up(&T.nfy, T.hNfy.data(), nrep);
+    if (e) return fail(ctx, e);
+    if ((e = dmalloc(ctx, &T.goff, nrep + 1)) || (e = dmalloc(ctx, &T.eSelf, nrep)) ||
+        (e = dmalloc(ctx, &T.eNet, nrep)) || (e = dmalloc(ctx, &T.stepCtr, nrep)) ||
+        (e = dmalloc(ctx, &T.masks, (size_t)nrep * 4)) ||
+        (e = dmalloc(ctx, &T.atomE, 2 * (size_t)ctx->n)))
+        return fail(ctx, e);
+    if ((e = mdg_b_spme_setup(ctx))) return fail(ctx, e);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, MDG_EHIP);
+    if (status) *status = MDG_OK;
+    return ctx;
+}
+
+extern "C" int mdg_batch_size(const mdg_ctx *ctx) { return ctx && ctx->bt ? ctx->bt->R : 0; }
+
 extern "C" void mdg_destroy(mdg_ctx *ctx)
 {
     if (!ctx) return;
@@ -338,6 +432,17 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->adjS, ctx->adjPot};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (ctx->bt) {
+        MdgBatch &T = *ctx->bt;
+        mdg_b_spme_free(ctx);
+        void *bp[] = {T.aoff, T.goff, T.arep, T.grep, T.colOff, T.nfx, T.nfy, T.mesh, T.gridOff,
+                      T.hOff, T.boxes, T.eSelf, T.eNet, T.cluStart, T.crep, T.chE, T.atomE,
+                      T.grpKin, T.repInfl, T.repRec, T.masks, T.rec, T.stepCtr};
+        for (void *p : bp)
+            if (p) hipFree(p);
+        delete ctx->bt;
+        ctx->bt = nullptr;
+    }
     for (int k = 0; k < 8; k++)
         if (ctx->scanBuf[k]) hipFree(ctx->scanBuf[k]);
     for (hipEvent_t e : ctx->ev)
@@ -364,6 +469,73 @@ static void charge_sums(mdg_ctx *ctx, const double q[])
     ctx->eNet = ctx->B.vol > 0 && ctx->P.alpha > 0
         ? -(0.5 * PI_C * ctx->P.chargeConst) * ((systq / ctx->P.alpha) * (systq / ctx->P.alpha) / ctx->B.vol)
         : 0.;
+}
+
+// Per-replica self and net-charge energies (spme.c:376-389, 680 with each
+// replica's own charges and volume).
+static int batch_charge_sums(mdg_ctx *ctx, const double q[])
+{
+    MdgBatch &T = *ctx->bt;
+    T.hESelf.assign(T.R, 0.);
+    T.hENet.assign(T.R, 0.);
+    T.hSystq.assign(T.R, 0.);
+    for (int r = 0; r < T.R; r++) {
+        double sq = 0., q2 = 0.;
+        for (int i = T.hAoff[r]; i < T.hAoff[r + 1]; i++) {
+            sq += q[i];
+            q2 += q[i] * q[i];
+        }
+        const double a = ctx->P.alpha, v = T.hBox[r].vol;
+        T.hSystq[r] = sq;
+        T.hESelf[r] = -ctx->P.chargeConst * a * q2 / SQRTPI_C;
+        T.hENet[r] = -(0.5 * PI_C * ctx->P.chargeConst) * ((sq / a) * (sq / a) / v);
+    }
+    RETCHK(h2d(ctx, T.eSelf, T.hESelf.data(), T.R));
+    RETCHK(h2d(ctx, T.eNet, T.hENet.data(), T.R));
+    return sync(ctx);
+}
+
+// Replica of every constraint group and each replica's group range (the
+// groups follow the atom order, so a replica's groups are contiguous).
+static int batch_topology(mdg_ctx *ctx, const double q[])
+{
+    MdgBatch &T = *ctx->bt;
+    const int ng = ctx->nGrp;
+    std::vector<int> off(ng + 1), atoms(ctx->n);
+    RETCHK(d2h(ctx, off.data(), ctx->grpAtomOff, ng + 1));
+    RETCHK(d2h(ctx, atoms.data(), ctx->grpAtoms, ctx->n));
+    RETCHK(sync(ctx));
+    std::vector<int> rep(ctx->n), grep(std::max(ng, 1));
+    for (int r = 0; r < T.R; r++)
+        for (int i = T.hAoff[r]; i < T.hAoff[r + 1]; i++) rep[i] = r;
+    T.hGoff.assign(T.R + 1, ng);
+    for (int g = ng - 1; g >= 0; g--) {
+        const int r = rep[atoms[off[g]]];
+        for (int k = off[g]; k < off[g + 1]; k++)
+            if (rep[atoms[k]] != r) {
+                ctx->err = "a constraint group spans two replicas";
+                return MDG_EARG;
+            }
+        if (g > 0 && rep[atoms[off[g - 1]]] > r) {
+            ctx->err = "constraint groups out of replica order";
+            return MDG_EARG;
+        }
+        grep[g] = r;
+        T.hGoff[r] = g;
+    }
+    for (int r = T.R - 1; r >= 0; r--)   // replicas without groups: empty ranges
+        if (T.hGoff[r] > T.hGoff[r + 1]) T.hGoff[r] = T.hGoff[r + 1];
+    if (T.grep) HIPCHK(hipFree(T.grep));
+    if (T.grpKin) HIPCHK(hipFree(T.grpKin));
+    RETCHK(dmalloc(ctx, &T.grep, std::max(ng, 1)));
+    RETCHK(dmalloc(ctx, &T.grpKin, std::max(ng, 1)));
+    RETCHK(h2d(ctx, T.grep, grep.data(), ng));
+    RETCHK(h2d(ctx, T.goff, T.hGoff.data(), T.R + 1));
+    if (!ctx->shakeW3) {
+        ctx->err = "replica batches integrate rigid three-site molecules";
+        return MDG_EUNSUPPORTED;
+    }
+    return batch_charge_sums(ctx, q);
 }
 
 extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps[],
@@ -408,6 +580,7 @@ extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps
     }
     RETCHK(mdg_build_groups(ctx, constList, nConst));
     RETCHK(mdg_build_exclusions(ctx, 0, nullptr, nConst, constList, 0, nullptr));
+    if (ctx->bt) RETCHK(batch_topology(ctx, q));
     mdg_invalidate_graphs(ctx);
     return sync(ctx);
 }
@@ -424,6 +597,7 @@ extern "C" int mdg_set_nonbond_params(mdg_ctx *ctx, const double q[], const doub
     RETCHK(h2d(ctx, ctx->eps, eps, n));
     RETCHK(h2d(ctx, ctx->sig, sig, n));
     charge_sums(ctx, q);
+    if (ctx->bt) RETCHK(batch_charge_sums(ctx, q));
     mdg_invalidate_graphs(ctx);
     if (ctx->cl.cpJ && ctx->sx) RETCHK(mdg_slot_params(ctx));
     return sync(ctx);

@@ -618,7 +618,21 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.first = ctx->shkFirst;
     A.kend = ctx->shkKend;
     A.mask = ctx->shkMask;
+    A.boxes = nullptr;
+    A.grep = nullptr;
+    A.grpKin = nullptr;
     const bool w3 = ctx->shakeW3;
+    const bool batch = ctx->bt != nullptr;
+    if (batch) {
+        if (!w3 || sync) {
+            ctx->err = "replica batches integrate rigid three-site molecules on the device";
+            return MDG_EUNSUPPORTED;
+        }
+        A.boxes = ctx->bt->boxes;
+        A.grep = ctx->bt->grep;
+        A.grpKin = ctx->bt->grpKin;
+        A.mask = ctx->bt->masks;
+    }
     const int bs = w3 ? 64 : 128;
     const int nb = nblk(ctx->nGrp, bs);
     // The device-resident step resets the mask in the kernels that run
@@ -629,16 +643,20 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         LAUNCHCHK();
     }
     if (stage == 1) {
-        hipLaunchKernelGGL(w3 ? k_w3_vv1_a : k_vv1_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
+        hipLaunchKernelGGL(batch ? k_w3_vv1_a<true> : w3 ? k_w3_vv1_a<false> : k_vv1_a, dim3(nb),
+                           dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
-        hipLaunchKernelGGL(w3 ? k_w3_vv1_b : k_vv1_b, dim3(nb), dim3(bs), 0, ctx->stream, A);
+        hipLaunchKernelGGL(batch ? k_w3_vv1_b<true> : w3 ? k_w3_vv1_b<false> : k_vv1_b, dim3(nb),
+                           dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         // SHAKE virial: sum of the per-group sums (fixed order)
         if (sync) RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, ctx->nGrp, 1, ctx->scal + S_VIRSHAKE));
     } else {
-        hipLaunchKernelGGL(w3 ? k_w3_vv2_a : k_vv2_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
+        hipLaunchKernelGGL(batch ? k_w3_vv2_a<true> : w3 ? k_w3_vv2_a<false> : k_vv2_a, dim3(nb),
+                           dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
-        hipLaunchKernelGGL(w3 ? k_w3_vv2_b : k_vv2_b, dim3(nb), dim3(bs), 0, ctx->stream, A);
+        hipLaunchKernelGGL(batch ? k_w3_vv2_b<true> : w3 ? k_w3_vv2_b<false> : k_vv2_b, dim3(nb),
+                           dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         if (sync) RETCHK(mdg_reduce_partials(ctx, A.part, nb, 7, ctx->scal + S_KIN));
     }

@@ -84,6 +84,43 @@ struct ClusterList {
     float *sortKey = nullptr;
 };
 
+// Replica batch (mdg_batch.hip): many independent boxes concatenated in
+// one context.  Atoms, constraint groups, clusters and chunks of replica r
+// are contiguous; every kernel of a step covers all replicas and reads the
+// replica's box, mesh and offsets from these tables.
+struct MdgFftGroup {
+    int m = 0, count = 0;        // cubic mesh edge and replicas in the group
+    long g0 = 0, h0 = 0;         // first real-mesh / half-spectrum element
+    rocfft_plan fwd = nullptr, bwd = nullptr, fwd32 = nullptr, bwd32 = nullptr;
+};
+
+struct MdgBatch {
+    int R = 0;
+    std::vector<int> hAoff, hGoff, hMesh, hColOff, hNfx, hNfy;
+    std::vector<long> hGridOff, hHOff;
+    std::vector<MdgBox> hBox;
+    std::vector<double> hESelf, hENet, hSystq;
+    int *aoff = nullptr, *goff = nullptr, *arep = nullptr, *grep = nullptr;
+    int *colOff = nullptr, *nfx = nullptr, *nfy = nullptr, *mesh = nullptr;
+    long *gridOff = nullptr, *hOff = nullptr;
+    MdgBox *boxes = nullptr;
+    double *eSelf = nullptr, *eNet = nullptr;
+    int *cluStart = nullptr;     // per replica first cluster (R + 1), per build
+    int *crep = nullptr;         // per cluster: replica (-1 empty)
+    double *chE = nullptr;       // per chunk: eDir, eVdw, virDir, virVdw, nIn
+    double *atomE = nullptr;     // per atom: correction energy, virial
+    double *grpKin = nullptr;    // per group: sum m v^2
+    double *repInfl = nullptr;   // per replica: sum theta|F|^2, st0..st5
+    double *repRec = nullptr;    // per replica: sums of the reciprocal forces
+    unsigned long long *masks = nullptr;   // per replica SHAKE masks
+    MDG_STEP *rec = nullptr;     // [step][replica]
+    int recCap = 0;              // steps
+    int *stepCtr = nullptr;      // per replica: next record index
+    long Gtot = 0, Htot = 0;
+    std::vector<MdgFftGroup> fft;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
 struct mdg_ctx {
     int precision = MDG_FP64;
     int n = 0;
@@ -178,6 +215,7 @@ struct mdg_ctx {
 
     // graph
     hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    MdgBatch *bt = nullptr;      // replica batch, when created by mdg_batch_create
     int graphStepsPerLaunch = 0;
     int step = 0;                // last completed MD step
     int stepsSinceList = 0;
@@ -211,7 +249,7 @@ enum {
 // regions of ctx->epart (per-block partial sums), each PART_R doubles
 #define PART_R (1 << 18)
 // the cluster pair kernel loops over chunks with at most this many blocks
-#define MDG_PAIR_MAX_BLOCKS 16384
+#define MDG_PAIR_MAX_BLOCKS 4096
 enum {
     PART_PAIR = 0, PART_EXCL = PART_R, PART_INFL = 2 * PART_R, PART_GATHER = 3 * PART_R,
     PART_KIN = 4 * PART_R, PART_MISC = 5 * PART_R, PART_REF = 6 * PART_R, PART_TOTAL = 8 * PART_R
@@ -390,3 +428,10 @@ void mdg_invalidate_graphs(mdg_ctx *ctx);
 int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst);
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
                  double *virshake_out, bool sync);
+
+// replica batch pieces
+int mdg_b_build_list(mdg_ctx *ctx);                  // mdg_list.hip
+int mdg_b_spme_setup(mdg_ctx *ctx);                  // mdg_spme.hip
+int mdg_b_spme(mdg_ctx *ctx);
+void mdg_b_spme_free(mdg_ctx *ctx);
+int mdg_b_pair(mdg_ctx *ctx);                        // mdg_pair.hip

@@ -596,6 +596,69 @@ __global__ void k_chunk_fill(int nclu, const int *__restrict__ cpOff, const int 
     }
 }
 
+// Column sort shared by both builders: atoms placed by column (colOf,
+// counts in C.colCnt), sorted by (z key, atom) inside each column, columns
+// padded to whole clusters, slot <-> atom maps written.
+static int sort_columns(mdg_ctx *ctx, const int *colOf, unsigned long long *keys)
+{
+    const int n = ctx->n;
+    ClusterList &C = ctx->cl;
+    RETCHK(mdg_scan_int(ctx, C.colCnt, C.colStart, C.ncol));
+    hipLaunchKernelGGL(k_col_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, colOf,
+                       C.colStart, C.colCnt + C.ncol + 1, C.sortKey, keys);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_segment_sort, dim3(C.ncol), dim3(256), 0, ctx->stream, keys, C.colStart,
+                       ctx->ctrl);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_col_pad, dim3(nblk(C.ncol, 256)), dim3(256), 0, ctx->stream, C.ncol,
+                       C.colCnt, C.colPad);
+    LAUNCHCHK();
+    RETCHK(mdg_scan_int(ctx, C.colPad, C.colSlot, C.ncol));
+    hipLaunchKernelGGL(k_col_slots, dim3(C.ncol), dim3(64), 0, ctx->stream, C.ncol, C.colStart,
+                       C.colCnt, C.colSlot, keys, C.slotAtom, C.atomSlot);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+// Pair-count scan, list compaction and work chunks, shared by both
+// builders (C.tmpCnt / tmpJ / tmpMask filled by the pair search).  The
+// first call sizes the list from its own count with headroom.
+static int compact_cluster_list(mdg_ctx *ctx)
+{
+    ClusterList &C = ctx->cl;
+    const int nclu = C.nclu;
+    RETCHK(mdg_scan_int(ctx, C.tmpCnt, C.cpOff, nclu));
+    if (C.cap == 0) {
+        // first build: size the list from this count with headroom
+        int total = 0;
+        HIPCHK(hipMemcpyAsync(&total, C.cpOff + nclu, sizeof(int), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        C.cap = (long)(total * 1.5) + 4096;
+        HIPCHK(hipMalloc(&C.cpJ, C.cap * sizeof(int)));
+        HIPCHK(hipMalloc(&C.cpI, C.cap * sizeof(int)));
+        HIPCHK(hipMalloc(&C.cpMask, C.cap * sizeof(unsigned long long)));
+        C.maxChunks = (int)(C.cap / MDG_CHUNK) + nclu + 64;
+        HIPCHK(hipMalloc(&C.chunk, (size_t)C.maxChunks * sizeof(int2)));
+        HIPCHK(hipMalloc(&C.chOff, (nclu + 1) * sizeof(int)));
+    }
+    hipLaunchKernelGGL(k_cluster_compact, dim3(nclu), dim3(64), 0, ctx->stream, nclu, C.kmax,
+                       C.tmpCnt, C.tmpJ, C.tmpMask, C.cpOff, C.cap, C.cpJ, C.cpMask, C.cpI,
+                       ctx->ctrl);
+    LAUNCHCHK();
+    // work chunks of at most MDG_CHUNK entries for the pair kernel; the
+    // per-I chunk counts reuse tmpCnt's space after the compaction
+    int *nch = C.sortTmp;   // n ints, free again after the column sort (n >= nclu)
+    hipLaunchKernelGGL(k_chunk_count, dim3(nblk(nclu, 256)), dim3(256), 0, ctx->stream, nclu,
+                       C.tmpCnt, nch);
+    LAUNCHCHK();
+    RETCHK(mdg_scan_int(ctx, nch, C.chOff, nclu));
+    hipLaunchKernelGGL(k_chunk_fill, dim3(nblk(nclu, 256)), dim3(256), 0, ctx->stream, nclu,
+                       C.cpOff, C.chOff, C.maxChunks, C.chunk, ctx->ctrl);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
 // Builds the cluster-pair list from ctx->x,y,z (device).  Every call is
 // graph-capturable once the buffers exist (first call sizes them).
 int mdg_build_cluster_list(mdg_ctx *ctx)
@@ -640,20 +703,7 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
     hipLaunchKernelGGL(k_col_assign, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->B, n,
                        C.nfx, C.nfy, ctx->x, ctx->y, ctx->z, colOf, C.colCnt, C.sortKey);
     LAUNCHCHK();
-    RETCHK(mdg_scan_int(ctx, C.colCnt, C.colStart, C.ncol));
-    hipLaunchKernelGGL(k_col_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, colOf,
-                       C.colStart, C.colCnt + C.ncol + 1, C.sortKey, keys);
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_segment_sort, dim3(C.ncol), dim3(256), 0, ctx->stream, keys, C.colStart,
-                       ctx->ctrl);
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_col_pad, dim3(nblk(C.ncol, 256)), dim3(256), 0, ctx->stream, C.ncol,
-                       C.colCnt, C.colPad);
-    LAUNCHCHK();
-    RETCHK(mdg_scan_int(ctx, C.colPad, C.colSlot, C.ncol));
-    hipLaunchKernelGGL(k_col_slots, dim3(C.ncol), dim3(64), 0, ctx->stream, C.ncol, C.colStart,
-                       C.colCnt, C.colSlot, keys, C.slotAtom, C.atomSlot);
-    LAUNCHCHK();
+    RETCHK(sort_columns(ctx, colOf, keys));
     // The number of clusters varies with the padding; kernels run over the
     // maximum and empty clusters produce nothing.
     const int nclu = C.nclu;
@@ -680,34 +730,211 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
                        ctx->cellOf, ctx->frozen, ctx->exOff, ctx->exLst, C.kmax, C.tmpCnt, C.tmpJ,
                        C.tmpMask, ctx->ctrl);
     LAUNCHCHK();
-    RETCHK(mdg_scan_int(ctx, C.tmpCnt, C.cpOff, nclu));
-    if (C.cap == 0) {
-        // first build: size the list from this count with headroom
-        int total = 0;
-        HIPCHK(hipMemcpyAsync(&total, C.cpOff + nclu, sizeof(int), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        C.cap = (long)(total * 1.5) + 4096;
-        HIPCHK(hipMalloc(&C.cpJ, C.cap * sizeof(int)));
-        HIPCHK(hipMalloc(&C.cpI, C.cap * sizeof(int)));
-        HIPCHK(hipMalloc(&C.cpMask, C.cap * sizeof(unsigned long long)));
-        C.maxChunks = (int)(C.cap / MDG_CHUNK) + nclu + 64;
-        HIPCHK(hipMalloc(&C.chunk, (size_t)C.maxChunks * sizeof(int2)));
-        HIPCHK(hipMalloc(&C.chOff, (nclu + 1) * sizeof(int)));
+    return compact_cluster_list(ctx);
+}
+
+// ---------------------------------------------------------------- replica batch
+
+// Column of each atom in its own replica's xy grid (replica r owns columns
+// colOff[r] .. colOff[r+1]-1), so clusters never mix replicas.
+__global__ void k_b_col_assign(const MdgBox *__restrict__ boxes, const int *__restrict__ arep,
+                               const int *__restrict__ colOff, const int *__restrict__ nfx,
+                               const int *__restrict__ nfy, int n, const double *__restrict__ x,
+                               const double *__restrict__ y, const double *__restrict__ z,
+                               int *__restrict__ colOf, int *__restrict__ colCnt,
+                               float *__restrict__ zkey)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = arep[i];
+    const MdgBox &B = boxes[r];
+    double a = x[i] * B.u1 + y[i] * B.u2 + z[i] * B.u3 + 0.5;
+    double b = x[i] * B.v1 + y[i] * B.v2 + z[i] * B.v3 + 0.5;
+    double c = x[i] * B.w1 + y[i] * B.w2 + z[i] * B.w3 + 0.5;
+    a -= floor(a);
+    b -= floor(b);
+    c -= floor(c);
+    const int ncx = nfx[r], ncy = nfy[r];
+    int cx = min((int)(a * ncx), ncx - 1), cy = min((int)(b * ncy), ncy - 1);
+    int col = colOff[r] + cx + ncx * cy;
+    colOf[i] = col;
+    zkey[i] = (float)c;
+    atomicAdd(&colCnt[col], 1);
+}
+
+// First cluster of each replica (and the end, at R) from the column slots.
+__global__ void k_b_clu_ranges(int R, const int *__restrict__ colOff, const int *__restrict__ colSlot,
+                               int *__restrict__ cluStart)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= R) cluStart[r] = colSlot[colOff[r]] / MDG_CL;
+}
+
+// Bounding box and replica of each cluster (the replica's box wraps it).
+__global__ void k_b_cluster_bbox(const MdgBox *__restrict__ boxes, const int *__restrict__ arep,
+                                 int nclu, const int *__restrict__ slotAtom,
+                                 const double *__restrict__ x, const double *__restrict__ y,
+                                 const double *__restrict__ z, float4 *__restrict__ bb,
+                                 float4 *__restrict__ bbh, int *__restrict__ crep)
+{
+    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nclu) return;
+    const int a0 = slotAtom[I * MDG_CL];   // a used cluster starts with an atom
+    if (a0 < 0) {
+        crep[I] = -1;
+        bb[I] = make_float4(0.f, 0.f, 0.f, 0.f);
+        bbh[I] = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
+        return;
     }
-    hipLaunchKernelGGL(k_cluster_compact, dim3(nclu), dim3(64), 0, ctx->stream, nclu, C.kmax,
-                       C.tmpCnt, C.tmpJ, C.tmpMask, C.cpOff, C.cap, C.cpJ, C.cpMask, C.cpI,
-                       ctx->ctrl);
+    const int r = arep[a0];
+    const MdgBox &B = boxes[r];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int k = 0; k < MDG_CL; k++) {
+        int a = slotAtom[I * MDG_CL + k];
+        if (a < 0) continue;
+        double s[3] = {x[a] * B.u1 + y[a] * B.u2 + z[a] * B.u3, x[a] * B.v1 + y[a] * B.v2 + z[a] * B.v3,
+                       x[a] * B.w1 + y[a] * B.w2 + z[a] * B.w3};
+        for (int d = 0; d < 3; d++) s[d] -= floor(s[d] + 0.5);
+        double p[3] = {s[0] * B.a1 + s[1] * B.b1 + s[2] * B.c1, s[0] * B.a2 + s[1] * B.b2 + s[2] * B.c2,
+                       s[0] * B.a3 + s[1] * B.b3 + s[2] * B.c3};
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], p[d]);
+            hi[d] = fmax(hi[d], p[d]);
+        }
+    }
+    crep[I] = r;
+    bb[I] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                        (float)(0.5 * (lo[2] + hi[2])), 0.f);
+    bbh[I] = make_float4((float)(0.5 * (hi[0] - lo[0])) + 1e-3f, (float)(0.5 * (hi[1] - lo[1])) + 1e-3f,
+                         (float)(0.5 * (hi[2] - lo[2])) + 1e-3f, 0.f);
+}
+
+// k_cluster_pairs for a batch: J runs over I's own replica only, with that
+// replica's box, and the O(N^2) predicate of verlet_list (list.c:896-1040).
+__global__ void __launch_bounds__(256)
+k_b_cluster_pairs(MdgParams P, const MdgBox *__restrict__ boxes, const int *__restrict__ crep,
+                  const int *__restrict__ cluStart, int nclu, const int *__restrict__ slotAtom,
+                  const float4 *__restrict__ bb, const float4 *__restrict__ bbh,
+                  const double *__restrict__ x, const double *__restrict__ y,
+                  const double *__restrict__ z, const int *__restrict__ frozen,
+                  const int *__restrict__ exOff, const int *__restrict__ exLst, int kmax,
+                  int *__restrict__ tmpCnt, int *__restrict__ tmpJ,
+                  unsigned long long *__restrict__ tmpMask, int *__restrict__ ctrl)
+{
+    const int lane = threadIdx.x & 63;
+    const int I = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (I >= nclu) return;
+    const int r = crep[I];
+    if (r < 0) {
+        if (lane == 0) tmpCnt[I] = 0;
+        return;
+    }
+    const MdgBox &B = boxes[r];
+    const int jend = cluStart[r + 1];
+    const float4 ci = bb[I], hi = bbh[I];
+    const float L0 = (float)B.a1, L1 = (float)B.b2, L2 = (float)B.c3;
+    const float rl = (float)(P.cutOff + P.delr) + 0.01f;
+    const int a = lane >> 3, b = lane & 7;
+    const int iat = slotAtom[I * MDG_CL + a];
+    int found = 0;
+    for (int J0 = I; J0 < jend; J0 += 64) {
+        int J = J0 + lane;
+        bool cand = false;
+        if (J < jend) {
+            float4 cj = bb[J], hj = bbh[J];
+            float dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
+            dx -= L0 * rintf(dx / L0);
+            dy -= L1 * rintf(dy / L1);
+            dz -= L2 * rintf(dz / L2);
+            dx = fmaxf(fabsf(dx) - hi.x - hj.x, 0.f);
+            dy = fmaxf(fabsf(dy) - hi.y - hj.y, 0.f);
+            dz = fmaxf(fabsf(dz) - hi.z - hj.z, 0.f);
+            cand = dx * dx + dy * dy + dz * dz <= rl * rl;
+        }
+        unsigned long long cm = __ballot(cand);
+        while (cm) {
+            int src = __ffsll((long long)cm) - 1;
+            cm &= cm - 1;
+            int Jc = J0 + src;
+            int jat = slotAtom[Jc * MDG_CL + b];
+            bool pr = false;
+            if (iat >= 0 && jat >= 0 && (Jc != I || a < b))
+                pr = ref_pair_pred(P, B, 0, 0, 0, 0, iat, jat, x, y, z, nullptr, nullptr, nullptr,
+                                   nullptr, frozen, exOff, exLst);
+            unsigned long long m = __ballot(pr);
+            if (m) {
+                if (found < kmax && lane == 0) {
+                    tmpJ[(long)I * kmax + found] = Jc;
+                    tmpMask[(long)I * kmax + found] = m;
+                }
+                found++;
+            }
+        }
+    }
+    if (lane == 0) {
+        tmpCnt[I] = min(found, kmax);
+        if (found > kmax) atomicOr(&ctrl[CTRL_LIST_OVF], 1);
+    }
+}
+
+// Cluster-pair list of every replica of a batch (all O(N^2) builders).
+int mdg_b_build_list(mdg_ctx *ctx)
+{
+    MdgBatch &T = *ctx->bt;
+    const int n = ctx->n, R = T.R;
+    ClusterList &C = ctx->cl;
+    if (!C.colCnt) {
+        C.ncol = T.hColOff[R];
+        int maxslot = n + (MDG_CL - 1) * C.ncol;
+        int maxclu = maxslot / MDG_CL + 1;
+        HIPCHK(hipMalloc(&C.colCnt, 2 * (C.ncol + 1) * sizeof(int)));
+        HIPCHK(hipMalloc(&C.colStart, (C.ncol + 1) * sizeof(int)));
+        HIPCHK(hipMalloc(&C.colPad, (C.ncol + 1) * sizeof(int)));
+        HIPCHK(hipMalloc(&C.colSlot, (C.ncol + 1) * sizeof(int)));
+        HIPCHK(hipMalloc(&C.sortTmp, n * sizeof(int)));
+        HIPCHK(hipMalloc(&C.sortKey, n * sizeof(float)));
+        HIPCHK(hipMalloc(&C.slotAtom, ((size_t)maxclu * MDG_CL + 64) * sizeof(int)));
+        HIPCHK(hipMalloc(&C.atomSlot, n * sizeof(int)));
+        HIPCHK(hipMalloc(&C.bb, maxclu * sizeof(float4)));
+        HIPCHK(hipMalloc(&C.bbh, maxclu * sizeof(float4)));
+        HIPCHK(hipMalloc(&C.cpOff, (maxclu + 1) * sizeof(int)));
+        HIPCHK(hipMalloc(&C.tmpCnt, maxclu * sizeof(int)));
+        HIPCHK(hipMalloc(&T.cluStart, (R + 1) * sizeof(int)));
+        HIPCHK(hipMalloc(&T.crep, maxclu * sizeof(int)));
+        // every cluster of the largest replica, padding included
+        int kmax = 64;
+        for (int r = 0; r < R; r++) {
+            int nr = T.hAoff[r + 1] - T.hAoff[r], ncr = T.hColOff[r + 1] - T.hColOff[r];
+            kmax = std::max(kmax, (nr + (MDG_CL - 1) * ncr) / MDG_CL + 1);
+        }
+        C.kmax = std::max(kmax, (n + maxclu - 1) / maxclu);   // keys reuse tmpMask
+        HIPCHK(hipMalloc(&C.tmpJ, (size_t)maxclu * C.kmax * sizeof(int)));
+        HIPCHK(hipMalloc(&C.tmpMask, (size_t)maxclu * C.kmax * sizeof(unsigned long long)));
+        C.nclu = maxclu;
+        C.cap = 0;
+    }
+    unsigned long long *keys = (unsigned long long *)C.tmpMask;
+    int *colOf = C.sortTmp;
+    RETCHK(mdg_fill_i32(ctx, C.slotAtom, (long)C.nclu * MDG_CL + 64, -1));
+    RETCHK(mdg_fill_i32(ctx, C.colCnt, 2 * (C.ncol + 1), 0));
+    hipLaunchKernelGGL(k_b_col_assign, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, T.boxes,
+                       T.arep, T.colOff, T.nfx, T.nfy, n, ctx->x, ctx->y, ctx->z, colOf, C.colCnt,
+                       C.sortKey);
     LAUNCHCHK();
-    // work chunks of at most MDG_CHUNK entries for the pair kernel; the
-    // per-I chunk counts reuse tmpCnt's space after the compaction
-    int *nch = C.sortTmp;   // n ints, free again after the column sort (n >= nclu)
-    hipLaunchKernelGGL(k_chunk_count, dim3(nblk(nclu, 256)), dim3(256), 0, ctx->stream, nclu,
-                       C.tmpCnt, nch);
+    RETCHK(sort_columns(ctx, colOf, keys));
+    hipLaunchKernelGGL(k_b_clu_ranges, dim3(nblk(R + 1, 256)), dim3(256), 0, ctx->stream, R,
+                       T.colOff, C.colSlot, T.cluStart);
     LAUNCHCHK();
-    RETCHK(mdg_scan_int(ctx, nch, C.chOff, nclu));
-    hipLaunchKernelGGL(k_chunk_fill, dim3(nblk(nclu, 256)), dim3(256), 0, ctx->stream, nclu,
-                       C.cpOff, C.chOff, C.maxChunks, C.chunk, ctx->ctrl);
+    const int nclu = C.nclu;
+    hipLaunchKernelGGL(k_b_cluster_bbox, dim3(nblk(nclu, 128)), dim3(128), 0, ctx->stream, T.boxes,
+                       T.arep, nclu, C.slotAtom, ctx->x, ctx->y, ctx->z, C.bb, C.bbh, T.crep);
     LAUNCHCHK();
+    hipLaunchKernelGGL(k_b_cluster_pairs, dim3(nblk(nclu, 4)), dim3(256), 0, ctx->stream, ctx->P,
+                       T.boxes, T.crep, T.cluStart, nclu, C.slotAtom, C.bb, C.bbh, ctx->x, ctx->y,
+                       ctx->z, ctx->frozen, ctx->exOff, ctx->exLst, C.kmax, C.tmpCnt, C.tmpJ,
+                       C.tmpMask, ctx->ctrl);
+    LAUNCHCHK();
+    RETCHK(compact_cluster_list(ctx));
+    if (!T.chE) HIPCHK(hipMalloc(&T.chE, (size_t)C.maxChunks * 5 * sizeof(double)));
     return MDG_OK;
 }

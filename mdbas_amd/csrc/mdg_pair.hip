@@ -256,10 +256,15 @@ struct ClusterArgs {
     const int *cpOff, *cpJ;
     const unsigned long long *cpMask;
     const int *chOff;                // per i-cluster chunk offsets (nclu + 1)
+    int maxChunks;                   // chunk capacity (an overflowing build is flagged)
+    int cap;                         // entry capacity of cpJ / cpMask
     const int2 *chunk;               // chunk -> (I, first entry)
     long long *fix, *fiy, *fiz;      // MIXED: int64 fixed-point slot forces
     double *fdx, *fdy, *fdz;         // FP64: slot forces
     double *part;                    // per block: eDir, eVdw, virDir, virVdw, nIn
+    const MdgBox *boxes;             // batch: box of each replica
+    const int *crep;                 // batch: replica of each cluster
+    double *chE;                     // batch: per chunk eDir, eVdw, virDir, virVdw, nIn
 };
 
 // Sum over the 8 lanes of each aligned group of 8 with DPP moves (plain
@@ -298,18 +303,33 @@ __device__ inline double dpp_sum8(double v)
 // each j-cluster pair's j-force sums run over 8 adjacent lanes (DPP) and
 // the i-force sums over lanes a, a+8, ..., a+56 happen once per chunk.
 // Mask bit a*8+b marks the listed pair (I_a, J_b).
-template <bool MIXED>
+template <bool MIXED, bool BATCH>
 __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
 {
     const int lane = threadIdx.x & 63;
     const int a = lane & 7, b = lane >> 3;
     const int bit = (a << 3) | b;
     double e_acc[5] = {0, 0, 0, 0, 0};
-    const int nch = A.chOff[A.nclu];
+    const int nch = min(A.chOff[A.nclu], A.maxChunks);
     for (int W = blockIdx.x * 4 + (threadIdx.x >> 6); W < nch; W += gridDim.x * 4) {
         // chunk W: entries [e0, e1) of i-cluster I (at most MDG_CHUNK of them)
         const int2 ch = A.chunk[W];
-        const int I = ch.x, e0 = ch.y, e1 = min(e0 + MDG_CHUNK, A.cpOff[I + 1]);
+        const int I = ch.x, e0 = ch.y, e1 = min(min(e0 + MDG_CHUNK, A.cpOff[I + 1]), A.cap);
+        if (e0 >= e1) {   // only after a flagged list overflow
+            if (BATCH && lane < 5) A.chE[(long)W * 5 + lane] = 0.;
+            continue;
+        }
+        // the box: the run's, or the replica's in a batch
+        const MdgBox &Bx = BATCH ? A.boxes[A.crep[I]] : A.B;
+        double L0 = A.F.L0, L1 = A.F.L1, L2 = A.F.L2, iL0 = A.F.iL0, iL1 = A.F.iL1, iL2 = A.F.iL2;
+        if (BATCH) {
+            L0 = Bx.a1;
+            L1 = Bx.type == 1 ? Bx.a1 : Bx.b2;
+            L2 = Bx.type == 1 ? Bx.a1 : Bx.c3;
+            iL0 = 1.0 / L0;
+            iL1 = 1.0 / L1;
+            iL2 = 1.0 / L2;
+        }
         const int si = I * MDG_CL + a;
         const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
         float qi, ei, gi;
@@ -348,12 +368,12 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
             double dx = xj - xi, dy = yj - yi, dz = zj - zi;
             double r2;
             if (MIXED) {
-                dx = fma(-A.F.L0, rint(dx * A.F.iL0), dx);
-                dy = fma(-A.F.L1, rint(dy * A.F.iL1), dy);
-                dz = fma(-A.F.L2, rint(dz * A.F.iL2), dz);
+                dx = fma(-L0, rint(dx * iL0), dx);
+                dy = fma(-L1, rint(dy * iL1), dy);
+                dz = fma(-L2, rint(dz * iL2), dz);
                 r2 = dx * dx + dy * dy + dz * dz;
             } else {
-                r2 = mdg_dist(A.B, dx, dy, dz);
+                r2 = mdg_dist(Bx, dx, dy, dz);
             }
             const bool in = ((m >> bit) & 1ull) && r2 <= A.P.cutOff2;
             if (MIXED) {
@@ -416,11 +436,11 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
                     fjx = fs * dx * rt;
                     fjy = fs * dy * rt;
                     fjz = fs * dz * rt;
-                    e_acc[0] += el;
-                    e_acc[1] += evw;
-                    e_acc[2] += dEl * r;
-                    e_acc[3] += dVw * r;
-                    e_acc[4] += 1.;
+                    ed += el;
+                    ev += evw;
+                    vd += dEl * r;
+                    vv += dVw * r;
+                    nin++;
                 }
                 fxid += fjx;
                 fyid += fjy;
@@ -447,11 +467,6 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
                 mdg_atomic_add_fix(&A.fiy[si], gy, MDG_FSCALE);
                 mdg_atomic_add_fix(&A.fiz[si], gz, MDG_FSCALE);
             }
-            e_acc[0] += ed;
-            e_acc[1] += ev;
-            e_acc[2] += vd;
-            e_acc[3] += vv;
-            e_acc[4] += nin;
         } else {
             for (int o = 8; o < 64; o <<= 1) {
                 fxid += __shfl_xor(fxid, o, 64);
@@ -464,8 +479,21 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
                 atomicAdd(&A.fdz[si], fzid);
             }
         }
+        if (BATCH) {
+            // energies per chunk, summed per replica later (a replica's
+            // chunks are contiguous)
+            const double c[5] = {mdg_wave_sum(ed), mdg_wave_sum(ev), mdg_wave_sum(vd),
+                                 mdg_wave_sum(vv), mdg_wave_sum((double)nin)};
+            if (lane < 5) A.chE[(long)W * 5 + lane] = c[lane];
+        } else {
+            e_acc[0] += ed;
+            e_acc[1] += ev;
+            e_acc[2] += vd;
+            e_acc[3] += vv;
+            e_acc[4] += nin;
+        }
     }
-    mdg_block_partials<5>(e_acc, A.part);
+    if (!BATCH) mdg_block_partials<5>(e_acc, A.part);
 }
 
 // Slot-ordered positions for the pair kernel (padding slots get copies of
@@ -555,6 +583,8 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     A.cpMask = C.cpMask;
     A.chOff = C.chOff;
     A.chunk = C.chunk;
+    A.maxChunks = C.maxChunks;
+    A.cap = (int)C.cap;
     A.fix = ctx->fix;
     A.fiy = ctx->fiy;
     A.fiz = ctx->fiz;
@@ -562,6 +592,9 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     A.fdy = ctx->fdy;
     A.fdz = ctx->fdz;
     A.part = ctx->epart + PART_PAIR;
+    A.boxes = ctx->bt ? ctx->bt->boxes : nullptr;
+    A.crep = ctx->bt ? ctx->bt->crep : nullptr;
+    A.chE = ctx->bt ? ctx->bt->chE : nullptr;
     return A;
 }
 
@@ -582,9 +615,9 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     if (ctx->precision == MDG_MIXED)
-        hipLaunchKernelGGL(k_pair_cluster<true>, dim3(nb), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL((k_pair_cluster<true, false>), dim3(nb), dim3(256), 0, ctx->stream, A);
     else
-        hipLaunchKernelGGL(k_pair_cluster<false>, dim3(nb), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL((k_pair_cluster<false, false>), dim3(nb), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
     return MDG_OK;
@@ -604,10 +637,10 @@ __global__ void __launch_bounds__(256) k_adj_cluster(ClusterArgs A)
     const int lane = threadIdx.x & 63;
     const int a = lane & 7, b = lane >> 3;
     const int bit = (a << 3) | b;
-    const int nch = A.chOff[A.nclu];
+    const int nch = min(A.chOff[A.nclu], A.maxChunks);
     for (int W = blockIdx.x * 4 + (threadIdx.x >> 6); W < nch; W += gridDim.x * 4) {
         const int2 ch = A.chunk[W];
-        const int I = ch.x, e0 = ch.y, e1 = min(e0 + MDG_CHUNK, A.cpOff[I + 1]);
+        const int I = ch.x, e0 = ch.y, e1 = min(min(e0 + MDG_CHUNK, A.cpOff[I + 1]), A.cap);
         const int si = I * MDG_CL + a;
         const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
         const double qi = A.sq[si], ei = A.seps[si], gi = A.ssig[si];
@@ -673,5 +706,34 @@ int mdg_adj_pair(mdg_ctx *ctx)
     ClusterArgs A = cluster_args(ctx);
     hipLaunchKernelGGL(k_adj_cluster, dim3(mdg_pair_blocks(ctx)), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
+    return MDG_OK;
+}
+
+// Pair forces of every replica of a batch: slot positions (and zeroed
+// accumulators), then the cluster kernel with per-replica boxes and
+// per-chunk energies.
+int mdg_b_pair(mdg_ctx *ctx)
+{
+    ClusterList &C = ctx->cl;
+    const int nslot = C.nclu * MDG_CL;
+    const bool mixed = ctx->precision == MDG_MIXED;
+    if (mixed)
+        hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
+    else
+        hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
+    LAUNCHCHK();
+    ClusterArgs A = cluster_args(ctx);
+    const int nb = mdg_pair_blocks(ctx);
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
+    if (mixed)
+        hipLaunchKernelGGL((k_pair_cluster<true, true>), dim3(nb), dim3(256), 0, ctx->stream, A);
+    else
+        hipLaunchKernelGGL((k_pair_cluster<false, true>), dim3(nb), dim3(256), 0, ctx->stream, A);
+    LAUNCHCHK();
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
     return MDG_OK;
 }

@@ -4,6 +4,7 @@
 // captured once per variant (plain / with list rebuild) in a hipGraph and
 // replayed.
 #include "mdg_internal.h"
+#include "mdg_shake.h"
 
 #include <cstring>
 
@@ -109,10 +110,26 @@ struct RecArgs {
 
 // Adds rows [0, nrow) of a row-major partial table with NC columns into
 // acc[0..NC) (this thread's rows only).
+// Four rows per thread are loaded before they are added, so a thread has
+// several loads in flight (the partials were just written, on every XCD).
 template <int NC>
 __device__ inline void rows_add(const double *__restrict__ p, int nrow, double *acc)
 {
-    for (int r = threadIdx.x; r < nrow; r += blockDim.x)
+    constexpr int U = 4;
+    const int step = blockDim.x;
+    int r = threadIdx.x;
+    for (; r + (U - 1) * step < nrow; r += U * step) {
+        double v[U][NC];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) v[u][c] = p[(long)(r + u * step) * NC + c];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[c] += v[u][c];
+    }
+    for (; r < nrow; r += step)
 #pragma unroll
         for (int c = 0; c < NC; c++) acc[c] += p[(long)r * NC + c];
 }
@@ -132,7 +149,7 @@ __global__ void __launch_bounds__(1024) k_record(RecArgs R, int *ctrl, MDG_STEP 
     rows_add<5>(R.pPair, R.nbPair, v);
     rows_add<2>(R.pExcl, R.nbExcl, v + 5);
     rows_add<7>(R.pInfl, R.nbInfl, v + 7);
-    for (int r = threadIdx.x; r < R.nbKin; r += blockDim.x) v[14] += R.pKin[(long)r * 7];
+    for (int r = threadIdx.x; r < R.nbKin; r += blockDim.x) v[14] += R.pKin[(long)r * 7];   // ~160 rows
     rows_add<1>(R.grpVir, R.nGrpVir, v + 15);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -208,6 +225,15 @@ void mdg_invalidate_graphs(mdg_ctx *ctx)
 
 static int ensure_records(mdg_ctx *ctx, int n)
 {
+    if (ctx->bt) {
+        MdgBatch &T = *ctx->bt;
+        if (T.recCap >= n) return MDG_OK;
+        if (T.rec) HIPCHK(hipFree(T.rec));
+        T.recCap = std::max(n, 64);
+        HIPCHK(hipMalloc(&T.rec, (size_t)T.recCap * T.R * sizeof(MDG_STEP)));
+        mdg_invalidate_graphs(ctx);
+        return MDG_OK;
+    }
     if (ctx->recCap >= n) return MDG_OK;
     if (ctx->recDev) HIPCHK(hipFree(ctx->recDev));
     ctx->recCap = std::max(n, 64);
@@ -218,8 +244,11 @@ static int ensure_records(mdg_ctx *ctx, int n)
 
 // Forces of the current positions on the device (energy() of
 // energy.c:103-205), optionally rebuilding the cluster list first.
+static int b_energy(mdg_ctx *ctx, bool rebuild);
+
 int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
 {
+    if (ctx->bt) return b_energy(ctx, rebuild);
     const int n = ctx->n;
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
@@ -295,6 +324,8 @@ static RecArgs rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
     return R;
 }
 
+static int b_record(mdg_ctx *ctx, bool rebuild);
+
 // One MD step (mdbas.c:244-301).
 int mdg_device_step(mdg_ctx *ctx, bool rebuild)
 {
@@ -304,9 +335,12 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     RETCHK(mdg_device_energy(ctx, rebuild));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[10], ctx->stream));
     RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
-                       rec_args(ctx, nblk(ctx->nGrp, ctx->shakeW3 ? 64 : 128), ctx->nGrp),
-                       ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
+    if (ctx->bt)
+        RETCHK(b_record(ctx, rebuild));
+    else
+        hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
+                           rec_args(ctx, nblk(ctx->nGrp, ctx->shakeW3 ? 64 : 128), ctx->nGrp),
+                           ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
     return MDG_OK;
@@ -341,6 +375,13 @@ __global__ void k_set_int(int *p, int v)
     if (threadIdx.x == 0) *p = v;
 }
 
+// every word to all ones (the SHAKE masks' reset value)
+__global__ void k_not64(long m, unsigned long long *p)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) p[i] = ~0ull;
+}
+
 __global__ void k_reset_mask(unsigned long long *mask)
 {
     if (threadIdx.x < 4) mask[threadIdx.x] = ~0ull;
@@ -349,12 +390,22 @@ __global__ void k_reset_mask(unsigned long long *mask)
 // Record index 0 and a fresh SHAKE mask, before a sequence of steps.
 int mdg_reset_step_state(mdg_ctx *ctx)
 {
+    if (ctx->bt) {
+        RETCHK(mdg_fill_i32(ctx, ctx->bt->stepCtr, ctx->bt->R, 0));
+        const long nm = (long)ctx->bt->R * SHK_MASKW;
+        hipLaunchKernelGGL(k_not64, dim3(nblk(nm, 256)), dim3(256), 0, ctx->stream, nm,
+                           ctx->bt->masks);
+        LAUNCHCHK();
+        return MDG_OK;
+    }
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, ctx->ctrl + CTRL_STEP, 0);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, ctx->stream, ctx->shkMask);
     LAUNCHCHK();
     return MDG_OK;
 }
+
+static int b_group_kinetic(mdg_ctx *ctx);
 
 extern "C" int mdg_upload_state(mdg_ctx *ctx, const double x[], const double y[],
                                 const double z[], const double vx[], const double vy[],
@@ -389,6 +440,19 @@ extern "C" int mdg_download_state(mdg_ctx *ctx, double x[], double y[], double z
 extern "C" int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec)
 {
     if (!ctx || !ctx->theta || !ctx->grpAtomOff || !ctx->exSymOff) return MDG_EARG;
+    if (ctx->bt) {
+        RETCHK(ensure_records(ctx, 64));
+        RETCHK(mdg_reset_step_state(ctx));
+        RETCHK(b_energy(ctx, true));
+        RETCHK(b_group_kinetic(ctx));
+        RETCHK(b_record(ctx, true));
+        RETCHK(mdg_check_flags(ctx));
+        if (rec)
+            HIPCHK(hipMemcpy(rec, ctx->bt->rec, ctx->bt->R * sizeof(MDG_STEP),
+                             hipMemcpyDeviceToHost));
+        ctx->stepsSinceList = 0;
+        return MDG_OK;
+    }
     if (ctx->B.type == 3) {
         // the cluster list bins atoms in Cartesian columns
         ctx->err = "the device-resident path needs a cubic or orthorhombic box";
@@ -509,8 +573,10 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
     }
     RETCHK(mdg_check_flags(ctx));
     if (records && nsteps > 0) {
-        HIPCHK(hipMemcpy(records, ctx->recDev, nsteps * sizeof(MDG_STEP), hipMemcpyDeviceToHost));
-        for (int s = 0; s < nsteps; s++) records[s].step = stepBase + s;
+        const int R = ctx->bt ? ctx->bt->R : 1;
+        HIPCHK(hipMemcpy(records, ctx->bt ? ctx->bt->rec : ctx->recDev,
+                         (size_t)nsteps * R * sizeof(MDG_STEP), hipMemcpyDeviceToHost));
+        for (long s = 0; s < (long)nsteps * R; s++) records[s].step = stepBase + (int)(s / R);
     }
     return MDG_OK;
 }
@@ -576,5 +642,263 @@ extern "C" int mdg_list_pairs(mdg_ctx *ctx, int *pi, int *pj, long cap, long *np
                 k++;
             }
     *npairs = k;
+    return MDG_OK;
+}
+
+// ---------------------------------------------------------------- replica batch
+
+struct BFinArgs {
+    MdgParams P;
+    int n, hasFrozen;
+    const MdgBox *boxes;
+    const int *arep, *aoff, *atomSlot;
+    const long long *fix, *fiy, *fiz;
+    const double *fdx, *fdy, *fdz;
+    const float *frx, *fry, *frz;
+    const double *frx64, *fry64, *frz64;
+    const double *repRec;                // per replica sums of the reciprocal forces
+    const int *exOff, *exLst;
+    const double *x, *y, *z, *q;
+    const int *frozen;
+    double *fx, *fy, *fz, *vx, *vy, *vz;
+    double *atomE;                       // per atom: correction energy, virial
+    unsigned long long *masks;           // SHAKE masks, reset for VV stage 2
+    int nmask;
+};
+
+// k_finalize for a batch: each atom's reciprocal-force mean is its own
+// replica's, the correction uses the replica's box, and the correction
+// energy is kept per atom for the per-replica sums.
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_b_finalize(BFinArgs A)
+{
+    if (blockIdx.x == 0)
+        for (int t = threadIdx.x; t < A.nmask; t += blockDim.x) A.masks[t] = ~0ull;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const int r = A.arep[i];
+    const MdgBox &B = A.boxes[r];
+    const double nr = (double)(A.aoff[r + 1] - A.aoff[r]);
+    const double m0 = A.repRec[3 * r] / nr, m1 = A.repRec[3 * r + 1] / nr,
+                 m2 = A.repRec[3 * r + 2] / nr;
+    const int s = A.atomSlot[i];
+    double gx, gy, gz;
+    if (MIXED) {
+        gx = (double)A.fix[s] * (1.0 / MDG_FSCALE) + ((double)A.frx[i] - m0);
+        gy = (double)A.fiy[s] * (1.0 / MDG_FSCALE) + ((double)A.fry[i] - m1);
+        gz = (double)A.fiz[s] * (1.0 / MDG_FSCALE) + ((double)A.frz[i] - m2);
+    } else {
+        gx = (A.frx64[i] - m0) + A.fdx[s];
+        gy = (A.fry64[i] - m1) + A.fdy[s];
+        gz = (A.frz64[i] - m2) + A.fdz[s];
+    }
+    double ec = 0., vc = 0.;
+    for (int k = A.exOff[i]; k < A.exOff[i + 1]; k++) {
+        const int j = A.exLst[k];
+        const int lo = min(i, j), hi = max(i, j);
+        double dx = A.x[hi] - A.x[lo], dy = A.y[hi] - A.y[lo], dz = A.z[hi] - A.z[lo];
+        const double r2 = mdg_dist(B, dx, dy, dz);
+        const double rr = sqrt(r2), rt = 1. / rr;
+        double dC;
+        const double e = ewald_corr64(A.P.alpha, A.P.chargeConst * A.q[lo] * A.q[hi], rr, rt, dC);
+        const double sgn = (i == lo) ? 1.0 : -1.0;
+        gx += sgn * (dC * dx * rt);
+        gy += sgn * (dC * dy * rt);
+        gz += sgn * (dC * dz * rt);
+        if (i == lo) {
+            ec += e;
+            vc += dC * rr;
+        }
+    }
+    if (A.hasFrozen && A.frozen[i]) {
+        gx = gy = gz = 0.;
+        A.vx[i] = A.vy[i] = A.vz[i] = 0.;
+    }
+    A.fx[i] = gx;
+    A.fy[i] = gy;
+    A.fz[i] = gz;
+    A.atomE[2 * i] = ec;
+    A.atomE[2 * i + 1] = vc;
+}
+
+// sum m v^2 per group (the step-0 record, before any VV stage 2)
+__global__ void k_b_group_kin(int nGrp, const int *__restrict__ grpAtomOff,
+                              const int *__restrict__ grpAtoms, const double *__restrict__ mass,
+                              const double *__restrict__ vx, const double *__restrict__ vy,
+                              const double *__restrict__ vz, double *__restrict__ grpKin,
+                              double *__restrict__ grpVir)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nGrp) return;
+    double s = 0.;
+    for (int k = grpAtomOff[g]; k < grpAtomOff[g + 1]; k++) {
+        const int i = grpAtoms[k];
+        s += mass[i] * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    }
+    grpKin[g] = s;
+    grpVir[g] = 0.;
+}
+
+struct BRecArgs {
+    double chargeConst;
+    int R, recCap;
+    const MdgBox *boxes;
+    const double *eSelf, *eNet;
+    const int *aoff, *goff, *cluStart, *chOff;
+    int maxChunks;
+    const double *chE, *atomE, *grpKin, *grpVir, *repInfl;
+    int *stepCtr;
+    MDG_STEP *rec;                       // [step][replica]
+    unsigned long long *masks;
+};
+
+// One MDG_STEP per replica (one block each): the replica's chunk, atom and
+// group sums in a fixed order, assembled as k_record does; resets the
+// replica's SHAKE mask for the next VV stage 1.
+__global__ void __launch_bounds__(256) k_b_record(BRecArgs R, int rebuilt)
+{
+    __shared__ double red[4][10];
+    const int r = blockIdx.x;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int c0 = min(R.chOff[R.cluStart[r]], R.maxChunks);
+    const int c1 = min(R.chOff[R.cluStart[r + 1]], R.maxChunks);
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+        for (int k = 0; k < 5; k++) v[k] += R.chE[(long)c * 5 + k];
+    for (int i = R.aoff[r] + threadIdx.x; i < R.aoff[r + 1]; i += blockDim.x) {
+        v[5] += R.atomE[2 * (long)i];
+        v[6] += R.atomE[2 * (long)i + 1];
+    }
+    for (int g = R.goff[r] + threadIdx.x; g < R.goff[r + 1]; g += blockDim.x) {
+        v[7] += R.grpKin[g];
+        v[8] += R.grpVir[g];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < 9; k++) {
+        const double s = mdg_wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
+    if (threadIdx.x < SHK_MASKW) R.masks[(long)r * SHK_MASKW + threadIdx.x] = ~0ull;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 9; k++) v[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    const double *inf = R.repInfl + 7 * (long)r;
+    const double fact = (TWOPI_R / R.boxes[r].vol) * R.chargeConst;
+    const double er = inf[0];
+    const double st0 = fact * (inf[1] + er) + R.eNet[r];
+    const double st3 = fact * (inf[4] + er) + R.eNet[r];
+    const double st5 = fact * (inf[6] + er) + R.eNet[r];
+    const double eRec = fact * er + R.eSelf[r] + R.eNet[r];
+    const int idx = R.stepCtr[r];
+    MDG_STEP s;
+    s.step = idx;
+    s.listRebuilt = rebuilt;
+    s.eDir = v[0];
+    s.eRec = eRec;
+    s.eCorr = v[5];
+    s.elec = s.eDir + eRec + s.eCorr;
+    s.vdw = v[1];
+    s.pot = s.elec + s.vdw;
+    s.kin = 0.5 * v[7];
+    s.tot = s.pot + s.kin;
+    s.virelec = v[2] + (-(st0 + st3 + st5)) + v[6];
+    s.virvdw = v[3];
+    s.virshake = v[8];
+    s.nInCut = v[4];
+    if (idx < R.recCap) R.rec[(long)idx * R.R + r] = s;
+    R.stepCtr[r] = idx + 1;
+}
+
+static int b_energy(mdg_ctx *ctx, bool rebuild)
+{
+    MdgBatch &T = *ctx->bt;
+    const int n = ctx->n;
+    if (rebuild) {
+        if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
+        RETCHK(mdg_b_build_list(ctx));
+        RETCHK(ensure_slots(ctx));
+        RETCHK(mdg_slot_params(ctx));
+        if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
+    }
+    RETCHK(mdg_b_spme(ctx));
+    RETCHK(mdg_b_pair(ctx));
+    BFinArgs F;
+    F.P = ctx->P;
+    F.n = n;
+    F.hasFrozen = ctx->hasFrozen ? 1 : 0;
+    F.boxes = T.boxes;
+    F.arep = T.arep;
+    F.aoff = T.aoff;
+    F.atomSlot = ctx->cl.atomSlot;
+    F.fix = ctx->fix;
+    F.fiy = ctx->fiy;
+    F.fiz = ctx->fiz;
+    F.fdx = ctx->fdx;
+    F.fdy = ctx->fdy;
+    F.fdz = ctx->fdz;
+    F.frx = ctx->frx;
+    F.fry = ctx->fry;
+    F.frz = ctx->frz;
+    F.frx64 = ctx->frx64;
+    F.fry64 = ctx->fry64;
+    F.frz64 = ctx->frz64;
+    F.repRec = T.repRec;
+    F.exOff = ctx->exSymOff;
+    F.exLst = ctx->exSymLst;
+    F.x = ctx->x;
+    F.y = ctx->y;
+    F.z = ctx->z;
+    F.q = ctx->q;
+    F.frozen = ctx->frozen;
+    F.fx = ctx->fx;
+    F.fy = ctx->fy;
+    F.fz = ctx->fz;
+    F.vx = ctx->vx;
+    F.vy = ctx->vy;
+    F.vz = ctx->vz;
+    F.atomE = T.atomE;
+    F.masks = T.masks;
+    F.nmask = T.R * SHK_MASKW;
+    if (ctx->precision == MDG_MIXED)
+        hipLaunchKernelGGL(k_b_finalize<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
+    else
+        hipLaunchKernelGGL(k_b_finalize<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+static int b_group_kinetic(mdg_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_b_group_kin, dim3(nblk(ctx->nGrp, 256)), dim3(256), 0, ctx->stream,
+                       ctx->nGrp, ctx->grpAtomOff, ctx->grpAtoms, ctx->mass, ctx->vx, ctx->vy,
+                       ctx->vz, ctx->bt->grpKin, ctx->grpVir);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+static int b_record(mdg_ctx *ctx, bool rebuild)
+{
+    MdgBatch &T = *ctx->bt;
+    BRecArgs R;
+    R.chargeConst = ctx->P.chargeConst;
+    R.R = T.R;
+    R.recCap = T.recCap;
+    R.boxes = T.boxes;
+    R.eSelf = T.eSelf;
+    R.eNet = T.eNet;
+    R.aoff = T.aoff;
+    R.goff = T.goff;
+    R.cluStart = T.cluStart;
+    R.chOff = ctx->cl.chOff;
+    R.maxChunks = ctx->cl.maxChunks;
+    R.chE = T.chE;
+    R.atomE = T.atomE;
+    R.grpKin = T.grpKin;
+    R.grpVir = ctx->grpVir;
+    R.repInfl = T.repInfl;
+    R.stepCtr = T.stepCtr;
+    R.rec = T.rec;
+    R.masks = T.masks;
+    hipLaunchKernelGGL(k_b_record, dim3(T.R), dim3(256), 0, ctx->stream, R, rebuild ? 1 : 0);
+    LAUNCHCHK();
     return MDG_OK;
 }

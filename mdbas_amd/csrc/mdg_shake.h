@@ -28,6 +28,11 @@ struct ShakeArgs {
     double *virScr;       // [SHK_SLOTS][nGrp] SHAKE virial at those states
     int *first, *kend;    // per group: first converged sweep, last checked sweep
     unsigned long long *mask;   // device-wide AND of converged-sweep bits
+    // replica batch: box and mask per replica, replica per group, and the
+    // per-group sum m v^2 (stage 2)
+    const MdgBox *boxes;
+    const int *grep;
+    double *grpKin;
 };
 
 // Publishes the group's converged-at-sweep-k bits into the device-wide
@@ -48,11 +53,41 @@ __device__ inline void shake_publish_mask(const ShakeArgs &A, const unsigned lon
     }
 }
 
-// First sweep index at which every group was converged (-1 if none).
-__device__ inline int shake_global_stop(const ShakeArgs &A)
+// Batch form of shake_publish_mask: each replica has its own mask
+// (masks + SHK_MASKW * rep); a wavefront whose groups all belong to one
+// replica publishes once, a wavefront on a replica boundary lane by lane.
+// Idle lanes pass rep = -1.
+__device__ inline void shake_publish_mask_b(const ShakeArgs &A,
+                                            const unsigned long long (&bits)[SHK_MASKW], int kend,
+                                            bool settled, int rep)
+{
+    const int lane = threadIdx.x & 63;
+    int lo = rep < 0 ? 0x7fffffff : rep, hi = rep;
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    const bool uniform = lo == hi;
+    for (int w = 0; w < SHK_MASKW; w++) {
+        unsigned long long m = bits[w];
+        if (settled)
+            for (int b = 0; b < 64; b++)
+                if (w * 64 + b > kend && w * 64 + b < A.P.maxCycle) m |= 1ull << b;
+        if (uniform) {
+            for (int o = 32; o > 0; o >>= 1) m &= __shfl_xor(m, o, 64);
+            if (lane == 0 && hi >= 0 && m != ~0ull) atomicAnd(&A.mask[(long)hi * SHK_MASKW + w], m);
+        } else if (rep >= 0 && m != ~0ull) {
+            atomicAnd(&A.mask[(long)rep * SHK_MASKW + w], m);
+        }
+    }
+}
+
+// First sweep index at which every group was converged (-1 if none); mask
+// is the run's (or one replica's) SHK_MASKW words.
+__device__ inline int shake_global_stop(const ShakeArgs &A, const unsigned long long *mask)
 {
     for (int w = 0; w < SHK_MASKW; w++) {
-        unsigned long long m = A.mask[w];
+        unsigned long long m = mask[w];
         if (m) {
             int k = w * 64 + __ffsll((long long)m) - 1;
             return k < A.P.maxCycle ? k : -1;
@@ -60,6 +95,8 @@ __device__ inline int shake_global_stop(const ShakeArgs &A)
     }
     return -1;
 }
+
+__device__ inline int shake_global_stop(const ShakeArgs &A) { return shake_global_stop(A, A.mask); }
 
 
 // Which saved state leads to sweep C: the window slot, the end state, or
@@ -103,7 +140,7 @@ __device__ inline void shake_image(const MdgBox &B, double &xs, double &ys, doub
     }
 }
 
-__global__ void k_w3_vv1_a(ShakeArgs A);
-__global__ void k_w3_vv1_b(ShakeArgs A);
-__global__ void k_w3_vv2_a(ShakeArgs A);
-__global__ void k_w3_vv2_b(ShakeArgs A);
+template <bool BATCH> __global__ void k_w3_vv1_a(ShakeArgs A);
+template <bool BATCH> __global__ void k_w3_vv1_b(ShakeArgs A);
+template <bool BATCH> __global__ void k_w3_vv2_a(ShakeArgs A);
+template <bool BATCH> __global__ void k_w3_vv2_b(ShakeArgs A);

@@ -106,14 +106,14 @@ __device__ inline double w3_dist(const MdgBox &B, double &dx, double &dy, double
     return mdg_dist(B, dx, dy, dz);
 }
 
-__device__ inline void w3_dd(const ShakeArgs &A, W3 &G)
+__device__ inline void w3_dd(const ShakeArgs &A, const MdgBox &B, W3 &G)
 {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         double dx = G.x[IB[c]] - G.x[IA[c]];
         double dy = G.y[IB[c]] - G.y[IA[c]];
         double dz = G.z[IB[c]] - G.z[IA[c]];
-        w3_dist(A.B, dx, dy, dz);
+        w3_dist(B, dx, dy, dz);
         G.ddx[c] = dx;
         G.ddy[c] = dy;
         G.ddz[c] = dz;
@@ -134,8 +134,8 @@ __device__ inline void w3_dd_load(const ShakeArgs &A, W3 &G)
 }
 
 // shake.c:243-258
-__device__ inline double w3_check(const ShakeArgs &A, const W3 &G, double *dtx, double *dty,
-                                  double *dtz, double *rt2)
+__device__ inline double w3_check(const ShakeArgs &A, const MdgBox &B, const W3 &G, double *dtx,
+                                  double *dty, double *dtz, double *rt2)
 {
 #pragma clang fp contract(off)
     double maxd = 0.;
@@ -144,7 +144,7 @@ __device__ inline double w3_check(const ShakeArgs &A, const W3 &G, double *dtx, 
         dtx[c] = G.x[IB[c]] - G.x[IA[c]];
         dty[c] = G.y[IB[c]] - G.y[IA[c]];
         dtz[c] = G.z[IB[c]] - G.z[IA[c]];
-        rt2[c] = w3_dist(A.B, dtx[c], dty[c], dtz[c]);
+        rt2[c] = w3_dist(B, dtx[c], dty[c], dtz[c]);
         const double d = fabs(rt2[c] - G.rc2[c]) / G.src2[c];
         maxd = maxd >= d ? maxd : d;
     }
@@ -266,14 +266,14 @@ __device__ inline void w3_kick(const ShakeArgs &A, W3 &G)
 
 }  // namespace
 
-__device__ inline void w3_vv1_a_body(const ShakeArgs &A, int g, unsigned long long (&bits)[SHK_MASKW],
-                                  int &kend, bool &settled)
+__device__ inline void w3_vv1_a_body(const ShakeArgs &A, const MdgBox &B, int g,
+                                     unsigned long long (&bits)[SHK_MASKW], int &kend, bool &settled)
 {
 #pragma clang fp contract(off)
     W3 G;
     w3_load(A, g, G);
     const double dt = A.P.timeStep;
-    w3_dd(A, G);
+    w3_dd(A, B, G);
     w3_kick(A, G);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -289,7 +289,7 @@ __device__ inline void w3_vv1_a_body(const ShakeArgs &A, int g, unsigned long lo
     bool prev = false;
     settled = false;
     for (;;) {
-        const bool cv = w3_check(A, G, dtx, dty, dtz, rt2) < A.P.tolShake;
+        const bool cv = w3_check(A, B, G, dtx, dty, dtz, rt2) < A.P.tolShake;
         if (cv) bits[k >> 6] |= 1ull << (k & 63);
         if (cv && first < 0) first = k;
         if (first >= 0 && k - first < SHK_W) {
@@ -312,6 +312,28 @@ __device__ inline void w3_vv1_a_body(const ShakeArgs &A, int g, unsigned long lo
     kend = k;
 }
 
+// The box of group g: the run's, or its replica's in a batch.
+template <bool BATCH>
+__device__ inline const MdgBox &w3_box(const ShakeArgs &A, int g)
+{
+    return BATCH ? A.boxes[A.grep[g]] : A.B;
+}
+
+template <bool BATCH>
+__device__ inline void w3_publish(const ShakeArgs &A, int g, const unsigned long long (&bits)[SHK_MASKW],
+                                  int kend, bool settled)
+{
+    if (BATCH) shake_publish_mask_b(A, bits, kend, settled, g < A.nGrp ? A.grep[g] : -1);
+    else shake_publish_mask(A, bits, kend, settled);
+}
+
+template <bool BATCH>
+__device__ inline const unsigned long long *w3_mask(const ShakeArgs &A, int g)
+{
+    return BATCH ? A.mask + (long)A.grep[g] * SHK_MASKW : A.mask;
+}
+
+template <bool BATCH>
 __global__ void __launch_bounds__(64) k_w3_vv1_a(ShakeArgs A)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,15 +341,17 @@ __global__ void __launch_bounds__(64) k_w3_vv1_a(ShakeArgs A)
     unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
     int kend = -1;
     bool settled = true;
-    if (g < A.nGrp) w3_vv1_a_body(A, g, bits, kend, settled);
-    shake_publish_mask(A, bits, kend, settled);
+    if (g < A.nGrp) w3_vv1_a_body(A, w3_box<BATCH>(A, g), g, bits, kend, settled);
+    w3_publish<BATCH>(A, g, bits, kend, settled);
 }
 
+template <bool BATCH>
 __global__ void __launch_bounds__(64) k_w3_vv1_b(ShakeArgs A)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= A.nGrp) return;
-    const int C = shake_global_stop(A);
+    const MdgBox &B = w3_box<BATCH>(A, g);
+    const int C = shake_global_stop(A, w3_mask<BATCH>(A, g));
     if (C < 0) {
         atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 1);
         return;
@@ -342,19 +366,19 @@ __global__ void __launch_bounds__(64) k_w3_vv1_b(ShakeArgs A)
     double vir = slot == 0 ? 0. : A.virScr[(long)slot * A.nGrp + g];
     double dtx[3], dty[3], dtz[3], rt2[3];
     for (int k = at; k < C; k++) {
-        w3_check(A, G, dtx, dty, dtz, rt2);
+        w3_check(A, B, G, dtx, dty, dtz, rt2);
         w3_apply_r(A, G, dtx, dty, dtz, rt2, vir);
     }
     A.vir[g] = vir;
     w3_store(A, G, true);
 }
 
-__device__ inline void w3_vv2_a_body(const ShakeArgs &A, int g, unsigned long long (&bits)[SHK_MASKW],
-                                  int &kend, bool &settled)
+__device__ inline void w3_vv2_a_body(const ShakeArgs &A, const MdgBox &B, int g,
+                                     unsigned long long (&bits)[SHK_MASKW], int &kend, bool &settled)
 {
     W3 G;
     w3_load(A, g, G);
-    w3_dd(A, G);
+    w3_dd(A, B, G);
     w3_kick(A, G);
     w3_scr_store(A, G, 0, false);
     double den[3];
@@ -388,6 +412,7 @@ __device__ inline void w3_vv2_a_body(const ShakeArgs &A, int g, unsigned long lo
     kend = k;
 }
 
+template <bool BATCH>
 __global__ void __launch_bounds__(64) k_w3_vv2_a(ShakeArgs A)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,19 +420,21 @@ __global__ void __launch_bounds__(64) k_w3_vv2_a(ShakeArgs A)
     unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
     int kend = -1;
     bool settled = true;
-    if (g < A.nGrp) w3_vv2_a_body(A, g, bits, kend, settled);
-    shake_publish_mask(A, bits, kend, settled);
+    if (g < A.nGrp) w3_vv2_a_body(A, w3_box<BATCH>(A, g), g, bits, kend, settled);
+    w3_publish<BATCH>(A, g, bits, kend, settled);
 }
 
+template <bool BATCH>
 __global__ void __launch_bounds__(64) k_w3_vv2_b(ShakeArgs A)
 {
 #pragma clang fp contract(off)
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (g < A.nGrp) {
+        const MdgBox &B = w3_box<BATCH>(A, g);
         W3 G;
         w3_load(A, g, G);
-        const int C = shake_global_stop(A);
+        const int C = shake_global_stop(A, w3_mask<BATCH>(A, g));
         if (C < 0) {
             atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 2);
         } else {
@@ -434,9 +461,19 @@ __global__ void __launch_bounds__(64) k_w3_vv2_b(ShakeArgs A)
             acc[4] += m * G.vy[k] * G.vy[k];
             acc[5] += m * G.vy[k] * G.vz[k];
             acc[6] += m * G.vz[k] * G.vz[k];
-            shake_image(A.B, G.x[k], G.y[k], G.z[k]);
+            shake_image(B, G.x[k], G.y[k], G.z[k]);
         }
         w3_store(A, G, true);
+        if (BATCH) A.grpKin[g] = acc[0];
     }
     mdg_block_partials<7>(acc, A.part);
 }
+
+template __global__ void k_w3_vv1_a<false>(ShakeArgs);
+template __global__ void k_w3_vv1_a<true>(ShakeArgs);
+template __global__ void k_w3_vv1_b<false>(ShakeArgs);
+template __global__ void k_w3_vv1_b<true>(ShakeArgs);
+template __global__ void k_w3_vv2_a<false>(ShakeArgs);
+template __global__ void k_w3_vv2_a<true>(ShakeArgs);
+template __global__ void k_w3_vv2_b<false>(ShakeArgs);
+template __global__ void k_w3_vv2_b<true>(ShakeArgs);

@@ -15,8 +15,9 @@
 // is bitwise reproducible), runs the FFTs in FP32 and gathers in FP32.
 #include "mdg_internal.h"
 
-#include <complex>
+#include <algorithm>
 #include <cfloat>
+#include <complex>
 
 #define TWOPI 6.283185307179586     // global.h:53
 #define PI_ 3.141592653589793
@@ -277,14 +278,54 @@ static void bsp_moduli(int m, int p, std::vector<double> &mod2)
     }
 }
 
-static int make_plans(mdg_ctx *ctx, bool dbl, rocfft_plan *fwd, rocfft_plan *bwd)
+// spme.c:505-583 influence function on the half spectrum (i, j, k < m3/2+1),
+// |b1 b2 b3|^2 exp(-k^2/4a^2)/k^2 inside the reference's reciprocal cutoff.
+static void theta_host(const MdgBox &B, int m1, int m2, int m3, int p, double alpha,
+                       std::vector<double> &th)
 {
-    size_t len[3] = {(size_t)ctx->m3, (size_t)ctx->m2, (size_t)ctx->m1};
+    const int mh = m3 / 2 + 1;
+    std::vector<double> b1, b2, b3;
+    bsp_moduli(m1, p, b1);
+    bsp_moduli(m2, p, b2);
+    bsp_moduli(m3, p, b3);
+    double rAlpha2 = -0.25 / (alpha * alpha);
+    double rc = std::min((double)m1 * B.u, (double)m2 * B.v);
+    rc = std::min(rc, (double)m3 * B.w);
+    rc = rc * 1.05 * TWOPI;
+    const double rc2 = rc * rc;
+    th.assign((size_t)m1 * m2 * mh, 0.0);
+    for (int i = 0; i < m1; i++) {
+        int ii = i > m1 / 2 ? i - m1 : i;
+        double t1 = TWOPI * (double)ii;
+        for (int j = 0; j < m2; j++) {
+            int jj = j > m2 / 2 ? j - m2 : j;
+            double t2 = TWOPI * (double)jj;
+            for (int k = 0; k < mh; k++) {
+                int kk = k > m3 / 2 ? k - m3 : k;
+                double t3 = TWOPI * (double)kk;
+                double rx = t1 * B.u1 + t2 * B.v1 + t3 * B.w1;
+                double ry = t1 * B.u2 + t2 * B.v2 + t3 * B.w2;
+                double rz = t1 * B.u3 + t2 * B.v3 + t3 * B.w3;
+                double rm = rx * rx + ry * ry + rz * rz;
+                double v = 0.0;
+                if (rm > DBL_EPSILON && rm <= rc2) v = b1[i] * b2[j] * b3[k] * exp(rAlpha2 * rm) / rm;
+                th[((long)i * m2 + j) * mh + k] = v;
+            }
+        }
+    }
+}
+
+// Real-to-complex / complex-to-real 3-D plans of count meshes laid out one
+// after the other (index 3 fastest), not in place.
+static int make_plans_n(mdg_ctx *ctx, bool dbl, int m1, int m2, int m3, int count,
+                        rocfft_plan *fwd, rocfft_plan *bwd)
+{
+    size_t len[3] = {(size_t)m3, (size_t)m2, (size_t)m1};
     rocfft_precision prec = dbl ? rocfft_precision_double : rocfft_precision_single;
     if (rocfft_plan_create(fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                           prec, 3, len, 1, nullptr) != rocfft_status_success ||
+                           prec, 3, len, count, nullptr) != rocfft_status_success ||
         rocfft_plan_create(bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                           prec, 3, len, 1, nullptr) != rocfft_status_success) {
+                           prec, 3, len, count, nullptr) != rocfft_status_success) {
         ctx->err = "rocfft_plan_create failed";
         return MDG_EHIP;
     }
@@ -301,7 +342,12 @@ static int make_plans(mdg_ctx *ctx, bool dbl, rocfft_plan *fwd, rocfft_plan *bwd
     return MDG_OK;
 }
 
-int mdg_spme_setup(mdg_ctx *ctx)
+static int make_plans(mdg_ctx *ctx, bool dbl, rocfft_plan *fwd, rocfft_plan *bwd)
+{
+    return make_plans_n(ctx, dbl, ctx->m1, ctx->m2, ctx->m3, 1, fwd, bwd);
+}
+
+static int rocfft_init(mdg_ctx *ctx)
 {
     static bool s_setup = false;
     if (!s_setup) {
@@ -311,6 +357,12 @@ int mdg_spme_setup(mdg_ctx *ctx)
         }
         s_setup = true;
     }
+    return MDG_OK;
+}
+
+int mdg_spme_setup(mdg_ctx *ctx)
+{
+    RETCHK(rocfft_init(ctx));
     const int m1 = ctx->m1, m2 = ctx->m2, m3 = ctx->m3, p = ctx->P.nbsp;
     if (p < 2 || p > MAXP) {
         ctx->err = "spline order outside 2..12";
@@ -322,39 +374,9 @@ int mdg_spme_setup(mdg_ctx *ctx)
         ctx->err = "mesh or atom count beyond the partial-sum buffers";
         return MDG_EUNSUPPORTED;
     }
-    std::vector<double> b1, b2, b3;
-    bsp_moduli(m1, p, b1);
-    bsp_moduli(m2, p, b2);
-    bsp_moduli(m3, p, b3);
-    const MdgBox &B = ctx->B;
-    double rAlpha2 = -0.25 / (ctx->P.alpha * ctx->P.alpha);
-    double rc = std::min((double)m1 * B.u, (double)m2 * B.v);
-    rc = std::min(rc, (double)m3 * B.w);
-    rc = rc * 1.05 * TWOPI;
-    const double rc2 = rc * rc;
-    std::vector<double> th(H);
-    std::vector<float> thf(H);
-    for (int i = 0; i < m1; i++) {
-        int ii = i > m1 / 2 ? i - m1 : i;
-        double t1 = TWOPI * (double)ii;
-        for (int j = 0; j < m2; j++) {
-            int jj = j > m2 / 2 ? j - m2 : j;
-            double t2 = TWOPI * (double)jj;
-            for (int k = 0; k < ctx->mh; k++) {
-                int kk = k > m3 / 2 ? k - m3 : k;
-                double t3 = TWOPI * (double)kk;
-                double rx = t1 * B.u1 + t2 * B.v1 + t3 * B.w1;
-                double ry = t1 * B.u2 + t2 * B.v2 + t3 * B.w2;
-                double rz = t1 * B.u3 + t2 * B.v3 + t3 * B.w3;
-                double rm = rx * rx + ry * ry + rz * rz;
-                double v = 0.0;
-                if (rm > DBL_EPSILON && rm <= rc2) v = b1[i] * b2[j] * b3[k] * exp(rAlpha2 * rm) / rm;
-                long g = ((long)i * m2 + j) * ctx->mh + k;
-                th[g] = v;
-                thf[g] = (float)v;
-            }
-        }
-    }
+    std::vector<double> th;
+    theta_host(ctx->B, m1, m2, m3, p, ctx->P.alpha, th);
+    std::vector<float> thf(th.begin(), th.end());
     HIPCHK(hipMalloc(&ctx->theta, H * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->theta32, H * sizeof(float)));
     HIPCHK(hipMemcpy(ctx->theta, th.data(), H * sizeof(double), hipMemcpyHostToDevice));
@@ -540,5 +562,322 @@ int mdg_spme_pot(mdg_ctx *ctx, double *pot)
     hipLaunchKernelGGL(k_spme_potential, dim3(nblk(ctx->n, 256)), dim3(256), 0, ctx->stream,
                        ctx->P, ctx->B, ctx->n, ctx->x, ctx->y, ctx->z, ctx->grid64, scale, pot);
     LAUNCHCHK();
+    return MDG_OK;
+}
+
+// ---------------------------------------------------------------- replica batch
+
+// Charge spreading for every replica at once: atom l of replica r spreads
+// onto r's own cubic mesh (edge mesh[r], first element gridOff[r]) with
+// r's box.
+template <bool MIXED>
+__global__ void __launch_bounds__(256)
+k_b_spread(MdgParams P, const MdgBox *__restrict__ boxes, const int *__restrict__ arep,
+           const int *__restrict__ mesh, const long *__restrict__ gridOff, int n,
+           const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+           const double *__restrict__ q, double *grid64, long long *gridq)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int r = arep[l];
+    const MdgBox &B = boxes[r];
+    const int p = P.nbsp, m = mesh[r];
+    const long base = gridOff[r];
+    double s1, s2, s3;
+    scaled_coords(B, m, m, m, x[l], y[l], z[l], s1, s2, s3);
+    typedef typename std::conditional<MIXED, float, double>::type T;
+    T b1[MAXP], b2[MAXP], b3[MAXP], d[MAXP];
+    bspline_dev<T>(s1, p, b1, d);
+    bspline_dev<T>(s2, p, b2, d);
+    bspline_dev<T>(s3, p, b3, d);
+    const T ql = (T)q[l];
+    const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+    for (int i = 0; i < p; i++) {
+        const int o1 = m * wrap_index(i1 - i, m);
+        const T qa = ql * b1[i];
+        for (int j = 0; j < p; j++) {
+            const long o2 = base + (long)m * (wrap_index(i2 - j, m) + o1);
+            const T qb = qa * b2[j];
+            for (int k = 0; k < p; k++) {
+                const int kk = wrap_index(i3 - k, m);
+                if (MIXED) {
+                    long long v = __double2ll_rn((double)(qb * b3[k]) * MDG_QSCALE);
+                    atomicAdd((unsigned long long *)&gridq[kk + o2], (unsigned long long)v);
+                } else {
+                    atomicAdd(&grid64[kk + o2], (double)qb * (double)b3[k]);
+                }
+            }
+        }
+    }
+}
+
+// Influence function of replica blockIdx.x over its half spectrum, and its
+// energy / stress sums (one block per replica, fixed order).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_b_influence(const MdgBox *__restrict__ boxes, const int *__restrict__ mesh,
+              const long *__restrict__ hOff, double rAlpha2, const T *__restrict__ theta, T *cg,
+              double *__restrict__ repInfl)
+{
+    const int r = blockIdx.x;
+    const MdgBox &B = boxes[r];
+    const int m = mesh[r], mh = m / 2 + 1;
+    const long H = (long)m * m * mh, h0 = hOff[r];
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long g = threadIdx.x; g < H; g += blockDim.x) {
+        const int k = (int)(g % mh);
+        const int j = (int)((g / mh) % m);
+        const int i = (int)(g / ((long)mh * m));
+        const T th = theta[h0 + g];
+        T *c = cg + 2 * (h0 + g);
+        const T re = c[0], im = c[1];
+        if (th != (T)0) {
+            const double w = (k == 0 || (2 * k == m)) ? 1.0 : 2.0;
+            const double e = (double)th * ((double)re * (double)re + (double)im * (double)im);
+            const int ii = i > m / 2 ? i - m : i;
+            const int jj = j > m / 2 ? j - m : j;
+            const int kk = k > m / 2 ? k - m : k;
+            const double t1 = TWOPI * ii, t2 = TWOPI * jj, t3 = TWOPI * kk;
+            const double rx = t1 * B.u1 + t2 * B.v1 + t3 * B.w1;
+            const double ry = t1 * B.u2 + t2 * B.v2 + t3 * B.w2;
+            const double rz = t1 * B.u3 + t2 * B.v3 + t3 * B.w3;
+            const double rm = rx * rx + ry * ry + rz * rz;
+            const double vam = 2.0 * (1.0 / rm - rAlpha2) * e;
+            acc[0] += w * e;
+            acc[1] -= w * vam * rx * rx;
+            acc[2] -= w * vam * rx * ry;
+            acc[3] -= w * vam * rx * rz;
+            acc[4] -= w * vam * ry * ry;
+            acc[5] -= w * vam * ry * rz;
+            acc[6] -= w * vam * rz * rz;
+        }
+        c[0] = re * th;
+        c[1] = im * th;
+    }
+    mdg_block_partials<7>(acc, repInfl);   // row blockIdx.x = r
+}
+
+// Force gather for every replica (spme.c:593-646 with r's box and mesh).
+template <bool MIXED, typename TG, typename TF>
+__global__ void __launch_bounds__(256)
+k_b_gather(MdgParams P, const MdgBox *__restrict__ boxes, const int *__restrict__ arep,
+           const int *__restrict__ mesh, const long *__restrict__ gridOff, int n,
+           const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+           const double *__restrict__ q, const TG *__restrict__ grid, TF *__restrict__ frx,
+           TF *__restrict__ fry, TF *__restrict__ frz)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int r = arep[l];
+    const MdgBox &B = boxes[r];
+    const int p = P.nbsp, m = mesh[r];
+    const long base = gridOff[r];
+    double s1, s2, s3;
+    scaled_coords(B, m, m, m, x[l], y[l], z[l], s1, s2, s3);
+    typedef typename std::conditional<MIXED, float, double>::type T;
+    T b1[MAXP], b2[MAXP], b3[MAXP], d1[MAXP], d2[MAXP], d3[MAXP];
+    bspline_dev<T>(s1, p, b1, d1);
+    bspline_dev<T>(s2, p, b2, d2);
+    bspline_dev<T>(s3, p, b3, d3);
+    const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+    T fbx = 0, fby = 0, fbz = 0;
+    for (int i = 0; i < p; i++) {
+        const int o1 = m * wrap_index(i1 - i, m);
+        for (int j = 0; j < p; j++) {
+            const long o2 = base + (long)m * (wrap_index(i2 - j, m) + o1);
+            const T c1 = d1[i] * b2[j], c2 = b1[i] * d2[j], c3 = b1[i] * b2[j];
+            for (int k = 0; k < p; k++) {
+                const T qt = (T)grid[wrap_index(i3 - k, m) + o2];
+                fbx += qt * c1 * b3[k];
+                fby += qt * c2 * b3[k];
+                fbz += qt * c3 * d3[k];
+            }
+        }
+    }
+    const double gx = (double)fbx * m, gy = (double)fby * m, gz = (double)fbz * m;
+    const double c = -2.0 * (TWOPI / B.vol) * P.chargeConst * q[l];
+    frx[l] = (TF)(c * (gx * B.u1 + gy * B.v1 + gz * B.w1));
+    fry[l] = (TF)(c * (gx * B.u2 + gy * B.v2 + gz * B.w2));
+    frz[l] = (TF)(c * (gx * B.u3 + gy * B.v3 + gz * B.w3));
+}
+
+// Per replica (one block each): sums of the reciprocal forces of its atoms.
+template <typename TF>
+__global__ void __launch_bounds__(256)
+k_b_rec_sums(const int *__restrict__ aoff, const TF *__restrict__ frx, const TF *__restrict__ fry,
+             const TF *__restrict__ frz, double *__restrict__ repRec)
+{
+    const int r = blockIdx.x;
+    double acc[3] = {0, 0, 0};
+    for (int i = aoff[r] + threadIdx.x; i < aoff[r + 1]; i += blockDim.x) {
+        acc[0] += (double)frx[i];
+        acc[1] += (double)fry[i];
+        acc[2] += (double)frz[i];
+    }
+    mdg_block_partials<3>(acc, repRec);    // row blockIdx.x = r
+}
+
+// Meshes, influence functions and batched FFT plans of a replica batch:
+// replicas with the same mesh edge share one plan of that many transforms
+// and lie next to each other in the mesh arrays.
+int mdg_b_spme_setup(mdg_ctx *ctx)
+{
+    RETCHK(rocfft_init(ctx));
+    MdgBatch &T = *ctx->bt;
+    const int R = T.R, p = ctx->P.nbsp;
+    if (p < 2 || p > MAXP) {
+        ctx->err = "spline order outside 2..12";
+        return MDG_EUNSUPPORTED;
+    }
+    std::vector<int> sizes(T.hMesh);
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    T.hGridOff.assign(R, 0);
+    T.hHOff.assign(R, 0);
+    long g = 0, h = 0;
+    T.fft.clear();
+    for (int m : sizes) {
+        MdgFftGroup F;
+        F.m = m;
+        F.g0 = g;
+        F.h0 = h;
+        for (int r = 0; r < R; r++)
+            if (T.hMesh[r] == m) {
+                T.hGridOff[r] = g;
+                T.hHOff[r] = h;
+                g += (long)m * m * m;
+                h += (long)m * m * (m / 2 + 1);
+                F.count++;
+            }
+        T.fft.push_back(F);
+    }
+    T.Gtot = g;
+    T.Htot = h;
+    std::vector<double> th(h);
+    for (int r = 0; r < R; r++) {
+        std::vector<double> t;
+        const int m = T.hMesh[r];
+        theta_host(T.hBox[r], m, m, m, p, ctx->P.alpha, t);
+        std::copy(t.begin(), t.end(), th.begin() + T.hHOff[r]);
+    }
+    std::vector<float> thf(th.begin(), th.end());
+    HIPCHK(hipMalloc(&T.gridOff, R * sizeof(long)));
+    HIPCHK(hipMalloc(&T.hOff, R * sizeof(long)));
+    HIPCHK(hipMemcpy(T.gridOff, T.hGridOff.data(), R * sizeof(long), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(T.hOff, T.hHOff.data(), R * sizeof(long), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&ctx->theta, h * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->theta32, h * sizeof(float)));
+    HIPCHK(hipMemcpy(ctx->theta, th.data(), h * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->theta32, thf.data(), h * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&ctx->grid64, g * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->cgrid64, 2 * h * sizeof(double)));
+    HIPCHK(hipMalloc(&T.repInfl, R * 7 * sizeof(double)));
+    HIPCHK(hipMalloc(&T.repRec, R * 3 * sizeof(double)));
+    if (rocfft_execution_info_create(&ctx->info) != rocfft_status_success) {
+        ctx->err = "rocfft_execution_info_create failed";
+        return MDG_EHIP;
+    }
+    rocfft_execution_info_set_stream(ctx->info, ctx->stream);
+    const bool mixed = ctx->precision == MDG_MIXED;
+    for (MdgFftGroup &F : T.fft) {
+        if (mixed) RETCHK(make_plans_n(ctx, false, F.m, F.m, F.m, F.count, &F.fwd32, &F.bwd32));
+        else RETCHK(make_plans_n(ctx, true, F.m, F.m, F.m, F.count, &F.fwd, &F.bwd));
+    }
+    if (mixed) {
+        HIPCHK(hipMalloc(&ctx->grid32, g * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->cgrid32, 2 * h * sizeof(float)));
+        HIPCHK(hipMalloc(&ctx->gridq, g * sizeof(long long)));
+        RETCHK(mdg_fill_i64(ctx, ctx->gridq, g));
+    }
+    ctx->mh = 0;
+    return MDG_OK;
+}
+
+void mdg_b_spme_free(mdg_ctx *ctx)
+{
+    for (MdgFftGroup &F : ctx->bt->fft) {
+        if (F.fwd) rocfft_plan_destroy(F.fwd);
+        if (F.bwd) rocfft_plan_destroy(F.bwd);
+        if (F.fwd32) rocfft_plan_destroy(F.fwd32);
+        if (F.bwd32) rocfft_plan_destroy(F.bwd32);
+    }
+    ctx->bt->fft.clear();
+}
+
+// One SPME evaluation of every replica: reciprocal forces per atom (frx..
+// in MIXED, frx64.. in FP64), per-replica energy / stress sums (repInfl)
+// and sums of the reciprocal forces (repRec).
+int mdg_b_spme(mdg_ctx *ctx)
+{
+    MdgBatch &T = *ctx->bt;
+    const int n = ctx->n, R = T.R;
+    const double rAlpha2 = -0.25 / (ctx->P.alpha * ctx->P.alpha);
+    const bool mixed = ctx->precision == MDG_MIXED;
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[2], ctx->stream));
+    if (mixed) {
+        hipLaunchKernelGGL(k_b_spread<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P,
+                           T.boxes, T.arep, T.mesh, T.gridOff, n, ctx->x, ctx->y, ctx->z, ctx->q,
+                           nullptr, ctx->gridq);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(T.Gtot, 256), 8192)),
+                           dim3(256), 0, ctx->stream, T.Gtot, ctx->gridq, ctx->grid32);
+        LAUNCHCHK();
+        for (MdgFftGroup &F : T.fft) {
+            void *in[1] = {ctx->grid32 + F.g0}, *out[1] = {ctx->cgrid32 + 2 * F.h0};
+            if (rocfft_execute(F.fwd32, in, out, ctx->info) != rocfft_status_success) {
+                ctx->err = "rocfft_execute (forward) failed";
+                return MDG_EHIP;
+            }
+        }
+        hipLaunchKernelGGL(k_b_influence<float>, dim3(R), dim3(256), 0, ctx->stream, T.boxes,
+                           T.mesh, T.hOff, rAlpha2, ctx->theta32, ctx->cgrid32, T.repInfl);
+        LAUNCHCHK();
+        for (MdgFftGroup &F : T.fft) {
+            void *in[1] = {ctx->cgrid32 + 2 * F.h0}, *out[1] = {ctx->grid32 + F.g0};
+            if (rocfft_execute(F.bwd32, in, out, ctx->info) != rocfft_status_success) {
+                ctx->err = "rocfft_execute (inverse) failed";
+                return MDG_EHIP;
+            }
+        }
+        hipLaunchKernelGGL((k_b_gather<true, float, float>), dim3(nblk(n, 256)), dim3(256), 0,
+                           ctx->stream, ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n, ctx->x,
+                           ctx->y, ctx->z, ctx->q, ctx->grid32, ctx->frx, ctx->fry, ctx->frz);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_b_rec_sums<float>, dim3(R), dim3(256), 0, ctx->stream, T.aoff,
+                           ctx->frx, ctx->fry, ctx->frz, T.repRec);
+        LAUNCHCHK();
+    } else {
+        RETCHK(mdg_fill_f64(ctx, ctx->grid64, T.Gtot, 0.0));
+        hipLaunchKernelGGL(k_b_spread<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
+                           ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n, ctx->x, ctx->y, ctx->z,
+                           ctx->q, ctx->grid64, nullptr);
+        LAUNCHCHK();
+        for (MdgFftGroup &F : T.fft) {
+            void *in[1] = {ctx->grid64 + F.g0}, *out[1] = {ctx->cgrid64 + 2 * F.h0};
+            if (rocfft_execute(F.fwd, in, out, ctx->info) != rocfft_status_success) {
+                ctx->err = "rocfft_execute (forward) failed";
+                return MDG_EHIP;
+            }
+        }
+        hipLaunchKernelGGL(k_b_influence<double>, dim3(R), dim3(256), 0, ctx->stream, T.boxes,
+                           T.mesh, T.hOff, rAlpha2, ctx->theta, ctx->cgrid64, T.repInfl);
+        LAUNCHCHK();
+        for (MdgFftGroup &F : T.fft) {
+            void *in[1] = {ctx->cgrid64 + 2 * F.h0}, *out[1] = {ctx->grid64 + F.g0};
+            if (rocfft_execute(F.bwd, in, out, ctx->info) != rocfft_status_success) {
+                ctx->err = "rocfft_execute (inverse) failed";
+                return MDG_EHIP;
+            }
+        }
+        hipLaunchKernelGGL((k_b_gather<false, double, double>), dim3(nblk(n, 256)), dim3(256), 0,
+                           ctx->stream, ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n, ctx->x,
+                           ctx->y, ctx->z, ctx->q, ctx->grid64, ctx->frx64, ctx->fry64,
+                           ctx->frz64);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_b_rec_sums<double>, dim3(R), dim3(256), 0, ctx->stream, T.aoff,
+                           ctx->frx64, ctx->fry64, ctx->frz64, T.repRec);
+        LAUNCHCHK();
+    }
+    if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[3], ctx->stream));
     return MDG_OK;
 }
