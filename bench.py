@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (skip the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sharded "
@@ -107,6 +109,8 @@ def main():
     ctx = M.Context(wb, precision=M.MDG_MIXED)
     ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
     ctx.initial_energy()
+    tuning = None if args.no_tune else ctx.autotune(60)   # launch shapes, before the timed region
+    log(f"autotuned launch shapes: {tuning}")
     if W:
         ctx.run(W, use_graph=True)
     try:
@@ -190,6 +194,7 @@ def main():
                      "energy_drift_kcal_mol_ns_per_water_fp64": drift_64,
                      "drift_window_steps": K},
         "phases_ms": ph,
+        "tuning": tuning,
         "roofline": {"kernel": "k_pair_cluster<MIXED>", "bound": "valu", "achieved": achieved / 1e12,
                      "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK,
                      "traffic": None,

@@ -275,6 +275,14 @@ int mdg_backward(mdg_ctx *ctx, double dL_dE, double dL_dq[], double dL_deps[], d
 int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, double dJ_dq[],
                       double dJ_deps[], double dJ_dsig[]);
 
+/* Autotuner: times nsteps graph-replayed steps of the uploaded system for
+ * each candidate launch shape (the three-site SHAKE kernels' molecules per
+ * wavefront, the cluster pair kernel's grid) and keeps the fastest; the
+ * state is restored and the initial energy recomputed.  out[0] = SHAKE
+ * spread (16/spread molecules per wavefront), out[1] = pair grid cap.  No
+ * reference counterpart. */
+int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2]);
+
 /* Average kernel durations (ms) of the last mdg_run_steps, measured with
  * HIP events on the context stream: [0] pair kernel, [1] SPME (spline to
  * gather), [2] integrator, [3] list rebuild (per rebuild), [4] whole step.

@@ -86,7 +86,7 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_vv_integrate mdg_upload_state mdg_download_state mdg_initial_energy "
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
            "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj "
-           "mdg_batch_create mdg_batch_size").split()
+           "mdg_batch_create mdg_batch_size mdg_autotune").split()
 
 _LIB = None
 
@@ -338,6 +338,13 @@ class Context:
         self._chk(self.L.mdg_run_steps(self.h, int(nsteps), int(use_graph), recs),
                   "mdg_run_steps")
         return recs
+
+    def autotune(self, nsteps: int = 60):
+        """Times candidate launch shapes on this system and keeps the best;
+        returns them (the uploaded state is restored)."""
+        out = (C.c_int * 2)()
+        self._chk(self.L.mdg_autotune(self.h, int(nsteps), out), "mdg_autotune")
+        return dict(shake_spread=out[0], pair_blocks=out[1])
 
     # ---- adjoint --------------------------------------------------------
     def set_params(self, q, eps, sig):

@@ -281,7 +281,10 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
         return fail(ctx, MDG_EUNSUPPORTED);
     }
     if (hipGetDevice(&ctx->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "no HIP device / stream";
         return fail(ctx, MDG_EHIP);
     }
@@ -370,7 +373,10 @@ extern "C" mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *para
     ctx->B = T.hBox[0];
     ctx->keyLink = 0;
     if (hipGetDevice(&ctx->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "no HIP device / stream";
         return fail(ctx, MDG_EHIP);
     }
@@ -448,6 +454,9 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
     for (hipEvent_t e : ctx->ev)
         if (e) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) hipStreamDestroy(ctx->stream2);
+    if (ctx->evFork) hipEventDestroy(ctx->evFork);
+    if (ctx->evJoin) hipEventDestroy(ctx->evJoin);
     delete ctx;
 }
 

@@ -569,6 +569,15 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
     return MDG_OK;
 }
 
+// Blocks of the VV stage kernels (and so rows of their kinetic partials):
+// four lanes per three-site molecule, one thread per generic group.
+int mdg_vv_blocks(const mdg_ctx *ctx)
+{
+    if (!ctx->shakeW3) return nblk(ctx->nGrp, 128);
+    const long waves = nblk(ctx->nGrp, 16 / ctx->tune.shkSpread);
+    return nblk(waves * 64, 256);
+}
+
 // Runs one VV stage on the device state with forces fx.. (device, atom
 // order).  Per-group SHAKE virials go to grpVir (stage 1) and per-block
 // kinetic partials to epart + PART_KIN (stage 2); with sync the sums land
@@ -621,6 +630,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.boxes = nullptr;
     A.grep = nullptr;
     A.grpKin = nullptr;
+    A.spread = ctx->tune.shkSpread;
     const bool w3 = ctx->shakeW3;
     const bool batch = ctx->bt != nullptr;
     if (batch) {
@@ -633,8 +643,8 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         A.grpKin = ctx->bt->grpKin;
         A.mask = ctx->bt->masks;
     }
-    const int bs = w3 ? 64 : 128;
-    const int nb = nblk(ctx->nGrp, bs);
+    const int bs = w3 ? 256 : 128;
+    const int nb = mdg_vv_blocks(ctx);
     // The device-resident step resets the mask in the kernels that run
     // before each stage and reduces the partials in its record kernel; the
     // drop-in call does both here.

@@ -121,6 +121,12 @@ struct MdgBatch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
 
+// Launch shapes the autotuner chooses (mdg_autotune).
+struct MdgTune {
+    int shkSpread = 1;           // three-site SHAKE: quads per molecule slot (1, 2, 4, 8, 16)
+    int pairBlocks = 4096;       // cap on the cluster pair kernel's grid
+};
+
 struct mdg_ctx {
     int precision = MDG_FP64;
     int n = 0;
@@ -130,6 +136,11 @@ struct mdg_ctx {
     MdgBox B{};
     std::string err;
     hipStream_t stream = nullptr;
+    // second stream of the step: SPME runs on it beside the list rebuild and
+    // the pair kernel (forked and joined with events, so a captured step
+    // graph has the two branches)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
     int device = 0;
 
     // state (FP64, reference atom order)
@@ -216,6 +227,7 @@ struct mdg_ctx {
     // graph
     hipGraphExec_t graphExec[2] = {nullptr, nullptr};
     MdgBatch *bt = nullptr;      // replica batch, when created by mdg_batch_create
+    MdgTune tune;
     int graphStepsPerLaunch = 0;
     int step = 0;                // last completed MD step
     int stepsSinceList = 0;
@@ -249,7 +261,7 @@ enum {
 // regions of ctx->epart (per-block partial sums), each PART_R doubles
 #define PART_R (1 << 18)
 // the cluster pair kernel loops over chunks with at most this many blocks
-#define MDG_PAIR_MAX_BLOCKS 4096
+#define MDG_PAIR_MAX_BLOCKS 16384
 enum {
     PART_PAIR = 0, PART_EXCL = PART_R, PART_INFL = 2 * PART_R, PART_GATHER = 3 * PART_R,
     PART_KIN = 4 * PART_R, PART_MISC = 5 * PART_R, PART_REF = 6 * PART_R, PART_TOTAL = 8 * PART_R
@@ -283,7 +295,7 @@ enum {
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 static inline int mdg_pair_blocks(const mdg_ctx *ctx)
 {
-    return std::min(nblk(ctx->cl.maxChunks, 4), MDG_PAIR_MAX_BLOCKS);
+    return std::min(nblk(ctx->cl.maxChunks, 4), std::min(ctx->tune.pairBlocks, MDG_PAIR_MAX_BLOCKS));
 }
 
 // utils.c:606-609: round half away from zero through a truncating cast.
@@ -414,6 +426,7 @@ int mdg_spme_setup(mdg_ctx *ctx);
 int mdg_spme_free(mdg_ctx *ctx);
 int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8]);
 int mdg_spme_device(mdg_ctx *ctx);   // device-resident: frx.. per atom, partials in epart
+int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *));   // runs fn's launches on stream2
 int mdg_spme_pot(mdg_ctx *ctx, double *pot);   // FP64 dE_rec/dq per atom
 int mdg_adj_pair(mdg_ctx *ctx);      // real-space dE/d(q, eps, sig) into fdx/fdy/fdz slots
 
@@ -426,6 +439,7 @@ void mdg_invalidate_graphs(mdg_ctx *ctx);
 
 // integrate (mdg_integrate.hip)
 int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst);
+int mdg_vv_blocks(const mdg_ctx *ctx);
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
                  double *virshake_out, bool sync);
 

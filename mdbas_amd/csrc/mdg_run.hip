@@ -7,6 +7,7 @@
 #include "mdg_shake.h"
 
 #include <cstring>
+#include <initializer_list>
 
 #define TWOPI_R 6.283185307179586
 
@@ -250,6 +251,8 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
 {
     if (ctx->bt) return b_energy(ctx, rebuild);
     const int n = ctx->n;
+    // SPME on the second stream, beside the list rebuild and the pair kernel
+    RETCHK(mdg_spme_on(ctx, mdg_spme_device));
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
         RETCHK(mdg_build_cluster_list(ctx));
@@ -257,8 +260,8 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
         RETCHK(mdg_slot_params(ctx));
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
     }
-    RETCHK(mdg_spme_device(ctx));
     RETCHK(mdg_pair_cluster(ctx));
+    if (ctx->stream2) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     const bool mixed = ctx->precision == MDG_MIXED;
     FinArgs F;
     F.P = ctx->P;
@@ -339,7 +342,7 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
         RETCHK(b_record(ctx, rebuild));
     else
         hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
-                           rec_args(ctx, nblk(ctx->nGrp, ctx->shakeW3 ? 64 : 128), ctx->nGrp),
+                           rec_args(ctx, mdg_vv_blocks(ctx), ctx->nGrp),
                            ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
@@ -812,6 +815,7 @@ static int b_energy(mdg_ctx *ctx, bool rebuild)
 {
     MdgBatch &T = *ctx->bt;
     const int n = ctx->n;
+    RETCHK(mdg_spme_on(ctx, mdg_b_spme));
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
         RETCHK(mdg_b_build_list(ctx));
@@ -819,8 +823,8 @@ static int b_energy(mdg_ctx *ctx, bool rebuild)
         RETCHK(mdg_slot_params(ctx));
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
     }
-    RETCHK(mdg_b_spme(ctx));
     RETCHK(mdg_b_pair(ctx));
+    if (ctx->stream2) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     BFinArgs F;
     F.P = ctx->P;
     F.n = n;
@@ -901,4 +905,87 @@ static int b_record(mdg_ctx *ctx, bool rebuild)
     hipLaunchKernelGGL(k_b_record, dim3(T.R), dim3(256), 0, ctx->stream, R, rebuild ? 1 : 0);
     LAUNCHCHK();
     return MDG_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+
+// Wall time (ms, HIP events on the context stream) of nsteps steps from a
+// fresh start at state (x, v), after a warm-up that captures the graphs.
+static int time_steps(mdg_ctx *ctx, const double *state, int nsteps, float &ms)
+{
+    const size_t n = ctx->n;
+    double *d[6] = {ctx->x, ctx->y, ctx->z, ctx->vx, ctx->vy, ctx->vz};
+    for (int k = 0; k < 6; k++)
+        HIPCHK(hipMemcpyAsync(d[k], state + k * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    const int step0 = ctx->step;
+    RETCHK(mdg_initial_energy(ctx, nullptr));
+    RETCHK(mdg_run_steps(ctx, ctx->P.update, 1, nullptr));   // captures both variants
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    int st = mdg_run_steps(ctx, nsteps, 1, nullptr);
+    if (st == MDG_OK) {
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->step = step0;
+    return st;
+}
+
+// Picks the launch shapes of this system on this GPU by timing whole steps
+// (graph-replayed, as mdg_run_steps runs them): first the three-site SHAKE
+// spread, then the pair kernel's grid cap, each with the other held.  The
+// uploaded state is restored afterwards and the initial energy recomputed.
+// out (may be NULL): the chosen spread and pair-grid cap.
+extern "C" int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2])
+{
+    if (!ctx || nsteps <= 0) return MDG_EARG;
+    if (!ctx->cl.cpJ) {
+        ctx->err = "call mdg_initial_energy first";
+        return MDG_EARG;
+    }
+    const size_t n = ctx->n;
+    double *state = nullptr;
+    HIPCHK(hipMalloc(&state, 6 * n * sizeof(double)));
+    double *d[6] = {ctx->x, ctx->y, ctx->z, ctx->vx, ctx->vy, ctx->vz};
+    for (int k = 0; k < 6; k++)
+        HIPCHK(hipMemcpyAsync(state + k * n, d[k], n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    int st = MDG_OK;
+    auto sweep = [&](int *param, std::initializer_list<int> values) {
+        int best = *param;
+        float bestMs = 1e30f;
+        for (int v : values) {
+            if (st) return;
+            *param = v;
+            mdg_invalidate_graphs(ctx);
+            float ms = 0.f;
+            st = time_steps(ctx, state, nsteps, ms);
+            if (st == MDG_OK && ms < bestMs) {
+                bestMs = ms;
+                best = v;
+            }
+        }
+        *param = best;
+    };
+    if (ctx->shakeW3) sweep(&ctx->tune.shkSpread, {1, 2, 4, 8});
+    sweep(&ctx->tune.pairBlocks, {1024, 2048, 4096, 8192, 16384});
+    mdg_invalidate_graphs(ctx);
+    for (int k = 0; k < 6 && st == MDG_OK; k++)
+        if (hipMemcpyAsync(d[k], state + k * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx->stream) != hipSuccess)
+            st = MDG_EHIP;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(state);
+    if (st == MDG_OK) st = mdg_initial_energy(ctx, nullptr);
+    if (out) {
+        out[0] = ctx->tune.shkSpread;
+        out[1] = ctx->tune.pairBlocks;
+    }
+    return st;
 }

@@ -33,7 +33,21 @@ struct ShakeArgs {
     const MdgBox *boxes;
     const int *grep;
     double *grpKin;
+    // three-site kernels: one molecule per quad of lanes, on every
+    // spread-th quad of a wavefront (16 / spread molecules per wavefront)
+    int spread;
 };
+
+// Molecule of this lane's quad, or nGrp for a quad left idle.
+__device__ inline int shake_w3_group(const ShakeArgs &A)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int qd = lane >> 2;
+    if (qd % A.spread) return A.nGrp;
+    const int g = wave * (16 / A.spread) + qd / A.spread;
+    return g < A.nGrp ? g : A.nGrp;
+}
 
 // Publishes the group's converged-at-sweep-k bits into the device-wide
 // AND mask: bits for k <= kend are as checked; later sweeps count as

@@ -70,14 +70,36 @@ __device__ inline int wrap_index(int ii, int m)
     return ii;
 }
 
+// Thread t spreads atom (t * stride) mod n, stride coprime to n: the atoms
+// of one wavefront lie far apart, so their atomics do not collide on the
+// same mesh points (neighbouring atom indices are neighbouring atoms).
+// A stride near n / golden ratio with gcd(stride, n) = 1 (t -> t * stride
+// mod n is then a permutation of the atoms).
+static int mdg_scatter_stride(int n)
+{
+    if (n <= 2) return 1;
+    long s = std::max(1L, (long)(n * 0.6180339887498949));
+    auto gcd = [](long a, long b) {
+        while (b) {
+            long t = a % b;
+            a = b;
+            b = t;
+        }
+        return a;
+    };
+    while (gcd(s, n) != 1) s++;
+    return (int)s;
+}
+
 template <bool MIXED>
 __global__ void __launch_bounds__(256)
-k_spme_spread(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
+k_spme_spread(MdgParams P, MdgBox B, int n, int stride, const double *__restrict__ x,
               const double *__restrict__ y, const double *__restrict__ z,
               const double *__restrict__ q, double *grid64, long long *gridq)
 {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int l = (int)(((long)t * stride) % n);
     const int p = P.nbsp, m1 = P.m1, m2 = P.m2, m3 = P.m3;
     double s1, s2, s3;
     scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
@@ -420,7 +442,8 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
     const double fact1 = -2.0 * (TWOPI / ctx->B.vol) * ctx->P.chargeConst;
     if (mixed) {
         hipLaunchKernelGGL(k_spme_spread<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
-                           ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, nullptr, ctx->gridq);
+                           ctx->P, ctx->B, n, mdg_scatter_stride(n), ctx->x, ctx->y, ctx->z,
+                           ctx->q, nullptr, ctx->gridq);
         LAUNCHCHK();
         hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(G, 256), 8192)), dim3(256),
                            0, ctx->stream, G, ctx->gridq, ctx->grid32);
@@ -446,7 +469,8 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
     } else {
         RETCHK(mdg_fill_f64(ctx, ctx->grid64, G, 0.0));
         hipLaunchKernelGGL(k_spme_spread<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
-                           ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, ctx->grid64, nullptr);
+                           ctx->P, ctx->B, n, mdg_scatter_stride(n), ctx->x, ctx->y, ctx->z,
+                           ctx->q, ctx->grid64, nullptr);
         LAUNCHCHK();
         void *in[1] = {ctx->grid64}, *out[1] = {ctx->cgrid64};
         if (rocfft_execute(ctx->fwd, in, out, ctx->info) != rocfft_status_success) {
@@ -573,12 +597,13 @@ int mdg_spme_pot(mdg_ctx *ctx, double *pot)
 template <bool MIXED>
 __global__ void __launch_bounds__(256)
 k_b_spread(MdgParams P, const MdgBox *__restrict__ boxes, const int *__restrict__ arep,
-           const int *__restrict__ mesh, const long *__restrict__ gridOff, int n,
+           const int *__restrict__ mesh, const long *__restrict__ gridOff, int n, int stride,
            const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
            const double *__restrict__ q, double *grid64, long long *gridq)
 {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int l = (int)(((long)t * stride) % n);   // as k_spme_spread
     const int r = arep[l];
     const MdgBox &B = boxes[r];
     const int p = P.nbsp, m = mesh[r];
@@ -816,8 +841,8 @@ int mdg_b_spme(mdg_ctx *ctx)
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[2], ctx->stream));
     if (mixed) {
         hipLaunchKernelGGL(k_b_spread<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P,
-                           T.boxes, T.arep, T.mesh, T.gridOff, n, ctx->x, ctx->y, ctx->z, ctx->q,
-                           nullptr, ctx->gridq);
+                           T.boxes, T.arep, T.mesh, T.gridOff, n, mdg_scatter_stride(n), ctx->x,
+                           ctx->y, ctx->z, ctx->q, nullptr, ctx->gridq);
         LAUNCHCHK();
         hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(T.Gtot, 256), 8192)),
                            dim3(256), 0, ctx->stream, T.Gtot, ctx->gridq, ctx->grid32);
@@ -849,8 +874,8 @@ int mdg_b_spme(mdg_ctx *ctx)
     } else {
         RETCHK(mdg_fill_f64(ctx, ctx->grid64, T.Gtot, 0.0));
         hipLaunchKernelGGL(k_b_spread<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
-                           ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n, ctx->x, ctx->y, ctx->z,
-                           ctx->q, ctx->grid64, nullptr);
+                           ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n, mdg_scatter_stride(n),
+                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->grid64, nullptr);
         LAUNCHCHK();
         for (MdgFftGroup &F : T.fft) {
             void *in[1] = {ctx->grid64 + F.g0}, *out[1] = {ctx->cgrid64 + 2 * F.h0};
@@ -879,5 +904,24 @@ int mdg_b_spme(mdg_ctx *ctx)
         LAUNCHCHK();
     }
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[3], ctx->stream));
+    return MDG_OK;
+}
+
+// Runs fn (an SPME evaluation) on the context's second stream, forked from
+// and joined back into the main stream by events; rocFFT follows the
+// stream.  Without a second stream fn runs in place.
+int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *))
+{
+    if (!ctx->stream2) return fn(ctx);
+    hipStream_t main = ctx->stream;
+    HIPCHK(hipEventRecord(ctx->evFork, main));
+    HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->evFork, 0));
+    ctx->stream = ctx->stream2;
+    rocfft_execution_info_set_stream(ctx->info, ctx->stream2);
+    const int st = fn(ctx);
+    ctx->stream = main;
+    rocfft_execution_info_set_stream(ctx->info, main);
+    RETCHK(st);
+    HIPCHK(hipEventRecord(ctx->evJoin, ctx->stream2));
     return MDG_OK;
 }

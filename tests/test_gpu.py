@@ -423,3 +423,26 @@ def test_batch_mixed_matches_single_contexts(replicas):
         assert r0[r].pot == pytest.approx(s0.pot, rel=2e-6)
         assert relerr(np.stack([fb[k][r] for k in range(3)]), f) < 2e-6
         assert np.abs(rb["tot"][:, r] - rs["tot"]).max() < 2e-6 * abs(rs["tot"][0])
+
+
+def test_autotune_keeps_results(c0_box):
+    """Launch shapes change no arithmetic: after mdg_autotune the mixed
+    trajectory's positions are bit for bit those of the default shapes,
+    and the uploaded state is restored."""
+    wb = c0_box
+    out = []
+    for tune in (False, True):
+        ctx = M.Context(wb, precision=M.MDG_MIXED)
+        ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        ctx.initial_energy()
+        if tune:
+            say("autotune", ctx.autotune(40))
+            a = ctx.download()
+            assert all(np.array_equal(a[k], v) for k, v in enumerate((wb.x, wb.y, wb.z)))
+        r = M.records_to_dict(ctx.run(25, use_graph=True))
+        out.append((r, ctx.download()))
+        ctx.close()
+    (ra, sa), (rb, sb) = out
+    for u, v in zip(sa[:6], sb[:6]):
+        assert np.array_equal(u, v)
+    assert np.allclose(ra["tot"], rb["tot"], rtol=1e-13, atol=0)
