@@ -2,11 +2,13 @@
 box in mixed precision on one MI355X, with the accuracy against FP64, the
 pair kernel's roofline and the FP64 CPU oracle timed on the host.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|batch]
 
 One step = one MD step of the whole system (VV stage 1 with SHAKE, list
 rebuild every 20 steps, SPME + real-space Ewald/LJ forces, VV stage 2
-with SHAKE), all on the device.  Prints one JSON line.
+with SHAKE), all on the device.  Prints one JSON line.  The default
+workload is the 30k-atom box; c2 is the 192k-atom box and batch the 256
+uneven replica boxes (one context, every kernel covering all replicas).
 """
 from __future__ import annotations
 
@@ -26,6 +28,18 @@ FP32_PEAK = 157.3e12          # MI355X dense FP32 vector, FLOP/s
 HBM_PEAK = 8.0e12             # MI355X HBM3E, B/s
 FLOP_IN, FLOP_OUT = 64, 8     # per in-cutoff pair / per listed pair beyond rc
 
+WORKLOADS = {
+    "c1": ("tip3p_10k_mixed: 10,000 rigid TIP3P waters (30,000 atoms), 67.0 A cubic box, seed 1; "
+           "SPME 64^3 order 4, alpha 0.32; cutoff 12 A, LJ switch from 10 A, skin 2 A, list every "
+           "20 steps; SHAKE; dt 2 fs; FP32 pair/spline math, int64 fixed-point forces"),
+    "c2": ("tip3p_64k_mixed: 64,000 rigid TIP3P waters (192,000 atoms), 124.1 A cubic box, seed 2; "
+           "SPME 128^3 order 4, alpha 0.32; cutoff 12 A, switch 10 A, skin 2 A; SHAKE; dt 2 fs; "
+           "mixed"),
+    "batch": ("tip3p_replicas_256_mixed: 256 independent TIP3P boxes of 216-512 waters (0.0334 "
+              "molecules/A^3), 280-400 K, seed 3; cutoff 9 A, switch 7.5 A, alpha 0.35, SPME mesh "
+              "~1 A rounded up to a multiple of 4, order 4; SHAKE; dt 2 fs; mixed; one context"),
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -35,63 +49,69 @@ def rel(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
 
 
-def cpu_baseline(wb, cfg, budget_s=25.0):
+def sync_fn():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.synchronize
+    except Exception:
+        pass
+    return lambda: None
+
+
+def cpu_baseline(boxes, budget_s=25.0, max_steps=10):
     """The FP64 oracle (a from-scratch port of the reference's path) on one
-    host core: one link-cell list build, step energies and VV stages,
-    combined as step + build/update."""
+    host core: per box one list build, then MD steps (VV stages and
+    energies); seconds per step = step + build/update, summed over boxes."""
     import oracle as O
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except (AttributeError, OSError):
         pass
-    o = O.OracleSystem(wb, cfg)
-    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
-    vx, vy, vz = wb.vx.copy(), wb.vy.copy(), wb.vz.copy()
-    t0 = time.perf_counter()
-    o.build_list(x, y, z, build=True)
-    t_list = time.perf_counter() - t0
-    f, e = o.energy(x, y, z)
-    steps, t_step = 0, 0.0
-    t_begin = time.perf_counter()
-    while True:
+    t_all, steps_done, descr = 0.0, [], []
+    per_box = budget_s / len(boxes)
+    for wb in boxes:
+        cfg = dict(wb.params)
+        o = O.OracleSystem(wb, cfg)
+        x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+        vx, vy, vz = wb.vx.copy(), wb.vy.copy(), wb.vz.copy()
         t0 = time.perf_counter()
-        o.vv(1, x, y, z, vx, vy, vz, f)
+        o.build_list(x, y, z, build=True)
+        t_list = time.perf_counter() - t0
         f, e = o.energy(x, y, z)
-        o.vv(2, x, y, z, vx, vy, vz, f)
-        t_step += time.perf_counter() - t0
-        steps += 1
-        if time.perf_counter() - t_begin > budget_s * 0.6 or steps >= 10:
-            break
-    per_step = t_step / steps + t_list / cfg["update"]
-    ns_day = cfg["timeStep"] * 1e-3 * 86400.0 / per_step
-    return dict(value=ns_day, unit="ns/day", cores=1, kind="port",
-                pair_interactions_per_s=e["nInCut"] / per_step,
-                s_per_step=per_step,
-                sample=(f"FP64 oracle on 1 host core ({os.uname().machine}, nproc {os.cpu_count()}): "
-                        f"1 link-cell list build ({t_list:.2f} s) + {steps} MD steps of the same "
-                        f"30k-atom box ({t_step / steps:.2f} s/step), per step = step + build/20"))
+        steps, t_step, tb = 0, 0.0, time.perf_counter()
+        while True:
+            t0 = time.perf_counter()
+            o.vv(1, x, y, z, vx, vy, vz, f)
+            f, e = o.energy(x, y, z)
+            o.vv(2, x, y, z, vx, vy, vz, f)
+            t_step += time.perf_counter() - t0
+            steps += 1
+            if time.perf_counter() - tb > per_box * 0.6 or steps >= max_steps:
+                break
+        t_all += t_step / steps + t_list / cfg["update"]
+        steps_done.append(steps)
+        descr.append((t_list, t_step / steps))
+    return t_all, steps_done, descr
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default launch shapes (skip the autotuner)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sharded "
-              "multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
-        sys.exit(2)
+def pair_traffic(name):
+    """HBM bytes per pair-kernel launch of the 30k-atom box, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a
+    copy of known size); None for other workloads."""
+    p = os.path.join(HERE, "profiles", "pair_kernel_traffic.json")
+    if name != "c1" or not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)["bytes_per_launch"]
 
-    import mdbas_amd as M
-    from mdbas_amd import configs
-    cfg = configs.CONFIGS["c1"]
+
+def single_box(args, M, configs, name):
+    cfg = configs.CONFIGS[name]
     wb = configs.build(cfg)
     N, K, W = wb.n, args.steps, args.warmup
     log(f"workload {cfg['name']}: {N} atoms, K={K}, W={W}")
+    sync = sync_fn()
 
     # accuracy at the initial coordinates: mixed vs FP64 forces
     forces = {}
@@ -113,11 +133,6 @@ def main():
     log(f"autotuned launch shapes: {tuning}")
     if W:
         ctx.run(W, use_graph=True)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
     sync()
     t0 = time.perf_counter()
     recs = ctx.run(K, use_graph=True)                  # returns after a stream sync
@@ -131,9 +146,9 @@ def main():
     stats = ctx.list_stats()
     log(f"timed {K} steps: {wall:.3f} s, {ms_step:.4f} ms/step, {ns_day:.1f} ns/day")
 
-    # kernel durations: up to 200 more steps as plain launches with HIP
-    # events around the pair kernel, SPME, the integrator, the list rebuild
-    # and the whole step, on the context stream
+    # kernel durations: up to 200 more steps as plain launches on one
+    # stream, with HIP events around the pair kernel, SPME, the integrator,
+    # the list rebuild and the whole step
     ctx.set_profiling(True)
     Kp = min(K, 200)
     recs2 = M.records_to_dict(ctx.run(Kp, use_graph=0))
@@ -164,11 +179,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (FP64 oracle, 1 core)...")
-        cpu = cpu_baseline(wb, cfg)
+        t_step, steps, descr = cpu_baseline([wb], max_steps=10 if name == "c1" else 2)
+        cpu = dict(value=cfg["timeStep"] * 1e-3 * 86400.0 / t_step, unit="ns/day", cores=1,
+                   kind="port", pair_interactions_per_s=pin / t_step, s_per_step=t_step,
+                   sample=(f"FP64 oracle on 1 host core ({os.uname().machine}, nproc "
+                           f"{os.cpu_count()}): 1 link-cell list build ({descr[0][0]:.2f} s) + "
+                           f"{steps[0]} MD steps of the same {N}-atom box ({descr[0][1]:.2f} "
+                           f"s/step), per step = step + build/20"))
         log(f"cpu: {cpu}")
 
-    out = {
-        "metric": METRIC,
+    return {
+        "metric": METRIC if name == "c1" else METRIC.replace("30k-atom", f"{N // 1000}k-atom"),
         "value": ns_day,
         "unit": "ns/day",
         "n_gpus": 1,
@@ -180,12 +201,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": ("tip3p_10k_mixed: 10,000 rigid TIP3P waters (30,000 atoms), 67.0 A "
-                         "cubic box, seed 1; SPME 64^3 order 4, alpha 0.32; cutoff 12 A, LJ "
-                         "switch from 10 A, skin 2 A, list every 20 steps; SHAKE; dt 2 fs; "
-                         "FP32 pair/spline math, int64 fixed-point forces"),
-            "atoms": N, "dt_fs": cfg["timeStep"] * 1e3, "precision": "mixed"},
+        "config": {"workload": WORKLOADS[name], "atoms": N, "dt_fs": cfg["timeStep"] * 1e3,
+                   "precision": "mixed"},
         "pair_interactions_per_s": pis,
         "pair_interactions_per_step": pin,
         "listed_pairs": plist,
@@ -195,9 +212,9 @@ def main():
                      "drift_window_steps": K},
         "phases_ms": ph,
         "tuning": tuning,
-        "roofline": {"kernel": "k_pair_cluster<MIXED>", "bound": "valu", "achieved": achieved / 1e12,
-                     "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK,
-                     "traffic": None,
+        "roofline": {"kernel": "k_pair_cluster<MIXED>", "bound": "valu",
+                     "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK, "traffic": pair_traffic(name),
                      "flop_per_launch": flop, "launch_ms": ph["pair_ms"]},
         "roofline_spme": {"bound": "hbm", "achieved": spme_bytes / (ph["spme_ms"] * 1e-3) / 1e9,
                           "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -206,6 +223,94 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (ns_day / cpu["value"]) if cpu else None,
     }
+
+
+def batch(args, M, configs):
+    cfgs = configs.replica_configs(256)
+    boxes = [configs.build(c) for c in cfgs]
+    R, K, W = len(boxes), args.steps, args.warmup
+    N = sum(b.n for b in boxes)
+    log(f"workload batch: {R} replicas, {N} atoms, K={K}, W={W}")
+    sync = sync_fn()
+    b = M.BatchContext(boxes, precision=M.MDG_MIXED)
+    b.upload_boxes()
+    b.initial_energy()
+    tuning = None if args.no_tune else b.autotune(60)
+    log(f"autotuned launch shapes: {tuning}")
+    if W:
+        b.run(W, use_graph=True)
+    sync()
+    t0 = time.perf_counter()
+    recs = b.run(K, use_graph=True)
+    sync()
+    wall = time.perf_counter() - t0
+    D = M.batch_records(recs, R)
+    dt = cfgs[0]["timeStep"]
+    ns_day = dt * 1e-3 * K * 86400.0 / wall          # every replica advances this much
+    pis = float(D["nInCut"].sum()) / wall
+    log(f"timed {K} steps: {wall:.3f} s, {wall / K * 1e3:.4f} ms/step, {ns_day:.1f} ns/day per "
+        f"replica, {R * ns_day:.0f} replica-ns/day")
+    b.set_profiling(True)
+    b.run(min(K, 100), use_graph=0)
+    ph = b.phase_times()
+    b.set_profiling(False)
+    b.close()
+    cpu = None
+    if not args.no_cpu:
+        sample = [boxes[k] for k in (0, R // 3, 2 * R // 3, R - 1)]
+        t_sum, steps, descr = cpu_baseline(sample, budget_s=20.0, max_steps=5)
+        t_all = t_sum * R / len(sample)                # all replicas, one after another
+        cpu = dict(value=R * dt * 1e-3 * 86400.0 / t_all, unit="replica-ns/day", cores=1,
+                   kind="port", s_per_step_all_replicas=t_all,
+                   sample=(f"FP64 oracle on 1 host core: replicas 0, {R // 3}, {2 * R // 3}, "
+                           f"{R - 1} ({steps} MD steps each after one list build), scaled to all "
+                           f"{R} run one after another"))
+        log(f"cpu: {cpu}")
+    return {
+        "metric": "replica-ns/day & pair-interactions/s, 256 uneven SPME water replicas, mixed",
+        "value": R * ns_day,
+        "unit": "replica-ns/day",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["batch"], "replicas": R, "atoms": N, "dt_fs": dt * 1e3,
+                   "precision": "mixed"},
+        "ns_per_day_per_replica": ns_day,
+        "pair_interactions_per_s": pis,
+        "phases_ms": ph,
+        "tuning": tuning,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (R * ns_day / cpu["value"]) if cpu else None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c1")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (skip the autotuner)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sharded "
+              "multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import mdbas_amd as M
+    from mdbas_amd import configs
+    if args.workload == "batch":
+        out = batch(args, M, configs)
+    else:
+        out = single_box(args, M, configs, args.workload)
     print(json.dumps(out), flush=True)
 
 

@@ -80,6 +80,7 @@ struct ClusterList {
     // spatial sort scratch
     int ncol = 0, nfx = 0, nfy = 0;
     int *colCnt = nullptr, *colStart = nullptr, *colPad = nullptr, *colSlot = nullptr;
+    int *cluCol = nullptr;       // column of each cluster (-1 unused)
     int *sortTmp = nullptr;
     float *sortKey = nullptr;
 };
@@ -141,6 +142,7 @@ struct mdg_ctx {
     // graph has the two branches)
     hipStream_t stream2 = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
+    bool spmeForked = false;     // the current step's SPME went to stream2
     int device = 0;
 
     // state (FP64, reference atom order)

@@ -399,7 +399,7 @@ __global__ void k_col_pad(int ncol, const int *__restrict__ colCnt, int *__restr
 __global__ void k_col_slots(int ncol, const int *__restrict__ colStart,
                             const int *__restrict__ colCnt, const int *__restrict__ colSlot,
                             const unsigned long long *__restrict__ keys, int *__restrict__ slotAtom,
-                            int *__restrict__ atomSlot)
+                            int *__restrict__ atomSlot, int *__restrict__ cluCol)
 {
     int c = blockIdx.x;
     int s0 = colSlot[c], np = colSlot[c + 1] - s0, na = colCnt[c], a0 = colStart[c];
@@ -410,6 +410,7 @@ __global__ void k_col_slots(int ncol, const int *__restrict__ colStart,
             atomSlot[atom] = s0 + t;
         }
         slotAtom[s0 + t] = atom;
+        if ((t & (MDG_CL - 1)) == 0) cluCol[(s0 + t) / MDG_CL] = c;   // the cluster's column
     }
 }
 
@@ -496,6 +497,18 @@ __device__ inline bool ref_pair_pred(const MdgParams &P, const MdgBox &B, int ke
 
 // One wave per I cluster: bbox test against every J >= I, then the 64
 // exact atom-pair predicates of each surviving (I, J) as one ballot.
+//
+// Candidates are the clusters of the column rows around I's column: a
+// cluster lies inside its column, so clusters more than reach = rlist / w
+// + 1 columns apart (w the column width) cannot hold a pair within rlist.
+// The columns of one row that are in reach form at most two contiguous
+// cluster ranges (two when they wrap around the box).
+struct ColGrid {
+    const int *colSlot;          // first slot of each column (ncol + 1)
+    const int *cluCol;           // column of each cluster (-1 unused)
+    int ncx, ncy, rx, ry;        // columns per row / rows, reach in each
+};
+
 __global__ void __launch_bounds__(256)
 k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int nclu,
                 const int *__restrict__ slotAtom, const float4 *__restrict__ bb,
@@ -506,21 +519,50 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
                 const int *__restrict__ frozen, const int *__restrict__ exOff,
                 const int *__restrict__ exLst, int kmax, int *__restrict__ tmpCnt,
                 int *__restrict__ tmpJ, unsigned long long *__restrict__ tmpMask,
-                int *__restrict__ ctrl)
+                int *__restrict__ ctrl, ColGrid CG)
 {
     const int lane = threadIdx.x & 63;
     const int I = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (I >= nclu) return;
+    const int colI = CG.cluCol[I];
+    if (colI < 0) {
+        if (lane == 0) tmpCnt[I] = 0;
+        return;
+    }
     const float4 ci = bb[I], hi = bbh[I];
     const float L0 = (float)B.a1, L1 = (float)B.b2, L2 = (float)B.c3;
     const float rl = (float)(P.cutOff + P.delr) + 0.01f;
     const int a = lane >> 3, b = lane & 7;
     const int iat = slotAtom[I * MDG_CL + a];
+    const int cx = colI % CG.ncx, cy = colI / CG.ncx;
+    // rows: all of them once when the reach covers the box, else cy +- ry
+    const bool allRows = 2 * CG.ry + 1 >= CG.ncy, allCols = 2 * CG.rx + 1 >= CG.ncx;
+    const int nrow = allRows ? CG.ncy : 2 * CG.ry + 1;
     int found = 0;
-    for (int J0 = I; J0 < nclu; J0 += 64) {
+    for (int rr = 0; rr < nrow; rr++) {
+        const int row = allRows ? rr : (cy - CG.ry + rr + CG.ncy) % CG.ncy;
+        // the row's columns in reach as up to two index ranges [lo, hi)
+        int rlo[2], rhi[2], nr = 1;
+        if (allCols) {
+            rlo[0] = 0;
+            rhi[0] = CG.ncx;
+        } else {
+            const int lo = cx - CG.rx, hc = cx + CG.rx + 1;
+            if (lo < 0) {
+                rlo[0] = lo + CG.ncx; rhi[0] = CG.ncx; rlo[1] = 0; rhi[1] = hc; nr = 2;
+            } else if (hc > CG.ncx) {
+                rlo[0] = lo; rhi[0] = CG.ncx; rlo[1] = 0; rhi[1] = hc - CG.ncx; nr = 2;
+            } else {
+                rlo[0] = lo; rhi[0] = hc;
+            }
+        }
+        for (int q = 0; q < nr; q++) {
+            const int j0 = max(I, CG.colSlot[row * CG.ncx + rlo[q]] / MDG_CL);
+            const int j1 = CG.colSlot[row * CG.ncx + rhi[q]] / MDG_CL;
+    for (int J0 = j0; J0 < j1; J0 += 64) {
         int J = J0 + lane;
         bool cand = false;
-        if (J < nclu) {
+        if (J < j1) {
             float4 cj = bb[J], hj = bbh[J];
             float dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
             if (B.type == 1 || B.type == 2) {
@@ -551,6 +593,8 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
                 }
                 found++;
             }
+        }
+    }
         }
     }
     if (lane == 0) {
@@ -614,8 +658,9 @@ static int sort_columns(mdg_ctx *ctx, const int *colOf, unsigned long long *keys
                        C.colCnt, C.colPad);
     LAUNCHCHK();
     RETCHK(mdg_scan_int(ctx, C.colPad, C.colSlot, C.ncol));
+    RETCHK(mdg_fill_i32(ctx, C.cluCol, C.nclu, -1));
     hipLaunchKernelGGL(k_col_slots, dim3(C.ncol), dim3(64), 0, ctx->stream, C.ncol, C.colStart,
-                       C.colCnt, C.colSlot, keys, C.slotAtom, C.atomSlot);
+                       C.colCnt, C.colSlot, keys, C.slotAtom, C.atomSlot, C.cluCol);
     LAUNCHCHK();
     return MDG_OK;
 }
@@ -685,6 +730,7 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&C.bbh, maxclu * sizeof(float4)));
         HIPCHK(hipMalloc(&C.cpOff, (maxclu + 1) * sizeof(int)));
         HIPCHK(hipMalloc(&C.tmpCnt, maxclu * sizeof(int)));
+        HIPCHK(hipMalloc(&C.cluCol, maxclu * sizeof(int)));
         // neighbouring clusters within rc+skin plus two cluster diameters
         double rl = ctx->P.cutOff + ctx->P.delr + 9.0;
         double dens = (double)n / ctx->B.vol / MDG_CL;
@@ -724,11 +770,21 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
                            ctx->cellOf, ctx->cellCnt);
         LAUNCHCHK();
     }
+    ColGrid CG;
+    CG.colSlot = C.colSlot;
+    CG.cluCol = C.cluCol;
+    CG.ncx = C.nfx;
+    CG.ncy = C.nfy;
+    {
+        const double rl = ctx->P.cutOff + ctx->P.delr;
+        CG.rx = (int)(rl / (ctx->B.pa / C.nfx)) + 1;
+        CG.ry = (int)(rl / (ctx->B.pb / C.nfy)) + 1;
+    }
     hipLaunchKernelGGL(k_cluster_pairs, dim3(nblk(nclu, 4)), dim3(256), 0, ctx->stream, ctx->P,
                        ctx->B, ctx->keyLink, ctx->nlc[0], ctx->nlc[1], ctx->nlc[2], nclu,
                        C.slotAtom, C.bb, C.bbh, ctx->x, ctx->y, ctx->z, ctx->xu, ctx->yu, ctx->zu,
                        ctx->cellOf, ctx->frozen, ctx->exOff, ctx->exLst, C.kmax, C.tmpCnt, C.tmpJ,
-                       C.tmpMask, ctx->ctrl);
+                       C.tmpMask, ctx->ctrl, CG);
     LAUNCHCHK();
     return compact_cluster_list(ctx);
 }
@@ -899,6 +955,7 @@ int mdg_b_build_list(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&C.bbh, maxclu * sizeof(float4)));
         HIPCHK(hipMalloc(&C.cpOff, (maxclu + 1) * sizeof(int)));
         HIPCHK(hipMalloc(&C.tmpCnt, maxclu * sizeof(int)));
+        HIPCHK(hipMalloc(&C.cluCol, maxclu * sizeof(int)));
         HIPCHK(hipMalloc(&T.cluStart, (R + 1) * sizeof(int)));
         HIPCHK(hipMalloc(&T.crep, maxclu * sizeof(int)));
         // every cluster of the largest replica, padding included

@@ -261,7 +261,7 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
     }
     RETCHK(mdg_pair_cluster(ctx));
-    if (ctx->stream2) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
+    if (ctx->spmeForked) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     const bool mixed = ctx->precision == MDG_MIXED;
     FinArgs F;
     F.P = ctx->P;
@@ -824,7 +824,7 @@ static int b_energy(mdg_ctx *ctx, bool rebuild)
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
     }
     RETCHK(mdg_b_pair(ctx));
-    if (ctx->stream2) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
+    if (ctx->spmeForked) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     BFinArgs F;
     F.P = ctx->P;
     F.n = n;

@@ -909,10 +909,12 @@ int mdg_b_spme(mdg_ctx *ctx)
 
 // Runs fn (an SPME evaluation) on the context's second stream, forked from
 // and joined back into the main stream by events; rocFFT follows the
-// stream.  Without a second stream fn runs in place.
+// stream.  Without a second stream, or while profiling, fn runs in place.
 int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *))
 {
-    if (!ctx->stream2) return fn(ctx);
+    // profiled steps keep one stream, so each kernel's events time it alone
+    ctx->spmeForked = ctx->stream2 && !ctx->profiling;
+    if (!ctx->spmeForked) return fn(ctx);
     hipStream_t main = ctx->stream;
     HIPCHK(hipEventRecord(ctx->evFork, main));
     HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->evFork, 0));
