@@ -430,12 +430,13 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->cl.colCnt, ctx->cl.colStart, ctx->cl.colPad, ctx->cl.colSlot,
                     ctx->cl.sortTmp, ctx->cl.sortKey, ctx->cl.cluCol, ctx->sx, ctx->sy, ctx->sz, ctx->sp32,
                     ctx->sq, ctx->seps, ctx->ssig, ctx->fdx, ctx->fdy, ctx->fdz, ctx->scal,
-                    ctx->fix, ctx->fiy, ctx->fiz, ctx->frx, ctx->fry, ctx->frz, ctx->frx64,
+                    ctx->fix, ctx->frx, ctx->fry, ctx->frz, ctx->frx64,
                     ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,
                     ctx->grid64, ctx->cgrid64, ctx->grid32, ctx->cgrid32, ctx->gridq,
                     ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
                     ctx->shkMask, ctx->exSymOff, ctx->exSymLst, ctx->adjQ, ctx->adjE,
-                    ctx->adjS, ctx->adjPot};
+                    ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binStart, ctx->binOf,
+                    ctx->binRank, ctx->binAtoms, ctx->binOut, ctx->binNOut};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->bt) {

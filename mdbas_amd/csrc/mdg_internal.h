@@ -210,6 +210,13 @@ struct mdg_ctx {
     float *grid32 = nullptr;
     float *cgrid32 = nullptr;
     long long *gridq = nullptr;  // fixed-point spread mesh (mixed)
+    // binned spreading (mixed): atoms sorted by mesh bin every step
+    int nbx = 0, nby = 0, nbz = 0;   // bins per dimension (0: not used)
+    int *binCnt = nullptr;       // per bin; zero between steps
+    int *binStart = nullptr;     // exclusive scan of binCnt (nbins + 1)
+    int *binOf = nullptr, *binRank = nullptr, *binAtoms = nullptr;   // per atom
+    int *binOut = nullptr;       // atoms whose mesh points are not one run
+    int *binNOut = nullptr;      // [0] being counted, [1] this step's count
     rocfft_plan fwd = nullptr, bwd = nullptr, fwd32 = nullptr, bwd32 = nullptr;
     rocfft_execution_info info = nullptr;
     void *fftWork = nullptr;
@@ -365,6 +372,15 @@ __device__ inline void mdg_atomic_add_fix(long long *p, double v, double scale)
     if (iv != 0) atomicAdd((unsigned long long *)p, (unsigned long long)iv);
 }
 
+// Wave-uniform load through the scalar data cache (constant address
+// space): i must be the same in every lane of the wavefront, and the data
+// must not be written by the running kernel.
+template <class T>
+__device__ inline T mdg_uload(const T *p, long i)
+{
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
 // Wave-level reductions (64 lanes).
 __device__ inline double mdg_wave_sum(double v)
 {
@@ -429,6 +445,7 @@ int mdg_spme_free(mdg_ctx *ctx);
 int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8]);
 int mdg_spme_device(mdg_ctx *ctx);   // device-resident: frx.. per atom, partials in epart
 int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *));   // runs fn's launches on stream2
+int mdg_gather_blocks(const mdg_ctx *ctx);   // block partials of the reciprocal forces
 int mdg_spme_pot(mdg_ctx *ctx, double *pot);   // FP64 dE_rec/dq per atom
 int mdg_adj_pair(mdg_ctx *ctx);      // real-space dE/d(q, eps, sig) into fdx/fdy/fdz slots
 

@@ -532,7 +532,7 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
     const float4 ci = bb[I], hi = bbh[I];
     const float L0 = (float)B.a1, L1 = (float)B.b2, L2 = (float)B.c3;
     const float rl = (float)(P.cutOff + P.delr) + 0.01f;
-    const int a = lane >> 3, b = lane & 7;
+    const int a = lane & 7, b = lane >> 3;   // i, j atom: mask bit = lane = 8 j + i
     const int iat = slotAtom[I * MDG_CL + a];
     const int cx = colI % CG.ncx, cy = colI / CG.ncx;
     // rows: all of them once when the reach covers the box, else cy +- ry
@@ -890,7 +890,7 @@ k_b_cluster_pairs(MdgParams P, const MdgBox *__restrict__ boxes, const int *__re
     const float4 ci = bb[I], hi = bbh[I];
     const float L0 = (float)B.a1, L1 = (float)B.b2, L2 = (float)B.c3;
     const float rl = (float)(P.cutOff + P.delr) + 0.01f;
-    const int a = lane >> 3, b = lane & 7;
+    const int a = lane & 7, b = lane >> 3;   // i, j atom: mask bit = lane = 8 j + i
     const int iat = slotAtom[I * MDG_CL + a];
     int found = 0;
     for (int J0 = I; J0 < jend; J0 += 64) {

@@ -4,14 +4,14 @@
 //    energy.c:216-345 for the cutoff Coulomb functors) over a list in the
 //    reference format, FP64, one thread per i in neighOrder order.
 //  * k_excl64: the excluded-pair erf correction (energy.c:567-621).
-//  * k_pair_cluster<MIXED>: the same physics over the cluster-pair list,
-//    one 64-lane wavefront per chunk (up to 32 j-clusters of one
+//  * k_pair_fp64 / k_pair_mixed: the same physics over the cluster-pair
+//    list, one 64-lane wavefront per chunk (up to 32 j-clusters of one
 //    i-cluster), lane (a, b) = pair (i_a, j_b) of each 8x8 cluster pair.
 //    i forces stay in registers across the chunk and are reduced once over
-//    b; j forces are reduced over a with DPP moves after each cluster pair
-//    and added by one lane per j atom.  MIXED = FP32 pair
-//    arithmetic (displacements formed in FP64 first) with int64
-//    fixed-point force accumulation; otherwise FP64 throughout.
+//    b; j forces are reduced over a with DPP moves after each cluster pair.
+//    k_pair_mixed: FP32 pair arithmetic (displacements formed in FP64
+//    first) with int64 fixed-point force accumulation; k_pair_fp64: FP64
+//    throughout.
 #include "mdg_internal.h"
 
 // ewald.c:356-369
@@ -225,24 +225,30 @@ struct PairF {
     double cutOff2d, L0, L1, L2, iL0, iL1, iL2;
 };
 
-// erfc(x) for x >= 0 by the Chebyshev-fitted form of Numerical Recipes
-// (Press et al., 2nd ed., sec. 6.2, "erfcc"): fractional error < 1.2e-7
-// over the whole range.  Returns erfc; *ez receives exp(-x^2).
-__device__ inline float mdg_erfc_f32(float x, float *ez)
+// erfc(x) for x >= 0 as t Q(t) exp(-x^2), t = 1 / (1 + x / 2).  The
+// substitution is that of the Numerical Recipes "erfcc" form (Press et al.,
+// 2nd ed., sec. 6.2), which writes erfc as t exp(P(t) - x^2); here
+// Q(t) = erfcx(x) / t itself is the polynomial, so one exponential serves
+// both erfc and the force term exp(-x^2) (returned in *ez).  Q has degree 9,
+// fitted by mdbas_amd/erfc_fit.py (weighted least squares for the relative
+// error over t in [0, 1], i.e. every x >= 0): relative error of the fit
+// 1.6e-7; in FP32 about 2e-6 at x = 6, from the rounding of x^2 inside the
+// exponential.
+__device__ inline float mdg_erfc_q(float x, float *ez)
 {
-    float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
-    float p = fmaf(t, 0.17087277f, -0.82215223f);
-    p = fmaf(t, p, 1.48851587f);
-    p = fmaf(t, p, -1.13520398f);
-    p = fmaf(t, p, 0.27886807f);
-    p = fmaf(t, p, -0.18628806f);
-    p = fmaf(t, p, 0.09678418f);
-    p = fmaf(t, p, 0.37409196f);
-    p = fmaf(t, p, 1.00002368f);
-    p = fmaf(t, p, -1.26551223f);
-    float x2 = x * x;
-    *ez = __expf(-x2);
-    return t * __expf(p - x2);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+    float p = fmaf(t, -0.02552288340259605f, 0.06446153649429703f);
+    p = fmaf(t, p, 0.030753194918781607f);
+    p = fmaf(t, p, -0.1985488152101896f);
+    p = fmaf(t, p, 0.07946913878476664f);
+    p = fmaf(t, p, 0.05826160866113146f);
+    p = fmaf(t, p, 0.18041489004864147f);
+    p = fmaf(t, p, 0.24651220482179143f);
+    p = fmaf(t, p, 0.28210421927957413f);
+    p = fmaf(t, p, 0.28209474704728354f);
+    const float e = __expf(-(x * x));
+    *ez = e;
+    return t * p * e;
 }
 
 struct ClusterArgs {
@@ -265,6 +271,7 @@ struct ClusterArgs {
     const MdgBox *boxes;             // batch: box of each replica
     const int *crep;                 // batch: replica of each cluster
     double *chE;                     // batch: per chunk eDir, eVdw, virDir, virVdw, nIn
+    long fstr;                       // fiy - fix and fiz - fiy (one allocation)
 };
 
 // Sum over the 8 lanes of each aligned group of 8 with DPP moves (plain
@@ -302,13 +309,14 @@ __device__ inline double dpp_sum8(double v)
 // Lane mapping: a = lane & 7 is the i atom, b = lane >> 3 the j atom, so
 // each j-cluster pair's j-force sums run over 8 adjacent lanes (DPP) and
 // the i-force sums over lanes a, a+8, ..., a+56 happen once per chunk.
-// Mask bit a*8+b marks the listed pair (I_a, J_b).
-template <bool MIXED, bool BATCH>
-__global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
+// Mask bit 8 b + a (= the lane) marks the listed pair (I_a, J_b).  FP64
+// throughout.
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_pair_fp64(ClusterArgs A)
 {
     const int lane = threadIdx.x & 63;
     const int a = lane & 7, b = lane >> 3;
-    const int bit = (a << 3) | b;
+    const int bit = lane;
     double e_acc[5] = {0, 0, 0, 0, 0};
     const int nch = min(A.chOff[A.nclu], A.maxChunks);
     for (int W = blockIdx.x * 4 + (threadIdx.x >> 6); W < nch; W += gridDim.x * 4) {
@@ -321,163 +329,57 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
         }
         // the box: the run's, or the replica's in a batch
         const MdgBox &Bx = BATCH ? A.boxes[A.crep[I]] : A.B;
-        double L0 = A.F.L0, L1 = A.F.L1, L2 = A.F.L2, iL0 = A.F.iL0, iL1 = A.F.iL1, iL2 = A.F.iL2;
-        if (BATCH) {
-            L0 = Bx.a1;
-            L1 = Bx.type == 1 ? Bx.a1 : Bx.b2;
-            L2 = Bx.type == 1 ? Bx.a1 : Bx.c3;
-            iL0 = 1.0 / L0;
-            iL1 = 1.0 / L1;
-            iL2 = 1.0 / L2;
-        }
         const int si = I * MDG_CL + a;
         const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
-        float qi, ei, gi;
-        double qid, eid, gid;
-        if (MIXED) {
-            float4 p = A.sp32[si];
-            qi = p.x; ei = p.y; gi = p.z;
-        } else {
-            qid = A.sq[si]; eid = A.seps[si]; gid = A.ssig[si];
-        }
-        float fxi = 0.f, fyi = 0.f, fzi = 0.f;
+        const double qid = A.sq[si], eid = A.seps[si], gid = A.ssig[si];
         double fxid = 0., fyid = 0., fzid = 0.;
         double ed = 0., ev = 0., vd = 0., vv = 0.;
         int nin = 0;
-        // software pipeline: the next j-cluster's data is in flight while
-        // the current one is computed
-        int Jn = A.cpJ[e0];
-        unsigned long long mn = A.cpMask[e0];
-        double xn = A.sx[Jn * MDG_CL + b], yn = A.sy[Jn * MDG_CL + b], zn = A.sz[Jn * MDG_CL + b];
-        float4 pn = MIXED ? A.sp32[Jn * MDG_CL + b] : make_float4(0.f, 0.f, 0.f, 0.f);
         for (int e = e0; e < e1; e++) {
-            const int J = Jn;
-            const unsigned long long m = mn;
-            const double xj = xn, yj = yn, zj = zn;
-            const float4 pjc = pn;
-            if (e + 1 < e1) {
-                Jn = A.cpJ[e + 1];
-                mn = A.cpMask[e + 1];
-                const int sn = Jn * MDG_CL + b;
-                xn = A.sx[sn];
-                yn = A.sy[sn];
-                zn = A.sz[sn];
-                if (MIXED) pn = A.sp32[sn];
-            }
+            const int J = A.cpJ[e];
+            const unsigned long long m = A.cpMask[e];
             const int sj = J * MDG_CL + b;
-            double dx = xj - xi, dy = yj - yi, dz = zj - zi;
-            double r2;
-            if (MIXED) {
-                dx = fma(-L0, rint(dx * iL0), dx);
-                dy = fma(-L1, rint(dy * iL1), dy);
-                dz = fma(-L2, rint(dz * iL2), dz);
-                r2 = dx * dx + dy * dy + dz * dz;
-            } else {
-                r2 = mdg_dist(Bx, dx, dy, dz);
+            double dx = A.sx[sj] - xi, dy = A.sy[sj] - yi, dz = A.sz[sj] - zi;
+            const double r2 = mdg_dist(Bx, dx, dy, dz);
+            double fjx = 0., fjy = 0., fjz = 0.;
+            if (((m >> bit) & 1ull) && r2 <= A.P.cutOff2) {
+                const double r = sqrt(r2), rt = 1. / r;
+                double dEl, dVw;
+                const double el = ewald_dir64(A.P.alpha, A.P.chargeConst * qid * A.sq[sj], r, rt,
+                                              dEl);
+                const double evw = vdw_switch64(A.P, eid * A.seps[sj], gid + A.ssig[sj], r2, rt,
+                                                dVw);
+                const double fs = dEl + dVw;
+                fjx = fs * dx * rt;
+                fjy = fs * dy * rt;
+                fjz = fs * dz * rt;
+                ed += el;
+                ev += evw;
+                vd += dEl * r;
+                vv += dVw * r;
+                nin++;
             }
-            const bool in = ((m >> bit) & 1ull) && r2 <= A.P.cutOff2;
-            if (MIXED) {
-                float fjx = 0.f, fjy = 0.f, fjz = 0.f;
-                if (in) {
-                    const float r2f = (float)r2;
-                    const float rinv = __builtin_amdgcn_rsqf(r2f);
-                    const float r = r2f * rinv;
-                    const float qq = A.F.qc * qi * pjc.x;
-                    float ez;
-                    const float erfcv = mdg_erfc_f32(A.F.alpha * r, &ez);
-                    const float eel = qq * erfcv * rinv;
-                    const float del = -(eel + qq * A.F.c2a * ez) * rinv;
-                    const float veps = ei * pjc.y;
-                    const float s = (gi + pjc.z) * rinv;
-                    const float s2 = s * s;
-                    const float s6 = s2 * s2 * s2;
-                    const float s12 = s6 * s6;
-                    const float pv = 4.f * veps * (s12 - s6);
-                    const float dpv = 24.f * veps * (s6 - 2.f * s12);
-                    float S = 1.f, dS = 0.f;
-                    if (r2f > A.F.cutOn2) {
-                        const float sw1 = A.F.cutOff2 - r2f;
-                        S = sw1 * sw1 * (A.F.cutOff2 + 2.f * r2f - A.F.threeOn2) * A.F.switch2;
-                        dS = 12.f * r2f * sw1 * (A.F.cutOn2 - r2f) * A.F.switch2;
-                    }
-                    const float evdw = pv * S;
-                    const float dvdw = (pv * dS + dpv * S) * rinv;
-                    const float fs = (del + dvdw) * rinv;
-                    fjx = fs * (float)dx;
-                    fjy = fs * (float)dy;
-                    fjz = fs * (float)dz;
-                    ed += eel;
-                    ev += evdw;
-                    vd += del * r;
-                    vv += dvdw * r;
-                    nin++;
-                }
-                fxi += fjx;
-                fyi += fjy;
-                fzi += fjz;
-                fjx = dpp_sum8(fjx);
-                fjy = dpp_sum8(fjy);
-                fjz = dpp_sum8(fjz);
-                if (a == 0) {
-                    mdg_atomic_add_fix(&A.fix[sj], -(double)fjx, MDG_FSCALE);
-                    mdg_atomic_add_fix(&A.fiy[sj], -(double)fjy, MDG_FSCALE);
-                    mdg_atomic_add_fix(&A.fiz[sj], -(double)fjz, MDG_FSCALE);
-                }
-            } else {
-                double fjx = 0., fjy = 0., fjz = 0.;
-                if (in) {
-                    const double r = sqrt(r2), rt = 1. / r;
-                    double dEl, dVw;
-                    const double el = ewald_dir64(A.P.alpha, A.P.chargeConst * qid * A.sq[sj], r,
-                                                  rt, dEl);
-                    const double evw = vdw_switch64(A.P, eid * A.seps[sj], gid + A.ssig[sj], r2,
-                                                    rt, dVw);
-                    const double fs = dEl + dVw;
-                    fjx = fs * dx * rt;
-                    fjy = fs * dy * rt;
-                    fjz = fs * dz * rt;
-                    ed += el;
-                    ev += evw;
-                    vd += dEl * r;
-                    vv += dVw * r;
-                    nin++;
-                }
-                fxid += fjx;
-                fyid += fjy;
-                fzid += fjz;
-                fjx = dpp_sum8(fjx);
-                fjy = dpp_sum8(fjy);
-                fjz = dpp_sum8(fjz);
-                if (a == 0) {
-                    if (fjx != 0.) atomicAdd(&A.fdx[sj], -fjx);
-                    if (fjy != 0.) atomicAdd(&A.fdy[sj], -fjy);
-                    if (fjz != 0.) atomicAdd(&A.fdz[sj], -fjz);
-                }
+            fxid += fjx;
+            fyid += fjy;
+            fzid += fjz;
+            fjx = dpp_sum8(fjx);
+            fjy = dpp_sum8(fjy);
+            fjz = dpp_sum8(fjz);
+            if (a == 0) {
+                if (fjx != 0.) atomicAdd(&A.fdx[sj], -fjx);
+                if (fjy != 0.) atomicAdd(&A.fdy[sj], -fjy);
+                if (fjz != 0.) atomicAdd(&A.fdz[sj], -fjz);
             }
         }
-        if (MIXED) {
-            double gx = fxi, gy = fyi, gz = fzi;
-            for (int o = 8; o < 64; o <<= 1) {
-                gx += __shfl_xor(gx, o, 64);
-                gy += __shfl_xor(gy, o, 64);
-                gz += __shfl_xor(gz, o, 64);
-            }
-            if (b == 0) {
-                mdg_atomic_add_fix(&A.fix[si], gx, MDG_FSCALE);
-                mdg_atomic_add_fix(&A.fiy[si], gy, MDG_FSCALE);
-                mdg_atomic_add_fix(&A.fiz[si], gz, MDG_FSCALE);
-            }
-        } else {
-            for (int o = 8; o < 64; o <<= 1) {
-                fxid += __shfl_xor(fxid, o, 64);
-                fyid += __shfl_xor(fyid, o, 64);
-                fzid += __shfl_xor(fzid, o, 64);
-            }
-            if (b == 0) {
-                atomicAdd(&A.fdx[si], fxid);
-                atomicAdd(&A.fdy[si], fyid);
-                atomicAdd(&A.fdz[si], fzid);
-            }
+        for (int o = 8; o < 64; o <<= 1) {
+            fxid += __shfl_xor(fxid, o, 64);
+            fyid += __shfl_xor(fyid, o, 64);
+            fzid += __shfl_xor(fzid, o, 64);
+        }
+        if (b == 0) {
+            atomicAdd(&A.fdx[si], fxid);
+            atomicAdd(&A.fdy[si], fyid);
+            atomicAdd(&A.fdz[si], fzid);
         }
         if (BATCH) {
             // energies per chunk, summed per replica later (a replica's
@@ -491,6 +393,206 @@ __global__ void __launch_bounds__(256) k_pair_cluster(ClusterArgs A)
             e_acc[2] += vd;
             e_acc[3] += vv;
             e_acc[4] += nin;
+        }
+    }
+    if (!BATCH) mdg_block_partials<5>(e_acc, A.part);
+}
+
+// Mixed precision over the same list and lane mapping: displacements and
+// the cutoff test in FP64, pair arithmetic in FP32, energies summed in
+// FP64 (each FP32 term, before any rounding of a partial sum: the total
+// is a sum of ~1e7 terms of +-1e4 with heavy cancellation), virials in
+// FP32 over a chunk and FP64 across chunks, forces added as int64 fixed
+// point (deterministic).
+//
+// The loop runs on wave-uniform scalars: chunk, entry, j cluster and mask
+// come through the scalar cache, and the j slots are read with buffer loads
+// whose offset is J * (slot bytes * 8) in a scalar register plus a fixed
+// per-lane offset.  Two register sets alternate so the next entry's j data
+// is in flight while the current one is computed.  There is no branch per
+// pair: the list mask and the cutoff test combine as wave masks (the mask
+// bit of a pair is its lane), and lanes outside either evaluate a harmless
+// distance with zero coefficients.  The j-force sums of a cluster pair go
+// out as one atomic per component: lanes a = 0, 1, 2 of each j group add
+// x, y, z.
+struct MxJ {
+    double x, y, z;
+    float4 p;
+};
+
+struct MxSlots {
+    __amdgpu_buffer_rsrc_t x, y, z, p;
+    int vo8, vo16;                   // per-lane byte offsets of slot b
+};
+
+__device__ __forceinline__ MxJ mx_load(const MxSlots &R, int J)
+{
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    MxJ o;
+    o.x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(R.x, R.vo8, J * 64, 0));
+    o.y = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(R.y, R.vo8, J * 64, 0));
+    o.z = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(R.z, R.vo8, J * 64, 0));
+    const i4v v = __builtin_amdgcn_raw_buffer_load_b128(R.p, R.vo16, J * 128, 0);
+    o.p = make_float4(__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z),
+                      __int_as_float(v.w));
+    return o;
+}
+
+struct MxAcc {
+    float fx, fy, fz;                // i-atom force
+    double ed, ev;                   // energies, summed per pair in FP64
+    float vd, vv;                    // virials: FP32 over the chunk (at most
+                                     // MDG_CHUNK terms per lane), FP64 across chunks
+    int nin;                         // in-cutoff pairs (wave-uniform)
+};
+
+// floor(v * 2^32) as int64 for |v| < 2^31: the integer part and the
+// fraction are split in FP64, where both steps are exact for an FP32 v.
+__device__ inline long long mdg_f32_fix(float v)
+{
+    const double d = (double)v;
+    const double fl = floor(d);
+    const unsigned int lo = (unsigned int)((d - fl) * 4294967296.0);
+    return (long long)(((unsigned long long)(unsigned int)(int)fl << 32) | lo);
+}
+
+template <bool BATCH>
+__device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsigned long long m,
+                                        int J, double xi, double yi, double zi, float qi, float ei,
+                                        float gi, double L0, double L1, double L2, double iL0,
+                                        double iL1, double iL2, int a, int b, MxAcc &acc)
+{
+    double dx = j.x - xi, dy = j.y - yi, dz = j.z - zi;
+    dx = fma(-L0, rint(dx * iL0), dx);
+    dy = fma(-L1, rint(dy * iL1), dy);
+    dz = fma(-L2, rint(dz * iL2), dz);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const unsigned long long inm = m & __builtin_amdgcn_ballot_w64(r2 <= A.F.cutOff2d);
+    const bool in = __builtin_amdgcn_inverse_ballot_w64(inm);
+    acc.nin += __builtin_popcountll(inm);
+    const float r2f = in ? (float)r2 : A.F.cutOff2;
+    const float qq = in ? qi * j.p.x : 0.f;
+    const float veps = in ? ei * j.p.y : 0.f;
+    const float rinv = __builtin_amdgcn_rsqf(r2f);
+    const float r = r2f * rinv;
+    float ez;
+    const float erfcv = mdg_erfc_q(A.F.alpha * r, &ez);
+    const float eel = qq * erfcv * rinv;
+    const float del = -(eel + qq * A.F.c2a * ez) * rinv;
+    const float s = (gi + j.p.z) * rinv;
+    const float s2 = s * s;
+    const float s6 = s2 * s2 * s2;
+    const float s12 = s6 * s6;
+    const float pv = 4.f * veps * (s12 - s6);
+    const float dpv = 24.f * veps * (s6 - 2.f * s12);
+    // switch of vdw.c:144-184 with r2 clamped from below at cutOn2, where
+    // dS vanishes exactly; S is 1 below cutOn2
+    const bool sw = r2f > A.F.cutOn2;
+    const float u = sw ? r2f : A.F.cutOn2;
+    const float sw1 = A.F.cutOff2 - u;
+    const float Su = sw1 * sw1 * (A.F.cutOff2 + 2.f * u - A.F.threeOn2) * A.F.switch2;
+    const float S = sw ? Su : 1.f;
+    const float dS = 12.f * u * sw1 * (A.F.cutOn2 - u) * A.F.switch2;
+    const float evdw = pv * S;
+    const float dvdw = (pv * dS + dpv * S) * rinv;
+    const float fs = (del + dvdw) * rinv;
+    const float fjx = fs * (float)dx, fjy = fs * (float)dy, fjz = fs * (float)dz;
+    acc.ed += eel;
+    acc.ev += evdw;
+    acc.vd += del * r;
+    acc.vv += dvdw * r;
+    acc.fx += fjx;
+    acc.fy += fjy;
+    acc.fz += fjz;
+    const float gx = dpp_sum8(fjx), gy = dpp_sum8(fjy), gz = dpp_sum8(fjz);
+    const float gc = a == 0 ? gx : (a == 1 ? gy : gz);
+    if (a < 3) {
+        const long long iv = mdg_f32_fix(-gc);
+        if (iv != 0)
+            atomicAdd((unsigned long long *)(A.fix + a * A.fstr + J * MDG_CL + b),
+                      (unsigned long long)iv);
+    }
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
+{
+    const int lane = threadIdx.x & 63;
+    const int a = lane & 7, b = lane >> 3;
+    const long nsb = (long)A.nclu * MDG_CL;
+    MxSlots R;
+    R.x = __builtin_amdgcn_make_buffer_rsrc((void *)A.sx, 0, (int)(nsb * 8), 0x00020000);
+    R.y = __builtin_amdgcn_make_buffer_rsrc((void *)A.sy, 0, (int)(nsb * 8), 0x00020000);
+    R.z = __builtin_amdgcn_make_buffer_rsrc((void *)A.sz, 0, (int)(nsb * 8), 0x00020000);
+    R.p = __builtin_amdgcn_make_buffer_rsrc((void *)A.sp32, 0, (int)(nsb * 16), 0x00020000);
+    R.vo8 = b * 8;
+    R.vo16 = b * 16;
+    double e_acc[5] = {0, 0, 0, 0, 0};
+    const int nch = min(mdg_uload(A.chOff, A.nclu), A.maxChunks);
+    const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    for (int W = w0; W < nch; W += gridDim.x * 4) {
+        const long long ch = mdg_uload((const long long *)A.chunk, W);   // int2 (I, e0)
+        const int I = (int)(ch & 0xffffffffll), e0 = (int)(ch >> 32);
+        const int e1 = min(min(e0 + MDG_CHUNK, mdg_uload(A.cpOff, I + 1)), A.cap);
+        if (e0 >= e1) {   // only after a flagged list overflow
+            if (BATCH && lane < 5) A.chE[(long)W * 5 + lane] = 0.;
+            continue;
+        }
+        double L0 = A.F.L0, L1 = A.F.L1, L2 = A.F.L2, iL0 = A.F.iL0, iL1 = A.F.iL1, iL2 = A.F.iL2;
+        if (BATCH) {
+            const MdgBox &Bx = A.boxes[mdg_uload(A.crep, I)];
+            L0 = Bx.a1;
+            L1 = Bx.type == 1 ? Bx.a1 : Bx.b2;
+            L2 = Bx.type == 1 ? Bx.a1 : Bx.c3;
+            iL0 = 1.0 / L0;
+            iL1 = 1.0 / L1;
+            iL2 = 1.0 / L2;
+        }
+        const int si = I * MDG_CL + a;
+        const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
+        const float4 pi = A.sp32[si];
+        const float qi = A.F.qc * pi.x;
+        MxAcc acc = {0.f, 0.f, 0.f, 0., 0., 0.f, 0.f, 0};
+        // entries e (set 0) and e + 1 (set 1) alternate; indices past the
+        // end are clamped to the last entry (loaded, never computed)
+        int J0 = mdg_uload(A.cpJ, e0);
+        unsigned long long m0 = mdg_uload(A.cpMask, e0);
+        MxJ j0 = mx_load(R, J0);
+        for (int e = e0;;) {
+            const int J1 = mdg_uload(A.cpJ, min(e + 1, e1 - 1));
+            const unsigned long long m1 = mdg_uload(A.cpMask, min(e + 1, e1 - 1));
+            const MxJ j1 = mx_load(R, J1);
+            mx_pair<BATCH>(A, j0, m0, J0, xi, yi, zi, qi, pi.y, pi.z, L0, L1, L2, iL0, iL1, iL2, a,
+                           b, acc);
+            if (++e >= e1) break;
+            J0 = mdg_uload(A.cpJ, min(e + 1, e1 - 1));
+            m0 = mdg_uload(A.cpMask, min(e + 1, e1 - 1));
+            j0 = mx_load(R, J0);
+            mx_pair<BATCH>(A, j1, m1, J1, xi, yi, zi, qi, pi.y, pi.z, L0, L1, L2, iL0, iL1, iL2, a,
+                           b, acc);
+            if (++e >= e1) break;
+        }
+        // i forces: sums over the 8 j lanes of each i atom; lanes b = 0, 1,
+        // 2 of the group add x, y, z
+        double gx = acc.fx, gy = acc.fy, gz = acc.fz;
+        for (int o = 8; o < 64; o <<= 1) {
+            gx += __shfl_xor(gx, o, 64);
+            gy += __shfl_xor(gy, o, 64);
+            gz += __shfl_xor(gz, o, 64);
+        }
+        if (b < 3) mdg_atomic_add_fix(A.fix + b * A.fstr + si, b == 0 ? gx : (b == 1 ? gy : gz),
+                                      MDG_FSCALE);
+        if (BATCH) {
+            const double c[5] = {mdg_wave_sum(acc.ed), mdg_wave_sum(acc.ev),
+                                 mdg_wave_sum((double)acc.vd), mdg_wave_sum((double)acc.vv),
+                                 (double)acc.nin};
+            if (lane < 5) A.chE[(long)W * 5 + lane] = c[lane];
+        } else {
+            e_acc[0] += acc.ed;
+            e_acc[1] += acc.ev;
+            e_acc[2] += acc.vd;
+            e_acc[3] += acc.vv;
+            e_acc[4] += lane == 0 ? (double)acc.nin : 0.;
         }
     }
     if (!BATCH) mdg_block_partials<5>(e_acc, A.part);
@@ -588,6 +690,7 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     A.fix = ctx->fix;
     A.fiy = ctx->fiy;
     A.fiz = ctx->fiz;
+    A.fstr = ctx->fiy - ctx->fix;
     A.fdx = ctx->fdx;
     A.fdy = ctx->fdy;
     A.fdz = ctx->fdz;
@@ -615,9 +718,9 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     if (ctx->precision == MDG_MIXED)
-        hipLaunchKernelGGL((k_pair_cluster<true, false>), dim3(nb), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL((k_pair_mixed<false>), dim3(nb), dim3(256), 0, ctx->stream, A);
     else
-        hipLaunchKernelGGL((k_pair_cluster<false, false>), dim3(nb), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL((k_pair_fp64<false>), dim3(nb), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
     return MDG_OK;
@@ -630,13 +733,13 @@ int mdg_pair_cluster(mdg_ctx *ctx)
 //   dE/deps_i += eps_j 4 (s^12 - s^6) S(r)                (and i <-> j)
 //   dE/dsig_i += 4 eps_i eps_j (12 s^11 - 6 s^5) S(r) / r (both atoms)
 // with s = (sig_i + sig_j) / r and S the switch of vdw.c:144-184 (1 below
-// cutOn).  Same wavefront layout as k_pair_cluster; slot sums go to
+// cutOn).  Same wavefront layout as k_pair_fp64; slot sums go to
 // fdx (q), fdy (eps), fdz (sig).
 __global__ void __launch_bounds__(256) k_adj_cluster(ClusterArgs A)
 {
     const int lane = threadIdx.x & 63;
     const int a = lane & 7, b = lane >> 3;
-    const int bit = (a << 3) | b;
+    const int bit = lane;
     const int nch = min(A.chOff[A.nclu], A.maxChunks);
     for (int W = blockIdx.x * 4 + (threadIdx.x >> 6); W < nch; W += gridDim.x * 4) {
         const int2 ch = A.chunk[W];
@@ -730,9 +833,9 @@ int mdg_b_pair(mdg_ctx *ctx)
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     if (mixed)
-        hipLaunchKernelGGL((k_pair_cluster<true, true>), dim3(nb), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL((k_pair_mixed<true>), dim3(nb), dim3(256), 0, ctx->stream, A);
     else
-        hipLaunchKernelGGL((k_pair_cluster<false, true>), dim3(nb), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL((k_pair_fp64<true>), dim3(nb), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
     return MDG_OK;

@@ -41,10 +41,16 @@ template <bool MIXED>
 __global__ void __launch_bounds__(256) k_finalize(FinArgs A)
 {
     __shared__ double mean[3];
-    if (threadIdx.x < 3) {
-        double s = 0.;
-        for (int b = 0; b < A.nbGather; b++) s += A.gpart[(long)b * 3 + threadIdx.x];
-        mean[threadIdx.x] = s / (double)A.n;
+    if (threadIdx.x < 64) {
+        // the first wave sums the gather's block partials: lane-strided,
+        // then across the wave (a fixed order)
+        double s[3] = {0., 0., 0.};
+        for (int b = threadIdx.x; b < A.nbGather; b += 64)
+            for (int c = 0; c < 3; c++) s[c] += A.gpart[(long)b * 3 + c];
+        for (int c = 0; c < 3; c++) {
+            const double v = mdg_wave_sum(s[c]);
+            if (threadIdx.x == 0) mean[c] = v / (double)A.n;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) A.mask[threadIdx.x] = ~0ull;
     __syncthreads();
@@ -207,9 +213,11 @@ static int ensure_slots(mdg_ctx *ctx)
     HIPCHK(hipMalloc(&ctx->fdx, ns * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->fdy, ns * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->fdz, ns * sizeof(double)));
-    HIPCHK(hipMalloc(&ctx->fix, ns * sizeof(long long)));
-    HIPCHK(hipMalloc(&ctx->fiy, ns * sizeof(long long)));
-    HIPCHK(hipMalloc(&ctx->fiz, ns * sizeof(long long)));
+    // x, y, z fixed-point accumulators in one allocation (the mixed pair
+    // kernel addresses a component as fix + c * ns)
+    HIPCHK(hipMalloc(&ctx->fix, 3 * (size_t)ns * sizeof(long long)));
+    ctx->fiy = ctx->fix + ns;
+    ctx->fiz = ctx->fix + 2 * (size_t)ns;
     return MDG_OK;
 }
 
@@ -268,7 +276,7 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
     F.B = ctx->B;
     F.n = n;
     F.hasFrozen = ctx->hasFrozen ? 1 : 0;
-    F.nbGather = nblk(n, 256);
+    F.nbGather = mdg_gather_blocks(ctx);
     F.atomSlot = ctx->cl.atomSlot;
     F.fix = ctx->fix;
     F.fiy = ctx->fiy;
@@ -637,7 +645,7 @@ extern "C" int mdg_list_pairs(mdg_ctx *ctx, int *pi, int *pj, long cap, long *np
     for (int e = 0; e < ncp; e++)
         for (int bit = 0; bit < 64; bit++)
             if ((m[e] >> bit) & 1ull) {
-                int a = slot[I[e] * MDG_CL + (bit >> 3)], b = slot[J[e] * MDG_CL + (bit & 7)];
+                int a = slot[I[e] * MDG_CL + (bit & 7)], b = slot[J[e] * MDG_CL + (bit >> 3)];
                 if (k < cap && pi && pj) {
                     pi[k] = std::min(a, b);
                     pj[k] = std::max(a, b);

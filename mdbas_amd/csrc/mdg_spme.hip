@@ -49,6 +49,34 @@ __device__ inline void bspline_dev(double s, int p, T *bsp, T *bsd)
     }
 }
 
+// bspline_dev for a compile-time order: the same operations in the same
+// order, with both loops unrolled so the arrays stay in registers.
+template <typename T, int P>
+__device__ inline void bspline_fix(double s, T (&bsp)[P], T (&bsd)[P])
+{
+#pragma clang fp contract(off)
+    const int is = (int)s;
+    const T u = (T)(s - (double)is);
+    bsd[0] = (T)1;
+    bsd[1] = (T)-1;
+    bsp[0] = u;
+    bsp[1] = (T)1 - u;
+#pragma unroll
+    for (int k = 2; k < P; k++) {
+        bsp[k] = (T)0;
+#pragma unroll
+        for (int j = k; j > 0; j--) {
+            if (k == P - 1) bsd[j] = bsp[j] - bsp[j - 1];
+            T ts;
+            if (sizeof(T) == 8) ts = (T)(s + (double)j - (double)is);
+            else ts = u + (T)j;
+            bsp[j] = (ts * bsp[j] + ((T)(k + 1) - ts) * bsp[j - 1]) / (T)k;
+        }
+        if (k == P - 1) bsd[0] = bsp[0];
+        bsp[0] = u * bsp[0] / (T)k;
+    }
+}
+
 __device__ inline void scaled_coords(const MdgBox &B, int m1, int m2, int m3, double x, double y,
                                      double z, double &s1, double &s2, double &s3)
 {
@@ -126,6 +154,65 @@ k_spme_spread(MdgParams P, MdgBox B, int n, int stride, const double *__restrict
                 } else {
                     atomicAdd(&grid64[kk + o2], (double)qb * (double)b3[k]);
                 }
+            }
+        }
+    }
+}
+
+// Row-per-atom spreading for a compile-time order P: the 16 lanes of a DPP
+// row share one atom (every lane evaluates the atom's splines, a few dozen
+// flops), and lane r adds the mesh rows (i, j) = (r / P, r % P), r + 16, ...
+// along index 3, P points each.  The atom order is the scattered one of
+// k_spme_spread.  Products are formed in the same order as k_spme_spread,
+// so the mesh is the same bit for bit.
+#define SPME_LPA 16   // lanes per atom
+
+template <typename T, int P>
+__device__ inline T spme_pick(const T (&v)[P], int i)
+{
+    T r = v[0];
+#pragma unroll
+    for (int k = 1; k < P; k++) r = i == k ? v[k] : r;
+    return r;
+}
+
+template <bool MIXED, int P>
+__global__ void __launch_bounds__(256)
+k_spme_spread_rows(MdgParams Pm, MdgBox B, int n, int stride, const double *__restrict__ x,
+                   const double *__restrict__ y, const double *__restrict__ z,
+                   const double *__restrict__ q, double *grid64, long long *gridq)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int at = (int)(t / SPME_LPA), r = (int)(t % SPME_LPA);
+    if (at >= n) return;
+    const int l = (int)(((long)at * stride) % n);
+    const int m1 = Pm.m1, m2 = Pm.m2, m3 = Pm.m3;
+    double s1, s2, s3;
+    scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+    typedef typename std::conditional<MIXED, float, double>::type T;
+    T b1[P], b2[P], b3[P], d[P];
+    bspline_fix<T, P>(s1, b1, d);
+    bspline_fix<T, P>(s2, b2, d);
+    bspline_fix<T, P>(s3, b3, d);
+    const T ql = (T)q[l];
+    const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+    int kk[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) kk[k] = wrap_index(i3 - k, m3);
+#pragma unroll
+    for (int ij0 = 0; ij0 < P * P; ij0 += SPME_LPA) {
+        const int ij = ij0 + r;
+        if (ij >= P * P) break;
+        const int i = ij / P, j = ij % P;
+        const long o2 = (long)m3 * (wrap_index(i2 - j, m2) + (long)m2 * wrap_index(i1 - i, m1));
+        const T qb = ql * spme_pick<T, P>(b1, i) * spme_pick<T, P>(b2, j);
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+            if (MIXED) {
+                long long v = __double2ll_rn((double)(qb * b3[k]) * MDG_QSCALE);
+                atomicAdd((unsigned long long *)&gridq[kk[k] + o2], (unsigned long long)v);
+            } else {
+                atomicAdd(&grid64[kk[k] + o2], (double)qb * (double)b3[k]);
             }
         }
     }
@@ -237,6 +324,393 @@ k_spme_gather(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
         acc[2] = (double)(TF)fzl;
     }
     mdg_block_partials<3>(acc, part);
+}
+
+// Row-per-atom gather for a compile-time order P: lane r of the atom's
+// row sums its mesh rows (i, j) as in k_spme_spread_rows, the 16 partial
+// sums are added across the row, and lane 0 writes the force.  One
+// partial per block as k_spme_gather.
+template <bool MIXED, int P, typename TG, typename TF>
+__global__ void __launch_bounds__(256)
+k_spme_gather_rows(MdgParams Pm, MdgBox B, int n, const double *__restrict__ x,
+                   const double *__restrict__ y, const double *__restrict__ z,
+                   const double *__restrict__ q, const TG *__restrict__ grid, double fact1,
+                   TF *__restrict__ frx, TF *__restrict__ fry, TF *__restrict__ frz,
+                   double *__restrict__ part)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)(t / SPME_LPA), r = (int)(t % SPME_LPA);
+    typedef typename std::conditional<MIXED, float, double>::type T;
+    T fbx = 0, fby = 0, fbz = 0;
+    double s1 = 0., s2 = 0., s3 = 0.;
+    const int m1 = Pm.m1, m2 = Pm.m2, m3 = Pm.m3;
+    if (l < n) {
+        scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+        T b1[P], b2[P], b3[P], d1[P], d2[P], d3[P];
+        bspline_fix<T, P>(s1, b1, d1);
+        bspline_fix<T, P>(s2, b2, d2);
+        bspline_fix<T, P>(s3, b3, d3);
+        const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+        int kk[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) kk[k] = wrap_index(i3 - k, m3);
+#pragma unroll
+        for (int ij0 = 0; ij0 < P * P; ij0 += SPME_LPA) {
+            const int ij = ij0 + r;
+            if (ij >= P * P) break;
+            const int i = ij / P, j = ij % P;
+            const long o2 = (long)m3 * (wrap_index(i2 - j, m2) + (long)m2 * wrap_index(i1 - i, m1));
+            const T bi = spme_pick<T, P>(b1, i), bj = spme_pick<T, P>(b2, j);
+            const T c1 = spme_pick<T, P>(d1, i) * bj, c2 = bi * spme_pick<T, P>(d2, j);
+            const T c3 = bi * bj;
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                const T qt = (T)grid[kk[k] + o2];
+                fbx += qt * c1 * b3[k];
+                fby += qt * c2 * b3[k];
+                fbz += qt * c3 * d3[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < SPME_LPA; o <<= 1) {
+        fbx += __shfl_xor(fbx, o, SPME_LPA);
+        fby += __shfl_xor(fby, o, SPME_LPA);
+        fbz += __shfl_xor(fbz, o, SPME_LPA);
+    }
+    double acc[3] = {0, 0, 0};
+    if (l < n && r == 0) {
+        const double gx = (double)fbx * m1, gy = (double)fby * m2, gz = (double)fbz * m3;
+        const double c = fact1 * q[l];
+        const double fxl = c * (gx * B.u1 + gy * B.v1 + gz * B.w1);
+        const double fyl = c * (gx * B.u2 + gy * B.v2 + gz * B.w2);
+        const double fzl = c * (gx * B.u3 + gy * B.v3 + gz * B.w3);
+        frx[l] = (TF)fxl;
+        fry[l] = (TF)fyl;
+        frz[l] = (TF)fzl;
+        acc[0] = (double)(TF)fxl;
+        acc[1] = (double)(TF)fyl;
+        acc[2] = (double)(TF)fzl;
+    }
+    mdg_block_partials<3>(acc, part);
+}
+
+// Binned spreading without global atomics (mixed).  Every step the atoms
+// are sorted into bins of SPME_BIN^3 mesh points by the top mesh index of
+// their spline support (k_spme_bin, k_spme_bin_scan, k_spme_bin_place).
+// One block then owns a tile of SPME_TILE^3 mesh points: it reads the
+// atoms of the bins whose support can reach the tile, adds their
+// fixed-point contributions inside the tile in LDS (integer, so the sum
+// does not depend on the order), and writes the tile's FP32 mesh values.
+// Each contribution is formed as in k_spme_spread, so the mesh is the
+// same bit for bit, and no global clearing or conversion pass is needed.
+//
+// An atom's P mesh indices per dimension are the run top, top - 1, ...,
+// top - P + 1 (mod m) with top = wrap_index(i, m) whenever the spline base
+// i lies in [0, m]; an atom outside that (more than one mesh spacing out
+// of the box) goes to a short list every tile checks point by point.
+#define SPME_BIN 4
+#define SPME_TILE 8
+
+__device__ inline int spme_top(int i, int m) { return (i >= 0 && i <= m) ? (i == m ? 0 : i) : -1; }
+
+__global__ void __launch_bounds__(256)
+k_spme_bin(MdgParams Pm, MdgBox B, int n, int nby, int nbz, const double *__restrict__ x,
+           const double *__restrict__ y, const double *__restrict__ z, int *binCnt,
+           int *__restrict__ binOf, int *__restrict__ binRank, int *__restrict__ binOut,
+           int *binNOut)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    double s1, s2, s3;
+    scaled_coords(B, Pm.m1, Pm.m2, Pm.m3, x[l], y[l], z[l], s1, s2, s3);
+    const int t1 = spme_top((int)s1, Pm.m1), t2 = spme_top((int)s2, Pm.m2),
+              t3 = spme_top((int)s3, Pm.m3);
+    if (t1 < 0 || t2 < 0 || t3 < 0) {
+        binOut[atomicAdd(&binNOut[0], 1)] = l;
+        binOf[l] = -1;
+        return;
+    }
+    const int b = ((t1 / SPME_BIN) * nby + t2 / SPME_BIN) * nbz + t3 / SPME_BIN;
+    binOf[l] = b;
+    binRank[l] = atomicAdd(&binCnt[b], 1);
+}
+
+// One block: exclusive scan of the bin counts, which it clears for the next
+// step; the outlier count moves to binNOut[1].
+__global__ void __launch_bounds__(1024)
+k_spme_bin_scan(int nb, int *binCnt, int *__restrict__ binStart, int *binNOut)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x, per = (nb + 1023) / 1024;
+    const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; b++) s += binCnt[b];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int b = b0; b < b1; b++) {
+        const int c = binCnt[b];
+        binStart[b] = run;
+        run += c;
+        binCnt[b] = 0;
+    }
+    if (t == 1023) binStart[nb] = part[1023];
+    if (t == 0) {
+        binNOut[1] = binNOut[0];
+        binNOut[0] = 0;
+    }
+}
+
+__global__ void k_spme_bin_place(int n, const int *__restrict__ binOf,
+                                 const int *__restrict__ binRank, const int *__restrict__ binStart,
+                                 int *__restrict__ binAtoms)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int b = binOf[l];
+    if (b >= 0) binAtoms[binStart[b] + binRank[l]] = l;
+}
+
+// The bins (at most SPME_TILE / SPME_BIN + 1 + (P - 1) / SPME_BIN + 1 per
+// dimension) holding atoms whose top index lies in [X0, X0 + T + P - 2]
+// (mod m); the range is shorter than m - SPME_BIN, so no bin repeats.
+template <int P>
+__device__ inline int spme_tile_bins(int X0, int m, int *bins)
+{
+    int nb = 0, last = -1;
+    for (int d = 0; d < SPME_TILE + P - 1; d++) {
+        const int top = (X0 + d) % m, b = top / SPME_BIN;
+        if (b != last) {
+            bins[nb++] = b;
+            last = b;
+        }
+    }
+    return nb;
+}
+
+template <int P>
+__global__ void __launch_bounds__(256)
+k_spme_spread_tiles(MdgParams Pm, MdgBox B, int nty, int ntz, int nby, int nbz,
+                    const double *__restrict__ x, const double *__restrict__ y,
+                    const double *__restrict__ z, const double *__restrict__ q,
+                    const int *__restrict__ binStart, const int *__restrict__ binAtoms,
+                    const int *__restrict__ binOut, const int *__restrict__ binNOut,
+                    float *__restrict__ grid32)
+{
+    constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
+    __shared__ unsigned long long acc[T * T * T];
+    __shared__ int bx[NBD], by[NBD], bz[NBD], nbd[3];
+    __shared__ int off[NBD * NBD * NBD + 1], bbeg[NBD * NBD * NBD];
+    const int m1 = Pm.m1, m2 = Pm.m2, m3 = Pm.m3;
+    const int tile = blockIdx.x;
+    const int tx = tile / (nty * ntz), ty = (tile / ntz) % nty, tz = tile % ntz;
+    const int X0 = tx * T, Y0 = ty * T, Z0 = tz * T;
+    for (int i = threadIdx.x; i < T * T * T; i += blockDim.x) acc[i] = 0ull;
+    if (threadIdx.x == 0) {
+        nbd[0] = spme_tile_bins<P>(X0, m1, bx);
+        nbd[1] = spme_tile_bins<P>(Y0, m2, by);
+        nbd[2] = spme_tile_bins<P>(Z0, m3, bz);
+    }
+    __syncthreads();
+    const int nbt = nbd[0] * nbd[1] * nbd[2];
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int c = 0; c < nbt; c++) {
+            const int ix = c / (nbd[1] * nbd[2]), iy = (c / nbd[2]) % nbd[1], iz = c % nbd[2];
+            const int b = (bx[ix] * nby + by[iy]) * nbz + bz[iz];
+            off[c] = run;
+            bbeg[c] = binStart[b];
+            run += binStart[b + 1] - binStart[b];
+        }
+        off[nbt] = run;
+    }
+    __syncthreads();
+    const int ncand = off[nbt], nout = binNOut[1];
+    for (int c0 = 0; c0 < ncand + nout; c0 += blockDim.x) {
+        const int c = c0 + threadIdx.x;
+        if (c < ncand + nout) {
+            int l;
+            if (c < ncand) {
+                int lo = 0, hi = nbt;   // off[lo] <= c < off[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (off[mid] <= c) lo = mid;
+                    else hi = mid;
+                }
+                l = binAtoms[bbeg[lo] + (c - off[lo])];
+            } else {
+                l = binOut[c - ncand];
+            }
+            double s1, s2, s3;
+            scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+            float b1[P], b2[P], b3[P], d[P];
+            bspline_fix<float, P>(s1, b1, d);
+            bspline_fix<float, P>(s2, b2, d);
+            bspline_fix<float, P>(s3, b3, d);
+            const float ql = (float)q[l];
+            const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                const int lx = wrap_index(i1 - i, m1) - X0;
+                if (lx < 0 || lx >= T) continue;
+                const float qa = ql * b1[i];
+#pragma unroll
+                for (int j = 0; j < P; j++) {
+                    const int ly = wrap_index(i2 - j, m2) - Y0;
+                    if (ly < 0 || ly >= T) continue;
+                    const float qb = qa * b2[j];
+#pragma unroll
+                    for (int k = 0; k < P; k++) {
+                        const int lz = wrap_index(i3 - k, m3) - Z0;
+                        if (lz < 0 || lz >= T) continue;
+                        const long long v = __double2ll_rn((double)(qb * b3[k]) * MDG_QSCALE);
+                        atomicAdd(&acc[(lx * T + ly) * T + lz], (unsigned long long)v);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T * T * T; i += blockDim.x) {
+        const int gx = X0 + i / (T * T), gy = Y0 + (i / T) % T, gz = Z0 + i % T;
+        if (gx < m1 && gy < m2 && gz < m3)
+            grid32[gz + (long)m3 * (gy + (long)m2 * gx)] =
+                (float)((double)(long long)acc[i] * (1.0 / MDG_QSCALE));
+    }
+}
+
+// Whether the binned path covers this mesh and order (else the atomic one).
+static bool spme_binned(const mdg_ctx *ctx)
+{
+    const int p = ctx->P.nbsp, need = SPME_TILE + p + SPME_BIN;
+    return (p == 4 || p == 6 || p == 8) && ctx->m1 >= need && ctx->m2 >= need && ctx->m3 >= need;
+}
+
+static int spme_bin_alloc(mdg_ctx *ctx)
+{
+    if (ctx->binCnt || !spme_binned(ctx)) return MDG_OK;
+    ctx->nbx = (ctx->m1 + SPME_BIN - 1) / SPME_BIN;
+    ctx->nby = (ctx->m2 + SPME_BIN - 1) / SPME_BIN;
+    ctx->nbz = (ctx->m3 + SPME_BIN - 1) / SPME_BIN;
+    const long nb = (long)ctx->nbx * ctx->nby * ctx->nbz;
+    const int n = ctx->n;
+    HIPCHK(hipMalloc(&ctx->binCnt, nb * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binStart, (nb + 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binOf, n * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binRank, n * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binAtoms, n * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binOut, n * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binNOut, 2 * sizeof(int)));
+    RETCHK(mdg_fill_i32(ctx, ctx->binCnt, nb, 0));
+    RETCHK(mdg_fill_i32(ctx, ctx->binNOut, 2, 0));
+    return MDG_OK;
+}
+
+// Binned mixed spreading into grid32 (replaces the atomic spread and the
+// fixed-point conversion).
+static int spread_binned(mdg_ctx *ctx)
+{
+    const int n = ctx->n;
+    const int nb = ctx->nbx * ctx->nby * ctx->nbz;
+    const int ntx = (ctx->m1 + SPME_TILE - 1) / SPME_TILE, nty = (ctx->m2 + SPME_TILE - 1) / SPME_TILE,
+              ntz = (ctx->m3 + SPME_TILE - 1) / SPME_TILE;
+    hipLaunchKernelGGL(k_spme_bin, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, ctx->B, n,
+                       ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->binCnt, ctx->binOf,
+                       ctx->binRank, ctx->binOut, ctx->binNOut);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_spme_bin_scan, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->binCnt,
+                       ctx->binStart, ctx->binNOut);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_spme_bin_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
+                       ctx->binOf, ctx->binRank, ctx->binStart, ctx->binAtoms);
+    LAUNCHCHK();
+    const dim3 g(ntx * nty * ntz), b(256);
+    switch (ctx->P.nbsp) {
+    case 4:
+        hipLaunchKernelGGL(k_spme_spread_tiles<4>, g, b, 0, ctx->stream, ctx->P, ctx->B, nty, ntz,
+                           ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->q, ctx->binStart,
+                           ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->grid32);
+        break;
+    case 6:
+        hipLaunchKernelGGL(k_spme_spread_tiles<6>, g, b, 0, ctx->stream, ctx->P, ctx->B, nty, ntz,
+                           ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->q, ctx->binStart,
+                           ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->grid32);
+        break;
+    default:
+        hipLaunchKernelGGL(k_spme_spread_tiles<8>, g, b, 0, ctx->stream, ctx->P, ctx->B, nty, ntz,
+                           ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->q, ctx->binStart,
+                           ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->grid32);
+    }
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+// Orders with a row-per-atom kernel; others use the thread-per-atom ones.
+static bool spme_rows(int p) { return p == 4 || p == 6 || p == 8; }
+
+int mdg_gather_blocks(const mdg_ctx *ctx)
+{
+    return spme_rows(ctx->P.nbsp) ? nblk((long)ctx->n * SPME_LPA, 256) : nblk(ctx->n, 256);
+}
+
+template <bool MIXED>
+static void launch_spread(mdg_ctx *ctx, double *grid64, long long *gridq)
+{
+    const int n = ctx->n, st = mdg_scatter_stride(n);
+    const dim3 g(nblk((long)n * SPME_LPA, 256)), b(256);
+    switch (ctx->P.nbsp) {
+    case 4:
+        hipLaunchKernelGGL((k_spme_spread_rows<MIXED, 4>), g, b, 0, ctx->stream, ctx->P, ctx->B, n,
+                           st, ctx->x, ctx->y, ctx->z, ctx->q, grid64, gridq);
+        break;
+    case 6:
+        hipLaunchKernelGGL((k_spme_spread_rows<MIXED, 6>), g, b, 0, ctx->stream, ctx->P, ctx->B, n,
+                           st, ctx->x, ctx->y, ctx->z, ctx->q, grid64, gridq);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_spme_spread_rows<MIXED, 8>), g, b, 0, ctx->stream, ctx->P, ctx->B, n,
+                           st, ctx->x, ctx->y, ctx->z, ctx->q, grid64, gridq);
+        break;
+    default:
+        hipLaunchKernelGGL(k_spme_spread<MIXED>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
+                           ctx->P, ctx->B, n, st, ctx->x, ctx->y, ctx->z, ctx->q, grid64, gridq);
+    }
+}
+
+template <bool MIXED, typename TG, typename TF>
+static void launch_gather(mdg_ctx *ctx, const TG *grid, double fact1, TF *frx, TF *fry, TF *frz,
+                          double *part)
+{
+    const int n = ctx->n;
+    const dim3 g(nblk((long)n * SPME_LPA, 256)), b(256);
+    switch (ctx->P.nbsp) {
+    case 4:
+        hipLaunchKernelGGL((k_spme_gather_rows<MIXED, 4, TG, TF>), g, b, 0, ctx->stream, ctx->P,
+                           ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, grid, fact1, frx, fry, frz,
+                           part);
+        break;
+    case 6:
+        hipLaunchKernelGGL((k_spme_gather_rows<MIXED, 6, TG, TF>), g, b, 0, ctx->stream, ctx->P,
+                           ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, grid, fact1, frx, fry, frz,
+                           part);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_spme_gather_rows<MIXED, 8, TG, TF>), g, b, 0, ctx->stream, ctx->P,
+                           ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, grid, fact1, frx, fry, frz,
+                           part);
+        break;
+    default:
+        hipLaunchKernelGGL((k_spme_gather<MIXED, TG, TF>), dim3(nblk(n, 256)), dim3(256), 0,
+                           ctx->stream, ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q, grid,
+                           fact1, frx, fry, frz, part);
+    }
 }
 
 // dE_rec/dq_l = 2 (2 pi / V) C sum_g (theta*Q)(g) M_l(g): the convolved
@@ -417,6 +891,7 @@ int mdg_spme_setup(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&ctx->gridq, G * sizeof(long long)));
         RETCHK(mdg_fill_i64(ctx, ctx->gridq, G));
         RETCHK(make_plans(ctx, false, &ctx->fwd32, &ctx->bwd32));
+        RETCHK(spme_bin_alloc(ctx));
     }
     return MDG_OK;
 }
@@ -441,13 +916,15 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
     const double rAlpha2 = -0.25 / (ctx->P.alpha * ctx->P.alpha);
     const double fact1 = -2.0 * (TWOPI / ctx->B.vol) * ctx->P.chargeConst;
     if (mixed) {
-        hipLaunchKernelGGL(k_spme_spread<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
-                           ctx->P, ctx->B, n, mdg_scatter_stride(n), ctx->x, ctx->y, ctx->z,
-                           ctx->q, nullptr, ctx->gridq);
-        LAUNCHCHK();
-        hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(G, 256), 8192)), dim3(256),
-                           0, ctx->stream, G, ctx->gridq, ctx->grid32);
-        LAUNCHCHK();
+        if (spme_binned(ctx)) {
+            RETCHK(spread_binned(ctx));
+        } else {
+            launch_spread<true>(ctx, nullptr, ctx->gridq);
+            LAUNCHCHK();
+            hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(G, 256), 8192)),
+                               dim3(256), 0, ctx->stream, G, ctx->gridq, ctx->grid32);
+            LAUNCHCHK();
+        }
         void *in[1] = {ctx->grid32}, *out[1] = {ctx->cgrid32};
         if (rocfft_execute(ctx->fwd32, in, out, ctx->info) != rocfft_status_success) {
             ctx->err = "rocfft_execute (forward) failed";
@@ -461,16 +938,12 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
             ctx->err = "rocfft_execute (inverse) failed";
             return MDG_EHIP;
         }
-        hipLaunchKernelGGL((k_spme_gather<true, float, float>), dim3(nblk(n, 256)), dim3(256), 0,
-                           ctx->stream, ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q,
-                           ctx->grid32, fact1, (float *)frx, (float *)fry, (float *)frz,
-                           partGather);
+        launch_gather<true>(ctx, (const float *)ctx->grid32, fact1, (float *)frx, (float *)fry,
+                            (float *)frz, partGather);
         LAUNCHCHK();
     } else {
         RETCHK(mdg_fill_f64(ctx, ctx->grid64, G, 0.0));
-        hipLaunchKernelGGL(k_spme_spread<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
-                           ctx->P, ctx->B, n, mdg_scatter_stride(n), ctx->x, ctx->y, ctx->z,
-                           ctx->q, ctx->grid64, nullptr);
+        launch_spread<false>(ctx, ctx->grid64, nullptr);
         LAUNCHCHK();
         void *in[1] = {ctx->grid64}, *out[1] = {ctx->cgrid64};
         if (rocfft_execute(ctx->fwd, in, out, ctx->info) != rocfft_status_success) {
@@ -486,10 +959,8 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
             ctx->err = "rocfft_execute (inverse) failed";
             return MDG_EHIP;
         }
-        hipLaunchKernelGGL((k_spme_gather<false, double, double>), dim3(nblk(n, 256)), dim3(256),
-                           0, ctx->stream, ctx->P, ctx->B, n, ctx->x, ctx->y, ctx->z, ctx->q,
-                           ctx->grid64, fact1, (double *)frx, (double *)fry, (double *)frz,
-                           partGather);
+        launch_gather<false>(ctx, (const double *)ctx->grid64, fact1, (double *)frx,
+                             (double *)fry, (double *)frz, partGather);
         LAUNCHCHK();
     }
     return MDG_OK;

@@ -192,9 +192,41 @@ def test_mixed_force_error(which, c0_box, c1_box):
     f64, fm = out[M.MDG_FP64][1], out[M.MDG_MIXED][1]
     rms = relerr(fm, f64)
     mx = float((np.linalg.norm(fm - f64, axis=0) / np.linalg.norm(f64, axis=0)).max())
-    say("mixed", which, "rms rel", rms, "max rel", mx)
+    say("mixed", which, "rms rel", rms, "max rel", mx, "pot rel",
+        abs(out[M.MDG_MIXED][0].pot / out[M.MDG_FP64][0].pot - 1.0))
     assert rms <= 1e-5
     assert out[M.MDG_MIXED][0].pot == pytest.approx(out[M.MDG_FP64][0].pot, rel=5e-6)
+
+
+def test_mixed_spme_outlier_atoms(c0_box):
+    """Atoms more than one mesh spacing outside the box (whole molecules
+    shifted by a box vector) take the binned spread's point-by-point path,
+    with the reference's mesh-index rule (spme.c:453-459); FP64 spreads
+    them with atomics under the same rule.  Mixed vs FP64 to the mixed bar,
+    and the shifted system's energy differs from the unshifted one (the
+    rule is not periodic for such atoms, so the path is exercised)."""
+    wb = c0_box.copy()
+    L = wb.L
+    for w in range(0, wb.n_waters, 37):          # every 37th molecule, O H H
+        s = slice(3 * w, 3 * w + 3)
+        wb.x[s] += L
+        wb.y[s] -= 2.0 * L
+    pots, forces = {}, {}
+    for prec in (M.MDG_FP64, M.MDG_MIXED):
+        ctx = M.Context(wb, precision=prec)
+        ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        pots[prec] = ctx.initial_energy().pot
+        forces[prec] = np.stack(ctx.download()[6:9])
+        ctx.close()
+    rms = relerr(forces[M.MDG_MIXED], forces[M.MDG_FP64])
+    say("outliers rms rel", rms, "pot", pots[M.MDG_MIXED], pots[M.MDG_FP64])
+    assert rms <= 1e-5
+    assert pots[M.MDG_MIXED] == pytest.approx(pots[M.MDG_FP64], rel=5e-6)
+    c = M.Context(c0_box, precision=M.MDG_FP64)
+    c.upload(c0_box.x, c0_box.y, c0_box.z, c0_box.vx, c0_box.vy, c0_box.vz)
+    p0 = c.initial_energy().pot
+    c.close()
+    assert abs(p0 - pots[M.MDG_FP64]) > 1e-6 * abs(p0)
 
 
 def test_mixed_graph_replay_is_bitwise_deterministic(c0_box):
