@@ -4,6 +4,7 @@
 #include "mdg_internal.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -436,7 +437,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
                     ctx->shkMask, ctx->exSymOff, ctx->exSymLst, ctx->adjQ, ctx->adjE,
                     ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binStart, ctx->binOf,
-                    ctx->binRank, ctx->binAtoms, ctx->binOut, ctx->binNOut};
+                    ctx->binRank, ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->splI,
+                    ctx->splB, ctx->splD};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->bt) {
@@ -548,6 +550,34 @@ static int batch_topology(mdg_ctx *ctx, const double q[])
     return batch_charge_sums(ctx, q);
 }
 
+// shake.c:243-258 tests 0.5 * |r^2 - d^2| / d < tol with d = sqrt(rc2)
+// (x 0.5 is exact, so q = |r^2 - d^2| / d < 2 tol).  A correctly rounded
+// division is monotone in the numerator, so the test holds exactly when the
+// numerator is below the smallest a with fl(a / d) >= 2 tol, found here by
+// bisection over the bit patterns of non-negative doubles; the device then
+// compares without dividing.
+static double shake_threshold(double rc2, double tol)
+{
+    const double d = std::sqrt(rc2), T = 2.0 * tol;
+    if (!(T > 0.0) || !(d > 0.0)) return 0.0;   // never converged / degenerate
+    auto val = [](uint64_t u) {
+        double v;
+        std::memcpy(&v, &u, sizeof v);
+        return v;
+    };
+    uint64_t lo = 0, hi;   // fl(val(lo) / d) < T <= fl(val(hi) / d)
+    {
+        double h = T * d * 4.0;
+        std::memcpy(&hi, &h, sizeof hi);
+    }
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (val(mid) / d >= T) hi = mid;
+        else lo = mid;
+    }
+    return val(hi);
+}
+
 extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps[],
                                 const double sig[], const double mass[], const double rmass[],
                                 const int frozen[], const int nAtConst[],
@@ -568,7 +598,10 @@ extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps
     if (nAtConst) RETCHK(h2d(ctx, ctx->nAtConst, nAtConst, n));
     charge_sums(ctx, q);
     // constraints
-    if (ctx->ca) { hipFree(ctx->ca); hipFree(ctx->cb); hipFree(ctx->rc2); hipFree(ctx->dd); }
+    if (ctx->ca) {
+        hipFree(ctx->ca); hipFree(ctx->cb); hipFree(ctx->rc2); hipFree(ctx->dd);
+        hipFree(ctx->shkThr);
+    }
     ctx->nConst = nConst;
     std::vector<int> a(std::max(nConst, 1)), b(std::max(nConst, 1));
     std::vector<double> r2(std::max(nConst, 1));
@@ -584,6 +617,12 @@ extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps
     RETCHK(h2d(ctx, ctx->ca, a.data(), nConst));
     RETCHK(h2d(ctx, ctx->cb, b.data(), nConst));
     RETCHK(h2d(ctx, ctx->rc2, r2.data(), nConst));
+    {
+        std::vector<double> th(std::max(nConst, 1));
+        for (int c = 0; c < nConst; c++) th[c] = shake_threshold(r2[c], ctx->P.tolShake);
+        RETCHK(dmalloc(ctx, &ctx->shkThr, nConst));
+        RETCHK(h2d(ctx, ctx->shkThr, th.data(), nConst));
+    }
     if (ctx->grpAtomOff) {
         hipFree(ctx->grpAtomOff); hipFree(ctx->grpAtoms); hipFree(ctx->grpConOff);
         hipFree(ctx->grpCons); hipFree(ctx->grpCycle); hipFree(ctx->grpVir);

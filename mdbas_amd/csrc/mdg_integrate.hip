@@ -596,6 +596,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.ca = ctx->ca;
     A.cb = ctx->cb;
     A.rc2 = ctx->rc2;
+    A.thr = ctx->shkThr;
     A.rmass = ctx->rmass;
     A.mass = ctx->mass;
     A.nAtConst = ctx->nAtConst;

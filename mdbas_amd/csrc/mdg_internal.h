@@ -125,7 +125,8 @@ struct MdgBatch {
 // Launch shapes the autotuner chooses (mdg_autotune).
 struct MdgTune {
     int shkSpread = 1;           // three-site SHAKE: quads per molecule slot (1, 2, 4, 8, 16)
-    int pairBlocks = 4096;       // cap on the cluster pair kernel's grid
+    int pairBlocks = 1024;       // cap on the cluster pair kernel's grid (4 blocks per CU leave
+                                 // wave slots for the SPME kernels beside it)
 };
 
 struct mdg_ctx {
@@ -156,6 +157,7 @@ struct mdg_ctx {
     int nConst = 0;
     int *ca = nullptr, *cb = nullptr;
     double *rc2 = nullptr;
+    double *shkThr = nullptr;    // per constraint: exact threshold of the stage-1 check
     double *dd = nullptr;        // constraint vectors of the current VV stage [3][nConst]
     // constraint groups (connected components; singletons for free atoms)
     int nGrp = 0;
@@ -217,6 +219,9 @@ struct mdg_ctx {
     int *binOf = nullptr, *binRank = nullptr, *binAtoms = nullptr;   // per atom
     int *binOut = nullptr;       // atoms whose mesh points are not one run
     int *binNOut = nullptr;      // [0] being counted, [1] this step's count
+    int4 *splI = nullptr;        // per atom: spline base index per dimension (binned path)
+    float4 *splB = nullptr;      // per atom: FP32 spline values [3 * nq][n], nq = (order + 3) / 4
+    float4 *splD = nullptr;      // per atom: their derivatives, same layout
     rocfft_plan fwd = nullptr, bwd = nullptr, fwd32 = nullptr, bwd32 = nullptr;
     rocfft_execution_info info = nullptr;
     void *fftWork = nullptr;

@@ -982,7 +982,7 @@ extern "C" int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2])
         *param = best;
     };
     if (ctx->shakeW3) sweep(&ctx->tune.shkSpread, {1, 2, 4, 8});
-    sweep(&ctx->tune.pairBlocks, {1024, 2048, 4096, 8192, 16384});
+    sweep(&ctx->tune.pairBlocks, {512, 768, 1024, 2048, 4096, 8192, 16384});
     mdg_invalidate_graphs(ctx);
     for (int k = 0; k < 6 && st == MDG_OK; k++)
         if (hipMemcpyAsync(d[k], state + k * n, n * sizeof(double), hipMemcpyDeviceToDevice,

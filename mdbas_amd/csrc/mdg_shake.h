@@ -15,6 +15,8 @@ struct ShakeArgs {
     const int *atomOff, *atoms, *conOff, *cons;
     const int *ca, *cb;
     const double *rc2, *rmass, *mass;
+    const double *thr;    // per constraint: the stage-1 test 0.5 |r^2 - d^2| / d < tol
+                          // holds exactly when |r^2 - d^2| < thr (d = sqrt(rc2))
     const int *nAtConst;
     double *x, *y, *z, *vx, *vy, *vz;
     const double *fx, *fy, *fz;

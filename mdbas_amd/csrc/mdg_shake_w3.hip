@@ -26,6 +26,8 @@ constexpr int QP_L1 = 0xD5;    // [1,1,1,3]
 constexpr int QP_L2 = 0xEA;    // [2,2,2,3]
 constexpr int QP_X1 = 0xB1;    // [1,0,3,2]
 constexpr int QP_X2 = 0x4E;    // [2,3,0,1]
+constexpr int QP_R1 = 0xF1;    // [1,0,3,3]: lanes 0 <-> 1, lane 2 reads idle lane 3
+constexpr int QP_R2 = 0xDB;    // [3,2,1,3]: lanes 1 <-> 2, lane 0 reads idle lane 3
 
 template <int CTRL>
 __device__ inline double qperm(double v)
@@ -35,20 +37,41 @@ __device__ inline double qperm(double v)
     return __hiloint2double(hi, lo);
 }
 
-// The quad's largest value.
-__device__ inline double quad_max(double v)
+template <int CTRL>
+__device__ inline int qperm_i(int v)
 {
-    v = fmax(v, qperm<QP_X1>(v));
-    return fmax(v, qperm<QP_X2>(v));
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
 
-// One lane's share of a molecule.
+// True on every lane of the quad when c holds on all four lanes.
+__device__ inline bool quad_all(bool c)
+{
+    int v = c ? 1 : 0;
+    v &= qperm_i<QP_X1>(v);
+    v &= qperm_i<QP_X2>(v);
+    return v != 0;
+}
+
+// One lane's share of a molecule.  Lane k < 3 holds atom k and constraint
+// k = (a_k, b_k) = (0,1), (0,2), (1,2); lane 3 idles with zero data.
+//
+// Correction exchange: atom 0 receives the a-side terms of constraints 0
+// and 1, atom 1 the b-side of 0 and the a-side of 2, atom 2 the b-side of 1
+// and 2.  Lane k sends u = dd f_u in round 1 (lanes 0 <-> 1) and v = dd f_v
+// in round 2 (lanes 1 <-> 2) and keeps o = dd f_o for itself, where each f
+// is the lane's li, lj or 0; the atom's correction is (r1 + r2) + o, in
+// which exactly two terms are non-zero (a sum of two is the same in either
+// order, and adding an exact zero changes nothing).
 struct Q3 {
     int k;              // 0..2 live, 3 idle
     int at;             // atom k
     int cid;            // constraint k
     double x, y, z, vx, vy, vz, rm, nat;
-    double rc2, src2, ddx, ddy, ddz;
+    double rc2, thr;    // stage 1: converged while |r^2 - rc2| < thr
+    double ddx, ddy, ddz, ddsq;   // constraint vector at the stage start, |dd|^2
+    double shx, shy, shz;         // image shift of the constraint vector (stage 1)
+    double K;           // stage 1: dt (trmi + trmj); stage 2: (trmi + trmj) |dd|^2
+    double cu, cv, co;  // correction factors (per stage: +-trmi, +-trmj or 0)
     double rma, rmb;    // inverse masses of constraint k's atoms
 };
 
@@ -59,9 +82,12 @@ __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
     Q.cid = -1;
     Q.x = Q.y = Q.z = Q.vx = Q.vy = Q.vz = 0.;
     Q.rm = Q.nat = 0.;
-    Q.rc2 = 1.;
-    Q.src2 = 1.;
-    Q.ddx = Q.ddy = Q.ddz = 0.;
+    Q.rc2 = 0.;
+    Q.thr = 1.;
+    Q.ddx = Q.ddy = Q.ddz = Q.ddsq = 0.;
+    Q.shx = Q.shy = Q.shz = 0.;
+    Q.K = 1.;
+    Q.cu = Q.cv = Q.co = 0.;
     if (k < 3) {
         const int i = A.atoms[A.atomOff[g] + k];
         Q.at = i;
@@ -75,12 +101,43 @@ __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
         Q.nat = 1.0 / (double)A.nAtConst[i];
         Q.cid = A.cons[A.conOff[g] + k];
         Q.rc2 = A.rc2[Q.cid];
-        Q.src2 = sqrt(Q.rc2);
+        Q.thr = A.thr[Q.cid];
     }
-    // loop invariants, moved once here (the moves cannot be hoisted by the
-    // compiler out of the divergent sweep loops)
     Q.rma = qperm<QP_A>(Q.rm);
     Q.rmb = qperm<QP_B>(Q.rm);
+}
+
+// The lane's correction factors from its constraint's a-side and b-side
+// coefficients ti, tj (li = h ti, lj = -h tj for the sweep's multiplier h).
+__device__ inline void q_factors(Q3 &Q, double ti, double tj)
+{
+    const double mi = ti, mj = -tj;
+    Q.cu = Q.k == 0 ? mj : (Q.k == 1 ? mi : 0.);
+    Q.cv = Q.k == 1 ? mj : (Q.k == 2 ? mi : 0.);
+    Q.co = Q.k == 0 ? mi : (Q.k == 2 ? mj : 0.);
+}
+
+// Stage 1: lambda denominators and factors (trmi = dt rma, trmj = dt rmb;
+// li = 0.5 lambda trmi, lj = -0.5 lambda trmj, shake.c:262-332).
+__device__ inline void q_setup_r(const ShakeArgs &A, Q3 &Q)
+{
+#pragma clang fp contract(off)
+    const double dt = A.P.timeStep;
+    const double trmi = dt * Q.rma, trmj = dt * Q.rmb;
+    Q.K = Q.k < 3 ? dt * (trmi + trmj) : 1.;
+    Q.ddsq = Q.ddx * Q.ddx + Q.ddy * Q.ddy + Q.ddz * Q.ddz;
+    q_factors(Q, trmi, trmj);
+}
+
+// Stage 2: den = (trmi + trmj) |dd|^2 with trmi = 0.5 dt rma, trmj = 0.5 dt
+// rmb; li = lambda trmi, lj = -lambda trmj (shake.c:378-413).
+__device__ inline void q_setup_v(const ShakeArgs &A, Q3 &Q)
+{
+#pragma clang fp contract(off)
+    const double dt = A.P.timeStep;
+    const double trmi = 0.5 * dt * Q.rma, trmj = 0.5 * dt * Q.rmb;
+    Q.K = Q.k < 3 ? (trmi + trmj) * (Q.ddx * Q.ddx + Q.ddy * Q.ddy + Q.ddz * Q.ddz) : 1.;
+    q_factors(Q, trmi, trmj);
 }
 
 __device__ inline void q_store(const ShakeArgs &A, const Q3 &Q, bool pos)
@@ -101,42 +158,30 @@ __device__ inline void q_scr_store(const ShakeArgs &A, const Q3 &Q, int slot, bo
 {
     if (Q.k >= 3) return;
     const long n = A.P.n;
-    double *s = A.scr + (long)slot * 6 * n;
-    const int i = Q.at;
+    double *s = A.scr + (long)slot * 6 * n + Q.at;
     if (pos) {
-        s[i] = Q.x;
-        s[n + i] = Q.y;
-        s[2 * n + i] = Q.z;
+        s[0] = Q.x;
+        s[n] = Q.y;
+        s[2 * n] = Q.z;
     }
-    s[3 * n + i] = Q.vx;
-    s[4 * n + i] = Q.vy;
-    s[5 * n + i] = Q.vz;
+    s[3 * n] = Q.vx;
+    s[4 * n] = Q.vy;
+    s[5 * n] = Q.vz;
 }
 
 __device__ inline void q_scr_load(const ShakeArgs &A, Q3 &Q, int slot, bool pos)
 {
     if (Q.k >= 3) return;
     const long n = A.P.n;
-    const double *s = A.scr + (long)slot * 6 * n;
-    const int i = Q.at;
+    const double *s = A.scr + (long)slot * 6 * n + Q.at;
     if (pos) {
-        Q.x = s[i];
-        Q.y = s[n + i];
-        Q.z = s[2 * n + i];
+        Q.x = s[0];
+        Q.y = s[n];
+        Q.z = s[2 * n];
     }
-    Q.vx = s[3 * n + i];
-    Q.vy = s[4 * n + i];
-    Q.vz = s[5 * n + i];
-}
-
-// Minimum image of a short in-molecule vector: the reference's nint is 0
-// whenever |d u| < 0.5, leaving d unchanged, so that case skips the rest.
-__device__ inline double w3_dist(const MdgBox &B, double &dx, double &dy, double &dz)
-{
-#pragma clang fp contract(off)
-    if (B.type == 1 && fabs(dx * B.u1) < 0.5 && fabs(dy * B.u1) < 0.5 && fabs(dz * B.u1) < 0.5)
-        return dx * dx + dy * dy + dz * dz;
-    return mdg_dist(B, dx, dy, dz);
+    Q.vx = s[3 * n];
+    Q.vy = s[4 * n];
+    Q.vz = s[5 * n];
 }
 
 // x_b - x_a of constraint k (every lane takes part in the moves).
@@ -147,6 +192,32 @@ __device__ inline void q_bond(const Q3 &Q, double &dx, double &dy, double &dz)
     dz = qperm<QP_B>(Q.z) - qperm<QP_A>(Q.z);
 }
 
+// The image shift utils.c:38-80 subtracts from a constraint vector, as a
+// constant vector: a bond (about an angstrom) keeps its shift over the
+// sweeps of a stage, and d - shift is then the reference's expression.
+__device__ inline void q_shift(const MdgBox &B, Q3 &Q, double dx, double dy, double dz)
+{
+#pragma clang fp contract(off)
+    if (Q.k >= 3 || B.type == 3) return;
+    const double Ly = B.type == 1 ? B.a1 : B.b2, Lz = B.type == 1 ? B.a1 : B.c3;
+    const double uy = B.type == 1 ? B.u1 : B.v2, uz = B.type == 1 ? B.u1 : B.w3;
+    Q.shx = B.a1 * mdg_nint(dx * B.u1);
+    Q.shy = Ly * mdg_nint(dy * uy);
+    Q.shz = Lz * mdg_nint(dz * uz);
+}
+
+// Constraint vector with the stage's image shift and its |d|^2 as
+// (dx^2 + dy^2) + dz^2 (triclinic boxes take the full image every sweep).
+__device__ inline double q_img(const MdgBox &B, const Q3 &Q, double &dx, double &dy, double &dz)
+{
+#pragma clang fp contract(off)
+    if (B.type == 3) return mdg_dist(B, dx, dy, dz);
+    dx = dx - Q.shx;
+    dy = dy - Q.shy;
+    dz = dz - Q.shz;
+    return dx * dx + dy * dy + dz * dz;
+}
+
 // Constraint vectors at the stage's start (integrate.c:1536-1545 with
 // image_array), kept per constraint for the replay kernels.
 __device__ inline void q_dd(const ShakeArgs &A, const MdgBox &B, Q3 &Q)
@@ -154,7 +225,7 @@ __device__ inline void q_dd(const ShakeArgs &A, const MdgBox &B, Q3 &Q)
     double dx, dy, dz;
     q_bond(Q, dx, dy, dz);
     if (Q.k < 3) {
-        w3_dist(B, dx, dy, dz);
+        mdg_dist(B, dx, dy, dz);
         Q.ddx = dx;
         Q.ddy = dy;
         Q.ddz = dz;
@@ -172,36 +243,31 @@ __device__ inline void q_dd_load(const ShakeArgs &A, Q3 &Q)
     Q.ddz = A.dd[2 * A.nConst + Q.cid];
 }
 
-// shake.c:243-258: this lane's constraint vector and |r|^2; returns the
-// molecule's 0.5 * max |r^2 - d^2| / d.
-__device__ inline double q_check(const MdgBox &B, const Q3 &Q, double &dtx, double &dty,
-                                 double &dtz, double &rt2)
+// The image shift of this lane's constraint at the current positions.
+__device__ inline void q_shift_now(const MdgBox &B, Q3 &Q)
+{
+    double dx, dy, dz;
+    q_bond(Q, dx, dy, dz);
+    q_shift(B, Q, dx, dy, dz);
+}
+
+// shake.c:243-258: the molecule is converged when every constraint has
+// |r^2 - d^2| below its threshold; dt*, rt2 keep this lane's vector.
+__device__ inline bool q_check(const MdgBox &B, const Q3 &Q, double &dtx, double &dty,
+                               double &dtz, double &rt2)
 {
 #pragma clang fp contract(off)
     q_bond(Q, dtx, dty, dtz);
-    double d = 0.;
-    rt2 = 0.;
-    if (Q.k < 3) {
-        rt2 = w3_dist(B, dtx, dty, dtz);
-        d = fabs(rt2 - Q.rc2) / Q.src2;
-    }
-    return 0.5 * quad_max(d);
+    rt2 = q_img(B, Q, dtx, dty, dtz);
+    return quad_all(Q.k >= 3 || fabs(rt2 - Q.rc2) < Q.thr);
 }
 
-// Each atom's two corrections in constraint order: atom 0 gets (c0, a)
-// then (c1, a); atom 1 (c0, b) then (c2, a); atom 2 (c1, b) then (c2, b);
-// li / lj are this lane's constraint's a-side / b-side terms.
-__device__ inline double q_gather(int k, double li, double lj)
+// The atom's correction from the per-lane terms of its two constraints.
+__device__ inline double q_exchange(const Q3 &Q, double dd, double h)
 {
 #pragma clang fp contract(off)
-    const double a0 = qperm<QP_L0>(li), b0 = qperm<QP_A>(lj);
-    const double a1 = qperm<QP_B>(li), b1 = qperm<QP_L2>(lj);
-    const double first = k == 0 ? a0 : b0;
-    const double second = k == 2 ? b1 : a1;
-    double t = 0.;
-    t += first;
-    t += second;
-    return t;
+    const double u = dd * (h * Q.cu), v = dd * (h * Q.cv), o = dd * (h * Q.co);
+    return (qperm<QP_R1>(u) + qperm<QP_R2>(v)) + o;
 }
 
 // v + t0 + t1 + t2 over the three live lanes, in that order, on every lane.
@@ -213,82 +279,47 @@ __device__ inline double q_sum3(double v, double t)
 }
 
 // shake.c:262-332: one position sweep (lambda per constraint, corrections
-// summed per atom, applied once per constraint membership).
+// summed per atom, applied once per constraint membership); vir collects
+// this lane's constraint's virial terms.
 __device__ inline void q_apply_r(const ShakeArgs &A, Q3 &Q, double dtx, double dty, double dtz,
                                  double rt2, double &vir)
 {
 #pragma clang fp contract(off)
     const double dt = A.P.timeStep;
-    const double rma = Q.rma, rmb = Q.rmb;
-    double lix = 0., liy = 0., liz = 0., ljx = 0., ljy = 0., ljz = 0., tv = 0.;
+    const double dot = Q.ddx * dtx + Q.ddy * dty + Q.ddz * dtz;
+    double lambda = -(Q.rc2 - rt2) / (Q.K * dot);
+    lambda = Q.k < 3 ? lambda : 0.;
+    vir += lambda * Q.ddsq;
+    const double h = 0.5 * lambda;
+    const double xt = q_exchange(Q, Q.ddx, h), yt = q_exchange(Q, Q.ddy, h),
+                 zt = q_exchange(Q, Q.ddz, h);
     if (Q.k < 3) {
-        const double trmi = dt * rma, trmj = dt * rmb;
-        const double lambda = -(Q.rc2 - rt2) /
-            (dt * (trmi + trmj) * ((Q.ddx * dtx) + (Q.ddy * dty) + (Q.ddz * dtz)));
-        tv = lambda * (Q.ddx * Q.ddx + Q.ddy * Q.ddy + Q.ddz * Q.ddz);
-        const double li = 0.5 * lambda * trmi, lj = -0.5 * lambda * trmj;
-        lix = Q.ddx * li;
-        liy = Q.ddy * li;
-        liz = Q.ddz * li;
-        ljx = Q.ddx * lj;
-        ljy = Q.ddy * lj;
-        ljz = Q.ddz * lj;
-    }
-    vir = q_sum3(vir, tv);
-    const double xt = q_gather(Q.k, lix, ljx), yt = q_gather(Q.k, liy, ljy),
-                 zt = q_gather(Q.k, liz, ljz);
-    if (Q.k < 3) {
-        for (int m = 0; m < 2; m++) {   // the atom's two constraint memberships
-            Q.x += dt * xt * Q.nat;
-            Q.y += dt * yt * Q.nat;
-            Q.z += dt * zt * Q.nat;
-        }
-        for (int m = 0; m < 2; m++) {
-            Q.vx += xt * Q.nat;
-            Q.vy += yt * Q.nat;
-            Q.vz += zt * Q.nat;
-        }
+        // the correction once per constraint membership (two per atom)
+        const double ix = dt * xt * Q.nat, iy = dt * yt * Q.nat, iz = dt * zt * Q.nat;
+        Q.x = (Q.x + ix) + ix;
+        Q.y = (Q.y + iy) + iy;
+        Q.z = (Q.z + iz) + iz;
+        const double jx = xt * Q.nat, jy = yt * Q.nat, jz = zt * Q.nat;
+        Q.vx = (Q.vx + jx) + jx;
+        Q.vy = (Q.vy + jy) + jy;
+        Q.vz = (Q.vz + jz) + jz;
     }
 }
 
-// (trmi + trmj) |dd|^2 of this lane's constraint (shake.c:391-393).
-__device__ inline double q_den(const ShakeArgs &A, const Q3 &Q)
+// shake.c:378-413: one velocity sweep's lambda; converged when every
+// |lambda| of the molecule is below tolvel.  Corrections in xt, yt, zt.
+__device__ inline bool q_lambda_v(const Q3 &Q, double tolvel, double &xt, double &yt, double &zt)
 {
 #pragma clang fp contract(off)
-    const double dt = A.P.timeStep;
-    const double rma = Q.rma, rmb = Q.rmb;
-    const double trmi = 0.5 * dt * rma, trmj = 0.5 * dt * rmb;
-    return (trmi + trmj) * (Q.ddx * Q.ddx + Q.ddy * Q.ddy + Q.ddz * Q.ddz);
-}
-
-// shake.c:378-413: velocity lambdas of one sweep; returns the molecule's
-// max |lambda|, the per-atom corrections in xt, yt, zt.
-__device__ inline double q_lambda_v(const ShakeArgs &A, const Q3 &Q, double den, double &xt,
-                                    double &yt, double &zt)
-{
-#pragma clang fp contract(off)
-    const double dt = A.P.timeStep;
-    const double rma = Q.rma, rmb = Q.rmb;
     const double dvx = qperm<QP_B>(Q.vx) - qperm<QP_A>(Q.vx);
     const double dvy = qperm<QP_B>(Q.vy) - qperm<QP_A>(Q.vy);
     const double dvz = qperm<QP_B>(Q.vz) - qperm<QP_A>(Q.vz);
-    double lix = 0., liy = 0., liz = 0., ljx = 0., ljy = 0., ljz = 0., al = 0.;
-    if (Q.k < 3) {
-        const double trmi = 0.5 * dt * rma, trmj = 0.5 * dt * rmb;
-        const double lambda = (Q.ddx * dvx + Q.ddy * dvy + Q.ddz * dvz) / den;
-        al = fabs(lambda);
-        const double li = lambda * trmi, lj = -lambda * trmj;
-        lix = Q.ddx * li;
-        liy = Q.ddy * li;
-        liz = Q.ddz * li;
-        ljx = Q.ddx * lj;
-        ljy = Q.ddy * lj;
-        ljz = Q.ddz * lj;
-    }
-    xt = q_gather(Q.k, lix, ljx);
-    yt = q_gather(Q.k, liy, ljy);
-    zt = q_gather(Q.k, liz, ljz);
-    return quad_max(al);
+    double lambda = (Q.ddx * dvx + Q.ddy * dvy + Q.ddz * dvz) / Q.K;
+    lambda = Q.k < 3 ? lambda : 0.;
+    xt = q_exchange(Q, Q.ddx, lambda);
+    yt = q_exchange(Q, Q.ddy, lambda);
+    zt = q_exchange(Q, Q.ddz, lambda);
+    return quad_all(fabs(lambda) < tolvel);
 }
 
 // shake.c:418-436: one sweep's corrections, once per constraint membership.
@@ -296,11 +327,10 @@ __device__ inline void q_apply_v(Q3 &Q, double xt, double yt, double zt)
 {
 #pragma clang fp contract(off)
     if (Q.k >= 3) return;
-    for (int m = 0; m < 2; m++) {
-        Q.vx += xt * Q.nat;
-        Q.vy += yt * Q.nat;
-        Q.vz += zt * Q.nat;
-    }
+    const double jx = xt * Q.nat, jy = yt * Q.nat, jz = zt * Q.nat;
+    Q.vx = (Q.vx + jx) + jx;
+    Q.vy = (Q.vy + jy) + jy;
+    Q.vz = (Q.vz + jz) + jz;
 }
 
 __device__ inline void q_kick(const ShakeArgs &A, Q3 &Q)
@@ -312,6 +342,20 @@ __device__ inline void q_kick(const ShakeArgs &A, Q3 &Q)
     Q.vx += 0.5 * dt * A.fx[i] * Q.rm;
     Q.vy += 0.5 * dt * A.fy[i] * Q.rm;
     Q.vz += 0.5 * dt * A.fz[i] * Q.rm;
+}
+
+// Converged-sweep bits: the current 64-sweep word in a register, flushed to
+// the array when the word is complete or the loop ends.
+struct QBits {
+    unsigned long long w[SHK_MASKW];
+    unsigned long long cur;
+};
+
+__device__ inline void qb_flush(QBits &b, int word)
+{
+    for (int q = 0; q < SHK_MASKW; q++)
+        if (q == word) b.w[q] = b.cur;
+    b.cur = 0;
 }
 
 // The box of molecule g: the run's, or its replica's in a batch.
@@ -347,7 +391,8 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A)
     const int g = shake_w3_group(A);
     const int k = threadIdx.x & 3;
     // every lane reaches the wave-wide AND; idle molecules contribute all ones
-    unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
+    QBits bits;
+    for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = ~0ull;
     int kend = -1;
     bool settled = true;
     if (g < A.nGrp) {
@@ -356,26 +401,30 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A)
         q_load(A, g, k, Q);
         const double dt = A.P.timeStep;
         q_dd(A, B, Q);
+        q_setup_r(A, Q);
         q_kick(A, Q);
         if (k < 3) {
             Q.x += dt * Q.vx;
             Q.y += dt * Q.vy;
             Q.z += dt * Q.vz;
         }
+        q_shift_now(B, Q);
         q_scr_store(A, Q, 0, true);
-        for (int w = 0; w < SHK_MASKW; w++) bits[w] = 0;
+        for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = 0;
+        bits.cur = 0;
         double vir = 0.;
         int first = -1, it = 0;
         bool prev = false;
         settled = false;
         for (;;) {
             double dtx, dty, dtz, rt2;
-            const bool cv = q_check(B, Q, dtx, dty, dtz, rt2) < A.P.tolShake;
-            if (cv) bits[it >> 6] |= 1ull << (it & 63);
+            const bool cv = q_check(B, Q, dtx, dty, dtz, rt2);
+            if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) {
                 q_scr_store(A, Q, 1 + it - first, true);
-                if (k == 0) A.virScr[(long)(1 + it - first) * A.nGrp + g] = vir;
+                const double vm = q_sum3(0., vir);
+                if (k == 0) A.virScr[(long)(1 + it - first) * A.nGrp + g] = vm;
             }
             if (cv && prev) {
                 settled = true;
@@ -384,17 +433,20 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A)
             prev = cv;
             if (it + 1 >= A.P.maxCycle) break;
             q_apply_r(A, Q, dtx, dty, dtz, rt2, vir);
+            if ((it & 63) == 63) qb_flush(bits, it >> 6);
             it++;
         }
+        qb_flush(bits, it >> 6);
         q_scr_store(A, Q, SHK_W + 1, true);
+        const double vm = q_sum3(0., vir);
         if (k == 0) {
-            A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vir;
+            A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vm;
             A.first[g] = first;
             A.kend[g] = it;
         }
         kend = it;
     }
-    q_publish<BATCH>(A, g, bits, kend, settled);
+    q_publish<BATCH>(A, g, bits.w, kend, settled);
 }
 
 // Stage 1, part B: every molecule continues (or restarts from a saved
@@ -417,14 +469,18 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
     Q3 Q;
     q_load(A, g, k, Q);
     q_dd_load(A, Q);
+    q_setup_r(A, Q);
     q_scr_load(A, Q, slot, true);
-    double vir = slot == 0 ? 0. : A.virScr[(long)slot * A.nGrp + g];
+    q_shift_now(B, Q);
+    // the saved virial is the molecule's; it continues on lane 0
+    double vir = (slot == 0 || k != 0) ? 0. : A.virScr[(long)slot * A.nGrp + g];
     for (int it = at; it < C; it++) {
         double dtx, dty, dtz, rt2;
         q_check(B, Q, dtx, dty, dtz, rt2);
         q_apply_r(A, Q, dtx, dty, dtz, rt2, vir);
     }
-    if (k == 0) A.vir[g] = vir;
+    const double vm = q_sum3(0., vir);
+    if (k == 0) A.vir[g] = vm;
     q_store(A, Q, true);
 }
 
@@ -435,7 +491,8 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
 {
     const int g = shake_w3_group(A);
     const int k = threadIdx.x & 3;
-    unsigned long long bits[SHK_MASKW] = {~0ull, ~0ull, ~0ull, ~0ull};
+    QBits bits;
+    for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = ~0ull;
     int kend = -1;
     bool settled = true;
     if (g < A.nGrp) {
@@ -443,11 +500,12 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
         Q3 Q;
         q_load(A, g, k, Q);
         q_dd(A, B, Q);
+        q_setup_v(A, Q);
         q_kick(A, Q);
         q_scr_store(A, Q, 0, false);
-        const double den = q_den(A, Q);
         const double tolvel = A.P.tolShake * A.P.rTimeStep;
-        for (int w = 0; w < SHK_MASKW; w++) bits[w] = 0;
+        for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = 0;
+        bits.cur = 0;
         int first = -1, it = 0;
         bool prev = false;
         settled = false;
@@ -456,8 +514,8 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
             // sweep that is not the last one checked is applied, because the
             // reference's global loop goes on while any group is unconverged
             double xt, yt, zt;
-            const bool cv = q_lambda_v(A, Q, den, xt, yt, zt) < tolvel;
-            if (cv) bits[it >> 6] |= 1ull << (it & 63);
+            const bool cv = q_lambda_v(Q, tolvel, xt, yt, zt);
+            if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) q_scr_store(A, Q, 1 + it - first, false);
             if (cv && prev) {
@@ -467,8 +525,10 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
             prev = cv;
             if (it + 1 >= A.P.maxCycle) break;
             q_apply_v(Q, xt, yt, zt);
+            if ((it & 63) == 63) qb_flush(bits, it >> 6);
             it++;
         }
+        qb_flush(bits, it >> 6);
         q_scr_store(A, Q, SHK_W + 1, false);
         if (k == 0) {
             A.first[g] = first;
@@ -476,7 +536,7 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
         }
         kend = it;
     }
-    q_publish<BATCH>(A, g, bits, kend, settled);
+    q_publish<BATCH>(A, g, bits.w, kend, settled);
 }
 
 // Stage 2, part B: replay to the global sweep C, kinetic energy and stress
@@ -500,11 +560,11 @@ __global__ void __launch_bounds__(256) k_w3_vv2_b(ShakeArgs A)
             int slot, at;
             shake_pick_slot(C, first, kend, slot, at);
             q_dd_load(A, Q);
+            q_setup_v(A, Q);
             q_scr_load(A, Q, slot, false);
-            const double den = q_den(A, Q);
             for (int it = at; it < C; it++) {
                 double xt, yt, zt;
-                q_lambda_v(A, Q, den, xt, yt, zt);
+                q_lambda_v(Q, 0., xt, yt, zt);
                 q_apply_v(Q, xt, yt, zt);
             }
         }

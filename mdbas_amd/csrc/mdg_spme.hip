@@ -414,18 +414,80 @@ k_spme_gather_rows(MdgParams Pm, MdgBox B, int n, const double *__restrict__ x,
 
 __device__ inline int spme_top(int i, int m) { return (i >= 0 && i <= m) ? (i == m ? 0 : i) : -1; }
 
+// FP32 splines of one atom for a compile-time order, as stored by
+// k_spme_bin: nq float4 per dimension, values then derivatives.
+template <int P>
+struct SplineAtom {
+    static constexpr int NQ = (P + 3) / 4;
+    float b[3][4 * NQ], d[3][4 * NQ];
+    int base[3];
+};
+
+template <int P>
+__device__ inline void spline_load(const int4 *__restrict__ splI, const float4 *__restrict__ splB,
+                                   const float4 *__restrict__ splD, int n, int l, bool deriv,
+                                   SplineAtom<P> &S)
+{
+    constexpr int NQ = SplineAtom<P>::NQ;
+    const int4 bi = splI[l];
+    S.base[0] = bi.x;
+    S.base[1] = bi.y;
+    S.base[2] = bi.z;
+#pragma unroll
+    for (int dim = 0; dim < 3; dim++)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const float4 v = splB[(long)(dim * NQ + q) * n + l];
+            S.b[dim][4 * q] = v.x;
+            S.b[dim][4 * q + 1] = v.y;
+            S.b[dim][4 * q + 2] = v.z;
+            S.b[dim][4 * q + 3] = v.w;
+            if (deriv) {
+                const float4 w = splD[(long)(dim * NQ + q) * n + l];
+                S.d[dim][4 * q] = w.x;
+                S.d[dim][4 * q + 1] = w.y;
+                S.d[dim][4 * q + 2] = w.z;
+                S.d[dim][4 * q + 3] = w.w;
+            }
+        }
+}
+
+// Per atom: the scaled coordinates, the FP32 splines and derivatives
+// (stored once for the spread and the gather), and the bin of the top mesh
+// index (or the outlier list).
+template <int P>
 __global__ void __launch_bounds__(256)
 k_spme_bin(MdgParams Pm, MdgBox B, int n, int nby, int nbz, const double *__restrict__ x,
            const double *__restrict__ y, const double *__restrict__ z, int *binCnt,
            int *__restrict__ binOf, int *__restrict__ binRank, int *__restrict__ binOut,
-           int *binNOut)
+           int *binNOut, int4 *__restrict__ splI, float4 *__restrict__ splB,
+           float4 *__restrict__ splD)
 {
+    constexpr int NQ = SplineAtom<P>::NQ;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
-    double s1, s2, s3;
-    scaled_coords(B, Pm.m1, Pm.m2, Pm.m3, x[l], y[l], z[l], s1, s2, s3);
-    const int t1 = spme_top((int)s1, Pm.m1), t2 = spme_top((int)s2, Pm.m2),
-              t3 = spme_top((int)s3, Pm.m3);
+    double s[3];
+    scaled_coords(B, Pm.m1, Pm.m2, Pm.m3, x[l], y[l], z[l], s[0], s[1], s[2]);
+    const int i1 = (int)s[0], i2 = (int)s[1], i3 = (int)s[2];
+    splI[l] = make_int4(i1, i2, i3, 0);
+#pragma unroll
+    for (int dim = 0; dim < 3; dim++) {
+        float b[P], d[P], bq[4 * NQ], dq[4 * NQ];
+        bspline_fix<float, P>(s[dim], b, d);
+#pragma unroll
+        for (int k = 0; k < 4 * NQ; k++) {
+            bq[k] = k < P ? b[k] : 0.f;
+            dq[k] = k < P ? d[k] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            splB[(long)(dim * NQ + q) * n + l] =
+                make_float4(bq[4 * q], bq[4 * q + 1], bq[4 * q + 2], bq[4 * q + 3]);
+            splD[(long)(dim * NQ + q) * n + l] =
+                make_float4(dq[4 * q], dq[4 * q + 1], dq[4 * q + 2], dq[4 * q + 3]);
+        }
+    }
+    const int t1 = spme_top(i1, Pm.m1), t2 = spme_top(i2, Pm.m2), t3 = spme_top(i3, Pm.m3);
     if (t1 < 0 || t2 < 0 || t3 < 0) {
         binOut[atomicAdd(&binNOut[0], 1)] = l;
         binOf[l] = -1;
@@ -497,11 +559,11 @@ __device__ inline int spme_tile_bins(int X0, int m, int *bins)
 
 template <int P>
 __global__ void __launch_bounds__(256)
-k_spme_spread_tiles(MdgParams Pm, MdgBox B, int nty, int ntz, int nby, int nbz,
-                    const double *__restrict__ x, const double *__restrict__ y,
-                    const double *__restrict__ z, const double *__restrict__ q,
+k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
+                    const double *__restrict__ q,
                     const int *__restrict__ binStart, const int *__restrict__ binAtoms,
                     const int *__restrict__ binOut, const int *__restrict__ binNOut,
+                    const int4 *__restrict__ splI, const float4 *__restrict__ splB,
                     float *__restrict__ grid32)
 {
     constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
@@ -548,29 +610,24 @@ k_spme_spread_tiles(MdgParams Pm, MdgBox B, int nty, int ntz, int nby, int nbz,
             } else {
                 l = binOut[c - ncand];
             }
-            double s1, s2, s3;
-            scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
-            float b1[P], b2[P], b3[P], d[P];
-            bspline_fix<float, P>(s1, b1, d);
-            bspline_fix<float, P>(s2, b2, d);
-            bspline_fix<float, P>(s3, b3, d);
+            SplineAtom<P> S;
+            spline_load<P>(splI, splB, nullptr, Pm.n, l, false, S);
             const float ql = (float)q[l];
-            const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
 #pragma unroll
             for (int i = 0; i < P; i++) {
-                const int lx = wrap_index(i1 - i, m1) - X0;
+                const int lx = wrap_index(S.base[0] - i, m1) - X0;
                 if (lx < 0 || lx >= T) continue;
-                const float qa = ql * b1[i];
+                const float qa = ql * S.b[0][i];
 #pragma unroll
                 for (int j = 0; j < P; j++) {
-                    const int ly = wrap_index(i2 - j, m2) - Y0;
+                    const int ly = wrap_index(S.base[1] - j, m2) - Y0;
                     if (ly < 0 || ly >= T) continue;
-                    const float qb = qa * b2[j];
+                    const float qb = qa * S.b[1][j];
 #pragma unroll
                     for (int k = 0; k < P; k++) {
-                        const int lz = wrap_index(i3 - k, m3) - Z0;
+                        const int lz = wrap_index(S.base[2] - k, m3) - Z0;
                         if (lz < 0 || lz >= T) continue;
-                        const long long v = __double2ll_rn((double)(qb * b3[k]) * MDG_QSCALE);
+                        const long long v = __double2ll_rn((double)(qb * S.b[2][k]) * MDG_QSCALE);
                         atomicAdd(&acc[(lx * T + ly) * T + lz], (unsigned long long)v);
                     }
                 }
@@ -608,22 +665,27 @@ static int spme_bin_alloc(mdg_ctx *ctx)
     HIPCHK(hipMalloc(&ctx->binAtoms, n * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->binOut, n * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->binNOut, 2 * sizeof(int)));
+    const int nq = (ctx->P.nbsp + 3) / 4;
+    HIPCHK(hipMalloc(&ctx->splI, n * sizeof(int4)));
+    HIPCHK(hipMalloc(&ctx->splB, 3 * nq * (size_t)n * sizeof(float4)));
+    HIPCHK(hipMalloc(&ctx->splD, 3 * nq * (size_t)n * sizeof(float4)));
     RETCHK(mdg_fill_i32(ctx, ctx->binCnt, nb, 0));
     RETCHK(mdg_fill_i32(ctx, ctx->binNOut, 2, 0));
     return MDG_OK;
 }
 
 // Binned mixed spreading into grid32 (replaces the atomic spread and the
-// fixed-point conversion).
-static int spread_binned(mdg_ctx *ctx)
+// fixed-point conversion); leaves the per-atom splines for the gather.
+template <int P>
+static int spread_binned_p(mdg_ctx *ctx)
 {
     const int n = ctx->n;
     const int nb = ctx->nbx * ctx->nby * ctx->nbz;
     const int ntx = (ctx->m1 + SPME_TILE - 1) / SPME_TILE, nty = (ctx->m2 + SPME_TILE - 1) / SPME_TILE,
               ntz = (ctx->m3 + SPME_TILE - 1) / SPME_TILE;
-    hipLaunchKernelGGL(k_spme_bin, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, ctx->B, n,
-                       ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->binCnt, ctx->binOf,
-                       ctx->binRank, ctx->binOut, ctx->binNOut);
+    hipLaunchKernelGGL(k_spme_bin<P>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                       n, ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->binCnt, ctx->binOf,
+                       ctx->binRank, ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB, ctx->splD);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_spme_bin_scan, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->binCnt,
                        ctx->binStart, ctx->binNOut);
@@ -631,25 +693,108 @@ static int spread_binned(mdg_ctx *ctx)
     hipLaunchKernelGGL(k_spme_bin_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
                        ctx->binOf, ctx->binRank, ctx->binStart, ctx->binAtoms);
     LAUNCHCHK();
-    const dim3 g(ntx * nty * ntz), b(256);
-    switch (ctx->P.nbsp) {
-    case 4:
-        hipLaunchKernelGGL(k_spme_spread_tiles<4>, g, b, 0, ctx->stream, ctx->P, ctx->B, nty, ntz,
-                           ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->q, ctx->binStart,
-                           ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->grid32);
-        break;
-    case 6:
-        hipLaunchKernelGGL(k_spme_spread_tiles<6>, g, b, 0, ctx->stream, ctx->P, ctx->B, nty, ntz,
-                           ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->q, ctx->binStart,
-                           ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->grid32);
-        break;
-    default:
-        hipLaunchKernelGGL(k_spme_spread_tiles<8>, g, b, 0, ctx->stream, ctx->P, ctx->B, nty, ntz,
-                           ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->q, ctx->binStart,
-                           ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->grid32);
-    }
+    hipLaunchKernelGGL(k_spme_spread_tiles<P>, dim3(ntx * nty * ntz), dim3(256), 0, ctx->stream,
+                       ctx->P, nty, ntz, ctx->nby, ctx->nbz, ctx->q, ctx->binStart, ctx->binAtoms,
+                       ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB, ctx->grid32);
     LAUNCHCHK();
     return MDG_OK;
+}
+
+static int spread_binned(mdg_ctx *ctx)
+{
+    switch (ctx->P.nbsp) {
+    case 4: return spread_binned_p<4>(ctx);
+    case 6: return spread_binned_p<6>(ctx);
+    default: return spread_binned_p<8>(ctx);
+    }
+}
+
+// Mixed gather after a binned spread: lane r of the atom's row reads the
+// stored splines instead of evaluating them (k_spme_gather_rows otherwise).
+template <int P>
+__global__ void __launch_bounds__(256)
+k_spme_gather_pre(MdgParams Pm, MdgBox B, int n, const double *__restrict__ q,
+                  const int4 *__restrict__ splI, const float4 *__restrict__ splB,
+                  const float4 *__restrict__ splD, const float *__restrict__ grid, double fact1,
+                  float *__restrict__ frx, float *__restrict__ fry, float *__restrict__ frz,
+                  double *__restrict__ part)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)(t / SPME_LPA), r = (int)(t % SPME_LPA);
+    float fbx = 0.f, fby = 0.f, fbz = 0.f;
+    const int m1 = Pm.m1, m2 = Pm.m2, m3 = Pm.m3;
+    if (l < n) {
+        SplineAtom<P> S;
+        spline_load<P>(splI, splB, splD, n, l, true, S);
+        int kk[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) kk[k] = wrap_index(S.base[2] - k, m3);
+#pragma unroll
+        for (int ij0 = 0; ij0 < P * P; ij0 += SPME_LPA) {
+            const int ij = ij0 + r;
+            if (ij >= P * P) break;
+            const int i = ij / P, j = ij % P;
+            const long o2 =
+                (long)m3 * (wrap_index(S.base[1] - j, m2) + (long)m2 * wrap_index(S.base[0] - i, m1));
+            float bi = S.b[0][0], bj = S.b[1][0], di = S.d[0][0], dj = S.d[1][0];
+#pragma unroll
+            for (int k = 1; k < P; k++) {
+                bi = i == k ? S.b[0][k] : bi;
+                di = i == k ? S.d[0][k] : di;
+                bj = j == k ? S.b[1][k] : bj;
+                dj = j == k ? S.d[1][k] : dj;
+            }
+            const float c1 = di * bj, c2 = bi * dj, c3 = bi * bj;
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                const float qt = grid[kk[k] + o2];
+                fbx += qt * c1 * S.b[2][k];
+                fby += qt * c2 * S.b[2][k];
+                fbz += qt * c3 * S.d[2][k];
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < SPME_LPA; o <<= 1) {
+        fbx += __shfl_xor(fbx, o, SPME_LPA);
+        fby += __shfl_xor(fby, o, SPME_LPA);
+        fbz += __shfl_xor(fbz, o, SPME_LPA);
+    }
+    double acc[3] = {0, 0, 0};
+    if (l < n && r == 0) {
+        const double gx = (double)fbx * m1, gy = (double)fby * m2, gz = (double)fbz * m3;
+        const double c = fact1 * q[l];
+        const double fxl = c * (gx * B.u1 + gy * B.v1 + gz * B.w1);
+        const double fyl = c * (gx * B.u2 + gy * B.v2 + gz * B.w2);
+        const double fzl = c * (gx * B.u3 + gy * B.v3 + gz * B.w3);
+        frx[l] = (float)fxl;
+        fry[l] = (float)fyl;
+        frz[l] = (float)fzl;
+        acc[0] = (double)(float)fxl;
+        acc[1] = (double)(float)fyl;
+        acc[2] = (double)(float)fzl;
+    }
+    mdg_block_partials<3>(acc, part);
+}
+
+static void gather_pre(mdg_ctx *ctx, double fact1, float *frx, float *fry, float *frz,
+                       double *part)
+{
+    const int n = ctx->n;
+    const dim3 g(nblk((long)n * SPME_LPA, 256)), b(256);
+    switch (ctx->P.nbsp) {
+    case 4:
+        hipLaunchKernelGGL(k_spme_gather_pre<4>, g, b, 0, ctx->stream, ctx->P, ctx->B, n, ctx->q,
+                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part);
+        break;
+    case 6:
+        hipLaunchKernelGGL(k_spme_gather_pre<6>, g, b, 0, ctx->stream, ctx->P, ctx->B, n, ctx->q,
+                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part);
+        break;
+    default:
+        hipLaunchKernelGGL(k_spme_gather_pre<8>, g, b, 0, ctx->stream, ctx->P, ctx->B, n, ctx->q,
+                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part);
+    }
 }
 
 // Orders with a row-per-atom kernel; others use the thread-per-atom ones.
@@ -938,8 +1083,11 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
             ctx->err = "rocfft_execute (inverse) failed";
             return MDG_EHIP;
         }
-        launch_gather<true>(ctx, (const float *)ctx->grid32, fact1, (float *)frx, (float *)fry,
-                            (float *)frz, partGather);
+        if (spme_binned(ctx))
+            gather_pre(ctx, fact1, (float *)frx, (float *)fry, (float *)frz, partGather);
+        else
+            launch_gather<true>(ctx, (const float *)ctx->grid32, fact1, (float *)frx,
+                                (float *)fry, (float *)frz, partGather);
         LAUNCHCHK();
     } else {
         RETCHK(mdg_fill_f64(ctx, ctx->grid64, G, 0.0));
