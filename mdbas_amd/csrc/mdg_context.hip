@@ -436,9 +436,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->grid64, ctx->cgrid64, ctx->grid32, ctx->cgrid32, ctx->gridq,
                     ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
                     ctx->shkMask, ctx->exSymOff, ctx->exSymLst, ctx->adjQ, ctx->adjE,
-                    ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binStart, ctx->binOf,
-                    ctx->binRank, ctx->binAtoms, ctx->binOut, ctx->binNOut, ctx->splI,
-                    ctx->splB, ctx->splD};
+                    ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binBucket, ctx->binOut,
+                    ctx->binNOut, ctx->splI, ctx->splB, ctx->splD};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->bt) {

@@ -214,11 +214,10 @@ struct mdg_ctx {
     long long *gridq = nullptr;  // fixed-point spread mesh (mixed)
     // binned spreading (mixed): atoms sorted by mesh bin every step
     int nbx = 0, nby = 0, nbz = 0;   // bins per dimension (0: not used)
-    int *binCnt = nullptr;       // per bin; zero between steps
-    int *binStart = nullptr;     // exclusive scan of binCnt (nbins + 1)
-    int *binOf = nullptr, *binRank = nullptr, *binAtoms = nullptr;   // per atom
-    int *binOut = nullptr;       // atoms whose mesh points are not one run
-    int *binNOut = nullptr;      // [0] being counted, [1] this step's count
+    int *binCnt = nullptr;       // atoms per bin; zero between steps
+    int *binBucket = nullptr;    // per bin: up to SPME_BCAP atom indices
+    int *binOut = nullptr;       // atoms not in a bucket (points not one run, full bucket)
+    int *binNOut = nullptr;      // [0] count of binOut; zero between steps
     int4 *splI = nullptr;        // per atom: spline base index per dimension (binned path)
     float4 *splB = nullptr;      // per atom: FP32 spline values [3 * nq][n], nq = (order + 3) / 4
     float4 *splD = nullptr;      // per atom: their derivatives, same layout
@@ -449,7 +448,8 @@ int mdg_spme_setup(mdg_ctx *ctx);
 int mdg_spme_free(mdg_ctx *ctx);
 int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8]);
 int mdg_spme_device(mdg_ctx *ctx);   // device-resident: frx.. per atom, partials in epart
-int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *));   // runs fn's launches on stream2
+int mdg_spme_fork(mdg_ctx *ctx);   // marks the point stream2's work starts from
+int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *));   // runs fn's launches on stream2 from it
 int mdg_gather_blocks(const mdg_ctx *ctx);   // block partials of the reciprocal forces
 int mdg_spme_pot(mdg_ctx *ctx, double *pot);   // FP64 dE_rec/dq per atom
 int mdg_adj_pair(mdg_ctx *ctx);      // real-space dE/d(q, eps, sig) into fdx/fdy/fdz slots

@@ -259,8 +259,9 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
 {
     if (ctx->bt) return b_energy(ctx, rebuild);
     const int n = ctx->n;
-    // SPME on the second stream, beside the list rebuild and the pair kernel
-    RETCHK(mdg_spme_on(ctx, mdg_spme_device));
+    // SPME on the second stream, beside the list rebuild and the pair kernel;
+    // the pair branch is enqueued first so that it starts at the fork
+    RETCHK(mdg_spme_fork(ctx));
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
         RETCHK(mdg_build_cluster_list(ctx));
@@ -269,6 +270,7 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
     }
     RETCHK(mdg_pair_cluster(ctx));
+    RETCHK(mdg_spme_on(ctx, mdg_spme_device));
     if (ctx->spmeForked) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     const bool mixed = ctx->precision == MDG_MIXED;
     FinArgs F;
@@ -823,7 +825,7 @@ static int b_energy(mdg_ctx *ctx, bool rebuild)
 {
     MdgBatch &T = *ctx->bt;
     const int n = ctx->n;
-    RETCHK(mdg_spme_on(ctx, mdg_b_spme));
+    RETCHK(mdg_spme_fork(ctx));
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
         RETCHK(mdg_b_build_list(ctx));
@@ -832,6 +834,7 @@ static int b_energy(mdg_ctx *ctx, bool rebuild)
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
     }
     RETCHK(mdg_b_pair(ctx));
+    RETCHK(mdg_spme_on(ctx, mdg_b_spme));
     if (ctx->spmeForked) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     BFinArgs F;
     F.P = ctx->P;

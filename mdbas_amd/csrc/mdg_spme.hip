@@ -396,21 +396,24 @@ k_spme_gather_rows(MdgParams Pm, MdgBox B, int n, const double *__restrict__ x,
 }
 
 // Binned spreading without global atomics (mixed).  Every step the atoms
-// are sorted into bins of SPME_BIN^3 mesh points by the top mesh index of
-// their spline support (k_spme_bin, k_spme_bin_scan, k_spme_bin_place).
+// go into buckets of up to SPME_BCAP atoms, one per bin of SPME_BIN^3 mesh
+// points, by the top mesh index of their spline support (k_spme_bin).
 // One block then owns a tile of SPME_TILE^3 mesh points: it reads the
 // atoms of the bins whose support can reach the tile, adds their
 // fixed-point contributions inside the tile in LDS (integer, so the sum
 // does not depend on the order), and writes the tile's FP32 mesh values.
 // Each contribution is formed as in k_spme_spread, so the mesh is the
-// same bit for bit, and no global clearing or conversion pass is needed.
+// same bit for bit, and no global clearing or conversion pass is needed;
+// the gather clears the bucket counts for the next step.
 //
 // An atom's P mesh indices per dimension are the run top, top - 1, ...,
 // top - P + 1 (mod m) with top = wrap_index(i, m) whenever the spline base
-// i lies in [0, m]; an atom outside that (more than one mesh spacing out
-// of the box) goes to a short list every tile checks point by point.
+// i lies in [0, m].  An atom outside that (more than one mesh spacing out
+// of the box), or one that finds its bucket full, goes to a short list
+// every tile checks point by point.
 #define SPME_BIN 4
 #define SPME_TILE 8
+#define SPME_BCAP 32
 
 __device__ inline int spme_top(int i, int m) { return (i >= 0 && i <= m) ? (i == m ? 0 : i) : -1; }
 
@@ -453,15 +456,14 @@ __device__ inline void spline_load(const int4 *__restrict__ splI, const float4 *
 }
 
 // Per atom: the scaled coordinates, the FP32 splines and derivatives
-// (stored once for the spread and the gather), and the bin of the top mesh
-// index (or the outlier list).
+// (stored once for the spread and the gather), and the bucket of the top
+// mesh index (or the outlier list).
 template <int P>
 __global__ void __launch_bounds__(256)
 k_spme_bin(MdgParams Pm, MdgBox B, int n, int nby, int nbz, const double *__restrict__ x,
            const double *__restrict__ y, const double *__restrict__ z, int *binCnt,
-           int *__restrict__ binOf, int *__restrict__ binRank, int *__restrict__ binOut,
-           int *binNOut, int4 *__restrict__ splI, float4 *__restrict__ splB,
-           float4 *__restrict__ splD)
+           int *__restrict__ bucket, int *__restrict__ binOut, int *binNOut,
+           int4 *__restrict__ splI, float4 *__restrict__ splB, float4 *__restrict__ splD)
 {
     constexpr int NQ = SplineAtom<P>::NQ;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
@@ -488,56 +490,15 @@ k_spme_bin(MdgParams Pm, MdgBox B, int n, int nby, int nbz, const double *__rest
         }
     }
     const int t1 = spme_top(i1, Pm.m1), t2 = spme_top(i2, Pm.m2), t3 = spme_top(i3, Pm.m3);
-    if (t1 < 0 || t2 < 0 || t3 < 0) {
-        binOut[atomicAdd(&binNOut[0], 1)] = l;
-        binOf[l] = -1;
-        return;
+    if (t1 >= 0 && t2 >= 0 && t3 >= 0) {
+        const int b = ((t1 / SPME_BIN) * nby + t2 / SPME_BIN) * nbz + t3 / SPME_BIN;
+        const int r = atomicAdd(&binCnt[b], 1);
+        if (r < SPME_BCAP) {
+            bucket[(long)b * SPME_BCAP + r] = l;
+            return;
+        }
     }
-    const int b = ((t1 / SPME_BIN) * nby + t2 / SPME_BIN) * nbz + t3 / SPME_BIN;
-    binOf[l] = b;
-    binRank[l] = atomicAdd(&binCnt[b], 1);
-}
-
-// One block: exclusive scan of the bin counts, which it clears for the next
-// step; the outlier count moves to binNOut[1].
-__global__ void __launch_bounds__(1024)
-k_spme_bin_scan(int nb, int *binCnt, int *__restrict__ binStart, int *binNOut)
-{
-    __shared__ int part[1024];
-    const int t = threadIdx.x, per = (nb + 1023) / 1024;
-    const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
-    int s = 0;
-    for (int b = b0; b < b1; b++) s += binCnt[b];
-    part[t] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int v = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int run = part[t] - s;
-    for (int b = b0; b < b1; b++) {
-        const int c = binCnt[b];
-        binStart[b] = run;
-        run += c;
-        binCnt[b] = 0;
-    }
-    if (t == 1023) binStart[nb] = part[1023];
-    if (t == 0) {
-        binNOut[1] = binNOut[0];
-        binNOut[0] = 0;
-    }
-}
-
-__global__ void k_spme_bin_place(int n, const int *__restrict__ binOf,
-                                 const int *__restrict__ binRank, const int *__restrict__ binStart,
-                                 int *__restrict__ binAtoms)
-{
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
-    const int b = binOf[l];
-    if (b >= 0) binAtoms[binStart[b] + binRank[l]] = l;
+    binOut[atomicAdd(&binNOut[0], 1)] = l;
 }
 
 // The bins (at most SPME_TILE / SPME_BIN + 1 + (P - 1) / SPME_BIN + 1 per
@@ -560,41 +521,56 @@ __device__ inline int spme_tile_bins(int X0, int m, int *bins)
 template <int P>
 __global__ void __launch_bounds__(256)
 k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
-                    const double *__restrict__ q,
-                    const int *__restrict__ binStart, const int *__restrict__ binAtoms,
-                    const int *__restrict__ binOut, const int *__restrict__ binNOut,
-                    const int4 *__restrict__ splI, const float4 *__restrict__ splB,
-                    float *__restrict__ grid32)
+                    const double *__restrict__ q, const int *__restrict__ binCnt,
+                    const int *__restrict__ bucket, const int *__restrict__ binOut,
+                    const int *__restrict__ binNOut, const int4 *__restrict__ splI,
+                    const float4 *__restrict__ splB, float *__restrict__ grid32)
 {
     constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
+    constexpr int NB3 = NBD * NBD * NBD;
+    static_assert(NB3 <= 128, "bin list of a tile exceeds the scan width");
     __shared__ unsigned long long acc[T * T * T];
     __shared__ int bx[NBD], by[NBD], bz[NBD], nbd[3];
-    __shared__ int off[NBD * NBD * NBD + 1], bbeg[NBD * NBD * NBD];
+    __shared__ int off[NB3 + 1], bid[NB3];
     const int m1 = Pm.m1, m2 = Pm.m2, m3 = Pm.m3;
     const int tile = blockIdx.x;
     const int tx = tile / (nty * ntz), ty = (tile / ntz) % nty, tz = tile % ntz;
     const int X0 = tx * T, Y0 = ty * T, Z0 = tz * T;
     for (int i = threadIdx.x; i < T * T * T; i += blockDim.x) acc[i] = 0ull;
-    if (threadIdx.x == 0) {
-        nbd[0] = spme_tile_bins<P>(X0, m1, bx);
-        nbd[1] = spme_tile_bins<P>(Y0, m2, by);
-        nbd[2] = spme_tile_bins<P>(Z0, m3, bz);
-    }
+    if (threadIdx.x == 0) nbd[0] = spme_tile_bins<P>(X0, m1, bx);
+    if (threadIdx.x == 1) nbd[1] = spme_tile_bins<P>(Y0, m2, by);
+    if (threadIdx.x == 2) nbd[2] = spme_tile_bins<P>(Z0, m3, bz);
     __syncthreads();
     const int nbt = nbd[0] * nbd[1] * nbd[2];
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int c = 0; c < nbt; c++) {
-            const int ix = c / (nbd[1] * nbd[2]), iy = (c / nbd[2]) % nbd[1], iz = c % nbd[2];
-            const int b = (bx[ix] * nby + by[iy]) * nbz + bz[iz];
-            off[c] = run;
-            bbeg[c] = binStart[b];
-            run += binStart[b + 1] - binStart[b];
+    if (threadIdx.x < 64) {
+        // the first wave: bucket sizes of the tile's bins (two per lane) and
+        // their exclusive prefix sums
+        const int lane = threadIdx.x;
+        int cnt[2] = {0, 0};
+        for (int h = 0; h < 2; h++) {
+            const int c = lane + 64 * h;
+            if (c < nbt) {
+                const int ix = c / (nbd[1] * nbd[2]), iy = (c / nbd[2]) % nbd[1], iz = c % nbd[2];
+                const int b = (bx[ix] * nby + by[iy]) * nbz + bz[iz];
+                bid[c] = b;
+                cnt[h] = min(binCnt[b], SPME_BCAP);
+            }
         }
-        off[nbt] = run;
+        int run = 0;
+        for (int h = 0; h < 2; h++) {
+            int v = cnt[h];
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(v, o, 64);
+                if (lane >= o) v += u;
+            }
+            const int c = lane + 64 * h;
+            if (c < nbt) off[c] = run + v - cnt[h];
+            run += __shfl(v, 63, 64);
+        }
+        if (lane == 0) off[nbt] = run;
     }
     __syncthreads();
-    const int ncand = off[nbt], nout = binNOut[1];
+    const int ncand = off[nbt], nout = binNOut[0];
     for (int c0 = 0; c0 < ncand + nout; c0 += blockDim.x) {
         const int c = c0 + threadIdx.x;
         if (c < ncand + nout) {
@@ -606,7 +582,7 @@ k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
                     if (off[mid] <= c) lo = mid;
                     else hi = mid;
                 }
-                l = binAtoms[bbeg[lo] + (c - off[lo])];
+                l = bucket[(long)bid[lo] * SPME_BCAP + (c - off[lo])];
             } else {
                 l = binOut[c - ncand];
             }
@@ -659,12 +635,11 @@ static int spme_bin_alloc(mdg_ctx *ctx)
     const long nb = (long)ctx->nbx * ctx->nby * ctx->nbz;
     const int n = ctx->n;
     HIPCHK(hipMalloc(&ctx->binCnt, nb * sizeof(int)));
-    HIPCHK(hipMalloc(&ctx->binStart, (nb + 1) * sizeof(int)));
-    HIPCHK(hipMalloc(&ctx->binOf, n * sizeof(int)));
-    HIPCHK(hipMalloc(&ctx->binRank, n * sizeof(int)));
-    HIPCHK(hipMalloc(&ctx->binAtoms, n * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binBucket, nb * SPME_BCAP * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->binOut, n * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->binNOut, 2 * sizeof(int)));
+    RETCHK(mdg_fill_i32(ctx, ctx->binCnt, nb, 0));
+    RETCHK(mdg_fill_i32(ctx, ctx->binNOut, 2, 0));
     const int nq = (ctx->P.nbsp + 3) / 4;
     HIPCHK(hipMalloc(&ctx->splI, n * sizeof(int4)));
     HIPCHK(hipMalloc(&ctx->splB, 3 * nq * (size_t)n * sizeof(float4)));
@@ -680,21 +655,14 @@ template <int P>
 static int spread_binned_p(mdg_ctx *ctx)
 {
     const int n = ctx->n;
-    const int nb = ctx->nbx * ctx->nby * ctx->nbz;
     const int ntx = (ctx->m1 + SPME_TILE - 1) / SPME_TILE, nty = (ctx->m2 + SPME_TILE - 1) / SPME_TILE,
               ntz = (ctx->m3 + SPME_TILE - 1) / SPME_TILE;
     hipLaunchKernelGGL(k_spme_bin<P>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
-                       n, ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->binCnt, ctx->binOf,
-                       ctx->binRank, ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB, ctx->splD);
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_spme_bin_scan, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->binCnt,
-                       ctx->binStart, ctx->binNOut);
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_spme_bin_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
-                       ctx->binOf, ctx->binRank, ctx->binStart, ctx->binAtoms);
+                       n, ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->binCnt, ctx->binBucket,
+                       ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB, ctx->splD);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_spme_spread_tiles<P>, dim3(ntx * nty * ntz), dim3(256), 0, ctx->stream,
-                       ctx->P, nty, ntz, ctx->nby, ctx->nbz, ctx->q, ctx->binStart, ctx->binAtoms,
+                       ctx->P, nty, ntz, ctx->nby, ctx->nbz, ctx->q, ctx->binCnt, ctx->binBucket,
                        ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB, ctx->grid32);
     LAUNCHCHK();
     return MDG_OK;
@@ -717,9 +685,13 @@ k_spme_gather_pre(MdgParams Pm, MdgBox B, int n, const double *__restrict__ q,
                   const int4 *__restrict__ splI, const float4 *__restrict__ splB,
                   const float4 *__restrict__ splD, const float *__restrict__ grid, double fact1,
                   float *__restrict__ frx, float *__restrict__ fry, float *__restrict__ frz,
-                  double *__restrict__ part)
+                  double *__restrict__ part, int nbins, int *__restrict__ binCnt,
+                  int *__restrict__ binNOut)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // the spread tiles have read the buckets: clear them for the next step
+    for (long b = t; b < nbins; b += (long)gridDim.x * blockDim.x) binCnt[b] = 0;
+    if (t == 0) binNOut[0] = 0;
     const int l = (int)(t / SPME_LPA), r = (int)(t % SPME_LPA);
     float fbx = 0.f, fby = 0.f, fbz = 0.f;
     const int m1 = Pm.m1, m2 = Pm.m2, m3 = Pm.m3;
@@ -780,20 +752,23 @@ k_spme_gather_pre(MdgParams Pm, MdgBox B, int n, const double *__restrict__ q,
 static void gather_pre(mdg_ctx *ctx, double fact1, float *frx, float *fry, float *frz,
                        double *part)
 {
-    const int n = ctx->n;
+    const int n = ctx->n, nbins = ctx->nbx * ctx->nby * ctx->nbz;
     const dim3 g(nblk((long)n * SPME_LPA, 256)), b(256);
     switch (ctx->P.nbsp) {
     case 4:
         hipLaunchKernelGGL(k_spme_gather_pre<4>, g, b, 0, ctx->stream, ctx->P, ctx->B, n, ctx->q,
-                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part);
+                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part,
+                           nbins, ctx->binCnt, ctx->binNOut);
         break;
     case 6:
         hipLaunchKernelGGL(k_spme_gather_pre<6>, g, b, 0, ctx->stream, ctx->P, ctx->B, n, ctx->q,
-                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part);
+                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part,
+                           nbins, ctx->binCnt, ctx->binNOut);
         break;
     default:
         hipLaunchKernelGGL(k_spme_gather_pre<8>, g, b, 0, ctx->stream, ctx->P, ctx->B, n, ctx->q,
-                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part);
+                           ctx->splI, ctx->splB, ctx->splD, ctx->grid32, fact1, frx, fry, frz, part,
+                           nbins, ctx->binCnt, ctx->binNOut);
     }
 }
 
@@ -1529,13 +1504,18 @@ int mdg_b_spme(mdg_ctx *ctx)
 // Runs fn (an SPME evaluation) on the context's second stream, forked from
 // and joined back into the main stream by events; rocFFT follows the
 // stream.  Without a second stream, or while profiling, fn runs in place.
-int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *))
+int mdg_spme_fork(mdg_ctx *ctx)
 {
     // profiled steps keep one stream, so each kernel's events time it alone
     ctx->spmeForked = ctx->stream2 && !ctx->profiling;
+    if (ctx->spmeForked) HIPCHK(hipEventRecord(ctx->evFork, ctx->stream));
+    return MDG_OK;
+}
+
+int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *))
+{
     if (!ctx->spmeForked) return fn(ctx);
     hipStream_t main = ctx->stream;
-    HIPCHK(hipEventRecord(ctx->evFork, main));
     HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->evFork, 0));
     ctx->stream = ctx->stream2;
     rocfft_execution_info_set_stream(ctx->info, ctx->stream2);
