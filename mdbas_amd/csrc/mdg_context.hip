@@ -204,6 +204,10 @@ int mdg_upload_exclusions(mdg_ctx *ctx)
     }
     RETCHK(h2d(ctx, ctx->exOff, ctx->hExOff.data(), ctx->n + 1));
     RETCHK(h2d(ctx, ctx->exLst, ctx->hExLst.data(), ne));
+    ctx->exSpan = 0;
+    for (int i = 0; i < ctx->n; i++)
+        for (int k = ctx->hExOff[i]; k < ctx->hExOff[i + 1]; k++)
+            ctx->exSpan = std::max(ctx->exSpan, std::abs(ctx->hExLst[k] - i));
     // Symmetric copy for the per-atom correction in the device-resident
     // step: every pair (i, j > i) of row i, with its multiplicity, is listed
     // under both atoms (the same pairs the reference's correction loop
@@ -429,7 +433,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->cl.atomSlot, ctx->cl.bb, ctx->cl.bbh, ctx->cl.cpOff, ctx->cl.cpJ,
                     ctx->cl.cpMask, ctx->cl.cpI, ctx->cl.tmpCnt, ctx->cl.tmpJ, ctx->cl.tmpMask,
                     ctx->cl.colCnt, ctx->cl.colStart, ctx->cl.colPad, ctx->cl.colSlot,
-                    ctx->cl.sortTmp, ctx->cl.sortKey, ctx->cl.cluCol, ctx->sx, ctx->sy, ctx->sz, ctx->sp32,
+                    ctx->cl.sortTmp, ctx->cl.sortKey, ctx->cl.cluCol, ctx->cl.px, ctx->cl.py, ctx->cl.pz,
+                    ctx->cl.pcell, ctx->sx, ctx->sy, ctx->sz, ctx->sp32,
                     ctx->sq, ctx->seps, ctx->ssig, ctx->fdx, ctx->fdy, ctx->fdz, ctx->scal,
                     ctx->fix, ctx->frx, ctx->fry, ctx->frz, ctx->frx64,
                     ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,

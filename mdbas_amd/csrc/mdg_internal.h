@@ -81,6 +81,10 @@ struct ClusterList {
     int ncol = 0, nfx = 0, nfy = 0;
     int *colCnt = nullptr, *colStart = nullptr, *colPad = nullptr, *colSlot = nullptr;
     int *cluCol = nullptr;       // column of each cluster (-1 unused)
+    // per slot, for the list predicate: the atom's predicate coordinates
+    // (scaled for link cells, else positions) and link cell
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    int *pcell = nullptr;
     int *sortTmp = nullptr;
     float *sortKey = nullptr;
 };
@@ -172,6 +176,7 @@ struct mdg_ctx {
     // exclusions (CSR; symmetric for link cells, ordered otherwise)
     int *exOff = nullptr, *exLst = nullptr;
     int nExcl = 0;
+    int exSpan = 0;              // largest |i - j| of an excluded pair (the list skips the lookup beyond)
     int *exSymOff = nullptr, *exSymLst = nullptr;   // both orientations of every pair
     int nExSym = 0;
     // parameter adjoints dE/d(q, eps, sig) and the SPME mesh potential (mdg_adjoint.hip)

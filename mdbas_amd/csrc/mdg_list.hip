@@ -30,6 +30,44 @@ __device__ inline bool mdg_excluded(const int *__restrict__ exOff, const int *__
     return false;
 }
 
+// Block-aggregated counting: the atoms of one block that share a key are
+// counted in LDS and added to the global counter with one atomic per key
+// and block (thousands of atoms per cell or column would otherwise queue on
+// a few global counters).  Every thread of the block must call these, with
+// key = -1 when it has nothing to count.
+#define MDG_AGG_KEYS 2048
+
+__device__ inline void block_count(int key, int nkeys, int *cnt)
+{
+    __shared__ int h[MDG_AGG_KEYS];
+    if (nkeys > MDG_AGG_KEYS) {
+        if (key >= 0) atomicAdd(&cnt[key], 1);
+        return;
+    }
+    for (int i = threadIdx.x; i < nkeys; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    if (key >= 0) atomicAdd(&h[key], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nkeys; i += blockDim.x)
+        if (h[i]) atomicAdd(&cnt[i], h[i]);
+}
+
+// The thread's position among all atoms of its key (cursor[key] counts the
+// positions handed out), in no particular order.
+__device__ inline int block_rank(int key, int nkeys, int *cursor)
+{
+    __shared__ int h[MDG_AGG_KEYS], base[MDG_AGG_KEYS];
+    if (nkeys > MDG_AGG_KEYS) return key >= 0 ? atomicAdd(&cursor[key], 1) : 0;
+    for (int i = threadIdx.x; i < nkeys; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int r = key >= 0 ? atomicAdd(&h[key], 1) : 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nkeys; i += blockDim.x)
+        if (h[i]) base[i] = atomicAdd(&cursor[i], h[i]);
+    __syncthreads();
+    return key >= 0 ? base[key] + r : 0;
+}
+
 // utils.c:162-218 (image_update) on the device.
 __global__ void k_image_update(MdgBox B, int n, double *x, double *y, double *z)
 {
@@ -140,33 +178,37 @@ __global__ void k_lc_assign(MdgParams P, MdgBox B, int nx, int ny, int nz,
                             int *__restrict__ cellOf, int *__restrict__ cellCnt)
 {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    double a = (x[i] * B.u1 + y[i] * B.u2 + z[i] * B.u3) + 0.5;
-    double b = (x[i] * B.v1 + y[i] * B.v2 + z[i] * B.v3) + 0.5;
-    double c = (x[i] * B.w1 + y[i] * B.w2 + z[i] * B.w3) + 0.5;
-    xu[i] = a;
-    yu[i] = b;
-    zu[i] = c;
-    int ix = (int)((double)nx * a), iy = (int)((double)ny * b), iz = (int)((double)nz * c);
-    ix = ix <= nx - 1 ? ix : nx - 1;
-    iy = iy <= ny - 1 ? iy : ny - 1;
-    iz = iz <= nz - 1 ? iz : nz - 1;
-    int cell = ix + nx * (iy + ny * iz);
-    cellOf[i] = cell;
-    atomicAdd(&cellCnt[cell], 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int cell = -1;
+    if (i < P.n) {
+        double a = (x[i] * B.u1 + y[i] * B.u2 + z[i] * B.u3) + 0.5;
+        double b = (x[i] * B.v1 + y[i] * B.v2 + z[i] * B.v3) + 0.5;
+        double c = (x[i] * B.w1 + y[i] * B.w2 + z[i] * B.w3) + 0.5;
+        xu[i] = a;
+        yu[i] = b;
+        zu[i] = c;
+        int ix = (int)((double)nx * a), iy = (int)((double)ny * b), iz = (int)((double)nz * c);
+        ix = ix <= nx - 1 ? ix : nx - 1;
+        iy = iy <= ny - 1 ? iy : ny - 1;
+        iz = iz <= nz - 1 ? iz : nz - 1;
+        cell = ix + nx * (iy + ny * iz);
+        cellOf[i] = cell;
+    }
+    // the counts serve the reference-format list; the cluster list needs
+    // only the cells
+    if (cellCnt) block_count(cell, nx * ny * nz, cellCnt);
 }
 
 // Place (cell, descending atom) keys into the cell segments; the segment
 // sort then orders them, giving head/link chain order (list.c:1958-1980).
-__global__ void k_lc_place(int n, const int *__restrict__ cellOf, const int *__restrict__ cellStart,
-                           int *__restrict__ cursor, unsigned long long *__restrict__ keys)
+__global__ void k_lc_place(int n, int ncell, const int *__restrict__ cellOf,
+                           const int *__restrict__ cellStart, int *__restrict__ cursor,
+                           unsigned long long *__restrict__ keys)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int c = cellOf[i];
-    int p = cellStart[c] + atomicAdd(&cursor[c], 1);
-    keys[p] = (unsigned long long)(0x7fffffff - i);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = i < n ? cellOf[i] : -1;
+    const int r = block_rank(c, ncell, cursor);
+    if (c >= 0) keys[cellStart[c] + r] = (unsigned long long)(0x7fffffff - i);
 }
 
 __global__ void k_lc_order(int n, const unsigned long long *__restrict__ keys, int *__restrict__ order,
@@ -272,8 +314,8 @@ static int lc_sort(mdg_ctx *ctx, unsigned long long **keys, double **sxu, double
     LAUNCHCHK();
     RETCHK(mdg_scan_int(ctx, ctx->cellCnt, ctx->cellStart, ctx->ncell));
     int *cursor = ctx->cellCnt + ctx->ncell + 1;
-    hipLaunchKernelGGL(k_lc_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, ctx->cellOf,
-                       ctx->cellStart, cursor, *keys);
+    hipLaunchKernelGGL(k_lc_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, ctx->ncell,
+                       ctx->cellOf, ctx->cellStart, cursor, *keys);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_segment_sort, dim3(ctx->ncell), dim3(256), 0, ctx->stream, *keys,
                        ctx->cellStart, ctx->ctrl);
@@ -362,30 +404,32 @@ __global__ void k_col_assign(MdgBox B, int n, int ncx, int ncy, const double *__
                              int *__restrict__ colOf, int *__restrict__ colCnt,
                              float *__restrict__ zkey)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double a = x[i] * B.u1 + y[i] * B.u2 + z[i] * B.u3 + 0.5;
-    double b = x[i] * B.v1 + y[i] * B.v2 + z[i] * B.v3 + 0.5;
-    double c = x[i] * B.w1 + y[i] * B.w2 + z[i] * B.w3 + 0.5;
-    a -= floor(a);
-    b -= floor(b);
-    c -= floor(c);
-    int cx = min((int)(a * ncx), ncx - 1), cy = min((int)(b * ncy), ncy - 1);
-    int col = cx + ncx * cy;
-    colOf[i] = col;
-    zkey[i] = (float)c;
-    atomicAdd(&colCnt[col], 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int col = -1;
+    if (i < n) {
+        double a = x[i] * B.u1 + y[i] * B.u2 + z[i] * B.u3 + 0.5;
+        double b = x[i] * B.v1 + y[i] * B.v2 + z[i] * B.v3 + 0.5;
+        double c = x[i] * B.w1 + y[i] * B.w2 + z[i] * B.w3 + 0.5;
+        a -= floor(a);
+        b -= floor(b);
+        c -= floor(c);
+        int cx = min((int)(a * ncx), ncx - 1), cy = min((int)(b * ncy), ncy - 1);
+        col = cx + ncx * cy;
+        colOf[i] = col;
+        zkey[i] = (float)c;
+    }
+    block_count(col, ncx * ncy, colCnt);
 }
 
-__global__ void k_col_place(int n, const int *__restrict__ colOf, const int *__restrict__ colStart,
-                            int *__restrict__ cursor, const float *__restrict__ zkey,
-                            unsigned long long *__restrict__ keys)
+__global__ void k_col_place(int n, int ncol, const int *__restrict__ colOf,
+                            const int *__restrict__ colStart, int *__restrict__ cursor,
+                            const float *__restrict__ zkey, unsigned long long *__restrict__ keys)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int c = colOf[i];
-    int p = colStart[c] + atomicAdd(&cursor[c], 1);
-    keys[p] = ((unsigned long long)mdg_float_order(zkey[i]) << 32) | (unsigned int)i;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = i < n ? colOf[i] : -1;
+    const int r = block_rank(c, ncol, cursor);
+    if (c >= 0)
+        keys[colStart[c] + r] = ((unsigned long long)mdg_float_order(zkey[i]) << 32) | (unsigned int)i;
 }
 
 __global__ void k_col_pad(int ncol, const int *__restrict__ colCnt, int *__restrict__ colPad)
@@ -495,6 +539,79 @@ __device__ inline bool ref_pair_pred(const MdgParams &P, const MdgBox &B, int ke
     return !mdg_excluded(exOff, exLst, min(i, j), max(i, j));
 }
 
+// ref_pair_pred on slot-ordered copies of the predicate inputs (the same
+// FP64 expressions): atoms ia, ja with predicate coordinates (scaled for
+// link cells, else positions) and link cells ci, cj.  The exclusion table
+// is searched only when |ia - ja| is within the largest excluded span, and
+// frozen flags only when the system has frozen atoms.
+__device__ inline bool ref_pair_pred_slot(const MdgParams &P, const MdgBox &B, int keyLink, int nx,
+                                          int ny, int nz, int ia, int ja, double xi, double yi,
+                                          double zi, int ci, double xj, double yj, double zj,
+                                          int cj, bool hasFrozen, const int *__restrict__ frozen,
+                                          const int *__restrict__ exOff,
+                                          const int *__restrict__ exLst, int exSpan)
+{
+#pragma clang fp contract(off)
+    double r2;
+    if (keyLink) {
+        int ix = ci % nx, iy = (ci / nx) % ny, iz = ci / (nx * ny);
+        int jx = cj % nx, jy = (cj / nx) % ny, jz = cj / (nx * ny);
+        int d[3] = {jx - ix, jy - iy, jz - iz};
+        int nn[3] = {nx, ny, nz};
+        double c[3];
+        for (int k = 0; k < 3; k++) {
+            if (d[k] == 0 || d[k] == 1 || d[k] == -1) c[k] = 0.;
+            else if (d[k] == -(nn[k] - 1)) c[k] = 1.;     // j's cell wrapped past the top
+            else if (d[k] == nn[k] - 1) c[k] = -1.;       // j's cell wrapped past the bottom
+            else return false;                             // not neighbouring cells
+        }
+        double xt = xj - xi + c[0];
+        double yt = yj - yi + c[1];
+        double zt = zj - zi + c[2];
+        double xd = xt * B.a1 + yt * B.b1 + zt * B.c1;
+        double yd = xt * B.a2 + yt * B.b2 + zt * B.c2;
+        double zd = xt * B.a3 + yt * B.b3 + zt * B.c3;
+        r2 = xd * xd + yd * yd + zd * zd;
+    } else {
+        // x[hi] - x[lo], as the reference orders the pair
+        double dx, dy, dz;
+        if (ia < ja) {
+            dx = xj - xi;
+            dy = yj - yi;
+            dz = zj - zi;
+        } else {
+            dx = xi - xj;
+            dy = yi - yj;
+            dz = zi - zj;
+        }
+        r2 = mdg_dist(B, dx, dy, dz);
+    }
+    if (!(r2 <= P.cutnb2)) return false;
+    if (hasFrozen && frozen[ia] * frozen[ja]) return false;
+    const int lo = min(ia, ja), hi = max(ia, ja);
+    return !(hi - lo <= exSpan && mdg_excluded(exOff, exLst, lo, hi));
+}
+
+// Per slot: the predicate inputs of its atom (padding slots keep -1 and
+// are never tested).
+__global__ void k_pred_slots(int nslot, int keyLink, const int *__restrict__ slotAtom,
+                             const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, const double *__restrict__ xu,
+                             const double *__restrict__ yu, const double *__restrict__ zu,
+                             const int *__restrict__ cellOf, double *__restrict__ px,
+                             double *__restrict__ py, double *__restrict__ pz,
+                             int *__restrict__ pcell)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    const int a = slotAtom[s];
+    if (a < 0) return;
+    px[s] = keyLink ? xu[a] : x[a];
+    py[s] = keyLink ? yu[a] : y[a];
+    pz[s] = keyLink ? zu[a] : z[a];
+    pcell[s] = keyLink ? cellOf[a] : 0;
+}
+
 // One wave per I cluster: bbox test against every J >= I, then the 64
 // exact atom-pair predicates of each surviving (I, J) as one ballot.
 //
@@ -512,14 +629,12 @@ struct ColGrid {
 __global__ void __launch_bounds__(256)
 k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int nclu,
                 const int *__restrict__ slotAtom, const float4 *__restrict__ bb,
-                const float4 *__restrict__ bbh, const double *__restrict__ x,
-                const double *__restrict__ y, const double *__restrict__ z,
-                const double *__restrict__ xu, const double *__restrict__ yu,
-                const double *__restrict__ zu, const int *__restrict__ cellOf,
-                const int *__restrict__ frozen, const int *__restrict__ exOff,
-                const int *__restrict__ exLst, int kmax, int *__restrict__ tmpCnt,
-                int *__restrict__ tmpJ, unsigned long long *__restrict__ tmpMask,
-                int *__restrict__ ctrl, ColGrid CG)
+                const float4 *__restrict__ bbh, const double *__restrict__ px,
+                const double *__restrict__ py, const double *__restrict__ pz,
+                const int *__restrict__ pcell, bool hasFrozen, const int *__restrict__ frozen,
+                const int *__restrict__ exOff, const int *__restrict__ exLst, int exSpan, int kmax,
+                int *__restrict__ tmpCnt, int *__restrict__ tmpJ,
+                unsigned long long *__restrict__ tmpMask, int *__restrict__ ctrl, ColGrid CG)
 {
     const int lane = threadIdx.x & 63;
     const int I = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -533,7 +648,16 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
     const float L0 = (float)B.a1, L1 = (float)B.b2, L2 = (float)B.c3;
     const float rl = (float)(P.cutOff + P.delr) + 0.01f;
     const int a = lane & 7, b = lane >> 3;   // i, j atom: mask bit = lane = 8 j + i
-    const int iat = slotAtom[I * MDG_CL + a];
+    const int si = I * MDG_CL + a;
+    const int iat = slotAtom[si];
+    double xi = 0., yi = 0., zi = 0.;
+    int cli = 0;
+    if (iat >= 0) {
+        xi = px[si];
+        yi = py[si];
+        zi = pz[si];
+        cli = pcell[si];
+    }
     const int cx = colI % CG.ncx, cy = colI / CG.ncx;
     // rows: all of them once when the reach covers the box, else cy +- ry
     const bool allRows = 2 * CG.ry + 1 >= CG.ncy, allCols = 2 * CG.rx + 1 >= CG.ncx;
@@ -559,42 +683,69 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
         for (int q = 0; q < nr; q++) {
             const int j0 = max(I, CG.colSlot[row * CG.ncx + rlo[q]] / MDG_CL);
             const int j1 = CG.colSlot[row * CG.ncx + rhi[q]] / MDG_CL;
-    for (int J0 = j0; J0 < j1; J0 += 64) {
-        int J = J0 + lane;
-        bool cand = false;
-        if (J < j1) {
-            float4 cj = bb[J], hj = bbh[J];
-            float dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
-            if (B.type == 1 || B.type == 2) {
-                dx -= L0 * rintf(dx / L0);
-                dy -= L1 * rintf(dy / L1);
-                dz -= L2 * rintf(dz / L2);
-            }
-            dx = fmaxf(fabsf(dx) - hi.x - hj.x, 0.f);
-            dy = fmaxf(fabsf(dy) - hi.y - hj.y, 0.f);
-            dz = fmaxf(fabsf(dz) - hi.z - hj.z, 0.f);
-            cand = B.type == 3 || (dx * dx + dy * dy + dz * dz <= rl * rl);
-        }
-        unsigned long long cm = __ballot(cand);
-        while (cm) {
-            int src = __ffsll((long long)cm) - 1;
-            cm &= cm - 1;
-            int Jc = J0 + src;
-            int jat = slotAtom[Jc * MDG_CL + b];
-            bool pr = false;
-            if (iat >= 0 && jat >= 0 && (Jc != I || a < b))
-                pr = ref_pair_pred(P, B, keyLink, nx, ny, nz, iat, jat, x, y, z, xu, yu, zu,
-                                   cellOf, frozen, exOff, exLst);
-            unsigned long long m = __ballot(pr);
-            if (m) {
-                if (found < kmax && lane == 0) {
-                    tmpJ[(long)I * kmax + found] = Jc;
-                    tmpMask[(long)I * kmax + found] = m;
+            for (int J0 = j0; J0 < j1; J0 += 64) {
+                const int J = J0 + lane;
+                bool cand = false;
+                if (J < j1) {
+                    const float4 cj = bb[J], hj = bbh[J];
+                    float dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
+                    if (B.type == 1 || B.type == 2) {
+                        dx -= L0 * rintf(dx / L0);
+                        dy -= L1 * rintf(dy / L1);
+                        dz -= L2 * rintf(dz / L2);
+                    }
+                    dx = fmaxf(fabsf(dx) - hi.x - hj.x, 0.f);
+                    dy = fmaxf(fabsf(dy) - hi.y - hj.y, 0.f);
+                    dz = fmaxf(fabsf(dz) - hi.z - hj.z, 0.f);
+                    cand = B.type == 3 || (dx * dx + dy * dy + dz * dz <= rl * rl);
                 }
-                found++;
+                unsigned long long cm = __ballot(cand);
+                if (!cm) continue;
+                // candidates in order; the next one's slot data is requested
+                // before the current one is tested
+                int src = __ffsll((long long)cm) - 1;
+                cm &= cm - 1;
+                int sj = (J0 + src) * MDG_CL + b;
+                int jat = slotAtom[sj];
+                double xj = px[sj], yj = py[sj], zj = pz[sj];
+                int clj = pcell[sj];
+                for (;;) {
+                    const int Jc = J0 + src;
+                    const bool more = cm != 0;
+                    int nsrc = 0, njat = -1, nclj = 0;
+                    double nxj = 0., nyj = 0., nzj = 0.;
+                    if (more) {
+                        nsrc = __ffsll((long long)cm) - 1;
+                        cm &= cm - 1;
+                        const int ns = (J0 + nsrc) * MDG_CL + b;
+                        njat = slotAtom[ns];
+                        nxj = px[ns];
+                        nyj = py[ns];
+                        nzj = pz[ns];
+                        nclj = pcell[ns];
+                    }
+                    bool pr = false;
+                    if (iat >= 0 && jat >= 0 && (Jc != I || a < b))
+                        pr = ref_pair_pred_slot(P, B, keyLink, nx, ny, nz, iat, jat, xi, yi, zi,
+                                                cli, xj, yj, zj, clj, hasFrozen, frozen, exOff,
+                                                exLst, exSpan);
+                    const unsigned long long m = __ballot(pr);
+                    if (m) {
+                        if (found < kmax && lane == 0) {
+                            tmpJ[(long)I * kmax + found] = Jc;
+                            tmpMask[(long)I * kmax + found] = m;
+                        }
+                        found++;
+                    }
+                    if (!more) break;
+                    src = nsrc;
+                    jat = njat;
+                    xj = nxj;
+                    yj = nyj;
+                    zj = nzj;
+                    clj = nclj;
+                }
             }
-        }
-    }
         }
     }
     if (lane == 0) {
@@ -648,8 +799,8 @@ static int sort_columns(mdg_ctx *ctx, const int *colOf, unsigned long long *keys
     const int n = ctx->n;
     ClusterList &C = ctx->cl;
     RETCHK(mdg_scan_int(ctx, C.colCnt, C.colStart, C.ncol));
-    hipLaunchKernelGGL(k_col_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, colOf,
-                       C.colStart, C.colCnt + C.ncol + 1, C.sortKey, keys);
+    hipLaunchKernelGGL(k_col_place, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, C.ncol,
+                       colOf, C.colStart, C.colCnt + C.ncol + 1, C.sortKey, keys);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_segment_sort, dim3(C.ncol), dim3(256), 0, ctx->stream, keys, C.colStart,
                        ctx->ctrl);
@@ -731,6 +882,11 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&C.cpOff, (maxclu + 1) * sizeof(int)));
         HIPCHK(hipMalloc(&C.tmpCnt, maxclu * sizeof(int)));
         HIPCHK(hipMalloc(&C.cluCol, maxclu * sizeof(int)));
+        const size_t ns = (size_t)maxclu * MDG_CL + 64;
+        HIPCHK(hipMalloc(&C.px, ns * sizeof(double)));
+        HIPCHK(hipMalloc(&C.py, ns * sizeof(double)));
+        HIPCHK(hipMalloc(&C.pz, ns * sizeof(double)));
+        HIPCHK(hipMalloc(&C.pcell, ns * sizeof(int)));
         // neighbouring clusters within rc+skin plus two cluster diameters
         double rl = ctx->P.cutOff + ctx->P.delr + 9.0;
         double dens = (double)n / ctx->B.vol / MDG_CL;
@@ -763,13 +919,15 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
         RETCHK(ensure_double(ctx, &ctx->yu, n));
         RETCHK(ensure_double(ctx, &ctx->zu, n));
         RETCHK(ensure_int(ctx, &ctx->cellOf, n));
-        RETCHK(ensure_int(ctx, &ctx->cellCnt, 2 * (ctx->ncell + 1)));
-        RETCHK(mdg_fill_i32(ctx, ctx->cellCnt, ctx->ncell + 1, 0));
         hipLaunchKernelGGL(k_lc_assign, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P,
                            ctx->B, nx, ny, nz, ctx->x, ctx->y, ctx->z, ctx->xu, ctx->yu, ctx->zu,
-                           ctx->cellOf, ctx->cellCnt);
+                           ctx->cellOf, nullptr);
         LAUNCHCHK();
     }
+    hipLaunchKernelGGL(k_pred_slots, dim3(nblk((long)nclu * MDG_CL, 256)), dim3(256), 0,
+                       ctx->stream, nclu * MDG_CL, ctx->keyLink, C.slotAtom, ctx->x, ctx->y, ctx->z,
+                       ctx->xu, ctx->yu, ctx->zu, ctx->cellOf, C.px, C.py, C.pz, C.pcell);
+    LAUNCHCHK();
     ColGrid CG;
     CG.colSlot = C.colSlot;
     CG.cluCol = C.cluCol;
@@ -782,8 +940,8 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
     }
     hipLaunchKernelGGL(k_cluster_pairs, dim3(nblk(nclu, 4)), dim3(256), 0, ctx->stream, ctx->P,
                        ctx->B, ctx->keyLink, ctx->nlc[0], ctx->nlc[1], ctx->nlc[2], nclu,
-                       C.slotAtom, C.bb, C.bbh, ctx->x, ctx->y, ctx->z, ctx->xu, ctx->yu, ctx->zu,
-                       ctx->cellOf, ctx->frozen, ctx->exOff, ctx->exLst, C.kmax, C.tmpCnt, C.tmpJ,
+                       C.slotAtom, C.bb, C.bbh, C.px, C.py, C.pz, C.pcell, ctx->hasFrozen,
+                       ctx->frozen, ctx->exOff, ctx->exLst, ctx->exSpan, C.kmax, C.tmpCnt, C.tmpJ,
                        C.tmpMask, ctx->ctrl, CG);
     LAUNCHCHK();
     return compact_cluster_list(ctx);
