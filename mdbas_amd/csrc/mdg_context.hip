@@ -307,6 +307,10 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
         ctx->err = "link cells are implemented for linkRatio 1 (the 14-cell half shell)";
         return fail(ctx, MDG_EUNSUPPORTED);
     }
+    if (ctx->keyLink && (ctx->nlc[0] > 1023 || ctx->nlc[1] > 1023 || ctx->nlc[2] > 1023)) {
+        ctx->err = "more than 1023 link cells along an axis";
+        return fail(ctx, MDG_EUNSUPPORTED);
+    }
     ctx->ncell = ctx->nlc[0] * ctx->nlc[1] * ctx->nlc[2];
     if (int e = mdg_alloc_state(ctx)) return fail(ctx, e);
     ctx->m1 = ctx->P.m1;

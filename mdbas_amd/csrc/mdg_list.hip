@@ -554,8 +554,9 @@ __device__ inline bool ref_pair_pred_slot(const MdgParams &P, const MdgBox &B, i
 #pragma clang fp contract(off)
     double r2;
     if (keyLink) {
-        int ix = ci % nx, iy = (ci / nx) % ny, iz = ci / (nx * ny);
-        int jx = cj % nx, jy = (cj / nx) % ny, jz = cj / (nx * ny);
+        // ci, cj: cell coordinates packed 10 bits each
+        const int ix = ci & 1023, iy = (ci >> 10) & 1023, iz = ci >> 20;
+        const int jx = cj & 1023, jy = (cj >> 10) & 1023, jz = cj >> 20;
         int d[3] = {jx - ix, jy - iy, jz - iz};
         int nn[3] = {nx, ny, nz};
         double c[3];
@@ -568,9 +569,18 @@ __device__ inline bool ref_pair_pred_slot(const MdgParams &P, const MdgBox &B, i
         double xt = xj - xi + c[0];
         double yt = yj - yi + c[1];
         double zt = zj - zi + c[2];
-        double xd = xt * B.a1 + yt * B.b1 + zt * B.c1;
-        double yd = xt * B.a2 + yt * B.b2 + zt * B.c2;
-        double zd = xt * B.a3 + yt * B.b3 + zt * B.c3;
+        double xd, yd, zd;
+        if (B.type == 3) {
+            xd = xt * B.a1 + yt * B.b1 + zt * B.c1;
+            yd = xt * B.a2 + yt * B.b2 + zt * B.c2;
+            zd = xt * B.a3 + yt * B.b3 + zt * B.c3;
+        } else {
+            // the off-diagonal terms are products with +-0: the sums equal
+            // the diagonal products up to the sign of a zero, which r2 drops
+            xd = xt * B.a1;
+            yd = yt * B.b2;
+            zd = zt * B.c3;
+        }
         r2 = xd * xd + yd * yd + zd * zd;
     } else {
         // x[hi] - x[lo], as the reference orders the pair
@@ -592,9 +602,9 @@ __device__ inline bool ref_pair_pred_slot(const MdgParams &P, const MdgBox &B, i
     return !(hi - lo <= exSpan && mdg_excluded(exOff, exLst, lo, hi));
 }
 
-// Per slot: the predicate inputs of its atom (padding slots keep -1 and
-// are never tested).
-__global__ void k_pred_slots(int nslot, int keyLink, const int *__restrict__ slotAtom,
+// Per slot: the predicate inputs of its atom, the link cell as packed
+// coordinates (padding slots are never tested).
+__global__ void k_pred_slots(int nslot, int keyLink, int nx, int ny, const int *__restrict__ slotAtom,
                              const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, const double *__restrict__ xu,
                              const double *__restrict__ yu, const double *__restrict__ zu,
@@ -609,7 +619,13 @@ __global__ void k_pred_slots(int nslot, int keyLink, const int *__restrict__ slo
     px[s] = keyLink ? xu[a] : x[a];
     py[s] = keyLink ? yu[a] : y[a];
     pz[s] = keyLink ? zu[a] : z[a];
-    pcell[s] = keyLink ? cellOf[a] : 0;
+    int cp = 0;
+    if (keyLink) {
+        // cell coordinates, packed (cells per axis stay below 1024)
+        const int c = cellOf[a];
+        cp = (c % nx) | (((c / nx) % ny) << 10) | ((c / (nx * ny)) << 20);
+    }
+    pcell[s] = cp;
 }
 
 // One wave per I cluster: bbox test against every J >= I, then the 64
@@ -925,7 +941,8 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
         LAUNCHCHK();
     }
     hipLaunchKernelGGL(k_pred_slots, dim3(nblk((long)nclu * MDG_CL, 256)), dim3(256), 0,
-                       ctx->stream, nclu * MDG_CL, ctx->keyLink, C.slotAtom, ctx->x, ctx->y, ctx->z,
+                       ctx->stream, nclu * MDG_CL, ctx->keyLink, ctx->nlc[0], ctx->nlc[1],
+                       C.slotAtom, ctx->x, ctx->y, ctx->z,
                        ctx->xu, ctx->yu, ctx->zu, ctx->cellOf, C.px, C.py, C.pz, C.pcell);
     LAUNCHCHK();
     ColGrid CG;
