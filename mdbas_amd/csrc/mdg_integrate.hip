@@ -578,6 +578,13 @@ int mdg_vv_blocks(const mdg_ctx *ctx)
     return nblk(waves * 64, 256);
 }
 
+// Rows of the stage-1 SHAKE virial table: one per block for three-site
+// molecules of a single system, one per group otherwise.
+int mdg_vir_rows(const mdg_ctx *ctx)
+{
+    return (ctx->shakeW3 && !ctx->bt) ? mdg_vv_blocks(ctx) : ctx->nGrp;
+}
+
 // Runs one VV stage on the device state with forces fx.. (device, atom
 // order).  Per-group SHAKE virials go to grpVir (stage 1) and per-block
 // kinetic partials to epart + PART_KIN (stage 2); with sync the sums land
@@ -660,8 +667,11 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         hipLaunchKernelGGL(batch ? k_w3_vv1_b<true> : w3 ? k_w3_vv1_b<false> : k_vv1_b, dim3(nb),
                            dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
-        // SHAKE virial: sum of the per-group sums (fixed order)
-        if (sync) RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, ctx->nGrp, 1, ctx->scal + S_VIRSHAKE));
+        // SHAKE virial: sum of the per-group (generic) or per-block (three-site)
+        // sums, fixed order
+        if (sync)
+            RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, mdg_vir_rows(ctx), 1,
+                                       ctx->scal + S_VIRSHAKE));
     } else {
         hipLaunchKernelGGL(batch ? k_w3_vv2_a<true> : w3 ? k_w3_vv2_a<false> : k_vv2_a, dim3(nb),
                            dim3(bs), 0, ctx->stream, A);

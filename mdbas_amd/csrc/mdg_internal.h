@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -455,6 +456,8 @@ int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8])
 int mdg_spme_device(mdg_ctx *ctx);   // device-resident: frx.. per atom, partials in epart
 int mdg_spme_fork(mdg_ctx *ctx);   // marks the point stream2's work starts from
 int mdg_spme_on(mdg_ctx *ctx, int (*fn)(mdg_ctx *));   // runs fn's launches on stream2 from it
+// blocks (= partial rows) of the SPME influence kernel over H half-spectrum points
+inline int mdg_infl_blocks(long H) { return (int)std::min<long>(nblk(H, 256), 256); }
 int mdg_gather_blocks(const mdg_ctx *ctx);   // block partials of the reciprocal forces
 int mdg_spme_pot(mdg_ctx *ctx, double *pot);   // FP64 dE_rec/dq per atom
 int mdg_adj_pair(mdg_ctx *ctx);      // real-space dE/d(q, eps, sig) into fdx/fdy/fdz slots
@@ -469,6 +472,7 @@ void mdg_invalidate_graphs(mdg_ctx *ctx);
 // integrate (mdg_integrate.hip)
 int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst);
 int mdg_vv_blocks(const mdg_ctx *ctx);
+int mdg_vir_rows(const mdg_ctx *ctx);   // rows of grpVir the record sums
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
                  double *virshake_out, bool sync);
 

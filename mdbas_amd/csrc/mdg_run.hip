@@ -29,6 +29,32 @@ struct FinArgs {
     unsigned long long *mask;            // SHAKE mask, reset for VV stage 2
 };
 
+// Adds rows [0, nrow) of a row-major partial table with NC columns into
+// acc[0..NC) (this thread's rows only).
+// Four rows per thread are loaded before they are added, so a thread has
+// several loads in flight (the partials were just written, on every XCD).
+template <int NC>
+__device__ inline void rows_add(const double *__restrict__ p, int nrow, double *acc)
+{
+    constexpr int U = 4;
+    const int step = blockDim.x;
+    int r = threadIdx.x;
+    for (; r + (U - 1) * step < nrow; r += U * step) {
+        double v[U][NC];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) v[u][c] = p[(long)(r + u * step) * NC + c];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[c] += v[u][c];
+    }
+    for (; r < nrow; r += step)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c] += p[(long)r * NC + c];
+}
+
 // Per atom: F = slot pair force + (reciprocal force - mean reciprocal
 // force) + excluded-pair corrections.  The mean is over the reciprocal
 // part only (spme.c:648-670 runs on arrays energy() has just zeroed).
@@ -40,17 +66,21 @@ struct FinArgs {
 template <bool MIXED>
 __global__ void __launch_bounds__(256) k_finalize(FinArgs A)
 {
-    __shared__ double mean[3];
-    if (threadIdx.x < 64) {
-        // the first wave sums the gather's block partials: lane-strided,
-        // then across the wave (a fixed order)
+    __shared__ double mean[3], red[4][3];
+    {
+        // every block sums the gather's block partials (thread-strided with
+        // batched loads, then across the block: a fixed order)
         double s[3] = {0., 0., 0.};
-        for (int b = threadIdx.x; b < A.nbGather; b += 64)
-            for (int c = 0; c < 3; c++) s[c] += A.gpart[(long)b * 3 + c];
+        rows_add<3>(A.gpart, A.nbGather, s);
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         for (int c = 0; c < 3; c++) {
             const double v = mdg_wave_sum(s[c]);
-            if (threadIdx.x == 0) mean[c] = v / (double)A.n;
+            if (lane == 0) red[wid][c] = v;
         }
+        __syncthreads();
+        if (threadIdx.x < 3)
+            mean[threadIdx.x] = (((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+                                 red[3][threadIdx.x]) / (double)A.n;
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) A.mask[threadIdx.x] = ~0ull;
     __syncthreads();
@@ -115,31 +145,6 @@ struct RecArgs {
     unsigned long long *mask;
 };
 
-// Adds rows [0, nrow) of a row-major partial table with NC columns into
-// acc[0..NC) (this thread's rows only).
-// Four rows per thread are loaded before they are added, so a thread has
-// several loads in flight (the partials were just written, on every XCD).
-template <int NC>
-__device__ inline void rows_add(const double *__restrict__ p, int nrow, double *acc)
-{
-    constexpr int U = 4;
-    const int step = blockDim.x;
-    int r = threadIdx.x;
-    for (; r + (U - 1) * step < nrow; r += U * step) {
-        double v[U][NC];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int c = 0; c < NC; c++) v[u][c] = p[(long)(r + u * step) * NC + c];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int c = 0; c < NC; c++) acc[c] += v[u][c];
-    }
-    for (; r < nrow; r += step)
-#pragma unroll
-        for (int c = 0; c < NC; c++) acc[c] += p[(long)r * NC + c];
-}
 
 // One MDG_STEP record: reduces the step's partials (pair, correction,
 // influence function, kinetic energy, SHAKE virial) in one block, in a
@@ -166,13 +171,17 @@ __global__ void __launch_bounds__(1024) k_record(RecArgs R, int *ctrl, MDG_STEP 
     }
     if (threadIdx.x < 4) R.mask[threadIdx.x] = ~0ull;
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    // thread k adds value k over the wavefronts (in wavefront order)
+    __shared__ double tot[REC_NV];
     const int nw = blockDim.x >> 6;
-    for (int k = 0; k < REC_NV; k++) {
+    if (threadIdx.x < REC_NV) {
         double s = 0.;
-        for (int w = 0; w < nw; w++) s += red[w][k];
-        v[k] = s;
+        for (int w = 0; w < nw; w++) s += red[w][threadIdx.x];
+        tot[threadIdx.x] = s;
     }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < REC_NV; k++) v[k] = tot[k];
     const int idx = ctrl[CTRL_STEP];
     const double er = v[7];
     const double st0 = R.fact * (v[8] + er) + R.eNet;
@@ -325,7 +334,7 @@ static RecArgs rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
     R.recCap = ctx->recCap;
     R.nbPair = mdg_pair_blocks(ctx);
     R.nbExcl = nblk(ctx->n, 256);
-    R.nbInfl = nblk((long)ctx->m1 * ctx->m2 * ctx->mh, 256);
+    R.nbInfl = mdg_infl_blocks((long)ctx->m1 * ctx->m2 * ctx->mh);
     R.nbKin = nbKin;
     R.nGrpVir = nGrpVir;
     R.pPair = ctx->epart + PART_PAIR;
@@ -352,7 +361,7 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
         RETCHK(b_record(ctx, rebuild));
     else
         hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
-                           rec_args(ctx, mdg_vv_blocks(ctx), ctx->nGrp),
+                           rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)),
                            ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));

@@ -236,10 +236,11 @@ __global__ void __launch_bounds__(256)
 k_spme_influence(MdgBox B, int m1, int m2, int m3, int mh, double rAlpha2,
                  const T *__restrict__ theta, T *cg, double *__restrict__ part)
 {
-    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const long G = (long)m1 * m2 * mh;
-    if (g < G) {
+    // grid-stride: at most mdg_infl_blocks blocks, so the record sums few rows
+    for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < G;
+         g += (long)gridDim.x * blockDim.x) {
         const int k = (int)(g % mh);
         const int j = (int)((g / mh) % m2);
         const int i = (int)(g / ((long)mh * m2));
@@ -258,13 +259,13 @@ k_spme_influence(MdgBox B, int m1, int m2, int m3, int mh, double rAlpha2,
             const double rz = t1 * B.u3 + t2 * B.v3 + t3 * B.w3;
             const double rm = rx * rx + ry * ry + rz * rz;
             const double vam = 2.0 * (1.0 / rm - rAlpha2) * e;
-            acc[0] = w * e;
-            acc[1] = -w * vam * rx * rx;
-            acc[2] = -w * vam * rx * ry;
-            acc[3] = -w * vam * rx * rz;
-            acc[4] = -w * vam * ry * ry;
-            acc[5] = -w * vam * ry * rz;
-            acc[6] = -w * vam * rz * rz;
+            acc[0] += w * e;
+            acc[1] += -w * vam * rx * rx;
+            acc[2] += -w * vam * rx * ry;
+            acc[3] += -w * vam * rx * rz;
+            acc[4] += -w * vam * ry * ry;
+            acc[5] += -w * vam * ry * rz;
+            acc[6] += -w * vam * rz * rz;
         }
         cg[2 * g] = re * th;
         cg[2 * g + 1] = im * th;
@@ -986,7 +987,7 @@ int mdg_spme_setup(mdg_ctx *ctx)
     }
     ctx->mh = m3 / 2 + 1;
     const long G = (long)m1 * m2 * m3, H = (long)m1 * m2 * ctx->mh;
-    if (nblk(H, 256) * 7 > PART_R || nblk(ctx->n, 256) * 7 > PART_R) {
+    if ((long)mdg_gather_blocks(ctx) * 3 > PART_R || nblk(ctx->n, 256) * 7 > PART_R) {
         ctx->err = "mesh or atom count beyond the partial-sum buffers";
         return MDG_EUNSUPPORTED;
     }
@@ -1050,7 +1051,7 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
             ctx->err = "rocfft_execute (forward) failed";
             return MDG_EHIP;
         }
-        hipLaunchKernelGGL(k_spme_influence<float>, dim3(nblk(H, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_spme_influence<float>, dim3(mdg_infl_blocks(H)), dim3(256), 0, ctx->stream,
                            ctx->B, m1, m2, m3, mh, rAlpha2, ctx->theta32, ctx->cgrid32, partInfl);
         LAUNCHCHK();
         void *in2[1] = {ctx->cgrid32}, *out2[1] = {ctx->grid32};
@@ -1073,7 +1074,7 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
             ctx->err = "rocfft_execute (forward) failed";
             return MDG_EHIP;
         }
-        hipLaunchKernelGGL(k_spme_influence<double>, dim3(nblk(H, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_spme_influence<double>, dim3(mdg_infl_blocks(H)), dim3(256), 0,
                            ctx->stream, ctx->B, m1, m2, m3, mh, rAlpha2, ctx->theta, ctx->cgrid64,
                            partInfl);
         LAUNCHCHK();
@@ -1141,7 +1142,8 @@ int mdg_spme64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double eout[8])
     const long H = (long)ctx->m1 * ctx->m2 * ctx->mh;
     RETCHK(spme_run(ctx, false, ctx->epart + PART_INFL, ctx->epart + PART_GATHER, ctx->frx64,
                     ctx->fry64, ctx->frz64));
-    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_INFL, nblk(H, 256), 7, ctx->scal + S_EREC));
+    RETCHK(mdg_reduce_partials(ctx, ctx->epart + PART_INFL, mdg_infl_blocks(H), 7,
+                               ctx->scal + S_EREC));
     hipLaunchKernelGGL(k_add_rec, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, fx, fy, fz,
                        ctx->frx64, ctx->fry64, ctx->frz64, ctx->epart + PART_MISC);
     LAUNCHCHK();
