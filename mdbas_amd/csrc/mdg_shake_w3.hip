@@ -456,32 +456,43 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
 {
     const int g = shake_w3_group(A);
     const int k = threadIdx.x & 3;
-    if (g >= A.nGrp) return;
-    const int C = shake_global_stop(A, q_mask<BATCH>(A, g));
-    if (C < 0) {
-        if (k == 0) atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 1);
-        return;
+    double vblk = 0.;   // this lane's share of the block's SHAKE virial
+    if (g < A.nGrp) {
+        const int C = shake_global_stop(A, q_mask<BATCH>(A, g));
+        if (C < 0) {
+            if (k == 0) atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 1);
+        } else {
+            const MdgBox &B = q_box<BATCH>(A, g);
+            const int first = A.first[g], kend = A.kend[g];
+            int slot, at;
+            shake_pick_slot(C, first, kend, slot, at);
+            Q3 Q;
+            q_load(A, g, k, Q);
+            q_dd_load(A, Q);
+            q_setup_r(A, Q);
+            q_scr_load(A, Q, slot, true);
+            q_shift_now(B, Q);
+            // the saved virial is the molecule's; it continues on lane 0
+            double vir = (slot == 0 || k != 0) ? 0. : A.virScr[(long)slot * A.nGrp + g];
+            for (int it = at; it < C; it++) {
+                double dtx, dty, dtz, rt2;
+                q_check(B, Q, dtx, dty, dtz, rt2);
+                q_apply_r(A, Q, dtx, dty, dtz, rt2, vir);
+            }
+            const double vm = q_sum3(0., vir);
+            if (BATCH) {
+                if (k == 0) A.vir[g] = vm;   // per molecule, summed per replica
+            } else if (k == 0) {
+                vblk = vm;
+            }
+            q_store(A, Q, true);
+        }
     }
-    const MdgBox &B = q_box<BATCH>(A, g);
-    const int first = A.first[g], kend = A.kend[g];
-    int slot, at;
-    shake_pick_slot(C, first, kend, slot, at);
-    Q3 Q;
-    q_load(A, g, k, Q);
-    q_dd_load(A, Q);
-    q_setup_r(A, Q);
-    q_scr_load(A, Q, slot, true);
-    q_shift_now(B, Q);
-    // the saved virial is the molecule's; it continues on lane 0
-    double vir = (slot == 0 || k != 0) ? 0. : A.virScr[(long)slot * A.nGrp + g];
-    for (int it = at; it < C; it++) {
-        double dtx, dty, dtz, rt2;
-        q_check(B, Q, dtx, dty, dtz, rt2);
-        q_apply_r(A, Q, dtx, dty, dtz, rt2, vir);
+    if (!BATCH) {
+        // one partial per block, summed in a fixed order by the record
+        double acc[1] = {vblk};
+        mdg_block_partials<1>(acc, A.vir);
     }
-    const double vm = q_sum3(0., vir);
-    if (k == 0) A.vir[g] = vm;
-    q_store(A, Q, true);
 }
 
 // Stage 2, part A: constraint vectors, half kick, SHAKE-V sweeps with the
