@@ -289,7 +289,9 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evRecFork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evRecDone, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "no HIP device / stream";
         return fail(ctx, MDG_EHIP);
     }
@@ -385,7 +387,9 @@ extern "C" mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *para
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evRecFork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evRecDone, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "no HIP device / stream";
         return fail(ctx, MDG_EHIP);
     }
@@ -409,7 +413,7 @@ extern "C" mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *para
     if (e) return fail(ctx, e);
     if ((e = dmalloc(ctx, &T.goff, nrep + 1)) || (e = dmalloc(ctx, &T.eSelf, nrep)) ||
         (e = dmalloc(ctx, &T.eNet, nrep)) || (e = dmalloc(ctx, &T.stepCtr, nrep)) ||
-        (e = dmalloc(ctx, &T.masks, (size_t)nrep * 4)) ||
+        (e = dmalloc(ctx, &T.masks, (size_t)nrep * 2 * 4)) ||   // [stage][replica][4 words]
         (e = dmalloc(ctx, &T.atomE, 2 * (size_t)ctx->n)))
         return fail(ctx, e);
     if ((e = mdg_b_spme_setup(ctx))) return fail(ctx, e);
@@ -468,6 +472,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
     if (ctx->stream2) hipStreamDestroy(ctx->stream2);
     if (ctx->evFork) hipEventDestroy(ctx->evFork);
     if (ctx->evJoin) hipEventDestroy(ctx->evJoin);
+    if (ctx->evRecFork) hipEventDestroy(ctx->evRecFork);
+    if (ctx->evRecDone) hipEventDestroy(ctx->evRecDone);
     delete ctx;
 }
 

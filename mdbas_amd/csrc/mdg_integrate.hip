@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(128) k_vv2_b(ShakeArgs A)
 
 __global__ void k_ctrl_reset(int *ctrl, unsigned long long *mask)
 {
-    if (threadIdx.x < SHK_MASKW) mask[threadIdx.x] = ~0ull;
+    if (threadIdx.x < 2 * SHK_MASKW) mask[threadIdx.x] = ~0ull;
 }
 
 // Connected components of the constraint graph (host, once).
@@ -554,7 +554,7 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
     HIPCHK(hipMalloc(&ctx->shkVir, (size_t)SHK_SLOTS * std::max(ng, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->shkFirst, std::max(ng, 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->shkKend, std::max(ng, 1) * sizeof(int)));
-    HIPCHK(hipMalloc(&ctx->shkMask, SHK_MASKW * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&ctx->shkMask, 2 * SHK_MASKW * sizeof(unsigned long long)));
     HIPCHK(hipMalloc(&ctx->grpAtomOff, (ng + 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->grpConOff, (ng + 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->grpAtoms, n * sizeof(int)));
@@ -634,7 +634,9 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.virScr = ctx->shkVir;
     A.first = ctx->shkFirst;
     A.kend = ctx->shkKend;
-    A.mask = ctx->shkMask;
+    // each stage has its own mask (stage 2's beside stage 1's), so a step's
+    // record may run while the next step's stage 1 publishes its sweeps
+    A.mask = ctx->shkMask + (stage == 1 ? 0 : SHK_MASKW);
     A.boxes = nullptr;
     A.grep = nullptr;
     A.grpKin = nullptr;
@@ -649,7 +651,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         A.boxes = ctx->bt->boxes;
         A.grep = ctx->bt->grep;
         A.grpKin = ctx->bt->grpKin;
-        A.mask = ctx->bt->masks;
+        A.mask = ctx->bt->masks + (stage == 1 ? 0 : (long)ctx->bt->R * SHK_MASKW);
     }
     const int bs = w3 ? 256 : 128;
     const int nb = mdg_vv_blocks(ctx);
@@ -664,6 +666,12 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         hipLaunchKernelGGL(batch ? k_w3_vv1_a<true> : w3 ? k_w3_vv1_a<false> : k_vv1_a, dim3(nb),
                            dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
+        // the previous step's record (on stream2) reads the virial rows
+        // that the next kernel writes
+        if (ctx->recPending) {
+            HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evRecDone, 0));
+            ctx->recPending = false;
+        }
         hipLaunchKernelGGL(batch ? k_w3_vv1_b<true> : w3 ? k_w3_vv1_b<false> : k_vv1_b, dim3(nb),
                            dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();

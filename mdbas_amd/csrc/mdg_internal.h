@@ -148,6 +148,10 @@ struct mdg_ctx {
     // graph has the two branches)
     hipStream_t stream2 = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
+    // step blocks: each step's record kernel runs on stream2 beside the next
+    // step's first SHAKE kernel (joined before that step's second one)
+    hipEvent_t evRecFork = nullptr, evRecDone = nullptr;
+    bool recOverlap = false, recPending = false;
     bool spmeForked = false;     // the current step's SPME went to stream2
     int device = 0;
 
@@ -172,7 +176,7 @@ struct mdg_ctx {
     int *ctrl = nullptr;         // device control words (see CTRL_*)
     double *shkScr = nullptr, *shkVir = nullptr;   // SHAKE state window (mdg_integrate.hip)
     int *shkFirst = nullptr, *shkKend = nullptr;
-    unsigned long long *shkMask = nullptr;
+    unsigned long long *shkMask = nullptr;   // [stage 1 | stage 2][SHK_MASKW]
     bool shakeW3 = false;        // all groups are three-site molecules
     // exclusions (CSR; symmetric for link cells, ordered otherwise)
     int *exOff = nullptr, *exLst = nullptr;
@@ -245,6 +249,7 @@ struct mdg_ctx {
 
     // graph
     hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    hipGraphExec_t graphBlock = nullptr;   // neigh.update consecutive steps ending in a rebuild
     MdgBatch *bt = nullptr;      // replica batch, when created by mdg_batch_create
     MdgTune tune;
     int graphStepsPerLaunch = 0;

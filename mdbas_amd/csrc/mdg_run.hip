@@ -26,7 +26,7 @@ struct FinArgs {
     const int *frozen;
     double *fx, *fy, *fz, *vx, *vy, *vz;
     double *part;                        // per block: eCorr, virCorr
-    unsigned long long *mask;            // SHAKE mask, reset for VV stage 2
+    unsigned long long *mask;            // both SHAKE masks, reset here
 };
 
 // Adds rows [0, nrow) of a row-major partial table with NC columns into
@@ -82,7 +82,9 @@ __global__ void __launch_bounds__(256) k_finalize(FinArgs A)
             mean[threadIdx.x] = (((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
                                  red[3][threadIdx.x]) / (double)A.n;
     }
-    if (blockIdx.x == 0 && threadIdx.x < 4) A.mask[threadIdx.x] = ~0ull;
+    // both SHAKE masks: stage 1 is done with its own, the next stage 2
+    // has not started
+    if (blockIdx.x == 0 && threadIdx.x < 2 * SHK_MASKW) A.mask[threadIdx.x] = ~0ull;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[2] = {0, 0};
@@ -142,7 +144,6 @@ struct RecArgs {
     // per-block partial counts of this step's producers
     int nbPair, nbExcl, nbInfl, nbKin, nGrpVir;
     const double *pPair, *pExcl, *pInfl, *pKin, *grpVir;
-    unsigned long long *mask;
 };
 
 
@@ -169,7 +170,6 @@ __global__ void __launch_bounds__(1024) k_record(RecArgs R, int *ctrl, MDG_STEP 
         const double s = mdg_wave_sum(v[k]);
         if (lane == 0) red[wid][k] = s;
     }
-    if (threadIdx.x < 4) R.mask[threadIdx.x] = ~0ull;
     __syncthreads();
     // thread k adds value k over the wavefronts (in wavefront order)
     __shared__ double tot[REC_NV];
@@ -239,6 +239,10 @@ void mdg_invalidate_graphs(mdg_ctx *ctx)
             (void)hipGraphExecDestroy(ctx->graphExec[k]);
             ctx->graphExec[k] = nullptr;
         }
+    if (ctx->graphBlock) {
+        (void)hipGraphExecDestroy(ctx->graphBlock);
+        ctx->graphBlock = nullptr;
+    }
 }
 
 static int ensure_records(mdg_ctx *ctx, int n)
@@ -342,7 +346,6 @@ static RecArgs rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
     R.pInfl = ctx->epart + PART_INFL;
     R.pKin = ctx->epart + PART_KIN;
     R.grpVir = ctx->grpVir;
-    R.mask = ctx->shkMask;
     return R;
 }
 
@@ -357,13 +360,29 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     RETCHK(mdg_device_energy(ctx, rebuild));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[10], ctx->stream));
     RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
+    // In a step block the record goes to stream2, beside the next step's
+    // first SHAKE kernel; the next step's second one waits for it.
+    const bool fork = ctx->recOverlap && !ctx->profiling;
+    hipStream_t main = ctx->stream;
+    if (fork) {
+        HIPCHK(hipEventRecord(ctx->evRecFork, main));
+        HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->evRecFork, 0));
+        ctx->stream = ctx->stream2;
+    }
+    int st = MDG_OK;
     if (ctx->bt)
-        RETCHK(b_record(ctx, rebuild));
+        st = b_record(ctx, rebuild);
     else
         hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
                            rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)),
                            ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
+    ctx->stream = main;
+    RETCHK(st);
     LAUNCHCHK();
+    if (fork) {
+        HIPCHK(hipEventRecord(ctx->evRecDone, ctx->stream2));
+        ctx->recPending = true;
+    }
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
     return MDG_OK;
 }
@@ -406,7 +425,7 @@ __global__ void k_not64(long m, unsigned long long *p)
 
 __global__ void k_reset_mask(unsigned long long *mask)
 {
-    if (threadIdx.x < 4) mask[threadIdx.x] = ~0ull;
+    if (threadIdx.x < 2 * SHK_MASKW) mask[threadIdx.x] = ~0ull;
 }
 
 // Record index 0 and a fresh SHAKE mask, before a sequence of steps.
@@ -414,7 +433,7 @@ int mdg_reset_step_state(mdg_ctx *ctx)
 {
     if (ctx->bt) {
         RETCHK(mdg_fill_i32(ctx, ctx->bt->stepCtr, ctx->bt->R, 0));
-        const long nm = (long)ctx->bt->R * SHK_MASKW;
+        const long nm = (long)ctx->bt->R * 2 * SHK_MASKW;
         hipLaunchKernelGGL(k_not64, dim3(nblk(nm, 256)), dim3(256), 0, ctx->stream, nm,
                            ctx->bt->masks);
         LAUNCHCHK();
@@ -511,6 +530,35 @@ static int capture(mdg_ctx *ctx, int variant)
     return MDG_OK;
 }
 
+// neigh.update consecutive steps, from one just after a rebuild to the
+// next rebuild, captured as one graph: inside it each step's record runs
+// beside the next step's first SHAKE kernel, and consecutive steps follow
+// each other without a graph boundary.
+static int capture_block(mdg_ctx *ctx)
+{
+    const int U = ctx->P.update;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->recOverlap = true;
+    int st = MDG_OK;
+    for (int k = 0; k < U && st == MDG_OK; k++) st = mdg_device_step(ctx, k == U - 1);
+    if (st == MDG_OK && ctx->recPending) {
+        // the last record joins the captured stream
+        if (hipStreamWaitEvent(ctx->stream, ctx->evRecDone, 0) != hipSuccess) st = MDG_EHIP;
+    }
+    ctx->recOverlap = false;
+    ctx->recPending = false;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (st) return st;
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return MDG_EHIP;
+    }
+    HIPCHK(hipGraphInstantiate(&ctx->graphBlock, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return MDG_OK;
+}
+
 static void accumulate_phase(mdg_ctx *ctx, const hipEvent_t *e, bool rebuild, int &nReb)
 {
     float t, t1, t2;
@@ -531,6 +579,7 @@ static void accumulate_phase(mdg_ctx *ctx, const hipEvent_t *e, bool rebuild, in
 }
 
 // use_graph: 0 plain launches, 1 one graph per step variant (replayed),
+// whole rebuild periods as one block graph where they fit,
 // 2 the whole run captured as one graph.  With profiling on, every mode
 // runs plain launches with HIP events bracketing the pair kernel, SPME,
 // the integrator, the list rebuild and the step on the context stream,
@@ -570,9 +619,18 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
         HIPCHK(hipGraphExecDestroy(ge));
         ctx->step += nsteps;
     } else {
+        const int U = ctx->P.update;
         for (int s = 0; s < nsteps; s++) {
             const int step = ctx->step + 1;
             const bool rebuild = (step % ctx->P.update) == 0;
+            if (use_graph && !prof && U > 1 && step % U == 1 && nsteps - s >= U) {
+                // a whole rebuild period as one graph
+                if (!ctx->graphBlock) RETCHK(capture_block(ctx));
+                HIPCHK(hipGraphLaunch(ctx->graphBlock, ctx->stream));
+                ctx->step += U;
+                s += U - 1;
+                continue;
+            }
             if (use_graph && !prof) {
                 const int v = rebuild ? 1 : 0;
                 if (!ctx->graphExec[v]) RETCHK(capture(ctx, v));
@@ -771,12 +829,10 @@ struct BRecArgs {
     const double *chE, *atomE, *grpKin, *grpVir, *repInfl;
     int *stepCtr;
     MDG_STEP *rec;                       // [step][replica]
-    unsigned long long *masks;
 };
 
 // One MDG_STEP per replica (one block each): the replica's chunk, atom and
-// group sums in a fixed order, assembled as k_record does; resets the
-// replica's SHAKE mask for the next VV stage 1.
+// group sums in a fixed order, assembled as k_record does.
 __global__ void __launch_bounds__(256) k_b_record(BRecArgs R, int rebuilt)
 {
     __shared__ double red[4][10];
@@ -799,7 +855,7 @@ __global__ void __launch_bounds__(256) k_b_record(BRecArgs R, int rebuilt)
         const double s = mdg_wave_sum(v[k]);
         if (lane == 0) red[wid][k] = s;
     }
-    if (threadIdx.x < SHK_MASKW) R.masks[(long)r * SHK_MASKW + threadIdx.x] = ~0ull;
+
     __syncthreads();
     if (threadIdx.x != 0) return;
     for (int k = 0; k < 9; k++) v[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
@@ -881,7 +937,7 @@ static int b_energy(mdg_ctx *ctx, bool rebuild)
     F.vz = ctx->vz;
     F.atomE = T.atomE;
     F.masks = T.masks;
-    F.nmask = T.R * SHK_MASKW;
+    F.nmask = T.R * 2 * SHK_MASKW;
     if (ctx->precision == MDG_MIXED)
         hipLaunchKernelGGL(k_b_finalize<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
     else
@@ -921,7 +977,6 @@ static int b_record(mdg_ctx *ctx, bool rebuild)
     R.repInfl = T.repInfl;
     R.stepCtr = T.stepCtr;
     R.rec = T.rec;
-    R.masks = T.masks;
     hipLaunchKernelGGL(k_b_record, dim3(T.R), dim3(256), 0, ctx->stream, R, rebuild ? 1 : 0);
     LAUNCHCHK();
     return MDG_OK;
