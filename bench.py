@@ -206,13 +206,15 @@ def single_box(args, M, configs, name):
         "pair_interactions_per_s": pis,
         "pair_interactions_per_step": pin,
         "listed_pairs": plist,
+        "cluster_pairs": int(stats["cluster_pairs"]),
+        "pair_interactions_per_s_pair_kernel": pin2 / (ph["pair_ms"] * 1e-3),
         "accuracy": {"force_rms_rel_err_vs_fp64": eps_rms, "force_max_rel_err_vs_fp64": eps_max,
                      "energy_drift_kcal_mol_ns_per_water_mixed": drift_mixed,
                      "energy_drift_kcal_mol_ns_per_water_fp64": drift_64,
                      "drift_window_steps": K},
         "phases_ms": ph,
         "tuning": tuning,
-        "roofline": {"kernel": "k_pair_cluster<MIXED>", "bound": "valu",
+        "roofline": {"kernel": "k_pair_mixed", "bound": "valu",
                      "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK, "traffic": pair_traffic(name),
                      "flop_per_launch": flop, "launch_ms": ph["pair_ms"]},

@@ -458,7 +458,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
         mdg_b_spme_free(ctx);
         void *bp[] = {T.aoff, T.goff, T.arep, T.grep, T.colOff, T.nfx, T.nfy, T.mesh, T.gridOff,
                       T.hOff, T.boxes, T.eSelf, T.eNet, T.cluStart, T.crep, T.chE, T.atomE,
-                      T.grpKin, T.repInfl, T.repRec, T.masks, T.rec, T.stepCtr};
+                      T.grpKin, T.repInfl, T.repRec, T.masks, T.rec, T.stepCtr, T.binOff,
+                      T.tileOff, T.tileRep};
         for (void *p : bp)
             if (p) hipFree(p);
         delete ctx->bt;

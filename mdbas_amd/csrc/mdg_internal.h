@@ -124,6 +124,10 @@ struct MdgBatch {
     int *stepCtr = nullptr;      // per replica: next record index
     long Gtot = 0, Htot = 0;
     std::vector<MdgFftGroup> fft;
+    // binned spreading (mixed): per-replica bin and tile offsets, tile -> replica
+    bool binned = false;
+    int nTiles = 0;
+    int *binOff = nullptr, *tileOff = nullptr, *tileRep = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
 

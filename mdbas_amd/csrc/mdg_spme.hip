@@ -1338,6 +1338,236 @@ k_b_rec_sums(const int *__restrict__ aoff, const TF *__restrict__ frx, const TF 
     mdg_block_partials<3>(acc, repRec);    // row blockIdx.x = r
 }
 
+// Binned spreading for every replica (mixed): as k_spme_bin and
+// k_spme_spread_tiles, with replica r's box and cubic mesh; r's bins are
+// binOff[r] .. binOff[r+1]-1 and its tiles tileOff[r] .. tileOff[r+1]-1
+// (tileRep gives the replica of a tile).
+template <int P>
+__global__ void __launch_bounds__(256)
+k_b_bin(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__restrict__ arep,
+        const int *__restrict__ mesh, const int *__restrict__ binOff, int n,
+        const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+        int *binCnt, int *__restrict__ bucket, int *__restrict__ binOut, int *binNOut,
+        int4 *__restrict__ splI, float4 *__restrict__ splB)
+{
+    constexpr int NQ = SplineAtom<P>::NQ;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int r = arep[l], m = mesh[r];
+    double s[3];
+    scaled_coords(boxes[r], m, m, m, x[l], y[l], z[l], s[0], s[1], s[2]);
+    const int i1 = (int)s[0], i2 = (int)s[1], i3 = (int)s[2];
+    splI[l] = make_int4(i1, i2, i3, r);
+#pragma unroll
+    for (int dim = 0; dim < 3; dim++) {
+        float b[P], d[P], bq[4 * NQ];
+        bspline_fix<float, P>(s[dim], b, d);
+#pragma unroll
+        for (int k = 0; k < 4 * NQ; k++) bq[k] = k < P ? b[k] : 0.f;
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            splB[(long)(dim * NQ + q) * n + l] =
+                make_float4(bq[4 * q], bq[4 * q + 1], bq[4 * q + 2], bq[4 * q + 3]);
+    }
+    const int t1 = spme_top(i1, m), t2 = spme_top(i2, m), t3 = spme_top(i3, m);
+    if (t1 >= 0 && t2 >= 0 && t3 >= 0) {
+        const int nb = (m + SPME_BIN - 1) / SPME_BIN;
+        const int b = binOff[r] + ((t1 / SPME_BIN) * nb + t2 / SPME_BIN) * nb + t3 / SPME_BIN;
+        const int rk = atomicAdd(&binCnt[b], 1);
+        if (rk < SPME_BCAP) {
+            bucket[(long)b * SPME_BCAP + rk] = l;
+            return;
+        }
+    }
+    binOut[atomicAdd(&binNOut[0], 1)] = l;
+}
+
+template <int P>
+__global__ void __launch_bounds__(256)
+k_b_spread_tiles(MdgParams Pm, const int *__restrict__ mesh, const int *__restrict__ binOff,
+                 const int *__restrict__ tileOff, const int *__restrict__ tileRep,
+                 const long *__restrict__ gridOff, const double *__restrict__ q,
+                 const int *__restrict__ binCnt, const int *__restrict__ bucket,
+                 const int *__restrict__ binOut, const int *__restrict__ binNOut,
+                 const int4 *__restrict__ splI, const float4 *__restrict__ splB,
+                 float *__restrict__ grid32)
+{
+    constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
+    constexpr int NB3 = NBD * NBD * NBD;
+    static_assert(NB3 <= 128, "bin list of a tile exceeds the scan width");
+    __shared__ unsigned long long acc[T * T * T];
+    __shared__ int bx[NBD], by[NBD], bz[NBD], nbd[3];
+    __shared__ int off[NB3 + 1], bid[NB3];
+    const int tile = blockIdx.x, r = tileRep[tile], m = mesh[r];
+    const int nt = (m + T - 1) / T, nb = (m + SPME_BIN - 1) / SPME_BIN, lt = tile - tileOff[r];
+    const int tx = lt / (nt * nt), ty = (lt / nt) % nt, tz = lt % nt;
+    const int X0 = tx * T, Y0 = ty * T, Z0 = tz * T;
+    for (int i = threadIdx.x; i < T * T * T; i += blockDim.x) acc[i] = 0ull;
+    if (threadIdx.x == 0) nbd[0] = spme_tile_bins<P>(X0, m, bx);
+    if (threadIdx.x == 1) nbd[1] = spme_tile_bins<P>(Y0, m, by);
+    if (threadIdx.x == 2) nbd[2] = spme_tile_bins<P>(Z0, m, bz);
+    __syncthreads();
+    const int nbt = nbd[0] * nbd[1] * nbd[2];
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int cnt[2] = {0, 0};
+        for (int h = 0; h < 2; h++) {
+            const int c = lane + 64 * h;
+            if (c < nbt) {
+                const int ix = c / (nbd[1] * nbd[2]), iy = (c / nbd[2]) % nbd[1], iz = c % nbd[2];
+                const int b = binOff[r] + (bx[ix] * nb + by[iy]) * nb + bz[iz];
+                bid[c] = b;
+                cnt[h] = min(binCnt[b], SPME_BCAP);
+            }
+        }
+        int run = 0;
+        for (int h = 0; h < 2; h++) {
+            int v = cnt[h];
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(v, o, 64);
+                if (lane >= o) v += u;
+            }
+            const int c = lane + 64 * h;
+            if (c < nbt) off[c] = run + v - cnt[h];
+            run += __shfl(v, 63, 64);
+        }
+        if (lane == 0) off[nbt] = run;
+    }
+    __syncthreads();
+    const int ncand = off[nbt], nout = binNOut[0];
+    for (int c0 = 0; c0 < ncand + nout; c0 += blockDim.x) {
+        const int c = c0 + threadIdx.x;
+        if (c < ncand + nout) {
+            int l;
+            if (c < ncand) {
+                int lo = 0, hi = nbt;   // off[lo] <= c < off[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (off[mid] <= c) lo = mid;
+                    else hi = mid;
+                }
+                l = bucket[(long)bid[lo] * SPME_BCAP + (c - off[lo])];
+            } else {
+                l = binOut[c - ncand];
+            }
+            SplineAtom<P> S;
+            spline_load<P>(splI, splB, nullptr, Pm.n, l, false, S);
+            // the list of atoms outside the buckets is shared by all replicas
+            if (c >= ncand && splI[l].w != r) continue;
+            const float ql = (float)q[l];
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                const int lx = wrap_index(S.base[0] - i, m) - X0;
+                if (lx < 0 || lx >= T) continue;
+                const float qa = ql * S.b[0][i];
+#pragma unroll
+                for (int j = 0; j < P; j++) {
+                    const int ly = wrap_index(S.base[1] - j, m) - Y0;
+                    if (ly < 0 || ly >= T) continue;
+                    const float qb = qa * S.b[1][j];
+#pragma unroll
+                    for (int k = 0; k < P; k++) {
+                        const int lz = wrap_index(S.base[2] - k, m) - Z0;
+                        if (lz < 0 || lz >= T) continue;
+                        const long long v = __double2ll_rn((double)(qb * S.b[2][k]) * MDG_QSCALE);
+                        atomicAdd(&acc[(lx * T + ly) * T + lz], (unsigned long long)v);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const long base = gridOff[r];
+    for (int i = threadIdx.x; i < T * T * T; i += blockDim.x) {
+        const int gx = X0 + i / (T * T), gy = Y0 + (i / T) % T, gz = Z0 + i % T;
+        if (gx < m && gy < m && gz < m)
+            grid32[base + gz + (long)m * (gy + (long)m * gx)] =
+                (float)((double)(long long)acc[i] * (1.0 / MDG_QSCALE));
+    }
+}
+
+// Clears the bucket counts and the outlier count for the next step.
+__global__ void k_bin_clear(long nbins, int *__restrict__ binCnt, int *__restrict__ binNOut)
+{
+    for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < nbins;
+         b += (long)gridDim.x * blockDim.x)
+        binCnt[b] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) binNOut[0] = 0;
+}
+
+// Whether every replica's mesh takes the binned path.
+static bool b_spme_binned(const mdg_ctx *ctx)
+{
+    const int p = ctx->P.nbsp, need = SPME_TILE + p + SPME_BIN;
+    if (!(p == 4 || p == 6 || p == 8)) return false;
+    for (int m : ctx->bt->hMesh)
+        if (m < need) return false;
+    return true;
+}
+
+static int b_bin_alloc(mdg_ctx *ctx)
+{
+    MdgBatch &T = *ctx->bt;
+    const int R = T.R, n = ctx->n;
+    std::vector<int> bo(R + 1, 0), to(R + 1, 0), rep;
+    for (int r = 0; r < R; r++) {
+        const int m = T.hMesh[r], nb = (m + SPME_BIN - 1) / SPME_BIN, nt = (m + SPME_TILE - 1) / SPME_TILE;
+        bo[r + 1] = bo[r] + nb * nb * nb;
+        to[r + 1] = to[r] + nt * nt * nt;
+        for (int k = 0; k < nt * nt * nt; k++) rep.push_back(r);
+    }
+    T.nTiles = to[R];
+    const long nbins = bo[R];
+    HIPCHK(hipMalloc(&T.binOff, (R + 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&T.tileOff, (R + 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&T.tileRep, T.nTiles * sizeof(int)));
+    HIPCHK(hipMemcpy(T.binOff, bo.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(T.tileOff, to.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(T.tileRep, rep.data(), T.nTiles * sizeof(int), hipMemcpyHostToDevice));
+    ctx->nbx = (int)nbins;   // total bins of the batch (ctx->nby = ctx->nbz = 1)
+    ctx->nby = ctx->nbz = 1;
+    HIPCHK(hipMalloc(&ctx->binCnt, nbins * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binBucket, nbins * SPME_BCAP * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binOut, n * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->binNOut, 2 * sizeof(int)));
+    RETCHK(mdg_fill_i32(ctx, ctx->binCnt, nbins, 0));
+    RETCHK(mdg_fill_i32(ctx, ctx->binNOut, 2, 0));
+    const int nq = (ctx->P.nbsp + 3) / 4;
+    HIPCHK(hipMalloc(&ctx->splI, n * sizeof(int4)));
+    HIPCHK(hipMalloc(&ctx->splB, 3 * nq * (size_t)n * sizeof(float4)));
+    T.binned = true;
+    return MDG_OK;
+}
+
+template <int P>
+static int b_spread_binned_p(mdg_ctx *ctx)
+{
+    MdgBatch &T = *ctx->bt;
+    const int n = ctx->n;
+    hipLaunchKernelGGL(k_b_bin<P>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, T.boxes,
+                       T.arep, T.mesh, T.binOff, n, ctx->x, ctx->y, ctx->z, ctx->binCnt,
+                       ctx->binBucket, ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_b_spread_tiles<P>, dim3(T.nTiles), dim3(256), 0, ctx->stream, ctx->P,
+                       T.mesh, T.binOff, T.tileOff, T.tileRep, T.gridOff, ctx->q, ctx->binCnt,
+                       ctx->binBucket, ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB,
+                       ctx->grid32);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_bin_clear, dim3((int)std::min<long>(nblk(ctx->nbx, 256), 1024)),
+                       dim3(256), 0, ctx->stream, (long)ctx->nbx, ctx->binCnt, ctx->binNOut);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+static int b_spread_binned(mdg_ctx *ctx)
+{
+    switch (ctx->P.nbsp) {
+    case 4: return b_spread_binned_p<4>(ctx);
+    case 6: return b_spread_binned_p<6>(ctx);
+    default: return b_spread_binned_p<8>(ctx);
+    }
+}
+
 // Meshes, influence functions and batched FFT plans of a replica batch:
 // replicas with the same mesh edge share one plan of that many transforms
 // and lie next to each other in the mesh arrays.
@@ -1409,6 +1639,7 @@ int mdg_b_spme_setup(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&ctx->cgrid32, 2 * h * sizeof(float)));
         HIPCHK(hipMalloc(&ctx->gridq, g * sizeof(long long)));
         RETCHK(mdg_fill_i64(ctx, ctx->gridq, g));
+        if (b_spme_binned(ctx)) RETCHK(b_bin_alloc(ctx));
     }
     ctx->mh = 0;
     return MDG_OK;
@@ -1436,13 +1667,18 @@ int mdg_b_spme(mdg_ctx *ctx)
     const bool mixed = ctx->precision == MDG_MIXED;
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[2], ctx->stream));
     if (mixed) {
-        hipLaunchKernelGGL(k_b_spread<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P,
-                           T.boxes, T.arep, T.mesh, T.gridOff, n, mdg_scatter_stride(n), ctx->x,
-                           ctx->y, ctx->z, ctx->q, nullptr, ctx->gridq);
-        LAUNCHCHK();
-        hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(T.Gtot, 256), 8192)),
-                           dim3(256), 0, ctx->stream, T.Gtot, ctx->gridq, ctx->grid32);
-        LAUNCHCHK();
+        if (T.binned) {
+            RETCHK(b_spread_binned(ctx));
+        } else {
+            hipLaunchKernelGGL(k_b_spread<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
+                               ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n,
+                               mdg_scatter_stride(n), ctx->x, ctx->y, ctx->z, ctx->q, nullptr,
+                               ctx->gridq);
+            LAUNCHCHK();
+            hipLaunchKernelGGL(k_grid_fix2f, dim3((int)std::min<long>(nblk(T.Gtot, 256), 8192)),
+                               dim3(256), 0, ctx->stream, T.Gtot, ctx->gridq, ctx->grid32);
+            LAUNCHCHK();
+        }
         for (MdgFftGroup &F : T.fft) {
             void *in[1] = {ctx->grid32 + F.g0}, *out[1] = {ctx->cgrid32 + 2 * F.h0};
             if (rocfft_execute(F.fwd32, in, out, ctx->info) != rocfft_status_success) {
