@@ -1348,7 +1348,7 @@ k_b_bin(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__restrict__ 
         const int *__restrict__ mesh, const int *__restrict__ binOff, int n,
         const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
         int *binCnt, int *__restrict__ bucket, int *__restrict__ binOut, int *binNOut,
-        int4 *__restrict__ splI, float4 *__restrict__ splB)
+        int4 *__restrict__ splI, float4 *__restrict__ splB, float4 *__restrict__ splD)
 {
     constexpr int NQ = SplineAtom<P>::NQ;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1360,14 +1360,20 @@ k_b_bin(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__restrict__ 
     splI[l] = make_int4(i1, i2, i3, r);
 #pragma unroll
     for (int dim = 0; dim < 3; dim++) {
-        float b[P], d[P], bq[4 * NQ];
+        float b[P], d[P], bq[4 * NQ], dq[4 * NQ];
         bspline_fix<float, P>(s[dim], b, d);
 #pragma unroll
-        for (int k = 0; k < 4 * NQ; k++) bq[k] = k < P ? b[k] : 0.f;
+        for (int k = 0; k < 4 * NQ; k++) {
+            bq[k] = k < P ? b[k] : 0.f;
+            dq[k] = k < P ? d[k] : 0.f;
+        }
 #pragma unroll
-        for (int q = 0; q < NQ; q++)
+        for (int q = 0; q < NQ; q++) {
             splB[(long)(dim * NQ + q) * n + l] =
                 make_float4(bq[4 * q], bq[4 * q + 1], bq[4 * q + 2], bq[4 * q + 3]);
+            splD[(long)(dim * NQ + q) * n + l] =
+                make_float4(dq[4 * q], dq[4 * q + 1], dq[4 * q + 2], dq[4 * q + 3]);
+        }
     }
     const int t1 = spme_top(i1, m), t2 = spme_top(i2, m), t3 = spme_top(i3, m);
     if (t1 >= 0 && t2 >= 0 && t3 >= 0) {
@@ -1486,6 +1492,70 @@ k_b_spread_tiles(MdgParams Pm, const int *__restrict__ mesh, const int *__restri
     }
 }
 
+// Batch gather after a binned spread: the atom's row of 16 lanes reads its
+// stored splines (as k_spme_gather_pre) on its replica's mesh and box.
+template <int P>
+__global__ void __launch_bounds__(256)
+k_b_gather_pre(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__restrict__ mesh,
+               const long *__restrict__ gridOff, int n, const double *__restrict__ q,
+               const int4 *__restrict__ splI, const float4 *__restrict__ splB,
+               const float4 *__restrict__ splD, const float *__restrict__ grid,
+               float *__restrict__ frx, float *__restrict__ fry, float *__restrict__ frz)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)(t / SPME_LPA), rr = (int)(t % SPME_LPA);
+    float fbx = 0.f, fby = 0.f, fbz = 0.f;
+    int r = 0, m = 1;
+    if (l < n) {
+        SplineAtom<P> S;
+        spline_load<P>(splI, splB, splD, n, l, true, S);
+        r = splI[l].w;
+        m = mesh[r];
+        const long base = gridOff[r];
+        int kk[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) kk[k] = wrap_index(S.base[2] - k, m);
+#pragma unroll
+        for (int ij0 = 0; ij0 < P * P; ij0 += SPME_LPA) {
+            const int ij = ij0 + rr;
+            if (ij >= P * P) break;
+            const int i = ij / P, j = ij % P;
+            const long o2 = base + (long)m * (wrap_index(S.base[1] - j, m) +
+                                              (long)m * wrap_index(S.base[0] - i, m));
+            float bi = S.b[0][0], bj = S.b[1][0], di = S.d[0][0], dj = S.d[1][0];
+#pragma unroll
+            for (int k = 1; k < P; k++) {
+                bi = i == k ? S.b[0][k] : bi;
+                di = i == k ? S.d[0][k] : di;
+                bj = j == k ? S.b[1][k] : bj;
+                dj = j == k ? S.d[1][k] : dj;
+            }
+            const float c1 = di * bj, c2 = bi * dj, c3 = bi * bj;
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                const float qt = grid[kk[k] + o2];
+                fbx += qt * c1 * S.b[2][k];
+                fby += qt * c2 * S.b[2][k];
+                fbz += qt * c3 * S.d[2][k];
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < SPME_LPA; o <<= 1) {
+        fbx += __shfl_xor(fbx, o, SPME_LPA);
+        fby += __shfl_xor(fby, o, SPME_LPA);
+        fbz += __shfl_xor(fbz, o, SPME_LPA);
+    }
+    if (l < n && rr == 0) {
+        const MdgBox &B = boxes[r];
+        const double gx = (double)fbx * m, gy = (double)fby * m, gz = (double)fbz * m;
+        const double c = -2.0 * (TWOPI / B.vol) * Pm.chargeConst * q[l];
+        frx[l] = (float)(c * (gx * B.u1 + gy * B.v1 + gz * B.w1));
+        fry[l] = (float)(c * (gx * B.u2 + gy * B.v2 + gz * B.w2));
+        frz[l] = (float)(c * (gx * B.u3 + gy * B.v3 + gz * B.w3));
+    }
+}
+
 // Clears the bucket counts and the outlier count for the next step.
 __global__ void k_bin_clear(long nbins, int *__restrict__ binCnt, int *__restrict__ binNOut)
 {
@@ -1535,6 +1605,7 @@ static int b_bin_alloc(mdg_ctx *ctx)
     const int nq = (ctx->P.nbsp + 3) / 4;
     HIPCHK(hipMalloc(&ctx->splI, n * sizeof(int4)));
     HIPCHK(hipMalloc(&ctx->splB, 3 * nq * (size_t)n * sizeof(float4)));
+    HIPCHK(hipMalloc(&ctx->splD, 3 * nq * (size_t)n * sizeof(float4)));
     T.binned = true;
     return MDG_OK;
 }
@@ -1546,7 +1617,8 @@ static int b_spread_binned_p(mdg_ctx *ctx)
     const int n = ctx->n;
     hipLaunchKernelGGL(k_b_bin<P>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, T.boxes,
                        T.arep, T.mesh, T.binOff, n, ctx->x, ctx->y, ctx->z, ctx->binCnt,
-                       ctx->binBucket, ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB);
+                       ctx->binBucket, ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB,
+                       ctx->splD);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_b_spread_tiles<P>, dim3(T.nTiles), dim3(256), 0, ctx->stream, ctx->P,
                        T.mesh, T.binOff, T.tileOff, T.tileRep, T.gridOff, ctx->q, ctx->binCnt,
@@ -1556,6 +1628,30 @@ static int b_spread_binned_p(mdg_ctx *ctx)
     hipLaunchKernelGGL(k_bin_clear, dim3((int)std::min<long>(nblk(ctx->nbx, 256), 1024)),
                        dim3(256), 0, ctx->stream, (long)ctx->nbx, ctx->binCnt, ctx->binNOut);
     LAUNCHCHK();
+    return MDG_OK;
+}
+
+static int b_gather_binned(mdg_ctx *ctx)
+{
+    MdgBatch &T = *ctx->bt;
+    const int n = ctx->n;
+    const dim3 g(nblk((long)n * SPME_LPA, 256)), b(256);
+    switch (ctx->P.nbsp) {
+    case 4:
+        hipLaunchKernelGGL(k_b_gather_pre<4>, g, b, 0, ctx->stream, ctx->P, T.boxes, T.mesh,
+                           T.gridOff, n, ctx->q, ctx->splI, ctx->splB, ctx->splD, ctx->grid32,
+                           ctx->frx, ctx->fry, ctx->frz);
+        break;
+    case 6:
+        hipLaunchKernelGGL(k_b_gather_pre<6>, g, b, 0, ctx->stream, ctx->P, T.boxes, T.mesh,
+                           T.gridOff, n, ctx->q, ctx->splI, ctx->splB, ctx->splD, ctx->grid32,
+                           ctx->frx, ctx->fry, ctx->frz);
+        break;
+    default:
+        hipLaunchKernelGGL(k_b_gather_pre<8>, g, b, 0, ctx->stream, ctx->P, T.boxes, T.mesh,
+                           T.gridOff, n, ctx->q, ctx->splI, ctx->splB, ctx->splD, ctx->grid32,
+                           ctx->frx, ctx->fry, ctx->frz);
+    }
     return MDG_OK;
 }
 
@@ -1696,9 +1792,13 @@ int mdg_b_spme(mdg_ctx *ctx)
                 return MDG_EHIP;
             }
         }
-        hipLaunchKernelGGL((k_b_gather<true, float, float>), dim3(nblk(n, 256)), dim3(256), 0,
-                           ctx->stream, ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n, ctx->x,
-                           ctx->y, ctx->z, ctx->q, ctx->grid32, ctx->frx, ctx->fry, ctx->frz);
+        if (T.binned)
+            RETCHK(b_gather_binned(ctx));
+        else
+            hipLaunchKernelGGL((k_b_gather<true, float, float>), dim3(nblk(n, 256)), dim3(256), 0,
+                               ctx->stream, ctx->P, T.boxes, T.arep, T.mesh, T.gridOff, n,
+                               ctx->x, ctx->y, ctx->z, ctx->q, ctx->grid32, ctx->frx, ctx->fry,
+                               ctx->frz);
         LAUNCHCHK();
         hipLaunchKernelGGL(k_b_rec_sums<float>, dim3(R), dim3(256), 0, ctx->stream, T.aoff,
                            ctx->frx, ctx->fry, ctx->frz, T.repRec);
