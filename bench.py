@@ -127,6 +127,8 @@ def single_box(args, M, configs, name):
     log(f"force error mixed vs FP64: rms {eps_rms:.3e} max {eps_max:.3e}")
 
     ctx = M.Context(wb, precision=M.MDG_MIXED)
+    if args.prune_skin is not None:
+        ctx.set_prune(args.prune_skin)
     ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
     ctx.initial_energy()
     tuning = None if args.no_tune else ctx.autotune(60)   # launch shapes, before the timed region
@@ -134,6 +136,7 @@ def single_box(args, M, configs, name):
     if W:
         ctx.run(W, use_graph=True)
     sync()
+    p0 = ctx.prune_stats()["pruned_launches"]
     t0 = time.perf_counter()
     recs = ctx.run(K, use_graph=True)                  # returns after a stream sync
     sync()
@@ -144,6 +147,10 @@ def single_box(args, M, configs, name):
     pin = float(R["nInCut"].mean())
     pis = float(R["nInCut"].sum()) / wall
     stats = ctx.list_stats()
+    pst = ctx.prune_stats()
+    prune = dict(skin_A=pst["skin"], pruned_steps=(pst["pruned_launches"] - p0) / K,
+                 pruned_entries=int(pst["pruned_entries"]), full_entries=int(stats["cluster_pairs"]))
+    log(f"pruned list: {prune}")
     log(f"timed {K} steps: {wall:.3f} s, {ms_step:.4f} ms/step, {ns_day:.1f} ns/day")
 
     # kernel durations: up to 200 more steps as plain launches on one
@@ -207,6 +214,7 @@ def single_box(args, M, configs, name):
         "pair_interactions_per_step": pin,
         "listed_pairs": plist,
         "cluster_pairs": int(stats["cluster_pairs"]),
+        "pruned_list": prune,
         "pair_interactions_per_s_pair_kernel": pin2 / (ph["pair_ms"] * 1e-3),
         "accuracy": {"force_rms_rel_err_vs_fp64": eps_rms, "force_max_rel_err_vs_fp64": eps_max,
                      "energy_drift_kcal_mol_ns_per_water_mixed": drift_mixed,
@@ -235,6 +243,8 @@ def batch(args, M, configs):
     log(f"workload batch: {R} replicas, {N} atoms, K={K}, W={W}")
     sync = sync_fn()
     b = M.BatchContext(boxes, precision=M.MDG_MIXED)
+    if args.prune_skin is not None:
+        b.set_prune(args.prune_skin)
     b.upload_boxes()
     b.initial_energy()
     tuning = None if args.no_tune else b.autotune(60)
@@ -242,10 +252,16 @@ def batch(args, M, configs):
     if W:
         b.run(W, use_graph=True)
     sync()
+    p0 = b.prune_stats()["pruned_launches"]
     t0 = time.perf_counter()
     recs = b.run(K, use_graph=True)
     sync()
     wall = time.perf_counter() - t0
+    pst = b.prune_stats()
+    prune = dict(skin_A=pst["skin"], pruned_steps=(pst["pruned_launches"] - p0) / K,
+                 pruned_entries=int(pst["pruned_entries"]),
+                 full_entries=int(b.list_stats()["cluster_pairs"]))
+    log(f"pruned list: {prune}")
     D = M.batch_records(recs, R)
     dt = cfgs[0]["timeStep"]
     ns_day = dt * 1e-3 * K * 86400.0 / wall          # every replica advances this much
@@ -284,6 +300,7 @@ def batch(args, M, configs):
         "config": {"workload": WORKLOADS["batch"], "replicas": R, "atoms": N, "dt_fs": dt * 1e3,
                    "precision": "mixed"},
         "ns_per_day_per_replica": ns_day,
+        "pruned_list": prune,
         "pair_interactions_per_s": pis,
         "phases_ms": ph,
         "tuning": tuning,
@@ -299,6 +316,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c1")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--prune-skin", type=float, default=None,
+                    help="pruned-list distance beyond the cutoff, A (0: full list every step; "
+                         "default: the library's choice for the box)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (skip the autotuner)")
     args = ap.parse_args()

@@ -277,10 +277,12 @@ int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, double dJ_d
 
 /* Autotuner: times nsteps graph-replayed steps of the uploaded system for
  * each candidate launch shape (the three-site SHAKE kernels' molecules per
- * wavefront, the cluster pair kernel's grid) and keeps the fastest; the
+ * wavefront, the cluster pair kernel's grid, and whether the pruned list
+ * pays for its prune kernel when pruning is on) and keeps the fastest; the
  * state is restored and the initial energy recomputed.  out[0] = SHAKE
- * spread (16/spread molecules per wavefront), out[1] = pair grid cap.  No
- * reference counterpart. */
+ * spread (16/spread molecules per wavefront), out[1] = pair grid cap; the
+ * pruning choice reads back through mdg_prune_skin.  No reference
+ * counterpart. */
 int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2]);
 
 /* Average kernel durations (ms) of the last mdg_run_steps, measured with
@@ -289,6 +291,23 @@ int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2]);
  * Events are recorded only when profiling is switched on. */
 int mdg_set_profiling(mdg_ctx *ctx, int on);
 int mdg_phase_times(mdg_ctx *ctx, double out[5]);
+
+/* Dynamic pruning of the cluster-pair list (mixed precision).  After each
+ * device step a kernel keeps, per work chunk, the entries with an atom pair
+ * within cutOff + skin at that step's positions; the next step's pair
+ * kernel runs on that shorter list while no atom has moved more than
+ * skin / 2 since, and on the full list otherwise.  Forces, energies and
+ * virials are the same bit for bit either way.  skin = 0 switches it off;
+ * it is capped at the list skin (delr).  Default for mixed contexts:
+ * 0.8 A where every box edge exceeds 2 (cutOff + 0.8 A + 3 A), else off
+ * (the prune kernel's test needs that room, and small boxes gain
+ * little).  No reference counterpart: the reference evaluates every
+ * listed pair (energy.c:480-541), which this reproduces. */
+int mdg_set_prune(mdg_ctx *ctx, double skin);
+double mdg_prune_skin(const mdg_ctx *ctx);   /* the current pruning distance */
+/* [0] pair kernel launches that ran on the pruned list (since creation),
+ * [1] entries in the current pruned list. */
+int mdg_prune_stats(mdg_ctx *ctx, long out[2]);
 
 /* Counters of the device cluster-pair list: [0] cluster pairs,
  * [1] listed atom pairs (mask bits), [2] clusters. */

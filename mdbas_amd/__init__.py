@@ -86,7 +86,8 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_vv_integrate mdg_upload_state mdg_download_state mdg_initial_energy "
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
            "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj "
-           "mdg_batch_create mdg_batch_size mdg_autotune").split()
+           "mdg_batch_create mdg_batch_size mdg_autotune mdg_set_prune mdg_prune_stats "
+           "mdg_prune_skin").split()
 
 _LIB = None
 
@@ -115,11 +116,14 @@ def load():
     L.mdg_batch_create.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     L.mdg_destroy.argtypes = [C.c_void_p]
+    L.mdg_prune_skin.restype = C.c_double
+    L.mdg_prune_skin.argtypes = [C.c_void_p]
     L.mdg_last_error.restype = C.c_char_p
     L.mdg_last_error.argtypes = [C.c_void_p]
     for f in EXPORTS:
         fn = getattr(L, f)
-        if f not in ("mdg_create", "mdg_batch_create", "mdg_destroy", "mdg_last_error"):
+        if f not in ("mdg_create", "mdg_batch_create", "mdg_destroy", "mdg_last_error",
+                     "mdg_prune_skin"):
             fn.restype = C.c_int
     dp = C.POINTER(C.c_double)
     L.mdg_backward.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
@@ -377,6 +381,17 @@ class Context:
         self._chk(self.L.mdg_phase_times(self.h, P(out)), "mdg_phase_times")
         return dict(pair_ms=out[0], spme_ms=out[1], integrate_ms=out[2], rebuild_ms=out[3],
                     step_ms=out[4])
+
+    def set_prune(self, skin: float):
+        """Pruning distance beyond the cutoff for the mixed pair kernel's
+        shorter per-step list (0 switches it off)."""
+        self._chk(self.L.mdg_set_prune(self.h, C.c_double(skin)), "mdg_set_prune")
+
+    def prune_stats(self):
+        out = (C.c_long * 2)()
+        self._chk(self.L.mdg_prune_stats(self.h, out), "mdg_prune_stats")
+        return dict(pruned_launches=out[0], pruned_entries=out[1],
+                    skin=float(self.L.mdg_prune_skin(self.h)))
 
     def list_stats(self):
         out = (C.c_long * 3)()

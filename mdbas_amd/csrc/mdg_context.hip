@@ -258,6 +258,25 @@ static int exclusions_from_args(mdg_ctx *ctx, int **exclList, const int *exclPai
 
 // ---------------------------------------------------------------- create
 
+// Default list-pruning distance (mixed): on when every box edge leaves
+// room for the prune kernel's cluster-relative test (half an edge above
+// cutOff + skin plus a few angstroms of cluster extent), else off.
+static double default_prune_skin(const mdg_ctx *ctx)
+{
+    if (ctx->precision != MDG_MIXED) return 0.;
+    const double skin = std::min(MDG_PRUNE_SKIN, ctx->P.delr);
+    auto roomy = [&](const MdgBox &B) {
+        const double e = std::min(B.a1, B.type == 1 ? B.a1 : std::min(B.b2, B.c3));
+        return 0.5 * e > ctx->P.cutOff + skin + 3.0;
+    };
+    if (ctx->bt) {
+        for (const MdgBox &B : ctx->bt->hBox)
+            if (!roomy(B)) return 0.;
+        return skin;
+    }
+    return roomy(ctx->B) ? skin : 0.;
+}
+
 extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, const MDG_PBC *box,
                                const MDG_EWALD *ewald, const MDG_NEIGH *neigh, int precision,
                                int *status)
@@ -291,7 +310,8 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
         hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->evRecFork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->evRecDone, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->evRecDone, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evPruneDone, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "no HIP device / stream";
         return fail(ctx, MDG_EHIP);
     }
@@ -322,6 +342,7 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
         if (int e = mdg_spme_setup(ctx)) return fail(ctx, e);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, MDG_EHIP);
     if (status) *status = MDG_OK;
+    ctx->pruneSkin = default_prune_skin(ctx);
     return ctx;
 }
 
@@ -389,7 +410,8 @@ extern "C" mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *para
         hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->evRecFork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->evRecDone, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->evRecDone, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->evPruneDone, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "no HIP device / stream";
         return fail(ctx, MDG_EHIP);
     }
@@ -419,6 +441,7 @@ extern "C" mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *para
     if ((e = mdg_b_spme_setup(ctx))) return fail(ctx, e);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, MDG_EHIP);
     if (status) *status = MDG_OK;
+    ctx->pruneSkin = default_prune_skin(ctx);
     return ctx;
 }
 
@@ -450,7 +473,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
                     ctx->shkMask, ctx->exSymOff, ctx->exSymLst, ctx->adjQ, ctx->adjE,
                     ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binBucket, ctx->binOut,
-                    ctx->binNOut, ctx->splI, ctx->splB, ctx->splD};
+                    ctx->binNOut, ctx->splI, ctx->splB, ctx->splD,
+                    ctx->cl.pJ, ctx->cl.pMask, ctx->cl.pCnt, ctx->dispPart};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->bt) {
@@ -475,6 +499,7 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
     if (ctx->evJoin) hipEventDestroy(ctx->evJoin);
     if (ctx->evRecFork) hipEventDestroy(ctx->evRecFork);
     if (ctx->evRecDone) hipEventDestroy(ctx->evRecDone);
+    if (ctx->evPruneDone) hipEventDestroy(ctx->evPruneDone);
     delete ctx;
 }
 

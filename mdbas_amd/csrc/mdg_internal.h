@@ -17,6 +17,7 @@
 #define MDG_GRP_CONS 16          // max constraints in one constraint group
 #define MDG_SEG_MAX 4096         // max atoms in one cell segment sorted in LDS
 #define MDG_CHUNK 32             // j-clusters per wavefront in the pair kernel
+#define MDG_PRUNE_SKIN 0.8       // default pruning distance beyond the cutoff (mixed), A
 
 // Fixed-point scales: forces in internal units (10 J/mol/A) carry 2^32
 // fractional steps in an int64 (|F| < 2^31); energies and virials 2^24
@@ -88,6 +89,12 @@ struct ClusterList {
     int *pcell = nullptr;
     int *sortTmp = nullptr;
     float *sortKey = nullptr;
+    // pruned copy of the list (mixed pair kernel): per chunk of the full
+    // list, its entries with a pair within cutOff + pruneSkin at the slot
+    // positions of the last prune, at the same offsets
+    int *pJ = nullptr;
+    unsigned long long *pMask = nullptr;
+    int *pCnt = nullptr;         // per chunk: entries kept
 };
 
 // Replica batch (mdg_batch.hip): many independent boxes concatenated in
@@ -156,6 +163,10 @@ struct mdg_ctx {
     // step's first SHAKE kernel (joined before that step's second one)
     hipEvent_t evRecFork = nullptr, evRecDone = nullptr;
     bool recOverlap = false, recPending = false;
+    // ... and after the record the list prune (evPruneDone), joined before
+    // the next step's list rebuild / pair kernel
+    hipEvent_t evPruneDone = nullptr;
+    bool prunePending = false;
     bool spmeForked = false;     // the current step's SPME went to stream2
     int device = 0;
 
@@ -205,6 +216,10 @@ struct mdg_ctx {
 
     // forces / energies of the device-resident path (slot order)
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;        // slot positions
+    double *dispPart = nullptr;  // per k_slot_pos block: max squared move since the last prune
+    int nDispPart = 0;
+    double pruneSkin = 0.;       // > 0: the mixed pair kernel may use the pruned list
+    bool pruneFresh = true;      // the list was rebuilt since the last slot update
     float4 *sp32 = nullptr;                                     // slot q, eps, sig
     double *sq = nullptr, *seps = nullptr, *ssig = nullptr;
     double *fdx = nullptr, *fdy = nullptr, *fdz = nullptr;     // FP64 slot forces
@@ -270,7 +285,9 @@ enum {
     CTRL_STEP = 4,               // device-side step counter
     CTRL_REBUILT = 5,
     CTRL_SEG_OVF = 6,
-    CTRL_WORDS = 8
+    CTRL_PRUNE = 7,              // 1: the pruned list belongs to the current slot positions
+    CTRL_PRUNED = 8,             // pair launches that ran on the pruned list
+    CTRL_WORDS = 12
 };
 
 // reduced scalars in ctx->scal
@@ -457,6 +474,7 @@ int mdg_pair_ref64(mdg_ctx *ctx, int elecType, double *fx, double *fy, double *f
 int mdg_excl_corr64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double *eout);
 int mdg_pair_cluster(mdg_ctx *ctx);
 int mdg_slot_params(mdg_ctx *ctx);
+int mdg_prune_list(mdg_ctx *ctx);   // mdg_pair.hip: refresh the pruned list (mixed)
 
 // spme (mdg_spme.hip)
 int mdg_spme_setup(mdg_ctx *ctx);

@@ -853,7 +853,15 @@ static int compact_cluster_list(mdg_ctx *ctx)
         C.maxChunks = (int)(C.cap / MDG_CHUNK) + nclu + 64;
         HIPCHK(hipMalloc(&C.chunk, (size_t)C.maxChunks * sizeof(int2)));
         HIPCHK(hipMalloc(&C.chOff, (nclu + 1) * sizeof(int)));
+        if (ctx->precision == MDG_MIXED) {
+            HIPCHK(hipMalloc(&C.pJ, C.cap * sizeof(int)));
+            HIPCHK(hipMalloc(&C.pMask, C.cap * sizeof(unsigned long long)));
+            HIPCHK(hipMalloc(&C.pCnt, (size_t)C.maxChunks * sizeof(int)));
+        }
     }
+    // the pruned list belongs to the previous list: the next slot update
+    // starts over from the full one
+    ctx->pruneFresh = true;
     hipLaunchKernelGGL(k_cluster_compact, dim3(nclu), dim3(64), 0, ctx->stream, nclu, C.kmax,
                        C.tmpCnt, C.tmpJ, C.tmpMask, C.cpOff, C.cap, C.cpJ, C.cpMask, C.cpI,
                        ctx->ctrl);

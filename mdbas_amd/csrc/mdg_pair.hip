@@ -272,6 +272,14 @@ struct ClusterArgs {
     const int *crep;                 // batch: replica of each cluster
     double *chE;                     // batch: per chunk eDir, eVdw, virDir, virVdw, nIn
     long fstr;                       // fiy - fix and fiz - fiy (one allocation)
+    // pruned list (mixed): per chunk the kept entries at the chunk's offsets
+    const int *pJ;
+    const unsigned long long *pMask;
+    const int *pCnt;
+    const double *disp;              // per slot block: max squared move (k_slot_pos)
+    int nDisp;
+    double pruneLim;                 // pruned list usable while 4 max move^2 <= pruneLim
+    int *ctrl;
 };
 
 // Sum over the 8 lanes of each aligned group of 8 with DPP moves (plain
@@ -530,11 +538,28 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
     double e_acc[5] = {0, 0, 0, 0, 0};
     const int nch = min(mdg_uload(A.chOff, A.nclu), A.maxChunks);
     const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    // the pruned list when no atom moved more than skin / 2 since it was made
+    // (the entries it drops contribute exact zeros, so the results are the
+    // same bit for bit)
+    bool pruned = false;
+    if (A.pruneLim > 0.) {
+        double d2 = 0.;
+        for (int k = lane; k < A.nDisp; k += 64) d2 = fmax(d2, A.disp[k]);
+        for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+        pruned = __builtin_amdgcn_readfirstlane((int)(4. * d2 <= A.pruneLim)) != 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.ctrl[CTRL_PRUNE] = 0;
+        if (pruned) A.ctrl[CTRL_PRUNED] += 1;
+    }
+    const int *__restrict__ eJ = pruned ? A.pJ : A.cpJ;
+    const unsigned long long *__restrict__ eM = pruned ? A.pMask : A.cpMask;
     for (int W = w0; W < nch; W += gridDim.x * 4) {
         const long long ch = mdg_uload((const long long *)A.chunk, W);   // int2 (I, e0)
         const int I = (int)(ch & 0xffffffffll), e0 = (int)(ch >> 32);
-        const int e1 = min(min(e0 + MDG_CHUNK, mdg_uload(A.cpOff, I + 1)), A.cap);
-        if (e0 >= e1) {   // only after a flagged list overflow
+        const int e1 = pruned ? min(e0 + mdg_uload(A.pCnt, W), A.cap)
+                              : min(min(e0 + MDG_CHUNK, mdg_uload(A.cpOff, I + 1)), A.cap);
+        if (e0 >= e1) {   // an empty pruned chunk, or a flagged list overflow
             if (BATCH && lane < 5) A.chE[(long)W * 5 + lane] = 0.;
             continue;
         }
@@ -555,18 +580,18 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
         MxAcc acc = {0.f, 0.f, 0.f, 0., 0., 0.f, 0.f, 0};
         // entries e (set 0) and e + 1 (set 1) alternate; indices past the
         // end are clamped to the last entry (loaded, never computed)
-        int J0 = mdg_uload(A.cpJ, e0);
-        unsigned long long m0 = mdg_uload(A.cpMask, e0);
+        int J0 = mdg_uload(eJ, e0);
+        unsigned long long m0 = mdg_uload(eM, e0);
         MxJ j0 = mx_load(R, J0);
         for (int e = e0;;) {
-            const int J1 = mdg_uload(A.cpJ, min(e + 1, e1 - 1));
-            const unsigned long long m1 = mdg_uload(A.cpMask, min(e + 1, e1 - 1));
+            const int J1 = mdg_uload(eJ, min(e + 1, e1 - 1));
+            const unsigned long long m1 = mdg_uload(eM, min(e + 1, e1 - 1));
             const MxJ j1 = mx_load(R, J1);
             mx_pair<BATCH>(A, j0, m0, J0, xi, yi, zi, qi, pi.y, pi.z, L0, L1, L2, iL0, iL1, iL2, a,
                            b, acc);
             if (++e >= e1) break;
-            J0 = mdg_uload(A.cpJ, min(e + 1, e1 - 1));
-            m0 = mdg_uload(A.cpMask, min(e + 1, e1 - 1));
+            J0 = mdg_uload(eJ, min(e + 1, e1 - 1));
+            m0 = mdg_uload(eM, min(e + 1, e1 - 1));
             j0 = mx_load(R, J0);
             mx_pair<BATCH>(A, j1, m1, J1, xi, yi, zi, qi, pi.y, pi.z, L0, L1, L2, iL0, iL1, iL2, a,
                            b, acc);
@@ -598,31 +623,82 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
     if (!BATCH) mdg_block_partials<5>(e_acc, A.part);
 }
 
-// Slot-ordered positions for the pair kernel (padding slots get copies of
-// a far point; their mask bits are never set), and zeroed slot force
+// Slot-ordered positions for the pair kernels (padding slots get copies
+// of a far point; their mask bits are never set), and zeroed slot force
 // accumulators of the active precision.
+//
+// Mixed precision also measures how far the slots moved since the
+// positions the pruned list was made from: that list holds, per chunk, the
+// entries with a pair within cutOff + skin at those positions, so while no
+// atom has moved more than skin / 2 since, every pair it leaves out is
+// still beyond the cutoff.  The previous slot positions are those
+// positions exactly when the CTRL_PRUNE word is set (the prune kernel sets
+// it after reading them, every pair kernel clears it, and the FP64 variant
+// of this kernel, which rewrites the slots untracked, clears it too).  Each
+// block writes its largest squared move (minimum image), or +inf when the
+// pruned list does not belong to the previous positions.
+struct SlotMove {
+    int *ctrl;
+    int fresh;                       // the list was rebuilt: slots changed atoms
+    int track;                       // pruning on (mixed, rectangular boxes)
+    MdgBox B;
+    const MdgBox *boxes;             // batch: box of each replica
+    const int *crep;                 // batch: replica of each cluster
+    double *part;                    // per block: max squared move
+};
+
 template <bool MIXED>
-__global__ void k_slot_pos(int nslot, const int *__restrict__ slotAtom, const double *__restrict__ x,
-                           const double *__restrict__ y, const double *__restrict__ z,
-                           double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz,
-                           long long *fix, long long *fiy, long long *fiz, double *fdx, double *fdy,
-                           double *fdz)
+__global__ void __launch_bounds__(256)
+k_slot_pos(int nslot, const int *__restrict__ slotAtom, const double *__restrict__ x,
+           const double *__restrict__ y, const double *__restrict__ z, double *__restrict__ sx,
+           double *__restrict__ sy, double *__restrict__ sz, long long *fix, long long *fiy,
+           long long *fiz, double *fdx, double *fdy, double *fdz, SlotMove M)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslot) return;
-    int a = slotAtom[s];
-    sx[s] = a >= 0 ? x[a] : 1e7;
-    sy[s] = a >= 0 ? y[a] : 1e7;
-    sz[s] = a >= 0 ? z[a] : 1e7;
-    if (MIXED) {
-        fix[s] = 0;
-        fiy[s] = 0;
-        fiz[s] = 0;
-    } else {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!MIXED) {
+        if (s == 0) M.ctrl[CTRL_PRUNE] = 0;
+        if (s >= nslot) return;
+        const int a = slotAtom[s];
+        sx[s] = a >= 0 ? x[a] : 1e7;
+        sy[s] = a >= 0 ? y[a] : 1e7;
+        sz[s] = a >= 0 ? z[a] : 1e7;
         fdx[s] = 0.;
         fdy[s] = 0.;
         fdz[s] = 0.;
+        return;
     }
+    const bool track = M.track && !M.fresh && M.ctrl[CTRL_PRUNE] == 1;
+    double d2 = 0.;
+    if (s < nslot) {
+        const int a = slotAtom[s];
+        const double nx = a >= 0 ? x[a] : 1e7, ny = a >= 0 ? y[a] : 1e7,
+                     nz = a >= 0 ? z[a] : 1e7;
+        if (track && a >= 0) {
+            const MdgBox &Bx = M.boxes ? M.boxes[M.crep[s / MDG_CL]] : M.B;
+            const double L0 = Bx.a1, L1 = Bx.type == 1 ? Bx.a1 : Bx.b2,
+                         L2 = Bx.type == 1 ? Bx.a1 : Bx.c3;
+            double dx = nx - sx[s], dy = ny - sy[s], dz = nz - sz[s];
+            dx -= L0 * rint(dx / L0);
+            dy -= L1 * rint(dy / L1);
+            dz -= L2 * rint(dz / L2);
+            d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 <= 1e300)) d2 = INFINITY;   // NaN positions: no pruning
+        }
+        sx[s] = nx;
+        sy[s] = ny;
+        sz[s] = nz;
+        fix[s] = 0;
+        fiy[s] = 0;
+        fiz[s] = 0;
+    }
+    if (!M.part) return;
+    for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+    __shared__ double wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        M.part[blockIdx.x] =
+            track ? fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3])) : INFINITY;
 }
 
 __global__ void k_slot_param(int nslot, const int *__restrict__ slotAtom,
@@ -698,31 +774,187 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     A.boxes = ctx->bt ? ctx->bt->boxes : nullptr;
     A.crep = ctx->bt ? ctx->bt->crep : nullptr;
     A.chE = ctx->bt ? ctx->bt->chE : nullptr;
+    A.pJ = C.pJ;
+    A.pMask = C.pMask;
+    A.pCnt = C.pCnt;
+    A.disp = ctx->dispPart;
+    A.nDisp = ctx->dispPart ? ctx->nDispPart : 0;
+    // usable while 2 max move <= skin, with a relative margin for rounding
+    A.pruneLim = ctx->pruneSkin > 0. && C.pJ ? ctx->pruneSkin * ctx->pruneSkin * (1. - 1e-6) : -1.;
+    A.ctrl = ctx->ctrl;
     return A;
+}
+
+// Slot positions (k_slot_pos) of the active precision.
+static int slot_pos(mdg_ctx *ctx, bool mixed)
+{
+    ClusterList &C = ctx->cl;
+    const int nslot = C.nclu * MDG_CL;
+    SlotMove M;
+    M.ctrl = ctx->ctrl;
+    M.fresh = ctx->pruneFresh ? 1 : 0;
+    M.track = ctx->pruneSkin > 0. ? 1 : 0;
+    M.B = ctx->B;
+    M.boxes = ctx->bt ? ctx->bt->boxes : nullptr;
+    M.crep = ctx->bt ? ctx->bt->crep : nullptr;
+    M.part = ctx->dispPart;
+    if (mixed) {
+        ctx->pruneFresh = false;
+        hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, M);
+    } else {
+        hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
+                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
+                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz, M);
+    }
+    LAUNCHCHK();
+    return MDG_OK;
 }
 
 int mdg_pair_cluster(mdg_ctx *ctx)
 {
-    ClusterList &C = ctx->cl;
-    const int nslot = C.nclu * MDG_CL;
-    if (ctx->precision == MDG_MIXED)
-        hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
-                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
-                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
-    else
-        hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
-                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
-                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
-    LAUNCHCHK();
+    const bool mixed = ctx->precision == MDG_MIXED;
+    RETCHK(slot_pos(ctx, mixed));
     ClusterArgs A = cluster_args(ctx);
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
-    if (ctx->precision == MDG_MIXED)
+    if (mixed)
         hipLaunchKernelGGL((k_pair_mixed<false>), dim3(nb), dim3(256), 0, ctx->stream, A);
     else
         hipLaunchKernelGGL((k_pair_fp64<false>), dim3(nb), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
+    return MDG_OK;
+}
+
+// The pruned list from the current slot positions: per chunk of the full
+// list, the entries with a pair within rp = cutOff + skin, in order, at the
+// chunk's own offsets.  It reads only the slot positions and the full
+// list, so it can run beside the integrator; the next k_slot_pos measures
+// the moves from here.
+//
+// One wavefront per chunk (at most MDG_CHUNK = 32 entries): lane t + 32 h
+// takes entry t and the j atoms 4h .. 4h+3 of its J cluster and tests them
+// against the 8 i atoms, which makes the 32-bit half h of the entry's pair
+// mask (bit 8 j + i).  Distances go through the first i atom c: with the
+// i offsets u_i = image(x_i - x_c) at most e per axis, every pair within
+// rp has image(x_j - x_c) - u_i as its minimum-image vector whenever
+// rp + e < L / 2 (the vector is the unique representative below L / 2).
+// A chunk where that fails keeps its entries whole.  The kept entries are
+// packed with a wave prefix count.
+__device__ inline double readlane_d(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_prune(ClusterArgs A, int *__restrict__ pJ,
+                                               unsigned int *__restrict__ pMask32,
+                                               int *__restrict__ pCnt, double rp2, double rp)
+{
+    static_assert(MDG_CHUNK <= 32, "one lane pair per entry");
+    const int lane = threadIdx.x & 63;
+    const int t = lane & 31, h = lane >> 5, ia = lane & 7;
+    const int nch = min(mdg_uload(A.chOff, A.nclu), A.maxChunks);
+    const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctrl[CTRL_PRUNE] = 1;
+    for (int W = w0; W < nch; W += gridDim.x * 4) {
+        const long long ch = mdg_uload((const long long *)A.chunk, W);   // int2 (I, e0)
+        const int I = (int)(ch & 0xffffffffll), e0 = (int)(ch >> 32);
+        const int e1 = min(min(e0 + MDG_CHUNK, mdg_uload(A.cpOff, I + 1)), A.cap);
+        const int n = max(e1 - e0, 0);
+        double L0 = A.F.L0, L1 = A.F.L1, L2 = A.F.L2, iL0 = A.F.iL0, iL1 = A.F.iL1, iL2 = A.F.iL2;
+        if (BATCH && n > 0) {
+            const MdgBox &Bx = A.boxes[mdg_uload(A.crep, I)];
+            L0 = Bx.a1;
+            L1 = Bx.type == 1 ? Bx.a1 : Bx.b2;
+            L2 = Bx.type == 1 ? Bx.a1 : Bx.c3;
+            iL0 = 1.0 / L0;
+            iL1 = 1.0 / L1;
+            iL2 = 1.0 / L2;
+        }
+        const bool live = t < n;
+        const int J = live ? A.cpJ[e0 + t] : 0;
+        const unsigned long long m = live ? A.cpMask[e0 + t] : 0ull;
+        double xj[4], yj[4], zj[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int sj = J * MDG_CL + 4 * h + q;
+            xj[q] = A.sx[sj];
+            yj[q] = A.sy[sj];
+            zj[q] = A.sz[sj];
+        }
+        // i offsets: lane ia holds atom ia's (padding slots, far points
+        // never in a mask, are parked far away and left out of e)
+        const double xc = mdg_uload(A.sx, (long)I * MDG_CL), yc = mdg_uload(A.sy, (long)I * MDG_CL),
+                     zc = mdg_uload(A.sz, (long)I * MDG_CL);
+        const long si = (long)I * MDG_CL + ia;
+        const double xi = A.sx[si];
+        double ux = xi - xc, uy = A.sy[si] - yc, uz = A.sz[si] - zc;
+        ux = fma(-L0, rint(ux * iL0), ux);
+        uy = fma(-L1, rint(uy * iL1), uy);
+        uz = fma(-L2, rint(uz * iL2), uz);
+        const bool pad = !(xi < 1e6);
+        double ex = pad ? 0. : fabs(ux), ey = pad ? 0. : fabs(uy), ez = pad ? 0. : fabs(uz);
+        for (int o = 1; o < 8; o <<= 1) {
+            ex = fmax(ex, __shfl_xor(ex, o, 64));
+            ey = fmax(ey, __shfl_xor(ey, o, 64));
+            ez = fmax(ez, __shfl_xor(ez, o, 64));
+        }
+        if (pad) ux = uy = uz = 1e7;
+        const bool ok = __builtin_amdgcn_readfirstlane(
+                            (int)(rp + ex < 0.5 * L0 - 1e-6 && rp + ey < 0.5 * L1 - 1e-6 &&
+                                  rp + ez < 0.5 * L2 - 1e-6)) != 0;
+        unsigned int bits = 0xffffffffu;
+        if (ok) {
+            bits = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                double dx = xj[q] - xc, dy = yj[q] - yc, dz = zj[q] - zc;
+                xj[q] = fma(-L0, rint(dx * iL0), dx);
+                yj[q] = fma(-L1, rint(dy * iL1), dy);
+                zj[q] = fma(-L2, rint(dz * iL2), dz);
+            }
+#pragma unroll
+            for (int i = 0; i < MDG_CL; i++) {
+                const double vx = readlane_d(ux, i), vy = readlane_d(uy, i), vz = readlane_d(uz, i);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double dx = xj[q] - vx, dy = yj[q] - vy, dz = zj[q] - vz;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    bits |= (r2 <= rp2 ? 1u : 0u) << (8 * q + i);
+                }
+            }
+        }
+        const unsigned int keep = bits & (unsigned int)(m >> (32 * h));
+        const unsigned long long nz = __builtin_amdgcn_ballot_w64(keep != 0);
+        const unsigned int kept = (unsigned int)nz | (unsigned int)(nz >> 32);   // per entry t
+        const int pos = __builtin_popcount(kept & ((1u << t) - 1u));
+        if (live && ((kept >> t) & 1u)) {
+            if (h == 0) pJ[e0 + pos] = J;
+            pMask32[2 * (long)(e0 + pos) + h] = keep;
+        }
+        if (lane == 0) pCnt[W] = __builtin_popcount(kept);
+    }
+}
+
+int mdg_prune_list(mdg_ctx *ctx)
+{
+    ClusterList &C = ctx->cl;
+    if (ctx->precision != MDG_MIXED || ctx->pruneSkin <= 0. || !C.pJ || !ctx->dispPart)
+        return MDG_OK;
+    ClusterArgs A = cluster_args(ctx);
+    const double rp = ctx->P.cutOff + ctx->pruneSkin;
+    const int nb = std::min(nblk(C.maxChunks, 4), MDG_PAIR_MAX_BLOCKS);
+    if (ctx->bt)
+        hipLaunchKernelGGL((k_prune<true>), dim3(nb), dim3(256), 0, ctx->stream, A, C.pJ,
+                           (unsigned int *)C.pMask, C.pCnt, rp * rp, rp);
+    else
+        hipLaunchKernelGGL((k_prune<false>), dim3(nb), dim3(256), 0, ctx->stream, A, C.pJ,
+                           (unsigned int *)C.pMask, C.pCnt, rp * rp, rp);
+    LAUNCHCHK();
     return MDG_OK;
 }
 
@@ -800,12 +1032,7 @@ __global__ void __launch_bounds__(256) k_adj_cluster(ClusterArgs A)
 // Real-space parameter gradients into the FP64 slot accumulators.
 int mdg_adj_pair(mdg_ctx *ctx)
 {
-    ClusterList &C = ctx->cl;
-    const int nslot = C.nclu * MDG_CL;
-    hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream, nslot,
-                       C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz, ctx->fix,
-                       ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
-    LAUNCHCHK();
+    RETCHK(slot_pos(ctx, false));
     ClusterArgs A = cluster_args(ctx);
     hipLaunchKernelGGL(k_adj_cluster, dim3(mdg_pair_blocks(ctx)), dim3(256), 0, ctx->stream, A);
     LAUNCHCHK();
@@ -817,18 +1044,8 @@ int mdg_adj_pair(mdg_ctx *ctx)
 // per-chunk energies.
 int mdg_b_pair(mdg_ctx *ctx)
 {
-    ClusterList &C = ctx->cl;
-    const int nslot = C.nclu * MDG_CL;
     const bool mixed = ctx->precision == MDG_MIXED;
-    if (mixed)
-        hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
-                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
-                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
-    else
-        hipLaunchKernelGGL(k_slot_pos<false>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
-                           nslot, C.slotAtom, ctx->x, ctx->y, ctx->z, ctx->sx, ctx->sy, ctx->sz,
-                           ctx->fix, ctx->fiy, ctx->fiz, ctx->fdx, ctx->fdy, ctx->fdz);
-    LAUNCHCHK();
+    RETCHK(slot_pos(ctx, mixed));
     ClusterArgs A = cluster_args(ctx);
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));

@@ -227,6 +227,10 @@ static int ensure_slots(mdg_ctx *ctx)
     HIPCHK(hipMalloc(&ctx->fix, 3 * (size_t)ns * sizeof(long long)));
     ctx->fiy = ctx->fix + ns;
     ctx->fiz = ctx->fix + 2 * (size_t)ns;
+    if (ctx->precision == MDG_MIXED) {
+        ctx->nDispPart = nblk(ctx->cl.nclu * MDG_CL, 256);
+        HIPCHK(hipMalloc(&ctx->dispPart, ctx->nDispPart * sizeof(double)));
+    }
     return MDG_OK;
 }
 
@@ -270,6 +274,11 @@ static int b_energy(mdg_ctx *ctx, bool rebuild);
 
 int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
 {
+    if (ctx->prunePending) {
+        // the previous step's list prune reads the list and the slot positions
+        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evPruneDone, 0));
+        ctx->prunePending = false;
+    }
     if (ctx->bt) return b_energy(ctx, rebuild);
     const int n = ctx->n;
     // SPME on the second stream, beside the list rebuild and the pair kernel;
@@ -376,12 +385,21 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
         hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
                            rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)),
                            ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
+    if (fork && st == MDG_OK) {
+        HIPCHK(hipEventRecord(ctx->evRecDone, ctx->stream2));
+        ctx->recPending = true;
+    }
+    // the next step's pruned list, from this step's slot positions (in a
+    // step block beside the next step's SHAKE kernels, joined before its
+    // list rebuild / pair kernel)
+    const bool prune = st == MDG_OK && ctx->precision == MDG_MIXED && ctx->pruneSkin > 0.;
+    if (prune) st = mdg_prune_list(ctx);
     ctx->stream = main;
     RETCHK(st);
     LAUNCHCHK();
-    if (fork) {
-        HIPCHK(hipEventRecord(ctx->evRecDone, ctx->stream2));
-        ctx->recPending = true;
+    if (fork && prune) {
+        HIPCHK(hipEventRecord(ctx->evPruneDone, ctx->stream2));
+        ctx->prunePending = true;
     }
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
     return MDG_OK;
@@ -546,8 +564,12 @@ static int capture_block(mdg_ctx *ctx)
         // the last record joins the captured stream
         if (hipStreamWaitEvent(ctx->stream, ctx->evRecDone, 0) != hipSuccess) st = MDG_EHIP;
     }
+    if (st == MDG_OK && ctx->prunePending) {
+        if (hipStreamWaitEvent(ctx->stream, ctx->evPruneDone, 0) != hipSuccess) st = MDG_EHIP;
+    }
     ctx->recOverlap = false;
     ctx->recPending = false;
+    ctx->prunePending = false;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (st) return st;
     if (e != hipSuccess) {
@@ -657,6 +679,41 @@ extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *
         HIPCHK(hipMemcpy(records, ctx->bt ? ctx->bt->rec : ctx->recDev,
                          (size_t)nsteps * R * sizeof(MDG_STEP), hipMemcpyDeviceToHost));
         for (long s = 0; s < (long)nsteps * R; s++) records[s].step = stepBase + (int)(s / R);
+    }
+    return MDG_OK;
+}
+
+extern "C" int mdg_set_prune(mdg_ctx *ctx, double skin)
+{
+    if (!ctx || !(skin >= 0.)) return MDG_EARG;
+    if (skin > 0. && ctx->precision != MDG_MIXED) {
+        ctx->err = "list pruning is a mixed-precision option";
+        return MDG_EUNSUPPORTED;
+    }
+    if (skin > ctx->P.delr) skin = ctx->P.delr;
+    ctx->pruneSkin = skin;
+    mdg_invalidate_graphs(ctx);
+    return MDG_OK;
+}
+
+extern "C" double mdg_prune_skin(const mdg_ctx *ctx) { return ctx ? ctx->pruneSkin : 0.; }
+
+extern "C" int mdg_prune_stats(mdg_ctx *ctx, long out[2])
+{
+    if (!ctx || !out) return MDG_EARG;
+    int c[CTRL_WORDS];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(c, ctx->ctrl, sizeof(c), hipMemcpyDeviceToHost));
+    out[0] = c[CTRL_PRUNED];
+    out[1] = 0;
+    ClusterList &C = ctx->cl;
+    if (C.pCnt && C.chOff && ctx->pruneSkin > 0.) {
+        int nch = 0;
+        HIPCHK(hipMemcpy(&nch, C.chOff + C.nclu, sizeof(int), hipMemcpyDeviceToHost));
+        nch = std::min(nch, C.maxChunks);
+        std::vector<int> cnt(std::max(nch, 1));
+        if (nch > 0) HIPCHK(hipMemcpy(cnt.data(), C.pCnt, nch * sizeof(int), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nch; k++) out[1] += cnt[k];
     }
     return MDG_OK;
 }
@@ -1050,6 +1107,17 @@ extern "C" int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2])
     };
     if (ctx->shakeW3) sweep(&ctx->tune.shkSpread, {1, 2, 4, 8});
     sweep(&ctx->tune.pairBlocks, {512, 768, 1024, 2048, 4096, 8192, 16384});
+    if (ctx->pruneSkin > 0. && st == MDG_OK) {
+        // the pruned list, or the full list every step (no prune kernel)
+        const double skin = ctx->pruneSkin;
+        float msOn = 0.f, msOff = 0.f;
+        mdg_invalidate_graphs(ctx);
+        st = time_steps(ctx, state, nsteps, msOn);
+        ctx->pruneSkin = 0.;
+        mdg_invalidate_graphs(ctx);
+        if (st == MDG_OK) st = time_steps(ctx, state, nsteps, msOff);
+        ctx->pruneSkin = (st == MDG_OK && msOff < msOn) ? 0. : skin;
+    }
     mdg_invalidate_graphs(ctx);
     for (int k = 0; k < 6 && st == MDG_OK; k++)
         if (hipMemcpyAsync(d[k], state + k * n, n * sizeof(double), hipMemcpyDeviceToDevice,

@@ -252,6 +252,63 @@ def test_mixed_graph_replay_is_bitwise_deterministic(c0_box):
             assert np.array_equal(u, v)
 
 
+def _same_records(a, b):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("skin", [0.5, 0.02])
+def test_mixed_pruned_list_is_exact(c0_box, skin):
+    """The pruned list leaves out only entries whose pairs stay beyond the
+    cutoff while it is in use, so with it the mixed trajectory is the one
+    of the full list bit for bit: through two list rebuilds, in step-block
+    graphs, per-step graphs and plain launches.  With a 0.5 A skin most
+    steps run on the pruned list; with 0.02 A the move guard sends most
+    steps back to the full one."""
+    wb = c0_box
+    out = []
+    for s in (0.0, skin):
+        ctx = M.Context(wb, precision=M.MDG_MIXED)
+        ctx.set_prune(s)
+        ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        r0 = ctx.initial_energy()
+        r1 = M.records_to_dict(ctx.run(45, use_graph=True))
+        r2 = M.records_to_dict(ctx.run(7, use_graph=False))
+        out.append((r0, r1, r2, ctx.download(), ctx.prune_stats(), ctx.list_stats()))
+        ctx.close()
+    (a0, a1, a2, sa, pa, la), (b0, b1, b2, sb, pb, lb) = out
+    say("prune", skin, pa, pb, la)
+    assert a0.pot == b0.pot
+    _same_records(a1, b1)
+    _same_records(a2, b2)
+    for u, v in zip(sa, sb):
+        assert np.array_equal(u, v)
+    assert pa["pruned_launches"] == 0
+    if skin == 0.5:
+        assert pb["pruned_launches"] >= 40
+        assert 0 < pb["pruned_entries"] < lb["cluster_pairs"]
+
+
+def test_batch_pruned_list_is_exact(replicas):
+    """The replica batch with and without the pruned list: the same
+    trajectories bit for bit over 25 steps (one rebuild)."""
+    out = []
+    for s in (0.0, 1.0):
+        b = M.BatchContext(replicas, precision=M.MDG_MIXED)
+        b.set_prune(s)
+        b.upload_boxes()
+        b.initial_energy()
+        r = M.batch_records(b.run(25, use_graph=True), b.R)
+        out.append((r, b.download(), b.prune_stats()))
+        b.close()
+    (ra, sa, pa), (rb, sb, pb) = out
+    say("batch prune", pa, pb)
+    _same_records(ra, rb)
+    for u, v in zip(sa, sb):
+        assert np.array_equal(u, v)
+    assert pa["pruned_launches"] == 0 and pb["pruned_launches"] >= 15
+
+
 # ------------------------------------------------------------------ adjoint
 
 @pytest.mark.parametrize("prec", [M.MDG_FP64, M.MDG_MIXED])
