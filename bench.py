@@ -131,7 +131,9 @@ def single_box(args, M, configs, name):
         ctx.set_prune(args.prune_skin)
     ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
     ctx.initial_energy()
-    tuning = None if args.no_tune else ctx.autotune(60)   # launch shapes, before the timed region
+    if args.launch:
+        ctx.set_launch(*[int(v) for v in args.launch.split(",")])
+    tuning = None if args.no_tune or args.launch else ctx.autotune(60)   # before the timed region
     log(f"autotuned launch shapes: {tuning}")
     if W:
         ctx.run(W, use_graph=True)
@@ -247,7 +249,9 @@ def batch(args, M, configs):
         b.set_prune(args.prune_skin)
     b.upload_boxes()
     b.initial_energy()
-    tuning = None if args.no_tune else b.autotune(60)
+    if args.launch:
+        b.set_launch(*[int(v) for v in args.launch.split(",")])
+    tuning = None if args.no_tune or args.launch else b.autotune(60)
     log(f"autotuned launch shapes: {tuning}")
     if W:
         b.run(W, use_graph=True)
@@ -316,6 +320,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c1")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--launch", type=str, default=None,
+                    help="fixed launch shapes 'spread,pairblocks,spmefirst' instead of autotuning")
     ap.add_argument("--prune-skin", type=float, default=None,
                     help="pruned-list distance beyond the cutoff, A (0: full list every step; "
                          "default: the library's choice for the box)")

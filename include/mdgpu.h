@@ -284,6 +284,12 @@ int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, double dJ_d
  * pruning choice reads back through mdg_prune_skin.  No reference
  * counterpart. */
 int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2]);
+/* The launch shapes in use, as the autotuner sets them: [0] SHAKE spread,
+ * [1] pair grid cap, [2] 1 when steps without a rebuild enqueue the SPME
+ * branch before the pair branch.  mdg_set_launch restores a saved set
+ * (results do not depend on them).  No reference counterpart. */
+int mdg_get_launch(const mdg_ctx *ctx, int out[3]);
+int mdg_set_launch(mdg_ctx *ctx, const int in[3]);
 
 /* Average kernel durations (ms) of the last mdg_run_steps, measured with
  * HIP events on the context stream: [0] pair kernel, [1] SPME (spline to

@@ -87,7 +87,7 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
            "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj "
            "mdg_batch_create mdg_batch_size mdg_autotune mdg_set_prune mdg_prune_stats "
-           "mdg_prune_skin").split()
+           "mdg_prune_skin mdg_get_launch mdg_set_launch").split()
 
 _LIB = None
 
@@ -348,7 +348,7 @@ class Context:
         returns them (the uploaded state is restored)."""
         out = (C.c_int * 2)()
         self._chk(self.L.mdg_autotune(self.h, int(nsteps), out), "mdg_autotune")
-        return dict(shake_spread=out[0], pair_blocks=out[1])
+        return dict(self.get_launch(), prune_skin=float(self.L.mdg_prune_skin(self.h)))
 
     # ---- adjoint --------------------------------------------------------
     def set_params(self, q, eps, sig):
@@ -381,6 +381,15 @@ class Context:
         self._chk(self.L.mdg_phase_times(self.h, P(out)), "mdg_phase_times")
         return dict(pair_ms=out[0], spme_ms=out[1], integrate_ms=out[2], rebuild_ms=out[3],
                     step_ms=out[4])
+
+    def get_launch(self):
+        out = (C.c_int * 3)()
+        self._chk(self.L.mdg_get_launch(self.h, out), "mdg_get_launch")
+        return dict(shake_spread=out[0], pair_blocks=out[1], spme_first=out[2])
+
+    def set_launch(self, shake_spread: int, pair_blocks: int, spme_first: int):
+        v = (C.c_int * 3)(shake_spread, pair_blocks, spme_first)
+        self._chk(self.L.mdg_set_launch(self.h, v), "mdg_set_launch")
 
     def set_prune(self, skin: float):
         """Pruning distance beyond the cutoff for the mixed pair kernel's
