@@ -535,3 +535,28 @@ def test_autotune_keeps_results(c0_box):
     for u, v in zip(sa[:6], sb[:6]):
         assert np.array_equal(u, v)
     assert np.allclose(ra["tot"], rb["tot"], rtol=1e-13, atol=0)
+
+
+def test_launch_shapes_keep_results(c0_box):
+    """mdg_set_launch round-trips, and the branch order at the SPME fork,
+    the SHAKE spread and the pair grid change no per-atom arithmetic: the
+    mixed trajectories' positions and velocities agree bit for bit (the
+    step records' FP64 sums of per-block partials are grouped by the launch
+    shape, so they agree to 1e-13)."""
+    wb = c0_box
+    out = []
+    for shape in ((1, 1024, 0), (2, 512, 1)):
+        ctx = M.Context(wb, precision=M.MDG_MIXED)
+        ctx.set_launch(*shape)
+        assert tuple(ctx.get_launch().values()) == shape
+        ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        ctx.initial_energy()
+        r = M.records_to_dict(ctx.run(25, use_graph=True))
+        out.append((r, ctx.download()))
+        ctx.close()
+    (ra, sa), (rb, sb) = out
+    for u, v in zip(sa[:6], sb[:6]):
+        assert np.array_equal(u, v)
+    for k in ("tot", "pot", "kin", "eRec", "virshake"):
+        assert np.allclose(ra[k], rb[k], rtol=1e-13, atol=0), k
+    assert np.array_equal(ra["nInCut"], rb["nInCut"])
