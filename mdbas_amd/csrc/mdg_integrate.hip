@@ -663,8 +663,27 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         LAUNCHCHK();
     }
     if (stage == 1) {
-        hipLaunchKernelGGL(batch ? k_w3_vv1_a<true> : w3 ? k_w3_vv1_a<false> : k_vv1_a, dim3(nb),
-                           dim3(bs), 0, ctx->stream, A);
+        if (w3) {
+            // spare blocks: the previous step's record and list prune
+            SideWork S{};
+            S.nbMain = nb;
+            if (ctx->sidePending && !batch) {
+                S.rec = 1;
+                S.R = mdg_rec_args(ctx, nb, mdg_vir_rows(ctx));
+                S.ctrl = ctx->ctrl;
+                S.out = ctx->recDev;
+                S.rebuilt = ctx->sideRebuilt ? 1 : 0;
+                if (mdg_prune_on(ctx)) {
+                    S.nbPrune = std::min(mdg_prune_blocks(ctx), 2048);
+                    S.Pr = mdg_prune_args(ctx);
+                }
+                ctx->sidePending = false;
+            }
+            hipLaunchKernelGGL(batch ? k_w3_vv1_a<true> : k_w3_vv1_a<false>,
+                               dim3(nb + S.rec + S.nbPrune), dim3(bs), 0, ctx->stream, A, S);
+        } else {
+            hipLaunchKernelGGL(k_vv1_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
+        }
         LAUNCHCHK();
         // the previous step's record (on stream2) reads the virial rows
         // that the next kernel writes

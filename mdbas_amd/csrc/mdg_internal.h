@@ -169,6 +169,9 @@ struct mdg_ctx {
     // the next step's list rebuild / pair kernel
     hipEvent_t evPruneDone = nullptr;
     bool prunePending = false;
+    // step blocks of one box with three-site SHAKE: a step's record and
+    // list prune run in spare blocks of the next step's first SHAKE kernel
+    bool sidePending = false, sideRebuilt = false;
     bool spmeForked = false;     // the current step's SPME went to stream2
     int device = 0;
 

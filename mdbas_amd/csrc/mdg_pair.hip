@@ -13,6 +13,7 @@
 //    first) with int64 fixed-point force accumulation; k_pair_fp64: FP64
 //    throughout.
 #include "mdg_internal.h"
+#include "mdg_side.h"
 
 // ewald.c:356-369
 __device__ inline double ewald_dir64(double alpha, double qel, double r, double rt, double &dE)
@@ -828,132 +829,66 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     return MDG_OK;
 }
 
-// The pruned list from the current slot positions: per chunk of the full
-// list, the entries with a pair within rp = cutOff + skin, in order, at the
-// chunk's own offsets.  It reads only the slot positions and the full
-// list, so it can run beside the integrator; the next k_slot_pos measures
-// the moves from here.
-//
-// One wavefront per chunk (at most MDG_CHUNK = 32 entries): lane t + 32 h
-// takes entry t and the j atoms 4h .. 4h+3 of its J cluster and tests them
-// against the 8 i atoms, which makes the 32-bit half h of the entry's pair
-// mask (bit 8 j + i).  Distances go through the first i atom c: with the
-// i offsets u_i = image(x_i - x_c) at most e per axis, every pair within
-// rp has image(x_j - x_c) - u_i as its minimum-image vector whenever
-// rp + e < L / 2 (the vector is the unique representative below L / 2).
-// A chunk where that fails keeps its entries whole.  The kept entries are
-// packed with a wave prefix count.
-__device__ inline double readlane_d(double v, int l)
+// The list prune as a kernel of its own (prune_waves, mdg_side.h).
+template <bool BATCH>
+__global__ void __launch_bounds__(SIDE_BLOCK) k_prune(PruneArgs A)
 {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
-                            __builtin_amdgcn_readlane(__double2loint(v), l));
+    prune_waves<BATCH>(A, blockIdx.x * (SIDE_BLOCK / 64) + (threadIdx.x >> 6),
+                       gridDim.x * (SIDE_BLOCK / 64));
 }
 
-template <bool BATCH>
-__global__ void __launch_bounds__(256) k_prune(ClusterArgs A, int *__restrict__ pJ,
-                                               unsigned int *__restrict__ pMask32,
-                                               int *__restrict__ pCnt, double rp2, double rp)
+PruneArgs mdg_prune_args(mdg_ctx *ctx)
 {
-    static_assert(MDG_CHUNK <= 32, "one lane pair per entry");
-    const int lane = threadIdx.x & 63;
-    const int t = lane & 31, h = lane >> 5, ia = lane & 7;
-    const int nch = min(mdg_uload(A.chOff, A.nclu), A.maxChunks);
-    const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctrl[CTRL_PRUNE] = 1;
-    for (int W = w0; W < nch; W += gridDim.x * 4) {
-        const long long ch = mdg_uload((const long long *)A.chunk, W);   // int2 (I, e0)
-        const int I = (int)(ch & 0xffffffffll), e0 = (int)(ch >> 32);
-        const int e1 = min(min(e0 + MDG_CHUNK, mdg_uload(A.cpOff, I + 1)), A.cap);
-        const int n = max(e1 - e0, 0);
-        double L0 = A.F.L0, L1 = A.F.L1, L2 = A.F.L2, iL0 = A.F.iL0, iL1 = A.F.iL1, iL2 = A.F.iL2;
-        if (BATCH && n > 0) {
-            const MdgBox &Bx = A.boxes[mdg_uload(A.crep, I)];
-            L0 = Bx.a1;
-            L1 = Bx.type == 1 ? Bx.a1 : Bx.b2;
-            L2 = Bx.type == 1 ? Bx.a1 : Bx.c3;
-            iL0 = 1.0 / L0;
-            iL1 = 1.0 / L1;
-            iL2 = 1.0 / L2;
-        }
-        const bool live = t < n;
-        const int J = live ? A.cpJ[e0 + t] : 0;
-        const unsigned long long m = live ? A.cpMask[e0 + t] : 0ull;
-        double xj[4], yj[4], zj[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int sj = J * MDG_CL + 4 * h + q;
-            xj[q] = A.sx[sj];
-            yj[q] = A.sy[sj];
-            zj[q] = A.sz[sj];
-        }
-        // i offsets: lane ia holds atom ia's (padding slots, far points
-        // never in a mask, are parked far away and left out of e)
-        const double xc = mdg_uload(A.sx, (long)I * MDG_CL), yc = mdg_uload(A.sy, (long)I * MDG_CL),
-                     zc = mdg_uload(A.sz, (long)I * MDG_CL);
-        const long si = (long)I * MDG_CL + ia;
-        const double xi = A.sx[si];
-        double ux = xi - xc, uy = A.sy[si] - yc, uz = A.sz[si] - zc;
-        ux = fma(-L0, rint(ux * iL0), ux);
-        uy = fma(-L1, rint(uy * iL1), uy);
-        uz = fma(-L2, rint(uz * iL2), uz);
-        const bool pad = !(xi < 1e6);
-        double ex = pad ? 0. : fabs(ux), ey = pad ? 0. : fabs(uy), ez = pad ? 0. : fabs(uz);
-        for (int o = 1; o < 8; o <<= 1) {
-            ex = fmax(ex, __shfl_xor(ex, o, 64));
-            ey = fmax(ey, __shfl_xor(ey, o, 64));
-            ez = fmax(ez, __shfl_xor(ez, o, 64));
-        }
-        if (pad) ux = uy = uz = 1e7;
-        const bool ok = __builtin_amdgcn_readfirstlane(
-                            (int)(rp + ex < 0.5 * L0 - 1e-6 && rp + ey < 0.5 * L1 - 1e-6 &&
-                                  rp + ez < 0.5 * L2 - 1e-6)) != 0;
-        unsigned int bits = 0xffffffffu;
-        if (ok) {
-            bits = 0;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                double dx = xj[q] - xc, dy = yj[q] - yc, dz = zj[q] - zc;
-                xj[q] = fma(-L0, rint(dx * iL0), dx);
-                yj[q] = fma(-L1, rint(dy * iL1), dy);
-                zj[q] = fma(-L2, rint(dz * iL2), dz);
-            }
-#pragma unroll
-            for (int i = 0; i < MDG_CL; i++) {
-                const double vx = readlane_d(ux, i), vy = readlane_d(uy, i), vz = readlane_d(uz, i);
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const double dx = xj[q] - vx, dy = yj[q] - vy, dz = zj[q] - vz;
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    bits |= (r2 <= rp2 ? 1u : 0u) << (8 * q + i);
-                }
-            }
-        }
-        const unsigned int keep = bits & (unsigned int)(m >> (32 * h));
-        const unsigned long long nz = __builtin_amdgcn_ballot_w64(keep != 0);
-        const unsigned int kept = (unsigned int)nz | (unsigned int)(nz >> 32);   // per entry t
-        const int pos = __builtin_popcount(kept & ((1u << t) - 1u));
-        if (live && ((kept >> t) & 1u)) {
-            if (h == 0) pJ[e0 + pos] = J;
-            pMask32[2 * (long)(e0 + pos) + h] = keep;
-        }
-        if (lane == 0) pCnt[W] = __builtin_popcount(kept);
-    }
+    ClusterList &C = ctx->cl;
+    const ClusterArgs A = cluster_args(ctx);
+    PruneArgs P;
+    P.nclu = C.nclu;
+    P.maxChunks = C.maxChunks;
+    P.cap = (int)C.cap;
+    P.chOff = C.chOff;
+    P.cpOff = C.cpOff;
+    P.cpJ = C.cpJ;
+    P.cpMask = C.cpMask;
+    P.chunk = C.chunk;
+    P.sx = ctx->sx;
+    P.sy = ctx->sy;
+    P.sz = ctx->sz;
+    P.L0 = A.F.L0;
+    P.L1 = A.F.L1;
+    P.L2 = A.F.L2;
+    P.iL0 = A.F.iL0;
+    P.iL1 = A.F.iL1;
+    P.iL2 = A.F.iL2;
+    P.boxes = A.boxes;
+    P.crep = A.crep;
+    P.pJ = C.pJ;
+    P.pMask32 = (unsigned int *)C.pMask;
+    P.pCnt = C.pCnt;
+    P.rp = ctx->P.cutOff + ctx->pruneSkin;
+    P.rp2 = P.rp * P.rp;
+    P.ctrl = ctx->ctrl;
+    return P;
+}
+
+bool mdg_prune_on(const mdg_ctx *ctx)
+{
+    return ctx->precision == MDG_MIXED && ctx->pruneSkin > 0. && ctx->cl.pJ && ctx->dispPart;
+}
+
+int mdg_prune_blocks(const mdg_ctx *ctx)
+{
+    return std::min(nblk(ctx->cl.maxChunks, SIDE_BLOCK / 64), MDG_PAIR_MAX_BLOCKS);
 }
 
 int mdg_prune_list(mdg_ctx *ctx)
 {
-    ClusterList &C = ctx->cl;
-    if (ctx->precision != MDG_MIXED || ctx->pruneSkin <= 0. || !C.pJ || !ctx->dispPart)
-        return MDG_OK;
-    ClusterArgs A = cluster_args(ctx);
-    const double rp = ctx->P.cutOff + ctx->pruneSkin;
-    const int nb = std::min(nblk(C.maxChunks, 4), MDG_PAIR_MAX_BLOCKS);
+    if (!mdg_prune_on(ctx)) return MDG_OK;
+    const PruneArgs P = mdg_prune_args(ctx);
+    const int nb = mdg_prune_blocks(ctx);
     if (ctx->bt)
-        hipLaunchKernelGGL((k_prune<true>), dim3(nb), dim3(256), 0, ctx->stream, A, C.pJ,
-                           (unsigned int *)C.pMask, C.pCnt, rp * rp, rp);
+        hipLaunchKernelGGL((k_prune<true>), dim3(nb), dim3(SIDE_BLOCK), 0, ctx->stream, P);
     else
-        hipLaunchKernelGGL((k_prune<false>), dim3(nb), dim3(256), 0, ctx->stream, A, C.pJ,
-                           (unsigned int *)C.pMask, C.pCnt, rp * rp, rp);
+        hipLaunchKernelGGL((k_prune<false>), dim3(nb), dim3(SIDE_BLOCK), 0, ctx->stream, P);
     LAUNCHCHK();
     return MDG_OK;
 }

@@ -5,6 +5,7 @@
 // replayed.
 #include "mdg_internal.h"
 #include "mdg_shake.h"
+#include "mdg_side.h"
 
 #include <cstring>
 #include <initializer_list>
@@ -28,32 +29,6 @@ struct FinArgs {
     double *part;                        // per block: eCorr, virCorr
     unsigned long long *mask;            // both SHAKE masks, reset here
 };
-
-// Adds rows [0, nrow) of a row-major partial table with NC columns into
-// acc[0..NC) (this thread's rows only).
-// Four rows per thread are loaded before they are added, so a thread has
-// several loads in flight (the partials were just written, on every XCD).
-template <int NC>
-__device__ inline void rows_add(const double *__restrict__ p, int nrow, double *acc)
-{
-    constexpr int U = 4;
-    const int step = blockDim.x;
-    int r = threadIdx.x;
-    for (; r + (U - 1) * step < nrow; r += U * step) {
-        double v[U][NC];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int c = 0; c < NC; c++) v[u][c] = p[(long)(r + u * step) * NC + c];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int c = 0; c < NC; c++) acc[c] += v[u][c];
-    }
-    for (; r < nrow; r += step)
-#pragma unroll
-        for (int c = 0; c < NC; c++) acc[c] += p[(long)r * NC + c];
-}
 
 // Per atom: F = slot pair force + (reciprocal force - mean reciprocal
 // force) + excluded-pair corrections.  The mean is over the reciprocal
@@ -138,74 +113,11 @@ __global__ void k_kinetic(int n, const double *__restrict__ mass, const double *
     mdg_block_partials<7>(acc, part);
 }
 
-struct RecArgs {
-    double fact, eSelf, eNet;
-    int recCap;
-    // per-block partial counts of this step's producers
-    int nbPair, nbExcl, nbInfl, nbKin, nGrpVir;
-    const double *pPair, *pExcl, *pInfl, *pKin, *grpVir;
-};
-
-
-// One MDG_STEP record: reduces the step's partials (pair, correction,
-// influence function, kinetic energy, SHAKE virial) in one block, in a
-// fixed order, and assembles the energies (spme.c:674-697 for the
-// reciprocal energy and virial, energy.c:623-627 and 193-196 for the
-// sums); resets the SHAKE mask for the next step's VV stage 1.
-#define REC_NV 16
-__global__ void __launch_bounds__(1024) k_record(RecArgs R, int *ctrl, MDG_STEP *rec, int rebuilt)
+// The step record as a kernel of its own (record_block, mdg_side.h).
+__global__ void __launch_bounds__(SIDE_BLOCK) k_record(RecArgs R, int *ctrl, MDG_STEP *rec,
+                                                       int rebuilt)
 {
-    __shared__ double red[16][REC_NV];
-    double v[REC_NV];
-#pragma unroll
-    for (int k = 0; k < REC_NV; k++) v[k] = 0.;
-    rows_add<5>(R.pPair, R.nbPair, v);
-    rows_add<2>(R.pExcl, R.nbExcl, v + 5);
-    rows_add<7>(R.pInfl, R.nbInfl, v + 7);
-    for (int r = threadIdx.x; r < R.nbKin; r += blockDim.x) v[14] += R.pKin[(long)r * 7];   // ~160 rows
-    rows_add<1>(R.grpVir, R.nGrpVir, v + 15);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < REC_NV; k++) {
-        const double s = mdg_wave_sum(v[k]);
-        if (lane == 0) red[wid][k] = s;
-    }
-    __syncthreads();
-    // thread k adds value k over the wavefronts (in wavefront order)
-    __shared__ double tot[REC_NV];
-    const int nw = blockDim.x >> 6;
-    if (threadIdx.x < REC_NV) {
-        double s = 0.;
-        for (int w = 0; w < nw; w++) s += red[w][threadIdx.x];
-        tot[threadIdx.x] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < REC_NV; k++) v[k] = tot[k];
-    const int idx = ctrl[CTRL_STEP];
-    const double er = v[7];
-    const double st0 = R.fact * (v[8] + er) + R.eNet;
-    const double st3 = R.fact * (v[11] + er) + R.eNet;
-    const double st5 = R.fact * (v[13] + er) + R.eNet;
-    const double eRec = R.fact * er + R.eSelf + R.eNet;
-    const double virRec = -(st0 + st3 + st5);
-    MDG_STEP s;
-    s.step = idx;
-    s.listRebuilt = rebuilt;
-    s.eDir = v[0];
-    s.eRec = eRec;
-    s.eCorr = v[5];
-    s.elec = s.eDir + eRec + s.eCorr;
-    s.vdw = v[1];
-    s.pot = s.elec + s.vdw;
-    s.kin = 0.5 * v[14];
-    s.tot = s.pot + s.kin;
-    s.virelec = v[2] + virRec + v[6];
-    s.virvdw = v[3];
-    s.virshake = v[15];
-    s.nInCut = v[4];
-    if (idx < R.recCap) rec[idx] = s;
-    ctrl[CTRL_STEP] = idx + 1;
+    record_block(R, ctrl, rec, rebuilt);
 }
 
 static int ensure_slots(mdg_ctx *ctx)
@@ -342,7 +254,7 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
     return MDG_OK;
 }
 
-static RecArgs rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
+RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
 {
     RecArgs R;
     R.fact = (TWOPI_R / ctx->B.vol) * ctx->P.chargeConst;
@@ -373,8 +285,17 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     RETCHK(mdg_device_energy(ctx, rebuild));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[10], ctx->stream));
     RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
-    // In a step block the record goes to stream2, beside the next step's
-    // first SHAKE kernel; the next step's second one waits for it.
+    // In a step block of one box with three-site SHAKE, the record and the
+    // list prune go with the next step's first SHAKE kernel (its spare
+    // blocks); the block's last step runs them after the loop.
+    if (ctx->recOverlap && !ctx->profiling && !ctx->bt && ctx->shakeW3) {
+        ctx->sidePending = true;
+        ctx->sideRebuilt = rebuild;
+        if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
+        return MDG_OK;
+    }
+    // Else, in a step block (batch) the record goes to stream2, beside the
+    // next step's first SHAKE kernel; the next step's second one waits for it.
     const bool fork = ctx->recOverlap && !ctx->profiling;
     hipStream_t main = ctx->stream;
     if (fork) {
@@ -386,8 +307,8 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     if (ctx->bt)
         st = b_record(ctx, rebuild);
     else
-        hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
-                           rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)),
+        hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
+                           mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)),
                            ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     if (fork && st == MDG_OK) {
         HIPCHK(hipEventRecord(ctx->evRecDone, ctx->stream2));
@@ -527,8 +448,8 @@ extern "C" int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec)
     hipLaunchKernelGGL(k_kinetic, dim3(nblk(ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n,
                        ctx->mass, ctx->vx, ctx->vy, ctx->vz, ctx->epart + PART_KIN);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(1024), 0, ctx->stream,
-                       rec_args(ctx, nblk(ctx->n, 256), 0), ctx->ctrl, ctx->recDev, 1);
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
+                       mdg_rec_args(ctx, nblk(ctx->n, 256), 0), ctx->ctrl, ctx->recDev, 1);
     LAUNCHCHK();
     RETCHK(mdg_check_flags(ctx));
     if (rec) HIPCHK(hipMemcpy(rec, ctx->recDev, sizeof(MDG_STEP), hipMemcpyDeviceToHost));
@@ -564,6 +485,15 @@ static int capture_block(mdg_ctx *ctx)
     ctx->recOverlap = true;
     int st = MDG_OK;
     for (int k = 0; k < U && st == MDG_OK; k++) st = mdg_device_step(ctx, k == U - 1);
+    if (st == MDG_OK && ctx->sidePending) {
+        // the last step's record and prune
+        hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
+                           mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)), ctx->ctrl,
+                           ctx->recDev, ctx->sideRebuilt ? 1 : 0);
+        if (hipGetLastError() != hipSuccess) st = MDG_EHIP;
+        if (st == MDG_OK) st = mdg_prune_list(ctx);
+    }
+    ctx->sidePending = false;
     if (st == MDG_OK && ctx->recPending) {
         // the last record joins the captured stream
         if (hipStreamWaitEvent(ctx->stream, ctx->evRecDone, 0) != hipSuccess) st = MDG_EHIP;

@@ -2,6 +2,7 @@
 // mdg_integrate.hip, three-site molecules in mdg_shake_w3.hip).
 #pragma once
 #include "mdg_internal.h"
+#include "mdg_side.h"
 
 // Window of SHAKE states kept per group around its first convergence.
 #define SHK_W 8
@@ -156,7 +157,7 @@ __device__ inline void shake_image(const MdgBox &B, double &xs, double &ys, doub
     }
 }
 
-template <bool BATCH> __global__ void k_w3_vv1_a(ShakeArgs A);
+template <bool BATCH> __global__ void k_w3_vv1_a(ShakeArgs A, SideWork S);
 template <bool BATCH> __global__ void k_w3_vv1_b(ShakeArgs A);
 template <bool BATCH> __global__ void k_w3_vv2_a(ShakeArgs A);
 template <bool BATCH> __global__ void k_w3_vv2_b(ShakeArgs A);

@@ -383,11 +383,16 @@ __device__ inline void q_publish(const ShakeArgs &A, int g, const unsigned long 
 
 // Stage 1, part A: half kick, drift, SHAKE-R sweeps of each molecule on
 // its own, recording the sweeps at which it was converged and the states
-// of a window after its first convergence.
+// of a window after its first convergence.  Blocks past S.nbMain do the
+// previous step's record and list prune (mdg_side.h).
 template <bool BATCH>
-__global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A)
+__global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
 {
 #pragma clang fp contract(off)
+    if ((int)blockIdx.x >= S.nbMain) {
+        if (!BATCH) side_work(S);
+        return;
+    }
     const int g = shake_w3_group(A);
     const int k = threadIdx.x & 3;
     // every lane reaches the wave-wide AND; idle molecules contribute all ones
@@ -599,8 +604,8 @@ __global__ void __launch_bounds__(256) k_w3_vv2_b(ShakeArgs A)
     mdg_block_partials<7>(acc, A.part);
 }
 
-template __global__ void k_w3_vv1_a<false>(ShakeArgs);
-template __global__ void k_w3_vv1_a<true>(ShakeArgs);
+template __global__ void k_w3_vv1_a<false>(ShakeArgs, SideWork);
+template __global__ void k_w3_vv1_a<true>(ShakeArgs, SideWork);
 template __global__ void k_w3_vv1_b<false>(ShakeArgs);
 template __global__ void k_w3_vv1_b<true>(ShakeArgs);
 template __global__ void k_w3_vv2_a<false>(ShakeArgs);
