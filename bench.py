@@ -321,7 +321,8 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c1")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--launch", type=str, default=None,
-                    help="fixed launch shapes 'spread,pairblocks,spmefirst' instead of autotuning")
+                    help="fixed launch shapes 'spread,pairblocks,spmefirst,spmeprio' instead of "
+                         "autotuning")
     ap.add_argument("--prune-skin", type=float, default=None,
                     help="pruned-list distance beyond the cutoff, A (0: full list every step; "
                          "default: the library's choice for the box)")

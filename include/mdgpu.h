@@ -277,8 +277,9 @@ int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, double dJ_d
 
 /* Autotuner: times nsteps graph-replayed steps of the uploaded system for
  * each candidate launch shape (the three-site SHAKE kernels' molecules per
- * wavefront, the cluster pair kernel's grid, and whether the pruned list
- * pays for its prune kernel when pruning is on) and keeps the fastest; the
+ * wavefront, the cluster pair kernel's grid, the branch order and wave
+ * priority of the SPME kernels beside it, and whether the pruned list pays
+ * for its prune kernel when pruning is on) and keeps the fastest; the
  * state is restored and the initial energy recomputed.  out[0] = SHAKE
  * spread (16/spread molecules per wavefront), out[1] = pair grid cap; the
  * pruning choice reads back through mdg_prune_skin.  No reference
@@ -286,10 +287,12 @@ int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, double dJ_d
 int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2]);
 /* The launch shapes in use, as the autotuner sets them: [0] SHAKE spread,
  * [1] pair grid cap, [2] 1 when steps without a rebuild enqueue the SPME
- * branch before the pair branch.  mdg_set_launch restores a saved set
- * (results do not depend on them).  No reference counterpart. */
-int mdg_get_launch(const mdg_ctx *ctx, int out[3]);
-int mdg_set_launch(mdg_ctx *ctx, const int in[3]);
+ * branch before the pair branch, [3] 1 when the binned SPME kernels raise
+ * their wave priority above the pair kernel's beside them.  mdg_set_launch
+ * restores a saved set (per-atom results do not depend on them).  No
+ * reference counterpart. */
+int mdg_get_launch(const mdg_ctx *ctx, int out[4]);
+int mdg_set_launch(mdg_ctx *ctx, const int in[4]);
 
 /* Average kernel durations (ms) of the last mdg_run_steps, measured with
  * HIP events on the context stream: [0] pair kernel, [1] SPME (spline to

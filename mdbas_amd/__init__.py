@@ -383,12 +383,13 @@ class Context:
                     step_ms=out[4])
 
     def get_launch(self):
-        out = (C.c_int * 3)()
+        out = (C.c_int * 4)()
         self._chk(self.L.mdg_get_launch(self.h, out), "mdg_get_launch")
-        return dict(shake_spread=out[0], pair_blocks=out[1], spme_first=out[2])
+        return dict(shake_spread=out[0], pair_blocks=out[1], spme_first=out[2], spme_prio=out[3])
 
-    def set_launch(self, shake_spread: int, pair_blocks: int, spme_first: int):
-        v = (C.c_int * 3)(shake_spread, pair_blocks, spme_first)
+    def set_launch(self, shake_spread: int, pair_blocks: int, spme_first: int = 0,
+                   spme_prio: int = 0):
+        v = (C.c_int * 4)(shake_spread, pair_blocks, spme_first, spme_prio)
         self._chk(self.L.mdg_set_launch(self.h, v), "mdg_set_launch")
 
     def set_prune(self, skin: float):

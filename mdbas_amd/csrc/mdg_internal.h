@@ -44,6 +44,7 @@ struct MdgParams {
     int m1, m2, m3, nbsp;
     int elecType;
     int vdwOn;                   // vdwType == VSWITCH (vdw full / none evaluate nothing)
+    int spmePrio;                // binned SPME kernels raise their wave priority (s_setprio)
 };
 
 // Reference-format list on the device.
@@ -145,6 +146,8 @@ struct MdgTune {
                                  // wave slots for the SPME kernels beside it)
     int spmeFirst = 0;           // steps without a rebuild enqueue the SPME branch before the
                                  // pair branch (the first branch after a fork starts first)
+    int spmePrio = 0;            // the binned SPME kernels' waves issue ahead of the pair
+                                 // kernel's when both share a SIMD
 };
 
 struct mdg_ctx {

@@ -632,26 +632,29 @@ extern "C" int mdg_set_prune(mdg_ctx *ctx, double skin)
 
 extern "C" double mdg_prune_skin(const mdg_ctx *ctx) { return ctx ? ctx->pruneSkin : 0.; }
 
-extern "C" int mdg_get_launch(const mdg_ctx *ctx, int out[3])
+extern "C" int mdg_get_launch(const mdg_ctx *ctx, int out[4])
 {
     if (!ctx || !out) return MDG_EARG;
     out[0] = ctx->tune.shkSpread;
     out[1] = ctx->tune.pairBlocks;
     out[2] = ctx->tune.spmeFirst;
+    out[3] = ctx->tune.spmePrio;
     return MDG_OK;
 }
 
-extern "C" int mdg_set_launch(mdg_ctx *ctx, const int in[3])
+extern "C" int mdg_set_launch(mdg_ctx *ctx, const int in[4])
 {
     if (!ctx || !in) return MDG_EARG;
     if (!(in[0] == 1 || in[0] == 2 || in[0] == 4 || in[0] == 8 || in[0] == 16) || in[1] < 1 ||
-        (in[2] != 0 && in[2] != 1)) {
-        ctx->err = "launch shapes: SHAKE spread 1, 2, 4, 8 or 16, pair grid >= 1, SPME-first 0 or 1";
+        (in[2] != 0 && in[2] != 1) || (in[3] != 0 && in[3] != 1)) {
+        ctx->err = "launch shapes: SHAKE spread 1, 2, 4, 8 or 16, pair grid >= 1, SPME-first and "
+                   "SPME priority 0 or 1";
         return MDG_EARG;
     }
     ctx->tune.shkSpread = in[0];
     ctx->tune.pairBlocks = in[1];
     ctx->tune.spmeFirst = in[2];
+    ctx->tune.spmePrio = in[3];
     mdg_invalidate_graphs(ctx);
     return MDG_OK;
 }
@@ -1039,9 +1042,9 @@ static int time_steps(mdg_ctx *ctx, const double *state, int nsteps, float &ms)
 
 // Picks the launch shapes of this system on this GPU by timing whole steps
 // (graph-replayed, as mdg_run_steps runs them): the three-site SHAKE
-// spread, the pair kernel's grid cap, the branch order at the fork, and
-// (when pruning is on) the pruned list against the full one, one after the
-// other with the rest held.  The uploaded state is restored afterwards and
+// spread, the pair kernel's grid cap, the branch order at the fork, the
+// SPME kernels' wave priority, and (when pruning is on) the pruned list
+// against the full one, one after the other with the rest held.  The uploaded state is restored afterwards and
 // the initial energy recomputed.  out (may be NULL): the chosen spread and
 // pair-grid cap (mdg_get_launch has all of them).
 extern "C" int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2])
@@ -1078,6 +1081,7 @@ extern "C" int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2])
     if (ctx->shakeW3) sweep(&ctx->tune.shkSpread, {1, 2, 4, 8});
     sweep(&ctx->tune.pairBlocks, {512, 768, 1024, 2048, 4096, 8192, 16384});
     sweep(&ctx->tune.spmeFirst, {0, 1});
+    sweep(&ctx->tune.spmePrio, {0, 1});
     if (ctx->pruneSkin > 0. && st == MDG_OK) {
         // the pruned list, or the full list every step (no prune kernel)
         const double skin = ctx->pruneSkin;

@@ -466,6 +466,7 @@ k_spme_bin(MdgParams Pm, MdgBox B, int n, int nby, int nbz, const double *__rest
            int *__restrict__ bucket, int *__restrict__ binOut, int *binNOut,
            int4 *__restrict__ splI, float4 *__restrict__ splB, float4 *__restrict__ splD)
 {
+    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int NQ = SplineAtom<P>::NQ;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
@@ -527,6 +528,7 @@ k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
                     const int *__restrict__ binNOut, const int4 *__restrict__ splI,
                     const float4 *__restrict__ splB, float *__restrict__ grid32)
 {
+    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
     constexpr int NB3 = NBD * NBD * NBD;
     static_assert(NB3 <= 128, "bin list of a tile exceeds the scan width");
@@ -689,6 +691,7 @@ k_spme_gather_pre(MdgParams Pm, MdgBox B, int n, const double *__restrict__ q,
                   double *__restrict__ part, int nbins, int *__restrict__ binCnt,
                   int *__restrict__ binNOut)
 {
+    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     // the spread tiles have read the buckets: clear them for the next step
     for (long b = t; b < nbins; b += (long)gridDim.x * blockDim.x) binCnt[b] = 0;
@@ -1096,6 +1099,7 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
 // atom go to frx64.. (FP64 mode) or frx.. (MIXED).
 int mdg_spme_device(mdg_ctx *ctx)
 {
+    ctx->P.spmePrio = ctx->tune.spmePrio;
     const bool mixed = ctx->precision == MDG_MIXED;
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[2], ctx->stream));
     if (mixed)
@@ -1350,6 +1354,7 @@ k_b_bin(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__restrict__ 
         int *binCnt, int *__restrict__ bucket, int *__restrict__ binOut, int *binNOut,
         int4 *__restrict__ splI, float4 *__restrict__ splB, float4 *__restrict__ splD)
 {
+    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int NQ = SplineAtom<P>::NQ;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
@@ -1398,6 +1403,7 @@ k_b_spread_tiles(MdgParams Pm, const int *__restrict__ mesh, const int *__restri
                  const int4 *__restrict__ splI, const float4 *__restrict__ splB,
                  float *__restrict__ grid32)
 {
+    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
     constexpr int NB3 = NBD * NBD * NBD;
     static_assert(NB3 <= 128, "bin list of a tile exceeds the scan width");
@@ -1502,6 +1508,7 @@ k_b_gather_pre(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__rest
                const float4 *__restrict__ splD, const float *__restrict__ grid,
                float *__restrict__ frx, float *__restrict__ fry, float *__restrict__ frz)
 {
+    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int l = (int)(t / SPME_LPA), rr = (int)(t % SPME_LPA);
     float fbx = 0.f, fby = 0.f, fbz = 0.f;
@@ -1757,6 +1764,7 @@ void mdg_b_spme_free(mdg_ctx *ctx)
 // and sums of the reciprocal forces (repRec).
 int mdg_b_spme(mdg_ctx *ctx)
 {
+    ctx->P.spmePrio = ctx->tune.spmePrio;
     MdgBatch &T = *ctx->bt;
     const int n = ctx->n, R = T.R;
     const double rAlpha2 = -0.25 / (ctx->P.alpha * ctx->P.alpha);

@@ -545,7 +545,7 @@ def test_launch_shapes_keep_results(c0_box):
     shape, so they agree to 1e-13)."""
     wb = c0_box
     out = []
-    for shape in ((1, 1024, 0), (2, 512, 1)):
+    for shape in ((1, 1024, 0, 0), (2, 512, 1, 1)):
         ctx = M.Context(wb, precision=M.MDG_MIXED)
         ctx.set_launch(*shape)
         assert tuple(ctx.get_launch().values()) == shape
