@@ -159,7 +159,11 @@ class OracleSystem:
         n = self.wb.n
         np_ = np.zeros(n, np.int32)
         no = np.zeros(n, np.int32)
-        cap = 1
+        # room for the expected pair count (half the atoms within cutOff +
+        # delr of each atom, with headroom), so one pass normally suffices
+        rl = self.cfg["cutOff"] + self.cfg["delr"]
+        cap = max(getattr(self, "_cap", 0),
+                  int(0.5 * n * 4.18879 * rl ** 3 * n / self.wb.L ** 3 * 1.3) + 1024)
         nl = np.zeros(cap, np.int32)
         for _ in range(2):
             if self.keyLink:
@@ -177,6 +181,7 @@ class OracleSystem:
             cap = int(cnt)
             nl = np.zeros(cap, np.int32)
             build = False
+        self._cap = cap
         self.neighList, self.neighPair, self.neighOrder = nl[:cnt], np_, no
         return self.neighList, self.neighPair, self.neighOrder
 
