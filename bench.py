@@ -106,6 +106,17 @@ def pair_traffic(name):
         return json.load(f)["bytes_per_launch"]
 
 
+def spme_traffic(name):
+    """HBM bytes of the mixed SPME kernels per step of the 30k-atom box, from
+    the same committed FETCH_SIZE / WRITE_SIZE runs; None for other
+    workloads."""
+    p = os.path.join(HERE, "profiles", "spme_traffic.json")
+    if name != "c1" or not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)["bytes_per_step"]
+
+
 def single_box(args, M, configs, name):
     cfg = configs.CONFIGS[name]
     wb = configs.build(cfg)
@@ -231,7 +242,8 @@ def single_box(args, M, configs, name):
         "roofline_spme": {"bound": "hbm", "achieved": spme_bytes / (ph["spme_ms"] * 1e-3) / 1e9,
                           "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                           "frac": spme_bytes / (ph["spme_ms"] * 1e-3) / HBM_PEAK,
-                          "bytes_per_step": spme_bytes},
+                          "bytes_per_step": spme_bytes, "traffic": spme_traffic(name)},
+        "rebuild_ms_amortised": ph["rebuild_ms"] / cfg["update"],
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (ns_day / cpu["value"]) if cpu else None,
     }
