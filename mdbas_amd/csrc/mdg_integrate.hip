@@ -590,7 +590,7 @@ int mdg_vir_rows(const mdg_ctx *ctx)
 // kinetic partials to epart + PART_KIN (stage 2); with sync the sums land
 // in scal[S_VIRSHAKE] and scal[S_KIN..S_KIN+6] and are returned.
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
-                 double *virshake_out, bool sync)
+                 double *virshake_out, bool sync, const FinArgs *fin)
 {
     ShakeArgs A;
     A.P = ctx->P;
@@ -669,7 +669,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
             S.nbMain = nb;
             if (ctx->sidePending && !batch) {
                 S.rec = 1;
-                S.R = mdg_rec_args(ctx, nb, mdg_vir_rows(ctx));
+                S.R = mdg_rec_args(ctx, nb, mdg_vir_rows(ctx), mdg_excl_rows(ctx));
                 S.ctrl = ctx->ctrl;
                 S.out = ctx->recDev;
                 S.rebuilt = ctx->sideRebuilt ? 1 : 0;
@@ -700,8 +700,18 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
             RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, mdg_vir_rows(ctx), 1,
                                        ctx->scal + S_VIRSHAKE));
     } else {
-        hipLaunchKernelGGL(batch ? k_w3_vv2_a<true> : w3 ? k_w3_vv2_a<false> : k_vv2_a, dim3(nb),
-                           dim3(bs), 0, ctx->stream, A);
+        if (w3) {
+            FinArgs F{};
+            int fuse = 0;
+            if (fin && !batch && !sync) {
+                F = *fin;
+                fuse = ctx->precision == MDG_MIXED ? 1 : 2;
+            }
+            hipLaunchKernelGGL(batch ? k_w3_vv2_a<true> : k_w3_vv2_a<false>, dim3(nb), dim3(bs), 0,
+                               ctx->stream, A, F, fuse);
+        } else {
+            hipLaunchKernelGGL(k_vv2_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
+        }
         LAUNCHCHK();
         hipLaunchKernelGGL(batch ? k_w3_vv2_b<true> : w3 ? k_w3_vv2_b<false> : k_vv2_b, dim3(nb),
                            dim3(bs), 0, ctx->stream, A);

@@ -498,7 +498,7 @@ int mdg_spme_pot(mdg_ctx *ctx, double *pot);   // FP64 dE_rec/dq per atom
 int mdg_adj_pair(mdg_ctx *ctx);      // real-space dE/d(q, eps, sig) into fdx/fdy/fdz slots
 
 // device-resident step pieces (mdg_run.hip)
-int mdg_device_energy(mdg_ctx *ctx, bool rebuild);
+int mdg_device_energy(mdg_ctx *ctx, bool rebuild, bool fin = true);   // fin: run k_finalize
 int mdg_device_step(mdg_ctx *ctx, bool rebuild);
 int mdg_reset_step_state(mdg_ctx *ctx);
 int mdg_check_flags(mdg_ctx *ctx);
@@ -508,8 +508,11 @@ void mdg_invalidate_graphs(mdg_ctx *ctx);
 int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst);
 int mdg_vv_blocks(const mdg_ctx *ctx);
 int mdg_vir_rows(const mdg_ctx *ctx);   // rows of grpVir the record sums
+struct FinArgs;
+// fin (stage 2 of three-site molecules): the first stage-2 kernel also
+// assembles the forces that k_finalize would (mdg_side.h)
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
-                 double *virshake_out, bool sync);
+                 double *virshake_out, bool sync, const FinArgs *fin = nullptr);
 
 // replica batch pieces
 int mdg_b_build_list(mdg_ctx *ctx);                  // mdg_list.hip

@@ -12,23 +12,6 @@
 
 #define TWOPI_R 6.283185307179586
 
-struct FinArgs {
-    MdgParams P;
-    MdgBox B;
-    int n, hasFrozen, nbGather;
-    const int *atomSlot;
-    const long long *fix, *fiy, *fiz;
-    const double *fdx, *fdy, *fdz;
-    const float *frx, *fry, *frz;
-    const double *frx64, *fry64, *frz64;
-    const double *gpart;                 // per-block sums of the reciprocal forces
-    const int *exOff, *exLst;            // symmetric exclusion table
-    const double *x, *y, *z, *q;
-    const int *frozen;
-    double *fx, *fy, *fz, *vx, *vy, *vz;
-    double *part;                        // per block: eCorr, virCorr
-    unsigned long long *mask;            // both SHAKE masks, reset here
-};
 
 // Per atom: F = slot pair force + (reciprocal force - mean reciprocal
 // force) + excluded-pair corrections.  The mean is over the reciprocal
@@ -42,63 +25,15 @@ template <bool MIXED>
 __global__ void __launch_bounds__(256) k_finalize(FinArgs A)
 {
     __shared__ double mean[3], red[4][3];
-    {
-        // every block sums the gather's block partials (thread-strided with
-        // batched loads, then across the block: a fixed order)
-        double s[3] = {0., 0., 0.};
-        rows_add<3>(A.gpart, A.nbGather, s);
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        for (int c = 0; c < 3; c++) {
-            const double v = mdg_wave_sum(s[c]);
-            if (lane == 0) red[wid][c] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x < 3)
-            mean[threadIdx.x] = (((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
-                                 red[3][threadIdx.x]) / (double)A.n;
-    }
+    fin_mean(A, mean, red);
     // both SHAKE masks: stage 1 is done with its own, the next stage 2
     // has not started
     if (blockIdx.x == 0 && threadIdx.x < 2 * SHK_MASKW) A.mask[threadIdx.x] = ~0ull;
-    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[2] = {0, 0};
     if (i < A.n) {
-        const int s = A.atomSlot[i];
         double gx, gy, gz;
-        if (MIXED) {
-            gx = (double)A.fix[s] * (1.0 / MDG_FSCALE) + ((double)A.frx[i] - mean[0]);
-            gy = (double)A.fiy[s] * (1.0 / MDG_FSCALE) + ((double)A.fry[i] - mean[1]);
-            gz = (double)A.fiz[s] * (1.0 / MDG_FSCALE) + ((double)A.frz[i] - mean[2]);
-        } else {
-            gx = (A.frx64[i] - mean[0]) + A.fdx[s];
-            gy = (A.fry64[i] - mean[1]) + A.fdy[s];
-            gz = (A.frz64[i] - mean[2]) + A.fdz[s];
-        }
-        for (int k = A.exOff[i]; k < A.exOff[i + 1]; k++) {
-            const int j = A.exLst[k];
-            const int lo = min(i, j), hi = max(i, j);
-            double dx = A.x[hi] - A.x[lo], dy = A.y[hi] - A.y[lo], dz = A.z[hi] - A.z[lo];
-            const double r2 = mdg_dist(A.B, dx, dy, dz);
-            const double r = sqrt(r2), rt = 1. / r;
-            double dC;
-            const double e = ewald_corr64(A.P.alpha, A.P.chargeConst * A.q[lo] * A.q[hi], r, rt, dC);
-            const double sgn = (i == lo) ? 1.0 : -1.0;
-            gx += sgn * (dC * dx * rt);
-            gy += sgn * (dC * dy * rt);
-            gz += sgn * (dC * dz * rt);
-            if (i == lo) {
-                acc[0] += e;
-                acc[1] += dC * r;
-            }
-        }
-        if (A.hasFrozen && A.frozen[i]) {
-            gx = gy = gz = 0.;
-            A.vx[i] = A.vy[i] = A.vz[i] = 0.;
-        }
-        A.fx[i] = gx;
-        A.fy[i] = gy;
-        A.fz[i] = gz;
+        (void)fin_atom<MIXED>(A, i, mean, acc, gx, gy, gz);
     }
     mdg_block_partials<2>(acc, A.part);
 }
@@ -184,7 +119,7 @@ static int ensure_records(mdg_ctx *ctx, int n)
 // energy.c:103-205), optionally rebuilding the cluster list first.
 static int b_energy(mdg_ctx *ctx, bool rebuild);
 
-int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
+int mdg_device_energy(mdg_ctx *ctx, bool rebuild, bool fin)
 {
     if (ctx->prunePending) {
         // the previous step's list prune reads the list and the slot positions
@@ -210,7 +145,20 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
     RETCHK(mdg_pair_cluster(ctx));
     if (!spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_spme_device));
     if (ctx->spmeForked) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
+    if (!fin) return MDG_OK;
     const bool mixed = ctx->precision == MDG_MIXED;
+    const FinArgs F = mdg_fin_args(ctx);
+    if (mixed)
+        hipLaunchKernelGGL(k_finalize<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
+    else
+        hipLaunchKernelGGL(k_finalize<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+FinArgs mdg_fin_args(mdg_ctx *ctx)
+{
+    const int n = ctx->n;
     FinArgs F;
     F.P = ctx->P;
     F.B = ctx->B;
@@ -246,15 +194,17 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild)
     F.vz = ctx->vz;
     F.part = ctx->epart + PART_EXCL;
     F.mask = ctx->shkMask;
-    if (mixed)
-        hipLaunchKernelGGL(k_finalize<true>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
-    else
-        hipLaunchKernelGGL(k_finalize<false>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, F);
-    LAUNCHCHK();
-    return MDG_OK;
+    return F;
 }
 
-RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
+bool mdg_fin_fused(const mdg_ctx *ctx) { return !ctx->bt && ctx->shakeW3; }
+
+int mdg_excl_rows(const mdg_ctx *ctx)
+{
+    return mdg_fin_fused(ctx) ? mdg_vv_blocks(ctx) : nblk(ctx->n, 256);
+}
+
+RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir, int nbExcl)
 {
     RecArgs R;
     R.fact = (TWOPI_R / ctx->B.vol) * ctx->P.chargeConst;
@@ -262,7 +212,7 @@ RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir)
     R.eNet = ctx->eNet;
     R.recCap = ctx->recCap;
     R.nbPair = mdg_pair_blocks(ctx);
-    R.nbExcl = nblk(ctx->n, 256);
+    R.nbExcl = nbExcl;
     R.nbInfl = mdg_infl_blocks((long)ctx->m1 * ctx->m2 * ctx->mh);
     R.nbKin = nbKin;
     R.nGrpVir = nGrpVir;
@@ -282,9 +232,15 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[8], ctx->stream));
     RETCHK(mdg_vv_stage(ctx, 1, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[9], ctx->stream));
-    RETCHK(mdg_device_energy(ctx, rebuild));
+    const bool fused = mdg_fin_fused(ctx);
+    RETCHK(mdg_device_energy(ctx, rebuild, !fused));
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[10], ctx->stream));
-    RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
+    if (fused) {
+        const FinArgs F = mdg_fin_args(ctx);
+        RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false, &F));
+    } else {
+        RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
+    }
     // In a step block of one box with three-site SHAKE, the record and the
     // list prune go with the next step's first SHAKE kernel (its spare
     // blocks); the block's last step runs them after the loop.
@@ -308,7 +264,7 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
         st = b_record(ctx, rebuild);
     else
         hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
-                           mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)),
+                           mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx), mdg_excl_rows(ctx)),
                            ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     if (fork && st == MDG_OK) {
         HIPCHK(hipEventRecord(ctx->evRecDone, ctx->stream2));
@@ -449,7 +405,7 @@ extern "C" int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec)
                        ctx->mass, ctx->vx, ctx->vy, ctx->vz, ctx->epart + PART_KIN);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
-                       mdg_rec_args(ctx, nblk(ctx->n, 256), 0), ctx->ctrl, ctx->recDev, 1);
+                       mdg_rec_args(ctx, nblk(ctx->n, 256), 0, nblk(ctx->n, 256)), ctx->ctrl, ctx->recDev, 1);
     LAUNCHCHK();
     RETCHK(mdg_check_flags(ctx));
     if (rec) HIPCHK(hipMemcpy(rec, ctx->recDev, sizeof(MDG_STEP), hipMemcpyDeviceToHost));
@@ -488,7 +444,7 @@ static int capture_block(mdg_ctx *ctx)
     if (st == MDG_OK && ctx->sidePending) {
         // the last step's record and prune
         hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
-                           mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx)), ctx->ctrl,
+                           mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx), mdg_excl_rows(ctx)), ctx->ctrl,
                            ctx->recDev, ctx->sideRebuilt ? 1 : 0);
         if (hipGetLastError() != hipSuccess) st = MDG_EHIP;
         if (st == MDG_OK) st = mdg_prune_list(ctx);

@@ -159,5 +159,5 @@ __device__ inline void shake_image(const MdgBox &B, double &xs, double &ys, doub
 
 template <bool BATCH> __global__ void k_w3_vv1_a(ShakeArgs A, SideWork S);
 template <bool BATCH> __global__ void k_w3_vv1_b(ShakeArgs A);
-template <bool BATCH> __global__ void k_w3_vv2_a(ShakeArgs A);
+template <bool BATCH> __global__ void k_w3_vv2_a(ShakeArgs A, FinArgs F, int fuse);
 template <bool BATCH> __global__ void k_w3_vv2_b(ShakeArgs A);

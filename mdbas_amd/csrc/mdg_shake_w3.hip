@@ -459,6 +459,8 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
 template <bool BATCH>
 __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
 {
+    // the stage-2 mask (beside this stage's) for this step's stage 2
+    if (!BATCH && blockIdx.x == 0 && threadIdx.x < SHK_MASKW) A.mask[SHK_MASKW + threadIdx.x] = ~0ull;
     const int g = shake_w3_group(A);
     const int k = threadIdx.x & 3;
     double vblk = 0.;   // this lane's share of the block's SHAKE virial
@@ -502,9 +504,21 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
 
 // Stage 2, part A: constraint vectors, half kick, SHAKE-V sweeps with the
 // same bookkeeping as stage 1.
+//
+// With fuse set (1 mixed, 2 FP64; one box), the kernel first assembles each
+// atom's force as k_finalize does (fin_atom, mdg_side.h: the same
+// arithmetic, so k_finalize is not launched), keeps the correction
+// partials per block, and resets the stage-1 SHAKE mask for the next step
+// (k_w3_vv1_b resets the stage-2 one).
 template <bool BATCH>
-__global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
+__global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fuse)
 {
+    __shared__ double mean[3], red[4][3];
+    double facc[2] = {0., 0.};
+    if (!BATCH && fuse) {
+        fin_mean(F, mean, red);
+        if (blockIdx.x == 0 && threadIdx.x < SHK_MASKW) F.mask[threadIdx.x] = ~0ull;
+    }
     const int g = shake_w3_group(A);
     const int k = threadIdx.x & 3;
     QBits bits;
@@ -515,6 +529,12 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
         const MdgBox &B = q_box<BATCH>(A, g);
         Q3 Q;
         q_load(A, g, k, Q);
+        if (!BATCH && fuse && k < 3) {
+            double gx, gy, gz;
+            const bool fr = fuse == 1 ? fin_atom<true>(F, Q.at, mean, facc, gx, gy, gz)
+                                      : fin_atom<false>(F, Q.at, mean, facc, gx, gy, gz);
+            if (fr) Q.vx = Q.vy = Q.vz = 0.;
+        }
         q_dd(A, B, Q);
         q_setup_v(A, Q);
         q_kick(A, Q);
@@ -553,6 +573,7 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A)
         kend = it;
     }
     q_publish<BATCH>(A, g, bits.w, kend, settled);
+    if (!BATCH && fuse) mdg_block_partials<2>(facc, F.part);
 }
 
 // Stage 2, part B: replay to the global sweep C, kinetic energy and stress
@@ -608,7 +629,7 @@ template __global__ void k_w3_vv1_a<false>(ShakeArgs, SideWork);
 template __global__ void k_w3_vv1_a<true>(ShakeArgs, SideWork);
 template __global__ void k_w3_vv1_b<false>(ShakeArgs);
 template __global__ void k_w3_vv1_b<true>(ShakeArgs);
-template __global__ void k_w3_vv2_a<false>(ShakeArgs);
-template __global__ void k_w3_vv2_a<true>(ShakeArgs);
+template __global__ void k_w3_vv2_a<false>(ShakeArgs, FinArgs, int);
+template __global__ void k_w3_vv2_a<true>(ShakeArgs, FinArgs, int);
 template __global__ void k_w3_vv2_b<false>(ShakeArgs);
 template __global__ void k_w3_vv2_b<true>(ShakeArgs);

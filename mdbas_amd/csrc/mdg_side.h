@@ -42,6 +42,95 @@ __device__ inline void rows_add(const double *__restrict__ p, int nrow, double *
         for (int c = 0; c < NC; c++) acc[c] += p[(long)r * NC + c];
 }
 
+// ---------------------------------------------------------------- forces
+
+struct FinArgs {
+    MdgParams P;
+    MdgBox B;
+    int n, hasFrozen, nbGather;
+    const int *atomSlot;
+    const long long *fix, *fiy, *fiz;
+    const double *fdx, *fdy, *fdz;
+    const float *frx, *fry, *frz;
+    const double *frx64, *fry64, *frz64;
+    const double *gpart;                 // per-block sums of the reciprocal forces
+    const int *exOff, *exLst;            // symmetric exclusion table
+    const double *x, *y, *z, *q;
+    const int *frozen;
+    double *fx, *fy, *fz, *vx, *vy, *vz;
+    double *part;                        // per block: eCorr, virCorr
+    unsigned long long *mask;            // the SHAKE masks (stage 1, then stage 2)
+};
+
+// Mean reciprocal force over the atoms, from the gather's block partials
+// (thread-strided with batched loads, then across the block: a fixed
+// order), into mean[3]; every thread of the block takes part.
+__device__ inline void fin_mean(const FinArgs &A, double *mean, double (*red)[3])
+{
+    double s[3] = {0., 0., 0.};
+    rows_add<3>(A.gpart, A.nbGather, s);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = 0; c < 3; c++) {
+        const double v = mdg_wave_sum(s[c]);
+        if (lane == 0) red[wid][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        mean[threadIdx.x] = (((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+                             red[3][threadIdx.x]) / (double)A.n;
+    __syncthreads();
+}
+
+// Atom i's force: F = slot pair force + (reciprocal force - mean
+// reciprocal force) + excluded-pair corrections.  The mean is over the
+// reciprocal part only (spme.c:648-670 runs on arrays energy() has just
+// zeroed).  The correction of each excluded pair (energy.c:567-621) is
+// evaluated from both of its atoms with the reference's orientation
+// (delta = x_hi - x_lo), so every atom sums its own terms without atomics;
+// energy and virial (acc) count each pair once.  Frozen atoms get zero
+// force and velocity (utils.c:131-153); returns whether atom i is frozen.
+template <bool MIXED>
+__device__ inline bool fin_atom(const FinArgs &A, int i, const double *mean, double *acc,
+                                double &gx, double &gy, double &gz)
+{
+    const int s = A.atomSlot[i];
+    if (MIXED) {
+        gx = (double)A.fix[s] * (1.0 / MDG_FSCALE) + ((double)A.frx[i] - mean[0]);
+        gy = (double)A.fiy[s] * (1.0 / MDG_FSCALE) + ((double)A.fry[i] - mean[1]);
+        gz = (double)A.fiz[s] * (1.0 / MDG_FSCALE) + ((double)A.frz[i] - mean[2]);
+    } else {
+        gx = (A.frx64[i] - mean[0]) + A.fdx[s];
+        gy = (A.fry64[i] - mean[1]) + A.fdy[s];
+        gz = (A.frz64[i] - mean[2]) + A.fdz[s];
+    }
+    for (int k = A.exOff[i]; k < A.exOff[i + 1]; k++) {
+        const int j = A.exLst[k];
+        const int lo = min(i, j), hi = max(i, j);
+        double dx = A.x[hi] - A.x[lo], dy = A.y[hi] - A.y[lo], dz = A.z[hi] - A.z[lo];
+        const double r2 = mdg_dist(A.B, dx, dy, dz);
+        const double r = sqrt(r2), rt = 1. / r;
+        double dC;
+        const double e = ewald_corr64(A.P.alpha, A.P.chargeConst * A.q[lo] * A.q[hi], r, rt, dC);
+        const double sgn = (i == lo) ? 1.0 : -1.0;
+        gx += sgn * (dC * dx * rt);
+        gy += sgn * (dC * dy * rt);
+        gz += sgn * (dC * dz * rt);
+        if (i == lo) {
+            acc[0] += e;
+            acc[1] += dC * r;
+        }
+    }
+    const bool fr = A.hasFrozen && A.frozen[i];
+    if (fr) {
+        gx = gy = gz = 0.;
+        A.vx[i] = A.vy[i] = A.vz[i] = 0.;
+    }
+    A.fx[i] = gx;
+    A.fy[i] = gy;
+    A.fz[i] = gz;
+    return fr;
+}
+
 // ---------------------------------------------------------------- record
 
 struct RecArgs {
@@ -266,7 +355,13 @@ __device__ inline void side_work(const SideWork &S)
 }
 
 // Host side (mdg_run.hip, mdg_pair.hip).
-RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir);
+RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir, int nbExcl);
+FinArgs mdg_fin_args(mdg_ctx *ctx);
+// Device steps of one box of three-site molecules compute the forces in
+// the first stage-2 SHAKE kernel (no k_finalize): its blocks then hold the
+// correction partials.
+bool mdg_fin_fused(const mdg_ctx *ctx);
+int mdg_excl_rows(const mdg_ctx *ctx);
 PruneArgs mdg_prune_args(mdg_ctx *ctx);
 bool mdg_prune_on(const mdg_ctx *ctx);       // mixed, pruning on, list built
 int mdg_prune_blocks(const mdg_ctx *ctx);    // blocks of a prune kernel of its own
