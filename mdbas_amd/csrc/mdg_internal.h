@@ -145,7 +145,8 @@ struct MdgTune {
     int pairBlocks = 1024;       // cap on the cluster pair kernel's grid (4 blocks per CU leave
                                  // wave slots for the SPME kernels beside it)
     int spmeFirst = 0;           // steps without a rebuild enqueue the SPME branch before the
-                                 // pair branch (the first branch after a fork starts first)
+                                 // pair branch (the first branch after a fork starts first;
+                                 // rebuild steps always do)
     int spmePrio = 0;            // the binned SPME kernels' waves issue ahead of the pair
                                  // kernel's when both share a SIMD
 };

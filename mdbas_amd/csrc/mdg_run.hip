@@ -133,7 +133,10 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild, bool fin)
     // microseconds later: the pair branch unless the tuner found the SPME
     // branch the longer one on steps without a rebuild
     RETCHK(mdg_spme_fork(ctx));
-    const bool spmeFirst = ctx->tune.spmeFirst && !rebuild;
+    // (a rebuild step always starts the SPME branch first: the second
+    // branch starts only after the first one's nodes, and the rebuild has
+    // some thirty of them)
+    const bool spmeFirst = ctx->tune.spmeFirst || rebuild;
     if (spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_spme_device));
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
@@ -865,7 +868,10 @@ static int b_energy(mdg_ctx *ctx, bool rebuild)
     MdgBatch &T = *ctx->bt;
     const int n = ctx->n;
     RETCHK(mdg_spme_fork(ctx));
-    const bool spmeFirst = ctx->tune.spmeFirst && !rebuild;
+    // (a rebuild step always starts the SPME branch first: the second
+    // branch starts only after the first one's nodes, and the rebuild has
+    // some thirty of them)
+    const bool spmeFirst = ctx->tune.spmeFirst || rebuild;
     if (spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_b_spme));
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));

@@ -62,6 +62,28 @@ __global__ void k_fill(T *p, long n, T v)
     for (; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Exclusive scan of up to SCAN_BS * SCAN_ONE_K elements in one block:
+// thread t sums its run of k consecutive elements, the block scans the
+// run sums, and each thread writes its run's exclusive prefixes; the
+// total goes to out[n].  One launch instead of three to five.
+#define SCAN_ONE_K 64
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(SCAN_BS) k_scan_one(const Tin *__restrict__ in,
+                                                      Tout *__restrict__ out, int n, int k)
+{
+    __shared__ Tout tot;
+    const int b = threadIdx.x * k, e = min(b + k, n);
+    Tout s = Tout(0);
+    for (int i = b; i < e; i++) s += (Tout)in[i];
+    Tout run = block_exclusive_scan<Tout>(s, &tot);
+    for (int i = b; i < e; i++) {
+        const Tout v = (Tout)in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) out[n] = tot;
+}
+
 // Exclusive scan with the total stored at out[n].  Scratch comes from a
 // per-context pool grown on demand (never while a graph is being captured:
 // the pool is sized by the first, uncaptured, call of each size).
@@ -69,6 +91,12 @@ __global__ void k_fill(T *p, long n, T v)
 template <typename Tin, typename Tout>
 static int scan_impl(mdg_ctx *ctx, const Tin *in, Tout *out, int n, int level)
 {
+    if (level == 0 && n <= SCAN_BS * SCAN_ONE_K) {
+        hipLaunchKernelGGL((k_scan_one<Tin, Tout>), dim3(1), dim3(SCAN_BS), 0, ctx->stream, in,
+                           out, n, std::max(1, (n + SCAN_BS - 1) / SCAN_BS));
+        LAUNCHCHK();
+        return MDG_OK;
+    }
     int nb = nblk(n, SCAN_BS);
     size_t need = (size_t)(2 * nb + 2) * sizeof(Tout);
     if (ctx->scanSize[level] < need) {
