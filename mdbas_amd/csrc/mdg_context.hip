@@ -464,8 +464,7 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->cl.atomSlot, ctx->cl.bb, ctx->cl.bbh, ctx->cl.cpOff, ctx->cl.cpJ,
                     ctx->cl.cpMask, ctx->cl.cpI, ctx->cl.tmpCnt, ctx->cl.tmpJ, ctx->cl.tmpMask,
                     ctx->cl.colCnt, ctx->cl.colStart, ctx->cl.colPad, ctx->cl.colSlot,
-                    ctx->cl.sortTmp, ctx->cl.sortKey, ctx->cl.cluCol, ctx->cl.px, ctx->cl.py, ctx->cl.pz,
-                    ctx->cl.pcell, ctx->sx, ctx->sy, ctx->sz, ctx->sp32,
+                    ctx->cl.sortTmp, ctx->cl.sortKey, ctx->cl.cluCol, ctx->cl.pk, ctx->sx, ctx->sy, ctx->sz, ctx->sp32,
                     ctx->sq, ctx->seps, ctx->ssig, ctx->fdx, ctx->fdy, ctx->fdz, ctx->scal,
                     ctx->fix, ctx->frx, ctx->fry, ctx->frz, ctx->frx64,
                     ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,

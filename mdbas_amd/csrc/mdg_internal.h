@@ -85,9 +85,8 @@ struct ClusterList {
     int *colCnt = nullptr, *colStart = nullptr, *colPad = nullptr, *colSlot = nullptr;
     int *cluCol = nullptr;       // column of each cluster (-1 unused)
     // per slot, for the list predicate: the atom's predicate coordinates
-    // (scaled for link cells, else positions) and link cell
-    double *px = nullptr, *py = nullptr, *pz = nullptr;
-    int *pcell = nullptr;
+    // (scaled for link cells, else positions), link cell and atom
+    double4 *pk = nullptr;
     int *sortTmp = nullptr;
     float *sortKey = nullptr;
     // pruned copy of the list (mixed pair kernel): per chunk of the full

@@ -602,30 +602,30 @@ __device__ inline bool ref_pair_pred_slot(const MdgParams &P, const MdgBox &B, i
     return !(hi - lo <= exSpan && mdg_excluded(exOff, exLst, lo, hi));
 }
 
-// Per slot: the predicate inputs of its atom, the link cell as packed
-// coordinates (padding slots are never tested).
+// Per slot, in one 32-byte record: the predicate inputs of its atom (x, y,
+// z) and, in the fourth double's halves, the link cell as packed
+// coordinates (low) and the atom (high; -1 for padding, never tested).
 __global__ void k_pred_slots(int nslot, int keyLink, int nx, int ny, const int *__restrict__ slotAtom,
                              const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, const double *__restrict__ xu,
                              const double *__restrict__ yu, const double *__restrict__ zu,
-                             const int *__restrict__ cellOf, double *__restrict__ px,
-                             double *__restrict__ py, double *__restrict__ pz,
-                             int *__restrict__ pcell)
+                             const int *__restrict__ cellOf, double4 *__restrict__ pk)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot) return;
     const int a = slotAtom[s];
-    if (a < 0) return;
-    px[s] = keyLink ? xu[a] : x[a];
-    py[s] = keyLink ? yu[a] : y[a];
-    pz[s] = keyLink ? zu[a] : z[a];
+    if (a < 0) {
+        pk[s] = make_double4(0., 0., 0., __hiloint2double(-1, 0));
+        return;
+    }
     int cp = 0;
     if (keyLink) {
         // cell coordinates, packed (cells per axis stay below 1024)
         const int c = cellOf[a];
         cp = (c % nx) | (((c / nx) % ny) << 10) | ((c / (nx * ny)) << 20);
     }
-    pcell[s] = cp;
+    pk[s] = make_double4(keyLink ? xu[a] : x[a], keyLink ? yu[a] : y[a], keyLink ? zu[a] : z[a],
+                         __hiloint2double(a, cp));
 }
 
 // One wave per I cluster: bbox test against every J >= I, then the 64
@@ -636,6 +636,8 @@ __global__ void k_pred_slots(int nslot, int keyLink, int nx, int ny, const int *
 // + 1 columns apart (w the column width) cannot hold a pair within rlist.
 // The columns of one row that are in reach form at most two contiguous
 // cluster ranges (two when they wrap around the box).
+#define CAND_BATCH 4             // candidate j clusters whose slot records load together
+
 struct ColGrid {
     const int *colSlot;          // first slot of each column (ncol + 1)
     const int *cluCol;           // column of each cluster (-1 unused)
@@ -645,9 +647,8 @@ struct ColGrid {
 __global__ void __launch_bounds__(256)
 k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int nclu,
                 const int *__restrict__ slotAtom, const float4 *__restrict__ bb,
-                const float4 *__restrict__ bbh, const double *__restrict__ px,
-                const double *__restrict__ py, const double *__restrict__ pz,
-                const int *__restrict__ pcell, bool hasFrozen, const int *__restrict__ frozen,
+                const float4 *__restrict__ bbh, const double4 *__restrict__ pk,
+                bool hasFrozen, const int *__restrict__ frozen,
                 const int *__restrict__ exOff, const int *__restrict__ exLst, int exSpan, int kmax,
                 int *__restrict__ tmpCnt, int *__restrict__ tmpJ,
                 unsigned long long *__restrict__ tmpMask, int *__restrict__ ctrl, ColGrid CG)
@@ -665,15 +666,9 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
     const float rl = (float)(P.cutOff + P.delr) + 0.01f;
     const int a = lane & 7, b = lane >> 3;   // i, j atom: mask bit = lane = 8 j + i
     const int si = I * MDG_CL + a;
-    const int iat = slotAtom[si];
-    double xi = 0., yi = 0., zi = 0.;
-    int cli = 0;
-    if (iat >= 0) {
-        xi = px[si];
-        yi = py[si];
-        zi = pz[si];
-        cli = pcell[si];
-    }
+    const double4 pi = pk[si];
+    const int iat = __double2hiint(pi.w), cli = __double2loint(pi.w);
+    const double xi = pi.x, yi = pi.y, zi = pi.z;
     const int cx = colI % CG.ncx, cy = colI / CG.ncx;
     // rows: all of them once when the reach covers the box, else cy +- ry
     const bool allRows = 2 * CG.ry + 1 >= CG.ncy, allCols = 2 * CG.rx + 1 >= CG.ncx;
@@ -716,50 +711,42 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
                     cand = B.type == 3 || (dx * dx + dy * dy + dz * dz <= rl * rl);
                 }
                 unsigned long long cm = __ballot(cand);
-                if (!cm) continue;
-                // candidates in order; the next one's slot data is requested
-                // before the current one is tested
-                int src = __ffsll((long long)cm) - 1;
-                cm &= cm - 1;
-                int sj = (J0 + src) * MDG_CL + b;
-                int jat = slotAtom[sj];
-                double xj = px[sj], yj = py[sj], zj = pz[sj];
-                int clj = pcell[sj];
-                for (;;) {
-                    const int Jc = J0 + src;
-                    const bool more = cm != 0;
-                    int nsrc = 0, njat = -1, nclj = 0;
-                    double nxj = 0., nyj = 0., nzj = 0.;
-                    if (more) {
-                        nsrc = __ffsll((long long)cm) - 1;
-                        cm &= cm - 1;
-                        const int ns = (J0 + nsrc) * MDG_CL + b;
-                        njat = slotAtom[ns];
-                        nxj = px[ns];
-                        nyj = py[ns];
-                        nzj = pz[ns];
-                        nclj = pcell[ns];
-                    }
-                    bool pr = false;
-                    if (iat >= 0 && jat >= 0 && (Jc != I || a < b))
-                        pr = ref_pair_pred_slot(P, B, keyLink, nx, ny, nz, iat, jat, xi, yi, zi,
-                                                cli, xj, yj, zj, clj, hasFrozen, frozen, exOff,
-                                                exLst, exSpan);
-                    const unsigned long long m = __ballot(pr);
-                    if (m) {
-                        if (found < kmax && lane == 0) {
-                            tmpJ[(long)I * kmax + found] = Jc;
-                            tmpMask[(long)I * kmax + found] = m;
+                // candidates in order, CAND_BATCH at a time: their slot
+                // records are all requested before the first is tested
+                while (cm) {
+                    int src[CAND_BATCH];
+                    int cnt = 0;
+#pragma unroll
+                    for (int t = 0; t < CAND_BATCH; t++) {
+                        src[t] = cm ? __ffsll((long long)cm) - 1 : 0;
+                        if (cm) {
+                            cm &= cm - 1;
+                            cnt++;
                         }
-                        found++;
                     }
-                    if (!more) break;
-                    src = nsrc;
-                    jat = njat;
-                    xj = nxj;
-                    yj = nyj;
-                    zj = nzj;
-                    clj = nclj;
+                    double4 pj[CAND_BATCH];
+#pragma unroll
+                    for (int t = 0; t < CAND_BATCH; t++)
+                        pj[t] = pk[(J0 + (t < cnt ? src[t] : src[0])) * MDG_CL + b];
+#pragma unroll
+                    for (int t = 0; t < CAND_BATCH; t++) {
+                        if (t >= cnt) break;
+                        const int Jc = J0 + src[t];
+                        const int jat = __double2hiint(pj[t].w), clj = __double2loint(pj[t].w);
+                        bool pr = false;
+                        if (iat >= 0 && jat >= 0 && (Jc != I || a < b))
+                            pr = ref_pair_pred_slot(P, B, keyLink, nx, ny, nz, iat, jat, xi, yi,
+                                                    zi, cli, pj[t].x, pj[t].y, pj[t].z, clj,
+                                                    hasFrozen, frozen, exOff, exLst, exSpan);
+                        const unsigned long long m = __ballot(pr);
+                        if (m) {
+                            if (found < kmax && lane == 0) {
+                                tmpJ[(long)I * kmax + found] = Jc;
+                                tmpMask[(long)I * kmax + found] = m;
+                            }
+                            found++;
+                        }
+                    }
                 }
             }
         }
@@ -907,10 +894,7 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&C.tmpCnt, maxclu * sizeof(int)));
         HIPCHK(hipMalloc(&C.cluCol, maxclu * sizeof(int)));
         const size_t ns = (size_t)maxclu * MDG_CL + 64;
-        HIPCHK(hipMalloc(&C.px, ns * sizeof(double)));
-        HIPCHK(hipMalloc(&C.py, ns * sizeof(double)));
-        HIPCHK(hipMalloc(&C.pz, ns * sizeof(double)));
-        HIPCHK(hipMalloc(&C.pcell, ns * sizeof(int)));
+        HIPCHK(hipMalloc(&C.pk, ns * sizeof(double4)));
         // neighbouring clusters within rc+skin plus two cluster diameters
         double rl = ctx->P.cutOff + ctx->P.delr + 9.0;
         double dens = (double)n / ctx->B.vol / MDG_CL;
@@ -951,7 +935,7 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
     hipLaunchKernelGGL(k_pred_slots, dim3(nblk((long)nclu * MDG_CL, 256)), dim3(256), 0,
                        ctx->stream, nclu * MDG_CL, ctx->keyLink, ctx->nlc[0], ctx->nlc[1],
                        C.slotAtom, ctx->x, ctx->y, ctx->z,
-                       ctx->xu, ctx->yu, ctx->zu, ctx->cellOf, C.px, C.py, C.pz, C.pcell);
+                       ctx->xu, ctx->yu, ctx->zu, ctx->cellOf, C.pk);
     LAUNCHCHK();
     ColGrid CG;
     CG.colSlot = C.colSlot;
@@ -965,7 +949,7 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
     }
     hipLaunchKernelGGL(k_cluster_pairs, dim3(nblk(nclu, 4)), dim3(256), 0, ctx->stream, ctx->P,
                        ctx->B, ctx->keyLink, ctx->nlc[0], ctx->nlc[1], ctx->nlc[2], nclu,
-                       C.slotAtom, C.bb, C.bbh, C.px, C.py, C.pz, C.pcell, ctx->hasFrozen,
+                       C.slotAtom, C.bb, C.bbh, C.pk, ctx->hasFrozen,
                        ctx->frozen, ctx->exOff, ctx->exLst, ctx->exSpan, C.kmax, C.tmpCnt, C.tmpJ,
                        C.tmpMask, ctx->ctrl, CG);
     LAUNCHCHK();
