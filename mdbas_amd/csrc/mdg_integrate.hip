@@ -641,8 +641,29 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.grep = nullptr;
     A.grpKin = nullptr;
     A.spread = ctx->tune.shkSpread;
+    A.slotWrite = 0;
+    A.track = 0;
+    A.atomSlot = nullptr;
+    A.sx = A.sy = A.sz = nullptr;
+    A.fix = nullptr;
+    A.fstr = 0;
+    A.dispPart = nullptr;
     const bool w3 = ctx->shakeW3;
     const bool batch = ctx->bt != nullptr;
+    if (stage == 1 && ctx->slotsInStage1 && w3 && !batch && !sync) {
+        // a device step without a rebuild: the replay kernel writes the slots
+        A.slotWrite = 1;
+        A.track = ctx->pruneSkin > 0. && !ctx->pruneFresh ? 1 : 0;
+        ctx->pruneFresh = false;
+        A.atomSlot = ctx->cl.atomSlot;
+        A.sx = ctx->sx;
+        A.sy = ctx->sy;
+        A.sz = ctx->sz;
+        A.fix = ctx->fix;
+        A.fstr = ctx->fiy - ctx->fix;
+        A.dispPart = ctx->dispPart;
+        ctx->slotsDone = true;
+    }
     if (batch) {
         if (!w3 || sync) {
             ctx->err = "replica batches integrate rigid three-site molecules on the device";

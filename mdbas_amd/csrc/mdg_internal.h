@@ -228,6 +228,10 @@ struct mdg_ctx {
     int nDispPart = 0;
     double pruneSkin = 0.;       // > 0: the mixed pair kernel may use the pruned list
     bool pruneFresh = true;      // the list was rebuilt since the last slot update
+    // device steps without a rebuild (one box of three-site molecules,
+    // mixed): stage 1's replay kernel writes the slot positions
+    bool slotsInStage1 = false;  // set by the step for its stage 1
+    bool slotsDone = false;      // stage 1 wrote them: the pair step skips k_slot_pos
     float4 *sp32 = nullptr;                                     // slot q, eps, sig
     double *sq = nullptr, *seps = nullptr, *ssig = nullptr;
     double *fdx = nullptr, *fdy = nullptr, *fdz = nullptr;     // FP64 slot forces

@@ -816,8 +816,14 @@ static int slot_pos(mdg_ctx *ctx, bool mixed)
 int mdg_pair_cluster(mdg_ctx *ctx)
 {
     const bool mixed = ctx->precision == MDG_MIXED;
-    RETCHK(slot_pos(ctx, mixed));
+    int nDisp = ctx->nDispPart;
+    if (mixed && ctx->slotsDone)   // this step's stage 1 wrote the slots
+        nDisp = mdg_vv_blocks(ctx);
+    else
+        RETCHK(slot_pos(ctx, mixed));
+    ctx->slotsDone = false;
     ClusterArgs A = cluster_args(ctx);
+    if (A.nDisp) A.nDisp = nDisp;
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     if (mixed)

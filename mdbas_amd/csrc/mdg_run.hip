@@ -76,7 +76,10 @@ static int ensure_slots(mdg_ctx *ctx)
     ctx->fiz = ctx->fix + 2 * (size_t)ns;
     if (ctx->precision == MDG_MIXED) {
         ctx->nDispPart = nblk(ctx->cl.nclu * MDG_CL, 256);
-        HIPCHK(hipMalloc(&ctx->dispPart, ctx->nDispPart * sizeof(double)));
+        // room for one partial per stage-1 SHAKE block as well (at most one
+        // molecule per wavefront, four wavefronts per block)
+        const int cap = std::max(ctx->nDispPart, nblk(std::max(ctx->nGrp, 1), 4) + 1);
+        HIPCHK(hipMalloc(&ctx->dispPart, cap * sizeof(double)));
     }
     return MDG_OK;
 }
@@ -233,7 +236,13 @@ static int b_record(mdg_ctx *ctx, bool rebuild);
 int mdg_device_step(mdg_ctx *ctx, bool rebuild)
 {
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[8], ctx->stream));
-    RETCHK(mdg_vv_stage(ctx, 1, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
+    // without a rebuild the slots keep their atoms: stage 1 writes them
+    ctx->slotsInStage1 = !rebuild && ctx->precision == MDG_MIXED && !ctx->bt && ctx->shakeW3 &&
+                         ctx->sx && ctx->dispPart && ctx->cl.atomSlot;
+    ctx->slotsDone = false;
+    const int st1 = mdg_vv_stage(ctx, 1, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false);
+    ctx->slotsInStage1 = false;
+    RETCHK(st1);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[9], ctx->stream));
     const bool fused = mdg_fin_fused(ctx);
     RETCHK(mdg_device_energy(ctx, rebuild, !fused));

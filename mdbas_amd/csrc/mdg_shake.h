@@ -39,6 +39,16 @@ struct ShakeArgs {
     // three-site kernels: one molecule per quad of lanes, on every
     // spread-th quad of a wavefront (16 / spread molecules per wavefront)
     int spread;
+    // stage 1 of a device step without a list rebuild (one box, mixed):
+    // k_w3_vv1_b also writes the slot positions and clears the slot force
+    // accumulators, as k_slot_pos<true> would, with the per-block largest
+    // squared move into dispPart (see k_slot_pos)
+    int slotWrite, track;
+    const int *atomSlot;
+    double *sx, *sy, *sz;
+    long long *fix;
+    long fstr;
+    double *dispPart;
 };
 
 // Molecule of this lane's quad, or nGrp for a quad left idle.

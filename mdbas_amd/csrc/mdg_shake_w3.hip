@@ -464,7 +464,9 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
     const int g = shake_w3_group(A);
     const int k = threadIdx.x & 3;
     double vblk = 0.;   // this lane's share of the block's SHAKE virial
+    int myAtom = -1;    // this lane's atom (lanes 0..2 of a live molecule)
     if (g < A.nGrp) {
+        if (k < 3) myAtom = A.atoms[A.atomOff[g] + k];
         const int C = shake_global_stop(A, q_mask<BATCH>(A, g));
         if (C < 0) {
             if (k == 0) atomicOr(&A.ctrl[CTRL_SHAKE_FAIL], 1);
@@ -495,10 +497,42 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
             q_store(A, Q, true);
         }
     }
+    // slot positions (the atom's own store above is visible to this lane)
+    double d2 = 0.;
+    const bool track = A.track && A.ctrl[CTRL_PRUNE] == 1;
+    if (!BATCH && A.slotWrite && myAtom >= 0) {
+        const int s = A.atomSlot[myAtom];
+        const double nx = A.x[myAtom], ny = A.y[myAtom], nz = A.z[myAtom];
+        if (track) {
+            const double L0 = A.B.a1, L1 = A.B.type == 1 ? A.B.a1 : A.B.b2,
+                         L2 = A.B.type == 1 ? A.B.a1 : A.B.c3;
+            double dx = nx - A.sx[s], dy = ny - A.sy[s], dz = nz - A.sz[s];
+            dx -= L0 * rint(dx / L0);
+            dy -= L1 * rint(dy / L1);
+            dz -= L2 * rint(dz / L2);
+            d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 <= 1e300)) d2 = INFINITY;   // NaN positions: no pruning
+        }
+        A.sx[s] = nx;
+        A.sy[s] = ny;
+        A.sz[s] = nz;
+        A.fix[s] = 0;
+        A.fix[s + A.fstr] = 0;
+        A.fix[s + 2 * A.fstr] = 0;
+    }
     if (!BATCH) {
         // one partial per block, summed in a fixed order by the record
         double acc[1] = {vblk};
         mdg_block_partials<1>(acc, A.vir);
+    }
+    if (!BATCH && A.slotWrite) {
+        for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+        __shared__ double wmax[4];
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d2;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            A.dispPart[blockIdx.x] =
+                track ? fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3])) : INFINITY;
     }
 }
 
