@@ -688,12 +688,17 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
             // spare blocks: the previous step's record and list prune
             SideWork S{};
             S.nbMain = nb;
-            if (ctx->sidePending && !batch) {
-                S.rec = 1;
-                S.R = mdg_rec_args(ctx, nb, mdg_vir_rows(ctx), mdg_excl_rows(ctx));
-                S.ctrl = ctx->ctrl;
-                S.out = ctx->recDev;
+            if (ctx->sidePending) {
                 S.rebuilt = ctx->sideRebuilt ? 1 : 0;
+                if (batch) {
+                    S.nbRec = ctx->bt->R;
+                    S.BR = mdg_b_rec_args(ctx);
+                } else {
+                    S.nbRec = 1;
+                    S.R = mdg_rec_args(ctx, nb, mdg_vir_rows(ctx), mdg_excl_rows(ctx));
+                    S.ctrl = ctx->ctrl;
+                    S.out = ctx->recDev;
+                }
                 if (mdg_prune_on(ctx)) {
                     S.nbPrune = std::min(mdg_prune_blocks(ctx), 2048);
                     S.Pr = mdg_prune_args(ctx);
@@ -701,7 +706,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
                 ctx->sidePending = false;
             }
             hipLaunchKernelGGL(batch ? k_w3_vv1_a<true> : k_w3_vv1_a<false>,
-                               dim3(nb + S.rec + S.nbPrune), dim3(bs), 0, ctx->stream, A, S);
+                               dim3(nb + S.nbRec + S.nbPrune), dim3(bs), 0, ctx->stream, A, S);
         } else {
             hipLaunchKernelGGL(k_vv1_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
         }

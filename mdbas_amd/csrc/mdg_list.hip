@@ -646,7 +646,7 @@ struct ColGrid {
 
 __global__ void __launch_bounds__(256)
 k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int nclu,
-                const int *__restrict__ slotAtom, const float4 *__restrict__ bb,
+                const float4 *__restrict__ bb,
                 const float4 *__restrict__ bbh, const double4 *__restrict__ pk,
                 bool hasFrozen, const int *__restrict__ frozen,
                 const int *__restrict__ exOff, const int *__restrict__ exLst, int exSpan, int kmax,
@@ -949,7 +949,7 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
     }
     hipLaunchKernelGGL(k_cluster_pairs, dim3(nblk(nclu, 4)), dim3(256), 0, ctx->stream, ctx->P,
                        ctx->B, ctx->keyLink, ctx->nlc[0], ctx->nlc[1], ctx->nlc[2], nclu,
-                       C.slotAtom, C.bb, C.bbh, C.pk, ctx->hasFrozen,
+                       C.bb, C.bbh, C.pk, ctx->hasFrozen,
                        ctx->frozen, ctx->exOff, ctx->exLst, ctx->exSpan, C.kmax, C.tmpCnt, C.tmpJ,
                        C.tmpMask, ctx->ctrl, CG);
     LAUNCHCHK();

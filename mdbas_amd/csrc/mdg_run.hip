@@ -256,7 +256,7 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     // In a step block of one box with three-site SHAKE, the record and the
     // list prune go with the next step's first SHAKE kernel (its spare
     // blocks); the block's last step runs them after the loop.
-    if (ctx->recOverlap && !ctx->profiling && !ctx->bt && ctx->shakeW3) {
+    if (ctx->recOverlap && !ctx->profiling && ctx->shakeW3) {
         ctx->sidePending = true;
         ctx->sideRebuilt = rebuild;
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[11], ctx->stream));
@@ -455,10 +455,15 @@ static int capture_block(mdg_ctx *ctx)
     for (int k = 0; k < U && st == MDG_OK; k++) st = mdg_device_step(ctx, k == U - 1);
     if (st == MDG_OK && ctx->sidePending) {
         // the last step's record and prune
-        hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
-                           mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx), mdg_excl_rows(ctx)), ctx->ctrl,
-                           ctx->recDev, ctx->sideRebuilt ? 1 : 0);
-        if (hipGetLastError() != hipSuccess) st = MDG_EHIP;
+        if (ctx->bt) {
+            st = b_record(ctx, ctx->sideRebuilt);
+        } else {
+            hipLaunchKernelGGL(k_record, dim3(1), dim3(SIDE_BLOCK), 0, ctx->stream,
+                               mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx),
+                                            mdg_excl_rows(ctx)),
+                               ctx->ctrl, ctx->recDev, ctx->sideRebuilt ? 1 : 0);
+            if (hipGetLastError() != hipSuccess) st = MDG_EHIP;
+        }
         if (st == MDG_OK) st = mdg_prune_list(ctx);
     }
     ctx->sidePending = false;
@@ -805,71 +810,12 @@ __global__ void k_b_group_kin(int nGrp, const int *__restrict__ grpAtomOff,
     grpVir[g] = 0.;
 }
 
-struct BRecArgs {
-    double chargeConst;
-    int R, recCap;
-    const MdgBox *boxes;
-    const double *eSelf, *eNet;
-    const int *aoff, *goff, *cluStart, *chOff;
-    int maxChunks;
-    const double *chE, *atomE, *grpKin, *grpVir, *repInfl;
-    int *stepCtr;
-    MDG_STEP *rec;                       // [step][replica]
-};
 
-// One MDG_STEP per replica (one block each): the replica's chunk, atom and
-// group sums in a fixed order, assembled as k_record does.
-__global__ void __launch_bounds__(256) k_b_record(BRecArgs R, int rebuilt)
+// The batch records as a kernel of their own (b_record_block, mdg_side.h):
+// one block per replica.
+__global__ void __launch_bounds__(SIDE_BLOCK) k_b_record(BRecArgs R, int rebuilt)
 {
-    __shared__ double red[4][10];
-    const int r = blockIdx.x;
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int c0 = min(R.chOff[R.cluStart[r]], R.maxChunks);
-    const int c1 = min(R.chOff[R.cluStart[r + 1]], R.maxChunks);
-    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x)
-        for (int k = 0; k < 5; k++) v[k] += R.chE[(long)c * 5 + k];
-    for (int i = R.aoff[r] + threadIdx.x; i < R.aoff[r + 1]; i += blockDim.x) {
-        v[5] += R.atomE[2 * (long)i];
-        v[6] += R.atomE[2 * (long)i + 1];
-    }
-    for (int g = R.goff[r] + threadIdx.x; g < R.goff[r + 1]; g += blockDim.x) {
-        v[7] += R.grpKin[g];
-        v[8] += R.grpVir[g];
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int k = 0; k < 9; k++) {
-        const double s = mdg_wave_sum(v[k]);
-        if (lane == 0) red[wid][k] = s;
-    }
-
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < 9; k++) v[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
-    const double *inf = R.repInfl + 7 * (long)r;
-    const double fact = (TWOPI_R / R.boxes[r].vol) * R.chargeConst;
-    const double er = inf[0];
-    const double st0 = fact * (inf[1] + er) + R.eNet[r];
-    const double st3 = fact * (inf[4] + er) + R.eNet[r];
-    const double st5 = fact * (inf[6] + er) + R.eNet[r];
-    const double eRec = fact * er + R.eSelf[r] + R.eNet[r];
-    const int idx = R.stepCtr[r];
-    MDG_STEP s;
-    s.step = idx;
-    s.listRebuilt = rebuilt;
-    s.eDir = v[0];
-    s.eRec = eRec;
-    s.eCorr = v[5];
-    s.elec = s.eDir + eRec + s.eCorr;
-    s.vdw = v[1];
-    s.pot = s.elec + s.vdw;
-    s.kin = 0.5 * v[7];
-    s.tot = s.pot + s.kin;
-    s.virelec = v[2] + (-(st0 + st3 + st5)) + v[6];
-    s.virvdw = v[3];
-    s.virshake = v[8];
-    s.nInCut = v[4];
-    if (idx < R.recCap) R.rec[(long)idx * R.R + r] = s;
-    R.stepCtr[r] = idx + 1;
+    b_record_block(R, blockIdx.x, rebuilt);
 }
 
 static int b_energy(mdg_ctx *ctx, bool rebuild)
@@ -946,7 +892,7 @@ static int b_group_kinetic(mdg_ctx *ctx)
     return MDG_OK;
 }
 
-static int b_record(mdg_ctx *ctx, bool rebuild)
+BRecArgs mdg_b_rec_args(mdg_ctx *ctx)
 {
     MdgBatch &T = *ctx->bt;
     BRecArgs R;
@@ -968,7 +914,13 @@ static int b_record(mdg_ctx *ctx, bool rebuild)
     R.repInfl = T.repInfl;
     R.stepCtr = T.stepCtr;
     R.rec = T.rec;
-    hipLaunchKernelGGL(k_b_record, dim3(T.R), dim3(256), 0, ctx->stream, R, rebuild ? 1 : 0);
+    return R;
+}
+
+static int b_record(mdg_ctx *ctx, bool rebuild)
+{
+    hipLaunchKernelGGL(k_b_record, dim3(ctx->bt->R), dim3(SIDE_BLOCK), 0, ctx->stream,
+                       mdg_b_rec_args(ctx), rebuild ? 1 : 0);
     LAUNCHCHK();
     return MDG_OK;
 }

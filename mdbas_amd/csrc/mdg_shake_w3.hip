@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
 {
 #pragma clang fp contract(off)
     if ((int)blockIdx.x >= S.nbMain) {
-        if (!BATCH) side_work(S);
+        side_work<BATCH>(S);
         return;
     }
     const int g = shake_w3_group(A);

@@ -201,6 +201,73 @@ __device__ inline void record_block(const RecArgs &R, int *ctrl, MDG_STEP *rec, 
     ctrl[CTRL_STEP] = idx + 1;
 }
 
+// Batch: one MDG_STEP per replica, by one block each (replica r): the
+// replica's chunk, atom and group sums in a fixed order, assembled as
+// record_block does.
+struct BRecArgs {
+    double chargeConst;
+    int R, recCap;
+    const MdgBox *boxes;
+    const double *eSelf, *eNet;
+    const int *aoff, *goff, *cluStart, *chOff;
+    int maxChunks;
+    const double *chE, *atomE, *grpKin, *grpVir, *repInfl;
+    int *stepCtr;
+    MDG_STEP *rec;                       // [step][replica]
+};
+
+__device__ inline void b_record_block(const BRecArgs &R, int r, int rebuilt)
+{
+    __shared__ double red[SIDE_BLOCK / 64][10];
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int c0 = min(R.chOff[R.cluStart[r]], R.maxChunks);
+    const int c1 = min(R.chOff[R.cluStart[r + 1]], R.maxChunks);
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+        for (int k = 0; k < 5; k++) v[k] += R.chE[(long)c * 5 + k];
+    for (int i = R.aoff[r] + threadIdx.x; i < R.aoff[r + 1]; i += blockDim.x) {
+        v[5] += R.atomE[2 * (long)i];
+        v[6] += R.atomE[2 * (long)i + 1];
+    }
+    for (int g = R.goff[r] + threadIdx.x; g < R.goff[r + 1]; g += blockDim.x) {
+        v[7] += R.grpKin[g];
+        v[8] += R.grpVir[g];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < 9; k++) {
+        const double s = mdg_wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
+
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 9; k++) v[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    const double *inf = R.repInfl + 7 * (long)r;
+    const double fact = (6.283185307179586 / R.boxes[r].vol) * R.chargeConst;
+    const double er = inf[0];
+    const double st0 = fact * (inf[1] + er) + R.eNet[r];
+    const double st3 = fact * (inf[4] + er) + R.eNet[r];
+    const double st5 = fact * (inf[6] + er) + R.eNet[r];
+    const double eRec = fact * er + R.eSelf[r] + R.eNet[r];
+    const int idx = R.stepCtr[r];
+    MDG_STEP s;
+    s.step = idx;
+    s.listRebuilt = rebuilt;
+    s.eDir = v[0];
+    s.eRec = eRec;
+    s.eCorr = v[5];
+    s.elec = s.eDir + eRec + s.eCorr;
+    s.vdw = v[1];
+    s.pot = s.elec + s.vdw;
+    s.kin = 0.5 * v[7];
+    s.tot = s.pot + s.kin;
+    s.virelec = v[2] + (-(st0 + st3 + st5)) + v[6];
+    s.virvdw = v[3];
+    s.virshake = v[8];
+    s.nInCut = v[4];
+    if (idx < R.recCap) R.rec[(long)idx * R.R + r] = s;
+    R.stepCtr[r] = idx + 1;
+}
+
 // ---------------------------------------------------------------- prune
 
 struct PruneArgs {
@@ -329,33 +396,38 @@ __device__ inline void prune_waves(const PruneArgs &A, int w0, int nw)
 
 // ---------------------------------------------------------------- spare blocks
 
-// Work for the blocks a kernel launches beyond its own nbMain: block
-// nbMain writes the record (when rec is set), the nbPrune blocks after it
-// prune the list.  Blocks have SIDE_BLOCK threads.
+// Work for the blocks a kernel launches beyond its own nbMain: the nbRec
+// blocks after them write the record (one block; a batch one per
+// replica), the nbPrune blocks after those prune the list.  Blocks have
+// SIDE_BLOCK threads.
 struct SideWork {
-    int nbMain, rec, nbPrune, rebuilt;
+    int nbMain, nbRec, nbPrune, rebuilt;
     RecArgs R;
     int *ctrl;
     MDG_STEP *out;
+    BRecArgs BR;
     PruneArgs Pr;
 };
 
+template <bool BATCH>
 __device__ inline void side_work(const SideWork &S)
 {
     int b = blockIdx.x - S.nbMain;
-    if (S.rec) {
-        if (b == 0) {
+    if (b < S.nbRec) {
+        if (BATCH)
+            b_record_block(S.BR, b, S.rebuilt);
+        else
             record_block(S.R, S.ctrl, S.out, S.rebuilt);
-            return;
-        }
-        b--;
+        return;
     }
-    if (b < S.nbPrune) prune_waves<false>(S.Pr, b * (SIDE_BLOCK / 64) + (threadIdx.x >> 6),
+    b -= S.nbRec;
+    if (b < S.nbPrune) prune_waves<BATCH>(S.Pr, b * (SIDE_BLOCK / 64) + (threadIdx.x >> 6),
                                           S.nbPrune * (SIDE_BLOCK / 64));
 }
 
 // Host side (mdg_run.hip, mdg_pair.hip).
 RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir, int nbExcl);
+BRecArgs mdg_b_rec_args(mdg_ctx *ctx);
 FinArgs mdg_fin_args(mdg_ctx *ctx);
 // Device steps of one box of three-site molecules compute the forces in
 // the first stage-2 SHAKE kernel (no k_finalize): its blocks then hold the
