@@ -281,7 +281,8 @@ int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, double dJ_d
  * priority of the SPME kernels beside it, and whether the pruned list pays
  * for its prune kernel when pruning is on) and keeps the fastest; the
  * state is restored and the initial energy recomputed.  out[0] = SHAKE
- * spread (16/spread molecules per wavefront), out[1] = pair grid cap; the
+ * spread (16/spread molecules per wavefront, or 0 for one molecule per
+ * lane), out[1] = pair grid cap; the
  * pruning choice reads back through mdg_prune_skin.  No reference
  * counterpart. */
 int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2]);

@@ -570,10 +570,12 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
 }
 
 // Blocks of the VV stage kernels (and so rows of their kinetic partials):
-// four lanes per three-site molecule, one thread per generic group.
+// four lanes per three-site molecule (one lane at SHAKE spread 0), one
+// thread per generic group.
 int mdg_vv_blocks(const mdg_ctx *ctx)
 {
     if (!ctx->shakeW3) return nblk(ctx->nGrp, 128);
+    if (ctx->tune.shkSpread == 0) return nblk(ctx->nGrp, 256);
     const long waves = nblk(ctx->nGrp, 16 / ctx->tune.shkSpread);
     return nblk(waves * 64, 256);
 }
@@ -649,6 +651,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.fstr = 0;
     A.dispPart = nullptr;
     const bool w3 = ctx->shakeW3;
+    const bool m3 = w3 && ctx->tune.shkSpread == 0;   // one molecule per lane
     const bool batch = ctx->bt != nullptr;
     if (stage == 1 && ctx->slotsInStage1 && w3 && !batch && !sync) {
         // a device step without a rebuild: the replay kernel writes the slots
@@ -705,7 +708,8 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
                 }
                 ctx->sidePending = false;
             }
-            hipLaunchKernelGGL(batch ? k_w3_vv1_a<true> : k_w3_vv1_a<false>,
+            hipLaunchKernelGGL(m3 ? (batch ? k_m3_vv1_a<true> : k_m3_vv1_a<false>)
+                                  : (batch ? k_w3_vv1_a<true> : k_w3_vv1_a<false>),
                                dim3(nb + S.nbRec + S.nbPrune), dim3(bs), 0, ctx->stream, A, S);
         } else {
             hipLaunchKernelGGL(k_vv1_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
@@ -717,8 +721,11 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
             HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evRecDone, 0));
             ctx->recPending = false;
         }
-        hipLaunchKernelGGL(batch ? k_w3_vv1_b<true> : w3 ? k_w3_vv1_b<false> : k_vv1_b, dim3(nb),
-                           dim3(bs), 0, ctx->stream, A);
+        hipLaunchKernelGGL(m3      ? (batch ? k_m3_vv1_b<true> : k_m3_vv1_b<false>)
+                           : batch ? k_w3_vv1_b<true>
+                           : w3    ? k_w3_vv1_b<false>
+                                   : k_vv1_b,
+                           dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         // SHAKE virial: sum of the per-group (generic) or per-block (three-site)
         // sums, fixed order
@@ -733,14 +740,18 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
                 F = *fin;
                 fuse = ctx->precision == MDG_MIXED ? 1 : 2;
             }
-            hipLaunchKernelGGL(batch ? k_w3_vv2_a<true> : k_w3_vv2_a<false>, dim3(nb), dim3(bs), 0,
-                               ctx->stream, A, F, fuse);
+            hipLaunchKernelGGL(m3 ? (batch ? k_m3_vv2_a<true> : k_m3_vv2_a<false>)
+                                  : (batch ? k_w3_vv2_a<true> : k_w3_vv2_a<false>),
+                               dim3(nb), dim3(bs), 0, ctx->stream, A, F, fuse);
         } else {
             hipLaunchKernelGGL(k_vv2_a, dim3(nb), dim3(bs), 0, ctx->stream, A);
         }
         LAUNCHCHK();
-        hipLaunchKernelGGL(batch ? k_w3_vv2_b<true> : w3 ? k_w3_vv2_b<false> : k_vv2_b, dim3(nb),
-                           dim3(bs), 0, ctx->stream, A);
+        hipLaunchKernelGGL(m3      ? (batch ? k_m3_vv2_b<true> : k_m3_vv2_b<false>)
+                           : batch ? k_w3_vv2_b<true>
+                           : w3    ? k_w3_vv2_b<false>
+                                   : k_vv2_b,
+                           dim3(nb), dim3(bs), 0, ctx->stream, A);
         LAUNCHCHK();
         if (sync) RETCHK(mdg_reduce_partials(ctx, A.part, nb, 7, ctx->scal + S_KIN));
     }

@@ -140,7 +140,8 @@ struct MdgBatch {
 
 // Launch shapes the autotuner chooses (mdg_autotune).
 struct MdgTune {
-    int shkSpread = 1;           // three-site SHAKE: quads per molecule slot (1, 2, 4, 8, 16)
+    int shkSpread = 1;           // three-site SHAKE: quads per molecule slot (1, 2, 4, 8, 16),
+                                 // or 0: one molecule per lane
     int pairBlocks = 1024;       // cap on the cluster pair kernel's grid (4 blocks per CU leave
                                  // wave slots for the SPME kernels beside it)
     int spmeFirst = 0;           // steps without a rebuild enqueue the SPME branch before the

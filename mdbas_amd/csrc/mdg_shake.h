@@ -37,7 +37,8 @@ struct ShakeArgs {
     const int *grep;
     double *grpKin;
     // three-site kernels: one molecule per quad of lanes, on every
-    // spread-th quad of a wavefront (16 / spread molecules per wavefront)
+    // spread-th quad of a wavefront (16 / spread molecules per wavefront);
+    // 0: one molecule per lane (the k_m3_* kernels)
     int spread;
     // stage 1 of a device step without a list rebuild (one box, mixed):
     // k_w3_vv1_b also writes the slot positions and clears the slot force
@@ -62,21 +63,41 @@ __device__ inline int shake_w3_group(const ShakeArgs &A)
     return g < A.nGrp ? g : A.nGrp;
 }
 
+// Bits of word w that count as converged without a check: sweeps past
+// kend of a group that was converged at its last two checks, and every
+// sweep index from maxCycle on (the stopping rule never reads those, so a
+// word that lies wholly beyond the last sweep that matters is all ones).
+__device__ inline unsigned long long shake_fill_bits(int w, int kend, bool settled, int maxCycle)
+{
+    const int lo = (settled ? min(kend + 1, maxCycle) : maxCycle) - w * 64;
+    if (lo <= 0) return ~0ull;
+    if (lo >= 64) return 0ull;
+    return ~0ull << lo;
+}
+
 // Publishes the group's converged-at-sweep-k bits into the device-wide
 // AND mask: bits for k <= kend are as checked; later sweeps count as
 // converged when the group was converged at its last two checks.
+//
+// Atomics on one address serialise (about 11 ns each on MI355X), so the
+// bits are ANDed within the block first and each block issues one atomic
+// per word that is not all ones.  Every thread of the block must call this.
 __device__ inline void shake_publish_mask(const ShakeArgs &A, const unsigned long long (&bits)[SHK_MASKW],
                                           int kend, bool settled)
 {
-    // AND within the wavefront first, then one atomic per wave and word
-    // (all-ones words change nothing and are skipped)
+    __shared__ unsigned long long wm[SHK_MASKW][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int w = 0; w < SHK_MASKW; w++) {
-        unsigned long long m = bits[w];
-        if (settled)
-            for (int b = 0; b < 64; b++)
-                if (w * 64 + b > kend && w * 64 + b < A.P.maxCycle) m |= 1ull << b;
+        unsigned long long m = bits[w] | shake_fill_bits(w, kend, settled, A.P.maxCycle);
         for (int o = 32; o > 0; o >>= 1) m &= __shfl_xor(m, o, 64);
-        if ((threadIdx.x & 63) == 0 && m != ~0ull) atomicAnd(&A.mask[w], m);
+        if (lane == 0) wm[w][wid] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < SHK_MASKW) {
+        const int nw = (blockDim.x + 63) >> 6;
+        unsigned long long m = ~0ull;
+        for (int v = 0; v < nw; v++) m &= wm[threadIdx.x][v];
+        if (m != ~0ull) atomicAnd(&A.mask[threadIdx.x], m);
     }
 }
 
@@ -96,10 +117,7 @@ __device__ inline void shake_publish_mask_b(const ShakeArgs &A,
     }
     const bool uniform = lo == hi;
     for (int w = 0; w < SHK_MASKW; w++) {
-        unsigned long long m = bits[w];
-        if (settled)
-            for (int b = 0; b < 64; b++)
-                if (w * 64 + b > kend && w * 64 + b < A.P.maxCycle) m |= 1ull << b;
+        unsigned long long m = bits[w] | shake_fill_bits(w, kend, settled, A.P.maxCycle);
         if (uniform) {
             for (int o = 32; o > 0; o >>= 1) m &= __shfl_xor(m, o, 64);
             if (lane == 0 && hi >= 0 && m != ~0ull) atomicAnd(&A.mask[(long)hi * SHK_MASKW + w], m);
@@ -171,3 +189,8 @@ template <bool BATCH> __global__ void k_w3_vv1_a(ShakeArgs A, SideWork S);
 template <bool BATCH> __global__ void k_w3_vv1_b(ShakeArgs A);
 template <bool BATCH> __global__ void k_w3_vv2_a(ShakeArgs A, FinArgs F, int fuse);
 template <bool BATCH> __global__ void k_w3_vv2_b(ShakeArgs A);
+// one molecule per lane (SHAKE spread 0, mdg_shake_m3.hip)
+template <bool BATCH> __global__ void k_m3_vv1_a(ShakeArgs A, SideWork S);
+template <bool BATCH> __global__ void k_m3_vv1_b(ShakeArgs A);
+template <bool BATCH> __global__ void k_m3_vv2_a(ShakeArgs A, FinArgs F, int fuse);
+template <bool BATCH> __global__ void k_m3_vv2_b(ShakeArgs A);

@@ -537,16 +537,19 @@ def test_autotune_keeps_results(c0_box):
     assert np.allclose(ra["tot"], rb["tot"], rtol=1e-13, atol=0)
 
 
-def test_launch_shapes_keep_results(c0_box):
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_launch_shapes_keep_results(c0_box, prec):
     """mdg_set_launch round-trips, and the branch order at the SPME fork,
-    the SHAKE spread and the pair grid change no per-atom arithmetic: the
-    mixed trajectories' positions and velocities agree bit for bit (the
-    step records' FP64 sums of per-block partials are grouped by the launch
-    shape, so they agree to 1e-13)."""
+    the SHAKE spread (a molecule per quad of lanes, or per lane at spread
+    0) and the pair grid change no per-atom arithmetic: the trajectories'
+    positions and velocities agree bit for bit (the step records' FP64
+    sums of per-block partials are grouped by the launch shape, so they
+    agree to 1e-13).  FP64 forces are summed with FP64 atomics, whose order
+    varies from run to run, so FP64 trajectories agree to 1e-9 A."""
     wb = c0_box
     out = []
-    for shape in ((1, 1024, 0, 0), (2, 512, 1, 1)):
-        ctx = M.Context(wb, precision=M.MDG_MIXED)
+    for shape in ((1, 1024, 0, 0), (2, 512, 1, 1), (0, 768, 0, 1)):
+        ctx = M.Context(wb, precision=M.MDG_MIXED if prec == "mixed" else M.MDG_FP64)
         ctx.set_launch(*shape)
         assert tuple(ctx.get_launch().values()) == shape
         ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
@@ -554,9 +557,34 @@ def test_launch_shapes_keep_results(c0_box):
         r = M.records_to_dict(ctx.run(25, use_graph=True))
         out.append((r, ctx.download()))
         ctx.close()
+    ra, sa = out[0]
+    for rb, sb in out[1:]:
+        if prec == "mixed":
+            for u, v in zip(sa[:6], sb[:6]):
+                assert np.array_equal(u, v)
+        else:
+            assert max(np.abs(u - v).max() for u, v in zip(sa[:3], sb[:3])) < 1e-9
+        for k in ("tot", "pot", "kin", "eRec", "virshake"):
+            assert np.allclose(ra[k], rb[k], rtol=1e-13 if prec == "mixed" else 1e-10, atol=0), k
+        assert np.array_equal(ra["nInCut"], rb["nInCut"])
+
+
+def test_batch_shake_per_lane_matches_per_quad(replicas):
+    """A replica batch with one molecule per lane in the SHAKE kernels
+    (spread 0) follows the batch with one per quad bit for bit over 25
+    steps (through the rebuild at step 20); the per-replica records, sums
+    of per-molecule terms in a fixed order, agree exactly too."""
+    out = []
+    for spread in (1, 0):
+        b = M.BatchContext(replicas, precision=M.MDG_MIXED)
+        b.set_launch(spread, 1024, 0, 0)
+        b.upload_boxes()
+        b.initial_energy()
+        rb = M.batch_records(b.run(25, use_graph=True), b.R)
+        out.append((rb, b.download()))
+        b.close()
     (ra, sa), (rb, sb) = out
     for u, v in zip(sa[:6], sb[:6]):
         assert np.array_equal(u, v)
-    for k in ("tot", "pot", "kin", "eRec", "virshake"):
-        assert np.allclose(ra[k], rb[k], rtol=1e-13, atol=0), k
-    assert np.array_equal(ra["nInCut"], rb["nInCut"])
+    for k in ("tot", "kin", "virshake"):
+        assert np.array_equal(ra[k], rb[k]), k
