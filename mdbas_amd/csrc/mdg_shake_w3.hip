@@ -26,8 +26,6 @@ constexpr int QP_L1 = 0xD5;    // [1,1,1,3]
 constexpr int QP_L2 = 0xEA;    // [2,2,2,3]
 constexpr int QP_X1 = 0xB1;    // [1,0,3,2]
 constexpr int QP_X2 = 0x4E;    // [2,3,0,1]
-constexpr int QP_R1 = 0xF1;    // [1,0,3,3]: lanes 0 <-> 1, lane 2 reads idle lane 3
-constexpr int QP_R2 = 0xDB;    // [3,2,1,3]: lanes 1 <-> 2, lane 0 reads idle lane 3
 
 template <int CTRL>
 __device__ inline double qperm(double v)
@@ -57,11 +55,13 @@ __device__ inline bool quad_all(bool c)
 //
 // Correction exchange: atom 0 receives the a-side terms of constraints 0
 // and 1, atom 1 the b-side of 0 and the a-side of 2, atom 2 the b-side of 1
-// and 2.  Lane k sends u = dd f_u in round 1 (lanes 0 <-> 1) and v = dd f_v
-// in round 2 (lanes 1 <-> 2) and keeps o = dd f_o for itself, where each f
-// is the lane's li, lj or 0; the atom's correction is (r1 + r2) + o, in
-// which exactly two terms are non-zero (a sum of two is the same in either
-// order, and adding an exact zero changes nothing).
+// and 2: in constraint order, its first constraint is cA = (0, 0, 1)[k] and
+// its second cB = (1, 2, 2)[k], the lanes that quad_perm QP_A and QP_B read.
+// Each sweep only the constraints' multipliers move (two DPP reads); the
+// atom holds the vectors dd of cA and cB and its factors fA, fB (trmi, or
+// -trmj on the b side) from the stage's start, forms li = h fA and lj = h
+// fB as the constraint's lane would, and sums (0 + dd_cA l_A) + dd_cB l_B
+// as the reference accumulates xt[] (shake.c:300-310).
 struct Q3 {
     int k;              // 0..2 live, 3 idle
     int at;             // atom k
@@ -71,7 +71,8 @@ struct Q3 {
     double ddx, ddy, ddz, ddsq;   // constraint vector at the stage start, |dd|^2
     double shx, shy, shz;         // image shift of the constraint vector (stage 1)
     double K;           // stage 1: dt (trmi + trmj); stage 2: (trmi + trmj) |dd|^2
-    double cu, cv, co;  // correction factors (per stage: +-trmi, +-trmj or 0)
+    double fA, fB;      // the atom's factors in its constraints cA, cB
+    double dAx, dAy, dAz, dBx, dBy, dBz;   // dd of cA and cB
     double rma, rmb;    // inverse masses of constraint k's atoms
 };
 
@@ -87,7 +88,8 @@ __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
     Q.ddx = Q.ddy = Q.ddz = Q.ddsq = 0.;
     Q.shx = Q.shy = Q.shz = 0.;
     Q.K = 1.;
-    Q.cu = Q.cv = Q.co = 0.;
+    Q.fA = Q.fB = 0.;
+    Q.dAx = Q.dAy = Q.dAz = Q.dBx = Q.dBy = Q.dBz = 0.;
     if (k < 3) {
         const int i = A.atoms[A.atomOff[g] + k];
         Q.at = i;
@@ -107,14 +109,23 @@ __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
     Q.rmb = qperm<QP_B>(Q.rm);
 }
 
-// The lane's correction factors from its constraint's a-side and b-side
-// coefficients ti, tj (li = h ti, lj = -h tj for the sweep's multiplier h).
+// The atom's factors and constraint vectors from its constraint lanes'
+// a-side and b-side coefficients ti, tj (li = h ti, lj = -h tj for the
+// sweep's multiplier h): atom 0 is the a side of constraints 0 and 1, atom
+// 1 the b side of 0 and the a side of 2, atom 2 the b side of 1 and 2.
 __device__ inline void q_factors(Q3 &Q, double ti, double tj)
 {
-    const double mi = ti, mj = -tj;
-    Q.cu = Q.k == 0 ? mj : (Q.k == 1 ? mi : 0.);
-    Q.cv = Q.k == 1 ? mj : (Q.k == 2 ? mi : 0.);
-    Q.co = Q.k == 0 ? mi : (Q.k == 2 ? mj : 0.);
+    const double aI = qperm<QP_A>(ti), aJ = qperm<QP_A>(tj);
+    const double bI = qperm<QP_B>(ti), bJ = qperm<QP_B>(tj);
+    Q.fA = Q.k == 0 ? aI : -aJ;
+    Q.fB = Q.k < 2 ? bI : -bJ;
+    Q.dAx = qperm<QP_A>(Q.ddx);
+    Q.dAy = qperm<QP_A>(Q.ddy);
+    Q.dAz = qperm<QP_A>(Q.ddz);
+    Q.dBx = qperm<QP_B>(Q.ddx);
+    Q.dBy = qperm<QP_B>(Q.ddy);
+    Q.dBz = qperm<QP_B>(Q.ddz);
+    if (Q.k >= 3) Q.fA = Q.fB = 0.;
 }
 
 // Stage 1: lambda denominators and factors (trmi = dt rma, trmj = dt rmb;
@@ -262,12 +273,15 @@ __device__ inline bool q_check(const MdgBox &B, const Q3 &Q, double &dtx, double
     return quad_all(Q.k >= 3 || fabs(rt2 - Q.rc2) < Q.thr);
 }
 
-// The atom's correction from the per-lane terms of its two constraints.
-__device__ inline double q_exchange(const Q3 &Q, double dd, double h)
+// The atom's corrections from its two constraints' multipliers h (every
+// lane takes part in the moves).
+__device__ inline void q_exchange(const Q3 &Q, double h, double &xt, double &yt, double &zt)
 {
 #pragma clang fp contract(off)
-    const double u = dd * (h * Q.cu), v = dd * (h * Q.cv), o = dd * (h * Q.co);
-    return (qperm<QP_R1>(u) + qperm<QP_R2>(v)) + o;
+    const double lA = qperm<QP_A>(h) * Q.fA, lB = qperm<QP_B>(h) * Q.fB;
+    xt = (0. + Q.dAx * lA) + Q.dBx * lB;
+    yt = (0. + Q.dAy * lA) + Q.dBy * lB;
+    zt = (0. + Q.dAz * lA) + Q.dBz * lB;
 }
 
 // v + t0 + t1 + t2 over the three live lanes, in that order, on every lane.
@@ -290,20 +304,18 @@ __device__ inline void q_apply_r(const ShakeArgs &A, Q3 &Q, double dtx, double d
     double lambda = -(Q.rc2 - rt2) / (Q.K * dot);
     lambda = Q.k < 3 ? lambda : 0.;
     vir += lambda * Q.ddsq;
-    const double h = 0.5 * lambda;
-    const double xt = q_exchange(Q, Q.ddx, h), yt = q_exchange(Q, Q.ddy, h),
-                 zt = q_exchange(Q, Q.ddz, h);
-    if (Q.k < 3) {
-        // the correction once per constraint membership (two per atom)
-        const double ix = dt * xt * Q.nat, iy = dt * yt * Q.nat, iz = dt * zt * Q.nat;
-        Q.x = (Q.x + ix) + ix;
-        Q.y = (Q.y + iy) + iy;
-        Q.z = (Q.z + iz) + iz;
-        const double jx = xt * Q.nat, jy = yt * Q.nat, jz = zt * Q.nat;
-        Q.vx = (Q.vx + jx) + jx;
-        Q.vy = (Q.vy + jy) + jy;
-        Q.vz = (Q.vz + jz) + jz;
-    }
+    double xt, yt, zt;
+    q_exchange(Q, 0.5 * lambda, xt, yt, zt);
+    // the correction once per constraint membership (two per atom); the
+    // idle lane has nat = 0 and zero corrections
+    const double ix = dt * xt * Q.nat, iy = dt * yt * Q.nat, iz = dt * zt * Q.nat;
+    Q.x = (Q.x + ix) + ix;
+    Q.y = (Q.y + iy) + iy;
+    Q.z = (Q.z + iz) + iz;
+    const double jx = xt * Q.nat, jy = yt * Q.nat, jz = zt * Q.nat;
+    Q.vx = (Q.vx + jx) + jx;
+    Q.vy = (Q.vy + jy) + jy;
+    Q.vz = (Q.vz + jz) + jz;
 }
 
 // shake.c:378-413: one velocity sweep's lambda; converged when every
@@ -316,9 +328,7 @@ __device__ inline bool q_lambda_v(const Q3 &Q, double tolvel, double &xt, double
     const double dvz = qperm<QP_B>(Q.vz) - qperm<QP_A>(Q.vz);
     double lambda = (Q.ddx * dvx + Q.ddy * dvy + Q.ddz * dvz) / Q.K;
     lambda = Q.k < 3 ? lambda : 0.;
-    xt = q_exchange(Q, Q.ddx, lambda);
-    yt = q_exchange(Q, Q.ddy, lambda);
-    zt = q_exchange(Q, Q.ddz, lambda);
+    q_exchange(Q, lambda, xt, yt, zt);
     return quad_all(fabs(lambda) < tolvel);
 }
 
@@ -326,7 +336,6 @@ __device__ inline bool q_lambda_v(const Q3 &Q, double tolvel, double &xt, double
 __device__ inline void q_apply_v(Q3 &Q, double xt, double yt, double zt)
 {
 #pragma clang fp contract(off)
-    if (Q.k >= 3) return;
     const double jx = xt * Q.nat, jy = yt * Q.nat, jz = zt * Q.nat;
     Q.vx = (Q.vx + jx) + jx;
     Q.vy = (Q.vy + jy) + jy;
@@ -438,7 +447,12 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
             prev = cv;
             if (it + 1 >= A.P.maxCycle) break;
             q_apply_r(A, Q, dtx, dty, dtz, rt2, vir);
-            if ((it & 63) == 63) qb_flush(bits, it >> 6);
+            {
+                // inside the loop every live lane of the wavefront is at the
+                // same sweep, so the word is uniform
+                const int its = __builtin_amdgcn_readfirstlane(it);
+                if ((its & 63) == 63) qb_flush(bits, its >> 6);
+            }
             it++;
         }
         qb_flush(bits, it >> 6);
@@ -595,7 +609,12 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fu
             prev = cv;
             if (it + 1 >= A.P.maxCycle) break;
             q_apply_v(Q, xt, yt, zt);
-            if ((it & 63) == 63) qb_flush(bits, it >> 6);
+            {
+                // inside the loop every live lane of the wavefront is at the
+                // same sweep, so the word is uniform
+                const int its = __builtin_amdgcn_readfirstlane(it);
+                if ((its & 63) == 63) qb_flush(bits, its >> 6);
+            }
             it++;
         }
         qb_flush(bits, it >> 6);
