@@ -350,8 +350,11 @@ __global__ void __launch_bounds__(256) k_pair_fp64(ClusterArgs A)
             const int sj = J * MDG_CL + b;
             double dx = A.sx[sj] - xi, dy = A.sy[sj] - yi, dz = A.sz[sj] - zi;
             const double r2 = mdg_dist(Bx, dx, dy, dz);
+            const bool inCut = ((m >> bit) & 1ull) && r2 <= A.P.cutOff2;
+            // an entry with no pair inside the cutoff adds only exact zeros
+            if (__builtin_amdgcn_ballot_w64(inCut) == 0) continue;
             double fjx = 0., fjy = 0., fjz = 0.;
-            if (((m >> bit) & 1ull) && r2 <= A.P.cutOff2) {
+            if (inCut) {
                 const double r = sqrt(r2), rt = 1. / r;
                 double dEl, dVw;
                 const double el = ewald_dir64(A.P.alpha, A.P.chargeConst * qid * A.sq[sj], r, rt,
@@ -477,6 +480,11 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
     dz = fma(-L2, rint(dz * iL2), dz);
     const double r2 = dx * dx + dy * dy + dz * dz;
     const unsigned long long inm = m & __builtin_amdgcn_ballot_w64(r2 <= A.F.cutOff2d);
+    // an entry with no pair inside the cutoff adds exact zeros everywhere
+    // (zero coefficients, no j-force atomics): skip it, wave-uniformly.
+    // The list keeps entries with a pair within cutOff + delr (or + the
+    // prune skin), so a sixth to a third of them have none.
+    if (inm == 0) return;
     const bool in = __builtin_amdgcn_inverse_ballot_w64(inm);
     acc.nin += __builtin_popcountll(inm);
     const float r2f = in ? (float)r2 : A.F.cutOff2;
