@@ -469,7 +469,7 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->fix, ctx->frx, ctx->fry, ctx->frz, ctx->frx64,
                     ctx->fry64, ctx->frz64, ctx->epart, ctx->recDev, ctx->theta, ctx->theta32,
                     ctx->grid64, ctx->cgrid64, ctx->grid32, ctx->cgrid32, ctx->gridq,
-                    ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend,
+                    ctx->fftWork, ctx->cl.chunk, ctx->cl.chOff, ctx->shkScr, ctx->shkVir, ctx->shkFirst, ctx->shkKend, ctx->shkRing,
                     ctx->shkMask, ctx->exSymOff, ctx->exSymLst, ctx->adjQ, ctx->adjE,
                     ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binBucket, ctx->binOut,
                     ctx->binNOut, ctx->splI, ctx->splB, ctx->splD,

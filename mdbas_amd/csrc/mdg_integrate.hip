@@ -548,12 +548,14 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
     ctx->nGrp = ng;
     if (ctx->shkScr) {
         hipFree(ctx->shkScr); hipFree(ctx->shkVir); hipFree(ctx->shkFirst); hipFree(ctx->shkKend);
+        hipFree(ctx->shkRing);
         hipFree(ctx->shkMask);
     }
     HIPCHK(hipMalloc(&ctx->shkScr, (size_t)SHK_SLOTS * 6 * n * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->shkVir, (size_t)SHK_SLOTS * std::max(ng, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->shkFirst, std::max(ng, 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->shkKend, std::max(ng, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->shkRing, SHK_R * std::max(ng, 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->shkMask, 2 * SHK_MASKW * sizeof(unsigned long long)));
     HIPCHK(hipMalloc(&ctx->grpAtomOff, (ng + 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->grpConOff, (ng + 1) * sizeof(int)));
@@ -636,6 +638,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.virScr = ctx->shkVir;
     A.first = ctx->shkFirst;
     A.kend = ctx->shkKend;
+    A.ring = ctx->shkRing;
     // each stage has its own mask (stage 2's beside stage 1's), so a step's
     // record may run while the next step's stage 1 publishes its sweeps
     A.mask = ctx->shkMask + (stage == 1 ? 0 : SHK_MASKW);

@@ -200,6 +200,7 @@ struct mdg_ctx {
     int *ctrl = nullptr;         // device control words (see CTRL_*)
     double *shkScr = nullptr, *shkVir = nullptr;   // SHAKE state window (mdg_integrate.hip)
     int *shkFirst = nullptr, *shkKend = nullptr;
+    int *shkRing = nullptr;      // per group: the sweeps in the two ring slots
     unsigned long long *shkMask = nullptr;   // [stage 1 | stage 2][SHK_MASKW]
     bool shakeW3 = false;        // all groups are three-site molecules
     // exclusions (CSR; symmetric for link cells, ordered otherwise)

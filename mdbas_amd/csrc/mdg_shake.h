@@ -4,9 +4,13 @@
 #include "mdg_internal.h"
 #include "mdg_side.h"
 
-// Window of SHAKE states kept per group around its first convergence.
+// Window of SHAKE states kept per group around its first convergence, and
+// a ring of SHK_R for the last sweeps at which a whole wavefront was
+// converged.
 #define SHK_W 8
-#define SHK_SLOTS (SHK_W + 2)     // [0] start, [1..W] window, [W+1] end
+#define SHK_R 4
+// [0] start, [1..W] window, [W+1] end, [W+2 .. W+1+R] ring
+#define SHK_SLOTS (SHK_W + 2 + SHK_R)
 #define SHK_MASKW 4               // 256 sweep indices
 
 struct ShakeArgs {
@@ -30,6 +34,8 @@ struct ShakeArgs {
     double *scr;          // [SHK_SLOTS][6][n] saved states
     double *virScr;       // [SHK_SLOTS][nGrp] SHAKE virial at those states
     int *first, *kend;    // per group: first converged sweep, last checked sweep
+    int *ring;            // three-site kernels, per group: the sweep in each ring
+                          // slot (SHK_R ints, -1 when empty)
     unsigned long long *mask;   // device-wide AND of converged-sweep bits
     // replica batch: box and mask per replica, replica per group, and the
     // per-group sum m v^2 (stage 2)
@@ -144,12 +150,20 @@ __device__ inline int shake_global_stop(const ShakeArgs &A, const unsigned long 
 __device__ inline int shake_global_stop(const ShakeArgs &A) { return shake_global_stop(A, A.mask); }
 
 
-// Which saved state leads to sweep C: the window slot, the end state, or
-// the start (when C lies beyond the window but before the end).
-__device__ inline void shake_pick_slot(int C, int first, int kend, int &slot, int &at)
+// Which saved state leads to sweep C: a window or ring slot, the end
+// state, or the start (when C lies outside both and before the end).  ring:
+// the group's SHK_R ring sweeps (-1 when empty).
+__device__ inline void shake_pick_slot(int C, int first, int kend, const int *ring, int &slot, int &at)
 {
+    int rs = -1;
+    if (ring)
+        for (int q = 0; q < SHK_R; q++)
+            if (ring[q] == C) rs = q;
     if (first >= 0 && C >= first && C - first < SHK_W && C <= kend) {
         slot = 1 + C - first;
+        at = C;
+    } else if (rs >= 0) {
+        slot = SHK_W + 2 + rs;
         at = C;
     } else if (C >= kend) {
         slot = SHK_W + 1;

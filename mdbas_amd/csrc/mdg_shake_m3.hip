@@ -419,9 +419,13 @@ __global__ void __launch_bounds__(256) k_m3_vv1_a(ShakeArgs A, SideWork S)
         for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = 0;
         bits.cur = 0;
         double vir[3] = {0., 0., 0.};
-        int first = -1, it = 0;
+        int first = -1, it = 0, nr = 0;
+        int rr[SHK_R] = {-1, -1, -1, -1};
         bool prev = false;
         settled = false;
+        // molecules sweep on until every molecule of the wavefront has
+        // settled, with a ring of the states at wavefront-converged sweeps
+        // (see k_w3_vv1_a)
         for (;;) {
             double dtx[3], dty[3], dtz[3], rt2[3];
             const bool cv = m_check(B, M, dtx, dty, dtz, rt2);
@@ -431,7 +435,15 @@ __global__ void __launch_bounds__(256) k_m3_vv1_a(ShakeArgs A, SideWork S)
                 m_scr_store(A, M, 1 + it - first, true);
                 A.virScr[(long)(1 + it - first) * A.nGrp + g] = m_sum3(vir);
             }
-            if (cv && prev) {
+            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+                m_scr_store(A, M, SHK_W + 2 + (nr & (SHK_R - 1)), true);
+                A.virScr[(long)(SHK_W + 2 + (nr & (SHK_R - 1))) * A.nGrp + g] = m_sum3(vir);
+#pragma unroll
+                for (int q = 0; q < SHK_R; q++)
+                    if (q == (nr & (SHK_R - 1))) rr[q] = it;
+                nr++;
+            }
+            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
                 settled = true;
                 break;
             }
@@ -446,6 +458,7 @@ __global__ void __launch_bounds__(256) k_m3_vv1_a(ShakeArgs A, SideWork S)
         A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = m_sum3(vir);
         A.first[g] = first;
         A.kend[g] = it;
+        for (int q = 0; q < SHK_R; q++) A.ring[(long)SHK_R * g + q] = rr[q];
         kend = it;
     }
     m_publish<BATCH>(A, g, bits.w, kend, settled);
@@ -474,7 +487,7 @@ __global__ void __launch_bounds__(256) k_m3_vv1_b(ShakeArgs A)
             const MdgBox &B = m_box<BATCH>(A, g);
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, slot, at);
+            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
             m_dd_load(A, M);
             m_setup_r(A, M);
             m_scr_load(A, M, slot, true);
@@ -575,7 +588,8 @@ __global__ void __launch_bounds__(256) k_m3_vv2_a(ShakeArgs A, FinArgs F, int fu
 #pragma unroll
         for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = 0;
         bits.cur = 0;
-        int first = -1, it = 0;
+        int first = -1, it = 0, nr = 0;
+        int rr[SHK_R] = {-1, -1, -1, -1};
         bool prev = false;
         settled = false;
         for (;;) {
@@ -587,7 +601,14 @@ __global__ void __launch_bounds__(256) k_m3_vv2_a(ShakeArgs A, FinArgs F, int fu
             if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) m_scr_store(A, M, 1 + it - first, false);
-            if (cv && prev) {
+            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+                m_scr_store(A, M, SHK_W + 2 + (nr & (SHK_R - 1)), false);
+#pragma unroll
+                for (int q = 0; q < SHK_R; q++)
+                    if (q == (nr & (SHK_R - 1))) rr[q] = it;
+                nr++;
+            }
+            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
                 settled = true;
                 break;
             }
@@ -601,6 +622,7 @@ __global__ void __launch_bounds__(256) k_m3_vv2_a(ShakeArgs A, FinArgs F, int fu
         m_scr_store(A, M, SHK_W + 1, false);
         A.first[g] = first;
         A.kend[g] = it;
+        for (int q = 0; q < SHK_R; q++) A.ring[(long)SHK_R * g + q] = rr[q];
         kend = it;
     }
     m_publish<BATCH>(A, g, bits.w, kend, settled);
@@ -625,7 +647,7 @@ __global__ void __launch_bounds__(256) k_m3_vv2_b(ShakeArgs A)
         } else {
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, slot, at);
+            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
             m_dd_load(A, M);
             m_setup_v(A, M);
             m_scr_load(A, M, slot, false);

@@ -427,9 +427,17 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
         for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = 0;
         bits.cur = 0;
         double vir = 0.;
-        int first = -1, it = 0;
+        int first = -1, it = 0, nr = 0;
+        int rr[SHK_R] = {-1, -1, -1, -1};
         bool prev = false;
         settled = false;
+        // One system: molecules sweep on until every molecule of the
+        // wavefront has settled (the wavefront's time is the same; the later
+        // sweeps are the ones the replay would make).  A sweep at which the
+        // whole wavefront is converged may be the global stop: its state
+        // goes to the ring slot of its parity.  In a batch a wavefront can
+        // hold two replicas with different stops, so each molecule stops on
+        // its own.
         for (;;) {
             double dtx, dty, dtz, rt2;
             const bool cv = q_check(B, Q, dtx, dty, dtz, rt2);
@@ -440,7 +448,16 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
                 const double vm = q_sum3(0., vir);
                 if (k == 0) A.virScr[(long)(1 + it - first) * A.nGrp + g] = vm;
             }
-            if (cv && prev) {
+            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+                q_scr_store(A, Q, SHK_W + 2 + (nr & (SHK_R - 1)), true);
+                const double vm = q_sum3(0., vir);
+                if (k == 0) A.virScr[(long)(SHK_W + 2 + (nr & (SHK_R - 1))) * A.nGrp + g] = vm;
+#pragma unroll
+                for (int q = 0; q < SHK_R; q++)
+                    if (q == (nr & (SHK_R - 1))) rr[q] = it;
+                nr++;
+            }
+            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
                 settled = true;
                 break;
             }
@@ -462,6 +479,7 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
             A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vm;
             A.first[g] = first;
             A.kend[g] = it;
+            for (int q = 0; q < SHK_R; q++) A.ring[(long)SHK_R * g + q] = rr[q];
         }
         kend = it;
     }
@@ -488,7 +506,7 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
             const MdgBox &B = q_box<BATCH>(A, g);
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, slot, at);
+            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
             Q3 Q;
             q_load(A, g, k, Q);
             q_dd_load(A, Q);
@@ -590,19 +608,29 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fu
         const double tolvel = A.P.tolShake * A.P.rTimeStep;
         for (int w = 0; w < SHK_MASKW; w++) bits.w[w] = 0;
         bits.cur = 0;
-        int first = -1, it = 0;
+        int first = -1, it = 0, nr = 0;
+        int rr[SHK_R] = {-1, -1, -1, -1};
         bool prev = false;
         settled = false;
         for (;;) {
             // the check is the sweep's own max |lambda| (shake.c:421); every
             // sweep that is not the last one checked is applied, because the
-            // reference's global loop goes on while any group is unconverged
+            // reference's global loop goes on while any group is unconverged.
+            // The wavefront sweeps on until all its molecules have settled,
+            // with the ring of wavefront-converged states, as in stage 1.
             double xt, yt, zt;
             const bool cv = q_lambda_v(Q, tolvel, xt, yt, zt);
             if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) q_scr_store(A, Q, 1 + it - first, false);
-            if (cv && prev) {
+            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+                q_scr_store(A, Q, SHK_W + 2 + (nr & (SHK_R - 1)), false);
+#pragma unroll
+                for (int q = 0; q < SHK_R; q++)
+                    if (q == (nr & (SHK_R - 1))) rr[q] = it;
+                nr++;
+            }
+            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
                 settled = true;
                 break;
             }
@@ -622,6 +650,7 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fu
         if (k == 0) {
             A.first[g] = first;
             A.kend[g] = it;
+            for (int q = 0; q < SHK_R; q++) A.ring[(long)SHK_R * g + q] = rr[q];
         }
         kend = it;
     }
@@ -648,7 +677,7 @@ __global__ void __launch_bounds__(256) k_w3_vv2_b(ShakeArgs A)
         } else {
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, slot, at);
+            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
             q_dd_load(A, Q);
             q_setup_v(A, Q);
             q_scr_load(A, Q, slot, false);
