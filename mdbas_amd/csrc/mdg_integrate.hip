@@ -552,7 +552,7 @@ int mdg_build_groups(mdg_ctx *ctx, const MDG_CONSTRAINT *cons, int nConst)
         hipFree(ctx->shkMask);
     }
     HIPCHK(hipMalloc(&ctx->shkScr, (size_t)SHK_SLOTS * 6 * n * sizeof(double)));
-    HIPCHK(hipMalloc(&ctx->shkVir, (size_t)SHK_SLOTS * std::max(ng, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->shkVir, (size_t)SHK_SLOTS * 3 * std::max(ng, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->shkFirst, std::max(ng, 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->shkKend, std::max(ng, 1) * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->shkRing, SHK_R * std::max(ng, 1) * sizeof(int)));

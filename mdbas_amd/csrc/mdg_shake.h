@@ -32,7 +32,8 @@ struct ShakeArgs {
     int *ctrl;
     double *part;         // stage 2: kin, stresk[6]
     double *scr;          // [SHK_SLOTS][6][n] saved states
-    double *virScr;       // [SHK_SLOTS][nGrp] SHAKE virial at those states
+    double *virScr;       // SHAKE virial at those states: [SHK_SLOTS][nGrp] for generic
+                          // groups, [SHK_SLOTS][nGrp][3] (per constraint) for three-site
     int *first, *kend;    // per group: first converged sweep, last checked sweep
     int *ring;            // three-site kernels, per group: the sweep in each ring
                           // slot (SHK_R ints, -1 when empty)

@@ -336,6 +336,13 @@ __device__ inline void m_apply_v(M3 &M, const double (&xt)[3], const double (&yt
     }
 }
 
+// The constraints' running SHAKE virial sums at a saved state.
+__device__ inline void m_vir_store(const ShakeArgs &A, int g, int slot, const double (&vir)[3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) A.virScr[((long)slot * A.nGrp + g) * 3 + c] = vir[c];
+}
+
 // v + t0 + t1 + t2 in that order.
 __device__ inline double m_sum3(const double (&t)[3])
 {
@@ -421,6 +428,9 @@ __global__ void __launch_bounds__(256) k_m3_vv1_a(ShakeArgs A, SideWork S)
         double vir[3] = {0., 0., 0.};
         int first = -1, it = 0, nr = 0;
         int rr[SHK_R] = {-1, -1, -1, -1};
+        // one system: the wavefront sweeps as a whole; a batch (measured
+        // slower that way), molecule by molecule
+        const bool whole = !BATCH;
         bool prev = false;
         settled = false;
         // molecules sweep on until every molecule of the wavefront has
@@ -433,17 +443,17 @@ __global__ void __launch_bounds__(256) k_m3_vv1_a(ShakeArgs A, SideWork S)
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) {
                 m_scr_store(A, M, 1 + it - first, true);
-                A.virScr[(long)(1 + it - first) * A.nGrp + g] = m_sum3(vir);
+                m_vir_store(A, g, 1 + it - first, vir);
             }
-            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+            if (whole && __builtin_amdgcn_ballot_w64(!cv) == 0) {
                 m_scr_store(A, M, SHK_W + 2 + (nr & (SHK_R - 1)), true);
-                A.virScr[(long)(SHK_W + 2 + (nr & (SHK_R - 1))) * A.nGrp + g] = m_sum3(vir);
+                m_vir_store(A, g, SHK_W + 2 + (nr & (SHK_R - 1)), vir);
 #pragma unroll
                 for (int q = 0; q < SHK_R; q++)
                     if (q == (nr & (SHK_R - 1))) rr[q] = it;
                 nr++;
             }
-            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
+            if (whole ? __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0 : cv && prev) {
                 settled = true;
                 break;
             }
@@ -455,7 +465,7 @@ __global__ void __launch_bounds__(256) k_m3_vv1_a(ShakeArgs A, SideWork S)
         }
         mb_flush(bits, it >> 6);
         m_scr_store(A, M, SHK_W + 1, true);
-        A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = m_sum3(vir);
+        m_vir_store(A, g, SHK_W + 1, vir);
         A.first[g] = first;
         A.kend[g] = it;
         for (int q = 0; q < SHK_R; q++) A.ring[(long)SHK_R * g + q] = rr[q];
@@ -487,14 +497,16 @@ __global__ void __launch_bounds__(256) k_m3_vv1_b(ShakeArgs A)
             const MdgBox &B = m_box<BATCH>(A, g);
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
+            shake_pick_slot(C, first, kend, A.ring + (long)SHK_R * g, slot, at);
             m_dd_load(A, M);
             m_setup_r(A, M);
             m_scr_load(A, M, slot, true);
             m_shift_now(B, M);
-            // a restart from a saved state continues the saved molecule sum
-            // in the first constraint's accumulator, as the quad kernels do
-            double vir[3] = {slot == 0 ? 0. : A.virScr[(long)slot * A.nGrp + g], 0., 0.};
+            // each constraint continues its own running sum, as in the quad
+            // kernels
+            double vir[3] = {0., 0., 0.};
+            if (slot != 0)
+                for (int c = 0; c < 3; c++) vir[c] = A.virScr[((long)slot * A.nGrp + g) * 3 + c];
             for (int it = at; it < C; it++) {
                 double dtx[3], dty[3], dtz[3], rt2[3];
                 m_check(B, M, dtx, dty, dtz, rt2);
@@ -590,6 +602,9 @@ __global__ void __launch_bounds__(256) k_m3_vv2_a(ShakeArgs A, FinArgs F, int fu
         bits.cur = 0;
         int first = -1, it = 0, nr = 0;
         int rr[SHK_R] = {-1, -1, -1, -1};
+        // one system: the wavefront sweeps as a whole; a batch (measured
+        // slower that way), molecule by molecule
+        const bool whole = !BATCH;
         bool prev = false;
         settled = false;
         for (;;) {
@@ -601,14 +616,14 @@ __global__ void __launch_bounds__(256) k_m3_vv2_a(ShakeArgs A, FinArgs F, int fu
             if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) m_scr_store(A, M, 1 + it - first, false);
-            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+            if (whole && __builtin_amdgcn_ballot_w64(!cv) == 0) {
                 m_scr_store(A, M, SHK_W + 2 + (nr & (SHK_R - 1)), false);
 #pragma unroll
                 for (int q = 0; q < SHK_R; q++)
                     if (q == (nr & (SHK_R - 1))) rr[q] = it;
                 nr++;
             }
-            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
+            if (whole ? __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0 : cv && prev) {
                 settled = true;
                 break;
             }
@@ -647,7 +662,7 @@ __global__ void __launch_bounds__(256) k_m3_vv2_b(ShakeArgs A)
         } else {
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
+            shake_pick_slot(C, first, kend, A.ring + (long)SHK_R * g, slot, at);
             m_dd_load(A, M);
             m_setup_v(A, M);
             m_scr_load(A, M, slot, false);

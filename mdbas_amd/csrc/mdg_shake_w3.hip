@@ -429,15 +429,18 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
         double vir = 0.;
         int first = -1, it = 0, nr = 0;
         int rr[SHK_R] = {-1, -1, -1, -1};
+        // one system: the wavefront sweeps as a whole; a batch (measured
+        // slower that way), molecule by molecule
+        const bool whole = !BATCH;
         bool prev = false;
         settled = false;
-        // One system: molecules sweep on until every molecule of the
-        // wavefront has settled (the wavefront's time is the same; the later
-        // sweeps are the ones the replay would make).  A sweep at which the
-        // whole wavefront is converged may be the global stop: its state
-        // goes to the ring slot of its parity.  In a batch a wavefront can
-        // hold two replicas with different stops, so each molecule stops on
-        // its own.
+        // Molecules sweep on until every molecule of the wavefront has
+        // settled (the wavefront's time is the same; the later sweeps are
+        // the ones the replay would make).  A sweep at which the whole
+        // wavefront is converged may be the global stop: its state goes to
+        // the next of SHK_R ring slots.  In a replica batch each molecule
+        // stops on its own (a wavefront can hold two replicas with different
+        // stops, and the batch measured slower with whole-wavefront sweeps).
         for (;;) {
             double dtx, dty, dtz, rt2;
             const bool cv = q_check(B, Q, dtx, dty, dtz, rt2);
@@ -445,19 +448,17 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) {
                 q_scr_store(A, Q, 1 + it - first, true);
-                const double vm = q_sum3(0., vir);
-                if (k == 0) A.virScr[(long)(1 + it - first) * A.nGrp + g] = vm;
+                if (k < 3) A.virScr[((long)(1 + it - first) * A.nGrp + g) * 3 + k] = vir;
             }
-            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+            if (whole && __builtin_amdgcn_ballot_w64(!cv) == 0) {
                 q_scr_store(A, Q, SHK_W + 2 + (nr & (SHK_R - 1)), true);
-                const double vm = q_sum3(0., vir);
-                if (k == 0) A.virScr[(long)(SHK_W + 2 + (nr & (SHK_R - 1))) * A.nGrp + g] = vm;
+                if (k < 3) A.virScr[((long)(SHK_W + 2 + (nr & (SHK_R - 1))) * A.nGrp + g) * 3 + k] = vir;
 #pragma unroll
                 for (int q = 0; q < SHK_R; q++)
                     if (q == (nr & (SHK_R - 1))) rr[q] = it;
                 nr++;
             }
-            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
+            if (whole ? __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0 : cv && prev) {
                 settled = true;
                 break;
             }
@@ -474,9 +475,8 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
         }
         qb_flush(bits, it >> 6);
         q_scr_store(A, Q, SHK_W + 1, true);
-        const double vm = q_sum3(0., vir);
+        if (k < 3) A.virScr[((long)(SHK_W + 1) * A.nGrp + g) * 3 + k] = vir;
         if (k == 0) {
-            A.virScr[(long)(SHK_W + 1) * A.nGrp + g] = vm;
             A.first[g] = first;
             A.kend[g] = it;
             for (int q = 0; q < SHK_R; q++) A.ring[(long)SHK_R * g + q] = rr[q];
@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
             const MdgBox &B = q_box<BATCH>(A, g);
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
+            shake_pick_slot(C, first, kend, A.ring + (long)SHK_R * g, slot, at);
             Q3 Q;
             q_load(A, g, k, Q);
             q_dd_load(A, Q);
@@ -514,7 +514,9 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
             q_scr_load(A, Q, slot, true);
             q_shift_now(B, Q);
             // the saved virial is the molecule's; it continues on lane 0
-            double vir = (slot == 0 || k != 0) ? 0. : A.virScr[(long)slot * A.nGrp + g];
+            // each constraint continues its own running sum, so the result
+            // does not depend on the slot the replay starts from
+            double vir = (slot == 0 || k >= 3) ? 0. : A.virScr[((long)slot * A.nGrp + g) * 3 + k];
             for (int it = at; it < C; it++) {
                 double dtx, dty, dtz, rt2;
                 q_check(B, Q, dtx, dty, dtz, rt2);
@@ -610,6 +612,9 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fu
         bits.cur = 0;
         int first = -1, it = 0, nr = 0;
         int rr[SHK_R] = {-1, -1, -1, -1};
+        // one system: the wavefront sweeps as a whole; a batch (measured
+        // slower that way), molecule by molecule
+        const bool whole = !BATCH;
         bool prev = false;
         settled = false;
         for (;;) {
@@ -623,14 +628,14 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fu
             if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) q_scr_store(A, Q, 1 + it - first, false);
-            if (!BATCH && __builtin_amdgcn_ballot_w64(!cv) == 0) {
+            if (whole && __builtin_amdgcn_ballot_w64(!cv) == 0) {
                 q_scr_store(A, Q, SHK_W + 2 + (nr & (SHK_R - 1)), false);
 #pragma unroll
                 for (int q = 0; q < SHK_R; q++)
                     if (q == (nr & (SHK_R - 1))) rr[q] = it;
                 nr++;
             }
-            if (BATCH ? cv && prev : __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0) {
+            if (whole ? __builtin_amdgcn_ballot_w64(!(cv && prev)) == 0 : cv && prev) {
                 settled = true;
                 break;
             }
@@ -677,7 +682,7 @@ __global__ void __launch_bounds__(256) k_w3_vv2_b(ShakeArgs A)
         } else {
             const int first = A.first[g], kend = A.kend[g];
             int slot, at;
-            shake_pick_slot(C, first, kend, BATCH ? nullptr : A.ring + (long)SHK_R * g, slot, at);
+            shake_pick_slot(C, first, kend, A.ring + (long)SHK_R * g, slot, at);
             q_dd_load(A, Q);
             q_setup_v(A, Q);
             q_scr_load(A, Q, slot, false);
