@@ -355,6 +355,10 @@ __global__ void __launch_bounds__(128) k_vv1_b(ShakeArgs A)
         shake_r_check(A, G, dtx, dty, dtz, rt2);
         shake_r_apply(A, G, dtx, dty, dtz, rt2, vir);
     }
+    // the state at C passes the check, unless a group re-diverged after
+    // it was taken as settled
+    if (!(shake_r_check(A, G, dtx, dty, dtz, rt2) < A.P.tolShake))
+        atomicOr(&A.ctrl[CTRL_SHAKE_ORDER], 1);
     A.vir[g] = vir;
     grp_store(A, G, true);
 }
@@ -451,6 +455,8 @@ __global__ void __launch_bounds__(128) k_vv2_b(ShakeArgs A)
                 scr_load(A, G, slot, false);
                 const double tolvel = A.P.tolShake * A.P.rTimeStep;
                 for (int k = at; k < C; k++) shake_v_sweep(A, G, false, tolvel, true);
+                if (!(shake_v_sweep(A, G, false, tolvel, false) < tolvel))
+                    atomicOr(&A.ctrl[CTRL_SHAKE_ORDER], 1);
             }
         }
         for (int k = 0; k < G.na; k++) {

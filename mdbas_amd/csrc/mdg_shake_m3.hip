@@ -512,6 +512,12 @@ __global__ void __launch_bounds__(256) k_m3_vv1_b(ShakeArgs A)
                 m_check(B, M, dtx, dty, dtz, rt2);
                 m_apply_r(A, M, dtx, dty, dtz, rt2, vir);
             }
+            {
+                // the state at C passes the check, unless a molecule
+                // re-diverged after it was taken as settled
+                double dtx[3], dty[3], dtz[3], rt2[3];
+                if (!m_check(B, M, dtx, dty, dtz, rt2)) atomicOr(&A.ctrl[CTRL_SHAKE_ORDER], 1);
+            }
             const double vm = m_sum3(vir);
             if (BATCH) A.vir[g] = vm;   // per molecule, summed per replica
             else vblk = vm;
@@ -670,6 +676,11 @@ __global__ void __launch_bounds__(256) k_m3_vv2_b(ShakeArgs A)
                 double xt[3], yt[3], zt[3];
                 m_lambda_v(M, 0., xt, yt, zt);
                 m_apply_v(M, xt, yt, zt);
+            }
+            {
+                double xt[3], yt[3], zt[3];
+                if (!m_lambda_v(M, A.P.tolShake * A.P.rTimeStep, xt, yt, zt))
+                    atomicOr(&A.ctrl[CTRL_SHAKE_ORDER], 1);
             }
         }
         double ke[3];

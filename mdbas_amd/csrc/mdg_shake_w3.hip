@@ -522,6 +522,13 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
                 q_check(B, Q, dtx, dty, dtz, rt2);
                 q_apply_r(A, Q, dtx, dty, dtz, rt2, vir);
             }
+            {
+                // the state at C passes the check, unless a molecule
+                // re-diverged after it was taken as settled
+                double dtx, dty, dtz, rt2;
+                if (!q_check(B, Q, dtx, dty, dtz, rt2) && k == 0)
+                    atomicOr(&A.ctrl[CTRL_SHAKE_ORDER], 1);
+            }
             const double vm = q_sum3(0., vir);
             if (BATCH) {
                 if (k == 0) A.vir[g] = vm;   // per molecule, summed per replica
@@ -690,6 +697,11 @@ __global__ void __launch_bounds__(256) k_w3_vv2_b(ShakeArgs A)
                 double xt, yt, zt;
                 q_lambda_v(Q, 0., xt, yt, zt);
                 q_apply_v(Q, xt, yt, zt);
+            }
+            {
+                double xt, yt, zt;
+                if (!q_lambda_v(Q, A.P.tolShake * A.P.rTimeStep, xt, yt, zt) && k == 0)
+                    atomicOr(&A.ctrl[CTRL_SHAKE_ORDER], 1);
             }
         }
         double ke = 0.;
