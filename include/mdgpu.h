@@ -289,7 +289,8 @@ int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2]);
 /* The launch shapes in use, as the autotuner sets them: [0] SHAKE spread,
  * [1] pair grid cap, [2] 1 when steps without a rebuild enqueue the SPME
  * branch before the pair branch, [3] 1 when the binned SPME kernels raise
- * their wave priority above the pair kernel's beside them.  mdg_set_launch
+ * their wave priority above the pair kernel's beside them, 2 when the
+ * mixed pair kernel raises its own instead.  mdg_set_launch
  * restores a saved set (per-atom results do not depend on them).  No
  * reference counterpart. */
 int mdg_get_launch(const mdg_ctx *ctx, int out[4]);

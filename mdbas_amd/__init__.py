@@ -391,7 +391,8 @@ class Context:
                    spme_prio: int = 0):
         """Launch shapes: SHAKE spread (0: a molecule per lane; 1, 2, 4, 8, 16: a
         molecule per quad of lanes on every spread-th quad), the pair kernel's
-        grid cap, SPME branch first at the fork (0/1), SPME wave priority (0/1)."""
+        grid cap, SPME branch first at the fork (0/1), wave priority (0: none, 1: the
+        binned SPME kernels raise theirs, 2: the mixed pair kernel raises its own)."""
         v = (C.c_int * 4)(shake_spread, pair_blocks, spme_first, spme_prio)
         self._chk(self.L.mdg_set_launch(self.h, v), "mdg_set_launch")
 

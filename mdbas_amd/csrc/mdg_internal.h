@@ -44,7 +44,8 @@ struct MdgParams {
     int m1, m2, m3, nbsp;
     int elecType;
     int vdwOn;                   // vdwType == VSWITCH (vdw full / none evaluate nothing)
-    int spmePrio;                // binned SPME kernels raise their wave priority (s_setprio)
+    int spmePrio;                // 1: the binned SPME kernels raise their wave priority
+                                 // (s_setprio); 2: the cluster pair kernel does
 };
 
 // Reference-format list on the device.
@@ -147,8 +148,8 @@ struct MdgTune {
     int spmeFirst = 0;           // steps without a rebuild enqueue the SPME branch before the
                                  // pair branch (the first branch after a fork starts first;
                                  // rebuild steps always do)
-    int spmePrio = 0;            // the binned SPME kernels' waves issue ahead of the pair
-                                 // kernel's when both share a SIMD
+    int spmePrio = 0;            // 1: the binned SPME kernels' waves issue ahead of the pair
+                                 // kernel's when both share a SIMD; 2: the pair kernel's ahead
 };
 
 struct mdg_ctx {

@@ -534,6 +534,8 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
 template <bool BATCH>
 __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
 {
+    // ahead of the SPME kernels beside it when the launch shape asks for it
+    if (A.P.spmePrio == 2) __builtin_amdgcn_s_setprio(2);
     const int lane = threadIdx.x & 63;
     const int a = lane & 7, b = lane >> 3;
     const long nsb = (long)A.nclu * MDG_CL;
@@ -742,6 +744,7 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     ClusterList &C = ctx->cl;
     ClusterArgs A;
     A.P = ctx->P;
+    A.P.spmePrio = ctx->tune.spmePrio;
     A.B = ctx->B;
     A.F.alpha = (float)ctx->P.alpha;
     A.F.c2a = (float)(2.0 * ctx->P.alpha / SQRTPI);

@@ -619,9 +619,9 @@ extern "C" int mdg_set_launch(mdg_ctx *ctx, const int in[4])
 {
     if (!ctx || !in) return MDG_EARG;
     if (!(in[0] == 0 || in[0] == 1 || in[0] == 2 || in[0] == 4 || in[0] == 8 || in[0] == 16) ||
-        in[1] < 1 || (in[2] != 0 && in[2] != 1) || (in[3] != 0 && in[3] != 1)) {
+        in[1] < 1 || (in[2] != 0 && in[2] != 1) || in[3] < 0 || in[3] > 2) {
         ctx->err = "launch shapes: SHAKE spread 0, 1, 2, 4, 8 or 16, pair grid >= 1, SPME-first "
-                   "and SPME priority 0 or 1";
+                   "0 or 1, priority 0, 1 (SPME kernels) or 2 (pair kernel)";
         return MDG_EARG;
     }
     ctx->tune.shkSpread = in[0];
@@ -1004,7 +1004,7 @@ extern "C" int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2])
     if (ctx->shakeW3) sweep(&ctx->tune.shkSpread, {0, 1, 2, 4, 8});
     sweep(&ctx->tune.pairBlocks, {512, 768, 1024, 2048, 4096, 8192, 16384});
     sweep(&ctx->tune.spmeFirst, {0, 1});
-    sweep(&ctx->tune.spmePrio, {0, 1});
+    sweep(&ctx->tune.spmePrio, {0, 1, 2});
     if (ctx->pruneSkin > 0. && st == MDG_OK) {
         // the pruned list, or the full list every step (no prune kernel)
         const double skin = ctx->pruneSkin;

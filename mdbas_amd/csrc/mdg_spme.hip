@@ -466,7 +466,7 @@ k_spme_bin(MdgParams Pm, MdgBox B, int n, int nby, int nbz, const double *__rest
            int *__restrict__ bucket, int *__restrict__ binOut, int *binNOut,
            int4 *__restrict__ splI, float4 *__restrict__ splB, float4 *__restrict__ splD)
 {
-    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
+    if (Pm.spmePrio == 1) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int NQ = SplineAtom<P>::NQ;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
@@ -528,7 +528,7 @@ k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
                     const int *__restrict__ binNOut, const int4 *__restrict__ splI,
                     const float4 *__restrict__ splB, float *__restrict__ grid32)
 {
-    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
+    if (Pm.spmePrio == 1) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
     constexpr int NB3 = NBD * NBD * NBD;
     static_assert(NB3 <= 128, "bin list of a tile exceeds the scan width");
@@ -691,7 +691,7 @@ k_spme_gather_pre(MdgParams Pm, MdgBox B, int n, const double *__restrict__ q,
                   double *__restrict__ part, int nbins, int *__restrict__ binCnt,
                   int *__restrict__ binNOut)
 {
-    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
+    if (Pm.spmePrio == 1) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     // the spread tiles have read the buckets: clear them for the next step
     for (long b = t; b < nbins; b += (long)gridDim.x * blockDim.x) binCnt[b] = 0;
@@ -1354,7 +1354,7 @@ k_b_bin(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__restrict__ 
         int *binCnt, int *__restrict__ bucket, int *__restrict__ binOut, int *binNOut,
         int4 *__restrict__ splI, float4 *__restrict__ splB, float4 *__restrict__ splD)
 {
-    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
+    if (Pm.spmePrio == 1) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int NQ = SplineAtom<P>::NQ;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
@@ -1403,7 +1403,7 @@ k_b_spread_tiles(MdgParams Pm, const int *__restrict__ mesh, const int *__restri
                  const int4 *__restrict__ splI, const float4 *__restrict__ splB,
                  float *__restrict__ grid32)
 {
-    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
+    if (Pm.spmePrio == 1) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
     constexpr int NB3 = NBD * NBD * NBD;
     static_assert(NB3 <= 128, "bin list of a tile exceeds the scan width");
@@ -1508,7 +1508,7 @@ k_b_gather_pre(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__rest
                const float4 *__restrict__ splD, const float *__restrict__ grid,
                float *__restrict__ frx, float *__restrict__ fry, float *__restrict__ frz)
 {
-    if (Pm.spmePrio) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
+    if (Pm.spmePrio == 1) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int l = (int)(t / SPME_LPA), rr = (int)(t % SPME_LPA);
     float fbx = 0.f, fby = 0.f, fbz = 0.f;
