@@ -117,25 +117,74 @@ def spme_traffic(name):
         return json.load(f)["bytes_per_step"]
 
 
+DRIFT_STEPS = 1000           # fixed drift window, independent of --steps
+CPU_STEPS = {"c1": 100, "c2": 20}   # oracle MD steps timed on one core (c1: 5 list rebuilds)
+
+
+def drift_per_water(recs, dt_ps, n_waters):
+    """Slope of a linear fit to E_tot(t), kcal/mol/ns per water."""
+    e = recs["tot"] / 418.4
+    t_ns = np.arange(len(e)) * dt_ps * 1e-3
+    return float(np.polyfit(t_ns, e, 1)[0]) / n_waters
+
+
+def oracle_md(wb, nsteps):
+    """The FP64 oracle's MD loop (oc_md_run: the first list build, the
+    step-0 energy, then nsteps steps with a list rebuild every 20) on one
+    pinned host core; returns (seconds, E_tot per step)."""
+    import oracle as O
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    o = O.OracleSystem(wb)
+    s = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    err, fo, et, ep, ek, sec = o.md_run(nsteps, *s)
+    if err:
+        raise RuntimeError(f"oracle MD run failed: {err}")
+    return sec, et
+
+
+def oracle_forces(wb):
+    import oracle as O
+    o = O.OracleSystem(wb)
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    o.build_list(x, y, z)
+    return o.energy(x, y, z)
+
+
 def single_box(args, M, configs, name):
     cfg = configs.CONFIGS[name]
     wb = configs.build(cfg)
     N, K, W = wb.n, args.steps, args.warmup
+    dt = cfg["timeStep"]
     log(f"workload {cfg['name']}: {N} atoms, K={K}, W={W}")
     sync = sync_fn()
 
-    # accuracy at the initial coordinates: mixed vs FP64 forces
-    forces = {}
+    # accuracy at the initial coordinates: mixed vs the device FP64 path,
+    # and both against the FP64 oracle
+    forces, pots = {}, {}
     for prec in (M.MDG_FP64, M.MDG_MIXED):
         c = M.Context(wb, precision=prec)
         c.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
-        c.initial_energy()
+        pots[prec] = c.initial_energy().pot
         forces[prec] = np.stack(c.download()[6:9])
         c.close()
     f64, fmx = forces[M.MDG_FP64], forces[M.MDG_MIXED]
     eps_rms = rel(fmx, f64)
     eps_max = float((np.linalg.norm(fmx - f64, axis=0) / np.linalg.norm(f64, axis=0)).max())
     log(f"force error mixed vs FP64: rms {eps_rms:.3e} max {eps_max:.3e}")
+    acc_oracle = None
+    if not args.no_cpu:
+        fo, eo = oracle_forces(wb)
+        acc_oracle = {
+            "force_rms_rel_err_mixed_vs_oracle": rel(fmx, fo),
+            "force_max_rel_err_mixed_vs_oracle": float(
+                (np.linalg.norm(fmx - fo, axis=0) / np.linalg.norm(fo, axis=0)).max()),
+            "force_rms_rel_err_fp64_vs_oracle": rel(f64, fo),
+            "pot_rel_err_mixed_vs_oracle": abs(pots[M.MDG_MIXED] / eo["pot"] - 1.0),
+            "pot_rel_err_fp64_vs_oracle": abs(pots[M.MDG_FP64] / eo["pot"] - 1.0)}
+        log(f"against the oracle: {acc_oracle}")
 
     ctx = M.Context(wb, precision=M.MDG_MIXED)
     if args.prune_skin is not None:
@@ -156,7 +205,7 @@ def single_box(args, M, configs, name):
     wall = time.perf_counter() - t0
     R = M.records_to_dict(recs)
     ms_step = wall / K * 1e3
-    ns_day = cfg["timeStep"] * 1e-3 * K * 86400.0 / wall
+    ns_day = dt * 1e-3 * K * 86400.0 / wall
     pin = float(R["nInCut"].mean())
     pis = float(R["nInCut"].sum()) / wall
     stats = ctx.list_stats()
@@ -182,30 +231,62 @@ def single_box(args, M, configs, name):
     G = cfg["grid"] ** 3
     spme_bytes = 56 * N + 32 * G
 
-    # energy drift of the mixed run against an FP64 run of the same steps
-    e_tot = R["tot"]
-    t_ns = np.arange(K) * cfg["timeStep"] * 1e-3
-    drift_mixed = float(np.polyfit(t_ns, e_tot / 418.4, 1)[0]) / wb.n_waters
+    # energy drift over a fixed window of DRIFT_STEPS steps after the
+    # warm-up, mixed and FP64 from the same state; the FP64 run is also
+    # timed (ns/day of the plain FP64 method on the GPU)
+    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    ctx.initial_energy()
+    ctx.run(W, use_graph=True)
+    rd = M.records_to_dict(ctx.run(DRIFT_STEPS, use_graph=True))
+    drift_mixed = drift_per_water(rd, dt, wb.n_waters)
     c64 = M.Context(wb, precision=M.MDG_FP64)
     c64.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
     c64.initial_energy()
-    if W:
-        c64.run(W, use_graph=True)
-    r64 = M.records_to_dict(c64.run(K, use_graph=True))
+    c64.run(W, use_graph=True)
+    sync()
+    t0 = time.perf_counter()
+    r64 = M.records_to_dict(c64.run(DRIFT_STEPS, use_graph=True))
+    sync()
+    wall64 = time.perf_counter() - t0
+    drift_64 = drift_per_water(r64, dt, wb.n_waters)
+    fp64_timing = {"ns_per_day": dt * 1e-3 * DRIFT_STEPS * 86400.0 / wall64,
+                   "ms_per_step": wall64 / DRIFT_STEPS * 1e3, "steps": DRIFT_STEPS,
+                   "launch": "default shapes (not autotuned)"}
+    log(f"FP64 on the GPU: {fp64_timing}; drift mixed {drift_mixed:.3f} fp64 {drift_64:.3f}")
+
+    # both device paths from the initial state for the CPU leg's steps, to
+    # set beside the oracle's trajectory
+    ncpu = CPU_STEPS[name]
+    dev_tot = {}
+    for prec, c in ((M.MDG_MIXED, ctx), (M.MDG_FP64, c64)):
+        c.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        r0 = c.initial_energy()
+        rr = M.records_to_dict(c.run(ncpu, use_graph=True))
+        dev_tot[prec] = np.concatenate([[r0.tot], rr["tot"]])
     c64.close()
-    drift_64 = float(np.polyfit(t_ns, r64["tot"] / 418.4, 1)[0]) / wb.n_waters
     ctx.close()
 
     cpu = None
+    traj_vs_oracle = None
     if not args.no_cpu:
-        log("CPU baseline (FP64 oracle, 1 core)...")
-        t_step, steps, descr = cpu_baseline([wb], max_steps=10 if name == "c1" else 2)
-        cpu = dict(value=cfg["timeStep"] * 1e-3 * 86400.0 / t_step, unit="ns/day", cores=1,
-                   kind="port", pair_interactions_per_s=pin / t_step, s_per_step=t_step,
-                   sample=(f"FP64 oracle on 1 host core ({os.uname().machine}, nproc "
-                           f"{os.cpu_count()}): 1 link-cell list build ({descr[0][0]:.2f} s) + "
-                           f"{steps[0]} MD steps of the same {N}-atom box ({descr[0][1]:.2f} "
-                           f"s/step), per step = step + build/20"))
+        log(f"CPU baseline (FP64 oracle, 1 core, {ncpu} steps)...")
+        sec, et = oracle_md(wb, ncpu)
+        scale = abs(et[0])
+        traj_vs_oracle = {
+            "steps": ncpu,
+            "max_rel_dEtot_fp64": float(np.abs(dev_tot[M.MDG_FP64] - et).max() / scale),
+            "max_rel_dEtot_mixed": float(np.abs(dev_tot[M.MDG_MIXED] - et).max() / scale),
+            "oracle_drift_kcal_mol_ns_per_water": drift_per_water({"tot": et[1:]}, dt,
+                                                                  wb.n_waters)}
+        log(f"trajectory vs oracle: {traj_vs_oracle}")
+        nreb = ncpu // cfg["update"]
+        cpu = dict(value=dt * 1e-3 * ncpu * 86400.0 / sec, unit="ns/day", cores=1,
+                   kind="port", pair_interactions_per_s=pin * ncpu / sec,
+                   s_per_step=sec / ncpu,
+                   sample=(f"FP64 oracle MD loop on 1 pinned host core ({os.uname().machine}, "
+                           f"nproc {os.cpu_count()}): {ncpu} steps of the same {N}-atom box "
+                           f"with {nreb} list rebuilds, plus the first list build and the step-0 "
+                           f"energy, all inside the timed {sec:.1f} s"))
         log(f"cpu: {cpu}")
 
     return {
@@ -221,7 +302,7 @@ def single_box(args, M, configs, name):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": WORKLOADS[name], "atoms": N, "dt_fs": cfg["timeStep"] * 1e3,
+        "config": {"workload": WORKLOADS[name], "atoms": N, "dt_fs": dt * 1e3,
                    "precision": "mixed"},
         "pair_interactions_per_s": pis,
         "pair_interactions_per_step": pin,
@@ -229,10 +310,13 @@ def single_box(args, M, configs, name):
         "cluster_pairs": int(stats["cluster_pairs"]),
         "pruned_list": prune,
         "pair_interactions_per_s_pair_kernel": pin2 / (ph["pair_ms"] * 1e-3),
+        "fp64_gpu": fp64_timing,
         "accuracy": {"force_rms_rel_err_vs_fp64": eps_rms, "force_max_rel_err_vs_fp64": eps_max,
+                     "vs_oracle": acc_oracle,
                      "energy_drift_kcal_mol_ns_per_water_mixed": drift_mixed,
                      "energy_drift_kcal_mol_ns_per_water_fp64": drift_64,
-                     "drift_window_steps": K},
+                     "drift_window_steps": DRIFT_STEPS,
+                     "trajectory_vs_oracle": traj_vs_oracle},
         "phases_ms": ph,
         "tuning": tuning,
         "roofline": {"kernel": "k_pair_mixed", "bound": "valu",

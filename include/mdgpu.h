@@ -121,6 +121,8 @@ typedef struct {
     double eDir, eRec, eCorr;
     double virelec, virvdw, virshake;
     double nInCut;                     /* listed pairs with r <= rc this step */
+    double ebond, eub, eang, edihe, eimpr;   /* bonded terms (ENERGY bond, ub, ang, dihe, impr);
+                                              * elec / vdw include the 1-4 pairs */
 } MDG_STEP;
 
 /* ---- context ---------------------------------------------------------- */
@@ -147,6 +149,23 @@ int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps[], const d
                      const double mass[], const double rmass[], const int frozen[],
                      const int nAtConst[], const MDG_CONSTRAINT constList[], int nConst);
 
+/* Bonded topology for the device-resident step (mdg_run_steps): the term
+ * tables of energy()'s bonded calls (param->nBond / nUb / nAngle /
+ * nDihedral / nImproper entries of bond / ub / angle / dihe / impr,
+ * internal.c:21-604) and, under ctrl->keyNb14, the 1-4 pairs with eps14 /
+ * sig14 and param->scal14 (ewald14_energy, energy.c:654-742).  Rebuilds
+ * the exclusion table from every term and the constraints of
+ * mdg_set_topology, as makelist does on its first call (list.c:108-894 /
+ * 1125-1791), and derives the 1-4 list from it.  Each device step then
+ * adds these forces (FP64 terms, int64 fixed-point accumulation) and the
+ * step records carry the bonded energies.  Call after mdg_set_topology.
+ * No single reference counterpart: the reference hands the same tables to
+ * energy() on every step (energy.h:30-37). */
+int mdg_set_bonded(mdg_ctx *ctx, const MDG_CTRL *ctrl, const MDG_PARAM *param,
+                   const MDG_BOND bond[], const MDG_BOND ub[], const MDG_ANGLE angle[],
+                   const MDG_DIHE dihe[], const MDG_DIHE impr[], const double eps14[],
+                   const double sig14[]);
+
 /* ---- drop-in mirrors (host arrays in, host arrays out) ------------------ */
 
 /* makelist, list.c:29-106 / list.h:28-31.  First call (ctrl->newjob == 1):
@@ -154,8 +173,9 @@ int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps[], const d
  * in the reference format through malloc'd out-parameters; the link-cell
  * build wraps x,y,z in place as list.c:1927 does.  Later calls rebuild when
  * param->step % neigh->update == 0 (realloc'ing *neighList if it grows).
- * Only bonds, constraints and angles feed the exclusions; dihedrals or
- * impropers give MDG_EUNSUPPORTED. */
+ * Bonds, constraints (under keyConstH), angles, dihedrals and impropers
+ * feed the exclusions (list.c:108-894 / 1125-1791); *neighList14 receives
+ * the neigh->nPair14 1-4 pairs as (i, j) couples (list.c:864-873). */
 int mdg_makelist(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_PBC *box, MDG_NEIGH *neigh,
                  MDG_CONSTRAINT constList[], MDG_BOND bond[], MDG_ANGLE angle[],
                  MDG_DIHE dihe[], MDG_DIHE impr[], double x[], double y[], double z[],
@@ -185,23 +205,37 @@ int mdg_spme_energy(mdg_ctx *ctx, MDG_PARAM *param, MDG_EWALD *ewald, MDG_PBC *b
 
 /* nonbond_energy, energy.c:216-345 / energy.h:39-42: cutoff Coulomb with
  * the SHIFT1 / SHIFT2 / SWITCH functor of ctrl->elecType (elec.c:137-263)
- * plus switched LJ over the given list; accumulates into ener and f. */
+ * plus switched LJ over the given list; accumulates into ener and f.
+ * Deviation from the reference's parameter list: a leading ctrl.  The
+ * reference picks the functor through the global pointers ptr_coulomb /
+ * ptr_vdw that init_energy_ptrs sets from CTRL (energy.c:35-93); a
+ * re-entrant library has no such globals, so the caller's CTRL is passed
+ * instead. */
 int mdg_nonbond_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
                        MDG_PBC *box, const double x[], const double y[], const double z[],
                        double fx[], double fy[], double fz[], const double q[],
                        const double eps[], const double sig[], const int neighList[],
                        const int neighPair[], const int neighOrder[]);
 
-/* energy, energy.c:103-205 / energy.h:30-37, for the nonbonded-only
- * systems this path covers: zeroes energies, stress and forces, calls the
- * Ewald or plain nonbonded path, sums pot / virpot, zeroes frozen atoms. */
+/* energy, energy.c:103-205 / energy.h:30-37, the reference's full
+ * parameter list: zeroes the energy and virial fields, the stress and the
+ * forces; runs the Ewald path (ewald_energy, keyEwald != 0) or the plain
+ * cutoff path (nonbond_energy), each with its 1-4 pairs when keyNb14 is
+ * set (ewald14_energy / nonbond14_energy over neighList14, neigh->nPair14
+ * pairs, with eps14 / sig14 and param->scal14); then bond_energy,
+ * angle_energy, ub_energy, dihedral_energy and improper_energy
+ * (internal.c:21-604) for param->nBond / nAngle / nUb / nDihedral /
+ * nImproper terms; sums pot and virpot; zeroes frozen atoms' forces and
+ * velocities.  Every term runs on the GPU in FP64. */
 int mdg_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
-               MDG_EWALD *ewald, MDG_PBC *box, MDG_NEIGH *neigh, const double x[],
-               const double y[], const double z[], double vx[], double vy[], double vz[],
-               double fx[], double fy[], double fz[], const double q[], const double eps[],
-               const double sig[], const int frozen[], const int neighList[],
-               const int neighPair[], const int neighOrder[], int **exclList,
-               const int exclPair[]);
+               MDG_EWALD *ewald, MDG_PBC *box, MDG_NEIGH *neigh, MDG_BOND bond[],
+               MDG_BOND ub[], MDG_ANGLE angle[], MDG_DIHE dihe[], MDG_DIHE impr[],
+               const double x[], const double y[], const double z[], double vx[], double vy[],
+               double vz[], double fx[], double fy[], double fz[], const double q[],
+               const double eps[], const double sig[], const double eps14[],
+               const double sig14[], const int frozen[], const int neighList[],
+               const int neighPair[], const int neighOrder[], const int neighList14[],
+               int **exclList, const int exclPair[]);
 
 /* vv_integrate, integrate.c:1479-1634 / integrate.h:72-76, NVE: stage 1
  * = half kick, drift, SHAKE-R; stage 2 = half kick, SHAKE-V, kinetic

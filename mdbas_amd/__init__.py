@@ -77,7 +77,8 @@ class MDG_EWALD(C.Structure):
 
 class MDG_STEP(C.Structure):
     _fields_ = [("step", C.c_int), ("listRebuilt", C.c_int)] + [(k, C.c_double) for k in (
-        "pot kin tot elec vdw eDir eRec eCorr virelec virvdw virshake nInCut").split()]
+        "pot kin tot elec vdw eDir eRec eCorr virelec virvdw virshake nInCut "
+        "ebond eub eang edihe eimpr").split()]
 
 
 # Every entry point include/mdgpu.h declares (checked by the ABI test).
@@ -87,7 +88,7 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
            "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj "
            "mdg_batch_create mdg_batch_size mdg_autotune mdg_set_prune mdg_prune_stats "
-           "mdg_prune_skin mdg_get_launch mdg_set_launch").split()
+           "mdg_prune_skin mdg_get_launch mdg_set_launch mdg_set_bonded").split()
 
 _LIB = None
 
@@ -187,6 +188,11 @@ def reference_structs(wb, cfg: dict):
     ng = MDG_NEIGH()
     ng.update = cfg["update"]
     ng.linkRatio = cfg["linkRatio"]
+    if hasattr(wb, "bonds"):          # a molecule box with bonded terms (molecules.py)
+        ctrl.keyNb14 = 1
+        p.nBond, p.nUb, p.nAngle = len(wb.bonds), len(wb.ubs), len(wb.angles)
+        p.nDihedral, p.nImproper = len(wb.dihes), len(wb.imprs)
+        p.scal14 = wb.scal14
     return ctrl, p, b, e, ng
 
 
@@ -215,6 +221,10 @@ class Context:
                                           len(wb.ca)), "mdg_set_topology")
         self.n = n
         self._lists = None
+        if hasattr(wb, "bonds"):
+            t = self._terms()
+            self._chk(self.L.mdg_set_bonded(self.h, C.byref(self.ctrl), C.byref(self.param), *t,
+                                            P(wb.eps14), P(wb.sig14)), "mdg_set_bonded")
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -247,9 +257,10 @@ class Context:
                                el=C.POINTER(C.POINTER(C.c_int))(), ep=C.POINTER(C.c_int)())
         d = self._lists
         self.param.step = step
+        t = self._terms()
         self._chk(self.L.mdg_makelist(self.h, C.byref(self.ctrl), C.byref(self.param),
-                                      C.byref(self.pbc), C.byref(self.neigh), self.cons, None,
-                                      None, None, None, P(x), P(y), P(z), P(self.wb.frozen),
+                                      C.byref(self.pbc), C.byref(self.neigh), self.cons, t[0],
+                                      t[2], t[3], t[4], P(x), P(y), P(z), P(self.wb.frozen),
                                       C.byref(d["nl"]), C.byref(d["np"]), C.byref(d["no"]),
                                       C.byref(d["n14"]), C.byref(d["el"]), C.byref(d["ep"])),
                   "mdg_makelist")
@@ -267,7 +278,51 @@ class Context:
             if i < rows and ep[i]:
                 lst.extend(d["el"][i][k] for k in range(ep[i]))
         self._excl = (d["el"], d["ep"])
+        n14 = self.neigh.nPair14
+        self.list14 = np.ctypeslib.as_array(d["n14"], shape=(max(1, 2 * n14),))[:2 * n14].copy()
         return nl, np_, no, off, np.array(lst, np.int32)
+
+    def _terms(self):
+        """The bonded term tables (bond, ub, angle, dihe, impr) as ctypes
+        arrays, or Nones for a water box."""
+        if not hasattr(self.wb, "bonds"):
+            return (None,) * 5
+        if not hasattr(self, "_term_arrays"):
+            wb = self.wb
+            self._term_arrays = (wb.c_bonds(), wb.c_ubs(), wb.c_angles(), wb.c_dihes(),
+                                 wb.c_imprs())
+        return self._term_arrays
+
+    def energy(self, x, y, z, nl, np_, no, elec_type=None, keyEwald=None):
+        """The drop-in energy() (mdg_energy, energy.c:103-205) over a
+        reference-format list: zeroed forces, Ewald (or plain cutoff) pairs
+        with their 1-4 pairs, every bonded term.  Returns forces (3,N) and
+        the ENERGY struct."""
+        n = self.n
+        f = np.zeros((3, n))
+        v = np.zeros((3, n))
+        e = MDG_ENERGY()
+        el, ep = self._excl
+        t = self._terms()
+        wb = self.wb
+        saved = (self.ctrl.elecType, self.ctrl.keyEwald)
+        if elec_type is not None:
+            self.ctrl.elecType = elec_type
+        if keyEwald is not None:
+            self.ctrl.keyEwald = keyEwald
+        eps14 = getattr(wb, "eps14", None)
+        sig14 = getattr(wb, "sig14", None)
+        l14 = getattr(self, "list14", np.zeros(0, np.int32))
+        try:
+            self._chk(self.L.mdg_energy(self.h, C.byref(self.ctrl), C.byref(self.param), C.byref(e),
+                                        C.byref(self.ewald), C.byref(self.pbc), C.byref(self.neigh),
+                                        *t, P(x), P(y), P(z), P(v[0]), P(v[1]), P(v[2]), P(f[0]),
+                                        P(f[1]), P(f[2]), P(wb.q), P(wb.eps), P(wb.sig),
+                                        P(eps14), P(sig14), P(wb.frozen), P(nl), P(np_), P(no),
+                                        P(l14) if len(l14) else None, el, ep), "mdg_energy")
+        finally:
+            self.ctrl.elecType, self.ctrl.keyEwald = saved
+        return f, e
 
     def ewald_energy(self, x, y, z, nl, np_, no, f=None):
         """mdg_ewald_energy over a reference-format list; returns forces (3,N)

@@ -139,22 +139,30 @@ int mdg_alloc_state(mdg_ctx *ctx)
 
 // list.c:108-894 (ordered, pair kept at the smaller index) and
 // list.c:1125-1791 (symmetric).  Bonds and constraints are appended as
-// given; angle pairs and the 1-3 connectivity walk only when absent.  A
-// 1-4 path is DIHE_NONPARAM_ERROR (110) since dihedrals are not handled.
+// given and recorded as connections; angle, dihedral and improper pairs
+// only when absent, a new (a, d) of a dihedral also becoming a 1-4 pair;
+// then the connectivity walk a-b-c(-d): (a, c) when absent, and for every
+// d the reference's capacity check (a 1-4 path beyond 5 per dihedral is
+// DIHE_NONPARAM_ERROR, 110) before (a, d) is added as exclusion and 1-4
+// pair when absent.
 int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nConst,
-                         const MDG_CONSTRAINT *cons, int nAngle, const MDG_ANGLE *angle)
+                         const MDG_CONSTRAINT *cons, int nAngle, const MDG_ANGLE *angle,
+                         int nDihe, const MDG_DIHE *dihe, int nImpr, const MDG_DIHE *impr,
+                         std::vector<int> *list14)
 {
     const int n = ctx->n;
     const bool sym = ctx->keyLink != 0;
     std::vector<std::vector<int>> ex(n), con(n);
-    auto add = [&](int i, int j, bool dedupe) {
+    std::vector<int> l14;
+    auto add = [&](int i, int j, bool dedupe) -> bool {
         int ii = i, jj = j;
         if (!sym && jj < ii) std::swap(ii, jj);
         if (dedupe)
             for (int v : ex[ii])
-                if (v == jj) return;
+                if (v == jj) return false;
         ex[ii].push_back(jj);
         if (sym) ex[jj].push_back(ii);
+        return true;
     };
     for (int i = 0; i < nBond; i++) {
         add(bond[i].a, bond[i].b, false);
@@ -171,6 +179,27 @@ int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nCon
         add(angle[i].a, angle[i].c, true);
         add(angle[i].b, angle[i].c, true);
     }
+    for (int i = 0; i < nDihe; i++) {
+        const MDG_DIHE &d = dihe[i];
+        add(d.a, d.b, true);
+        add(d.a, d.c, true);
+        if (add(d.a, d.d, true)) {
+            l14.push_back(d.a);
+            l14.push_back(d.d);
+        }
+        add(d.b, d.c, true);
+        add(d.b, d.d, true);
+        add(d.c, d.d, true);
+    }
+    for (int i = 0; i < nImpr; i++) {
+        const MDG_DIHE &d = impr[i];
+        add(d.a, d.b, true);
+        add(d.a, d.c, true);
+        add(d.a, d.d, true);
+        add(d.b, d.c, true);
+        add(d.b, d.d, true);
+        add(d.c, d.d, true);
+    }
     for (int ia = 0; ia < n; ia++)
         for (int ib : con[ia])
             for (int ic : con[ib]) {
@@ -178,8 +207,14 @@ int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nCon
                 add(ia, ic, true);
                 for (int id : con[ic]) {
                     if (id == ia || id == ib) continue;
-                    ctx->err = "1-4 pair without dihedral parameters";
-                    return 110;
+                    if ((long)l14.size() / 2 >= 5L * nDihe) {
+                        ctx->err = "1-4 pair without dihedral parameters";
+                        return 110;
+                    }
+                    if (add(ia, id, true)) {
+                        l14.push_back(ia);
+                        l14.push_back(id);
+                    }
                 }
             }
     if (!sym && n > 0 && !ex[n - 1].empty()) return 111;
@@ -191,6 +226,7 @@ int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nCon
     }
     ctx->hExOff[n] = (int)ctx->hExLst.size();
     ctx->exclBuilt = true;
+    if (list14) *list14 = l14;
     return mdg_upload_exclusions(ctx);
 }
 
@@ -454,6 +490,7 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
     for (int k = 0; k < 2; k++)
         if (ctx->graphExec[k]) hipGraphExecDestroy(ctx->graphExec[k]);
     mdg_spme_free(ctx);
+    mdg_bonded_free(ctx);
     void *ptrs[] = {ctx->x, ctx->y, ctx->z, ctx->vx, ctx->vy, ctx->vz, ctx->fx, ctx->fy, ctx->fz,
                     ctx->q, ctx->eps, ctx->sig, ctx->mass, ctx->rmass, ctx->frozen, ctx->nAtConst,
                     ctx->ca, ctx->cb, ctx->rc2, ctx->dd, ctx->grpAtomOff, ctx->grpAtoms,
@@ -642,6 +679,7 @@ extern "C" int mdg_set_topology(mdg_ctx *ctx, const double q[], const double eps
         hipFree(ctx->shkThr);
     }
     ctx->nConst = nConst;
+    ctx->hCons.assign(constList, constList + nConst);
     std::vector<int> a(std::max(nConst, 1)), b(std::max(nConst, 1));
     std::vector<double> r2(std::max(nConst, 1));
     for (int c = 0; c < nConst; c++) {
@@ -723,22 +761,23 @@ extern "C" int mdg_makelist(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_
                             int **neighPair, int **neighOrder, int **neighList14,
                             int ***exclList, int **exclPair)
 {
-    (void)box; (void)dihe; (void)impr;
+    (void)box;
     if (!ctx || !ctrl || !param || !neigh || !x || !y || !z) return MDG_EARG;
     const int n = ctx->n;
     if (param->nAtom != n) return MDG_EARG;
+    if ((param->nBond > 0 && !bond) || (param->nAngle > 0 && !angle) ||
+        (param->nDihedral > 0 && !dihe) || (param->nImproper > 0 && !impr))
+        return MDG_EARG;
     if (ctrl->newjob == 1) {
-        if (param->nDihedral > 0 || param->nImproper > 0) {
-            ctx->err = "dihedral and improper exclusions are not implemented";
-            return MDG_EUNSUPPORTED;
-        }
         neigh->linkRatio = std::min(std::max(neigh->linkRatio, 1), 5);
         ctrl->keyLink = ctx->keyLink;
         if (frozen) {
             RETCHK(h2d(ctx, ctx->frozen, (const int *)frozen, n));
         }
+        std::vector<int> l14;
         RETCHK(mdg_build_exclusions(ctx, param->nBond, bond, ctrl->keyConstH ? param->nConst : 0,
-                                    constList, param->nAngle, angle));
+                                    constList, param->nAngle, angle, param->nDihedral, dihe,
+                                    param->nImproper, impr, &l14));
         // exclusion table in the reference layout (N-1 rows ordered, N rows symmetric)
         const int rows = ctx->keyLink ? n : n - 1;
         *exclPair = (int *)malloc(n * sizeof(int));
@@ -749,8 +788,12 @@ extern "C" int mdg_makelist(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_
             for (int k = 0; k < (*exclPair)[i]; k++)
                 (*exclList)[i][k] = ctx->hExLst[ctx->hExOff[i] + k];
         }
-        neigh->nPair14 = 0;
-        if (neighList14) *neighList14 = (int *)malloc(sizeof(int));
+        // list.c:864-873: the 1-4 pairs as (i, j) couples
+        neigh->nPair14 = (int)(l14.size() / 2);
+        if (neighList14) {
+            *neighList14 = (int *)malloc(std::max<size_t>(l14.size(), 1) * sizeof(int));
+            std::copy(l14.begin(), l14.end(), *neighList14);
+        }
         RETCHK(h2d(ctx, ctx->x, (const double *)x, n));
         RETCHK(h2d(ctx, ctx->y, (const double *)y, n));
         RETCHK(h2d(ctx, ctx->z, (const double *)z, n));
@@ -853,14 +896,16 @@ extern "C" int mdg_spme_energy(mdg_ctx *ctx, MDG_PARAM *param, MDG_EWALD *ewald,
     return MDG_OK;
 }
 
-extern "C" int mdg_ewald_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
-                                MDG_EWALD *ewald, MDG_PBC *box, const double x[], const double y[],
-                                const double z[], double fx[], double fy[], double fz[],
-                                const double q[], const double eps[], const double sig[],
-                                const int neighList[], const int neighPair[],
-                                const int neighOrder[], int **exclList, const int exclPair[])
+// energy.c:450-643 on the device: uploads positions, parameters, list and
+// exclusions and forces fx (the caller's +=), runs SPME (keyEwald 2), the
+// real-space pairs and the excluded-pair correction into ctx->fx..; the
+// forces stay on the device.  e[0..3] = elec, vdw, virelec, virvdw to add.
+static int ewald_on_device(mdg_ctx *ctx, const MDG_CTRL *ctrl, const double *x, const double *y,
+                           const double *z, const double *fx, const double *fy, const double *fz,
+                           const double *q, const double *eps, const double *sig,
+                           const int *neighList, const int *neighPair, const int *neighOrder,
+                           int **exclList, const int *exclPair, double e[4])
 {
-    if (!ctx || !ctrl || !ener || !x || !fx || !q || !neighPair || !neighOrder) return MDG_EARG;
     const int n = ctx->n;
     if (ctrl->keyEwald == 1) {
         ctx->err = "classic k-space Ewald (keyEwald 1) is not on this path";
@@ -870,9 +915,9 @@ extern "C" int mdg_ewald_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, 
     RETCHK(upload_xyzq(ctx, x, y, z, q, eps, sig));
     RETCHK(upload_ref_list(ctx, neighList, neighPair, neighOrder));
     RETCHK(exclusions_from_args(ctx, exclList, exclPair));
-    RETCHK(h2d(ctx, ctx->fx, (const double *)fx, n));
-    RETCHK(h2d(ctx, ctx->fy, (const double *)fy, n));
-    RETCHK(h2d(ctx, ctx->fz, (const double *)fz, n));
+    RETCHK(h2d(ctx, ctx->fx, fx, n));
+    RETCHK(h2d(ctx, ctx->fy, fy, n));
+    RETCHK(h2d(ctx, ctx->fz, fz, n));
     double erec[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (ctrl->keyEwald == 2) {
         if (!ctx->theta) return MDG_EARG;
@@ -884,15 +929,70 @@ extern "C" int mdg_ewald_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, 
     RETCHK(mdg_excl_corr64(ctx, ctx->fx, ctx->fy, ctx->fz, ctx->scal + S_REFCORR));
     double s[8];
     RETCHK(d2h(ctx, s, ctx->scal + S_REF, 8));
-    RETCHK(d2h(ctx, fx, ctx->fx, n));
-    RETCHK(d2h(ctx, fy, ctx->fy, n));
-    RETCHK(d2h(ctx, fz, ctx->fz, n));
     RETCHK(sync(ctx));
     // energy.c:623-627
-    ener->elec += s[0] + erec[0] + s[6];
-    ener->vdw += s[1];
-    ener->virelec += s[2] + erec[1] + s[7];
-    ener->virvdw += s[3];
+    e[0] = s[0] + erec[0] + s[6];
+    e[1] = s[1];
+    e[2] = s[2] + erec[1] + s[7];
+    e[3] = s[3];
+    return MDG_OK;
+}
+
+// energy.c:216-345 on the device (see ewald_on_device).
+static int nonbond_on_device(mdg_ctx *ctx, const MDG_CTRL *ctrl, const double *x, const double *y,
+                             const double *z, const double *fx, const double *fy,
+                             const double *fz, const double *q, const double *eps,
+                             const double *sig, const int *neighList, const int *neighPair,
+                             const int *neighOrder, double e[4])
+{
+    const int n = ctx->n;
+    // energy.c:45-65: FULL and NOELEC evaluate no electrostatics
+    int et = ctrl->elecType;
+    if (et == 0) et = 1;
+    if (et < 1 || et > 4) return MDG_EUNSUPPORTED;
+    ctx->P.vdwOn = ctrl->vdwType == 2;
+    RETCHK(upload_xyzq(ctx, x, y, z, q, eps, sig));
+    RETCHK(upload_ref_list(ctx, neighList, neighPair, neighOrder));
+    RETCHK(h2d(ctx, ctx->fx, fx, n));
+    RETCHK(h2d(ctx, ctx->fy, fy, n));
+    RETCHK(h2d(ctx, ctx->fz, fz, n));
+    RETCHK(mdg_pair_ref64(ctx, et, ctx->fx, ctx->fy, ctx->fz, ctx->rl.neighList, ctx->rl.off,
+                          ctx->rl.neighOrder, ctx->scal + S_REF));
+    double s[6];
+    RETCHK(d2h(ctx, s, ctx->scal + S_REF, 6));
+    RETCHK(sync(ctx));
+    e[0] = s[0];
+    e[1] = s[1];
+    e[2] = s[2];
+    e[3] = s[3];
+    return MDG_OK;
+}
+
+static int download_forces(mdg_ctx *ctx, double *fx, double *fy, double *fz)
+{
+    RETCHK(d2h(ctx, fx, ctx->fx, ctx->n));
+    RETCHK(d2h(ctx, fy, ctx->fy, ctx->n));
+    RETCHK(d2h(ctx, fz, ctx->fz, ctx->n));
+    return sync(ctx);
+}
+
+extern "C" int mdg_ewald_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
+                                MDG_EWALD *ewald, MDG_PBC *box, const double x[], const double y[],
+                                const double z[], double fx[], double fy[], double fz[],
+                                const double q[], const double eps[], const double sig[],
+                                const int neighList[], const int neighPair[],
+                                const int neighOrder[], int **exclList, const int exclPair[])
+{
+    (void)param; (void)ewald; (void)box;
+    if (!ctx || !ctrl || !ener || !x || !fx || !q || !neighPair || !neighOrder) return MDG_EARG;
+    double e[4];
+    RETCHK(ewald_on_device(ctx, ctrl, x, y, z, fx, fy, fz, q, eps, sig, neighList, neighPair,
+                           neighOrder, exclList, exclPair, e));
+    RETCHK(download_forces(ctx, fx, fy, fz));
+    ener->elec += e[0];
+    ener->vdw += e[1];
+    ener->virelec += e[2];
+    ener->virvdw += e[3];
     return MDG_OK;
 }
 
@@ -905,65 +1005,86 @@ extern "C" int mdg_nonbond_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param
 {
     (void)param; (void)box;
     if (!ctx || !ctrl || !ener || !x || !fx || !q || !neighPair || !neighOrder) return MDG_EARG;
-    const int n = ctx->n;
-    // energy.c:45-65: FULL and NOELEC evaluate no electrostatics
-    int et = ctrl->elecType;
-    if (et == 0) et = 1;
-    if (et < 1 || et > 4) return MDG_EUNSUPPORTED;
-    ctx->P.vdwOn = ctrl->vdwType == 2;
-    RETCHK(upload_xyzq(ctx, x, y, z, q, eps, sig));
-    RETCHK(upload_ref_list(ctx, neighList, neighPair, neighOrder));
-    RETCHK(h2d(ctx, ctx->fx, (const double *)fx, n));
-    RETCHK(h2d(ctx, ctx->fy, (const double *)fy, n));
-    RETCHK(h2d(ctx, ctx->fz, (const double *)fz, n));
-    RETCHK(mdg_pair_ref64(ctx, et, ctx->fx, ctx->fy, ctx->fz, ctx->rl.neighList, ctx->rl.off,
-                          ctx->rl.neighOrder, ctx->scal + S_REF));
-    double s[6];
-    RETCHK(d2h(ctx, s, ctx->scal + S_REF, 6));
-    RETCHK(d2h(ctx, fx, ctx->fx, n));
-    RETCHK(d2h(ctx, fy, ctx->fy, n));
-    RETCHK(d2h(ctx, fz, ctx->fz, n));
-    RETCHK(sync(ctx));
-    ener->elec += s[0];
-    ener->vdw += s[1];
-    ener->virelec += s[2];
-    ener->virvdw += s[3];
+    double e[4];
+    RETCHK(nonbond_on_device(ctx, ctrl, x, y, z, fx, fy, fz, q, eps, sig, neighList, neighPair,
+                             neighOrder, e));
+    RETCHK(download_forces(ctx, fx, fy, fz));
+    ener->elec += e[0];
+    ener->vdw += e[1];
+    ener->virelec += e[2];
+    ener->virvdw += e[3];
     return MDG_OK;
 }
 
 extern "C" int mdg_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
-                          MDG_EWALD *ewald, MDG_PBC *box, MDG_NEIGH *neigh, const double x[],
-                          const double y[], const double z[], double vx[], double vy[],
-                          double vz[], double fx[], double fy[], double fz[], const double q[],
-                          const double eps[], const double sig[], const int frozen[],
+                          MDG_EWALD *ewald, MDG_PBC *box, MDG_NEIGH *neigh, MDG_BOND bond[],
+                          MDG_BOND ub[], MDG_ANGLE angle[], MDG_DIHE dihe[], MDG_DIHE impr[],
+                          const double x[], const double y[], const double z[], double vx[],
+                          double vy[], double vz[], double fx[], double fy[], double fz[],
+                          const double q[], const double eps[], const double sig[],
+                          const double eps14[], const double sig14[], const int frozen[],
                           const int neighList[], const int neighPair[], const int neighOrder[],
-                          int **exclList, const int exclPair[])
+                          const int neighList14[], int **exclList, const int exclPair[])
 {
-    (void)neigh;
-    if (!ctx || !ctrl || !param || !ener || !box) return MDG_EARG;
-    if (param->nBond > 0 || param->nAngle > 0 || param->nUb > 0 || param->nDihedral > 0 ||
-        param->nImproper > 0 || ctrl->keyNb14) {
-        ctx->err = "bonded and 1-4 terms are not on this path";
-        return MDG_EUNSUPPORTED;
-    }
+    (void)ewald;
+    if (!ctx || !ctrl || !param || !ener || !box || !x || !fx || !q || !neighPair || !neighOrder)
+        return MDG_EARG;
+    if (param->nAtom != ctx->n) return MDG_EARG;
+    if ((param->nBond > 0 && !bond) || (param->nUb > 0 && !ub) || (param->nAngle > 0 && !angle) ||
+        (param->nDihedral > 0 && !dihe) || (param->nImproper > 0 && !impr))
+        return MDG_EARG;
+    const int n14 = ctrl->keyNb14 && neigh ? neigh->nPair14 : 0;
+    if (n14 > 0 && (!neighList14 || !eps14 || !sig14)) return MDG_EARG;
     // energy.c:115-144
     ener->elec = ener->vdw = ener->bond = ener->ub = ener->ang = ener->dihe = ener->impr = 0.;
     ener->virelec = ener->virvdw = ener->virbond = ener->virub = ener->virpot = 0.;
     box->stress1 = box->stress2 = box->stress3 = box->stress4 = box->stress5 = 0.;
     box->stress6 = box->stress7 = box->stress8 = box->stress9 = 0.;
-    for (int i = 0; i < ctx->n; i++) fx[i] = fy[i] = fz[i] = 0.;
-    int st;
+    const int n = ctx->n;
+    std::vector<double> zero(n, 0.);
+    // energy.c:151-165: Ewald or plain nonbonded terms (into the zeroed
+    // forces), each followed by its 1-4 pairs when keyNb14 is set
+    double e[4];
     if (ctrl->keyEwald != 0)
-        st = mdg_ewald_energy(ctx, ctrl, param, ener, ewald, box, x, y, z, fx, fy, fz, q, eps, sig,
-                              neighList, neighPair, neighOrder, exclList, exclPair);
+        RETCHK(ewald_on_device(ctx, ctrl, x, y, z, zero.data(), zero.data(), zero.data(), q, eps,
+                               sig, neighList, neighPair, neighOrder, exclList, exclPair, e));
     else
-        st = mdg_nonbond_energy(ctx, ctrl, param, ener, box, x, y, z, fx, fy, fz, q, eps, sig,
-                                neighList, neighPair, neighOrder);
-    if (st) return st;
+        RETCHK(nonbond_on_device(ctx, ctrl, x, y, z, zero.data(), zero.data(), zero.data(), q,
+                                 eps, sig, neighList, neighPair, neighOrder, e));
+    ener->elec += e[0];
+    ener->vdw += e[1];
+    ener->virelec += e[2];
+    ener->virvdw += e[3];
+    // energy.c:157-189: 1-4 pairs, bonds, angles, Urey-Bradley, dihedrals,
+    // impropers, all in one device kernel
+    MdgBonded &D = ctx->bd;
+    D.scal14 = param->scal14;
+    D.ewald14 = ctrl->keyEwald != 0;
+    D.elec14 = ctrl->elecType;
+    D.vdw14 = ctrl->vdwType == 2;
+    RETCHK(mdg_bonded_upload(ctx, param->nBond, bond, param->nUb, ub, param->nAngle, angle,
+                             param->nDihedral, dihe, param->nImproper, impr, n14, neighList14,
+                             eps14, sig14));
+    RETCHK(mdg_bonded_forces(ctx, ctx->fx, ctx->fy, ctx->fz, nullptr));
+    double b[11];
+    RETCHK(d2h(ctx, b, ctx->scal + S_BONDED, 11));
+    RETCHK(download_forces(ctx, fx, fy, fz));
+    ener->elec += b[7];
+    ener->vdw += b[8];
+    ener->virelec += b[9];
+    ener->virvdw += b[10];
+    ener->bond += b[0];
+    ener->ub += b[1];
+    ener->ang += b[2];
+    ener->dihe += b[3];
+    ener->impr += b[4];
+    ener->virbond += b[5];
+    ener->virub += b[6];
+    // energy.c:193-196
     ener->pot = ener->elec + ener->vdw + ener->bond + ener->ang + ener->ub + ener->dihe + ener->impr;
     ener->virpot = ener->virbond + ener->virub + ener->virelec + ener->virvdw;
     if (param->nFrozen > 0 && frozen)   // utils.c:131-153
-        for (int i = 0; i < ctx->n; i++)
+        for (int i = 0; i < n; i++)
             if (frozen[i]) vx[i] = vy[i] = vz[i] = fx[i] = fy[i] = fz[i] = 0.;
     return MDG_OK;
 }

@@ -139,6 +139,23 @@ struct MdgBatch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
 
+// Bonded topology on the device (mdg_bonded.hip): the term tables in the
+// reference layouts, the 1-4 pair list and its LJ parameters, and how the
+// 1-4 pairs are evaluated (ewald14_energy or nonbond14_energy).
+struct MdgBonded {
+    MDG_BOND *bond = nullptr, *ub = nullptr;
+    MDG_ANGLE *angle = nullptr;
+    MDG_DIHE *dihe = nullptr, *impr = nullptr;
+    int *l14 = nullptr;          // 1-4 pairs (i, j)
+    double *eps14 = nullptr, *sig14 = nullptr;
+    long long *fix = nullptr;    // int64 per-atom accumulators (x | y | z) of the drop-in
+    int nBond = 0, nUb = 0, nAngle = 0, nDihe = 0, nImpr = 0, n14 = 0;
+    int capBond = 0, capUb = 0, capAngle = 0, capDihe = 0, capImpr = 0, cap14 = 0;
+    double scal14 = 0.;
+    int ewald14 = 1, elec14 = 0, vdw14 = 1;
+    bool on = false;             // the device-resident step adds these terms
+};
+
 // Launch shapes the autotuner chooses (mdg_autotune).
 struct MdgTune {
     int shkSpread = 1;           // three-site SHAKE: quads per molecule slot (1, 2, 4, 8, 16),
@@ -213,6 +230,7 @@ struct mdg_ctx {
     // parameter adjoints dE/d(q, eps, sig) and the SPME mesh potential (mdg_adjoint.hip)
     double *adjQ = nullptr, *adjE = nullptr, *adjS = nullptr, *adjPot = nullptr;
     std::vector<int> hExOff, hExLst;
+    std::vector<MDG_CONSTRAINT> hCons;   // the constraint list of mdg_set_topology
     bool exclBuilt = false;
 
     // link-cell / Verlet scratch
@@ -224,6 +242,7 @@ struct mdg_ctx {
 
     RefList rl;
     ClusterList cl;
+    MdgBonded bd;
 
     // forces / energies of the device-resident path (slot order)
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;        // slot positions
@@ -315,7 +334,8 @@ enum {
     S_VIRSHAKE = 24,
     S_REF = 25,                                      // drop-in pair loop (6)
     S_REFCORR = 31,                                  // drop-in correction (2)
-    S_COUNT = 40
+    S_BONDED = 33,                                   // bonded and 1-4 sums (11)
+    S_COUNT = 48
 };
 
 // regions of ctx->epart (per-block partial sums), each PART_R doubles
@@ -324,7 +344,8 @@ enum {
 #define MDG_PAIR_MAX_BLOCKS 16384
 enum {
     PART_PAIR = 0, PART_EXCL = PART_R, PART_INFL = 2 * PART_R, PART_GATHER = 3 * PART_R,
-    PART_KIN = 4 * PART_R, PART_MISC = 5 * PART_R, PART_REF = 6 * PART_R, PART_TOTAL = 8 * PART_R
+    PART_KIN = 4 * PART_R, PART_MISC = 5 * PART_R, PART_REF = 6 * PART_R,
+    PART_BOND = 7 * PART_R, PART_TOTAL = 8 * PART_R
 };
 
 // ---------------------------------------------------------------- helpers
@@ -468,7 +489,20 @@ __device__ inline void mdg_block_partials(double (&v)[NV], double *out)
 // ---------------------------------------------------------------- internal API
 int mdg_alloc_state(mdg_ctx *ctx);
 int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nConst,
-                         const MDG_CONSTRAINT *cons, int nAngle, const MDG_ANGLE *angle);
+                         const MDG_CONSTRAINT *cons, int nAngle, const MDG_ANGLE *angle,
+                         int nDihe = 0, const MDG_DIHE *dihe = nullptr, int nImpr = 0,
+                         const MDG_DIHE *impr = nullptr, std::vector<int> *list14 = nullptr);
+
+// bonded terms and 1-4 pairs (mdg_bonded.hip)
+int mdg_bonded_upload(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nUb, const MDG_BOND *ub,
+                      int nAngle, const MDG_ANGLE *angle, int nDihe, const MDG_DIHE *dihe,
+                      int nImpr, const MDG_DIHE *impr, int n14, const int *list14,
+                      const double *eps14, const double *sig14);
+int mdg_bonded_forces(mdg_ctx *ctx, double *fx, double *fy, double *fz, long long *fix,
+                      bool reduce = true);
+int mdg_bonded_blocks(const mdg_ctx *ctx);
+int mdg_fix_to_f64(mdg_ctx *ctx, long long *fix, double *fx, double *fy, double *fz);
+void mdg_bonded_free(mdg_ctx *ctx);
 int mdg_upload_exclusions(mdg_ctx *ctx);
 
 // scan / sort utilities (mdg_util.hip)

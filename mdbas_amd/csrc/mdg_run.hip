@@ -149,6 +149,9 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild, bool fin)
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[7], ctx->stream));
     }
     RETCHK(mdg_pair_cluster(ctx));
+    // bonded terms and 1-4 pairs (energy.c:157-189) into their int64
+    // per-atom accumulators, added by the force assembly
+    if (ctx->bd.on) RETCHK(mdg_bonded_forces(ctx, nullptr, nullptr, nullptr, ctx->bd.fix, false));
     if (!spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_spme_device));
     if (ctx->spmeForked) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     if (!fin) return MDG_OK;
@@ -200,6 +203,7 @@ FinArgs mdg_fin_args(mdg_ctx *ctx)
     F.vz = ctx->vz;
     F.part = ctx->epart + PART_EXCL;
     F.mask = ctx->shkMask;
+    F.bfix = ctx->bd.on ? ctx->bd.fix : nullptr;
     return F;
 }
 
@@ -227,6 +231,8 @@ RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir, int nbExcl)
     R.pInfl = ctx->epart + PART_INFL;
     R.pKin = ctx->epart + PART_KIN;
     R.grpVir = ctx->grpVir;
+    R.nbBond = ctx->bd.on ? mdg_bonded_blocks(ctx) : 0;
+    R.pBond = ctx->epart + PART_BOND;
     return R;
 }
 
@@ -279,8 +285,14 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
                            mdg_rec_args(ctx, mdg_vv_blocks(ctx), mdg_vir_rows(ctx), mdg_excl_rows(ctx)),
                            ctx->ctrl, ctx->recDev, rebuild ? 1 : 0);
     if (fork && st == MDG_OK) {
-        HIPCHK(hipEventRecord(ctx->evRecDone, ctx->stream2));
-        ctx->recPending = true;
+        // a failure here must not leave ctx->stream on stream2: record the
+        // status and fall through to the restore below
+        if (hipEventRecord(ctx->evRecDone, ctx->stream2) == hipSuccess) {
+            ctx->recPending = true;
+        } else {
+            ctx->err = "hipEventRecord failed at the record fork";
+            st = MDG_EHIP;
+        }
     }
     // the next step's pruned list, from this step's slot positions (in a
     // step block beside the next step's SHAKE kernels, joined before its

@@ -60,6 +60,7 @@ struct FinArgs {
     double *fx, *fy, *fz, *vx, *vy, *vz;
     double *part;                        // per block: eCorr, virCorr
     unsigned long long *mask;            // the SHAKE masks (stage 1, then stage 2)
+    long long *bfix;                     // bonded / 1-4 forces, int64 per atom (x | y | z), or null
 };
 
 // Mean reciprocal force over the atoms, from the gather's block partials
@@ -120,6 +121,15 @@ __device__ inline bool fin_atom(const FinArgs &A, int i, const double *mean, dou
             acc[1] += dC * r;
         }
     }
+    if (A.bfix) {
+        // bonded and 1-4 forces (mdg_bonded.hip), cleared for the next step
+        gx += (double)A.bfix[i] * (1.0 / MDG_FSCALE);
+        gy += (double)A.bfix[A.n + i] * (1.0 / MDG_FSCALE);
+        gz += (double)A.bfix[2 * (long)A.n + i] * (1.0 / MDG_FSCALE);
+        A.bfix[i] = 0;
+        A.bfix[A.n + i] = 0;
+        A.bfix[2 * (long)A.n + i] = 0;
+    }
     const bool fr = A.hasFrozen && A.frozen[i];
     if (fr) {
         gx = gy = gz = 0.;
@@ -137,8 +147,8 @@ struct RecArgs {
     double fact, eSelf, eNet;
     int recCap;
     // per-block partial counts of this step's producers
-    int nbPair, nbExcl, nbInfl, nbKin, nGrpVir;
-    const double *pPair, *pExcl, *pInfl, *pKin, *grpVir;
+    int nbPair, nbExcl, nbInfl, nbKin, nGrpVir, nbBond;
+    const double *pPair, *pExcl, *pInfl, *pKin, *grpVir, *pBond;
 };
 
 // One MDG_STEP record by one block of SIDE_BLOCK threads: reduces the
@@ -146,7 +156,7 @@ struct RecArgs {
 // SHAKE virial) in a fixed order and assembles the energies
 // (spme.c:674-697 for the reciprocal energy and virial, energy.c:623-627
 // and 193-196 for the sums).
-#define REC_NV 16
+#define REC_NV 27   // 16 nonbonded / kinetic / SHAKE sums + 11 bonded and 1-4 sums
 __device__ inline void record_block(const RecArgs &R, int *ctrl, MDG_STEP *rec, int rebuilt)
 {
     __shared__ double red[SIDE_BLOCK / 64][REC_NV];
@@ -159,6 +169,7 @@ __device__ inline void record_block(const RecArgs &R, int *ctrl, MDG_STEP *rec, 
     rows_add<7>(R.pInfl, R.nbInfl, v + 7);
     for (int r = threadIdx.x; r < R.nbKin; r += blockDim.x) v[14] += R.pKin[(long)r * 7];
     rows_add<1>(R.grpVir, R.nGrpVir, v + 15);
+    if (R.nbBond) rows_add<11>(R.pBond, R.nbBond, v + 16);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < REC_NV; k++) {
@@ -188,13 +199,20 @@ __device__ inline void record_block(const RecArgs &R, int *ctrl, MDG_STEP *rec, 
     s.eDir = v[0];
     s.eRec = eRec;
     s.eCorr = v[5];
-    s.elec = s.eDir + eRec + s.eCorr;
-    s.vdw = v[1];
-    s.pot = s.elec + s.vdw;
+    // energy.c:623-627 plus the 1-4 pairs (energy.c:431-435 / 726-730)
+    s.elec = (s.eDir + eRec + s.eCorr) + v[23];
+    s.vdw = v[1] + v[24];
+    s.ebond = v[16];
+    s.eub = v[17];
+    s.eang = v[18];
+    s.edihe = v[19];
+    s.eimpr = v[20];
+    // energy.c:193-194
+    s.pot = s.elec + s.vdw + s.ebond + s.eang + s.eub + s.edihe + s.eimpr;
     s.kin = 0.5 * v[14];
     s.tot = s.pot + s.kin;
-    s.virelec = v[2] + virRec + v[6];
-    s.virvdw = v[3];
+    s.virelec = (v[2] + virRec + v[6]) + v[25];
+    s.virvdw = v[3] + v[26];
     s.virshake = v[15];
     s.nInCut = v[4];
     if (idx < R.recCap) rec[idx] = s;
@@ -258,6 +276,7 @@ __device__ inline void b_record_block(const BRecArgs &R, int r, int rebuilt)
     s.elec = s.eDir + eRec + s.eCorr;
     s.vdw = v[1];
     s.pot = s.elec + s.vdw;
+    s.ebond = s.eub = s.eang = s.edihe = s.eimpr = 0.;
     s.kin = 0.5 * v[7];
     s.tot = s.pot + s.kin;
     s.virelec = v[2] + (-(st0 + st3 + st5)) + v[6];

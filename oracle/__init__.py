@@ -86,7 +86,8 @@ def ref():
         R.ref_list_count.restype = C.c_long
         for f in ("ref_makelist", "ref_list_count", "ref_list_copy", "ref_excl_copy",
                   "ref_vdw_switch", "ref_coulomb", "ref_nonbond", "ref_ewald_rec", "ref_vv",
-                  "ref_constvel"):
+                  "ref_constvel", "ref_set_bonded", "ref_n14", "ref_list14_copy", "ref_bonded",
+                  "ref_nb14"):
             getattr(R, f).argtypes = None
         _REF = R
     return _REF
@@ -136,10 +137,27 @@ class OracleSystem:
         self.b = make_box(wb.L)
         self.keyLink = lib().oc_choose_link(C.byref(self.p), C.byref(self.b))
         self.exOff = np.zeros(wb.n + 1, np.int32)
-        cap = 8 * len(wb.ca) + 16
-        self.exLst = np.zeros(cap, np.int32)
-        ne = lib().oc_exclusions(wb.n, 0, None, None, len(wb.ca), P(wb.ca), P(wb.cb), 0, None,
-                                 None, None, int(self.keyLink), P(self.exOff), P(self.exLst), cap)
+        self.bonded_topology = hasattr(wb, "bonds")
+        if self.bonded_topology:
+            t = topology_arrays(wb)
+            cap = 16 * (len(wb.ca) + len(wb.bonds) + len(wb.angles) + len(wb.dihes)
+                        + len(wb.imprs)) + 16
+            self.exLst = np.zeros(cap, np.int32)
+            l14 = np.zeros(2 * (5 * len(wb.dihes) + 1), np.int32)
+            n14 = C.c_int(0)
+            ne = lib().oc_exclusions14(wb.n, len(wb.bonds), P(t["ba"]), P(t["bb"]), len(wb.ca),
+                                       P(wb.ca), P(wb.cb), len(wb.angles), P(t["aa"]),
+                                       P(t["ab"]), P(t["ac"]), len(wb.dihes), P(t["d4"]),
+                                       len(wb.imprs), P(t["i4"]), int(self.keyLink),
+                                       P(self.exOff), P(self.exLst), cap, P(l14), C.byref(n14))
+            self.list14 = l14[:2 * n14.value].copy()
+        else:
+            cap = 8 * len(wb.ca) + 16
+            self.exLst = np.zeros(cap, np.int32)
+            ne = lib().oc_exclusions(wb.n, 0, None, None, len(wb.ca), P(wb.ca), P(wb.cb), 0, None,
+                                     None, None, int(self.keyLink), P(self.exOff), P(self.exLst),
+                                     cap)
+            self.list14 = np.zeros(0, np.int32)
         if ne < 0:
             raise RuntimeError(f"oracle exclusions failed: {ne}")
         self.exLst = self.exLst[:ne].copy()
@@ -224,6 +242,70 @@ class OracleSystem:
                        eCorr=e.eCorr, virelec=e.virDir + vr + e.virCorr, virvdw=e.virVdw,
                        nInCut=e.nInCut, nListed=e.nListed, stressRec=st)
 
+    def bonded(self, x, y, z, f=None):
+        """Bonded energies (internal.c:21-604) in energy()'s order; forces
+        accumulate into f.  Returns (f, [bond, ub, ang, dihe, impr, virbond,
+        virub])."""
+        wb = self.wb
+        f = np.zeros((3, wb.n)) if f is None else f
+        out = np.zeros(7)
+        lib().oc_bonded(C.byref(self.b), len(wb.bonds), wb.c_bonds(), len(wb.ubs), wb.c_ubs(),
+                        len(wb.angles), wb.c_angles(), len(wb.dihes), wb.c_dihes(),
+                        len(wb.imprs), wb.c_imprs(), P(x), P(y), P(z), P(f[0]), P(f[1]),
+                        P(f[2]), P(out))
+        return f, out
+
+    def nb14(self, ewald, elec_type, x, y, z, f=None, vdw_on=1):
+        """1-4 pairs (ewald14_energy or nonbond14_energy); returns (f,
+        [elec, vdw, virelec, virvdw])."""
+        wb = self.wb
+        f = np.zeros((3, wb.n)) if f is None else f
+        out = np.zeros(4)
+        lib().oc_nb14(C.byref(self.p), C.byref(self.b), int(ewald), int(elec_type), int(vdw_on),
+                      C.c_double(wb.scal14), len(self.list14) // 2, P(self.list14), P(x), P(y),
+                      P(z), P(f[0]), P(f[1]), P(f[2]), P(self.q), P(wb.eps14), P(wb.sig14),
+                      P(out))
+        return f, out
+
+    def energy_full(self, x, y, z):
+        """energy() (energy.c:103-205) on the Ewald path with 1-4 pairs and
+        every bonded term: forces (3,N) and the ENERGY fields."""
+        f, e = self.energy(x, y, z)
+        f, e14 = self.nb14(1, 0, x, y, z, f)
+        f, eb = self.bonded(x, y, z, f)
+        elec = e["elec"] + e14[0]
+        vdw = e["vdw"] + e14[1]
+        d = dict(e, elec=elec, vdw=vdw, bond=eb[0], ub=eb[1], ang=eb[2], dihe=eb[3], impr=eb[4],
+                 virbond=eb[5], virub=eb[6], virelec=e["virelec"] + e14[2],
+                 virvdw=e["virvdw"] + e14[3])
+        d["pot"] = elec + vdw + eb[0] + eb[2] + eb[1] + eb[3] + eb[4]
+        return f, d
+
+    def md_run_full(self, nsteps, x, y, z, vx, vy, vz):
+        """The MD loop of mdbas.c:244-363 (as oc_md_run) with energy_full:
+        list build, step-0 energy, then per step VV stage 1, the list update
+        every `update` steps, energy_full, VV stage 2.  Returns (err, f,
+        etot, epot) with per-step energies (length nsteps + 1)."""
+        upd = self.cfg["update"]
+        self.build_list(x, y, z, build=True)
+        f, e = self.energy_full(x, y, z)
+        m = self.wb.mass
+        ek = 0.5 * float((m * (vx * vx + vy * vy + vz * vz)).sum())
+        et, ep = [e["pot"] + ek], [e["pot"]]
+        for step in range(1, nsteps + 1):
+            err, _, _, _ = self.vv(1, x, y, z, vx, vy, vz, f)
+            if err:
+                return err, f, np.array(et), np.array(ep)
+            if step % upd == 0:
+                self.build_list(x, y, z, build=False)
+            f, e = self.energy_full(x, y, z)
+            err, ek, _, _ = self.vv(2, x, y, z, vx, vy, vz, f)
+            if err:
+                return err, f, np.array(et), np.array(ep)
+            et.append(e["pot"] + ek)
+            ep.append(e["pot"])
+        return 0, f, np.array(et), np.array(ep)
+
     def vv(self, stage, x, y, z, vx, vy, vz, f):
         ek = C.c_double(0)
         vs = C.c_double(0)
@@ -249,6 +331,15 @@ class OracleSystem:
         return err, f, et, ep, ek, sec.value
 
 
+def topology_arrays(wb):
+    """Flat int32 index arrays of a bonded topology for the C oracle."""
+    i32 = lambda v: np.ascontiguousarray(np.array(v, np.int32).reshape(-1))  # noqa: E731
+    return dict(ba=i32([b[1] for b in wb.bonds]), bb=i32([b[2] for b in wb.bonds]),
+                aa=i32([a[1] for a in wb.angles]), ab=i32([a[2] for a in wb.angles]),
+                ac=i32([a[3] for a in wb.angles]),
+                d4=i32([d[2:6] for d in wb.dihes]), i4=i32([d[2:6] for d in wb.imprs]))
+
+
 class RefSystem:
     """The compiled reference (oracle/_ref) driven on the same arrays."""
 
@@ -265,6 +356,38 @@ class RefSystem:
                                          cfg["update"], noLink, cfg["timeStep"],
                                          cfg["tolShake"], cfg["maxCycle"], len(wb.ca), P(wb.ca),
                                          P(wb.cb), P(wb.rc2), P(wb.frozen), P(wb.nAtConst)))
+        if hasattr(wb, "bonds"):
+            self.set_bonded()
+
+    def set_bonded(self):
+        """Hands the molecule topology to the reference structs."""
+        wb = self.wb
+        ref().ref_set_bonded(self.h, len(wb.bonds), wb.c_bonds(), len(wb.ubs), wb.c_ubs(),
+                             len(wb.angles), wb.c_angles(), len(wb.dihes), wb.c_dihes(),
+                             len(wb.imprs), wb.c_imprs(), C.c_double(wb.scal14))
+
+    def list14(self):
+        R = ref()
+        n = R.ref_n14(self.h)
+        out = np.zeros(max(1, 2 * n), np.int32)
+        R.ref_list14_copy(self.h, P(out))
+        return out[:2 * n]
+
+    def bonded(self, x, y, z, f=None):
+        n = self.wb.n
+        f = np.zeros((3, n)) if f is None else f
+        out = np.zeros(7)
+        ref().ref_bonded(self.h, P(x), P(y), P(z), P(f[0]), P(f[1]), P(f[2]), P(out))
+        return f, out
+
+    def nb14(self, ewald, elec_type, x, y, z, f=None, vdw_type=2):
+        n = self.wb.n
+        f = np.zeros((3, n)) if f is None else f
+        out = np.zeros(4)
+        ref().ref_nb14(self.h, C.c_int(ewald), C.c_int(elec_type), C.c_int(vdw_type), P(x), P(y),
+                       P(z), P(f[0]), P(f[1]), P(f[2]), P(self.wb.q), P(self.wb.eps14),
+                       P(self.wb.sig14), P(out))
+        return f, out
 
     def makelist(self, x, y, z, step=0):
         R = ref()
@@ -279,9 +402,10 @@ class RefSystem:
 
     def exclusions(self):
         n = self.wb.n
-        cap = 8 * len(self.wb.ca) + 16
         off = np.zeros(n + 1, np.int32)
-        lst = np.zeros(cap, np.int32)
+        lst = np.zeros(1, np.int32)
+        cap = ref().ref_excl_copy(self.h, P(off), P(lst), C.c_int(0))
+        lst = np.zeros(max(cap, 1), np.int32)
         ne = ref().ref_excl_copy(self.h, P(off), P(lst), C.c_int(cap))
         return off, lst[:ne]
 

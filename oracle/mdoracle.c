@@ -176,13 +176,120 @@ static int iv_has(const ivec *a, int x)
  * at the smaller index (list.c:141-150); in symmetric mode at both
  * (list.c:1159-1181).  dedupe mirrors the reference: bonds and constraints
  * are appended unconditionally, angle and walk pairs only if absent. */
-static void ex_add(ivec *ex, int i, int j, int sym, int dedupe)
+static int ex_add(ivec *ex, int i, int j, int sym, int dedupe)
 {
     int ii = i, jj = j;
     if (!sym && jj < ii) { ii = j; jj = i; }
-    if (dedupe && iv_has(&ex[ii], jj)) return;
+    if (dedupe && iv_has(&ex[ii], jj)) return 0;
     iv_push(&ex[ii], jj);
     if (sym) iv_push(&ex[jj], ii);
+    return 1;
+}
+
+/* The exclusion builders with dihedrals, impropers and the 1-4 list
+ * (list.c:108-894 ordered, 1125-1791 symmetric):
+ *   bonds and constraints appended unconditionally (and recorded as
+ *   connections), angle pairs (a,b) (a,c) (b,c) when absent; per dihedral
+ *   the pairs (a,b) (a,c) (a,d) (b,c) (b,d) (c,d) when absent, where a new
+ *   (a,d) also becomes 1-4 pair (a, d); impropers the same six pairs with
+ *   no 1-4 pair; then the connectivity walk a-b-c(-d) over bonds and
+ *   constraints: (a,c) when absent, and for each d the check
+ *   nPair14 >= 5 nDihedral -> DIHE_NONPARAM_ERROR (110, list.c:835-836)
+ *   before (a,d) is added as exclusion and 1-4 pair when absent.
+ * Returns the number of exclusion entries (CSR off/lst), -110 / -111 on
+ * the reference's errors, or -(needed+1000) when cap is too small.
+ * *n14 receives the 1-4 pair count; list14 (room for 5 nDihedral pairs)
+ * the pairs as (i, j). */
+int oc_exclusions14(int n, int nBond, const int *ba, const int *bb,
+                    int nConst, const int *ca, const int *cb,
+                    int nAngle, const int *aa, const int *ab, const int *ac,
+                    int nDihe, const int *d4, int nImpr, const int *i4,
+                    int sym, int *off, int *lst, int cap, int *list14, int *n14)
+{
+    ivec *ex = (ivec *)calloc((size_t)n, sizeof(ivec));
+    ivec *con = (ivec *)calloc((size_t)n, sizeof(ivec));
+    int err = 0, np14 = 0;
+
+    for (int i = 0; i < nBond; i++) {                  /* list.c:137-174 */
+        ex_add(ex, ba[i], bb[i], sym, 0);
+        iv_push(&con[ba[i]], bb[i]);
+        iv_push(&con[bb[i]], ba[i]);
+    }
+    for (int i = 0; i < nConst; i++) {                 /* list.c:176-216 */
+        ex_add(ex, ca[i], cb[i], sym, 0);
+        iv_push(&con[ca[i]], cb[i]);
+        iv_push(&con[cb[i]], ca[i]);
+    }
+    for (int i = 0; i < nAngle; i++) {                 /* list.c:218-326 */
+        ex_add(ex, aa[i], ab[i], sym, 1);
+        ex_add(ex, aa[i], ac[i], sym, 1);
+        ex_add(ex, ab[i], ac[i], sym, 1);
+    }
+    for (int i = 0; i < nDihe; i++) {                  /* list.c:334-549 */
+        const int a = d4[4 * i], b = d4[4 * i + 1], c = d4[4 * i + 2], d = d4[4 * i + 3];
+        ex_add(ex, a, b, sym, 1);
+        ex_add(ex, a, c, sym, 1);
+        if (ex_add(ex, a, d, sym, 1)) {
+            list14[2 * np14] = a;
+            list14[2 * np14 + 1] = d;
+            np14++;
+        }
+        ex_add(ex, b, c, sym, 1);
+        ex_add(ex, b, d, sym, 1);
+        ex_add(ex, c, d, sym, 1);
+    }
+    for (int i = 0; i < nImpr; i++) {                  /* list.c:551-762 */
+        const int a = i4[4 * i], b = i4[4 * i + 1], c = i4[4 * i + 2], d = i4[4 * i + 3];
+        ex_add(ex, a, b, sym, 1);
+        ex_add(ex, a, c, sym, 1);
+        ex_add(ex, a, d, sym, 1);
+        ex_add(ex, b, c, sym, 1);
+        ex_add(ex, b, d, sym, 1);
+        ex_add(ex, c, d, sym, 1);
+    }
+    for (int ia = 0; ia < n && !err; ia++) {          /* list.c:764-862 */
+        for (int j = 0; j < con[ia].n && !err; j++) {
+            int ib = con[ia].v[j];
+            for (int k = 0; k < con[ib].n && !err; k++) {
+                int ic = con[ib].v[k];
+                if (ic == ia) continue;
+                ex_add(ex, ia, ic, sym, 1);
+                for (int l = 0; l < con[ic].n; l++) {
+                    int id = con[ic].v[l];
+                    if (id == ia || id == ib) continue;
+                    if (np14 >= 5 * nDihe) { err = 110; break; }
+                    if (ex_add(ex, ia, id, sym, 1)) {
+                        list14[2 * np14] = ia;
+                        list14[2 * np14 + 1] = id;
+                        np14++;
+                    }
+                }
+            }
+        }
+    }
+
+    int total = 0;
+    for (int i = 0; i < n; i++) total += ex[i].n;
+    if (!err && !sym && n > 0 && ex[n - 1].n != 0) err = 111;   /* list.c:877 */
+    int ret;
+    if (err) {
+        ret = -err;
+    } else if (total > cap) {
+        ret = -total - 1000;
+    } else {
+        int o = 0;
+        for (int i = 0; i < n; i++) {
+            off[i] = o;
+            for (int k = 0; k < ex[i].n; k++) lst[o++] = ex[i].v[k];
+        }
+        off[n] = o;
+        ret = total;
+    }
+    if (n14) *n14 = np14;
+    for (int i = 0; i < n; i++) { free(ex[i].v); free(con[i].v); }
+    free(ex);
+    free(con);
+    return ret;
 }
 
 int oc_exclusions(int n, int nBond, const int *ba, const int *bb,
@@ -1226,4 +1333,260 @@ void oc_energy_grad(const oc_param *p, const oc_box *b,
             dq[i] += q[j] * ec;
             dq[j] += q[i] * ec;
         }
+}
+
+/* ---------------------------------------------------------------- */
+/* Bonded terms: internal.c:21-604.                                  */
+/* ---------------------------------------------------------------- */
+
+/* One bond or Urey-Bradley term (internal.c:33-80, 120-150): harmonic
+ * 0.5 k (r - r0)^2, or Morse k (e^2 - 2e) + k with e = exp(-beta (r - r0))
+ * for bonds of type 1 (BMORSE).  Forces f_a += dE/dr * d/r, f_b -= the
+ * same, with d = x_b - x_a under the minimum image. */
+static void oc_bond_term(const oc_box *b, const oc_bond *t, int morse_ok,
+                         const double *x, const double *y, const double *z,
+                         double *fx, double *fy, double *fz, double *e, double *vir)
+{
+    int i = t->a, j = t->b;
+    double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+    double r = sqrt(oc_dist(b, d));
+    double db;
+    if (morse_ok && t->type == 1) {
+        double ma = exp(-t->beta * (r - t->r0));
+        double mb = SQ(ma);
+        *e += t->k * (mb - 2. * ma) + t->k;
+        db = 2. * t->k * t->beta * (ma - mb);
+    } else {
+        *e += 0.5 * t->k * SQ(r - t->r0);
+        db = t->k * (r - t->r0);
+    }
+    *vir += db * r;
+    double tfx = db * d[0] / r, tfy = db * d[1] / r, tfz = db * d[2] / r;
+    fx[i] += tfx; fy[i] += tfy; fz[i] += tfz;
+    fx[j] += -tfx; fy[j] += -tfy; fz[j] += -tfz;
+}
+
+/* angle_energy, internal.c:164-245: harmonic in theta = acos(cos t). */
+static void oc_angle_term(const oc_box *b, const oc_angle *t,
+                          const double *x, const double *y, const double *z,
+                          double *fx, double *fy, double *fz, double *e)
+{
+    int i = t->a, j = t->b, k = t->c;
+    double dab[3] = {x[i] - x[j], y[i] - y[j], z[i] - z[j]};
+    double rab = sqrt(oc_dist(b, dab)), rabt = 1. / rab;
+    double dbc[3] = {x[k] - x[j], y[k] - y[j], z[k] - z[j]};
+    double rbc = sqrt(oc_dist(b, dbc)), rbct = 1. / rbc;
+    double cost = (dab[0] * dbc[0] + dab[1] * dbc[1] + dab[2] * dbc[2]) / (rab * rbc);
+    double sint = sqrt(1. - cost * cost);
+    if (sint < 1.e-8) sint = 1.e-8;
+    double theta = acos(cost);
+    *e += 0.5 * t->k * SQ(theta - t->theta0);
+    double da = t->k * (theta - t->theta0) / sint;
+    double fa[3], fc[3];
+    for (int c = 0; c < 3; c++) {
+        fa[c] = da * (dbc[c] * rbct - dab[c] * cost * rabt) * rabt;
+        fc[c] = da * (dab[c] * rabt - dbc[c] * cost * rbct) * rbct;
+    }
+    fx[i] += fa[0]; fy[i] += fa[1]; fz[i] += fa[2];
+    fx[j] += -fa[0] - fc[0]; fy[j] += -fa[1] - fc[1]; fz[j] += -fa[2] - fc[2];
+    fx[k] += fc[0]; fy[k] += fc[1]; fz[k] += fc[2];
+}
+
+/* dihedral_energy / improper_energy, internal.c:247-604 (the same body):
+ * phi from the normals pb = dab x dbc, pc = dbc x dcd with the sign of
+ * dbc . (pc x pb); cosine k (1 + cos(m phi - phi0)) (type 1 and default)
+ * or harmonic 0.5 k dphi^2 with dphi wrapped by nint (type 2). */
+static void oc_dihe_term(const oc_box *b, const oc_dihe *t,
+                         const double *x, const double *y, const double *z,
+                         double *fx, double *fy, double *fz, double *e)
+{
+    int i = t->a, j = t->b, k = t->c, l = t->d;
+    double dab[3] = {x[i] - x[j], y[i] - y[j], z[i] - z[j]};
+    oc_dist(b, dab);
+    double dbc[3] = {x[j] - x[k], y[j] - y[k], z[j] - z[k]};
+    double rbc = sqrt(oc_dist(b, dbc));
+    double dcd[3] = {x[k] - x[l], y[k] - y[l], z[k] - z[l]};
+    oc_dist(b, dcd);
+    double pb[3] = {dab[1] * dbc[2] - dab[2] * dbc[1], dab[2] * dbc[0] - dab[0] * dbc[2],
+                    dab[0] * dbc[1] - dab[1] * dbc[0]};
+    double r2pb = SQ(pb[0]) + SQ(pb[1]) + SQ(pb[2]), rpb = sqrt(r2pb);
+    double pc[3] = {dbc[1] * dcd[2] - dbc[2] * dcd[1], dbc[2] * dcd[0] - dbc[0] * dcd[2],
+                    dbc[0] * dcd[1] - dbc[1] * dcd[0]};
+    double r2pc = SQ(pc[0]) + SQ(pc[1]) + SQ(pc[2]), rpc = sqrt(r2pc);
+    double pbpc = pb[0] * pc[0] + pb[1] * pc[1] + pb[2] * pc[2];
+    double cosp = pbpc / (rpb * rpc);
+    double sinp = (dbc[0] * (pc[1] * pb[2] - pc[2] * pb[1]) + dbc[1] * (pb[0] * pc[2] - pb[2] * pc[0]) +
+                   dbc[2] * (pc[0] * pb[1] - pc[1] * pb[0])) / (rpb * rpc * rbc);
+    double phi = atan2(sinp, cosp);
+    if (sinp >= 0.) sinp = fabs(sinp) > DBL_EPSILON ? fabs(sinp) : DBL_EPSILON;
+    else sinp = -(fabs(sinp) > DBL_EPSILON ? fabs(sinp) : DBL_EPSILON);
+    double ed, dd;
+    if (t->type == 2) {
+        phi = phi - t->phi0;
+        phi = phi - oc_nint(phi / OC_TWOPI) * OC_TWOPI;
+        ed = 0.5 * t->k * (phi * phi);
+        dd = t->k * phi / (rpb * rpc * sinp);
+    } else {
+        ed = t->k * (1. + cos(t->mult * phi - t->phi0));
+        dd = -t->k * t->mult * sin(t->mult * phi - t->phi0) / (rpb * rpc * sinp);
+    }
+    *e += ed;
+    double fa[3], fb[3], fc[3], fd[3];
+    fa[0] = dd * ((-pc[1] * dbc[2] + pc[2] * dbc[1]) - pbpc * (-pb[1] * dbc[2] + pb[2] * dbc[1]) / r2pb);
+    fa[1] = dd * ((pc[0] * dbc[2] - pc[2] * dbc[0]) - pbpc * (pb[0] * dbc[2] - pb[2] * dbc[0]) / r2pb);
+    fa[2] = dd * ((-pc[0] * dbc[1] + pc[1] * dbc[0]) - pbpc * (-pb[0] * dbc[1] + pb[1] * dbc[0]) / r2pb);
+    fc[0] = dd * ((-pc[1] * dab[2] + pc[2] * dab[1]) - pbpc * (-pb[1] * dab[2] + pb[2] * dab[1]) / r2pb);
+    fc[1] = dd * ((pc[0] * dab[2] - pc[2] * dab[0]) - pbpc * (pb[0] * dab[2] - pb[2] * dab[0]) / r2pb);
+    fc[2] = dd * ((-pc[0] * dab[1] + pc[1] * dab[0]) - pbpc * (-pb[0] * dab[1] + pb[1] * dab[0]) / r2pb);
+    fb[0] = dd * ((-pb[1] * dcd[2] + pb[2] * dcd[1]) - pbpc * (-pc[1] * dcd[2] + pc[2] * dcd[1]) / r2pc);
+    fb[1] = dd * ((pb[0] * dcd[2] - pb[2] * dcd[0]) - pbpc * (pc[0] * dcd[2] - pc[2] * dcd[0]) / r2pc);
+    fb[2] = dd * ((-pb[0] * dcd[1] + pb[1] * dcd[0]) - pbpc * (-pc[0] * dcd[1] + pc[1] * dcd[0]) / r2pc);
+    fd[0] = dd * ((-pb[1] * dbc[2] + pb[2] * dbc[1]) - pbpc * (-pc[1] * dbc[2] + pc[2] * dbc[1]) / r2pc);
+    fd[1] = dd * ((pb[0] * dbc[2] - pb[2] * dbc[0]) - pbpc * (pc[0] * dbc[2] - pc[2] * dbc[0]) / r2pc);
+    fd[2] = dd * ((-pb[0] * dbc[1] + pb[1] * dbc[0]) - pbpc * (-pc[0] * dbc[1] + pc[1] * dbc[0]) / r2pc);
+    fx[i] += fa[0]; fy[i] += fa[1]; fz[i] += fa[2];
+    fx[j] += -fa[0] - fc[0] + fb[0]; fy[j] += -fa[1] - fc[1] + fb[1]; fz[j] += -fa[2] - fc[2] + fb[2];
+    fx[k] += fc[0] - fb[0] - fd[0]; fy[k] += fc[1] - fb[1] - fd[1]; fz[k] += fc[2] - fb[2] - fd[2];
+    fx[l] += fd[0]; fy[l] += fd[1]; fz[l] += fd[2];
+}
+
+void oc_bonded(const oc_box *b, int nBond, const oc_bond *bond, int nUb, const oc_bond *ub,
+               int nAngle, const oc_angle *angle, int nDihe, const oc_dihe *dihe,
+               int nImpr, const oc_dihe *impr,
+               const double *x, const double *y, const double *z,
+               double *fx, double *fy, double *fz, double out[7])
+{
+    double eb = 0., eu = 0., ea = 0., ed = 0., ei = 0., vb = 0., vu = 0.;
+    for (int l = 0; l < nBond; l++) oc_bond_term(b, &bond[l], 1, x, y, z, fx, fy, fz, &eb, &vb);
+    for (int l = 0; l < nAngle; l++) oc_angle_term(b, &angle[l], x, y, z, fx, fy, fz, &ea);
+    for (int l = 0; l < nUb; l++) oc_bond_term(b, &ub[l], 0, x, y, z, fx, fy, fz, &eu, &vu);
+    for (int l = 0; l < nDihe; l++) oc_dihe_term(b, &dihe[l], x, y, z, fx, fy, fz, &ed);
+    for (int l = 0; l < nImpr; l++) oc_dihe_term(b, &impr[l], x, y, z, fx, fy, fz, &ei);
+    out[0] = eb; out[1] = eu; out[2] = ea; out[3] = ed; out[4] = ei; out[5] = vb; out[6] = vu;
+}
+
+/* ---------------------------------------------------------------- */
+/* 1-4 pairs: nonbond14_energy energy.c:356-439, ewald14_energy       */
+/* energy.c:654-742, with coulomb14_* elec.c:272-465, vdw14_* vdw.c:  */
+/* 202-311, ewald_dir14 / ewald_corr14 ewald.c:386-413.               */
+/* ---------------------------------------------------------------- */
+
+/* coulomb14_shift1/shift2/switch (2, 3, 4): scal14 x the plain functor,
+ * zero beyond the cutoff; NOELEC and FULL map to coulomb14_none
+ * (energy.c:37-54). */
+static double oc_coulomb14(const oc_param *p, int type, double scal14, double qel, double r2,
+                           double rt, double *de)
+{
+    double e = 0.;
+    *de = 0.;
+    if (type == 2 || type == 3) {
+        if (r2 <= p->cutOff2) {
+            double s1, s2, sf, dsf;
+            if (type == 2) {
+                s1 = rt * r2 * p->rcutOff;
+                s2 = r2 * p->rcutOff2;
+                sf = 1. - 2. * s1 + s2;
+                dsf = 2. * (s2 - s1);
+            } else {
+                s1 = r2 * p->rcutOff2;
+                s2 = SQ(s1);
+                sf = 1. - 2. * s1 + s2;
+                dsf = 4. * (s2 - s1);
+            }
+            double pe = scal14 * p->chargeConst * qel * rt;
+            e = pe * sf;
+            *de = pe * rt * (dsf - sf);
+        }
+    } else if (type == 4) {
+        if (r2 <= p->cutOn2) {
+            double pe = scal14 * p->chargeConst * qel * rt;
+            e = pe;
+            *de = -pe * rt;
+        } else if (r2 <= p->cutOff2) {
+            double sw = p->cutOff2 - r2;
+            double sf = SQ(sw) * (p->cutOff2 + 2. * r2 - 3. * p->cutOn2) * p->switch2;
+            double dsf = 12. * r2 * sw * (p->cutOn2 - r2) * p->switch2;
+            double pe = scal14 * p->chargeConst * qel * rt;
+            e = pe * sf;
+            *de = pe * rt * (dsf - sf);
+        }
+    }
+    return e;
+}
+
+/* vdw14_switch (vdw.c:270-311); vdw14_none when vdwOn is 0. */
+static double oc_vdw14(const oc_param *p, int vdwOn, double scal14, double veps, double vsig,
+                       double r2, double rt, double *dv)
+{
+    double e = 0.;
+    *dv = 0.;
+    if (!vdwOn) return 0.;
+    double s6 = vsig * rt;
+    s6 = s6 * s6 * s6 * s6 * s6 * s6;
+    double s12 = SQ(s6);
+    if (r2 <= p->cutOn2) {
+        e = 4. * scal14 * veps * (s12 - s6);
+        *dv = 24. * scal14 * veps * rt * (s6 - 2. * s12);
+    } else if (r2 <= p->cutOff2) {
+        double sw = p->cutOff2 - r2;
+        double sf = SQ(sw) * (p->cutOff2 + 2. * r2 - 3. * p->cutOn2) * p->switch2;
+        double dsf = 12. * r2 * sw * (p->cutOn2 - r2) * p->switch2;
+        double pv = 4. * scal14 * veps * (s12 - s6);
+        double dpv = 24. * scal14 * veps * (s6 - 2. * s12);
+        e = pv * sf;
+        *dv = (pv * dsf + dpv * sf) * rt;
+    }
+    return e;
+}
+
+void oc_nb14(const oc_param *p, const oc_box *b, int ewald, int elecType, int vdwOn,
+             double scal14, int nPair14, const int *list14,
+             const double *x, const double *y, const double *z,
+             double *fx, double *fy, double *fz,
+             const double *q, const double *eps14, const double *sig14, double out[4])
+{
+    double elec = 0., evdw = 0., virel = 0., virvdw = 0.;
+    double eD = 0., eC = 0., vD = 0., vC = 0.;
+    for (int k = 0; k < nPair14; k++) {
+        int i = list14[2 * k], j = list14[2 * k + 1];
+        double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+        double r2 = oc_dist(b, d);
+        double r = sqrt(r2), rt = 1. / r;
+        double veps = eps14[i] * eps14[j], vsig = sig14[i] + sig14[j];
+        double dv;
+        double ev = oc_vdw14(p, vdwOn, scal14, veps, vsig, r2, rt, &dv);
+        double fs;
+        if (ewald) {
+            /* ewald_dir14 / ewald_corr14 */
+            double qel = p->chargeConst * q[i] * q[j];
+            double ar = p->alpha * r;
+            double ed = scal14 * qel * erfc(ar) * rt;
+            double dd = 2.0 * scal14 * qel * p->alpha * exp(-SQ(ar)) / OC_SQRTPI;
+            dd = -(ed + dd) * rt;
+            double ec = (scal14 - 1.0) * qel * erf(ar) * rt;
+            double dc = 2.0 * (scal14 - 1.0) * qel * p->alpha * exp(-SQ(ar)) / OC_SQRTPI;
+            dc = (dc - ec) * rt;
+            eD += ed; eC += ec; vD += dd * r; vC += dc * r;
+            fs = dd + dc + dv;
+        } else {
+            double de;
+            elec += oc_coulomb14(p, elecType, scal14, q[i] * q[j], r2, rt, &de);
+            virel += de * r;
+            fs = de + dv;
+        }
+        evdw += ev;
+        virvdw += dv * r;
+        double fxj = fs * d[0] * rt, fyj = fs * d[1] * rt, fzj = fs * d[2] * rt;
+        fx[i] += fxj; fy[i] += fyj; fz[i] += fzj;
+        fx[j] += -fxj; fy[j] += -fyj; fz[j] += -fzj;
+    }
+    if (ewald) {
+        out[0] = eD + eC;
+        out[2] = vD + vC;
+    } else {
+        out[0] = elec;
+        out[2] = virel;
+    }
+    out[1] = evdw;
+    out[3] = virvdw;
 }

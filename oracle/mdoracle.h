@@ -200,6 +200,42 @@ int oc_md_run(const oc_param *p, const oc_box *b, int keyLink, int nsteps,
               const double *rc2, double *etot, double *epot, double *ekin,
               double *seconds);
 
+/* ---- bonded terms and 1-4 pairs ---------------------------------- */
+
+/* Term records with the reference's layouts (BOND, ANGLE, DIHE of
+ * global.h:247-267; enums are int-sized). */
+typedef struct { int type; int a, b; double k, r0, beta; } oc_bond;
+typedef struct { int type; int a, b, c; double k, theta0; } oc_angle;
+typedef struct { int type; int order; int a, b, c, d; double k, phi0, mult; } oc_dihe;
+
+/* Exclusions with dihedrals, impropers and the 1-4 list (list.c:108-894,
+ * 1125-1791); d4 / i4 hold the a,b,c,d quadruples.  list14 needs room for
+ * 5 nDihe pairs.  Return as oc_exclusions. */
+int oc_exclusions14(int n, int nBond, const int *ba, const int *bb,
+                    int nConst, const int *ca, const int *cb,
+                    int nAngle, const int *aa, const int *ab, const int *ac,
+                    int nDihe, const int *d4, int nImpr, const int *i4,
+                    int sym, int *off, int *lst, int cap, int *list14, int *n14);
+
+/* bond_energy, ub_energy, angle_energy, dihedral_energy, improper_energy
+ * (internal.c:21-604) in energy()'s order (energy.c:163-189); forces
+ * accumulate.  out = ebond, eub, eangle, edihe, eimpr, virbond, virub. */
+void oc_bonded(const oc_box *b, int nBond, const oc_bond *bond, int nUb, const oc_bond *ub,
+               int nAngle, const oc_angle *angle, int nDihe, const oc_dihe *dihe,
+               int nImpr, const oc_dihe *impr,
+               const double *x, const double *y, const double *z,
+               double *fx, double *fy, double *fz, double out[7]);
+
+/* 1-4 pairs: ewald14_energy (ewald = 1, energy.c:654-742) or
+ * nonbond14_energy with the coulomb14 functor of elecType (energy.c:
+ * 356-439); switched LJ when vdwOn.  out = elec, vdw, virelec, virvdw
+ * (the amounts the reference adds to ENERGY). */
+void oc_nb14(const oc_param *p, const oc_box *b, int ewald, int elecType, int vdwOn,
+             double scal14, int nPair14, const int *list14,
+             const double *x, const double *y, const double *z,
+             double *fx, double *fy, double *fz,
+             const double *q, const double *eps14, const double *sig14, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,7 @@
 #include "shake.h"
 #include "integrate.h"
 #include "energy.h"
+#include "internal.h"
 
 FILE *outFile;   /* defined by mdbas.c in the reference */
 
@@ -45,6 +46,9 @@ typedef struct {
     int *neighList, *neighPair, *neighOrder, *neighList14, **exclList, *exclPair;
     int listed;
     int shakeAlloc;
+    BOND *bond, *ub;
+    ANGLE *angle;
+    DIHE *dihe, *impr;
 } ref_sys;
 
 static void ensure_out(void)
@@ -134,8 +138,8 @@ ref_sys *ref_create(int n, int boxType, const double *box9,
 int ref_makelist(ref_sys *s, double *x, double *y, double *z, int step)
 {
     s->param.step = step;
-    makelist(&s->ctrl, &s->param, &s->box, &s->neigh, s->cons, NULL, NULL, NULL, NULL,
-             x, y, z, s->frozen, &s->neighList, &s->neighPair, &s->neighOrder,
+    makelist(&s->ctrl, &s->param, &s->box, &s->neigh, s->cons, s->bond, s->angle, s->dihe,
+             s->impr, x, y, z, s->frozen, &s->neighList, &s->neighPair, &s->neighOrder,
              &s->neighList14, &s->exclList, &s->exclPair);
     s->listed = 1;
     return s->ctrl.keyLink;
@@ -262,4 +266,79 @@ void ref_constvel(ref_sys *s, double *x, double *y, double *z,
                   double *vx, double *vy, double *vz, double *mass)
 {
     init_constvel(&s->param, &s->box, s->cons, x, y, z, vx, vy, vz, mass, s->nAtConst);
+}
+
+/* Bonded topology (global.h:247-267 records) for makelist's exclusions
+ * and the bonded energies; scal14 is PARAM.scal14. */
+void ref_set_bonded(ref_sys *s, int nBond, const BOND *bond, int nUb, const BOND *ub,
+                    int nAngle, const ANGLE *angle, int nDihe, const DIHE *dihe,
+                    int nImpr, const DIHE *impr, double scal14)
+{
+    s->param.nBond = nBond;
+    s->param.nUb = nUb;
+    s->param.nAngle = nAngle;
+    s->param.nDihedral = nDihe;
+    s->param.nImproper = nImpr;
+    s->param.scal14 = scal14;
+    s->bond = (BOND *)malloc((size_t)(nBond + 1) * sizeof(BOND));
+    s->ub = (BOND *)malloc((size_t)(nUb + 1) * sizeof(BOND));
+    s->angle = (ANGLE *)malloc((size_t)(nAngle + 1) * sizeof(ANGLE));
+    s->dihe = (DIHE *)malloc((size_t)(nDihe + 1) * sizeof(DIHE));
+    s->impr = (DIHE *)malloc((size_t)(nImpr + 1) * sizeof(DIHE));
+    memcpy(s->bond, bond, (size_t)nBond * sizeof(BOND));
+    memcpy(s->ub, ub, (size_t)nUb * sizeof(BOND));
+    memcpy(s->angle, angle, (size_t)nAngle * sizeof(ANGLE));
+    memcpy(s->dihe, dihe, (size_t)nDihe * sizeof(DIHE));
+    memcpy(s->impr, impr, (size_t)nImpr * sizeof(DIHE));
+}
+
+int ref_n14(ref_sys *s) { return s->neigh.nPair14; }
+
+void ref_list14_copy(ref_sys *s, int *out)
+{
+    memcpy(out, s->neighList14, (size_t)(2 * s->neigh.nPair14) * sizeof(int));
+}
+
+/* energy()'s bonded calls (energy.c:163-189) in its order; forces
+ * accumulate.  out = bond, ub, ang, dihe, impr, virbond, virub. */
+void ref_bonded(ref_sys *s, const double *x, const double *y, const double *z,
+                double *fx, double *fy, double *fz, double *out)
+{
+    memset(&s->ener, 0, sizeof s->ener);
+    if (s->param.nBond > 0) bond_energy(&s->param, &s->ener, &s->box, s->bond, x, y, z, fx, fy, fz);
+    if (s->param.nAngle > 0) angle_energy(&s->param, &s->ener, &s->box, s->angle, x, y, z, fx, fy, fz);
+    if (s->param.nUb > 0) ub_energy(&s->param, &s->ener, &s->box, s->ub, x, y, z, fx, fy, fz);
+    if (s->param.nDihedral > 0)
+        dihedral_energy(&s->param, &s->ener, &s->box, s->dihe, x, y, z, fx, fy, fz);
+    if (s->param.nImproper > 0)
+        improper_energy(&s->param, &s->ener, &s->box, s->impr, x, y, z, fx, fy, fz);
+    out[0] = s->ener.bond;
+    out[1] = s->ener.ub;
+    out[2] = s->ener.ang;
+    out[3] = s->ener.dihe;
+    out[4] = s->ener.impr;
+    out[5] = s->ener.virbond;
+    out[6] = s->ener.virub;
+}
+
+/* ewald14_energy (ewald = 1) or nonbond14_energy over the 1-4 list of the
+ * last makelist; out = elec, vdw, virelec, virvdw added. */
+void ref_nb14(ref_sys *s, int ewald, int elecType, int vdwType, const double *x, const double *y,
+              const double *z, double *fx, double *fy, double *fz, const double *q,
+              const double *eps14, const double *sig14, double *out)
+{
+    s->ctrl.elecType = (enum ELEC_TYPE)elecType;
+    s->ctrl.vdwType = (enum VDW_TYPE)vdwType;
+    init_energy_ptrs(&s->ctrl);
+    memset(&s->ener, 0, sizeof s->ener);
+    if (ewald)
+        ewald14_energy(&s->param, &s->ener, &s->ewald, &s->box, &s->neigh, x, y, z, fx, fy, fz,
+                       q, eps14, sig14, s->neighList14);
+    else
+        nonbond14_energy(&s->param, &s->ener, &s->box, &s->neigh, x, y, z, fx, fy, fz, q, eps14,
+                         sig14, s->neighList14);
+    out[0] = s->ener.elec;
+    out[1] = s->ener.vdw;
+    out[2] = s->ener.virelec;
+    out[3] = s->ener.virvdw;
 }

@@ -39,6 +39,12 @@ def c1_box():
 
 
 @pytest.fixture(scope="session")
+def c2_box():
+    from mdbas_amd import configs
+    return configs.build(configs.CONFIGS["c2"])
+
+
+@pytest.fixture(scope="session")
 def small_box():
     """216 waters in an 18.6 A box, 9 A cutoff (the smallest replica)."""
     from mdbas_amd import configs

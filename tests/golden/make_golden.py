@@ -2,7 +2,8 @@
 
 The reference ships no tests, fixtures or example inputs, so its own
 compiled code (oracle/Makefile builds list.c, ewald.c, vdw.c, elec.c,
-utils.c, shake.c, integrate.c, init.c and energy.c from the reference tree)
+utils.c, shake.c, integrate.c, init.c, energy.c and internal.c from the
+reference tree)
 is driven here on synthetic inputs and its outputs are stored as small
 fixtures.  The test suite checks the FP64 oracle against them, which pins
 the oracle on machines where the reference tree is absent.
@@ -94,7 +95,11 @@ def lists():
              ("link3k", dict(configs.COMMON, n_waters=3000, L=(3000 / 0.0334) ** (1 / 3), seed=5,
                              grid=32, cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35)),
              ("w64", dict(configs.COMMON, n_waters=64, L=(64 / 0.0334) ** (1 / 3), seed=9,
-                          grid=16, cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35))]
+                          grid=16, cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35)),
+             # the two benchmark boxes: 30,000 atoms (4^3 link cells) and
+             # 192,000 atoms (8^3 link cells), 12 A cutoff, 2 A skin
+             ("c1", configs.CONFIGS["c1"]),
+             ("c2", configs.CONFIGS["c2"])]
     for name, cfg in cases:
         wb = configs.build(cfg)
         rs = O.RefSystem(wb, cfg)
@@ -166,12 +171,47 @@ def dynamics():
     np.savez_compressed(os.path.join(OUT, "dynamics.npz"), **out)
 
 
+def bonded():
+    """The molecule boxes of mdbas_amd.molecules through the compiled
+    reference: exclusions and 1-4 list of makelist (list.c:108-894,
+    1125-1791), the pair-list digest, the bonded energies and forces of
+    internal.c (bond, angle, Urey-Bradley, dihedral, improper) and the 1-4
+    pair energies and forces of ewald14_energy / nonbond14_energy
+    (energy.c:356-439, 654-742) for the Ewald path and SHIFT1 / SHIFT2 /
+    SWITCH."""
+    from mdbas_amd import molecules as Mo
+    out = {}
+    for name in ("flex216", "flex729"):
+        cfg = Mo.MOL_CONFIGS[name]
+        wb = Mo.build(cfg)
+        rs = O.RefSystem(wb, cfg)
+        x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+        kl, nl, np_, no = rs.makelist(x, y, z, 0)
+        off, lst = rs.exclusions()
+        out[f"{name}_keylink"] = np.array(kl)
+        out[f"{name}_digest"] = np.array(digest(nl, np_, no))
+        out[f"{name}_excl_digest"] = np.array(digest(off, lst))
+        out[f"{name}_list14"] = rs.list14()
+        f, e = rs.bonded(x, y, z)
+        out[f"{name}_bonded_E"] = e
+        if name == "flex216":
+            out[f"{name}_bonded_f"] = f
+            out[f"{name}_pos"] = np.stack([x, y, z])
+        for ew, et in ((1, 0), (0, 2), (0, 3), (0, 4)):
+            f, e = rs.nb14(ew, et, x, y, z)
+            out[f"{name}_nb14_{ew}{et}_E"] = e
+            if name == "flex216":
+                out[f"{name}_nb14_{ew}{et}_f"] = f
+    np.savez_compressed(os.path.join(OUT, "bonded.npz"), **out)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         O.build(ref=True)
     functors()
     lists()
     dynamics()
+    bonded()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), "bytes")

@@ -548,7 +548,7 @@ def test_launch_shapes_keep_results(c0_box, prec):
     varies from run to run, so FP64 trajectories agree to 1e-9 A."""
     wb = c0_box
     out = []
-    for shape in ((1, 1024, 0, 0), (2, 512, 1, 1), (0, 768, 0, 1)):
+    for shape in ((1, 1024, 0, 0), (2, 512, 1, 1), (0, 768, 0, 1), (1, 1024, 0, 2)):
         ctx = M.Context(wb, precision=M.MDG_MIXED if prec == "mixed" else M.MDG_FP64)
         ctx.set_launch(*shape)
         assert tuple(ctx.get_launch().values()) == shape

@@ -126,12 +126,19 @@ def test_functor_derivative_by_finite_differences(which, c0_box):
 
 # ------------------------------------------------------------------ lists
 
-@pytest.mark.parametrize("name", ["c0", "link3k", "w64"])
+@pytest.mark.parametrize("name", ["c0", "link3k", "w64", "c1", "c2"])
 def test_lists_match_reference_golden(name, gl):
+    """The oracle's list builders against digests of the compiled
+    reference's lists (list.c:896-1123, 1793-2619): the first build (with
+    its in-place wrap) and the update build after a seeded displacement,
+    element for element (neighList, neighPair, neighOrder), on c0 (O(N^2)),
+    a 3k-water link-cell box, 64 waters, and the two benchmark boxes c1
+    (30,000 atoms, 4^3 cells) and c2 (192,000 atoms, 8^3 cells)."""
     from tests.conftest import _small_link_cfg
     cfg = {"c0": configs.CONFIGS["c0"], "link3k": _small_link_cfg(),
            "w64": dict(configs.COMMON, n_waters=64, L=(64 / 0.0334) ** (1 / 3), seed=9, grid=16,
-                       cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35)}[name]
+                       cutOff=9.0, cutOn=7.5, timeStep=0.002, alpha=0.35),
+           "c1": configs.CONFIGS["c1"], "c2": configs.CONFIGS["c2"]}[name]
     wb = configs.build(cfg)
     o = O.OracleSystem(wb, cfg)
     assert o.keyLink == int(gl[f"{name}_keylink"])
