@@ -94,6 +94,21 @@ typedef struct {                       /* EWALD, global.h:311-316 */
     double prec, tol, tol1, alpha;
 } MDG_EWALD;
 
+#define MDG_FINAMELEN 512              /* FINAMELEN, global.h:61 */
+
+typedef struct {                       /* ATOM, global.h:165-169 */
+    char label[5], segn[5], resn[5];
+    int type, resi, ires;
+} MDG_ATOM;
+
+typedef struct {                       /* IO, global.h:186-193 */
+    FILE *confFile, *forfFile, *propFile, *restFile, *rconFile, *trajFile;
+    FILE *simuFile;
+    char confName[MDG_FINAMELEN], forfName[MDG_FINAMELEN], propName[MDG_FINAMELEN];
+    char restName[MDG_FINAMELEN], rconName[MDG_FINAMELEN], trajName[MDG_FINAMELEN];
+    char simuName[MDG_FINAMELEN];
+} MDG_IO;
+
 /* ---- library codes ---------------------------------------------------- */
 
 enum {
@@ -123,6 +138,7 @@ typedef struct {
     double nInCut;                     /* listed pairs with r <= rc this step */
     double ebond, eub, eang, edihe, eimpr;   /* bonded terms (ENERGY bond, ub, ang, dihe, impr);
                                               * elec / vdw include the 1-4 pairs */
+    double virbond, virub;             /* bonded virials (ENERGY virbond, virub) */
 } MDG_STEP;
 
 /* ---- context ---------------------------------------------------------- */
@@ -362,6 +378,81 @@ int mdg_list_stats(mdg_ctx *ctx, long out[3]);
  * bit-exact comparison with the reference list: writes up to cap pairs,
  * returns the count through *npairs. */
 int mdg_list_pairs(mdg_ctx *ctx, int *pairs_i, int *pairs_j, long cap, long *npairs);
+
+/* ---- reference input, output and set-up (host side, no device work) ---- */
+
+/* read_SIMU, io.c:29-726: defaults, then every keyword of io->simuName
+ * (case-insensitive, up to "end"), then the derived cut-off constants
+ * (io.c:714-723).  Returns 60 (no file), 61 (unknown keyword), 62 (bad
+ * value), 63 (missing value) where the reference exits with those codes. */
+int mdg_read_simu(MDG_IO *io, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_BATH *bath,
+                  MDG_NEIGH *neigh, MDG_EWALD *ewald, MDG_PBC *box);
+/* read_CONF, io.c:728-783: '*' title lines, the atom count, then
+ * "%d %d %4s %4s %lf %lf %lf %4s %d %lf" per atom; *atom, *x, *y, *z are
+ * malloc'd. */
+int mdg_read_conf(MDG_IO *io, MDG_PARAM *param, MDG_ATOM **atom, double **x, double **y,
+                  double **z);
+/* read_rest, io.c:785-870: the binary REST file mdg_write_rest writes. */
+int mdg_read_rest(MDG_IO *io, MDG_PARAM *param, MDG_ENERGY *ener, MDG_BATH *bath,
+                  MDG_ATOM **atom, double **x, double **y, double **z, double **vx, double **vy,
+                  double **vz, double **fx, double **fy, double **fz);
+/* read_FORF, io.c:872-1199: atoms, bonds, constraints, urey-bradley,
+ * angles, dihedrals, impropers, vdw sections (energies in kcal/mol scaled
+ * to internal units, angles in degrees to radians, eps = sqrt(eps*418.4));
+ * term arrays are malloc'd, per-atom arrays are the caller's. */
+int mdg_read_forf(MDG_IO *io, MDG_PARAM *param, MDG_ATOM atom[], MDG_CONSTRAINT **constList,
+                  MDG_BOND **bond, MDG_ANGLE **angle, MDG_DIHE **dihe, MDG_DIHE **impr,
+                  MDG_BOND **ub, double *eps, double *sig, double *eps14, double *sig14,
+                  double *mass, double *q, int *frozen, int *nAtConst);
+/* setup, init.c:248-616: masses of frozen atoms, consth constraints from
+ * bonds with a hydrogen, terms dropped among frozen or fully constrained
+ * atoms; the term arrays are reallocated (freed and set NULL when empty). */
+int mdg_setup(MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ATOM atom[], MDG_CONSTRAINT **constList,
+              MDG_BOND **bond, MDG_ANGLE **angle, MDG_DIHE **dihe, MDG_DIHE **impr,
+              MDG_BOND **ub, double mass[], double rmass[], int frozen[], int nAtConst[]);
+/* init_system's Coulomb constant by mdType, init.c:102-109. */
+double mdg_charge_const(int mdType);
+/* init_box, init.c:840-908. */
+void mdg_init_box(MDG_PBC *box);
+/* get_kinfromtemp with get_degfree, utils.c:551-582. */
+void mdg_get_kinfromtemp(MDG_PARAM *param, const MDG_PBC *box);
+/* image_update, utils.c:162-218 (host arrays, in place). */
+void mdg_image_update(const MDG_PARAM *param, const MDG_PBC *box, double x[], double y[],
+                      double z[]);
+/* kinetic, utils.c:502-515. */
+double mdg_kinetic(const MDG_PARAM *param, const double vx[], const double vy[],
+                   const double vz[], const double mass[]);
+/* init_spme's Ewald width, tolerances and mesh, spme.c:32-73: call before
+ * mdg_create so the context gets the reference's (power-of-two) mesh. */
+int mdg_spme_mesh(const MDG_CTRL *ctrl, const MDG_PARAM *param, MDG_EWALD *ewald,
+                  const MDG_PBC *box);
+/* init_vel + init_constvel, init.c:627-833, with the 3N standard normal
+ * deviates supplied by the caller (gauss: vx of every atom, then vy, then
+ * vz) in place of the reference's Box-Muller draws from its dSFMT stream.
+ * Returns 310 when the constraint projection does not converge. */
+int mdg_init_vel(const MDG_PARAM *param, const MDG_PBC *box, const MDG_CONSTRAINT constList[],
+                 const double x[], const double y[], const double z[], double vx[], double vy[],
+                 double vz[], const double mass[], const double rmass[], const int frozen[],
+                 const int nAtConst[], const double gauss[]);
+
+/* write_CONF, io.c:1201-1228 (to io->rconName). */
+int mdg_write_conf(MDG_IO *io, const MDG_PARAM *param, const MDG_ATOM atom[], const double x[],
+                   const double y[], const double z[]);
+/* write_prop, io.c:1230-1274: appends one 29-double record. */
+int mdg_write_prop(MDG_IO *io, const MDG_PARAM *param, const MDG_ENERGY *ener,
+                   const MDG_PBC *box);
+/* write_rest, io.c:1276-1369. */
+int mdg_write_rest(MDG_IO *io, const MDG_PARAM *param, const MDG_ENERGY *ener,
+                   const MDG_BATH *bath, const MDG_ATOM atom[], const double x[],
+                   const double y[], const double z[], const double vx[], const double vy[],
+                   const double vz[], const double fx[], const double fy[], const double fz[]);
+/* write_DCD_header, io.c:1371-1460. */
+int mdg_write_dcd_header(MDG_IO *io, const MDG_CTRL *ctrl, const MDG_PARAM *param,
+                         const MDG_PBC *box, const int frozen[]);
+/* write_DCD_traj, io.c:1462-1522: the first frame after the header holds
+ * every atom, later frames the free atoms only. */
+int mdg_write_dcd_traj(MDG_IO *io, const MDG_PARAM *param, const MDG_PBC *box, const double x[],
+                       const double y[], const double z[], const int frozen[]);
 
 #ifdef __cplusplus
 }

@@ -75,10 +75,22 @@ class MDG_EWALD(C.Structure):
                 ("tol1", C.c_double), ("alpha", C.c_double)]
 
 
+class MDG_ATOM(C.Structure):
+    _fields_ = [("label", C.c_char * 5), ("segn", C.c_char * 5), ("resn", C.c_char * 5),
+                ("type", C.c_int), ("resi", C.c_int), ("ires", C.c_int)]
+
+
+class MDG_IO(C.Structure):
+    _fields_ = ([(k, C.c_void_p) for k in
+                 "confFile forfFile propFile restFile rconFile trajFile simuFile".split()] +
+                [(k, C.c_char * 512) for k in
+                 "confName forfName propName restName rconName trajName simuName".split()])
+
+
 class MDG_STEP(C.Structure):
     _fields_ = [("step", C.c_int), ("listRebuilt", C.c_int)] + [(k, C.c_double) for k in (
         "pot kin tot elec vdw eDir eRec eCorr virelec virvdw virshake nInCut "
-        "ebond eub eang edihe eimpr").split()]
+        "ebond eub eang edihe eimpr virbond virub").split()]
 
 
 # Every entry point include/mdgpu.h declares (checked by the ABI test).
@@ -88,7 +100,11 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
            "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj "
            "mdg_batch_create mdg_batch_size mdg_autotune mdg_set_prune mdg_prune_stats "
-           "mdg_prune_skin mdg_get_launch mdg_set_launch mdg_set_bonded").split()
+           "mdg_prune_skin mdg_get_launch mdg_set_launch mdg_set_bonded "
+           "mdg_read_simu mdg_read_conf mdg_read_rest mdg_read_forf mdg_setup "
+           "mdg_charge_const mdg_init_box mdg_get_kinfromtemp mdg_image_update mdg_kinetic "
+           "mdg_spme_mesh mdg_init_vel mdg_write_conf mdg_write_prop mdg_write_rest "
+           "mdg_write_dcd_header mdg_write_dcd_traj").split()
 
 _LIB = None
 
@@ -124,8 +140,14 @@ def load():
     for f in EXPORTS:
         fn = getattr(L, f)
         if f not in ("mdg_create", "mdg_batch_create", "mdg_destroy", "mdg_last_error",
-                     "mdg_prune_skin"):
+                     "mdg_prune_skin", "mdg_charge_const", "mdg_kinetic", "mdg_init_box",
+                     "mdg_get_kinfromtemp", "mdg_image_update"):
             fn.restype = C.c_int
+    L.mdg_charge_const.restype = C.c_double
+    L.mdg_charge_const.argtypes = [C.c_int]
+    L.mdg_kinetic.restype = C.c_double
+    for f in ("mdg_init_box", "mdg_get_kinfromtemp", "mdg_image_update"):
+        getattr(L, f).restype = None
     dp = C.POINTER(C.c_double)
     L.mdg_backward.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
     L.mdg_backward_traj.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp]
@@ -554,3 +576,84 @@ def batch_records(recs, R: int):
 def records_to_dict(recs):
     keys = [f[0] for f in MDG_STEP._fields_]
     return {k: np.array([getattr(r, k) for r in recs]) for k in keys}
+
+
+# ---- reference decks (host-side readers of libmdgpu) ------------------------
+
+class _InDir:
+    def __init__(self, d):
+        self.d = d
+
+    def __enter__(self):
+        self.old = os.getcwd()
+        os.chdir(self.d)
+
+    def __exit__(self, *a):
+        os.chdir(self.old)
+
+
+def read_deck(dirpath: str, simu: str = "SIMU") -> dict:
+    """init_system's reading chain (init.c:30-172) through libmdgpu's host
+    readers, in dirpath: mdg_read_simu, the Coulomb constant,
+    mdg_read_rest (under "restart") or mdg_read_conf, mdg_read_forf,
+    mdg_setup, mdg_get_kinfromtemp, mdg_init_box, mdg_image_update.
+    Returns the structs and numpy copies of every array."""
+    L = load()
+    io, ctrl, param, bath = MDG_IO(), MDG_CTRL(), MDG_PARAM(), MDG_BATH()
+    neigh, ewald, box, ener = MDG_NEIGH(), MDG_EWALD(), MDG_PBC(), MDG_ENERGY()
+    io.simuName = simu.encode()
+    dpp = C.POINTER(C.c_double)
+    with _InDir(dirpath):
+        def chk(rc, what):
+            if rc:
+                raise MdgError(f"{what} -> {rc}")
+        chk(L.mdg_read_simu(C.byref(io), C.byref(ctrl), C.byref(param), C.byref(bath),
+                            C.byref(neigh), C.byref(ewald), C.byref(box)), "mdg_read_simu")
+        param.chargeConst = L.mdg_charge_const(ctrl.mdType)
+        atom = C.POINTER(MDG_ATOM)()
+        xyz = [dpp() for _ in range(9)]
+        if ctrl.keyRest:
+            chk(L.mdg_read_rest(C.byref(io), C.byref(param), C.byref(ener), C.byref(bath),
+                                C.byref(atom), *[C.byref(a) for a in xyz]), "mdg_read_rest")
+        else:
+            chk(L.mdg_read_conf(C.byref(io), C.byref(param), C.byref(atom),
+                                *[C.byref(a) for a in xyz[:3]]), "mdg_read_conf")
+        n = param.nAtom
+        arr = {k: np.zeros(n) for k in ("q", "mass", "rmass", "eps", "sig", "eps14", "sig14")}
+        frozen = np.zeros(n, np.int32)
+        nat = np.zeros(n, np.int32)
+        cons = C.POINTER(MDG_CONSTRAINT)()
+        from .molecules import MdgAngle, MdgBond, MdgDihe
+        bond, ub = C.POINTER(MdgBond)(), C.POINTER(MdgBond)()
+        angle = C.POINTER(MdgAngle)()
+        dihe, impr = C.POINTER(MdgDihe)(), C.POINTER(MdgDihe)()
+        chk(L.mdg_read_forf(C.byref(io), C.byref(param), atom, C.byref(cons), C.byref(bond),
+                            C.byref(angle), C.byref(dihe), C.byref(impr), C.byref(ub),
+                            P(arr["eps"]), P(arr["sig"]), P(arr["eps14"]), P(arr["sig14"]),
+                            P(arr["mass"]), P(arr["q"]), P(frozen), P(nat)), "mdg_read_forf")
+        chk(L.mdg_setup(C.byref(ctrl), C.byref(param), atom, C.byref(cons), C.byref(bond),
+                        C.byref(angle), C.byref(dihe), C.byref(impr), C.byref(ub),
+                        P(arr["mass"]), P(arr["rmass"]), P(frozen), P(nat)), "mdg_setup")
+        L.mdg_get_kinfromtemp(C.byref(param), C.byref(box))
+        L.mdg_init_box(C.byref(box))
+        pos = [np.ctypeslib.as_array(a, shape=(n,)).copy() if a else np.zeros(n) for a in xyz]
+        L.mdg_image_update(C.byref(param), C.byref(box), P(pos[0]), P(pos[1]), P(pos[2]))
+    atoms = [MDG_ATOM.from_buffer_copy(atom[i]) for i in range(n)]
+    atom_bytes = C.string_at(atom, n * C.sizeof(MDG_ATOM))
+
+    def rows(ptr, count, T):
+        return [T.from_buffer_copy(ptr[k]) for k in range(count)]
+    out = dict(io=io, ctrl=ctrl, param=param, bath=bath, neigh=neigh, ewald=ewald, box=box,
+               ener=ener, atoms=atoms, atom_bytes=atom_bytes, frozen=frozen, nAtConst=nat,
+               cons=rows(cons, param.nConst, MDG_CONSTRAINT), bond=rows(bond, param.nBond, MdgBond),
+               ub=rows(ub, param.nUb, MdgBond), angle=rows(angle, param.nAngle, MdgAngle),
+               dihe=rows(dihe, param.nDihedral, MdgDihe), impr=rows(impr, param.nImproper, MdgDihe))
+    for k, a in zip(("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"), pos):
+        out[k] = a
+    out.update(arr)
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    for p_ in [atom, cons, bond, ub, angle, dihe, impr] + xyz:
+        if p_:
+            libc.free(C.cast(p_, C.c_void_p))
+    return out

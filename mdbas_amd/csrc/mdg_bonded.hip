@@ -323,6 +323,21 @@ int mdg_bonded_upload(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nUb, co
                       int nImpr, const MDG_DIHE *impr, int n14, const int *list14,
                       const double *eps14, const double *sig14)
 {
+    // every atom index a kernel will dereference lies in [0, n)
+    const int n = ctx->n;
+    auto bad = [n](int a) { return a < 0 || a >= n; };
+    for (int i = 0; i < nBond; i++)
+        if (bad(bond[i].a) || bad(bond[i].b)) return MDG_EARG;
+    for (int i = 0; i < nUb; i++)
+        if (bad(ub[i].a) || bad(ub[i].b)) return MDG_EARG;
+    for (int i = 0; i < nAngle; i++)
+        if (bad(angle[i].a) || bad(angle[i].b) || bad(angle[i].c)) return MDG_EARG;
+    for (int i = 0; i < nDihe; i++)
+        if (bad(dihe[i].a) || bad(dihe[i].b) || bad(dihe[i].c) || bad(dihe[i].d)) return MDG_EARG;
+    for (int i = 0; i < nImpr; i++)
+        if (bad(impr[i].a) || bad(impr[i].b) || bad(impr[i].c) || bad(impr[i].d)) return MDG_EARG;
+    for (int i = 0; i < 2 * n14; i++)
+        if (bad(list14[i])) return MDG_EARG;
     MdgBonded &D = ctx->bd;
     RETCHK(upload_terms(ctx, &D.bond, &D.capBond, bond, nBond));
     RETCHK(upload_terms(ctx, &D.ub, &D.capUb, ub, nUb));

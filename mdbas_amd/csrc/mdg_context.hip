@@ -152,6 +152,17 @@ int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nCon
 {
     const int n = ctx->n;
     const bool sym = ctx->keyLink != 0;
+    auto bad = [n](int a) { return a < 0 || a >= n; };
+    for (int i = 0; i < nBond; i++)
+        if (bad(bond[i].a) || bad(bond[i].b)) return MDG_EARG;
+    for (int i = 0; i < nConst; i++)
+        if (bad(cons[i].a) || bad(cons[i].b)) return MDG_EARG;
+    for (int i = 0; i < nAngle; i++)
+        if (bad(angle[i].a) || bad(angle[i].b) || bad(angle[i].c)) return MDG_EARG;
+    for (int i = 0; i < nDihe; i++)
+        if (bad(dihe[i].a) || bad(dihe[i].b) || bad(dihe[i].c) || bad(dihe[i].d)) return MDG_EARG;
+    for (int i = 0; i < nImpr; i++)
+        if (bad(impr[i].a) || bad(impr[i].b) || bad(impr[i].c) || bad(impr[i].d)) return MDG_EARG;
     std::vector<std::vector<int>> ex(n), con(n);
     std::vector<int> l14;
     auto add = [&](int i, int j, bool dedupe) -> bool {

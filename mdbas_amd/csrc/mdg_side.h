@@ -214,6 +214,8 @@ __device__ inline void record_block(const RecArgs &R, int *ctrl, MDG_STEP *rec, 
     s.virelec = (v[2] + virRec + v[6]) + v[25];
     s.virvdw = v[3] + v[26];
     s.virshake = v[15];
+    s.virbond = v[21];
+    s.virub = v[22];
     s.nInCut = v[4];
     if (idx < R.recCap) rec[idx] = s;
     ctrl[CTRL_STEP] = idx + 1;
@@ -277,6 +279,7 @@ __device__ inline void b_record_block(const BRecArgs &R, int r, int rebuilt)
     s.vdw = v[1];
     s.pot = s.elec + s.vdw;
     s.ebond = s.eub = s.eang = s.edihe = s.eimpr = 0.;
+    s.virbond = s.virub = 0.;
     s.kin = 0.5 * v[7];
     s.tot = s.pot + s.kin;
     s.virelec = v[2] + (-(st0 + st3 + st5)) + v[6];

@@ -3,13 +3,14 @@
  * (test infrastructure only).
  *
  * oracle/Makefile compiles the reference's own src/list.c, ewald.c, vdw.c,
- * elec.c, utils.c, shake.c, integrate.c, init.c, energy.c, memory.c and
- * errors.c from where they lie under the reference tree, links them with
- * this file into oracle/_ref/libmdbas_ref.so, exports only the ref_*
- * symbols and lets the linker drop every function nothing here reaches.
- * That drops energy.c's ewald_energy (it calls spme_energy, which lives
- * in spme.c, which needs the absent fftw3.h) and init.c's file readers,
- * so no stand-in for any missing header or library is written.
+ * elec.c, utils.c, shake.c, integrate.c, init.c, energy.c, internal.c,
+ * memory.c, errors.c and io.c from where they lie under the reference
+ * tree, links them with this file into oracle/_ref/libmdbas_ref.so,
+ * exports only the ref_* symbols and lets the linker drop every function
+ * nothing here reaches.  That drops energy.c's ewald_energy (it calls
+ * spme_energy, which lives in spme.c, which needs the absent fftw3.h) and
+ * init.c's init_system (it calls init_spme), so no stand-in for any
+ * missing header or library is written.
  *
  * This file plays the role of the reference's mdbas.c for the objects it
  * links: it owns outFile and the structs the reference functions take.
@@ -342,3 +343,135 @@ void ref_nb14(ref_sys *s, int ewald, int elecType, int vdwType, const double *x,
     out[2] = s->ener.virelec;
     out[3] = s->ener.virvdw;
 }
+
+/* ---- reference input and output (io.c, init.c setup) -------------------- */
+
+#include "io.h"
+
+void setup(CTRL *ctrl, PARAM *param, ATOM atom[], CONSTRAINT **constList, BOND **bond,
+           ANGLE **angle, DIHE **dihe, DIHE **impr, BOND **ub, double mass[], double rmass[],
+           int frozen[], int nAtConst[]);
+
+typedef struct {
+    IO io;
+    CTRL ctrl;
+    PARAM param;
+    BATH bath;
+    NEIGH neigh;
+    EWALD ewald;
+    PBC box;
+    ENERGY ener;
+    ATOM *atom;
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz;
+    double *q, *eps, *sig, *eps14, *sig14, *mass, *rmass;
+    int *frozen, *nAtConst;
+    CONSTRAINT *cons;
+    BOND *bond, *ub;
+    ANGLE *angle;
+    DIHE *dihe, *impr;
+} ref_deck;
+
+/* init_system's reading chain (init.c:88-172) in the current directory:
+ * read_SIMU, the Coulomb constant, read_CONF (or read_rest under
+ * "restart"), read_FORF, setup, get_kinfromtemp, init_box, image_update. */
+ref_deck *ref_read_deck(const char *simu)
+{
+    ensure_out();
+    ref_deck *d = (ref_deck *)calloc(1, sizeof(ref_deck));
+    strncpy(d->io.simuName, simu, FINAMELEN - 1);
+    read_SIMU(&d->io, &d->ctrl, &d->param, &d->bath, &d->neigh, &d->ewald, &d->box);
+    if (d->ctrl.mdType == 1)
+        d->param.chargeConst = chgcharmm * kcaltoiu;
+    else if (d->ctrl.mdType == 2)
+        d->param.chargeConst = chgnamd * kcaltoiu;
+    else if (d->ctrl.mdType == 3)
+        d->param.chargeConst = chgdlpolyiu;
+    else
+        d->param.chargeConst = mu0 * X2(clight) * X2(elemchg) * NA * 0.1 / (angstr);
+    if (d->ctrl.keyRest)
+        read_rest(&d->io, &d->param, &d->ener, &d->bath, &d->atom, &d->x, &d->y, &d->z, &d->vx,
+                  &d->vy, &d->vz, &d->fx, &d->fy, &d->fz);
+    else
+        read_CONF(&d->io, &d->param, &d->atom, &d->x, &d->y, &d->z);
+    int n = d->param.nAtom;
+    double **arr[] = {&d->q, &d->eps, &d->sig, &d->eps14, &d->sig14, &d->mass, &d->rmass};
+    for (unsigned k = 0; k < sizeof arr / sizeof arr[0]; k++)
+        *arr[k] = (double *)calloc((size_t)n, sizeof(double));
+    d->frozen = (int *)calloc((size_t)n, sizeof(int));
+    d->nAtConst = (int *)calloc((size_t)n, sizeof(int));
+    read_FORF(&d->io, &d->param, d->atom, &d->cons, &d->bond, &d->angle, &d->dihe, &d->impr,
+              &d->ub, d->eps, d->sig, d->eps14, d->sig14, d->mass, d->q, d->frozen, d->nAtConst);
+    setup(&d->ctrl, &d->param, d->atom, &d->cons, &d->bond, &d->angle, &d->dihe, &d->impr,
+          &d->ub, d->mass, d->rmass, d->frozen, d->nAtConst);
+    get_kinfromtemp(&d->param, &d->box);
+    init_box(&d->box);
+    image_update(&d->param, &d->box, d->x, d->y, d->z);
+    return d;
+}
+
+/* The settings structs, byte for byte. */
+void ref_deck_structs(const ref_deck *d, void *ctrl, void *param, void *bath, void *neigh,
+                      void *ewald, void *box, void *ener, void *io)
+{
+    memcpy(ctrl, &d->ctrl, sizeof d->ctrl);
+    memcpy(param, &d->param, sizeof d->param);
+    memcpy(bath, &d->bath, sizeof d->bath);
+    memcpy(neigh, &d->neigh, sizeof d->neigh);
+    memcpy(ewald, &d->ewald, sizeof d->ewald);
+    memcpy(box, &d->box, sizeof d->box);
+    memcpy(ener, &d->ener, sizeof d->ener);
+    memcpy(io, &d->io, sizeof d->io);
+}
+
+/* Per-atom data: atoms, x y z, q eps sig eps14 sig14 mass rmass (double
+ * rows of n), frozen and nAtConst. */
+void ref_deck_atoms(const ref_deck *d, void *atom, double *xyz, double *par, int *frozen,
+                    int *nAtConst)
+{
+    size_t n = (size_t)d->param.nAtom;
+    memcpy(atom, d->atom, n * sizeof(ATOM));
+    memcpy(xyz, d->x, n * sizeof(double));
+    memcpy(xyz + n, d->y, n * sizeof(double));
+    memcpy(xyz + 2 * n, d->z, n * sizeof(double));
+    const double *p[] = {d->q, d->eps, d->sig, d->eps14, d->sig14, d->mass, d->rmass};
+    for (size_t k = 0; k < 7; k++) memcpy(par + k * n, p[k], n * sizeof(double));
+    memcpy(frozen, d->frozen, n * sizeof(int));
+    memcpy(nAtConst, d->nAtConst, n * sizeof(int));
+}
+
+/* Term tables after setup (counts in PARAM). */
+void ref_deck_terms(const ref_deck *d, void *cons, void *bond, void *ub, void *angle, void *dihe,
+                    void *impr)
+{
+    const PARAM *p = &d->param;
+    if (p->nConst) memcpy(cons, d->cons, (size_t)p->nConst * sizeof(CONSTRAINT));
+    if (p->nBond) memcpy(bond, d->bond, (size_t)p->nBond * sizeof(BOND));
+    if (p->nUb) memcpy(ub, d->ub, (size_t)p->nUb * sizeof(BOND));
+    if (p->nAngle) memcpy(angle, d->angle, (size_t)p->nAngle * sizeof(ANGLE));
+    if (p->nDihedral) memcpy(dihe, d->dihe, (size_t)p->nDihedral * sizeof(DIHE));
+    if (p->nImproper) memcpy(impr, d->impr, (size_t)p->nImproper * sizeof(DIHE));
+}
+
+/* The reference writers on the deck's atoms and the given state, into the
+ * names of io (paths relative to the current directory): RCON, one PROP
+ * record, REST, and a DCD header with two frames. */
+void ref_deck_write(ref_deck *d, const void *io, const void *ener, const double *xyz,
+                    const double *vxyz, const double *fxyz, double chiT, double chiP)
+{
+    size_t n = (size_t)d->param.nAtom;
+    memcpy(&d->io, io, sizeof d->io);
+    memcpy(&d->ener, ener, sizeof d->ener);
+    d->bath.chiT = chiT;
+    d->bath.chiP = chiP;
+    double *x = (double *)xyz, *y = x + n, *z = y + n;
+    double *vx = (double *)vxyz, *vy = vx + n, *vz = vy + n;
+    double *fx = (double *)fxyz, *fy = fx + n, *fz = fy + n;
+    write_CONF(&d->io, &d->param, d->atom, x, y, z);
+    write_prop(&d->io, &d->param, &d->ener, &d->box);
+    write_rest(&d->io, &d->param, &d->ener, &d->bath, d->atom, x, y, z, vx, vy, vz, fx, fy, fz);
+    write_DCD_header(&d->io, &d->ctrl, &d->param, &d->box, d->frozen);
+    write_DCD_traj(&d->io, &d->param, &d->box, x, y, z, d->frozen);
+    write_DCD_traj(&d->io, &d->param, &d->box, vx, vy, vz, d->frozen);
+}
+
+void ref_deck_set_step(ref_deck *d, int step) { d->param.step = step; }

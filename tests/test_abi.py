@@ -17,7 +17,7 @@ REF_INC = "/root/reference/include"
 STRUCTS = [("MDG_PBC", "PBC"), ("MDG_CTRL", "CTRL"), ("MDG_PARAM", "PARAM"), ("MDG_BATH", "BATH"),
            ("MDG_BOND", "BOND"), ("MDG_ANGLE", "ANGLE"), ("MDG_DIHE", "DIHE"),
            ("MDG_NEIGH", "NEIGH"), ("MDG_ENERGY", "ENERGY"), ("MDG_CONSTRAINT", "CONSTRAINT"),
-           ("MDG_EWALD", "EWALD")]
+           ("MDG_EWALD", "EWALD"), ("MDG_ATOM", "ATOM"), ("MDG_IO", "IO")]
 
 
 def declared():
