@@ -173,10 +173,10 @@ def reference_structs(wb, cfg: dict):
     ctrl.keyMd = 1
     ctrl.mdType = 1
     ctrl.keyConstH = 1
-    ctrl.keyEwald = 2
+    ctrl.keyEwald = cfg.get("keyEwald", 2)   # 0: plain cutoff Coulomb of elecType
     ctrl.keyAlpha = 1
     ctrl.keyMmax = 1
-    ctrl.elecType = 0
+    ctrl.elecType = cfg.get("elecType", 0)
     ctrl.vdwType = 2
     ctrl.integrator = 1
     ctrl.ens = 0

@@ -420,6 +420,10 @@ static int adj_ready(mdg_ctx *ctx)
         ctx->err = "call mdg_initial_energy first (the adjoint uses the device list)";
         return MDG_EARG;
     }
+    if (ctx->P.ewald != 2) {
+        ctx->err = "the adjoint differentiates the Ewald/SPME potential (keyEwald 2)";
+        return MDG_EUNSUPPORTED;
+    }
     return MDG_OK;
 }
 

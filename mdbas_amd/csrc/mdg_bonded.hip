@@ -444,7 +444,7 @@ extern "C" int mdg_set_bonded(mdg_ctx *ctx, const MDG_CTRL *ctrl, const MDG_PARA
                                 impr, &l14));
     MdgBonded &D = ctx->bd;
     D.scal14 = param->scal14;
-    D.ewald14 = 1;              // the device-resident step runs the Ewald path
+    D.ewald14 = ctx->P.ewald != 0;   // ewald14_energy, else nonbond14_energy (energy.c:151-165)
     D.elec14 = ctrl->elecType;
     D.vdw14 = ctrl->vdwType == 2;
     const int n14 = ctrl->keyNb14 ? (int)(l14.size() / 2) : 0;

@@ -82,6 +82,7 @@ static void params_from_mirror(const MDG_CTRL *ctrl, const MDG_PARAM *p, const M
     P.m3 = ew ? ew->m3max : 0;
     P.nbsp = ew ? ew->nbsp : 4;
     P.elecType = ctrl ? ctrl->elecType : 0;
+    P.ewald = ctrl ? ctrl->keyEwald : 2;
     P.vdwOn = ctrl ? (ctrl->vdwType == 2) : 1;
 }
 
@@ -385,7 +386,7 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
     ctx->m1 = ctx->P.m1;
     ctx->m2 = ctx->P.m2;
     ctx->m3 = ctx->P.m3;
-    if (ctx->m1 > 0 && ctx->m2 > 0 && ctx->m3 > 0)
+    if (ctx->P.ewald != 0 && ctx->m1 > 0 && ctx->m2 > 0 && ctx->m3 > 0)
         if (int e = mdg_spme_setup(ctx)) return fail(ctx, e);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, MDG_EHIP);
     if (status) *status = MDG_OK;
@@ -434,6 +435,10 @@ extern "C" mdg_ctx *mdg_batch_create(const MDG_CTRL *ctrl, const MDG_PARAM *para
         const MdgBox &B = T.hBox[r];
         if (B.type != 1 && B.type != 2) {
             ctx->err = "batch replicas need cubic or orthorhombic boxes";
+            return fail(ctx, MDG_EUNSUPPORTED);
+        }
+        if (ctx->P.ewald != 2) {
+            ctx->err = "a replica batch runs the Ewald/SPME method (keyEwald 2)";
             return fail(ctx, MDG_EUNSUPPORTED);
         }
         // makelist's builder choice (list.c:38-57): this path is the O(N^2) one

@@ -43,6 +43,9 @@ struct MdgParams {
     int maxCycle, update, linkRatio;
     int m1, m2, m3, nbsp;
     int elecType;
+    int ewald;                   // keyEwald: 2 Ewald real space + SPME; 0 the cutoff
+                                 // Coulomb functor of elecType (energy.c:216-345); 1
+                                 // (classic k-space sum) is not on the device path
     int vdwOn;                   // vdwType == VSWITCH (vdw full / none evaluate nothing)
     int spmePrio;                // 1: the binned SPME kernels raise their wave priority
                                  // (s_setprio); 2: the cluster pair kernel does

@@ -223,8 +223,15 @@ int mdg_excl_corr64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double *eo
 // FP32 constants of the mixed pair kernel.
 struct PairF {
     float alpha, c2a, qc, cutOn2, cutOff2, switch2, threeOn2;
+    float rcutOff, rcutOff2;         // 1/rc, 1/rc^2 (SHIFT1 / SHIFT2)
     double cutOff2d, L0, L1, L2, iL0, iL1, iL2;
 };
+
+// The electrostatics of a cluster pair kernel: the Ewald real-space term
+// (energy.c:496-503, keyEwald 2), none (elec noelec / full evaluate
+// nothing, energy.c:45-48), or the cutoff functors SHIFT1 / SHIFT2 /
+// SWITCH of elec.c:137-263 (keyEwald 0, energy.c:216-345).
+enum { PAIR_EWALD = 0, PAIR_NONE = 1, PAIR_SHIFT1 = 2, PAIR_SHIFT2 = 3, PAIR_SWITCH = 4 };
 
 // erfc(x) for x >= 0 as t Q(t) exp(-x^2), t = 1 / (1 + x / 2).  The
 // substitution is that of the Numerical Recipes "erfcc" form (Press et al.,
@@ -320,7 +327,7 @@ __device__ inline double dpp_sum8(double v)
 // the i-force sums over lanes a, a+8, ..., a+56 happen once per chunk.
 // Mask bit 8 b + a (= the lane) marks the listed pair (I_a, J_b).  FP64
 // throughout.
-template <bool BATCH>
+template <bool BATCH, int ELEC>
 __global__ void __launch_bounds__(256) k_pair_fp64(ClusterArgs A)
 {
     const int lane = threadIdx.x & 63;
@@ -356,11 +363,13 @@ __global__ void __launch_bounds__(256) k_pair_fp64(ClusterArgs A)
             double fjx = 0., fjy = 0., fjz = 0.;
             if (inCut) {
                 const double r = sqrt(r2), rt = 1. / r;
-                double dEl, dVw;
-                const double el = ewald_dir64(A.P.alpha, A.P.chargeConst * qid * A.sq[sj], r, rt,
-                                              dEl);
-                const double evw = vdw_switch64(A.P, eid * A.seps[sj], gid + A.ssig[sj], r2, rt,
-                                                dVw);
+                double dEl = 0., dVw = 0., el = 0., evw = 0.;
+                if (ELEC == PAIR_EWALD)
+                    el = ewald_dir64(A.P.alpha, A.P.chargeConst * qid * A.sq[sj], r, rt, dEl);
+                else if (ELEC != PAIR_NONE)
+                    el = coulomb64(A.P, ELEC, qid * A.sq[sj], r2, rt, dEl);
+                if (A.P.vdwOn)
+                    evw = vdw_switch64(A.P, eid * A.seps[sj], gid + A.ssig[sj], r2, rt, dVw);
                 const double fs = dEl + dVw;
                 fjx = fs * dx * rt;
                 fjy = fs * dy * rt;
@@ -468,11 +477,12 @@ __device__ inline long long mdg_f32_fix(float v)
     return (long long)(((unsigned long long)(unsigned int)(int)fl << 32) | lo);
 }
 
-template <bool BATCH>
+template <bool BATCH, int ELEC>
 __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsigned long long m,
                                         int J, double xi, double yi, double zi, float qi, float ei,
-                                        float gi, double L0, double L1, double L2, double iL0,
-                                        double iL1, double iL2, int a, int b, MxAcc &acc)
+                                        float gi, double qid, double L0, double L1, double L2,
+                                        double iL0, double iL1, double iL2, int a, int b,
+                                        MxAcc &acc)
 {
     double dx = j.x - xi, dy = j.y - yi, dz = j.z - zi;
     dx = fma(-L0, rint(dx * iL0), dx);
@@ -489,13 +499,9 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
     acc.nin += __builtin_popcountll(inm);
     const float r2f = in ? (float)r2 : A.F.cutOff2;
     const float qq = in ? qi * j.p.x : 0.f;
-    const float veps = in ? ei * j.p.y : 0.f;
+    const float veps = (in && A.P.vdwOn) ? ei * j.p.y : 0.f;
     const float rinv = __builtin_amdgcn_rsqf(r2f);
     const float r = r2f * rinv;
-    float ez;
-    const float erfcv = mdg_erfc_q(A.F.alpha * r, &ez);
-    const float eel = qq * erfcv * rinv;
-    const float del = -(eel + qq * A.F.c2a * ez) * rinv;
     const float s = (gi + j.p.z) * rinv;
     const float s2 = s * s;
     const float s6 = s2 * s2 * s2;
@@ -512,9 +518,67 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
     const float dS = 12.f * u * sw1 * (A.F.cutOn2 - u) * A.F.switch2;
     const float evdw = pv * S;
     const float dvdw = (pv * dS + dpv * S) * rinv;
+    float eel, del;
+    double eel64 = 0.;   // the cutoff functors' pair energy, see below
+    if (ELEC == PAIR_EWALD) {
+        float ez;
+        const float erfcv = mdg_erfc_q(A.F.alpha * r, &ez);
+        eel = qq * erfcv * rinv;
+        del = -(eel + qq * A.F.c2a * ez) * rinv;
+    } else if (ELEC == PAIR_NONE) {
+        eel = del = 0.f;
+    } else {
+        // pe * f and pe / r * (f' r - f), elec.c:137-263; SWITCH shares
+        // the LJ switch S (1 below cutOn2) and its r dS/dr.  The shifted
+        // forms are evaluated as squares, f = (1 - s)^2 with s = r / rc
+        // (SHIFT1) or r^2 / rc^2 (SHIFT2), the same polynomials as the
+        // reference's 1 - 2 s + s^2 without its cancellation near rc, where
+        // FP32 would lose the small f
+        const float pe = qq * rinv;
+        float f, df;
+        if (ELEC == PAIR_SHIFT1) {
+            const float s1 = r * A.F.rcutOff, t = 1.f - s1;
+            f = t * t;
+            df = -2.f * s1 * t;       // 2 (s^2 - s)
+        } else if (ELEC == PAIR_SHIFT2) {
+            const float s1 = r2f * A.F.rcutOff2, t = 1.f - s1;
+            f = t * t;
+            df = -4.f * s1 * t;       // 4 (s^2 - s)
+        } else {
+            f = S;
+            df = dS;
+        }
+        eel = pe * f;
+        del = pe * rinv * (df - f);
+        // The unscreened pair energies sum ~1e9 in magnitude to a total
+        // ~1e3 times smaller, and FP32 terms carry fixed rounding biases
+        // (the charge product of each atom-type pair, the approximate
+        // reciprocal square root), so the energy term is formed in FP64
+        // from the FP64 r^2 (the force keeps FP32)
+        if (in) {
+            double ri = (double)rinv;
+            ri = ri * (1.5 - 0.5 * r2 * ri * ri);          // Newton step: ~1e-14
+            ri = ri * (1.5 - 0.5 * r2 * ri * ri);
+            double f64;
+            if (ELEC == PAIR_SHIFT1) {
+                const double t = 1. - r2 * ri * A.P.rcutOff;
+                f64 = t * t;
+            } else if (ELEC == PAIR_SHIFT2) {
+                const double t = 1. - r2 * A.P.rcutOff2;
+                f64 = t * t;
+            } else if (r2 <= A.P.cutOn2) {
+                f64 = 1.;
+            } else {
+                const double sw1 = A.P.cutOff2 - r2;
+                f64 = sw1 * sw1 * (A.P.cutOff2 + 2. * r2 - 3. * A.P.cutOn2) * A.P.switch2;
+            }
+            eel64 = (qid * A.sq[J * MDG_CL + b]) * ri * f64;
+        }
+    }
     const float fs = (del + dvdw) * rinv;
     const float fjx = fs * (float)dx, fjy = fs * (float)dy, fjz = fs * (float)dz;
-    acc.ed += eel;
+    if (ELEC == PAIR_EWALD || ELEC == PAIR_NONE) acc.ed += eel;
+    else acc.ed += eel64;
     acc.ev += evdw;
     acc.vd += del * r;
     acc.vv += dvdw * r;
@@ -531,7 +595,7 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
     }
 }
 
-template <bool BATCH>
+template <bool BATCH, int ELEC>
 __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
 {
     // ahead of the SPME kernels beside it when the launch shape asks for it
@@ -588,6 +652,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
         const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
         const float4 pi = A.sp32[si];
         const float qi = A.F.qc * pi.x;
+        const double qid = ELEC >= PAIR_SHIFT1 ? A.P.chargeConst * A.sq[si] : 0.;
         MxAcc acc = {0.f, 0.f, 0.f, 0., 0., 0.f, 0.f, 0};
         // entries e (set 0) and e + 1 (set 1) alternate; indices past the
         // end are clamped to the last entry (loaded, never computed)
@@ -598,14 +663,14 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
             const int J1 = mdg_uload(eJ, min(e + 1, e1 - 1));
             const unsigned long long m1 = mdg_uload(eM, min(e + 1, e1 - 1));
             const MxJ j1 = mx_load(R, J1);
-            mx_pair<BATCH>(A, j0, m0, J0, xi, yi, zi, qi, pi.y, pi.z, L0, L1, L2, iL0, iL1, iL2, a,
-                           b, acc);
+            mx_pair<BATCH, ELEC>(A, j0, m0, J0, xi, yi, zi, qi, pi.y, pi.z, qid, L0, L1, L2, iL0,
+                                 iL1, iL2, a, b, acc);
             if (++e >= e1) break;
             J0 = mdg_uload(eJ, min(e + 1, e1 - 1));
             m0 = mdg_uload(eM, min(e + 1, e1 - 1));
             j0 = mx_load(R, J0);
-            mx_pair<BATCH>(A, j1, m1, J1, xi, yi, zi, qi, pi.y, pi.z, L0, L1, L2, iL0, iL1, iL2, a,
-                           b, acc);
+            mx_pair<BATCH, ELEC>(A, j1, m1, J1, xi, yi, zi, qi, pi.y, pi.z, qid, L0, L1, L2, iL0,
+                                 iL1, iL2, a, b, acc);
             if (++e >= e1) break;
         }
         // i forces: sums over the 8 j lanes of each i atom; lanes b = 0, 1,
@@ -753,6 +818,8 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     A.F.cutOff2 = (float)ctx->P.cutOff2;
     A.F.switch2 = (float)ctx->P.switch2;
     A.F.threeOn2 = (float)(3.0 * ctx->P.cutOn2);
+    A.F.rcutOff = (float)ctx->P.rcutOff;
+    A.F.rcutOff2 = (float)ctx->P.rcutOff2;
     A.F.cutOff2d = ctx->P.cutOff2;
     A.F.L0 = ctx->B.a1;
     A.F.L1 = ctx->B.type == 1 ? ctx->B.a1 : ctx->B.b2;
@@ -824,6 +891,34 @@ static int slot_pos(mdg_ctx *ctx, bool mixed)
     return MDG_OK;
 }
 
+// The kernel variant of the context's electrostatics.
+static int pair_elec(const mdg_ctx *ctx)
+{
+    if (ctx->P.ewald != 0) return PAIR_EWALD;
+    const int t = ctx->P.elecType;
+    return (t >= PAIR_SHIFT1 && t <= PAIR_SWITCH) ? t : PAIR_NONE;
+}
+
+template <bool BATCH>
+static void launch_pair(mdg_ctx *ctx, bool mixed, int nb, const ClusterArgs &A)
+{
+#define MDG_PAIR_LAUNCH(E)                                                                         \
+    do {                                                                                           \
+        if (mixed)                                                                                 \
+            hipLaunchKernelGGL((k_pair_mixed<BATCH, E>), dim3(nb), dim3(256), 0, ctx->stream, A);  \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_pair_fp64<BATCH, E>), dim3(nb), dim3(256), 0, ctx->stream, A);   \
+    } while (0)
+    switch (pair_elec(ctx)) {
+    case PAIR_EWALD: MDG_PAIR_LAUNCH(PAIR_EWALD); break;
+    case PAIR_SHIFT1: MDG_PAIR_LAUNCH(PAIR_SHIFT1); break;
+    case PAIR_SHIFT2: MDG_PAIR_LAUNCH(PAIR_SHIFT2); break;
+    case PAIR_SWITCH: MDG_PAIR_LAUNCH(PAIR_SWITCH); break;
+    default: MDG_PAIR_LAUNCH(PAIR_NONE); break;
+    }
+#undef MDG_PAIR_LAUNCH
+}
+
 int mdg_pair_cluster(mdg_ctx *ctx)
 {
     const bool mixed = ctx->precision == MDG_MIXED;
@@ -837,10 +932,7 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     if (A.nDisp) A.nDisp = nDisp;
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
-    if (mixed)
-        hipLaunchKernelGGL((k_pair_mixed<false>), dim3(nb), dim3(256), 0, ctx->stream, A);
-    else
-        hipLaunchKernelGGL((k_pair_fp64<false>), dim3(nb), dim3(256), 0, ctx->stream, A);
+    launch_pair<false>(ctx, mixed, nb, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
     return MDG_OK;
@@ -1001,10 +1093,7 @@ int mdg_b_pair(mdg_ctx *ctx)
     ClusterArgs A = cluster_args(ctx);
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
-    if (mixed)
-        hipLaunchKernelGGL((k_pair_mixed<true>), dim3(nb), dim3(256), 0, ctx->stream, A);
-    else
-        hipLaunchKernelGGL((k_pair_fp64<true>), dim3(nb), dim3(256), 0, ctx->stream, A);
+    launch_pair<true>(ctx, mixed, nb, A);
     LAUNCHCHK();
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[1], ctx->stream));
     return MDG_OK;

@@ -131,16 +131,22 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild, bool fin)
     }
     if (ctx->bt) return b_energy(ctx, rebuild);
     const int n = ctx->n;
+    const bool ew = ctx->P.ewald != 0;
+    if (ctx->P.ewald == 1) {
+        ctx->err = "classic k-space Ewald (keyEwald 1) is not on this path: use SPME";
+        return MDG_EUNSUPPORTED;
+    }
     // SPME on the second stream, beside the list rebuild and the pair kernel;
     // the branch enqueued first starts at the fork, the other one a few
     // microseconds later: the pair branch unless the tuner found the SPME
     // branch the longer one on steps without a rebuild
-    RETCHK(mdg_spme_fork(ctx));
+    if (ew) RETCHK(mdg_spme_fork(ctx));
+    else ctx->spmeForked = false;
     // (a rebuild step always starts the SPME branch first: the second
     // branch starts only after the first one's nodes, and the rebuild has
     // some thirty of them)
     const bool spmeFirst = ctx->tune.spmeFirst || rebuild;
-    if (spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_spme_device));
+    if (ew && spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_spme_device));
     if (rebuild) {
         if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[6], ctx->stream));
         RETCHK(mdg_build_cluster_list(ctx));
@@ -152,7 +158,7 @@ int mdg_device_energy(mdg_ctx *ctx, bool rebuild, bool fin)
     // bonded terms and 1-4 pairs (energy.c:157-189) into their int64
     // per-atom accumulators, added by the force assembly
     if (ctx->bd.on) RETCHK(mdg_bonded_forces(ctx, nullptr, nullptr, nullptr, ctx->bd.fix, false));
-    if (!spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_spme_device));
+    if (ew && !spmeFirst) RETCHK(mdg_spme_on(ctx, mdg_spme_device));
     if (ctx->spmeForked) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
     if (!fin) return MDG_OK;
     const bool mixed = ctx->precision == MDG_MIXED;
@@ -218,12 +224,13 @@ RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir, int nbExcl)
 {
     RecArgs R;
     R.fact = (TWOPI_R / ctx->B.vol) * ctx->P.chargeConst;
-    R.eSelf = ctx->eSelf;
-    R.eNet = ctx->eNet;
+    const bool ew = ctx->P.ewald != 0;   // else no reciprocal, self or net-charge terms
+    R.eSelf = ew ? ctx->eSelf : 0.;
+    R.eNet = ew ? ctx->eNet : 0.;
     R.recCap = ctx->recCap;
     R.nbPair = mdg_pair_blocks(ctx);
     R.nbExcl = nbExcl;
-    R.nbInfl = mdg_infl_blocks((long)ctx->m1 * ctx->m2 * ctx->mh);
+    R.nbInfl = ew ? mdg_infl_blocks((long)ctx->m1 * ctx->m2 * ctx->mh) : 0;
     R.nbKin = nbKin;
     R.nGrpVir = nGrpVir;
     R.pPair = ctx->epart + PART_PAIR;
@@ -403,7 +410,11 @@ extern "C" int mdg_download_state(mdg_ctx *ctx, double x[], double y[], double z
 
 extern "C" int mdg_initial_energy(mdg_ctx *ctx, MDG_STEP *rec)
 {
-    if (!ctx || !ctx->theta || !ctx->grpAtomOff || !ctx->exSymOff) return MDG_EARG;
+    if (!ctx || !ctx->grpAtomOff || !ctx->exSymOff) return MDG_EARG;
+    if (ctx->P.ewald != 0 && !ctx->theta) {
+        ctx->err = "no SPME mesh: the context was created without ewald m1max..m3max";
+        return MDG_EARG;
+    }
     if (ctx->bt) {
         RETCHK(ensure_records(ctx, 64));
         RETCHK(mdg_reset_step_state(ctx));

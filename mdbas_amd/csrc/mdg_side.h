@@ -68,6 +68,11 @@ struct FinArgs {
 // order), into mean[3]; every thread of the block takes part.
 __device__ inline void fin_mean(const FinArgs &A, double *mean, double (*red)[3])
 {
+    if (A.P.ewald == 0) {   // no reciprocal part (cutoff electrostatics)
+        if (threadIdx.x < 3) mean[threadIdx.x] = 0.;
+        __syncthreads();
+        return;
+    }
     double s[3] = {0., 0., 0.};
     rows_add<3>(A.gpart, A.nbGather, s);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -95,16 +100,22 @@ __device__ inline bool fin_atom(const FinArgs &A, int i, const double *mean, dou
                                 double &gx, double &gy, double &gz)
 {
     const int s = A.atomSlot[i];
+    const bool ew = A.P.ewald != 0;   // else cutoff electrostatics: pair forces only
     if (MIXED) {
-        gx = (double)A.fix[s] * (1.0 / MDG_FSCALE) + ((double)A.frx[i] - mean[0]);
-        gy = (double)A.fiy[s] * (1.0 / MDG_FSCALE) + ((double)A.fry[i] - mean[1]);
-        gz = (double)A.fiz[s] * (1.0 / MDG_FSCALE) + ((double)A.frz[i] - mean[2]);
+        gx = (double)A.fix[s] * (1.0 / MDG_FSCALE);
+        gy = (double)A.fiy[s] * (1.0 / MDG_FSCALE);
+        gz = (double)A.fiz[s] * (1.0 / MDG_FSCALE);
+        if (ew) {
+            gx += (double)A.frx[i] - mean[0];
+            gy += (double)A.fry[i] - mean[1];
+            gz += (double)A.frz[i] - mean[2];
+        }
     } else {
-        gx = (A.frx64[i] - mean[0]) + A.fdx[s];
-        gy = (A.fry64[i] - mean[1]) + A.fdy[s];
-        gz = (A.frz64[i] - mean[2]) + A.fdz[s];
+        gx = ew ? (A.frx64[i] - mean[0]) + A.fdx[s] : A.fdx[s];
+        gy = ew ? (A.fry64[i] - mean[1]) + A.fdy[s] : A.fdy[s];
+        gz = ew ? (A.frz64[i] - mean[2]) + A.fdz[s] : A.fdz[s];
     }
-    for (int k = A.exOff[i]; k < A.exOff[i + 1]; k++) {
+    for (int k = A.exOff[i]; ew && k < A.exOff[i + 1]; k++) {
         const int j = A.exLst[k];
         const int lo = min(i, j), hi = max(i, j);
         double dx = A.x[hi] - A.x[lo], dy = A.y[hi] - A.y[lo], dz = A.z[hi] - A.z[lo];

@@ -242,6 +242,46 @@ class OracleSystem:
                        eCorr=e.eCorr, virelec=e.virDir + vr + e.virCorr, virvdw=e.virVdw,
                        nInCut=e.nInCut, nListed=e.nListed, stressRec=st)
 
+    def nonbond_energy(self, elec_type, x, y, z):
+        """energy() on the plain cutoff path (keyEwald 0, energy.c:151-161):
+        zeroed forces, nonbond_energy with the coulomb functor of elec_type
+        (SHIFT1 2, SHIFT2 3, SWITCH 4) and the LJ switch over the current
+        list.  Returns forces (3,N) and a dict."""
+        n = self.wb.n
+        f = np.zeros((3, n))
+        e = [C.c_double(0) for _ in range(4)]
+        lib().oc_nonbond(C.byref(self.p), C.byref(self.b), int(elec_type), P(x), P(y), P(z),
+                         P(f[0]), P(f[1]), P(f[2]), P(self.q), P(self.eps), P(self.sig),
+                         P(self.neighList), P(self.neighPair), P(self.neighOrder),
+                         *[C.byref(v) for v in e])
+        elec, vdw, ve, vv = (v.value for v in e)
+        return f, dict(elec=elec, vdw=vdw, pot=elec + vdw, virelec=ve, virvdw=vv)
+
+    def md_run_plain(self, elec_type, nsteps, x, y, z, vx, vy, vz):
+        """The MD loop of mdbas.c:244-363 on the plain cutoff path: list
+        build, step-0 energy, then per step VV stage 1, the list update
+        every `update` steps, nonbond_energy, VV stage 2.  Returns (err, f,
+        etot, epot) with per-step energies (length nsteps + 1)."""
+        upd = self.cfg["update"]
+        self.build_list(x, y, z, build=True)
+        f, e = self.nonbond_energy(elec_type, x, y, z)
+        m = self.wb.mass
+        ek = 0.5 * float((m * (vx * vx + vy * vy + vz * vz)).sum())
+        et, ep = [e["pot"] + ek], [e["pot"]]
+        for step in range(1, nsteps + 1):
+            err, _, _, _ = self.vv(1, x, y, z, vx, vy, vz, f)
+            if err:
+                return err, f, np.array(et), np.array(ep)
+            if step % upd == 0:
+                self.build_list(x, y, z, build=False)
+            f, e = self.nonbond_energy(elec_type, x, y, z)
+            err, ek, _, _ = self.vv(2, x, y, z, vx, vy, vz, f)
+            if err:
+                return err, f, np.array(et), np.array(ep)
+            et.append(e["pot"] + ek)
+            ep.append(e["pot"])
+        return 0, f, np.array(et), np.array(ep)
+
     def bonded(self, x, y, z, f=None):
         """Bonded energies (internal.c:21-604) in energy()'s order; forces
         accumulate into f.  Returns (f, [bond, ub, ang, dihe, impr, virbond,

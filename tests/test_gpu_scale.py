@@ -127,7 +127,12 @@ def oracle_energy():
 @pytest.mark.parametrize("which", ["c1", "c2"])
 def test_device_fp64_energy_at_scale(which, c1_box, c2_box, oracle_energy):
     """FP64 device energy and forces of the uploaded state (list build,
-    SPME, real space, excluded-pair correction) against the oracle."""
+    SPME, real space, excluded-pair correction) against the oracle.  The
+    potential is a sum of large terms of both signs (at 30k atoms the
+    reciprocal part alone is -2.6e8 against a total of -5.8e6, and the
+    direct part sums 1e7 pair terms of both signs) summed in another order
+    than the oracle's, so the total and each part are held to 1e-12 of the
+    summed magnitudes of the parts; forces to 1e-12 relative."""
     wb = _box(which, c1_box, c2_box)
     fo, eo = oracle_energy(wb, which)
     ctx = M.Context(wb, precision=M.MDG_FP64)
@@ -137,10 +142,11 @@ def test_device_fp64_energy_at_scale(which, c1_box, c2_box, oracle_energy):
     ctx.close()
     say("fp64", which, r.pot, eo["pot"], r.eRec, eo["eRec"], relerr(f, fo))
     assert r.nInCut == eo["nInCut"]
-    assert r.pot == pytest.approx(eo["pot"], rel=1e-12)
-    assert r.eRec == pytest.approx(eo["eRec"], rel=1e-12)
-    assert r.eDir == pytest.approx(eo["eDir"], rel=1e-12)
-    assert r.vdw == pytest.approx(eo["vdw"], rel=1e-11)
+    scale = abs(eo["eDir"]) + abs(eo["eRec"]) + abs(eo["eCorr"]) + abs(eo["vdw"])
+    say("pot difference / scale", abs(r.pot - eo["pot"]) / scale)
+    assert abs(r.pot - eo["pot"]) <= 1e-12 * scale
+    for k in ("eRec", "eDir", "eCorr", "vdw"):
+        assert abs(getattr(r, k) - eo[k]) <= 1e-12 * scale, k
     assert relerr(f, fo) < 1e-12
 
 
@@ -222,8 +228,10 @@ def test_batch256_pair_set_bit_exact(replicas256):
 
 def test_batch256_energies_against_oracle(replicas256):
     """Step-0 energies and forces of every one of the 256 replicas: FP64
-    batch against the oracle to 1e-12, mixed batch forces against the
-    oracle's to RMS <= 1e-5 per replica."""
+    batch against the oracle to 1e-12 (energy: of the summed magnitudes of
+    its parts), mixed batch forces against the oracle's to RMS <= 1e-5 per
+    replica and the mixed potential to 1e-6 of the summed magnitudes (in
+    these small boxes the FP32 mesh energy is large beside the total)."""
     reps = replicas256
     out = {}
     for prec in (M.MDG_FP64, M.MDG_MIXED):
@@ -234,7 +242,7 @@ def test_batch256_energies_against_oracle(replicas256):
         out[prec] = ([r0[r].pot for r in range(b.R)], [r0[r].nInCut for r in range(b.R)],
                      [np.stack([f[k][r] for k in range(3)]) for r in range(b.R)])
         b.close()
-    worst = [0.0, 0.0, 0.0]
+    worst = [0.0, 0.0, 0.0, 0.0]
     for r, wb in enumerate(reps):
         o = O.OracleSystem(wb)
         x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
@@ -243,14 +251,17 @@ def test_batch256_energies_against_oracle(replicas256):
         p64, n64, f64 = (v[r] for v in out[M.MDG_FP64])
         pm, nm, fm = (v[r] for v in out[M.MDG_MIXED])
         assert n64 == eo["nInCut"] == nm, r
-        assert p64 == pytest.approx(eo["pot"], rel=1e-12), r
+        scale = abs(eo["eDir"]) + abs(eo["eRec"]) + abs(eo["eCorr"]) + abs(eo["vdw"])
+        assert abs(p64 - eo["pot"]) <= 1e-12 * scale, r
         assert relerr(f64, fo) < 1e-12, r
         worst[0] = max(worst[0], abs(p64 / eo["pot"] - 1))
         worst[1] = max(worst[1], relerr(f64, fo))
         worst[2] = max(worst[2], relerr(fm, fo))
+        worst[3] = max(worst[3], abs(pm - eo["pot"]) / scale)
         assert relerr(fm, fo) <= 1e-5, r
-        assert pm == pytest.approx(eo["pot"], rel=5e-6), r
-    say("batch256 worst: fp64 pot", worst[0], "fp64 f", worst[1], "mixed f rms", worst[2])
+        assert abs(pm - eo["pot"]) <= 1e-6 * scale, r
+    say("batch256 worst: fp64 pot", worst[0], "fp64 f", worst[1], "mixed f rms", worst[2],
+        "mixed pot / scale", worst[3])
 
 
 def test_batch256_fp64_trajectories_against_oracle(replicas256):

@@ -110,3 +110,20 @@ def test_driver_molecule_deck_against_oracle(mode, tmp_path):
     check(prop, rest, s, et, ep, 1e-10, 1e-9)
     # bonded energies are in the PROP record (bond, ang, ub, dihe, impr)
     assert (prop[:, 10] != 0).all() and (prop[:, 13] != 0).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["dropin", "device"])
+def test_driver_plain_cutoff_deck_against_oracle(mode, c0_oracle, tmp_path):
+    """The same water deck with "elec switch" (keyEwald 0: the cutoff
+    SWITCH functor through nonbond_energy, no SPME): 25 FP64 steps against
+    the oracle's plain-path MD loop."""
+    wb = c0_oracle[0]
+    o = O.OracleSystem(wb)
+    s = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    err, fo, et, ep = o.md_run_plain(4, 25, *s)
+    assert err == 0
+    d = str(tmp_path)
+    decks.write_deck(d, wb, configs.CONFIGS["c0"], 25, elec="switch")
+    prop, rest = run(d, "-m", mode, "-p", "fp64")
+    check(prop, rest, s, et, ep, 1e-10, 1e-9)
