@@ -122,8 +122,12 @@ enum {
     MDG_EUNSUPPORTED = 1002,           /* a setting this path does not implement */
     MDG_ELIST = 1003,                  /* device list capacity exceeded */
     MDG_EARG = 1004,                   /* bad argument (NULL, size mismatch) */
-    MDG_ESHAKE_ORDER = 1005            /* a constraint group re-diverged after converging */
+    MDG_ESHAKE_ORDER = 1005,           /* a constraint group re-diverged after converging */
+    MDG_ESCRATCH = 1006                /* input larger than a fixed scratch array */
 };
+
+/* Largest system mdg_nonbond_energy_noalloc takes (its fixed scratch). */
+#define MDG_NOALLOC_MAX_ATOMS 262144
 
 typedef struct mdg_ctx mdg_ctx;
 
@@ -262,6 +266,27 @@ int mdg_vv_integrate(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY 
                      double *y, double *z, double *vx, double *vy, double *vz, double *fx,
                      double *fy, double *fz, double *mass, double *rmass, int *nAtConst,
                      int stage);
+
+/* No counterpart in the reference: mdg_nonbond_energy's computation
+ * (nonbond_energy, energy.c:216-345) for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  It allocates nothing and takes no
+ * context: every array is the caller's DEVICE memory (x, y, z, q, eps, sig
+ * and fx, fy, fz of param->nAtom atoms; neighList, neighPair, neighOrder
+ * in the reference list format), forces and ener[0..3] (elec, vdw,
+ * virelec, virvdw) accumulate (+=) as in the reference, and the work is
+ * enqueued on stream (a hipStream_t; NULL for the default stream) without
+ * a host wait.  List offsets and partial sums go to a fixed device array of
+ * the library (one per GPU, held until the process ends), so at most
+ * MDG_NOALLOC_MAX_ATOMS atoms (else MDG_ESCRATCH), and calls that overlap
+ * on one GPU race; ordered calls do not.  ctrl->elecType selects the
+ * functor as ptr_coulomb does (energy.c:35-93), ctrl->vdwType the LJ
+ * switch. */
+int mdg_nonbond_energy_noalloc(const MDG_CTRL *ctrl, const MDG_PARAM *param, const MDG_PBC *box,
+                               const double *x, const double *y, const double *z, double *fx,
+                               double *fy, double *fz, const double *q, const double *eps,
+                               const double *sig, const int *neighList, const int *neighPair,
+                               const int *neighOrder, double *ener, void *stream);
 
 /* ---- device-resident MD ------------------------------------------------ */
 

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdgpu.so")
 
 MDG_FP64, MDG_MIXED = 0, 1
-ERRORS = {1001: "HIP/rocFFT error", 1002: "unsupported setting", 1003: "list capacity",
+ERRORS = {1006: "input larger than the fixed scratch", 1001: "HIP/rocFFT error", 1002: "unsupported setting", 1003: "list capacity",
           1004: "bad argument", 1005: "SHAKE group re-diverged", 311: "SHAKE not converged",
           110: "1-4 pair without dihedral", 111: "exclusion list last atom", 412: "link cells"}
 
@@ -104,7 +104,7 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_read_simu mdg_read_conf mdg_read_rest mdg_read_forf mdg_setup "
            "mdg_charge_const mdg_init_box mdg_get_kinfromtemp mdg_image_update mdg_kinetic "
            "mdg_spme_mesh mdg_init_vel mdg_write_conf mdg_write_prop mdg_write_rest "
-           "mdg_write_dcd_header mdg_write_dcd_traj").split()
+           "mdg_write_dcd_header mdg_write_dcd_traj mdg_nonbond_energy_noalloc").split()
 
 _LIB = None
 

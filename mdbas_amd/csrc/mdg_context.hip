@@ -86,6 +86,13 @@ static void params_from_mirror(const MDG_CTRL *ctrl, const MDG_PARAM *p, const M
     P.vdwOn = ctrl ? (ctrl->vdwType == 2) : 1;
 }
 
+// The same conversions for callers without a context.
+void mdg_box_from(const MDG_PBC *in, MdgBox &b) { box_from_mirror(in, b); }
+void mdg_params_from(const MDG_CTRL *ctrl, const MDG_PARAM *p, MdgParams &P)
+{
+    params_from_mirror(ctrl, p, nullptr, nullptr, P);
+}
+
 template <typename T>
 static int dmalloc(mdg_ctx *ctx, T **p, size_t n)
 {

@@ -490,6 +490,17 @@ __device__ inline void mdg_block_partials(double (&v)[NV], double *out)
 }
 
 // ---------------------------------------------------------------- internal API
+void mdg_box_from(const MDG_PBC *in, MdgBox &b);
+void mdg_params_from(const MDG_CTRL *ctrl, const MDG_PARAM *p, MdgParams &P);
+// the reference-list pair kernel (k_pair_ref64) on a stream: per-block
+// partials [eElec, eVdw, virElec, virVdw, nIn, nListed] into part
+int mdg_pair_ref64_launch(hipStream_t stream, const MdgParams &P, const MdgBox &B, int elecType,
+                          const double *x, const double *y, const double *z, const double *q,
+                          const double *eps, const double *sig, const int *neighList,
+                          const long *off, const int *order, double *fx, double *fy, double *fz,
+                          double *part);
+int mdg_reduce_partials_on(hipStream_t stream, const double *part, int nblocks, int nv,
+                           double *out);
 int mdg_alloc_state(mdg_ctx *ctx);
 int mdg_build_exclusions(mdg_ctx *ctx, int nBond, const MDG_BOND *bond, int nConst,
                          const MDG_CONSTRAINT *cons, int nAngle, const MDG_ANGLE *angle,

@@ -167,42 +167,39 @@ k_excl64(MdgParams P, MdgBox B, const double *__restrict__ x, const double *__re
     mdg_block_partials<2>(acc, part);
 }
 
+int mdg_pair_ref64_launch(hipStream_t stream, const MdgParams &P, const MdgBox &B, int elecType,
+                          const double *x, const double *y, const double *z, const double *q,
+                          const double *eps, const double *sig, const int *neighList,
+                          const long *off, const int *order, double *fx, double *fy, double *fz,
+                          double *part)
+{
+    const int nb = nblk(P.n, 256);
+#define MDG_REF_LAUNCH(E)                                                                          \
+    hipLaunchKernelGGL(k_pair_ref64<E>, dim3(nb), dim3(256), 0, stream, P, B, x, y, z, q, eps,    \
+                       sig, neighList, off, order, fx, fy, fz, part)
+    switch (elecType) {
+    case 0: MDG_REF_LAUNCH(0); break;
+    case 1: MDG_REF_LAUNCH(1); break;
+    case 2: MDG_REF_LAUNCH(2); break;
+    case 3: MDG_REF_LAUNCH(3); break;
+    case 4: MDG_REF_LAUNCH(4); break;
+    default: return MDG_EUNSUPPORTED;
+    }
+#undef MDG_REF_LAUNCH
+    return hipGetLastError() == hipSuccess ? MDG_OK : MDG_EHIP;
+}
+
 int mdg_pair_ref64(mdg_ctx *ctx, int elecType, double *fx, double *fy, double *fz,
                    const int *neighList, const long *off, const int *order, double *eout)
 {
     const int n = ctx->n, nb = nblk(n, 256);
     double *part = ctx->epart + PART_REF;
-    switch (elecType) {
-    case 0:
-        hipLaunchKernelGGL(k_pair_ref64<0>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
-                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
-                           order, fx, fy, fz, part);
-        break;
-    case 2:
-        hipLaunchKernelGGL(k_pair_ref64<2>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
-                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
-                           order, fx, fy, fz, part);
-        break;
-    case 3:
-        hipLaunchKernelGGL(k_pair_ref64<3>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
-                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
-                           order, fx, fy, fz, part);
-        break;
-    case 4:
-        hipLaunchKernelGGL(k_pair_ref64<4>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
-                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
-                           order, fx, fy, fz, part);
-        break;
-    case 1:
-        hipLaunchKernelGGL(k_pair_ref64<1>, dim3(nb), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
-                           ctx->x, ctx->y, ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off,
-                           order, fx, fy, fz, part);
-        break;
-    default:
-        ctx->err = "unsupported elecType";
-        return MDG_EUNSUPPORTED;
-    }
-    LAUNCHCHK();
+    const int st = mdg_pair_ref64_launch(ctx->stream, ctx->P, ctx->B, elecType, ctx->x, ctx->y,
+                                         ctx->z, ctx->q, ctx->eps, ctx->sig, neighList, off, order,
+                                         fx, fy, fz, part);
+    if (st == MDG_EUNSUPPORTED) ctx->err = "unsupported elecType";
+    if (st == MDG_EHIP) ctx->err = "kernel launch failed (k_pair_ref64)";
+    RETCHK(st);
     RETCHK(mdg_reduce_partials(ctx, part, nb, 6, eout));
     return MDG_OK;
 }

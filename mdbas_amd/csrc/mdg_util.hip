@@ -171,6 +171,13 @@ __global__ void k_reduce_partials(const double *__restrict__ part, int nblocks, 
     }
 }
 
+int mdg_reduce_partials_on(hipStream_t stream, const double *part, int nblocks, int nv,
+                           double *out)
+{
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nv), dim3(256), 0, stream, part, nblocks, nv, out);
+    return hipGetLastError() == hipSuccess ? MDG_OK : MDG_EHIP;
+}
+
 int mdg_reduce_partials(mdg_ctx *ctx, const double *part, int nblocks, int nv, double *out)
 {
     hipLaunchKernelGGL(k_reduce_partials, dim3(nv), dim3(256), 0, ctx->stream, part, nblocks,
