@@ -50,13 +50,19 @@ def rel(a, b):
 
 
 def sync_fn():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            return torch.cuda.synchronize
-    except Exception:
-        pass
-    return lambda: None
+    """hipDeviceSynchronize of the HIP runtime libmdgpu.so links (the one
+    runtime in the process; every GPU call of the benchmark goes through
+    the library)."""
+    import ctypes
+    import mdbas_amd as M
+    M.load()
+    rt = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib",
+                                  "libamdhip64.so.7"))
+
+    def sync():
+        if rt.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    return sync
 
 
 def cpu_baseline(boxes, budget_s=25.0, max_steps=10):

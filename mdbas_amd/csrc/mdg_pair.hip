@@ -481,11 +481,25 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
                                         double iL0, double iL1, double iL2, int a, int b,
                                         MxAcc &acc)
 {
-    double dx = j.x - xi, dy = j.y - yi, dz = j.z - zi;
-    dx = fma(-L0, rint(dx * iL0), dx);
-    dy = fma(-L1, rint(dy * iL1), dy);
-    dz = fma(-L2, rint(dz * iL2), dz);
-    const double r2 = dx * dx + dy * dy + dz * dz;
+    double dx, dy, dz, r2;
+    {
+#pragma clang fp contract(off)
+        dx = j.x - xi;
+        dy = j.y - yi;
+        dz = j.z - zi;
+        // minimum image: the shift L * n is exact for |n| <= 1 (slot
+        // positions are never more than a box edge apart) and so is the
+        // difference when n != 0 (Sterbenz), so rint's ties-to-even and the
+        // reference's nint (utils.c:606-609) give the same displacement
+        // wherever |dx| < L / 2 - rounding, i.e. for every pair that can be
+        // inside the cutoff
+        dx = dx - L0 * rint(dx * iL0);
+        dy = dy - L1 * rint(dy * iL1);
+        dz = dz - L2 * rint(dz * iL2);
+        // the reference's r^2 (utils.c:76), each product rounded, so the
+        // cutoff test decides as energy.c:500 does
+        r2 = (dx * dx + dy * dy) + dz * dz;
+    }
     const unsigned long long inm = m & __builtin_amdgcn_ballot_w64(r2 <= A.F.cutOff2d);
     // an entry with no pair inside the cutoff adds exact zeros everywhere
     // (zero coefficients, no j-force atomics): skip it, wave-uniformly.
