@@ -257,10 +257,18 @@ int mdg_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
                const int neighPair[], const int neighOrder[], const int neighList14[],
                int **exclList, const int exclPair[]);
 
-/* vv_integrate, integrate.c:1479-1634 / integrate.h:72-76, NVE: stage 1
+/* vv_integrate, integrate.c:1479-1634 / integrate.h:72-76.  NVE: stage 1
  * = half kick, drift, SHAKE-R; stage 2 = half kick, SHAKE-V, kinetic
  * energy, kinetic stress, wrap.  SHAKE keeps the reference's global
- * stopping rule (shake.c:243-338) without a host round trip. */
+ * stopping rule (shake.c:243-338) without a host round trip.  ctrl->ens
+ * selects the thermostat or barostat as the reference's switch does
+ * (integrate.c:1490-1511): NVT_B vv_nvt_b, NPT_B vv_npt_b, NVT_H vv_nvt_h,
+ * NPT_H vv_npt_h (integrate.c:1636-2618), reading bath and the targets
+ * param->kinTemp0 / temp0 / press0 and ener->virpot, and updating bath
+ * (chiT, chiP), ener (kin, virshake, conint, consv) and, for NPT, the box
+ * (every derived member, as scale_box / vv_scale_box do); later calls on
+ * the context follow that box.  Per-atom work runs on the device, the bath
+ * scalars on the host.  A changing box needs an MDG_FP64 context. */
 int mdg_vv_integrate(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_ENERGY *ener,
                      MDG_PBC *box, MDG_BATH *bath, MDG_CONSTRAINT constList[], double *x,
                      double *y, double *z, double *vx, double *vy, double *vz, double *fx,

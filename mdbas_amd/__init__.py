@@ -398,6 +398,39 @@ class Context:
                                           int(stage)), "mdg_vv_integrate")
         return e
 
+    def vv_integrate_ens(self, st, stage, x, y, z, vx, vy, vz, f):
+        """vv_integrate with a thermostat/barostat (st["ens"] 1..4): st is
+        the state dict of oracle.ens_state (bath, targets, the ENERGY
+        members, box vectors, nine stresses), updated in place."""
+        b9 = [float(v) for v in st["box9"]]
+        pbc = self.pbc
+        (pbc.a1, pbc.a2, pbc.a3, pbc.b1, pbc.b2, pbc.b3, pbc.c1, pbc.c2, pbc.c3) = b9
+        self.L.mdg_init_box(C.byref(pbc))
+        for k in range(9):
+            setattr(pbc, f"stress{k + 1}", float(st["stress"][k]))
+        bath = MDG_BATH(tauT=st["tauT"], chiT=st["chiT"], tauP=st["tauP"], chiP=st["chiP"])
+        e = MDG_ENERGY(virpot=st["virpot"], conint=st["conint"], consv=st["consv"], kin=st["kin"],
+                       virshake=st["virshake"])
+        self.param.kinTemp0, self.param.temp0 = st["kinTemp0"], st["temp0"]
+        self.param.press0 = st["press0"]
+        self.ctrl.ens = int(st["ens"])
+        try:
+            self._chk(self.L.mdg_vv_integrate(self.h, C.byref(self.ctrl), C.byref(self.param),
+                                              C.byref(e), C.byref(pbc), C.byref(bath), self.cons,
+                                              P(x), P(y), P(z), P(vx), P(vy), P(vz), P(f[0]),
+                                              P(f[1]), P(f[2]), P(self.wb.mass), P(self.wb.rmass),
+                                              P(self.wb.nAtConst), int(stage)),
+                      "mdg_vv_integrate")
+        finally:
+            self.ctrl.ens = 0
+        st["chiT"], st["chiP"] = bath.chiT, bath.chiP
+        for k in ("conint", "consv", "kin", "virshake"):
+            st[k] = getattr(e, k)
+        st["box9"] = np.array([pbc.a1, pbc.a2, pbc.a3, pbc.b1, pbc.b2, pbc.b3, pbc.c1, pbc.c2,
+                               pbc.c3])
+        st["stress"] = np.array([getattr(pbc, f"stress{k + 1}") for k in range(9)])
+        return e
+
     # ---- device-resident ------------------------------------------------
     def upload(self, x, y, z, vx, vy, vz):
         self._chk(self.L.mdg_upload_state(self.h, P(x), P(y), P(z), P(vx), P(vy), P(vz)),

@@ -88,6 +88,12 @@ static void params_from_mirror(const MDG_CTRL *ctrl, const MDG_PARAM *p, const M
 
 // The same conversions for callers without a context.
 void mdg_box_from(const MDG_PBC *in, MdgBox &b) { box_from_mirror(in, b); }
+
+static void box_vectors(const MDG_PBC *in, double v[9])
+{
+    const double w[9] = {in->a1, in->a2, in->a3, in->b1, in->b2, in->b3, in->c1, in->c2, in->c3};
+    for (int k = 0; k < 9; k++) v[k] = w[k];
+}
 void mdg_params_from(const MDG_CTRL *ctrl, const MDG_PARAM *p, MdgParams &P)
 {
     params_from_mirror(ctrl, p, nullptr, nullptr, P);
@@ -350,6 +356,7 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
     ctx->n = param->nAtom;
     params_from_mirror(ctrl, param, ewald, neigh, ctx->P);
     box_from_mirror(box, ctx->B);
+    box_vectors(box, ctx->boxKey);
     if (ctx->B.type < 1 || ctx->B.type > 3) {
         ctx->err = "a periodic box is required";
         return fail(ctx, MDG_EUNSUPPORTED);
@@ -582,6 +589,47 @@ static void charge_sums(mdg_ctx *ctx, const double q[])
         : 0.;
 }
 
+// A box the caller changed between calls (the NPT integrators rescale it):
+// the link-cell counts of the next list update (list.c:2380-2396), the
+// net-charge energy (spme.c:680) and the influence function follow it.
+int mdg_refresh_box(mdg_ctx *ctx, const MDG_PBC *box)
+{
+    if (!box || ctx->bt) return MDG_OK;
+    double v[9];
+    box_vectors(box, v);
+    if (box->type == ctx->B.type && std::equal(v, v + 9, ctx->boxKey)) return MDG_OK;
+    if (box->type != ctx->B.type) {
+        ctx->err = "the box type changed";
+        return MDG_EUNSUPPORTED;
+    }
+    if (ctx->precision == MDG_MIXED) {
+        // the mixed kernels keep box-dependent state on the device path
+        ctx->err = "a changing box needs an FP64 context";
+        return MDG_EUNSUPPORTED;
+    }
+    box_from_mirror(box, ctx->B);
+    std::copy(v, v + 9, ctx->boxKey);
+    if (ctx->keyLink) {
+        const double cutnb = ctx->P.cutOff + ctx->P.delr;
+        const int r = ctx->P.linkRatio;
+        ctx->nlc[0] = (int)(ctx->B.pa * (double)r / cutnb);
+        ctx->nlc[1] = (int)(ctx->B.pb * (double)r / cutnb);
+        ctx->nlc[2] = (int)(ctx->B.pc * (double)r / cutnb);
+        if (ctx->nlc[0] < 2 * r + 1 || ctx->nlc[1] < 2 * r + 1 || ctx->nlc[2] < 2 * r + 1) {
+            ctx->err = "the box shrank below three link cells along an axis";
+            return 412;
+        }
+        ctx->ncell = ctx->nlc[0] * ctx->nlc[1] * ctx->nlc[2];
+    }
+    ctx->eNet = ctx->B.vol > 0 && ctx->P.alpha > 0
+        ? -(0.5 * PI_C * ctx->P.chargeConst) *
+              ((ctx->systq / ctx->P.alpha) * (ctx->systq / ctx->P.alpha) / ctx->B.vol)
+        : 0.;
+    if (ctx->theta) RETCHK(mdg_spme_rebox(ctx));
+    mdg_invalidate_graphs(ctx);
+    return sync(ctx);
+}
+
 // Per-replica self and net-charge energies (spme.c:376-389, 680 with each
 // replica's own charges and volume).
 static int batch_charge_sums(mdg_ctx *ctx, const double q[])
@@ -784,8 +832,8 @@ extern "C" int mdg_makelist(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_
                             int **neighPair, int **neighOrder, int **neighList14,
                             int ***exclList, int **exclPair)
 {
-    (void)box;
     if (!ctx || !ctrl || !param || !neigh || !x || !y || !z) return MDG_EARG;
+    if (box) RETCHK(mdg_refresh_box(ctx, box));
     const int n = ctx->n;
     if (param->nAtom != n) return MDG_EARG;
     if ((param->nBond > 0 && !bond) || (param->nAngle > 0 && !angle) ||
@@ -1006,8 +1054,9 @@ extern "C" int mdg_ewald_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, 
                                 const int neighList[], const int neighPair[],
                                 const int neighOrder[], int **exclList, const int exclPair[])
 {
-    (void)param; (void)ewald; (void)box;
+    (void)param; (void)ewald;
     if (!ctx || !ctrl || !ener || !x || !fx || !q || !neighPair || !neighOrder) return MDG_EARG;
+    if (box) RETCHK(mdg_refresh_box(ctx, box));
     double e[4];
     RETCHK(ewald_on_device(ctx, ctrl, x, y, z, fx, fy, fz, q, eps, sig, neighList, neighPair,
                            neighOrder, exclList, exclPair, e));
@@ -1026,8 +1075,9 @@ extern "C" int mdg_nonbond_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param
                                   const double sig[], const int neighList[],
                                   const int neighPair[], const int neighOrder[])
 {
-    (void)param; (void)box;
+    (void)param;
     if (!ctx || !ctrl || !ener || !x || !fx || !q || !neighPair || !neighOrder) return MDG_EARG;
+    if (box) RETCHK(mdg_refresh_box(ctx, box));
     double e[4];
     RETCHK(nonbond_on_device(ctx, ctrl, x, y, z, fx, fy, fz, q, eps, sig, neighList, neighPair,
                              neighOrder, e));
@@ -1053,6 +1103,7 @@ extern "C" int mdg_energy(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, MDG_EN
     if (!ctx || !ctrl || !param || !ener || !box || !x || !fx || !q || !neighPair || !neighOrder)
         return MDG_EARG;
     if (param->nAtom != ctx->n) return MDG_EARG;
+    RETCHK(mdg_refresh_box(ctx, box));
     if ((param->nBond > 0 && !bond) || (param->nUb > 0 && !ub) || (param->nAngle > 0 && !angle) ||
         (param->nDihedral > 0 && !dihe) || (param->nImproper > 0 && !impr))
         return MDG_EARG;
@@ -1120,12 +1171,11 @@ extern "C" int mdg_vv_integrate(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, 
                                 double *vz, double *fx, double *fy, double *fz, double *mass,
                                 double *rmass, int *nAtConst, int stage)
 {
-    (void)bath;
     if (!ctx || !param || !ener || !x || !vx || !fx || !mass || !rmass) return MDG_EARG;
-    if (ctrl && ctrl->ens != 0) {
-        ctx->err = "only the NVE ensemble is on this path";
-        return MDG_EUNSUPPORTED;
-    }
+    const int ens = ctrl ? (int)ctrl->ens : 0;
+    if (ens < 0 || ens > 4) return MDG_EARG;
+    if (ens != 0 && (!bath || !box)) return MDG_EARG;
+    if (box) RETCHK(mdg_refresh_box(ctx, box));
     const int n = ctx->n;
     ctx->P.timeStep = param->timeStep;
     ctx->P.rTimeStep = 1. / param->timeStep;
@@ -1149,6 +1199,17 @@ extern "C" int mdg_vv_integrate(mdg_ctx *ctx, MDG_CTRL *ctrl, MDG_PARAM *param, 
     RETCHK(h2d(ctx, ctx->fy, (const double *)fy, n));
     RETCHK(h2d(ctx, ctx->fz, (const double *)fz, n));
     double kin[8] = {0}, vs = 0.;
+    if (ens != 0) {
+        // integrate.c:1495-1506: the thermostat and barostat integrators
+        RETCHK(mdg_vv_ensemble(ctx, param, ener, box, bath, ens, stage));
+        RETCHK(d2h(ctx, x, ctx->x, n));
+        RETCHK(d2h(ctx, y, ctx->y, n));
+        RETCHK(d2h(ctx, z, ctx->z, n));
+        RETCHK(d2h(ctx, vx, ctx->vx, n));
+        RETCHK(d2h(ctx, vy, ctx->vy, n));
+        RETCHK(d2h(ctx, vz, ctx->vz, n));
+        return sync(ctx);
+    }
     int st = mdg_vv_stage(ctx, stage, ctx->fx, ctx->fy, ctx->fz, kin, &vs, true);
     if (st) return st;
     RETCHK(d2h(ctx, x, ctx->x, n));

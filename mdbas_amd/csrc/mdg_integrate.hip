@@ -256,17 +256,30 @@ __device__ inline void vv1_a_body(const ShakeArgs &A, int g, unsigned long long 
     Grp G;
     grp_load(A, g, G);
     const double dt = A.P.timeStep;
-    dd_compute(A, G);
-    for (int k = 0; k < G.na; k++) {
-        int i = G.at[k];
-        G.vx[k] += 0.5 * dt * A.fx[i] * G.rm[k];
-        G.vy[k] += 0.5 * dt * A.fy[i] * G.rm[k];
-        G.vz[k] += 0.5 * dt * A.fz[i] * G.rm[k];
-    }
-    for (int k = 0; k < G.na; k++) {
-        G.x[k] += dt * G.vx[k];
-        G.y[k] += dt * G.vy[k];
-        G.z[k] += dt * G.vz[k];
+    if (A.ddStored)
+        dd_load(A, G);
+    else
+        dd_compute(A, G);
+    if (!A.noKick)
+        for (int k = 0; k < G.na; k++) {
+            int i = G.at[k];
+            G.vx[k] += 0.5 * dt * A.fx[i] * G.rm[k];
+            G.vy[k] += 0.5 * dt * A.fy[i] * G.rm[k];
+            G.vz[k] += 0.5 * dt * A.fz[i] * G.rm[k];
+        }
+    if (A.npt) {
+        // integrate.c:1876-1878 (com zero), 2398-2400
+        for (int k = 0; k < G.na; k++) {
+            G.x[k] = A.dscale * (G.x[k] - A.com0) + dt * G.vx[k] + A.com0;
+            G.y[k] = A.dscale * (G.y[k] - A.com1) + dt * G.vy[k] + A.com1;
+            G.z[k] = A.dscale * (G.z[k] - A.com2) + dt * G.vz[k] + A.com2;
+        }
+    } else {
+        for (int k = 0; k < G.na; k++) {
+            G.x[k] += dt * G.vx[k];
+            G.y[k] += dt * G.vy[k];
+            G.z[k] += dt * G.vz[k];
+        }
     }
     if (G.nc == 0) {
         grp_store(A, G, true);
@@ -372,13 +385,17 @@ __device__ inline void vv2_a_body(const ShakeArgs &A, int g, unsigned long long 
     Grp G;
     grp_load(A, g, G);
     const double dt = A.P.timeStep;
-    dd_compute(A, G);
-    for (int k = 0; k < G.na; k++) {
-        int i = G.at[k];
-        G.vx[k] += 0.5 * dt * A.fx[i] * G.rm[k];
-        G.vy[k] += 0.5 * dt * A.fy[i] * G.rm[k];
-        G.vz[k] += 0.5 * dt * A.fz[i] * G.rm[k];
-    }
+    if (A.ddStored)
+        dd_load(A, G);
+    else
+        dd_compute(A, G);
+    if (!A.noKick)
+        for (int k = 0; k < G.na; k++) {
+            int i = G.at[k];
+            G.vx[k] += 0.5 * dt * A.fx[i] * G.rm[k];
+            G.vy[k] += 0.5 * dt * A.fy[i] * G.rm[k];
+            G.vz[k] += 0.5 * dt * A.fz[i] * G.rm[k];
+        }
     if (G.nc == 0) {
         grp_store(A, G, false);
         for (int w = 0; w < SHK_MASKW; w++) bits[w] = ~0ull;
@@ -470,7 +487,9 @@ __global__ void __launch_bounds__(128) k_vv2_b(ShakeArgs A)
             acc[6] += m * G.vz[k] * G.vz[k];
             // image_update, utils.c:162-218
             double xs = G.x[k], ys = G.y[k], zs = G.z[k];
-            if (A.B.type == 1) {
+            if (A.noWrap) {
+                // wrapped by a later kernel, with the box the step ends in
+            } else if (A.B.type == 1) {
                 xs = xs - A.B.a1 * mdg_nint(xs * A.B.u1);
                 ys = ys - A.B.a1 * mdg_nint(ys * A.B.u1);
                 zs = zs - A.B.a1 * mdg_nint(zs * A.B.u1);
@@ -600,8 +619,12 @@ int mdg_vir_rows(const mdg_ctx *ctx)
 // kinetic partials to epart + PART_KIN (stage 2); with sync the sums land
 // in scal[S_VIRSHAKE] and scal[S_KIN..S_KIN+6] and are returned.
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
-                 double *virshake_out, bool sync, const FinArgs *fin)
+                 double *virshake_out, bool sync, const FinArgs *fin, const VvOpts *opt)
 {
+    if (opt && (!sync || ctx->bt)) {
+        ctx->err = "integrator options need a synchronous single-system stage";
+        return MDG_EARG;
+    }
     ShakeArgs A;
     A.P = ctx->P;
     A.B = ctx->B;
@@ -659,7 +682,17 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.fix = nullptr;
     A.fstr = 0;
     A.dispPart = nullptr;
-    const bool w3 = ctx->shakeW3;
+    if (opt) {
+        A.ddStored = opt->ddStored;
+        A.noKick = opt->noKick;
+        A.npt = opt->npt;
+        A.noWrap = opt->noWrap;
+        A.dscale = opt->dscale;
+        A.com0 = opt->com[0];
+        A.com1 = opt->com[1];
+        A.com2 = opt->com[2];
+    }
+    const bool w3 = ctx->shakeW3 && !opt;             // options run the generic kernels
     const bool m3 = w3 && ctx->tune.shkSpread == 0;   // one molecule per lane
     const bool batch = ctx->bt != nullptr;
     if (stage == 1 && ctx->slotsInStage1 && w3 && !batch && !sync) {
@@ -687,7 +720,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         A.mask = ctx->bt->masks + (stage == 1 ? 0 : (long)ctx->bt->R * SHK_MASKW);
     }
     const int bs = w3 ? 256 : 128;
-    const int nb = mdg_vv_blocks(ctx);
+    const int nb = w3 ? mdg_vv_blocks(ctx) : nblk(ctx->nGrp, 128);
     // The device-resident step resets the mask in the kernels that run
     // before each stage and reduces the partials in its record kernel; the
     // drop-in call does both here.
@@ -739,7 +772,7 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         // SHAKE virial: sum of the per-group (generic) or per-block (three-site)
         // sums, fixed order
         if (sync)
-            RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, mdg_vir_rows(ctx), 1,
+            RETCHK(mdg_reduce_partials(ctx, ctx->grpVir, w3 ? mdg_vir_rows(ctx) : ctx->nGrp, 1,
                                        ctx->scal + S_VIRSHAKE));
     } else {
         if (w3) {

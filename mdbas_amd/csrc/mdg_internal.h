@@ -292,6 +292,10 @@ struct mdg_ctx {
     void *fftWork = nullptr;
     size_t fftWorkSize = 0;
     double eSelf = 0., eNet = 0., systq = 0.;
+    // the box vectors (a1..c3) the derived state above (link-cell counts,
+    // net-charge energy, influence function) was made for
+    double boxKey[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double *ensSave = nullptr;   // positions and velocities [6][n] across a barostat trial cycle
 
     // timing
     bool profiling = false;
@@ -338,7 +342,8 @@ enum {
     S_REF = 25,                                      // drop-in pair loop (6)
     S_REFCORR = 31,                                  // drop-in correction (2)
     S_BONDED = 33,                                   // bonded and 1-4 sums (11)
-    S_COUNT = 48
+    S_ENS = 48,                                      // thermostat sums (7)
+    S_COUNT = 56
 };
 
 // regions of ctx->epart (per-block partial sums), each PART_R doubles
@@ -566,8 +571,23 @@ int mdg_vir_rows(const mdg_ctx *ctx);   // rows of grpVir the record sums
 struct FinArgs;
 // fin (stage 2 of three-site molecules): the first stage-2 kernel also
 // assembles the forces that k_finalize would (mdg_side.h)
+// Options of the thermostat and barostat integrators (mdg_ensemble.hip): a
+// stage on the generic-group kernels with the ShakeArgs fields of the same
+// names; needs sync and a single system.
+struct VvOpts {
+    int ddStored = 0, noKick = 0, npt = 0, noWrap = 0;
+    double dscale = 1., com[3] = {0., 0., 0.};
+};
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
-                 double *virshake_out, bool sync, const FinArgs *fin = nullptr);
+                 double *virshake_out, bool sync, const FinArgs *fin = nullptr,
+                 const VvOpts *opt = nullptr);
+// thermostats and barostats behind the drop-in vv_integrate (mdg_ensemble.hip)
+int mdg_vv_ensemble(mdg_ctx *ctx, const MDG_PARAM *param, MDG_ENERGY *ener, MDG_PBC *box,
+                    MDG_BATH *bath, int ens, int stage);
+// follow a box the caller changed (an NPT step): derived vectors, link-cell
+// counts, net-charge energy and the SPME influence function
+int mdg_refresh_box(mdg_ctx *ctx, const MDG_PBC *box);
+int mdg_spme_rebox(mdg_ctx *ctx);
 
 // replica batch pieces
 int mdg_b_build_list(mdg_ctx *ctx);                  // mdg_list.hip

@@ -655,9 +655,9 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
             L0 = Bx.a1;
             L1 = Bx.type == 1 ? Bx.a1 : Bx.b2;
             L2 = Bx.type == 1 ? Bx.a1 : Bx.c3;
-            iL0 = 1.0 / L0;
-            iL1 = 1.0 / L1;
-            iL2 = 1.0 / L2;
+            iL0 = Bx.u1;
+            iL1 = Bx.type == 1 ? Bx.u1 : Bx.v2;
+            iL2 = Bx.type == 1 ? Bx.u1 : Bx.w3;
         }
         const int si = I * MDG_CL + a;
         const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
@@ -835,9 +835,11 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     A.F.L0 = ctx->B.a1;
     A.F.L1 = ctx->B.type == 1 ? ctx->B.a1 : ctx->B.b2;
     A.F.L2 = ctx->B.type == 1 ? ctx->B.a1 : ctx->B.c3;
-    A.F.iL0 = 1.0 / A.F.L0;
-    A.F.iL1 = 1.0 / A.F.L1;
-    A.F.iL2 = 1.0 / A.F.L2;
+    // the reference's reciprocal edges (u1, v2, w3 of init_box), as dist
+    // multiplies by them (utils.c:44-54)
+    A.F.iL0 = ctx->B.u1;
+    A.F.iL1 = ctx->B.type == 1 ? ctx->B.u1 : ctx->B.v2;
+    A.F.iL2 = ctx->B.type == 1 ? ctx->B.u1 : ctx->B.w3;
     A.nclu = C.nclu;
     A.sx = ctx->sx;
     A.sy = ctx->sy;

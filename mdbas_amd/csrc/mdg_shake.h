@@ -57,6 +57,13 @@ struct ShakeArgs {
     long long *fix;
     long fstr;
     double *dispPart;
+    // thermostat and barostat integrators (generic-group kernels only):
+    // ddStored takes the constraint vectors from dd, formed at the call's
+    // entry, instead of from the positions the kernel loads; noKick skips
+    // the half kick; npt drifts as x = dscale (x - com) + dt v + com; noWrap
+    // leaves stage 2's image_update to a later kernel
+    int ddStored = 0, noKick = 0, npt = 0, noWrap = 0;
+    double dscale = 1., com0 = 0., com1 = 0., com2 = 0.;
 };
 
 // Molecule of this lane's quad, or nGrp for a quad left idle.

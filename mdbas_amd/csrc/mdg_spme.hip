@@ -980,6 +980,19 @@ static int rocfft_init(mdg_ctx *ctx)
     return MDG_OK;
 }
 
+// The influence function for the context's current box (after a barostat
+// step), into the buffers mdg_spme_setup made.
+int mdg_spme_rebox(mdg_ctx *ctx)
+{
+    const long H = (long)ctx->m1 * ctx->m2 * ctx->mh;
+    std::vector<double> th;
+    theta_host(ctx->B, ctx->m1, ctx->m2, ctx->m3, ctx->P.nbsp, ctx->P.alpha, th);
+    std::vector<float> thf(th.begin(), th.end());
+    HIPCHK(hipMemcpy(ctx->theta, th.data(), H * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->theta32, thf.data(), H * sizeof(float), hipMemcpyHostToDevice));
+    return MDG_OK;
+}
+
 int mdg_spme_setup(mdg_ctx *ctx)
 {
     RETCHK(rocfft_init(ctx));

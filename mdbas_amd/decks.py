@@ -80,8 +80,11 @@ def labels_of(box) -> list:
 def write_simu(path: str, box, cfg: dict, nsteps: int, *, restart: bool = True,
                prop: int = 1, traj: int = 0, resconf: int = 0, printout: int = 1,
                seed: int | None = None, nb14: float | None = None, elec: str = "spme",
-               mesh: int | None = None):
-    """The SIMU deck for box with the workload settings of cfg."""
+               mesh: int | None = None, ensemble: str = "nve", taut: float | None = None,
+               taup: float | None = None, pressure: float | None = None):
+    """The SIMU deck for box with the workload settings of cfg; ensemble is
+    the SIMU value ("nve", "nvt beren", "npt hoover", ...), with the bath's
+    taut / taup (ps) and the target pressure (bar) when given."""
     m = mesh if mesh is not None else cfg["grid"]
     rec = m // 2          # recvec r gives a 2r mesh (spme.c:54-73) for power-of-two r
     lines = ["charmm",
@@ -102,9 +105,12 @@ def write_simu(path: str, box, cfg: dict, nsteps: int, *, restart: bool = True,
              f"shake tol {float(cfg['tolShake'])!r}",
              f"shake maxcycle {cfg['maxCycle']}",
              "integrator velocity",
-             "ensemble nve",
+             f"ensemble {ensemble}",
              f"temperature {float(cfg.get('temp', 300.0))!r}",
              f"print {printout}"]
+    for key, val in (("taut", taut), ("taup", taup), ("pressure", pressure)):
+        if val is not None:
+            lines.append(f"{key} {float(val)!r}")
     if nb14 is not None:
         lines.append(f"nb14 {float(nb14)!r}")
     if seed is not None:

@@ -431,6 +431,9 @@ int main(int argc, char **argv)
         fail(MDG_EUNSUPPORTED, "classic Ewald sum (elec ewald): use elec spme", NULL);
     }
     if (R.ctrl.integrator != 1) fail(MDG_EUNSUPPORTED, "leap-frog integrator", NULL);
+    if (device && R.ctrl.ens != 0)
+        fail(MDG_EUNSUPPORTED, "device-resident steps are NVE: run thermostats with -m dropin",
+             NULL);
 
     int st = 0;
     R.ctx = mdg_create(&R.ctrl, p, &R.box, &R.ewald, &R.neigh, device ? precision : MDG_FP64, &st);

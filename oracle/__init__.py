@@ -42,6 +42,34 @@ class OEnergy(C.Structure):
                 ("stressRec", C.c_double * 6), ("nInCut", C.c_int), ("nListed", C.c_int)]
 
 
+class OBath(C.Structure):
+    _fields_ = [("ens", C.c_int)] + [(k, C.c_double) for k in (
+        "tauT tauP chiT chiP kinTemp0 temp0 press0 virpot conint consv kin virshake".split())] + [
+        ("stress", C.c_double * 9)]
+
+
+# Thermostat/barostat state handed between VV stages: BATH, the PARAM
+# targets, the ENERGY members the integrators touch, the box vectors
+# (a1..c3) and the nine box stresses.
+ENS_SCALARS = "tauT tauP chiT chiP kinTemp0 temp0 press0 virpot conint consv kin virshake".split()
+
+
+def ens_state(ens, box9, **kw):
+    st = {k: float(kw.get(k, 0.)) for k in ENS_SCALARS}
+    st["ens"] = int(ens)
+    st["box9"] = np.array(box9, np.float64)
+    st["stress"] = np.zeros(9)
+    return st
+
+
+def box_from9(box9, btype: int = 1) -> OBox:
+    b = OBox()
+    b.type = btype
+    (b.a1, b.a2, b.a3, b.b1, b.b2, b.b3, b.c1, b.c2, b.c3) = (float(v) for v in box9)
+    lib().oc_init_box(C.byref(b))
+    return b
+
+
 def build(ref: bool = True) -> None:
     """Compile liboracle.so (and the reference library when its sources are
     present) with the committed Makefile."""
@@ -87,7 +115,7 @@ def ref():
         for f in ("ref_makelist", "ref_list_count", "ref_list_copy", "ref_excl_copy",
                   "ref_vdw_switch", "ref_coulomb", "ref_nonbond", "ref_ewald_rec", "ref_vv",
                   "ref_constvel", "ref_set_bonded", "ref_n14", "ref_list14_copy", "ref_bonded",
-                  "ref_nb14"):
+                  "ref_nb14", "ref_vv_ens"):
             getattr(R, f).argtypes = None
         _REF = R
     return _REF
@@ -357,6 +385,62 @@ class OracleSystem:
                               P(wb.nAtConst), C.byref(ek), C.byref(vs), C.byref(nc))
         return err, ek.value, vs.value, nc.value
 
+    def vv_ens(self, st, stage, x, y, z, vx, vy, vz, f):
+        """oc_vv_ens: one stage of a thermostat/barostat integrator; st
+        (ens_state) is updated in place, box included."""
+        wb = self.wb
+        b = box_from9(st["box9"], self.b.type)
+        s = OBath()
+        s.ens = st["ens"]
+        for k in ENS_SCALARS:
+            setattr(s, k, st[k])
+        for k in range(9):
+            s.stress[k] = st["stress"][k]
+        nc = C.c_int(0)
+        err = lib().oc_vv_ens(C.byref(self.p), C.byref(b), C.byref(s), int(stage), len(wb.ca),
+                              P(wb.ca), P(wb.cb), P(wb.rc2), P(x), P(y), P(z), P(vx), P(vy),
+                              P(vz), P(f[0]), P(f[1]), P(f[2]), P(wb.mass), P(wb.rmass),
+                              P(wb.nAtConst), C.byref(nc))
+        for k in ENS_SCALARS:
+            st[k] = getattr(s, k)
+        st["stress"] = np.array(s.stress[:])
+        st["box9"] = np.array([b.a1, b.a2, b.a3, b.b1, b.b2, b.b3, b.c1, b.c2, b.c3])
+        return err
+
+    def md_run_ens(self, st, nsteps, x, y, z, vx, vy, vz):
+        """The MD loop of mdbas.c:244-363 on the Ewald path with the
+        thermostat/barostat of st (ens_state, updated in place): consv
+        cleared at each step's start, VV stage 1, the list update every
+        `update` steps, energy() with virpot = virelec + virvdw
+        (energy.c:196), VV stage 2; the box follows st between calls.
+        Returns (err, f, records) with one record per step from step 0:
+        tot, pot, kin, consv, volume."""
+        upd = self.cfg["update"]
+        btype = self.b.type
+        self.build_list(x, y, z, build=True)
+        f, e = self.energy(x, y, z)
+        st["virpot"] = e["virelec"] + e["virvdw"]
+        m = self.wb.mass
+        ek = 0.5 * float((m * (vx * vx + vy * vy + vz * vz)).sum())
+        st["kin"] = ek
+        rec = [(e["pot"] + ek, e["pot"], ek, st["consv"], self.b.vol)]
+        for step in range(1, nsteps + 1):
+            st["consv"] = 0.
+            err = self.vv_ens(st, 1, x, y, z, vx, vy, vz, f)
+            if err:
+                return err, f, np.array(rec)
+            self.b = box_from9(st["box9"], btype)
+            if step % upd == 0:
+                self.build_list(x, y, z, build=False)
+            f, e = self.energy(x, y, z)
+            st["virpot"] = e["virelec"] + e["virvdw"]
+            err = self.vv_ens(st, 2, x, y, z, vx, vy, vz, f)
+            if err:
+                return err, f, np.array(rec)
+            self.b = box_from9(st["box9"], btype)
+            rec.append((e["pot"] + st["kin"], e["pot"], st["kin"], st["consv"], self.b.vol))
+        return 0, f, np.array(rec)
+
     def md_run(self, nsteps, x, y, z, vx, vy, vz):
         n = self.wb.n
         f = np.zeros((3, n))
@@ -471,6 +555,22 @@ class RefSystem:
         ref().ref_vv(self.h, C.c_int(stage), P(x), P(y), P(z), P(vx), P(vy), P(vz),
                      P(f[0]), P(f[1]), P(f[2]), P(self.wb.mass), P(self.wb.rmass), P(out))
         return out
+
+    def vv_ens(self, st, stage, x, y, z, vx, vy, vz, f):
+        """vv_integrate with st["ens"] through the compiled reference; st
+        (ens_state) is updated in place."""
+        a = np.zeros(31)
+        a[0] = st["ens"]
+        for i, k in enumerate(ENS_SCALARS):
+            a[1 + i] = st[k]
+        a[13:22] = st["box9"]
+        a[22:31] = st["stress"]
+        ref().ref_vv_ens(self.h, C.c_int(stage), P(a), P(x), P(y), P(z), P(vx), P(vy), P(vz),
+                         P(f[0]), P(f[1]), P(f[2]), P(self.wb.mass), P(self.wb.rmass))
+        for i, k in enumerate(ENS_SCALARS):
+            st[k] = float(a[1 + i])
+        st["box9"] = a[13:22].copy()
+        st["stress"] = a[22:31].copy()
 
     def constvel(self, x, y, z, vx, vy, vz):
         ref().ref_constvel(self.h, P(x), P(y), P(z), P(vx), P(vy), P(vz), P(self.wb.mass))

@@ -1184,6 +1184,334 @@ int oc_vv_nve(const oc_param *p, const oc_box *b, int stage,
 }
 
 /* ---------------------------------------------------------------- */
+/* Thermostats and barostats (integrate.c:1636-2618).               */
+/* ---------------------------------------------------------------- */
+
+#define OC_WATERCOMP 0.007372    /* global.h:56 */
+#define OC_RBOLTZUI 0.83144621   /* global.h:47 */
+#define OC_HOOVER 5              /* integrate.c:2176 */
+
+/* kinetic (utils.c:502-515) */
+static double ens_kin(int n, const double *vx, const double *vy, const double *vz,
+                      const double *m)
+{
+    double ek = 0.;
+    for (int i = 0; i < n; i++) ek += m[i] * (SQ(vx[i]) + SQ(vy[i]) + SQ(vz[i]));
+    return ek * 0.5;
+}
+
+/* stress_kinetic (utils.c:517-542) added to the nine box stresses the way
+ * every stage 2 does (integrate.c:1603-1611); the SHAKE stress of stage 2
+ * is zero. */
+static void ens_stress_kin(int n, const double *vx, const double *vy, const double *vz,
+                           const double *m, double st[9])
+{
+    double s[6] = {0., 0., 0., 0., 0., 0.};
+    for (int i = 0; i < n; i++) {
+        s[0] += m[i] * SQ(vx[i]);
+        s[1] += m[i] * vx[i] * vy[i];
+        s[2] += m[i] * vx[i] * vz[i];
+        s[3] += m[i] * SQ(vy[i]);
+        s[4] += m[i] * vy[i] * vz[i];
+        s[5] += m[i] * SQ(vz[i]);
+    }
+    const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int k = 0; k < 9; k++) st[k] += s[map[k]];
+}
+
+static void ens_scale(int n, double *vx, double *vy, double *vz, double f)
+{
+    for (int i = 0; i < n; i++) {
+        vx[i] *= f;
+        vy[i] *= f;
+        vz[i] *= f;
+    }
+}
+
+static void ens_kick(int n, double dt, double *vx, double *vy, double *vz, const double *fx,
+                     const double *fy, const double *fz, const double *rmass)
+{
+    for (int i = 0; i < n; i++) {
+        vx[i] += 0.5 * dt * fx[i] * rmass[i];
+        vy[i] += 0.5 * dt * fy[i] * rmass[i];
+        vz[i] += 0.5 * dt * fz[i] * rmass[i];
+    }
+}
+
+/* scale_box (utils.c:291-374) from saved vectors, vv_scale_box
+ * (utils.c:376-459) in place: both rescale the nine vectors and derive the
+ * rest exactly as init_box does. */
+static void ens_box(oc_box *b, const double cell0[9], double s)
+{
+    b->a1 = s * cell0[0]; b->a2 = s * cell0[1]; b->a3 = s * cell0[2];
+    b->b1 = s * cell0[3]; b->b2 = s * cell0[4]; b->b3 = s * cell0[5];
+    b->c1 = s * cell0[6]; b->c2 = s * cell0[7]; b->c3 = s * cell0[8];
+    oc_init_box(b);
+}
+
+static void ens_cell(const oc_box *b, double c[9])
+{
+    c[0] = b->a1; c[1] = b->a2; c[2] = b->a3;
+    c[3] = b->b1; c[4] = b->b2; c[5] = b->b3;
+    c[6] = b->c1; c[7] = b->c2; c[8] = b->c3;
+}
+
+/* One Nose-Hoover thermostat-barostat sub-step of vv_npt_h
+ * (integrate.c:2296-2382 = 2471-2545): half thermostat, barostat, half
+ * thermostat.  vs is the integrator's own virshake local. */
+static void ens_hoover(int n, oc_bath *s, double *vx, double *vy, double *vz, const double *m,
+                       double qmass, double pmass, double cqts, double chts, double vs,
+                       double *volume)
+{
+    for (int half = 0; half < 3; half++) {
+        if (half == 1) {
+            s->chiP += 0.5 * chts * (((2.0 * s->kin - s->virpot - vs) -
+                                      3.0 * s->press0 * *volume) / pmass - s->chiP * s->chiT);
+            const double g = exp(-chts * s->chiP);
+            ens_scale(n, vx, vy, vz, g);
+            s->kin *= SQ(g);
+            *volume *= exp(3.0 * chts * s->chiP);
+            s->chiP += 0.5 * chts * (((2.0 * s->kin - s->virpot - vs) -
+                                      3.0 * s->press0 * *volume) / pmass - s->chiP * s->chiT);
+            continue;
+        }
+        s->kin = ens_kin(n, vx, vy, vz, m);
+        s->chiT += 0.5 * cqts * (2.0 * (s->kin - s->kinTemp0) + pmass * SQ(s->chiP) -
+                                 OC_RBOLTZUI * s->temp0) / qmass;
+        const double l = exp(-cqts * s->chiT);
+        ens_scale(n, vx, vy, vz, l);
+        s->kin *= SQ(l);
+        s->conint += cqts * s->chiT * (OC_RBOLTZUI * s->temp0 + qmass / SQ(s->tauT));
+        s->chiT += 0.5 * cqts * (2.0 * (s->kin - s->kinTemp0) + pmass * SQ(s->chiP) -
+                                 OC_RBOLTZUI * s->temp0) / qmass;
+    }
+}
+
+int oc_vv_ens(const oc_param *p, oc_box *b, oc_bath *s, int stage,
+              int nConst, const int *ca, const int *cb, const double *rc2,
+              double *x, double *y, double *z,
+              double *vx, double *vy, double *vz,
+              const double *fx, const double *fy, const double *fz,
+              const double *mass, const double *rmass, const int *nAtConst, int *ncycle)
+{
+    const int n = p->nAtom;
+    const double dt = p->timeStep;
+    int err = 0;
+    /* virshake is a local of each reference integrator; where it is read
+     * before a SHAKE call has set it (vv_npt_b without constraints,
+     * vv_npt_h's first cycle and its stage 2) this reads zero, as the
+     * reference built with zero-initialised locals does */
+    double vs = 0.;
+    if (s->ens < 1 || s->ens > 4) return 1;
+    double *dd = NULL, *ddy = NULL, *ddz = NULL;
+    if (nConst > 0) {   /* constraint vectors at entry, entry box */
+        dd = (double *)malloc((size_t)nConst * 3 * sizeof(double));
+        ddy = dd + nConst;
+        ddz = dd + 2 * nConst;
+        for (int i = 0; i < nConst; i++) {
+            dd[i] = x[cb[i]] - x[ca[i]];
+            ddy[i] = y[cb[i]] - y[ca[i]];
+            ddz[i] = z[cb[i]] - z[ca[i]];
+            oc_dist2(b, &dd[i], &ddy[i], &ddz[i]);
+        }
+    }
+#define SHAKE_R()                                                                           \
+    do {                                                                                    \
+        if (nConst > 0) {                                                                   \
+            int e_ = oc_shake_r(p, b, nConst, ca, cb, rc2, x, y, z, vx, vy, vz, dd, ddy, ddz, \
+                                rmass, nAtConst, &vs, ncycle);                              \
+            err = err ? err : e_;                                                           \
+            s->virshake = vs;                                                               \
+        }                                                                                   \
+    } while (0)
+#define SHAKE_V()                                                                           \
+    do {                                                                                    \
+        if (nConst > 0) {                                                                   \
+            int e_ = oc_shake_v(p, nConst, ca, cb, vx, vy, vz, dd, ddy, ddz, rmass,          \
+                                nAtConst, ncycle);                                          \
+            err = err ? err : e_;                                                           \
+        }                                                                                   \
+    } while (0)
+
+    if (s->ens == 1) {                                    /* vv_nvt_b, integrate.c:1636-1768 */
+        ens_kick(n, dt, vx, vy, vz, fx, fy, fz, rmass);
+        if (stage == 1) {
+            for (int i = 0; i < n; i++) {
+                x[i] += dt * vx[i];
+                y[i] += dt * vy[i];
+                z[i] += dt * vz[i];
+            }
+            SHAKE_R();
+        } else {
+            SHAKE_V();
+            s->kin = ens_kin(n, vx, vy, vz, mass);
+            const double l = sqrt(1.0 + dt / s->tauT * (s->kinTemp0 / s->kin - 1.0));
+            ens_scale(n, vx, vy, vz, l);
+            s->kin *= SQ(l);
+            ens_stress_kin(n, vx, vy, vz, mass, s->stress);
+        }
+    } else if (s->ens == 3) {                             /* vv_nvt_h, integrate.c:1998-2169 */
+        const double qmass = 2.0 * s->kinTemp0 * SQ(s->tauT);
+        if (stage == 1) {
+            s->kin = ens_kin(n, vx, vy, vz, mass);
+            s->chiT += 0.5 * dt * (s->kin - s->kinTemp0) / qmass;
+            const double l = exp(-0.5 * dt * s->chiT);
+            ens_scale(n, vx, vy, vz, l);
+            ens_kick(n, dt, vx, vy, vz, fx, fy, fz, rmass);
+            s->kin *= SQ(l);
+            s->conint += 0.5 * dt * s->chiT * qmass / SQ(s->tauT);
+            s->chiT += 0.5 * dt * (s->kin - s->kinTemp0) / qmass;
+            for (int i = 0; i < n; i++) {
+                x[i] += dt * vx[i];
+                y[i] += dt * vy[i];
+                z[i] += dt * vz[i];
+            }
+            SHAKE_R();
+        } else {
+            ens_kick(n, dt, vx, vy, vz, fx, fy, fz, rmass);
+            SHAKE_V();
+            s->kin = ens_kin(n, vx, vy, vz, mass);
+            s->chiT += 0.5 * dt * (s->kin - s->kinTemp0) / qmass;
+            const double l = exp(-0.5 * dt * s->chiT);
+            ens_scale(n, vx, vy, vz, l);
+            s->kin *= SQ(l);
+            s->conint += 0.5 * dt * s->chiT * qmass / SQ(s->tauT);
+            s->chiT += 0.5 * dt * (s->kin - s->kinTemp0) / qmass;
+            s->consv = s->conint + 0.5 * qmass * SQ(s->chiT);
+            ens_stress_kin(n, vx, vy, vz, mass, s->stress);
+        }
+    } else if (s->ens == 2) {                             /* vv_npt_b, integrate.c:1770-1996 */
+        const double volume = b->vol;
+        if (stage == 1) {
+            s->kin = ens_kin(n, vx, vy, vz, mass);
+            ens_kick(n, dt, vx, vy, vz, fx, fy, fz, rmass);
+            const int nose = nConst > 0 ? 2 : 1;
+            double *old = NULL;
+            if (nose > 1) {   /* positions and kicked velocities before the trial cycle */
+                old = (double *)malloc((size_t)n * 6 * sizeof(double));
+                memcpy(old, x, n * sizeof(double));
+                memcpy(old + n, y, n * sizeof(double));
+                memcpy(old + 2 * n, z, n * sizeof(double));
+                memcpy(old + 3 * n, vx, n * sizeof(double));
+                memcpy(old + 4 * n, vy, n * sizeof(double));
+                memcpy(old + 5 * n, vz, n * sizeof(double));
+            }
+            for (int k = 0; k < nose; k++) {
+                double c = 1.;
+                if (k == nose - 1) {
+                    const double pp = (2. * s->kin - s->virpot - vs) / (3. * volume);
+                    const double g = 1. + OC_WATERCOMP * dt * (pp - s->press0) / s->tauP;
+                    c = cbrt(g);
+                    double cell[9];
+                    ens_cell(b, cell);
+                    ens_box(b, cell, c);
+                }
+                for (int i = 0; i < n; i++) {
+                    x[i] = c * x[i] + dt * vx[i];
+                    y[i] = c * y[i] + dt * vy[i];
+                    z[i] = c * z[i] + dt * vz[i];
+                }
+                SHAKE_R();
+                if (k < nose - 1) {
+                    memcpy(x, old, n * sizeof(double));
+                    memcpy(y, old + n, n * sizeof(double));
+                    memcpy(z, old + 2 * n, n * sizeof(double));
+                    memcpy(vx, old + 3 * n, n * sizeof(double));
+                    memcpy(vy, old + 4 * n, n * sizeof(double));
+                    memcpy(vz, old + 5 * n, n * sizeof(double));
+                }
+            }
+            free(old);
+        } else {
+            ens_kick(n, dt, vx, vy, vz, fx, fy, fz, rmass);
+            s->kin = ens_kin(n, vx, vy, vz, mass);
+            const double l = sqrt(1.0 + dt / s->tauT * (s->kinTemp0 / s->kin - 1.0));
+            ens_scale(n, vx, vy, vz, l);
+            SHAKE_V();
+            s->kin = ens_kin(n, vx, vy, vz, mass);
+            ens_stress_kin(n, vx, vy, vz, mass, s->stress);
+        }
+    } else {                                              /* vv_npt_h, integrate.c:2171-2618 */
+        const double hts = 0.5 * dt;
+        const double chts = hts / (double)OC_HOOVER;
+        const double cqts = 0.25 * dt / (double)OC_HOOVER;
+        double cell0[9];
+        ens_cell(b, cell0);
+        double volume = b->vol;
+        const double volume0 = b->vol;
+        double masst = 0.;
+        for (int i = 0; i < n; i++) masst += mass[i];
+        const double qmass = 2.0 * s->kinTemp0 * SQ(s->tauT);
+        const double pmass = 2.0 * s->kinTemp0 * SQ(s->tauP);
+        if (stage == 1) {
+            const double chiT0 = s->chiT, chiP0 = s->chiP, cons0 = s->conint;
+            const int nose = nConst > 0 ? 2 : 1;
+            for (int k = 0; k < nose; k++) {
+                for (int kk = 0; kk < OC_HOOVER; kk++)
+                    ens_hoover(n, s, vx, vy, vz, mass, qmass, pmass, cqts, chts, vs, &volume);
+                ens_box(b, cell0, cbrt(volume / volume0));
+                ens_kick(n, dt, vx, vy, vz, fx, fy, fz, rmass);
+                double com[3] = {0., 0., 0.};
+                for (int i = 0; i < n; i++) {
+                    com[0] += mass[i] * x[i];
+                    com[1] += mass[i] * y[i];
+                    com[2] += mass[i] * z[i];
+                }
+                com[0] /= masst;
+                com[1] /= masst;
+                com[2] /= masst;
+                const double c = exp(dt * s->chiP);
+                for (int i = 0; i < n; i++) {
+                    x[i] = c * (x[i] - com[0]) + dt * vx[i] + com[0];
+                    y[i] = c * (y[i] - com[1]) + dt * vy[i] + com[1];
+                    z[i] = c * (z[i] - com[2]) + dt * vz[i] + com[2];
+                }
+                SHAKE_R();
+                /* the second cycle restarts the bath from the step's
+                 * start but carries on from this cycle's positions and
+                 * velocities (integrate.c:2415-2437 saves them, it does
+                 * not restore them) */
+                if (k < nose - 1) {
+                    volume = volume0;
+                    s->chiT = chiT0;
+                    s->chiP = chiP0;
+                    s->conint = cons0;
+                }
+            }
+        } else {
+            ens_kick(n, dt, vx, vy, vz, fx, fy, fz, rmass);
+            SHAKE_V();
+            for (int kk = 0; kk < OC_HOOVER; kk++)
+                ens_hoover(n, s, vx, vy, vz, mass, qmass, pmass, cqts, chts, vs, &volume);
+            double vom[3] = {0., 0., 0.};
+            for (int i = 0; i < n; i++) {
+                vom[0] += mass[i] * vx[i];
+                vom[1] += mass[i] * vy[i];
+                vom[2] += mass[i] * vz[i];
+            }
+            vom[0] /= masst;
+            vom[1] /= masst;
+            vom[2] /= masst;
+            for (int i = 0; i < n; i++) {
+                vx[i] -= vom[0];
+                vy[i] -= vom[1];
+                vz[i] -= vom[2];
+            }
+            ens_box(b, cell0, cbrt(volume / volume0));
+            s->consv = s->conint + s->press0 * volume +
+                       0.5 * (qmass * SQ(s->chiT) + pmass * SQ(s->chiP));
+            s->kin = ens_kin(n, vx, vy, vz, mass);
+            ens_stress_kin(n, vx, vy, vz, mass, s->stress);
+        }
+    }
+#undef SHAKE_R
+#undef SHAKE_V
+    if (stage == 2) oc_image_update(b, n, x, y, z);       /* integrate.c:1620-1627 */
+    free(dd);
+    return err;
+}
+
+/* ---------------------------------------------------------------- */
 /* Whole MD run (mdbas.c:160-300).                                   */
 /* ---------------------------------------------------------------- */
 

@@ -33,6 +33,11 @@
  *                                          in q and E_vdw in eps.
  *   oc_vv_nve / oc_shake_*               : compiled reference vv_integrate /
  *                                          vv_shake_r / vv_shake_v.
+ *   oc_vv_ens                            : compiled reference vv_integrate for
+ *                                          NVT_B, NPT_B, NVT_H, NPT_H (with and
+ *                                          without constraints); the Berendsen
+ *                                          kinetic-energy fixed point and the
+ *                                          Nose-Hoover conserved quantity.
  */
 #ifndef MDORACLE_H
 #define MDORACLE_H
@@ -185,6 +190,29 @@ int oc_vv_nve(const oc_param *p, const oc_box *b, int stage,
               const double *fx, const double *fy, const double *fz,
               const double *mass, const double *rmass, const int *nAtConst,
               double *ekin, double *virshake, int *ncycle);
+
+/* Thermostat / barostat state of one VV call: BATH (global.h:240-245),
+ * the PARAM targets, and the ENERGY members the integrators read (virpot)
+ * and write (kin, virshake, conint, consv); stress[9] accumulates into the
+ * box stresses stress1..stress9 as stage 2 does. */
+typedef struct {
+    int ens;                       /* ENSEMBLE_TYPE: 1 NVT_B, 2 NPT_B, 3 NVT_H, 4 NPT_H */
+    double tauT, tauP, chiT, chiP;
+    double kinTemp0, temp0, press0;
+    double virpot, conint, consv, kin, virshake;
+    double stress[9];
+} oc_bath;
+
+/* vv_nvt_b / vv_npt_b / vv_nvt_h / vv_npt_h (integrate.c:1636-2618), one
+ * stage.  The NPT integrators rescale *b (scale_box, vv_scale_box,
+ * utils.c:291-459).  Returns 0, 311 when SHAKE does not converge, 1 for an
+ * ensemble outside 1..4. */
+int oc_vv_ens(const oc_param *p, oc_box *b, oc_bath *s, int stage,
+              int nConst, const int *ca, const int *cb, const double *rc2,
+              double *x, double *y, double *z,
+              double *vx, double *vy, double *vz,
+              const double *fx, const double *fy, const double *fz,
+              const double *mass, const double *rmass, const int *nAtConst, int *ncycle);
 
 /* Whole MD run: energy at step 0, then nsteps of VV stage 1, list update
  * every p->update steps, energy, VV stage 2 (mdbas.c:160-300).

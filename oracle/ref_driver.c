@@ -242,19 +242,82 @@ double ref_ewald_rec(ref_sys *s, int kmax, const double *x, const double *y, con
 }
 
 /* vv_integrate (integrate.c:1479-1634) for NVE.  out = kin, virshake. */
+/* integrate.c and shake.c keep their work arrays in file-scope globals
+ * sized for one system and (integrators_allocate_arrays,
+ * integrate.c:37-109) one ensemble: allocate again whenever the system,
+ * its size or the ensemble differs from the last allocation (the old
+ * arrays are left allocated). */
+static void integ_alloc(ref_sys *s)
+{
+    static const ref_sys *sys;
+    static int ens = -1, n = -1, nc = -1;
+    if (sys != s || ens != (int)s->ctrl.ens || n != s->param.nAtom || nc != s->param.nConst) {
+        integrators_allocate_arrays(&s->ctrl, &s->param);
+        shake_allocate_arrays(&s->param);
+        sys = s;
+        ens = (int)s->ctrl.ens;
+        n = s->param.nAtom;
+        nc = s->param.nConst;
+    }
+    s->shakeAlloc = 1;
+}
+
 void ref_vv(ref_sys *s, int stage, double *x, double *y, double *z,
             double *vx, double *vy, double *vz, double *fx, double *fy, double *fz,
             double *mass, double *rmass, double *out)
 {
-    if (!s->shakeAlloc) {
-        integrators_allocate_arrays(&s->ctrl, &s->param);
-        shake_allocate_arrays(&s->param);
-        s->shakeAlloc = 1;
-    }
+    integ_alloc(s);
     vv_integrate(&s->ctrl, &s->param, &s->ener, &s->box, &s->bath, s->cons, x, y, z,
                  vx, vy, vz, fx, fy, fz, mass, rmass, s->nAtConst, stage);
     out[0] = s->ener.kin;
     out[1] = s->ener.virshake;
+}
+
+/* vv_integrate with a thermostat or barostat (integrate.c:1479-1516 ->
+ * 1636-2618).  st (in/out): ens, tauT, tauP, chiT, chiP, kinTemp0, temp0,
+ * press0, virpot, conint, consv, kin, virshake, box a1..c3 (9), then the
+ * nine box stresses; the box is re-derived by init_box on the way in, and
+ * the stresses start from st and accumulate. */
+void ref_vv_ens(ref_sys *s, int stage, double *st, double *x, double *y, double *z,
+                double *vx, double *vy, double *vz, double *fx, double *fy, double *fz,
+                double *mass, double *rmass)
+{
+    s->ctrl.ens = (enum ENSEMBLE_TYPE)(int)st[0];
+    integ_alloc(s);
+    s->bath.tauT = st[1];
+    s->bath.tauP = st[2];
+    s->bath.chiT = st[3];
+    s->bath.chiP = st[4];
+    s->param.kinTemp0 = st[5];
+    s->param.temp0 = st[6];
+    s->param.press0 = st[7];
+    s->ener.virpot = st[8];
+    s->ener.conint = st[9];
+    s->ener.consv = st[10];
+    s->ener.kin = st[11];
+    s->ener.virshake = st[12];
+    s->box.a1 = st[13]; s->box.a2 = st[14]; s->box.a3 = st[15];
+    s->box.b1 = st[16]; s->box.b2 = st[17]; s->box.b3 = st[18];
+    s->box.c1 = st[19]; s->box.c2 = st[20]; s->box.c3 = st[21];
+    init_box(&s->box);
+    s->box.stress1 = st[22]; s->box.stress2 = st[23]; s->box.stress3 = st[24];
+    s->box.stress4 = st[25]; s->box.stress5 = st[26]; s->box.stress6 = st[27];
+    s->box.stress7 = st[28]; s->box.stress8 = st[29]; s->box.stress9 = st[30];
+    vv_integrate(&s->ctrl, &s->param, &s->ener, &s->box, &s->bath, s->cons, x, y, z,
+                 vx, vy, vz, fx, fy, fz, mass, rmass, s->nAtConst, stage);
+    st[3] = s->bath.chiT;
+    st[4] = s->bath.chiP;
+    st[9] = s->ener.conint;
+    st[10] = s->ener.consv;
+    st[11] = s->ener.kin;
+    st[12] = s->ener.virshake;
+    st[13] = s->box.a1; st[14] = s->box.a2; st[15] = s->box.a3;
+    st[16] = s->box.b1; st[17] = s->box.b2; st[18] = s->box.b3;
+    st[19] = s->box.c1; st[20] = s->box.c2; st[21] = s->box.c3;
+    st[22] = s->box.stress1; st[23] = s->box.stress2; st[24] = s->box.stress3;
+    st[25] = s->box.stress4; st[26] = s->box.stress5; st[27] = s->box.stress6;
+    st[28] = s->box.stress7; st[29] = s->box.stress8; st[30] = s->box.stress9;
+    s->ctrl.ens = NVE;
 }
 
 /* init_constvel (init.c:724-833): projects velocities onto the

@@ -127,3 +127,47 @@ def test_driver_plain_cutoff_deck_against_oracle(mode, c0_oracle, tmp_path):
     decks.write_deck(d, wb, configs.CONFIGS["c0"], 25, elec="switch")
     prop, rest = run(d, "-m", mode, "-p", "fp64")
     check(prop, rest, s, et, ep, 1e-10, 1e-9)
+
+
+ENSEMBLE_DECKS = [("nvt beren", 1), ("npt beren", 2), ("nvt hoover", 3), ("npt hoover", 4)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ensemble,ens", ENSEMBLE_DECKS, ids=[e[0].replace(" ", "_")
+                                                              for e in ENSEMBLE_DECKS])
+def test_driver_ensemble_deck_against_oracle(ensemble, ens, c0_oracle, tmp_path):
+    """The water deck with a thermostat or barostat ("ensemble nvt beren",
+    "npt hoover", ..., taut 0.1 ps, taup 0.5 ps, pressure 1 bar), 15 FP64
+    drop-in steps (list rebuilt every 20 steps, so once at the start)
+    against the oracle's MD loop with oc_vv_ens: PROP total and potential
+    energy, consv and volume to 1e-10 relative, final positions to 1e-8 A
+    (the device sums the kinetic energy in another order, and the bath
+    feeds it back into every velocity)."""
+    wb = c0_oracle[0]
+    cfg = configs.CONFIGS["c0"]
+    o = O.OracleSystem(wb)
+    s = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    ndeg = 3 * wb.n - 3 - len(wb.ca)                   # utils.c:569-582
+    temp0 = float(cfg["temp"])
+    st = O.ens_state(ens, wb.box9, tauT=0.1, tauP=0.5, temp0=temp0,
+                     kinTemp0=0.5 * temp0 * float(ndeg) * 0.83144621,
+                     press0=1.0 * 6.02214129e-3)
+    err, f, rec = o.md_run_ens(st, 15, *s)
+    assert err == 0
+    d = str(tmp_path)
+    decks.write_deck(d, wb, cfg, 15, ensemble=ensemble, taut=0.1, taup=0.5, pressure=1.0)
+    prop, rest = run(d, "-m", "dropin", "-p", "fp64")
+    assert prop.shape[0] == 16
+    scale = abs(rec[0, 0])
+    for col, k, name in ((5, 0, "tot"), (7, 1, "pot"), (22, 3, "consv")):
+        err_k = np.abs(prop[:, col] * KCAL - rec[:, k]).max() / scale
+        print(name, err_k, flush=True)
+        assert err_k < 1e-10, name
+    assert np.abs(prop[:, 4] - rec[:, 4]).max() < 1e-10 * rec[0, 4]
+    if ens in (2, 4):
+        assert rec[-1, 4] != rec[0, 4]
+    if ens in (3, 4):
+        assert np.abs(rec[1:, 3]).min() > 0.
+    dx = max(np.abs(rest[k] - s[i]).max() for i, k in enumerate("xyz"))
+    print("pos", dx, flush=True)
+    assert dx < 1e-8
