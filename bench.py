@@ -65,6 +65,29 @@ def sync_fn():
     return sync
 
 
+def copy_calibration(mib: int):
+    """One device-to-device copy of mib MiB through the HIP runtime (a copy
+    kernel that reads and writes exactly that many bytes): the known size
+    against which the FETCH_SIZE / WRITE_SIZE counters of a profiled run are
+    scaled."""
+    import ctypes
+    rt = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib",
+                                  "libamdhip64.so.7"))
+    nbytes = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if rt.hipMalloc(ctypes.byref(a), nbytes) or rt.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if rt.hipMemset(a, 1, nbytes) or rt.hipDeviceSynchronize():
+            raise RuntimeError("hipMemset failed")
+        if rt.hipMemcpy(b, a, nbytes, 3) or rt.hipDeviceSynchronize():   # device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        rt.hipFree(a)
+        rt.hipFree(b)
+    return {"copy_calibration_mib": mib, "bytes_read": mib << 20, "bytes_written": mib << 20}
+
+
 def cpu_baseline(boxes, budget_s=25.0, max_steps=10):
     """The FP64 oracle (a from-scratch port of the reference's path) on one
     host core: per box one list build, then MD steps (VV stages and
@@ -430,7 +453,13 @@ def main():
                          "default: the library's choice for the box)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (skip the autotuner)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device once (the known-size run that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sharded "
               "multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)

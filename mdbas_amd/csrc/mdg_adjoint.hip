@@ -17,11 +17,11 @@
 //      - the SHAKE maps' vector-Jacobian products, from the implicit
 //        function theorem on each rigid three-site molecule (the constraint
 //        solution that the reference's iteration converges to);
-//      - (dF/dx)^T w = -H w and (dF/dtheta)^T w, as central differences of
-//        the exact forces and of the exact parameter gradient along w
-//        (directional derivatives of first derivatives, step 5e-6 A at the
-//        largest component), with the terms of the mean-force removal of
-//        spme.c:648-670 kept exact.
+//      - (dF/dx)^T w = -H w and (dF/dtheta)^T w, analytic: the pair
+//        terms' second radial derivatives over the device list, the
+//        excluded-pair correction, and SPME through the mesh of
+//        sum_l q_l grad W_l . w_l and the splines' second derivatives, with
+//        the terms of the mean-force removal of spme.c:648-670.
 #include "mdg_internal.h"
 
 namespace {
@@ -322,37 +322,16 @@ __global__ void k_axpy(long m, const double *a, double s, const double *b, doubl
     if (i < m) y[i] = a[i] + s * b[i];
 }
 
-// fbar = dt/2 vb2 / m + fp (per component), max |fbar| partials per block.
+// fbar = dt/2 vb2 / m + fp (per component).
 __global__ void __launch_bounds__(256) k_fbar(int n, double dt, const double *rmass,
-                                              const double *vb2, const double *fp, double *fbar,
-                                              double *part)
+                                              const double *vb2, const double *fp, double *fbar)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double mx = 0.;
-    if (i < n)
-        for (int c = 0; c < 3; c++) {
-            const long o = (long)c * n + i;
-            fbar[o] = 0.5 * dt * rmass[i] * vb2[o] + fp[o];
-            mx = fmax(mx, fabs(fbar[o]));
-        }
-    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-    __shared__ double red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-}
-
-__global__ void k_absmax(int n, const double *v, double *part)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double mx = 0.;
-    if (i < n)
-        for (int c = 0; c < 3; c++) mx = fmax(mx, fabs(v[(long)c * n + i]));
-    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-    __shared__ double red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    if (i >= n) return;
+    for (int c = 0; c < 3; c++) {
+        const long o = (long)c * n + i;
+        fbar[o] = 0.5 * dt * rmass[i] * vb2[o] + fp[o];
+    }
 }
 
 // ------------------------------------------------------------ host side
@@ -475,28 +454,6 @@ static int set_positions(mdg_ctx *ctx, const double *base, double s, const doubl
     return MDG_OK;
 }
 
-static int host_max(mdg_ctx *ctx, const double *part, int nb, double &mx)
-{
-    std::vector<double> h(nb);
-    HIPCHK(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    mx = 0.;
-    for (double v : h) mx = std::max(mx, v);
-    return MDG_OK;
-}
-
-// Raw force (before the mean removal of spme.c:648-670), m = mean of
-// the raw reciprocal forces: out = F + m, per component.
-__global__ void k_raw_force(int n, const double *fx, const double *fy, const double *fz,
-                            const double *m, double *out)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = fx[i] + m[0];
-    out[n + i] = fy[i] + m[1];
-    out[2 * n + i] = fz[i] + m[2];
-}
-
 // y[c*n + i] += s * (F_c[i] + m[c])
 __global__ void k_add_raw(int n, double s, const double *fx, const double *fy, const double *fz,
                           const double *m, double *y)
@@ -531,90 +488,162 @@ __global__ void __launch_bounds__(1024) k_mean3(int n, const double *a, const do
     }
 }
 
-// positions = base + d (the same vector for every atom)
-__global__ void k_shift(int n, const double *base, double dx, double dy, double dz, double *x,
-                        double *y, double *z)
+// u in slot order: su[c * ns + s] = u[c * n + atom of slot s], 0 on
+// padding slots.
+__global__ void k_hvp_slot_u(int nslot, long ns, const int *__restrict__ slotAtom, int n,
+                             const double *__restrict__ u, double *__restrict__ su)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    const int a = slotAtom[s];
+    for (int c = 0; c < 3; c++) su[c * ns + s] = a >= 0 ? u[(long)c * n + a] : 0.;
+}
+
+struct HvpFin {
+    MdgParams P;
+    MdgBox B;
+    int n;
+    long ns;
+    const int *atomSlot, *exOff, *exLst;   // symmetric exclusion table
+    const double *x, *y, *z, *q, *u;
+    const double *slot;                    // slot sums: hx, hy, hz, mq, me, ms (ns each)
+    double *hu, *mq, *me, *ms;             // atom order (overwritten)
+};
+
+// Per atom: the real-space slot sums plus the excluded-pair correction
+// (phi = -C q_i q_j erf(a r)/r, ewald.c:371-384) over the symmetric table,
+// each partner contributing -(g du + h (d . du) d) to (H u)_i and
+// C q_j phi'/(qel r) (d . du) to d/dq_i (grad E . u).
+__global__ void __launch_bounds__(256) k_hvp_finalize(HvpFin F)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    x[i] = base[i] + dx;
-    y[i] = base[n + i] + dy;
-    z[i] = base[2 * n + i] + dz;
+    if (i >= F.n) return;
+    const int s = F.atomSlot[i];
+    double h[3] = {F.slot[s], F.slot[F.ns + s], F.slot[2 * F.ns + s]};
+    double gq = F.slot[3 * F.ns + s];
+    const double C = F.P.chargeConst, al = F.P.alpha;
+    const double ui[3] = {F.u[i], F.u[F.n + i], F.u[2 * (long)F.n + i]};
+    for (int k = F.exOff[i]; k < F.exOff[i + 1]; k++) {
+        const int j = F.exLst[k];
+        double d[3] = {F.x[j] - F.x[i], F.y[j] - F.y[i], F.z[j] - F.z[i]};
+        const double r = sqrt(mdg_dist(F.B, d[0], d[1], d[2])), rt = 1. / r, ar = al * r;
+        const double ef = erf(ar), gs = 2. * al * exp(-ar * ar) / SQRTPI;
+        const double c1 = (ef * rt - gs) * rt;                                 // per unit qel
+        const double c2 = -2. * ef * rt * rt * rt + gs * (2. * rt * rt + 2. * al * al);
+        const double qel = C * F.q[i] * F.q[j];
+        const double g = qel * c1 * rt, hh = (qel * c2 - g) * rt * rt;
+        const double du[3] = {F.u[j] - ui[0], F.u[F.n + j] - ui[1], F.u[2 * (long)F.n + j] - ui[2]};
+        const double t = d[0] * du[0] + d[1] * du[1] + d[2] * du[2];
+        for (int c = 0; c < 3; c++) h[c] -= g * du[c] + hh * t * d[c];
+        gq += C * F.q[j] * c1 * rt * t;
+    }
+    for (int c = 0; c < 3; c++) F.hu[(long)c * F.n + i] = h[c];
+    F.mq[i] = gq;
+    F.me[i] = F.slot[4 * F.ns + s];
+    F.ms[i] = F.slot[5 * F.ns + s];
+}
+
+// The second-derivative scratch: u in slot order (3 ns), the slot sums
+// (6 ns), then in atom order H u (3n), its mixed terms (3n) and the
+// reciprocal part along the uniform translation (3n + n).
+static long hvp_ns(const mdg_ctx *ctx) { return (long)ctx->cl.nclu * MDG_CL + 64; }
+
+static int ensure_hvp(mdg_ctx *ctx)
+{
+    if (ctx->hvpBuf) return MDG_OK;
+    const long ns = hvp_ns(ctx), n = ctx->n;
+    HIPCHK(hipMalloc(&ctx->hvpBuf, (size_t)(9 * ns + 10 * n) * sizeof(double)));
+    return MDG_OK;
 }
 
 // Scratch of the force vector-Jacobian product.
 struct VjpScratch {
-    double *fsave;   // 3n
     double *m;       // 8: [0..2] mean raw reciprocal force, [4..6] mean of u
 };
 
 // Adds (dF/dx)^T u to xb (when xb) and (dF/dtheta)^T u to the parameter
 // adjoints at positions base (3n).  F = Fr - m with Fr = -grad E and m the
 // mean raw reciprocal force that spme.c:648-670 removes, so
-//   (dF/dx)^T u     = (dFr/dx) u - (dm/dx)^T s,           s = sum_i u_i
-//   (dF/dtheta)^T u = -d/dh dE/dtheta (x + h u) + (1/N) d/dh dE_rec/dq (x + h s)
-//   (dm/dx)^T s     = (1/N) (dFr_rec/dx) (s, s, ..., s)
-// (the Hessian of E is symmetric); each directional derivative is a
-// central difference with the largest displacement 5e-6 A.
-static int force_vjp(mdg_ctx *ctx, const double *base, const double *u, double umax, double *xb,
+//   (dF/dx)^T u     = -H u + (1/N) H_rec (s, ..., s),          s = sum_i u_i
+//   (dF/dtheta)^T u = -d/dtheta (grad E . u) + (1/N) d/dq (grad E_rec . (s, ..., s))
+// with H u and the mixed terms analytic: the real-space pairs over the
+// device list (k_hvp_cluster), the excluded-pair correction
+// (k_hvp_finalize) and SPME (mdg_spme_hvp: the dipole mesh of u through
+// the influence function, and the splines' second derivatives).
+static int force_vjp(mdg_ctx *ctx, const double *base, const double *u, double *xb,
                      const VjpScratch &W)
 {
-    if (umax <= 0.) return MDG_OK;
     const int n = ctx->n, nbA = nblk(n, 256);
-    const double h = 5e-6 / umax;
-    // along u
-    RETCHK(set_positions(ctx, base, h, u));
-    if (xb) {
-        RETCHK(mdg_device_energy(ctx, false));
-        hipLaunchKernelGGL(k_mean3, dim3(1), dim3(1024), 0, ctx->stream, n, ctx->frx64, ctx->fry64,
-                           ctx->frz64, W.m);
-        hipLaunchKernelGGL(k_raw_force, dim3(nbA), dim3(256), 0, ctx->stream, n, ctx->fx, ctx->fy,
-                           ctx->fz, W.m, W.fsave);
-        LAUNCHCHK();
-    }
-    RETCHK(adj_grad(ctx, -0.5 / h, true));
-    RETCHK(set_positions(ctx, base, -h, u));
-    if (xb) {
-        RETCHK(mdg_device_energy(ctx, false));
-        hipLaunchKernelGGL(k_mean3, dim3(1), dim3(1024), 0, ctx->stream, n, ctx->frx64, ctx->fry64,
-                           ctx->frz64, W.m);
-        hipLaunchKernelGGL(k_add_raw, dim3(nbA), dim3(256), 0, ctx->stream, n, -0.5 / h, ctx->fx,
-                           ctx->fy, ctx->fz, W.m, xb);
-        for (int c = 0; c < 3; c++)
-            hipLaunchKernelGGL(k_axpy, dim3(nbA), dim3(256), 0, ctx->stream, (long)n,
-                               xb + (long)c * n, 0.5 / h, W.fsave + (long)c * n, xb + (long)c * n);
-        LAUNCHCHK();
-    }
-    RETCHK(adj_grad(ctx, 0.5 / h, true));
-    // along the uniform translation s (the mean removal's terms)
+    const long ns = hvp_ns(ctx);
+    RETCHK(ensure_hvp(ctx));
+    RETCHK(set_positions(ctx, base, 0., nullptr));
+    double *su = ctx->hvpBuf, *slot = su + 3 * ns;
+    double *hu = slot + 6 * ns, *mq = hu + 3 * (long)n, *me = mq + n, *ms = me + n;
+    double *hu2 = ms + n, *mq2 = hu2 + 3 * (long)n;
+    RETCHK(mdg_fill_f64(ctx, slot, 6 * ns, 0.0));
+    const int nslot = ctx->cl.nclu * MDG_CL;
+    hipLaunchKernelGGL(k_hvp_slot_u, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream, nslot, ns,
+                       ctx->cl.slotAtom, n, u, su);
+    LAUNCHCHK();
+    HvpSlots H;
+    H.ux = su;
+    H.uy = su + ns;
+    H.uz = su + 2 * ns;
+    H.hx = slot;
+    H.hy = slot + ns;
+    H.hz = slot + 2 * ns;
+    H.mq = slot + 3 * ns;
+    H.me = slot + 4 * ns;
+    H.ms = slot + 5 * ns;
+    RETCHK(mdg_hvp_pair(ctx, H));
+    HvpFin F;
+    F.P = ctx->P;
+    F.B = ctx->B;
+    F.n = n;
+    F.ns = ns;
+    F.atomSlot = ctx->cl.atomSlot;
+    F.exOff = ctx->exSymOff;
+    F.exLst = ctx->exSymLst;
+    F.x = ctx->x;
+    F.y = ctx->y;
+    F.z = ctx->z;
+    F.q = ctx->q;
+    F.u = u;
+    F.slot = slot;
+    F.hu = hu;
+    F.mq = mq;
+    F.me = me;
+    F.ms = ms;
+    hipLaunchKernelGGL(k_hvp_finalize, dim3(nbA), dim3(256), 0, ctx->stream, F);
+    LAUNCHCHK();
+    RETCHK(mdg_spme_hvp(ctx, u, nullptr, hu, mq));
+    // -H u and -d/dtheta (grad E . u)
+    if (xb)
+        hipLaunchKernelGGL(k_axpy, dim3(nblk(3L * n, 256)), dim3(256), 0, ctx->stream, 3L * n, xb,
+                           -1.0, hu, xb);
+    hipLaunchKernelGGL(k_axpy, dim3(nbA), dim3(256), 0, ctx->stream, (long)n, ctx->adjQ, -1.0, mq,
+                       ctx->adjQ);
+    hipLaunchKernelGGL(k_axpy, dim3(nbA), dim3(256), 0, ctx->stream, (long)n, ctx->adjE, -1.0, me,
+                       ctx->adjE);
+    hipLaunchKernelGGL(k_axpy, dim3(nbA), dim3(256), 0, ctx->stream, (long)n, ctx->adjS, -1.0, ms,
+                       ctx->adjS);
+    // the mean removal's terms along the uniform translation s
     hipLaunchKernelGGL(k_mean3, dim3(1), dim3(1024), 0, ctx->stream, n, u, u + n, u + 2 * (long)n,
                        W.m + 4);
     LAUNCHCHK();
     double sm[3];
     HIPCHK(hipMemcpyAsync(sm, W.m + 4, sizeof sm, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    const double sx = sm[0] * n, sy = sm[1] * n, sz = sm[2] * n;
-    const double smax = std::max(fabs(sx), std::max(fabs(sy), fabs(sz)));
-    if (smax <= 0.) return MDG_OK;
-    const double hs = 5e-6 / smax;
-    for (int side = 0; side < 2; side++) {
-        const double d = side == 0 ? hs : -hs;
-        hipLaunchKernelGGL(k_shift, dim3(nbA), dim3(256), 0, ctx->stream, n, base, d * sx, d * sy,
-                           d * sz, ctx->x, ctx->y, ctx->z);
-        LAUNCHCHK();
-        RETCHK(mdg_spme_pot(ctx, ctx->adjPot));
-        const double f = (side == 0 ? 0.5 : -0.5) / (hs * n);
-        // dq += (1/N) d/dh pot;  xb -= (1/N) d/dh Fr_rec
-        hipLaunchKernelGGL(k_axpy, dim3(nbA), dim3(256), 0, ctx->stream, (long)n, ctx->adjQ, f,
-                           ctx->adjPot, ctx->adjQ);
-        if (xb) {
-            const double *fr[3] = {ctx->frx64, ctx->fry64, ctx->frz64};
-            for (int c = 0; c < 3; c++)
-                hipLaunchKernelGGL(k_axpy, dim3(nbA), dim3(256), 0, ctx->stream, (long)n,
-                                   xb + (long)c * n, -f, fr[c], xb + (long)c * n);
-        }
-        LAUNCHCHK();
-    }
+    const double sv[3] = {sm[0] * n, sm[1] * n, sm[2] * n};
+    RETCHK(mdg_fill_f64(ctx, hu2, 4L * n, 0.0));
+    RETCHK(mdg_spme_hvp(ctx, nullptr, sv, hu2, mq2));
+    const double inv = 1.0 / n;
+    if (xb)
+        hipLaunchKernelGGL(k_axpy, dim3(nblk(3L * n, 256)), dim3(256), 0, ctx->stream, 3L * n, xb,
+                           inv, hu2, xb);
+    hipLaunchKernelGGL(k_axpy, dim3(nbA), dim3(256), 0, ctx->stream, (long)n, ctx->adjQ, inv, mq2,
+                       ctx->adjQ);
+    LAUNCHCHK();
     return MDG_OK;
 }
 
@@ -644,9 +673,8 @@ extern "C" int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, 
     };
     double *ck = alloc(sn * nseg), *seg = alloc(sn * (K + 1)), *fin = alloc(sn);
     double *xb = alloc(3 * (size_t)n), *vb = alloc(3 * (size_t)n), *vb2 = alloc(3 * (size_t)n);
-    double *fp = alloc(3 * (size_t)n), *fbar = alloc(3 * (size_t)n), *fsave = alloc(3 * (size_t)n);
+    double *fp = alloc(3 * (size_t)n), *fbar = alloc(3 * (size_t)n);
     VjpScratch W;
-    W.fsave = fsave;
     W.m = alloc(8);
     int st = MDG_OK;
     auto fail = [&](int code) {
@@ -658,7 +686,7 @@ extern "C" int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, 
             ctx->err = "hipMalloc failed (trajectory adjoint buffers)";
             return fail(MDG_EHIP);
         }
-    if (bufs.size() != 10) {
+    if (bufs.size() != 9) {
         ctx->err = "hipMalloc failed (trajectory adjoint buffers)";
         return fail(MDG_EHIP);
     }
@@ -695,7 +723,6 @@ extern "C" int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, 
     A.vb = vb;
     A.vb2 = vb2;
     A.fp = fp;
-    double *part = ctx->epart + PART_MISC;
     const int nbA = nblk(n, 256), nbG = nblk(ctx->nGrp, 64);
     for (int c = nseg - 1; c >= 0 && st == MDG_OK; c--) {
         const int k0 = c * K, kn = std::min(K, T - k0);
@@ -727,10 +754,8 @@ extern "C" int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, 
             A.cur = cur;
             hipLaunchKernelGGL(k_traj_stage2, dim3(nbG), dim3(64), 0, ctx->stream, A);
             hipLaunchKernelGGL(k_fbar, dim3(nbA), dim3(256), 0, ctx->stream, n, A.dt, ctx->rmass,
-                               vb2, fp, fbar, part);
-            double umax = 0.;
-            if ((st = host_max(ctx, part, nbA, umax))) break;
-            if ((st = force_vjp(ctx, cur, fbar, umax, xb, W))) break;
+                               vb2, fp, fbar);
+            if ((st = force_vjp(ctx, cur, fbar, xb, W))) break;
             hipLaunchKernelGGL(k_traj_stage1, dim3(nbG), dim3(64), 0, ctx->stream, A);
             if (hipGetLastError() != hipSuccess) {
                 ctx->err = "kernel launch failed (trajectory adjoint)";
@@ -739,10 +764,7 @@ extern "C" int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, 
         }
         // F_0 of the whole trajectory depends on the parameters too
         if (st == MDG_OK && c == 0) {
-            hipLaunchKernelGGL(k_absmax, dim3(nbA), dim3(256), 0, ctx->stream, n, fp, part);
-            double umax = 0.;
-            st = host_max(ctx, part, nbA, umax);
-            if (st == MDG_OK) st = force_vjp(ctx, seg, fp, umax, nullptr, W);
+            st = force_vjp(ctx, seg, fp, nullptr, W);
         }
     }
     // leave the context at the end of the trajectory

@@ -540,7 +540,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->shkMask, ctx->exSymOff, ctx->exSymLst, ctx->adjQ, ctx->adjE,
                     ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binBucket, ctx->binOut,
                     ctx->binNOut, ctx->splI, ctx->splB, ctx->splD,
-                    ctx->cl.pJ, ctx->cl.pMask, ctx->cl.pCnt, ctx->dispPart};
+                    ctx->cl.pJ, ctx->cl.pMask, ctx->cl.pCnt, ctx->dispPart, ctx->ensSave,
+                    ctx->gridU, ctx->hvpBuf};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->bt) {

@@ -232,6 +232,9 @@ struct mdg_ctx {
     int nExSym = 0;
     // parameter adjoints dE/d(q, eps, sig) and the SPME mesh potential (mdg_adjoint.hip)
     double *adjQ = nullptr, *adjE = nullptr, *adjS = nullptr, *adjPot = nullptr;
+    // trajectory adjoint: the mesh of theta * Q_u (G) and the second-
+    // derivative scratch (slot-order u, H u and mixed terms; atom-order sums)
+    double *gridU = nullptr, *hvpBuf = nullptr;
     std::vector<int> hExOff, hExLst;
     std::vector<MDG_CONSTRAINT> hCons;   // the constraint list of mdg_set_topology
     bool exclBuilt = false;
@@ -584,6 +587,20 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
 // thermostats and barostats behind the drop-in vv_integrate (mdg_ensemble.hip)
 int mdg_vv_ensemble(mdg_ctx *ctx, const MDG_PARAM *param, MDG_ENERGY *ener, MDG_PBC *box,
                     MDG_BATH *bath, int ens, int stage);
+// second derivatives for the trajectory adjoint (mdg_pair.hip, mdg_spme.hip)
+struct HvpSlots {
+    const double *ux, *uy, *uz;      // the direction u in slot order
+    double *hx, *hy, *hz;            // H u
+    double *mq, *me, *ms;            // d/d(q, eps, sig) (grad E . u)
+};
+int mdg_hvp_pair(mdg_ctx *ctx, const HvpSlots &H);
+struct HvpVec {
+    double s[3];                     // the direction of every atom when u is null
+};
+// the SPME part at the context's positions: hu (3n) += H_rec u and mq (n)
+// += d/dq (grad E_rec . u); u (3n, component-major) or, when u is null,
+// the same vector s for every atom
+int mdg_spme_hvp(mdg_ctx *ctx, const double *u, const double s[3], double *hu, double *mq);
 // follow a box the caller changed (an NPT step): derived vectors, link-cell
 // counts, net-charge energy and the SPME influence function
 int mdg_refresh_box(mdg_ctx *ctx, const MDG_PBC *box);

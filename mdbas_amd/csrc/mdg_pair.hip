@@ -1096,6 +1096,116 @@ int mdg_adj_pair(mdg_ctx *ctx)
     return MDG_OK;
 }
 
+// Radial derivatives per unit veps of the switched LJ of vdw.c:144-184,
+// V = pv S, pv = 4 (s^12 - s^6), s = vsig / r: d1 = V', d2 = V'',
+// ds = dV'/dvsig.
+__device__ inline void hvp_lj(const MdgParams &P, double vsig, double r, double &d1, double &d2,
+                              double &ds)
+{
+    const double r2 = r * r, rt = 1. / r, s = vsig * rt;
+    const double s2 = s * s, s6 = s2 * s2 * s2, s12 = s6 * s6, s5 = s2 * s2 * s, s11 = s6 * s5;
+    const double pv = 4. * (s12 - s6);
+    const double pv1 = 24. * (s6 - 2. * s12) * rt;
+    const double pv2 = 4. * (156. * s12 - 42. * s6) * rt * rt;
+    const double pvs = 4. * (12. * s11 - 6. * s5) * rt;
+    const double pv1s = 24. * (6. * s5 - 24. * s11) * rt * rt;
+    double S = 1., S1 = 0., S2 = 0.;
+    if (r2 > P.cutOn2) {
+        const double sw1 = P.cutOff2 - r2, on = P.cutOn2 - r2;
+        S = sw1 * sw1 * (P.cutOff2 + 2. * r2 - 3. * P.cutOn2) * P.switch2;
+        S1 = 12. * r * sw1 * on * P.switch2;
+        S2 = 12. * P.switch2 * (sw1 * on - 2. * r2 * on - 2. * r2 * sw1);
+    }
+    d1 = pv1 * S + pv * S1;
+    d2 = pv2 * S + 2. * pv1 * S1 + pv * S2;
+    ds = pv1s * S + pvs * S1;
+}
+
+// Second derivatives of the real-space pair energy over the device list
+// (the pairs of k_pair_fp64, r^2 <= cutOff^2), for a direction u in slot
+// order: with d = x_j - x_i, phi the pair energy, g = phi'/r and
+// h = (phi'' - g)/r^2,
+//   (H u)_j = -(H u)_i = g du + h (d . du) d,   du = u_j - u_i,
+// and the pair's share of d/dtheta (grad E . u) = d/dtheta g (d . du) for
+// theta = q_i, q_j, eps_i, eps_j, sig_i, sig_j.  Same wavefront layout as
+// k_adj_cluster; FP64, slot sums with atomics.
+__global__ void __launch_bounds__(256) k_hvp_cluster(ClusterArgs A, HvpSlots H)
+{
+    const int lane = threadIdx.x & 63;
+    const int a = lane & 7, b = lane >> 3;
+    const int bit = lane;
+    const int nch = min(A.chOff[A.nclu], A.maxChunks);
+    const double C = A.P.chargeConst, al = A.P.alpha;
+    for (int W = blockIdx.x * 4 + (threadIdx.x >> 6); W < nch; W += gridDim.x * 4) {
+        const int2 ch = A.chunk[W];
+        const int I = ch.x, e0 = ch.y, e1 = min(min(e0 + MDG_CHUNK, A.cpOff[I + 1]), A.cap);
+        const int si = I * MDG_CL + a;
+        const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
+        const double qi = A.sq[si], ei = A.seps[si], gi = A.ssig[si];
+        const double uxi = H.ux[si], uyi = H.uy[si], uzi = H.uz[si];
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int e = e0; e < e1; e++) {
+            const int J = A.cpJ[e];
+            const unsigned long long m = A.cpMask[e];
+            const int sj = J * MDG_CL + b;
+            double dx = A.sx[sj] - xi, dy = A.sy[sj] - yi, dz = A.sz[sj] - zi;
+            const double r2 = mdg_dist(A.B, dx, dy, dz);
+            double w[6] = {0, 0, 0, 0, 0, 0};
+            if (((m >> bit) & 1ull) && r2 <= A.P.cutOff2) {
+                const double r = sqrt(r2), rt = 1. / r, ar = al * r;
+                const double qj = A.sq[sj], ej = A.seps[sj];
+                const double ec = erfc(ar), gs = 2. * al * exp(-ar * ar) / SQRTPI;
+                const double el1 = -(ec * rt + gs) * rt;                      // per unit qel
+                const double el2 = 2. * ec * rt * rt * rt + gs * (2. * rt * rt + 2. * al * al);
+                double l1, l2, ls;
+                hvp_lj(A.P, gi + A.ssig[sj], r, l1, l2, ls);
+                const double qel = C * qi * qj, veps = ei * ej;
+                const double d1 = qel * el1 + veps * l1, d2 = qel * el2 + veps * l2;
+                const double g = d1 * rt, h = (d2 - g) * rt * rt;
+                const double dux = H.ux[sj] - uxi, duy = H.uy[sj] - uyi, duz = H.uz[sj] - uzi;
+                const double t = dx * dux + dy * duy + dz * duz;
+                w[0] = g * dux + h * t * dx;
+                w[1] = g * duy + h * t * dy;
+                w[2] = g * duz + h * t * dz;
+                const double cq = C * el1 * rt * t, cl = l1 * rt * t, cs = veps * ls * rt * t;
+                acc[0] -= w[0];
+                acc[1] -= w[1];
+                acc[2] -= w[2];
+                acc[3] += qj * cq;
+                acc[4] += ej * cl;
+                acc[5] += cs;
+                w[3] = qi * cq;
+                w[4] = ei * cl;
+                w[5] = cs;
+            }
+            double *out[6] = {H.hx, H.hy, H.hz, H.mq, H.me, H.ms};
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                const double v = dpp_sum8(w[k]);
+                if (a == 0 && v != 0.) atomicAdd(&out[k][sj], v);
+            }
+        }
+        double *out[6] = {H.hx, H.hy, H.hz, H.mq, H.me, H.ms};
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            double v = acc[k];
+            for (int o = 8; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+            if (b == 0) atomicAdd(&out[k][si], v);
+        }
+    }
+}
+
+// Real-space H u and mixed terms into the slot accumulators of H (which
+// the caller zeroes), at the context's positions, over the current list.
+int mdg_hvp_pair(mdg_ctx *ctx, const HvpSlots &H)
+{
+    RETCHK(slot_pos(ctx, false));
+    ClusterArgs A = cluster_args(ctx);
+    hipLaunchKernelGGL(k_hvp_cluster, dim3(mdg_pair_blocks(ctx)), dim3(256), 0, ctx->stream, A, H);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
 // Pair forces of every replica of a batch: slot positions (and zeroed
 // accumulators), then the cluster kernel with per-replica boxes and
 // per-chunk energies.

@@ -1202,6 +1202,180 @@ int mdg_spme_pot(mdg_ctx *ctx, double *pot)
     return MDG_OK;
 }
 
+// ---------------------------------------------------------------- second derivatives
+
+// Spline values, first and second derivatives along one axis in bspgen's
+// layout: b[i] = M_p(u+i), d[i] = M_p'(u+i), dd[i] = M_p''(u+i) =
+// M_{p-2}(u+i) - 2 M_{p-2}(u+i-1) + M_{p-2}(u+i-2).
+__device__ inline void hvp_splines(double s, int p, double *b, double *d, double *dd)
+{
+    double b2[MAXP], t[MAXP];
+    bspline_dev<double>(s, p, b, d);
+    bspline_dev<double>(s, p - 2, b2, t);
+    for (int i = 0; i < p; i++) {
+        const double v0 = i < p - 2 ? b2[i] : 0.;
+        const double v1 = (i >= 1 && i - 1 < p - 2) ? b2[i - 1] : 0.;
+        const double v2 = (i >= 2 && i - 2 < p - 2) ? b2[i - 2] : 0.;
+        dd[i] = v0 - 2. * v1 + v2;
+    }
+}
+
+__device__ inline void hvp_u(const double *u, const double *s, int n, int l, double ul[3])
+{
+    if (u) {
+        ul[0] = u[l];
+        ul[1] = u[n + l];
+        ul[2] = u[2 * (long)n + l];
+    } else {
+        ul[0] = s[0];
+        ul[1] = s[1];
+        ul[2] = s[2];
+    }
+}
+
+// Q_u(g) = sum_l q_l grad W_l(g) . u_l onto the FP64 mesh (atomics): the
+// charge spread of spme.c:449-491 with each spline product replaced by its
+// derivative along u_l.
+__global__ void __launch_bounds__(256)
+k_hvp_spread(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
+             const double *__restrict__ y, const double *__restrict__ z,
+             const double *__restrict__ q, const double *__restrict__ u, HvpVec sv, double *grid)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int p = P.nbsp, m1 = P.m1, m2 = P.m2, m3 = P.m3;
+    double s1, s2, s3;
+    scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+    double b1[MAXP], b2[MAXP], b3[MAXP], d1[MAXP], d2[MAXP], d3[MAXP];
+    bspline_dev<double>(s1, p, b1, d1);
+    bspline_dev<double>(s2, p, b2, d2);
+    bspline_dev<double>(s3, p, b3, d3);
+    double ul[3];
+    hvp_u(u, sv.s, n, l, ul);
+    const double ua = m1 * (B.u1 * ul[0] + B.u2 * ul[1] + B.u3 * ul[2]);
+    const double ub = m2 * (B.v1 * ul[0] + B.v2 * ul[1] + B.v3 * ul[2]);
+    const double uc = m3 * (B.w1 * ul[0] + B.w2 * ul[1] + B.w3 * ul[2]);
+    const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+    for (int i = 0; i < p; i++) {
+        const int o1 = m2 * wrap_index(i1 - i, m1);
+        for (int j = 0; j < p; j++) {
+            const long o2 = (long)m3 * (wrap_index(i2 - j, m2) + o1);
+            for (int k = 0; k < p; k++) {
+                const double v = q[l] * (d1[i] * b2[j] * b3[k] * ua + b1[i] * d2[j] * b3[k] * ub +
+                                         b1[i] * b2[j] * d3[k] * uc);
+                atomicAdd(&grid[wrap_index(i3 - k, m3) + o2], v);
+            }
+        }
+    }
+}
+
+// cg *= theta on the half spectrum (the influence function of spme.c:505-583).
+__global__ void k_theta_mul(long H, const double *__restrict__ theta, double *cg)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= H) return;
+    cg[2 * g] *= theta[g];
+    cg[2 * g + 1] *= theta[g];
+}
+
+// Per atom, with phi = theta * Q (grid) and phi_u = theta * Q_u (gridU),
+// f2 = 2 (2 pi / V) C:
+//   hu_l += f2 q_l sum_g [phi (Hess W_l) u_l + phi_u grad W_l]
+//   mq_l += f2 sum_g [phi grad W_l . u_l + phi_u W_l].
+__global__ void __launch_bounds__(256)
+k_hvp_gather(MdgParams P, MdgBox B, int n, const double *__restrict__ x,
+             const double *__restrict__ y, const double *__restrict__ z,
+             const double *__restrict__ q, const double *__restrict__ u, HvpVec sv,
+             const double *__restrict__ grid, const double *__restrict__ gridU, double f2,
+             double *hu, double *mq)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int p = P.nbsp, m[3] = {P.m1, P.m2, P.m3};
+    double s[3];
+    scaled_coords(B, m[0], m[1], m[2], x[l], y[l], z[l], s[0], s[1], s[2]);
+    double b[3][MAXP], d[3][MAXP], dd[3][MAXP];
+    for (int a = 0; a < 3; a++) hvp_splines(s[a], p, b[a], d[a], dd[a]);
+    const double U[3][3] = {{B.u1, B.u2, B.u3}, {B.v1, B.v2, B.v3}, {B.w1, B.w2, B.w3}};
+    double ul[3];
+    hvp_u(u, sv.s, n, l, ul);
+    double ua[3];
+    for (int a = 0; a < 3; a++) ua[a] = m[a] * (U[a][0] * ul[0] + U[a][1] * ul[1] + U[a][2] * ul[2]);
+    const int i0[3] = {(int)s[0], (int)s[1], (int)s[2]};
+    double h[3] = {0., 0., 0.}, mql = 0.;
+    for (int i = 0; i < p; i++) {
+        const int o1 = m[1] * wrap_index(i0[0] - i, m[0]);
+        for (int j = 0; j < p; j++) {
+            const long o2 = (long)m[2] * (wrap_index(i0[1] - j, m[1]) + o1);
+            for (int k = 0; k < p; k++) {
+                const long g = wrap_index(i0[2] - k, m[2]) + o2;
+                const double ph = grid[g], pu = gridU[g];
+                const double w1[3] = {d[0][i] * b[1][j] * b[2][k], b[0][i] * d[1][j] * b[2][k],
+                                      b[0][i] * b[1][j] * d[2][k]};
+                double w2[3][3];
+                w2[0][0] = dd[0][i] * b[1][j] * b[2][k];
+                w2[1][1] = b[0][i] * dd[1][j] * b[2][k];
+                w2[2][2] = b[0][i] * b[1][j] * dd[2][k];
+                w2[0][1] = w2[1][0] = d[0][i] * d[1][j] * b[2][k];
+                w2[0][2] = w2[2][0] = d[0][i] * b[1][j] * d[2][k];
+                w2[1][2] = w2[2][1] = b[0][i] * d[1][j] * d[2][k];
+                for (int a = 0; a < 3; a++) {
+                    const double ca = w2[a][0] * ua[0] + w2[a][1] * ua[1] + w2[a][2] * ua[2];
+                    const double c = (ph * ca + pu * w1[a]) * m[a];
+                    h[0] += c * U[a][0];
+                    h[1] += c * U[a][1];
+                    h[2] += c * U[a][2];
+                    mql += ph * w1[a] * ua[a];
+                }
+                mql += pu * b[0][i] * b[1][j] * b[2][k];
+            }
+        }
+    }
+    for (int c = 0; c < 3; c++) hu[(long)c * n + l] += f2 * q[l] * h[c];
+    mq[l] += f2 * mql;
+}
+
+int mdg_spme_hvp(mdg_ctx *ctx, const double *u, const double s[3], double *hu, double *mq)
+{
+    const int n = ctx->n, m1 = ctx->m1, m2 = ctx->m2, m3 = ctx->m3, mh = ctx->mh;
+    const long G = (long)m1 * m2 * m3, H = (long)m1 * m2 * mh;
+    if (ctx->P.nbsp < 4) {
+        ctx->err = "second derivatives of the splines need order 4 or more";
+        return MDG_EUNSUPPORTED;
+    }
+    if (!ctx->gridU) HIPCHK(hipMalloc(&ctx->gridU, G * sizeof(double)));
+    // phi = theta * Q at these positions, in grid64
+    RETCHK(spme_run(ctx, false, ctx->epart + PART_INFL, ctx->epart + PART_MISC, ctx->frx64,
+                    ctx->fry64, ctx->frz64));
+    HvpVec sv;
+    sv.s[0] = s ? s[0] : 0.;
+    sv.s[1] = s ? s[1] : 0.;
+    sv.s[2] = s ? s[2] : 0.;
+    RETCHK(mdg_fill_f64(ctx, ctx->gridU, G, 0.0));
+    hipLaunchKernelGGL(k_hvp_spread, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                       n, ctx->x, ctx->y, ctx->z, ctx->q, u, sv, ctx->gridU);
+    LAUNCHCHK();
+    void *in[1] = {ctx->gridU}, *out[1] = {ctx->cgrid64};
+    if (rocfft_execute(ctx->fwd, in, out, ctx->info) != rocfft_status_success) {
+        ctx->err = "rocfft_execute (forward) failed";
+        return MDG_EHIP;
+    }
+    hipLaunchKernelGGL(k_theta_mul, dim3(nblk(H, 256)), dim3(256), 0, ctx->stream, H, ctx->theta,
+                       ctx->cgrid64);
+    LAUNCHCHK();
+    void *in2[1] = {ctx->cgrid64}, *out2[1] = {ctx->gridU};
+    if (rocfft_execute(ctx->bwd, in2, out2, ctx->info) != rocfft_status_success) {
+        ctx->err = "rocfft_execute (inverse) failed";
+        return MDG_EHIP;
+    }
+    const double f2 = 2.0 * (TWOPI / ctx->B.vol) * ctx->P.chargeConst;
+    hipLaunchKernelGGL(k_hvp_gather, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
+                       n, ctx->x, ctx->y, ctx->z, ctx->q, u, sv, ctx->grid64, ctx->gridU, f2, hu,
+                       mq);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
 // ---------------------------------------------------------------- replica batch
 
 // Charge spreading for every replica at once: atom l of replica r spreads

@@ -441,6 +441,57 @@ class OracleSystem:
             rec.append((e["pot"] + st["kin"], e["pot"], st["kin"], st["consv"], self.b.vol))
         return 0, f, np.array(rec)
 
+    def sym_exclusions(self):
+        """The exclusion table with each pair at both atoms (sym = 1)."""
+        if not hasattr(self, "_sym"):
+            wb = self.wb
+            off = np.zeros(wb.n + 1, np.int32)
+            cap = 8 * len(wb.ca) + 16
+            lst = np.zeros(cap, np.int32)
+            ne = lib().oc_exclusions(wb.n, 0, None, None, len(wb.ca), P(wb.ca), P(wb.cb), 0, None,
+                                     None, None, 1, P(off), P(lst), cap)
+            if ne < 0:
+                raise RuntimeError(f"oracle exclusions failed: {ne}")
+            self._sym = (off, lst[:ne].copy())
+        return self._sym
+
+    def hvp(self, x, y, z, u):
+        """oc_hvp over the current list: H u (3, n) and d/d(q, eps, sig)
+        (grad E . u)."""
+        n = self.wb.n
+        u = np.ascontiguousarray(u, np.float64).reshape(3, n)
+        hu = np.zeros((3, n))
+        mq, me, ms = np.zeros(n), np.zeros(n), np.zeros(n)
+        off, lst = self.sym_exclusions()
+        lib().oc_hvp(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z), P(self.q), P(self.eps),
+                     P(self.sig), P(self.neighList), P(self.neighPair), P(self.neighOrder), P(off),
+                     P(lst), P(u), P(hu), P(mq), P(me), P(ms))
+        return hu, mq, me, ms
+
+    def spme_hvp(self, x, y, z, u=None):
+        """oc_spme_hvp: H_rec u, d/dq (grad E_rec . u) and grad E_rec."""
+        n = self.wb.n
+        hu, mq, g = np.zeros((3, n)), np.zeros(n), np.zeros((3, n))
+        uu = None if u is None else np.ascontiguousarray(u, np.float64).reshape(3, n)
+        lib().oc_spme_hvp(C.byref(self.p), C.byref(self.b), P(x), P(y), P(z), P(self.q), P(uu),
+                          P(hu), P(mq), P(g))
+        return hu, mq, g
+
+    def traj_adjoint(self, nsteps, x, y, z, vx, vy, vz):
+        """oc_traj_adjoint from (x, v) (not modified): J and dJ/d(q, eps,
+        sig) of the time-averaged potential over steps 1..nsteps."""
+        wb, n = self.wb, self.wb.n
+        J = C.c_double(0)
+        dq, de, ds = np.zeros(n), np.zeros(n), np.zeros(n)
+        err = lib().oc_traj_adjoint(C.byref(self.p), C.byref(self.b), int(self.keyLink),
+                                    int(nsteps), P(x), P(y), P(z), P(vx), P(vy), P(vz), P(self.q),
+                                    P(self.eps), P(self.sig), P(wb.mass), P(wb.rmass),
+                                    P(wb.frozen), P(wb.nAtConst), len(wb.ca), P(wb.ca), P(wb.cb),
+                                    P(wb.rc2), C.byref(J), P(dq), P(de), P(ds))
+        if err:
+            raise RuntimeError(f"oc_traj_adjoint failed: {err}")
+        return J.value, dq, de, ds
+
     def md_run(self, nsteps, x, y, z, vx, vy, vz):
         n = self.wb.n
         f = np.zeros((3, n))
@@ -449,7 +500,7 @@ class OracleSystem:
         wb = self.wb
         err = lib().oc_md_run(C.byref(self.p), C.byref(self.b), int(self.keyLink), int(nsteps),
                               P(x), P(y), P(z), P(vx), P(vy), P(vz), P(f[0]), P(f[1]), P(f[2]),
-                              P(wb.q), P(wb.eps), P(wb.sig), P(wb.mass), P(wb.rmass),
+                              P(self.q), P(self.eps), P(self.sig), P(wb.mass), P(wb.rmass),
                               P(wb.frozen), P(wb.nAtConst), len(wb.ca), P(wb.ca), P(wb.cb),
                               P(wb.rc2), P(et), P(ep), P(ek), C.byref(sec))
         return err, f, et, ep, ek, sec.value

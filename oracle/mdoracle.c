@@ -1918,3 +1918,658 @@ void oc_nb14(const oc_param *p, const oc_box *b, int ewald, int elecType, int vd
     out[1] = evdw;
     out[3] = virvdw;
 }
+
+/* ---------------------------------------------------------------- */
+/* Second derivatives of the Ewald-path potential and the trajectory */
+/* adjoint of the VV + SHAKE dynamics.                               */
+/* ---------------------------------------------------------------- */
+
+/* Radial derivatives of one pair energy phi(r): d1 = phi', d2 = phi''.
+ * Ewald real space, phi = qel erfc(a r)/r (ewald.c:356-369). */
+static void hv_ewald_dir(double a, double qel, double r, double *d1, double *d2)
+{
+    double rt = 1. / r, ar = a * r;
+    double ec = erfc(ar), g = 2. * a * exp(-SQ(ar)) / OC_SQRTPI;
+    *d1 = -qel * (ec * rt + g) * rt;
+    *d2 = qel * (2. * ec * rt * rt * rt + g * (2. * rt * rt + 2. * a * a));
+}
+
+/* Excluded-pair correction, phi = -qel erf(a r)/r (ewald.c:371-384). */
+static void hv_ewald_corr(double a, double qel, double r, double *d1, double *d2)
+{
+    double rt = 1. / r, ar = a * r;
+    double ef = erf(ar), g = 2. * a * exp(-SQ(ar)) / OC_SQRTPI;
+    *d1 = qel * (ef * rt - g) * rt;
+    *d2 = qel * (-2. * ef * rt * rt * rt + g * (2. * rt * rt + 2. * a * a));
+}
+
+/* Switched LJ (vdw.c:144-184), V = pv S with pv = 4 veps (s^12 - s^6),
+ * s = vsig / r, S the CHARMM switch of r^2 (1 up to cutOn): d1 = V',
+ * d2 = V'', dsig = dV'/dvsig. */
+static void hv_lj(const oc_param *p, double veps, double vsig, double r, double *d1,
+                  double *d2, double *dsig)
+{
+    double r2 = r * r, rt = 1. / r, s = vsig * rt;
+    double s2 = SQ(s), s6 = s2 * s2 * s2, s12 = SQ(s6), s5 = s2 * s2 * s, s11 = s6 * s5;
+    double pv = 4. * veps * (s12 - s6);
+    double pv1 = 24. * veps * (s6 - 2. * s12) * rt;
+    double pv2 = 4. * veps * (156. * s12 - 42. * s6) * rt * rt;
+    double pvs = 4. * veps * (12. * s11 - 6. * s5) * rt;            /* d pv / d vsig */
+    double pv1s = 24. * veps * (6. * s5 - 24. * s11) * rt * rt;     /* d pv' / d vsig */
+    double S = 1., S1 = 0., S2 = 0.;
+    if (r2 > p->cutOn2) {
+        double sw1 = p->cutOff2 - r2, on = p->cutOn2 - r2;
+        S = SQ(sw1) * (p->cutOff2 + 2. * r2 - 3. * p->cutOn2) * p->switch2;
+        S1 = 12. * r * sw1 * on * p->switch2;
+        S2 = 12. * p->switch2 * (sw1 * on - 2. * r2 * on - 2. * r2 * sw1);
+    }
+    *d1 = pv1 * S + pv * S1;
+    *d2 = pv2 * S + 2. * pv1 * S1 + pv * S2;
+    *dsig = pv1s * S + pvs * S1;
+}
+
+/* One pair's share of H u: with d = x_j - x_i, g = phi'/r and
+ * h = (phi'' - g)/r^2, grad_j E = g d = -grad_i E and
+ * (H u)_j = -(H u)_i = g du + h (d . du) d,  du = u_j - u_i. */
+static void hv_pair_apply(double d1, double d2, double r, const double d[3], const double du[3],
+                          double w[3])
+{
+    double g = d1 / r, h = (d2 - g) / (r * r);
+    double t = d[0] * du[0] + d[1] * du[1] + d[2] * du[2];
+    for (int c = 0; c < 3; c++) w[c] = g * du[c] + h * t * d[c];
+}
+
+/* B-spline second derivatives in bspgen's layout: bsdd[i] = M_P''(u+i)
+ * = M_{P-2}(u+i) - 2 M_{P-2}(u+i-1) + M_{P-2}(u+i-2). */
+static void hv_bspline2(double s, int P, double *bsdd)
+{
+    double b[16], db[16];
+    oc_bspline(s, P - 2, b, db);
+    for (int i = 0; i < P; i++) {
+        double v0 = i < P - 2 ? b[i] : 0.;
+        double v1 = (i >= 1 && i - 1 < P - 2) ? b[i - 1] : 0.;
+        double v2 = (i >= 2 && i - 2 < P - 2) ? b[i - 2] : 0.;
+        bsdd[i] = v0 - 2. * v1 + v2;
+    }
+}
+
+/* The SPME mesh pieces of the second derivatives.  E_rec = fact sum_g
+ * Q(g) phi(g), phi = theta * Q, Q(g) = sum_l q_l W_l(g), fact = (2pi/V) C;
+ * with f2 = 2 fact and Q_u(g) = sum_l q_l grad W_l(g) . u_l,
+ * phi_u = theta * Q_u:
+ *   grad_l E_rec      = f2 q_l sum_g phi grad W_l             (grad, optional)
+ *   (H_rec u)_l       = f2 q_l sum_g [phi (Hess W_l) u_l + phi_u grad W_l]
+ *   d/dq_l (grad E_rec . u) = f2 sum_g [phi grad W_l . u_l + phi_u W_l].
+ * Mesh, splines, index wrap and influence function as spme_core. */
+static void hv_spme(const oc_param *p, const oc_box *b, const double *x, const double *y,
+                    const double *z, const double *q, const double *u, double *hu, double *mq,
+                    double *grad)
+{
+    int n = p->nAtom, m1 = p->m1, m2 = p->m2, m3 = p->m3, P = p->nbsp;
+    long G = (long)m1 * m2 * m3;
+    int mm[3] = {m1, m2, m3};
+    double U[3][3] = {{b->u1, b->u2, b->u3}, {b->v1, b->v2, b->v3}, {b->w1, b->w2, b->w3}};
+    double *sx = (double *)malloc((size_t)n * 3 * sizeof(double));
+    double *bp = (double *)malloc((size_t)n * 3 * P * sizeof(double));
+    double *bd = (double *)malloc((size_t)n * 3 * P * sizeof(double));
+    double *bdd = (double *)malloc((size_t)n * 3 * P * sizeof(double));
+    ocplx *fq = (ocplx *)calloc((size_t)G, sizeof(ocplx));
+    ocplx *fu = (ocplx *)calloc((size_t)G, sizeof(ocplx));
+    double *md[3];
+    for (int a = 0; a < 3; a++) {
+        md[a] = (double *)malloc((size_t)mm[a] * sizeof(double));
+        oc_bspmod(mm[a], P, md[a]);
+    }
+    for (int i = 0; i < n; i++) {
+        sx[3 * i + 0] = (double)m1 * (x[i] * b->u1 + y[i] * b->u2 + z[i] * b->u3 + 0.5);
+        sx[3 * i + 1] = (double)m2 * (x[i] * b->v1 + y[i] * b->v2 + z[i] * b->v3 + 0.5);
+        sx[3 * i + 2] = (double)m3 * (x[i] * b->w1 + y[i] * b->w2 + z[i] * b->w3 + 0.5);
+        for (int a = 0; a < 3; a++) {
+            oc_bspline(sx[3 * i + a], P, bp + (3 * i + a) * P, bd + (3 * i + a) * P);
+            hv_bspline2(sx[3 * i + a], P, bdd + (3 * i + a) * P);
+        }
+    }
+#define HV_IDX(l, a, i)                                                   \
+    ({                                                                    \
+        int ii_ = (int)sx[3 * (l) + (a)] - (i);                           \
+        if (ii_ >= mm[a]) ii_ = 0;                                        \
+        if (ii_ < 0) ii_ += mm[a];                                        \
+        ii_;                                                              \
+    })
+    /* Q and Q_u */
+    for (int l = 0; l < n; l++) {
+        const double *B[3], *D[3];
+        double ua[3] = {0., 0., 0.};
+        for (int a = 0; a < 3; a++) {
+            B[a] = bp + (3 * l + a) * P;
+            D[a] = bd + (3 * l + a) * P;
+            if (u) ua[a] = mm[a] * (U[a][0] * u[l] + U[a][1] * u[n + l] + U[a][2] * u[2 * n + l]);
+        }
+        for (int i = 0; i < P; i++)
+            for (int j = 0; j < P; j++)
+                for (int k = 0; k < P; k++) {
+                    long g = HV_IDX(l, 2, k) + (long)m3 * (HV_IDX(l, 1, j) + (long)m2 * HV_IDX(l, 0, i));
+                    fq[g] += q[l] * B[0][i] * B[1][j] * B[2][k];
+                    fu[g] += q[l] * (D[0][i] * B[1][j] * B[2][k] * ua[0] +
+                                     B[0][i] * D[1][j] * B[2][k] * ua[1] +
+                                     B[0][i] * B[1][j] * D[2][k] * ua[2]);
+                }
+    }
+    oc_fft3d((double *)fq, m1, m2, m3, +1);
+    oc_fft3d((double *)fu, m1, m2, m3, +1);
+    double rAlpha2 = -0.25 / SQ(p->alpha);
+    double rc = fmin((double)m1 * b->u, (double)m2 * b->v);
+    rc = fmin(rc, (double)m3 * b->w);
+    rc = rc * 1.05 * OC_TWOPI;
+    double rc2 = SQ(rc);
+    for (int i = 0; i < m1; i++) {
+        int ii = i > m1 / 2 ? i - m1 : i;
+        for (int j = 0; j < m2; j++) {
+            int jj = j > m2 / 2 ? j - m2 : j;
+            for (int k = 0; k < m3; k++) {
+                int kk = k > m3 / 2 ? k - m3 : k;
+                long g = k + (long)m3 * (j + (long)m2 * i);
+                double rx = OC_TWOPI * (ii * b->u1 + jj * b->v1 + kk * b->w1);
+                double ry = OC_TWOPI * (ii * b->u2 + jj * b->v2 + kk * b->w2);
+                double rz = OC_TWOPI * (ii * b->u3 + jj * b->v3 + kk * b->w3);
+                double rm = SQ(rx) + SQ(ry) + SQ(rz);
+                double th = 0.;
+                if (rm > DBL_EPSILON && rm <= rc2)
+                    th = md[0][i] * md[1][j] * md[2][k] * exp(rAlpha2 * rm) / rm;
+                fq[g] *= th;
+                fu[g] *= th;
+            }
+        }
+    }
+    oc_fft3d((double *)fq, m1, m2, m3, -1);
+    oc_fft3d((double *)fu, m1, m2, m3, -1);
+    double f2 = 2.0 * (OC_TWOPI / b->vol) * p->chargeConst;
+    for (int l = 0; l < n; l++) {
+        const double *B[3], *D[3], *DD[3];
+        double ul[3] = {0., 0., 0.};
+        if (u) { ul[0] = u[l]; ul[1] = u[n + l]; ul[2] = u[2 * n + l]; }
+        double ua[3];
+        for (int a = 0; a < 3; a++) {
+            B[a] = bp + (3 * l + a) * P;
+            D[a] = bd + (3 * l + a) * P;
+            DD[a] = bdd + (3 * l + a) * P;
+            ua[a] = mm[a] * (U[a][0] * ul[0] + U[a][1] * ul[1] + U[a][2] * ul[2]);
+        }
+        double h[3] = {0., 0., 0.}, gr[3] = {0., 0., 0.}, mql = 0.;
+        for (int i = 0; i < P; i++)
+            for (int j = 0; j < P; j++)
+                for (int k = 0; k < P; k++) {
+                    long g = HV_IDX(l, 2, k) + (long)m3 * (HV_IDX(l, 1, j) + (long)m2 * HV_IDX(l, 0, i));
+                    double ph = creal(fq[g]), pu = creal(fu[g]);
+                    /* dW/ds_a and d2W/ds_a ds_b */
+                    double w1[3] = {D[0][i] * B[1][j] * B[2][k], B[0][i] * D[1][j] * B[2][k],
+                                    B[0][i] * B[1][j] * D[2][k]};
+                    double w2[3][3];
+                    w2[0][0] = DD[0][i] * B[1][j] * B[2][k];
+                    w2[1][1] = B[0][i] * DD[1][j] * B[2][k];
+                    w2[2][2] = B[0][i] * B[1][j] * DD[2][k];
+                    w2[0][1] = w2[1][0] = D[0][i] * D[1][j] * B[2][k];
+                    w2[0][2] = w2[2][0] = D[0][i] * B[1][j] * D[2][k];
+                    w2[1][2] = w2[2][1] = B[0][i] * D[1][j] * D[2][k];
+                    double W = B[0][i] * B[1][j] * B[2][k];
+                    for (int a = 0; a < 3; a++) {
+                        double ca = 0.;   /* (Hess W u_l) along U_a, in s units */
+                        for (int bb = 0; bb < 3; bb++) ca += w2[a][bb] * ua[bb];
+                        for (int c = 0; c < 3; c++) {
+                            h[c] += (ph * ca * mm[a] + pu * w1[a] * mm[a]) * U[a][c];
+                            gr[c] += ph * w1[a] * mm[a] * U[a][c];
+                        }
+                        mql += ph * w1[a] * ua[a];
+                    }
+                    mql += pu * W;
+                }
+        for (int c = 0; c < 3; c++) {
+            if (hu) hu[c * n + l] += f2 * q[l] * h[c];
+            if (grad) grad[c * n + l] = f2 * q[l] * gr[c];
+        }
+        if (mq) mq[l] += f2 * mql;
+    }
+#undef HV_IDX
+    for (int a = 0; a < 3; a++) free(md[a]);
+    free(sx); free(bp); free(bd); free(bdd); free(fq); free(fu);
+}
+
+/* The SPME part alone (test access to hv_spme). */
+void oc_spme_hvp(const oc_param *p, const oc_box *b, const double *x, const double *y,
+                 const double *z, const double *q, const double *u, double *hu, double *mq,
+                 double *grad)
+{
+    hv_spme(p, b, x, y, z, q, u, hu, mq, grad);
+}
+
+/* H u and d/d(q, eps, sig) (grad_x E . u) of the Ewald-path potential at
+ * (x, y, z): real space over the list (pairs within cutOff), the
+ * excluded-pair correction over the symmetric exclusion table exOff/exLst,
+ * SPME.  hu (3n, component-major) and mq, me, ms (n) accumulate. */
+void oc_hvp(const oc_param *p, const oc_box *b, const double *x, const double *y,
+            const double *z, const double *q, const double *eps, const double *sig,
+            const int *neighList, const int *neighPair, const int *neighOrder,
+            const int *exOff, const int *exLst, const double *u, double *hu, double *mq,
+            double *me, double *ms)
+{
+    int n = p->nAtom;
+    double C = p->chargeConst;
+    long offset = 0;
+    for (int l = 0; l < n; l++) {
+        int i = neighOrder[l];
+        for (int k = 0; k < neighPair[i]; k++) {
+            int j = neighList[offset++];
+            double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+            double r2 = oc_dist(b, d);
+            if (r2 > p->cutOff2) continue;
+            double r = sqrt(r2);
+            double du[3] = {u[j] - u[i], u[n + j] - u[n + i], u[2 * n + j] - u[2 * n + i]};
+            double t = d[0] * du[0] + d[1] * du[1] + d[2] * du[2];
+            double e1, e2, l1, l2, ls, w[3];
+            hv_ewald_dir(p->alpha, 1.0, r, &e1, &e2);      /* per unit qel */
+            hv_lj(p, 1.0, sig[i] + sig[j], r, &l1, &l2, &ls);   /* per unit veps */
+            double qel = C * q[i] * q[j], veps = eps[i] * eps[j];
+            hv_pair_apply(qel * e1 + veps * l1, qel * e2 + veps * l2, r, d, du, w);
+            for (int c = 0; c < 3; c++) {
+                hu[c * n + j] += w[c];
+                hu[c * n + i] -= w[c];
+            }
+            mq[i] += C * q[j] * e1 / r * t;
+            mq[j] += C * q[i] * e1 / r * t;
+            me[i] += eps[j] * l1 / r * t;
+            me[j] += eps[i] * l1 / r * t;
+            ms[i] += veps * ls / r * t;
+            ms[j] += veps * ls / r * t;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int k = exOff[i]; k < exOff[i + 1]; k++) {
+            int j = exLst[k];
+            double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+            double r = sqrt(oc_dist(b, d));
+            double du[3] = {u[j] - u[i], u[n + j] - u[n + i], u[2 * n + j] - u[2 * n + i]};
+            double t = d[0] * du[0] + d[1] * du[1] + d[2] * du[2];
+            double c1, c2, w[3];
+            hv_ewald_corr(p->alpha, 1.0, r, &c1, &c2);
+            hv_pair_apply(C * q[i] * q[j] * c1, C * q[i] * q[j] * c2, r, d, du, w);
+            for (int c = 0; c < 3; c++) hu[c * n + i] -= w[c];   /* row i of the pair */
+            mq[i] += C * q[j] * c1 / r * t;
+        }
+    hv_spme(p, b, x, y, z, q, u, hu, mq, NULL);
+}
+
+/* Rigid three-site molecules for the trajectory adjoint: constraint 3w+c
+ * of molecule w joins atoms (a_c, b_c) = (0,1), (0,2), (1,2) of atoms
+ * 3w..3w+2, as the water builder lays them out. */
+static const int TA_CA[3] = {0, 0, 1}, TA_CB[3] = {1, 2, 2};
+
+static int ta_layout_ok(int n, int nConst, const int *ca, const int *cb)
+{
+    if (n % 3 || nConst != n) return 0;
+    for (int w = 0; w < n / 3; w++)
+        for (int c = 0; c < 3; c++)
+            if (ca[3 * w + c] != 3 * w + TA_CA[c] || cb[3 * w + c] != 3 * w + TA_CB[c]) return 0;
+    return 1;
+}
+
+static void ta_solve3(const double M[3][3], const double r[3], double y[3], int trans)
+{
+    double A[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) A[i][j] = trans ? M[j][i] : M[i][j];
+    /* Gaussian elimination with partial pivoting */
+    double bb[3] = {r[0], r[1], r[2]};
+    int piv[3] = {0, 1, 2};
+    for (int c = 0; c < 3; c++) {
+        int best = c;
+        for (int k = c + 1; k < 3; k++)
+            if (fabs(A[piv[k]][c]) > fabs(A[piv[best]][c])) best = k;
+        int t = piv[c]; piv[c] = piv[best]; piv[best] = t;
+        for (int k = c + 1; k < 3; k++) {
+            double f = A[piv[k]][c] / A[piv[c]][c];
+            for (int m = c; m < 3; m++) A[piv[k]][m] -= f * A[piv[c]][m];
+            bb[piv[k]] -= f * bb[piv[c]];
+        }
+    }
+    for (int c = 2; c >= 0; c--) {
+        double s = bb[piv[c]];
+        for (int m = c + 1; m < 3; m++) s -= A[piv[c]][m] * y[m];
+        y[c] = s / A[piv[c]][c];
+    }
+}
+
+/* (R M^-1 D^T)[c][e] for constraint rows r (of R) and d (of D). */
+static void ta_gram(const double r[3][3], const double d[3][3], const double rm[3], double G[3][3])
+{
+    for (int c = 0; c < 3; c++)
+        for (int e = 0; e < 3; e++) {
+            double s = 0.;
+            for (int k = 0; k < 3; k++) {
+                double sr = k == TA_CA[c] ? -1. : (k == TA_CB[c] ? 1. : 0.);
+                double sd = k == TA_CA[e] ? -1. : (k == TA_CB[e] ? 1. : 0.);
+                s += sr * sd * rm[k] * (r[c][0] * d[e][0] + r[c][1] * d[e][1] + r[c][2] * d[e][2]);
+            }
+            G[c][e] = s;
+        }
+}
+
+/* (R v)_c = r_c . (w_b v_b - w_a v_a), weights w (NULL: 1). */
+static void ta_apply(const double r[3][3], const double v[3][3], const double *w, double out[3])
+{
+    for (int c = 0; c < 3; c++) {
+        double wa = w ? w[TA_CA[c]] : 1., wb = w ? w[TA_CB[c]] : 1., s = 0.;
+        for (int k = 0; k < 3; k++) s += r[c][k] * (wb * v[TA_CB[c]][k] - wa * v[TA_CA[c]][k]);
+        out[c] = s;
+    }
+}
+
+static void ta_add_rt(const double r[3][3], const double z[3], double s, double v[3][3])
+{
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 3; k++) {
+            v[TA_CA[c]][k] -= s * z[c] * r[c][k];
+            v[TA_CB[c]][k] += s * z[c] * r[c][k];
+        }
+}
+
+static void ta_dirs(const oc_box *b, const double X[3][3], double r[3][3])
+{
+    for (int c = 0; c < 3; c++) {
+        for (int k = 0; k < 3; k++) r[c][k] = X[TA_CB[c]][k] - X[TA_CA[c]][k];
+        oc_dist2(b, &r[c][0], &r[c][1], &r[c][2]);
+    }
+}
+
+/* Saved state of step t: x, y, z, vx, vy, vz, fx, fy, fz (n each). */
+typedef struct {
+    double xp[3][3], xt[3][3], vt[3][3], dx[3][3], vtil[3][3], rm[3];
+    int at[3];
+} ta_mol;
+
+/* Molecule w at steps t-1 (S0) and t (S1): v' = v0 + dt/2 F0/m,
+ * x~ = x0 + dt v', dx = x1 - x~ (minimum image: the wrap is a lattice
+ * shift), v~ = v' + dx/dt + dt/2 F1/m (integrate.c:1518-1634). */
+static void ta_load(const oc_box *b, int n, double dt, const double *S0, const double *S1,
+                    const double *rmass, int w, ta_mol *M)
+{
+    for (int k = 0; k < 3; k++) {
+        int i = 3 * w + k;
+        M->at[k] = i;
+        M->rm[k] = rmass[i];
+        double d[3];
+        for (int c = 0; c < 3; c++) {
+            double vp = S0[(3 + c) * n + i] + 0.5 * dt * S0[(6 + c) * n + i] * rmass[i];
+            double xtil = S0[c * n + i] + dt * vp;
+            M->xp[k][c] = S0[c * n + i];
+            M->xt[k][c] = S1[c * n + i];
+            M->vt[k][c] = S1[(3 + c) * n + i];
+            d[c] = S1[c * n + i] - xtil;
+            M->vtil[k][c] = vp;
+        }
+        oc_dist2(b, &d[0], &d[1], &d[2]);
+        for (int c = 0; c < 3; c++) {
+            M->dx[k][c] = d[c];
+            M->vtil[k][c] += d[c] / dt + 0.5 * dt * S1[(6 + c) * n + i] * rmass[i];
+        }
+    }
+}
+
+/* Stage 2, SHAKE-V as the projection v = v~ + M^-1 B^T mu with B(x_t) v = 0:
+ * vb (adjoint of v_t) -> vb2 (adjoint of v~), x_t's part added to xb. */
+static void ta_stage2(const oc_box *b, int n, const ta_mol *M, const double *vb, double *xb,
+                      double *vb2)
+{
+    double r[3][3], S[3][3], dv[3][3], vbar[3][3], t[3], mu[3], kap[3];
+    ta_dirs(b, M->xt, r);
+    ta_gram(r, r, M->rm, S);
+    for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 3; c++) {
+            dv[k][c] = M->vt[k][c] - M->vtil[k][c];
+            vbar[k][c] = vb[c * n + M->at[k]];
+        }
+    ta_apply(r, dv, NULL, t);
+    ta_solve3(S, t, mu, 0);
+    ta_apply(r, vbar, M->rm, t);
+    ta_solve3(S, t, kap, 0);
+    double vtb[3][3];
+    memcpy(vtb, vbar, sizeof vtb);
+    ta_add_rt(r, kap, -1., vtb);
+    for (int c = 0; c < 3; c++) {
+        int ia = M->at[TA_CA[c]], ib = M->at[TA_CB[c]];
+        for (int k = 0; k < 3; k++) {
+            double G = mu[c] * (M->rm[TA_CB[c]] * vtb[TA_CB[c]][k] - M->rm[TA_CA[c]] * vtb[TA_CA[c]][k]) -
+                       kap[c] * (M->vt[TA_CB[c]][k] - M->vt[TA_CA[c]][k]);
+            xb[k * n + ib] += G;
+            xb[k * n + ia] -= G;
+        }
+    }
+    for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 3; c++) vb2[c * n + M->at[k]] = vtb[k][c];
+}
+
+/* Stage 1, drift + SHAKE-R as x_t = x~ + M^-1 A(x_{t-1})^T gamma with the
+ * constraints met at x_t, and v'' = v' + (x_t - x~)/dt: from xb (adjoint
+ * of x_t) and vb2 (of v''), the adjoints of x_{t-1} (into xb), v_{t-1}
+ * (into vb) and F_{t-1} (into fp). */
+static void ta_stage1(const oc_box *b, int n, double dt, const ta_mol *M, double *xb,
+                      const double *vb2, double *vb, double *fp)
+{
+    double dd[3][3], r[3][3], Q[3][3], T[3][3], t[3], gam[3], z[3];
+    ta_dirs(b, M->xp, dd);
+    ta_dirs(b, M->xt, r);
+    ta_gram(dd, dd, M->rm, Q);
+    ta_gram(r, dd, M->rm, T);
+    ta_apply(dd, M->dx, NULL, t);
+    ta_solve3(Q, t, gam, 0);
+    double xe[3][3], v2b[3][3];
+    for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 3; c++) {
+            v2b[k][c] = vb2[c * n + M->at[k]];
+            xe[k][c] = xb[c * n + M->at[k]] + v2b[k][c] / dt;
+        }
+    ta_apply(dd, xe, M->rm, t);
+    ta_solve3(T, t, z, 1);
+    double ub[3][3];
+    memcpy(ub, xe, sizeof ub);
+    ta_add_rt(r, z, -1., ub);
+    double xtb[3][3], xpb[3][3];
+    for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 3; c++) {
+            xtb[k][c] = ub[k][c] - v2b[k][c] / dt;
+            xpb[k][c] = xtb[k][c];
+        }
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 3; k++) {
+            double coef = gam[c] * (M->rm[TA_CB[c]] * ub[TA_CB[c]][k] - M->rm[TA_CA[c]] * ub[TA_CA[c]][k]);
+            xpb[TA_CB[c]][k] += coef;
+            xpb[TA_CA[c]][k] -= coef;
+        }
+    for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 3; c++) {
+            int o = c * n + M->at[k];
+            double vpb = v2b[k][c] + dt * xtb[k][c];
+            xb[o] = xpb[k][c];
+            vb[o] = vpb;
+            fp[o] = 0.5 * dt * M->rm[k] * vpb;
+        }
+}
+
+/* Forces F = -grad E + (1/N) sum_k grad_k E_rec (spme.c:648-670 removes
+ * the mean reciprocal force):
+ *   (dF/dx)^T u     = -H u + (1/N) H_rec (s, ..., s),       s = sum_l u_l
+ *   (dF/dtheta)^T u = -d/dtheta (grad E . u) + (1/N) d/dq (grad E_rec . (s, ..., s)).
+ * Adds the first to xb (when xb) and the second to gq, ge, gs. */
+static void ta_force_vjp(const oc_param *p, const oc_box *b, const double *x, const double *y,
+                         const double *z, const double *q, const double *eps, const double *sig,
+                         const int *nl, const int *np, const int *no, const int *exOff,
+                         const int *exLst, const double *u, double *xb, double *gq, double *ge,
+                         double *gs)
+{
+    int n = p->nAtom;
+    double *hu = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *mq = (double *)calloc((size_t)n, sizeof(double));
+    double *me = (double *)calloc((size_t)n, sizeof(double));
+    double *ms = (double *)calloc((size_t)n, sizeof(double));
+    double *us = (double *)malloc((size_t)3 * n * sizeof(double));
+    oc_hvp(p, b, x, y, z, q, eps, sig, nl, np, no, exOff, exLst, u, hu, mq, me, ms);
+    for (int i = 0; i < n; i++) {
+        gq[i] -= mq[i];
+        ge[i] -= me[i];
+        gs[i] -= ms[i];
+    }
+    if (xb)
+        for (long k = 0; k < 3L * n; k++) xb[k] -= hu[k];
+    double s[3] = {0., 0., 0.};
+    for (int c = 0; c < 3; c++)
+        for (int i = 0; i < n; i++) s[c] += u[c * n + i];
+    for (int c = 0; c < 3; c++)
+        for (int i = 0; i < n; i++) us[c * n + i] = s[c];
+    memset(hu, 0, (size_t)3 * n * sizeof(double));
+    memset(mq, 0, (size_t)n * sizeof(double));
+    hv_spme(p, b, x, y, z, q, us, hu, mq, NULL);
+    for (int i = 0; i < n; i++) gq[i] += mq[i] / (double)n;
+    if (xb)
+        for (long k = 0; k < 3L * n; k++) xb[k] += hu[k] / (double)n;
+    free(hu); free(mq); free(me); free(ms); free(us);
+}
+
+/* J = (1/T) sum_{t=1..T} E_pot(x_t) over T steps of the MD loop of
+ * oc_md_run (NVE, VV + SHAKE, list update every p->update steps) from
+ * (x0, v0), and its gradient with respect to q, eps, sig by a reverse
+ * sweep over the stored steps: the objective's own terms, SHAKE-V and
+ * SHAKE-R by the implicit function theorem on each rigid molecule, and
+ * the force vector-Jacobian products by ta_force_vjp, ending with F_0.
+ * Rigid three-site molecules only (ta_layout_ok), no frozen atoms.
+ * Returns 0, 311 (SHAKE), or 1 for a layout it does not cover. */
+int oc_traj_adjoint(const oc_param *p, const oc_box *b, int keyLink, int T,
+                    const double *x0, const double *y0, const double *z0,
+                    const double *vx0, const double *vy0, const double *vz0,
+                    const double *q, const double *eps, const double *sig,
+                    const double *mass, const double *rmass, const int *frozen,
+                    const int *nAtConst, int nConst, const int *ca, const int *cb,
+                    const double *rc2, double *J, double *dq, double *deps, double *dsig)
+{
+    int n = p->nAtom, err = 0;
+    if (T < 1 || !ta_layout_ok(n, nConst, ca, cb)) return 1;
+    for (int i = 0; i < n; i++)
+        if (frozen && frozen[i]) return 1;
+    long sn = 9L * n;
+    double *S = (double *)malloc((size_t)(T + 1) * sn * sizeof(double));
+    int *exOff = (int *)malloc((size_t)(n + 1) * sizeof(int));
+    int *sxOff = (int *)malloc((size_t)(n + 1) * sizeof(int));
+    int excap = 8 * nConst + 16;
+    int *exLst = (int *)malloc((size_t)excap * sizeof(int));
+    int *sxLst = (int *)malloc((size_t)excap * sizeof(int));
+    int ne = oc_exclusions(n, 0, NULL, NULL, nConst, ca, cb, 0, NULL, NULL, NULL, keyLink ? 1 : 0,
+                           exOff, exLst, excap);
+    int ns = oc_exclusions(n, 0, NULL, NULL, nConst, ca, cb, 0, NULL, NULL, NULL, 1, sxOff, sxLst,
+                           excap);
+    oc_list L;
+    L.cap = 1024;
+    L.nl = (int *)malloc((size_t)L.cap * sizeof(int));
+    L.np = (int *)malloc((size_t)n * sizeof(int));
+    L.no = (int *)malloc((size_t)n * sizeof(int));
+    double *xb = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *vb = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *vb2 = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *fp = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *fbar = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *gr = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *tq = (double *)calloc((size_t)n, sizeof(double));
+    double *te = (double *)calloc((size_t)n, sizeof(double));
+    double *ts = (double *)calloc((size_t)n, sizeof(double));
+    if (ne < 0 || ns < 0) { err = 1; goto done; }
+    for (int i = 0; i < n; i++) { dq[i] = 0.; deps[i] = 0.; dsig[i] = 0.; }
+    {
+        /* forward, as oc_md_run, keeping (x, v, F) of every step */
+        double *x = S, *y = S + n, *z = S + 2 * n;
+        memcpy(x, x0, n * sizeof(double)); memcpy(y, y0, n * sizeof(double));
+        memcpy(z, z0, n * sizeof(double)); memcpy(S + 3 * n, vx0, n * sizeof(double));
+        memcpy(S + 4 * n, vy0, n * sizeof(double)); memcpy(S + 5 * n, vz0, n * sizeof(double));
+        err = md_list(p, b, keyLink, 1, x, y, z, frozen, exOff, exLst, &L);
+        double Jsum = 0.;
+        for (int t = 0; t <= T && !err; t++) {
+            double *St = S + sn * t;
+            if (t > 0) {
+                memcpy(St, St - sn, (size_t)sn * sizeof(double));
+                err = oc_vv_nve(p, b, 1, nConst, ca, cb, rc2, St, St + n, St + 2 * n, St + 3 * n,
+                                St + 4 * n, St + 5 * n, St + 6 * n, St + 7 * n, St + 8 * n, mass,
+                                rmass, nAtConst, NULL, NULL, NULL);
+                if (err) break;
+                if (t % p->update == 0)
+                    err = md_list(p, b, keyLink, 0, St, St + n, St + 2 * n, frozen, exOff, exLst, &L);
+                if (err) break;
+            }
+            double *f = St + 6 * n;
+            memset(f, 0, (size_t)3 * n * sizeof(double));
+            double st[6], vr;
+            double eRec = oc_spme(p, b, St, St + n, St + 2 * n, f, f + n, f + 2 * n, q, st, &vr);
+            oc_energy e;
+            oc_ewald_real(p, b, St, St + n, St + 2 * n, f, f + n, f + 2 * n, q, eps, sig, L.nl, L.np,
+                          L.no, exOff, exLst, &e);
+            if (t > 0) {
+                Jsum += (e.eDir + eRec + e.eCorr) + e.eVdw;
+                err = oc_vv_nve(p, b, 2, nConst, ca, cb, rc2, St, St + n, St + 2 * n, St + 3 * n,
+                                St + 4 * n, St + 5 * n, f, f + n, f + 2 * n, mass, rmass, nAtConst,
+                                NULL, NULL, NULL);
+            }
+        }
+        if (err) goto done;
+        *J = Jsum / (double)T;
+    }
+    /* reverse sweep */
+    for (int t = T; t >= 1 && !err; t--) {
+        const double *Sc = S + sn * t, *Sp = S + sn * (t - 1);
+        const double *x = Sc, *y = Sc + n, *z = Sc + 2 * n, *f = Sc + 6 * n;
+        /* the list of x_t (the in-cutoff set of any valid list) */
+        double *xc = (double *)malloc((size_t)3 * n * sizeof(double));
+        memcpy(xc, Sc, (size_t)3 * n * sizeof(double));
+        err = md_list(p, b, keyLink, 0, xc, xc + n, xc + 2 * n, frozen, exOff, exLst, &L);
+        free(xc);
+        if (err) break;
+        /* objective: (1/T) dE/dtheta and (1/T) grad E = -(1/T)(F + m) */
+        oc_energy_grad(p, b, x, y, z, q, eps, sig, L.nl, L.np, L.no, exOff, exLst, tq, te, ts);
+        for (int i = 0; i < n; i++) {
+            dq[i] += tq[i] / (double)T;
+            deps[i] += te[i] / (double)T;
+            dsig[i] += ts[i] / (double)T;
+        }
+        hv_spme(p, b, x, y, z, q, NULL, NULL, NULL, gr);
+        double m[3] = {0., 0., 0.};
+        for (int c = 0; c < 3; c++) {
+            for (int i = 0; i < n; i++) m[c] -= gr[c * n + i];
+            m[c] /= (double)n;
+        }
+        for (int c = 0; c < 3; c++)
+            for (int i = 0; i < n; i++) xb[c * n + i] -= (f[c * n + i] + m[c]) / (double)T;
+        for (int w = 0; w < n / 3; w++) {
+            ta_mol M;
+            ta_load(b, n, p->timeStep, Sp, Sc, rmass, w, &M);
+            ta_stage2(b, n, &M, vb, xb, vb2);
+        }
+        for (long k = 0; k < 3L * n; k++)
+            fbar[k] = 0.5 * p->timeStep * rmass[k % n] * vb2[k] + fp[k];
+        ta_force_vjp(p, b, x, y, z, q, eps, sig, L.nl, L.np, L.no, sxOff, sxLst, fbar, xb, dq,
+                     deps, dsig);
+        for (int w = 0; w < n / 3; w++) {
+            ta_mol M;
+            ta_load(b, n, p->timeStep, Sp, Sc, rmass, w, &M);
+            ta_stage1(b, n, p->timeStep, &M, xb, vb2, vb, fp);
+        }
+    }
+    if (!err) {
+        /* F_0 depends on the parameters too */
+        double *xc = (double *)malloc((size_t)3 * n * sizeof(double));
+        memcpy(xc, S, (size_t)3 * n * sizeof(double));
+        err = md_list(p, b, keyLink, 0, xc, xc + n, xc + 2 * n, frozen, exOff, exLst, &L);
+        free(xc);
+        if (!err)
+            ta_force_vjp(p, b, S, S + n, S + 2 * n, q, eps, sig, L.nl, L.np, L.no, sxOff, sxLst, fp,
+                         NULL, dq, deps, dsig);
+    }
+done:
+    free(S); free(exOff); free(sxOff); free(exLst); free(sxLst);
+    free(L.nl); free(L.np); free(L.no);
+    free(xb); free(vb); free(vb2); free(fp); free(fbar); free(gr); free(tq); free(te); free(ts);
+    return err;
+}

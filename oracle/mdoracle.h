@@ -33,6 +33,12 @@
  *                                          in q and E_vdw in eps.
  *   oc_vv_nve / oc_shake_*               : compiled reference vv_integrate /
  *                                          vv_shake_r / vv_shake_v.
+ *   oc_hvp, oc_spme_hvp                  : central differences of the oracle's
+ *                                          own forces and reciprocal gradient;
+ *                                          the symmetry v.Hu = u.Hv; parameter
+ *                                          differences of grad E . u.
+ *   oc_traj_adjoint                      : central differences of J over whole
+ *                                          oracle trajectories (SHAKE to 1e-12).
  *   oc_vv_ens                            : compiled reference vv_integrate for
  *                                          NVT_B, NPT_B, NVT_H, NPT_H (with and
  *                                          without constraints); the Berendsen
@@ -227,6 +233,42 @@ int oc_md_run(const oc_param *p, const oc_box *b, int keyLink, int nsteps,
               const int *nAtConst, int nConst, const int *ca, const int *cb,
               const double *rc2, double *etot, double *epot, double *ekin,
               double *seconds);
+
+/* ---- second derivatives and the trajectory adjoint -------------- */
+
+/* H u (hu, 3n component-major) and d/d(q, eps, sig) (grad_x E . u) (mq,
+ * me, ms) of the Ewald-path potential at (x, y, z): real space over the
+ * list (pairs within cutOff), the excluded-pair correction over the
+ * SYMMETRIC exclusion table (sym = 1 of oc_exclusions), SPME (splines,
+ * their first and second derivatives, the dipole mesh sum_l q_l grad W_l
+ * . u_l through the influence function).  All outputs accumulate. */
+void oc_hvp(const oc_param *p, const oc_box *b, const double *x, const double *y,
+            const double *z, const double *q, const double *eps, const double *sig,
+            const int *neighList, const int *neighPair, const int *neighOrder,
+            const int *exOff, const int *exLst, const double *u, double *hu, double *mq,
+            double *me, double *ms);
+
+/* The SPME part of oc_hvp; grad (3n, may be NULL) receives the raw
+ * reciprocal gradient grad E_rec (before any mean removal); u may be NULL
+ * when only grad is wanted. */
+void oc_spme_hvp(const oc_param *p, const oc_box *b, const double *x, const double *y,
+                 const double *z, const double *q, const double *u, double *hu, double *mq,
+                 double *grad);
+
+/* J = (1/T) sum_{t=1..T} E_pot(x_t) of T steps of the NVE MD loop of
+ * oc_md_run from (x0, v0) (not modified), and dJ/d(q, eps, sig) by a
+ * reverse sweep over the stored steps: SHAKE-V and SHAKE-R through the
+ * implicit function theorem on each rigid molecule, the forces through
+ * oc_hvp with the mean-force removal of spme.c:648-670.  Rigid three-site
+ * molecules (constraints (0,1), (0,2), (1,2) of atoms 3w..3w+2), no frozen
+ * atoms.  Returns 0, 311 (SHAKE) or 1 (layout not covered). */
+int oc_traj_adjoint(const oc_param *p, const oc_box *b, int keyLink, int T,
+                    const double *x0, const double *y0, const double *z0,
+                    const double *vx0, const double *vy0, const double *vz0,
+                    const double *q, const double *eps, const double *sig,
+                    const double *mass, const double *rmass, const int *frozen,
+                    const int *nAtConst, int nConst, const int *ca, const int *cb,
+                    const double *rc2, double *J, double *dq, double *deps, double *dsig);
 
 /* ---- bonded terms and 1-4 pairs ---------------------------------- */
 

@@ -431,6 +431,30 @@ def test_backward_traj_200_steps_against_finite_differences(c0_box):
         assert abs(g - fd) <= 1e-3 * (scale or abs(fd)), name
 
 
+@pytest.mark.parametrize("T,K", [(1, 1), (5, 5), (50, 20), (200, 20)])
+def test_backward_traj_against_oracle_adjoint(c0_box, T, K):
+    """The device trajectory adjoint (analytic Hessian-vector products,
+    checkpoints every K steps) against the oracle's FP64 adjoint of the same
+    T-step objective (oc_traj_adjoint, every step stored, pinned against
+    differences of J on the CPU): J to 1e-12, each of dJ/dq, dJ/deps and
+    dJ/dsig to 1e-8 relative (norm over atoms).  (200, 20) is the c3
+    configuration."""
+    wb = c0_box
+    o = O.OracleSystem(wb)
+    say("oracle adjoint T", T)
+    Jo, dq, de, ds = o.traj_adjoint(T, wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    ctx = M.Context(wb, precision=M.MDG_FP64)
+    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    ctx.initial_energy()
+    J, g = ctx.backward_traj(T, K)
+    ctx.close()
+    errs = [relerr(g[0], dq), relerr(g[1], de), relerr(g[2], ds)]
+    say("traj adjoint vs oracle T", T, "J", J, Jo, "rel", errs)
+    assert J == pytest.approx(Jo, rel=1e-12)
+    for e in errs:
+        assert e < 1e-8
+
+
 # ------------------------------------------------------------------ replica batch
 
 @pytest.fixture(scope="module")
