@@ -464,13 +464,14 @@ struct MxAcc {
     int nin;                         // in-cutoff pairs (wave-uniform)
 };
 
-// floor(v * 2^32) as int64 for |v| < 2^31: the integer part and the
-// fraction are split in FP64, where both steps are exact for an FP32 v.
+// floor(v * 2^32) as int64 for |v| < 2^31: the integer part floor(v) and
+// the fraction v - floor(v) are both exact in FP32 (the fraction keeps a
+// subset of v's significand bits), and so is the fraction times 2^32
+// (< 2^32), whose truncation gives the low word.
 __device__ inline long long mdg_f32_fix(float v)
 {
-    const double d = (double)v;
-    const double fl = floor(d);
-    const unsigned int lo = (unsigned int)((d - fl) * 4294967296.0);
+    const float fl = floorf(v);
+    const unsigned int lo = (unsigned int)((v - fl) * 4294967296.0f);
     return (long long)(((unsigned long long)(unsigned int)(int)fl << 32) | lo);
 }
 

@@ -373,6 +373,28 @@ def test_spme_energy_force_consistency(small_box):
         assert f[0, i] + fm[0] == pytest.approx(-(ep - em) / (2 * h), rel=1e-5, abs=1e-3)
 
 
+def test_spme_invariant_under_mesh_translations(small_box):
+    """Moving every charge by one mesh spacing along an axis shifts the
+    charge mesh cyclically by one point, which changes the structure factor
+    by a phase only: the SPME energy and the forces are unchanged (to
+    rounding).  So a consistent one-point shift of the spline support (the
+    mesh-index convention of spme.c:449-491) changes no observable either;
+    that convention is the reference's as read, not pinned by any output."""
+    wb = small_box
+    o = O.OracleSystem(wb)
+    x, y, z = wb.x.copy(), wb.y.copy(), wb.z.copy()
+    # positions in the box: shifted, the scaled coordinates stay below
+    # m + 1, where the reference's index wrap (spme.c:453-459) is exact
+    e0, f0, _, _ = o.spme(x, y, z)
+    h = wb.L / wb.params["grid"]
+    for axis in range(3):
+        s = [x.copy(), y.copy(), z.copy()]
+        s[axis] += h
+        e1, f1, _, _ = o.spme(*s)
+        assert e1 == pytest.approx(e0, rel=1e-12)
+        assert np.abs(f1 - f0).max() < 1e-9 * np.abs(f0).max()
+
+
 # ------------------------------------------------------------------ integrator
 
 def test_constraint_velocity_projection_matches_reference(gd):
