@@ -182,6 +182,48 @@ def oracle_forces(wb):
     return o.energy(x, y, z)
 
 
+def drift_and_fp64(args, M, wb, cfg, name, ctx, sync):
+    """Energy drift over a fixed window of DRIFT_STEPS steps after the
+    warm-up, mixed and FP64 from the same state (the FP64 run is also timed:
+    ns/day of the plain FP64 method on the GPU), then both device paths
+    from the initial state for the CPU leg's steps, to set beside the
+    oracle's trajectory.  Closes ctx."""
+    dt, W = cfg["timeStep"], args.warmup
+    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    ctx.initial_energy()
+    ctx.run(W, use_graph=True)
+    rd = M.records_to_dict(ctx.run(DRIFT_STEPS, use_graph=True))
+    drift_mixed = drift_per_water(rd, dt, wb.n_waters)
+    c64 = M.Context(wb, precision=M.MDG_FP64)
+    c64.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    c64.initial_energy()
+    c64.run(W, use_graph=True)
+    sync()
+    t0 = time.perf_counter()
+    r64 = M.records_to_dict(c64.run(DRIFT_STEPS, use_graph=True))
+    sync()
+    wall64 = time.perf_counter() - t0
+    drift_64 = drift_per_water(r64, dt, wb.n_waters)
+    fp64_timing = {"ns_per_day": dt * 1e-3 * DRIFT_STEPS * 86400.0 / wall64,
+                   "ms_per_step": wall64 / DRIFT_STEPS * 1e3, "steps": DRIFT_STEPS,
+                   "launch": "default shapes (not autotuned)"}
+    log(f"FP64 on the GPU: {fp64_timing}; drift mixed {drift_mixed:.3f} fp64 {drift_64:.3f}")
+
+    # both device paths from the initial state for the CPU leg's steps, to
+    # set beside the oracle's trajectory
+    ncpu = CPU_STEPS[name]
+    dev_tot = {}
+    for prec, c in ((M.MDG_MIXED, ctx), (M.MDG_FP64, c64)):
+        c.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        r0 = c.initial_energy()
+        rr = M.records_to_dict(c.run(ncpu, use_graph=True))
+        dev_tot[prec] = np.concatenate([[r0.tot], rr["tot"]])
+    c64.close()
+    ctx.close()
+
+    return drift_mixed, drift_64, fp64_timing, dev_tot
+
+
 def single_box(args, M, configs, name):
     cfg = configs.CONFIGS[name]
     wb = configs.build(cfg)
@@ -260,44 +302,18 @@ def single_box(args, M, configs, name):
     G = cfg["grid"] ** 3
     spme_bytes = 56 * N + 32 * G
 
-    # energy drift over a fixed window of DRIFT_STEPS steps after the
-    # warm-up, mixed and FP64 from the same state; the FP64 run is also
-    # timed (ns/day of the plain FP64 method on the GPU)
-    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
-    ctx.initial_energy()
-    ctx.run(W, use_graph=True)
-    rd = M.records_to_dict(ctx.run(DRIFT_STEPS, use_graph=True))
-    drift_mixed = drift_per_water(rd, dt, wb.n_waters)
-    c64 = M.Context(wb, precision=M.MDG_FP64)
-    c64.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
-    c64.initial_energy()
-    c64.run(W, use_graph=True)
-    sync()
-    t0 = time.perf_counter()
-    r64 = M.records_to_dict(c64.run(DRIFT_STEPS, use_graph=True))
-    sync()
-    wall64 = time.perf_counter() - t0
-    drift_64 = drift_per_water(r64, dt, wb.n_waters)
-    fp64_timing = {"ns_per_day": dt * 1e-3 * DRIFT_STEPS * 86400.0 / wall64,
-                   "ms_per_step": wall64 / DRIFT_STEPS * 1e3, "steps": DRIFT_STEPS,
-                   "launch": "default shapes (not autotuned)"}
-    log(f"FP64 on the GPU: {fp64_timing}; drift mixed {drift_mixed:.3f} fp64 {drift_64:.3f}")
-
-    # both device paths from the initial state for the CPU leg's steps, to
-    # set beside the oracle's trajectory
-    ncpu = CPU_STEPS[name]
+    drift_mixed = drift_64 = fp64_timing = None
     dev_tot = {}
-    for prec, c in ((M.MDG_MIXED, ctx), (M.MDG_FP64, c64)):
-        c.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
-        r0 = c.initial_energy()
-        rr = M.records_to_dict(c.run(ncpu, use_graph=True))
-        dev_tot[prec] = np.concatenate([[r0.tot], rr["tot"]])
-    c64.close()
-    ctx.close()
+    if args.profile_run:
+        ctx.close()
+    else:
+        drift_mixed, drift_64, fp64_timing, dev_tot = drift_and_fp64(args, M, wb, cfg, name, ctx,
+                                                                     sync)
 
     cpu = None
     traj_vs_oracle = None
-    if not args.no_cpu:
+    ncpu = CPU_STEPS[name]
+    if not args.no_cpu and not args.profile_run:
         log(f"CPU baseline (FP64 oracle, 1 core, {ncpu} steps)...")
         sec, et = oracle_md(wb, ncpu)
         scale = abs(et[0])
@@ -305,8 +321,12 @@ def single_box(args, M, configs, name):
             "steps": ncpu,
             "max_rel_dEtot_fp64": float(np.abs(dev_tot[M.MDG_FP64] - et).max() / scale),
             "max_rel_dEtot_mixed": float(np.abs(dev_tot[M.MDG_MIXED] - et).max() / scale),
-            "oracle_drift_kcal_mol_ns_per_water": drift_per_water({"tot": et[1:]}, dt,
-                                                                  wb.n_waters)}
+            # a 0.2 ps window: the fitted slope is mostly the energy's own
+            # fluctuation, so it is given for the three runs side by side
+            "window_slope_kcal_mol_ns_per_water": {
+                "oracle": drift_per_water({"tot": et[1:]}, dt, wb.n_waters),
+                "fp64": drift_per_water({"tot": dev_tot[M.MDG_FP64][1:]}, dt, wb.n_waters),
+                "mixed": drift_per_water({"tot": dev_tot[M.MDG_MIXED][1:]}, dt, wb.n_waters)}}
         log(f"trajectory vs oracle: {traj_vs_oracle}")
         nreb = ncpu // cfg["update"]
         cpu = dict(value=dt * 1e-3 * ncpu * 86400.0 / sec, unit="ns/day", cores=1,
@@ -453,6 +473,9 @@ def main():
                          "default: the library's choice for the box)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (skip the autotuner)")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="stop after the timed and profiled passes (no drift, FP64 or CPU "
+                         "legs): the short run a profiler wraps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (the known-size run that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
