@@ -143,6 +143,8 @@ typedef struct {
     double ebond, eub, eang, edihe, eimpr;   /* bonded terms (ENERGY bond, ub, ang, dihe, impr);
                                               * elec / vdw include the 1-4 pairs */
     double virbond, virub;             /* bonded virials (ENERGY virbond, virub) */
+    double conint, consv, chiT;        /* thermostat of the device step (mdg_set_bath): ENERGY
+                                        * conint and consv, BATH chiT; zero for NVE */
 } MDG_STEP;
 
 /* ---- context ---------------------------------------------------------- */
@@ -302,6 +304,21 @@ int mdg_upload_state(mdg_ctx *ctx, const double x[], const double y[], const dou
                      const double vx[], const double vy[], const double vz[]);
 int mdg_download_state(mdg_ctx *ctx, double x[], double y[], double z[], double vx[],
                        double vy[], double vz[], double fx[], double fy[], double fz[]);
+
+/* The thermostat of the device-resident step (mdg_run_steps): ctrl->ens
+ * NVE, NVT_B (vv_nvt_b, integrate.c:1636-1768) or NVT_H (vv_nvt_h,
+ * integrate.c:1998-2169) with bath->tauT, bath->chiT, param->kinTemp0 and
+ * ener->conint as the starting state.  The bath scalars live on the
+ * device and are advanced by one-block kernels between the kinetic-energy
+ * partials and a velocity-scaling pass, inside the step (and its graph);
+ * each MDG_STEP carries conint, consv and chiT.  The barostats (NPT_B,
+ * NPT_H) change the box, which the device step holds fixed: they return
+ * MDG_EUNSUPPORTED here and run through mdg_vv_integrate.  A context made
+ * with ctrl->ens != NVE runs mdg_run_steps only after this call. */
+int mdg_set_bath(mdg_ctx *ctx, const MDG_CTRL *ctrl, const MDG_PARAM *param,
+                 const MDG_BATH *bath, const MDG_ENERGY *ener);
+/* The device thermostat's state now: bath->chiT, ener->conint, ener->consv. */
+int mdg_get_bath(mdg_ctx *ctx, MDG_BATH *bath, MDG_ENERGY *ener);
 
 /* Force and energy of the uploaded state (the step-0 energy() call of
  * mdbas.c:174), building the device list first. */

@@ -90,7 +90,7 @@ class MDG_IO(C.Structure):
 class MDG_STEP(C.Structure):
     _fields_ = [("step", C.c_int), ("listRebuilt", C.c_int)] + [(k, C.c_double) for k in (
         "pot kin tot elec vdw eDir eRec eCorr virelec virvdw virshake nInCut "
-        "ebond eub eang edihe eimpr virbond virub").split()]
+        "ebond eub eang edihe eimpr virbond virub conint consv chiT").split()]
 
 
 # Every entry point include/mdgpu.h declares (checked by the ABI test).
@@ -104,7 +104,8 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_read_simu mdg_read_conf mdg_read_rest mdg_read_forf mdg_setup "
            "mdg_charge_const mdg_init_box mdg_get_kinfromtemp mdg_image_update mdg_kinetic "
            "mdg_spme_mesh mdg_init_vel mdg_write_conf mdg_write_prop mdg_write_rest "
-           "mdg_write_dcd_header mdg_write_dcd_traj mdg_nonbond_energy_noalloc").split()
+           "mdg_write_dcd_header mdg_write_dcd_traj mdg_nonbond_energy_noalloc mdg_set_bath "
+           "mdg_get_bath").split()
 
 _LIB = None
 
@@ -446,6 +447,26 @@ class Context:
         r = MDG_STEP()
         self._chk(self.L.mdg_initial_energy(self.h, C.byref(r)), "mdg_initial_energy")
         return r
+
+    def set_bath(self, ens: int, tauT: float = 0.1, chiT: float = 0.0, kinTemp0: float = 0.0,
+                 conint: float = 0.0, consv: float = 0.0):
+        """mdg_set_bath: the device step's thermostat (0 NVE, 1 NVT_B, 3
+        NVT_H)."""
+        ctrl = MDG_CTRL()
+        C.pointer(ctrl)[0] = self.ctrl
+        ctrl.ens = int(ens)
+        param = MDG_PARAM()
+        C.pointer(param)[0] = self.param
+        param.kinTemp0 = float(kinTemp0)
+        bath = MDG_BATH(tauT=float(tauT), chiT=float(chiT))
+        e = MDG_ENERGY(conint=float(conint), consv=float(consv))
+        self._chk(self.L.mdg_set_bath(self.h, C.byref(ctrl), C.byref(param), C.byref(bath),
+                                      C.byref(e)), "mdg_set_bath")
+
+    def get_bath(self):
+        bath, e = MDG_BATH(), MDG_ENERGY()
+        self._chk(self.L.mdg_get_bath(self.h, C.byref(bath), C.byref(e)), "mdg_get_bath")
+        return dict(chiT=bath.chiT, conint=e.conint, consv=e.consv)
 
     def run(self, nsteps: int, use_graph: bool = True):
         recs = (MDG_STEP * max(1, nsteps))()

@@ -652,6 +652,10 @@ extern "C" int mdg_backward_traj(mdg_ctx *ctx, int nsteps, int ckpt, double *J, 
 {
     if (!ctx || nsteps <= 0 || ckpt <= 0) return MDG_EARG;
     RETCHK(adj_ready(ctx));
+    if (ctx->ens != 0) {
+        ctx->err = "the trajectory adjoint differentiates NVE dynamics";
+        return MDG_EUNSUPPORTED;
+    }
     if (ctx->precision != MDG_FP64) {
         ctx->err = "the trajectory adjoint runs in FP64 contexts";
         return MDG_EUNSUPPORTED;

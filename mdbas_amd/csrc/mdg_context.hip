@@ -357,6 +357,7 @@ extern "C" mdg_ctx *mdg_create(const MDG_CTRL *ctrl, const MDG_PARAM *param, con
     params_from_mirror(ctrl, param, ewald, neigh, ctx->P);
     box_from_mirror(box, ctx->B);
     box_vectors(box, ctx->boxKey);
+    ctx->ensCreate = ctrl ? (int)ctrl->ens : 0;
     if (ctx->B.type < 1 || ctx->B.type > 3) {
         ctx->err = "a periodic box is required";
         return fail(ctx, MDG_EUNSUPPORTED);
@@ -541,7 +542,7 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binBucket, ctx->binOut,
                     ctx->binNOut, ctx->splI, ctx->splB, ctx->splD,
                     ctx->cl.pJ, ctx->cl.pMask, ctx->cl.pCnt, ctx->dispPart, ctx->ensSave,
-                    ctx->gridU, ctx->hvpBuf};
+                    ctx->gridU, ctx->hvpBuf, ctx->bathDev};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->bt) {

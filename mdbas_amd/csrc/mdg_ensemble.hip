@@ -492,3 +492,178 @@ int mdg_vv_ensemble(mdg_ctx *ctx, const MDG_PARAM *param, MDG_ENERGY *ener, MDG_
     if (ens == 2 || ens == 4) RETCHK(mdg_refresh_box(ctx, box));
     return MDG_OK;
 }
+
+// ---------------------------------------------------------------- device-resident thermostat
+
+namespace {
+
+// bathDev layout
+enum { B_CHIT = 0, B_CONINT = 1, B_LAMBDA = 2, B_KIN = 3, B_REC_CONINT = 4, B_REC_CONSV = 5,
+       B_REC_CHIT = 6, B_WORDS = 8 };
+
+struct ThermoArgs {
+    int ens, stage, rows;
+    const double *part;          // kinetic partials, [rows][7], sum m v^2 first
+    double *bath;
+    double dt, tauT, kinTemp0;
+};
+
+// One block: K = sum of the partials' m v^2 / 2 (fixed order), then the
+// bath's scalar update of vv_nvt_b (integrate.c:1721-1731) or vv_nvt_h
+// (integrate.c:2033-2047 at stage 1, 2110-2124 at stage 2), in the
+// reference's expression order; lambda goes to bath[B_LAMBDA].
+__global__ void __launch_bounds__(256) k_thermo(ThermoArgs T)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[4];
+    double v = 0.;
+    for (int r = threadIdx.x; r < T.rows; r += blockDim.x) v += T.part[(long)r * 7];
+    v = mdg_wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double *b = T.bath;
+    double kin = ((red[0] + red[1]) + red[2] + red[3]) * 0.5;
+    double lam;
+    if (T.ens == 1) {
+        lam = sqrt(1.0 + T.dt / T.tauT * (T.kinTemp0 / kin - 1.0));
+        kin *= lam * lam;
+        // mdbas.c:246 clears consv every step; vv_nvt_b leaves it and conint
+        b[B_REC_CONINT] = b[B_CONINT];
+        b[B_REC_CONSV] = 0.;
+        b[B_REC_CHIT] = b[B_CHIT];
+    } else {
+        const double qmass = 2.0 * T.kinTemp0 * (T.tauT * T.tauT);
+        double chiT = b[B_CHIT], conint = b[B_CONINT];
+        chiT += 0.5 * T.dt * (kin - T.kinTemp0) / qmass;
+        lam = exp(-0.5 * T.dt * chiT);
+        kin *= lam * lam;
+        conint += 0.5 * T.dt * chiT * qmass / (T.tauT * T.tauT);
+        chiT += 0.5 * T.dt * (kin - T.kinTemp0) / qmass;
+        b[B_CHIT] = chiT;
+        b[B_CONINT] = conint;
+        if (T.stage == 2) {
+            b[B_REC_CONINT] = conint;
+            b[B_REC_CONSV] = conint + 0.5 * qmass * (chiT * chiT);
+            b[B_REC_CHIT] = chiT;
+        }
+    }
+    b[B_LAMBDA] = lam;
+    b[B_KIN] = kin;
+}
+
+// v *= lambda (bath[B_LAMBDA]); with part, block partials of m v^2 and the
+// kinetic stress of the scaled velocities in [gridDim.x][7] (the rows the
+// step's record sums), grid-stride over the atoms.
+__global__ void __launch_bounds__(256)
+k_vscale_bath(int n, double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
+              const double *__restrict__ m, const double *__restrict__ bath, double *part)
+{
+#pragma clang fp contract(off)
+    const double lam = bath[B_LAMBDA];
+    double a[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double u = vx[i] * lam, v = vy[i] * lam, w = vz[i] * lam;
+        vx[i] = u;
+        vy[i] = v;
+        vz[i] = w;
+        if (part) {
+            const double mi = m[i];
+            a[0] += mi * ((u * u + v * v) + w * w);
+            a[1] += mi * (u * u);
+            a[2] += mi * u * v;
+            a[3] += mi * u * w;
+            a[4] += mi * (v * v);
+            a[5] += mi * v * w;
+            a[6] += mi * (w * w);
+        }
+    }
+    if (part) mdg_block_partials<7>(a, part);
+}
+
+}  // namespace
+
+int mdg_thermo_stage(mdg_ctx *ctx, int stage)
+{
+    if (ctx->ens != 1 && ctx->ens != 3) return MDG_OK;
+    if (stage == 1 && ctx->ens != 3) return MDG_OK;   // Berendsen acts at stage 2 only
+    const int n = ctx->n;
+    ThermoArgs T;
+    T.ens = ctx->ens;
+    T.stage = stage;
+    T.bath = ctx->bathDev;
+    T.dt = ctx->P.timeStep;
+    T.tauT = ctx->tauT;
+    T.kinTemp0 = ctx->kinTemp0;
+    if (stage == 1) {
+        // the kinetic energy of the step's incoming velocities
+        T.rows = nblk(n, 256);
+        T.part = ctx->epart + PART_ENS;
+        hipLaunchKernelGGL(k_kin_parts, dim3(T.rows), dim3(256), 0, ctx->stream, n, ctx->vx,
+                           ctx->vy, ctx->vz, ctx->mass, ctx->epart + PART_ENS);
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_thermo, dim3(1), dim3(256), 0, ctx->stream, T);
+        hipLaunchKernelGGL(k_vscale_bath, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n,
+                           ctx->vx, ctx->vy, ctx->vz, ctx->mass, ctx->bathDev, nullptr);
+        LAUNCHCHK();
+        return MDG_OK;
+    }
+    // after SHAKE-V: the stage's kinetic partials, then the scaled
+    // velocities' partials in their place for the record
+    T.rows = mdg_vv_blocks(ctx);
+    T.part = ctx->epart + PART_KIN;
+    hipLaunchKernelGGL(k_thermo, dim3(1), dim3(256), 0, ctx->stream, T);
+    hipLaunchKernelGGL(k_vscale_bath, dim3(T.rows), dim3(256), 0, ctx->stream, n, ctx->vx, ctx->vy,
+                       ctx->vz, ctx->mass, ctx->bathDev, ctx->epart + PART_KIN);
+    LAUNCHCHK();
+    return MDG_OK;
+}
+
+extern "C" int mdg_set_bath(mdg_ctx *ctx, const MDG_CTRL *ctrl, const MDG_PARAM *param,
+                            const MDG_BATH *bath, const MDG_ENERGY *ener)
+{
+    if (!ctx || !ctrl || !param) return MDG_EARG;
+    const int ens = (int)ctrl->ens;
+    if (ens == 2 || ens == 4) {
+        ctx->err = "the barostats change the box, which the device step holds fixed: run them "
+                   "through mdg_vv_integrate";
+        return MDG_EUNSUPPORTED;
+    }
+    if (ens < 0 || ens > 4) return MDG_EARG;
+    if (ens != 0 && (!bath || !(bath->tauT > 0.) || !(param->kinTemp0 > 0.))) return MDG_EARG;
+    if (ens != 0 && ctx->bt) {
+        ctx->err = "the device thermostat runs on single systems";
+        return MDG_EUNSUPPORTED;
+    }
+    if (!ctx->bathDev) HIPCHK(hipMalloc(&ctx->bathDev, B_WORDS * sizeof(double)));
+    double b[B_WORDS] = {0};
+    b[B_CHIT] = bath ? bath->chiT : 0.;
+    b[B_CONINT] = ener ? ener->conint : 0.;
+    b[B_LAMBDA] = 1.;
+    b[B_REC_CONINT] = b[B_CONINT];
+    b[B_REC_CONSV] = ener ? ener->consv : 0.;
+    b[B_REC_CHIT] = b[B_CHIT];
+    HIPCHK(hipMemcpyAsync(ctx->bathDev, b, sizeof b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->ens = ctx->ensCreate = ens;
+    ctx->tauT = bath ? bath->tauT : 0.;
+    ctx->kinTemp0 = param->kinTemp0;
+    mdg_invalidate_graphs(ctx);
+    return MDG_OK;
+}
+
+extern "C" int mdg_get_bath(mdg_ctx *ctx, MDG_BATH *bath, MDG_ENERGY *ener)
+{
+    if (!ctx) return MDG_EARG;
+    double b[B_WORDS] = {0};
+    if (ctx->ens && ctx->bathDev) {
+        HIPCHK(hipMemcpyAsync(b, ctx->bathDev, sizeof b, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if (bath) bath->chiT = b[B_REC_CHIT];
+    if (ener) {
+        ener->conint = b[B_REC_CONINT];
+        ener->consv = b[B_REC_CONSV];
+    }
+    return MDG_OK;
+}

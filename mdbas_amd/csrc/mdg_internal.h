@@ -299,6 +299,12 @@ struct mdg_ctx {
     // net-charge energy, influence function) was made for
     double boxKey[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double *ensSave = nullptr;   // positions and velocities [6][n] across a barostat trial cycle
+    // device-resident thermostat (mdg_set_bath): the ensemble the context was
+    // made for and the one set; bathDev = chiT, conint, lambda, kin, then the
+    // last finished step's conint, consv, chiT (read by its record)
+    int ensCreate = 0, ens = 0;
+    double tauT = 0., kinTemp0 = 0.;
+    double *bathDev = nullptr;
 
     // timing
     bool profiling = false;
@@ -356,7 +362,7 @@ enum {
 enum {
     PART_PAIR = 0, PART_EXCL = PART_R, PART_INFL = 2 * PART_R, PART_GATHER = 3 * PART_R,
     PART_KIN = 4 * PART_R, PART_MISC = 5 * PART_R, PART_REF = 6 * PART_R,
-    PART_BOND = 7 * PART_R, PART_TOTAL = 8 * PART_R
+    PART_BOND = 7 * PART_R, PART_ENS = 8 * PART_R, PART_TOTAL = 9 * PART_R
 };
 
 // ---------------------------------------------------------------- helpers
@@ -584,6 +590,9 @@ struct VvOpts {
 int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, double *kin_out,
                  double *virshake_out, bool sync, const FinArgs *fin = nullptr,
                  const VvOpts *opt = nullptr);
+// the device-resident thermostat of one step, before stage 1 (stage 1) or
+// after stage 2 (stage 2) (mdg_ensemble.hip)
+int mdg_thermo_stage(mdg_ctx *ctx, int stage);
 // thermostats and barostats behind the drop-in vv_integrate (mdg_ensemble.hip)
 int mdg_vv_ensemble(mdg_ctx *ctx, const MDG_PARAM *param, MDG_ENERGY *ener, MDG_PBC *box,
                     MDG_BATH *bath, int ens, int stage);

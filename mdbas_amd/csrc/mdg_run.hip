@@ -240,6 +240,7 @@ RecArgs mdg_rec_args(mdg_ctx *ctx, int nbKin, int nGrpVir, int nbExcl)
     R.grpVir = ctx->grpVir;
     R.nbBond = ctx->bd.on ? mdg_bonded_blocks(ctx) : 0;
     R.pBond = ctx->epart + PART_BOND;
+    R.bath = ctx->ens ? ctx->bathDev : nullptr;
     return R;
 }
 
@@ -253,6 +254,7 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     ctx->slotsInStage1 = !rebuild && ctx->precision == MDG_MIXED && !ctx->bt && ctx->shakeW3 &&
                          ctx->sx && ctx->dispPart && ctx->cl.atomSlot;
     ctx->slotsDone = false;
+    if (ctx->ens) RETCHK(mdg_thermo_stage(ctx, 1));
     const int st1 = mdg_vv_stage(ctx, 1, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false);
     ctx->slotsInStage1 = false;
     RETCHK(st1);
@@ -266,6 +268,7 @@ int mdg_device_step(mdg_ctx *ctx, bool rebuild)
     } else {
         RETCHK(mdg_vv_stage(ctx, 2, ctx->fx, ctx->fy, ctx->fz, nullptr, nullptr, false));
     }
+    if (ctx->ens) RETCHK(mdg_thermo_stage(ctx, 2));
     // In a step block of one box with three-site SHAKE, the record and the
     // list prune go with the next step's first SHAKE kernel (its spare
     // blocks); the block's last step runs them after the loop.
@@ -539,6 +542,11 @@ static void accumulate_phase(mdg_ctx *ctx, const hipEvent_t *e, bool rebuild, in
 extern "C" int mdg_run_steps(mdg_ctx *ctx, int nsteps, int use_graph, MDG_STEP *records)
 {
     if (!ctx || nsteps < 0) return MDG_EARG;
+    if (ctx->ensCreate != ctx->ens) {
+        ctx->err = "the context was made for a thermostat: call mdg_set_bath first (NVT), or run "
+                   "the barostats through mdg_vv_integrate";
+        return MDG_EUNSUPPORTED;
+    }
     if (!ctx->cl.cpJ) {
         ctx->err = "call mdg_initial_energy first";
         return MDG_EARG;

@@ -160,6 +160,7 @@ struct RecArgs {
     // per-block partial counts of this step's producers
     int nbPair, nbExcl, nbInfl, nbKin, nGrpVir, nbBond;
     const double *pPair, *pExcl, *pInfl, *pKin, *grpVir, *pBond;
+    const double *bath;           // device thermostat: conint, consv, chiT at [4..6] (or null)
 };
 
 // One MDG_STEP record by one block of SIDE_BLOCK threads: reduces the
@@ -227,6 +228,9 @@ __device__ inline void record_block(const RecArgs &R, int *ctrl, MDG_STEP *rec, 
     s.virshake = v[15];
     s.virbond = v[21];
     s.virub = v[22];
+    s.conint = R.bath ? R.bath[4] : 0.;
+    s.consv = R.bath ? R.bath[5] : 0.;
+    s.chiT = R.bath ? R.bath[6] : 0.;
     s.nInCut = v[4];
     if (idx < R.recCap) rec[idx] = s;
     ctrl[CTRL_STEP] = idx + 1;
@@ -291,6 +295,7 @@ __device__ inline void b_record_block(const BRecArgs &R, int r, int rebuilt)
     s.pot = s.elec + s.vdw;
     s.ebond = s.eub = s.eang = s.edihe = s.eimpr = 0.;
     s.virbond = s.virub = 0.;
+    s.conint = s.consv = s.chiT = 0.;
     s.kin = 0.5 * v[7];
     s.tot = s.pot + s.kin;
     s.virelec = v[2] + (-(st0 + st3 + st5)) + v[6];

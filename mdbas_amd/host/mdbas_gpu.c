@@ -144,6 +144,7 @@ typedef struct {
 typedef struct Job {
     int step, traj, rest;
     MDG_ENERGY ener;
+    MDG_BATH bath;
     double *s;           /* 9 N doubles: x y z vx vy vz fx fy fz */
     struct Job *next;
 } Job;
@@ -178,7 +179,7 @@ static void *writer_main(void *arg)
         if (j->traj) st = st ? st : mdg_write_dcd_traj(&R->io, &p, &R->box, s, s + n, s + 2 * n, R->frozen);
         if (j->rest) {
             st = st ? st : mdg_write_conf(&R->io, &p, R->atom, s, s + n, s + 2 * n);
-            st = st ? st : mdg_write_rest(&R->io, &p, &j->ener, &R->bath, R->atom, s, s + n,
+            st = st ? st : mdg_write_rest(&R->io, &p, &j->ener, &j->bath, R->atom, s, s + n,
                                           s + 2 * n, s + 3 * n, s + 4 * n, s + 5 * n, s + 6 * n,
                                           s + 7 * n, s + 8 * n);
         }
@@ -196,6 +197,7 @@ static void enqueue(Run *R, int step, int traj, int rest)
     j->traj = traj;
     j->rest = rest;
     j->ener = R->ener;
+    j->bath = R->bath;
     j->s = (double *)malloc(9 * n * sizeof(double));
     double *s = j->s;
     CHECK(mdg_download_state(R->ctx, s, s + n, s + 2 * n, s + 3 * n, s + 4 * n, s + 5 * n,
@@ -343,6 +345,11 @@ static void run_device(Run *R, int graph)
             p->step++;
             energy_of(&R->ener, &rec[s]);
             R->ener.consv = 0.;
+            if (R->ctrl.ens) {   /* the device thermostat's state after this step */
+                R->ener.conint = rec[s].conint;
+                R->ener.consv = rec[s].consv;
+                R->bath.chiT = rec[s].chiT;
+            }
             if (R->ctrl.keyProp && due(p->step, R->ctrl.printProp))
                 CHECK(mdg_write_prop(&R->io, p, &R->ener, &R->box), NULL);
             if (due(p->step, R->ctrl.printOut)) print_energies(p, &R->ener, &R->box, &R->nPrint);
@@ -431,9 +438,9 @@ int main(int argc, char **argv)
         fail(MDG_EUNSUPPORTED, "classic Ewald sum (elec ewald): use elec spme", NULL);
     }
     if (R.ctrl.integrator != 1) fail(MDG_EUNSUPPORTED, "leap-frog integrator", NULL);
-    if (device && R.ctrl.ens != 0)
-        fail(MDG_EUNSUPPORTED, "device-resident steps are NVE: run thermostats with -m dropin",
-             NULL);
+    if (device && R.ctrl.ens != 0 && R.ctrl.ens != 1 && R.ctrl.ens != 3)
+        fail(MDG_EUNSUPPORTED, "device-resident steps run NVE, NVT_B or NVT_H: run the barostats "
+             "with -m dropin", NULL);
 
     int st = 0;
     R.ctx = mdg_create(&R.ctrl, p, &R.box, &R.ewald, &R.neigh, device ? precision : MDG_FP64, &st);
@@ -472,6 +479,7 @@ int main(int argc, char **argv)
     fprintf(out, "GPU path: %s, %s\n\n", device ? "device-resident steps" : "drop-in calls",
             device && precision == MDG_MIXED ? "mixed precision" : "FP64");
 
+    if (device && R.ctrl.ens) CHECK(mdg_set_bath(R.ctx, &R.ctrl, p, &R.bath, &R.ener), R.ctx);
     if (device) run_device(&R, graph);
     else run_dropin(&R);
 

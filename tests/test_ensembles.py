@@ -194,3 +194,64 @@ def test_dropin_ensemble_against_oracle(c0, ens, constrained):
         assert np.allclose(ta["box9"], tb["box9"], rtol=1e-13, atol=0)
         assert np.allclose(ta["stress"], tb["stress"], rtol=1e-10, atol=1e-10 * abs(ta["kin"]))
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("ens", [1, 3], ids=lambda e: ENSEMBLES[e])
+def test_device_thermostat_against_oracle(c0, ens, prec):
+    """mdg_set_bath + mdg_run_steps with NVT_B / NVT_H: 25 device-resident
+    steps (crossing the list rebuild at step 20) against the oracle's MD
+    loop with oc_vv_ens from the same bath (chiT 0.3, target 10% above the
+    box's kinetic energy): total and potential energy, kinetic energy,
+    consv and the final positions.  FP64 to 1e-10 of |E_tot(0)| and 1e-9 A;
+    mixed to 1e-5 and 1e-4 A."""
+    import mdbas_amd as M
+    wb = c0[0]
+    o = O.OracleSystem(wb)
+    s = [wb.x.copy(), wb.y.copy(), wb.z.copy(), wb.vx.copy(), wb.vy.copy(), wb.vz.copy()]
+    st = start_state(wb, ens)
+    kT0, chiT0 = st["kinTemp0"], st["chiT"]
+    err, f, rec = o.md_run_ens(st, 25, *s)
+    assert err == 0
+    g = M.Context(wb, precision=M.MDG_FP64 if prec == "fp64" else M.MDG_MIXED)
+    g.set_bath(ens, tauT=0.1, chiT=chiT0, kinTemp0=kT0)
+    g.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    g.initial_energy()
+    R = M.records_to_dict(g.run(25, use_graph=True))
+    x = g.download()
+    bath = g.get_bath()
+    g.close()
+    scale = abs(rec[0, 0])
+    bar, pos = (1e-10, 1e-9) if prec == "fp64" else (1e-5, 1e-4)
+    for k, col in (("tot", 0), ("pot", 1), ("kin", 2), ("consv", 3)):
+        e = np.abs(R[k] - rec[1:, col]).max() / scale
+        print(ENSEMBLES[ens], prec, k, e, flush=True)
+        assert e < bar, k
+    if ens == 3:
+        assert np.abs(R["consv"]).min() > 0.
+        assert bath["chiT"] == pytest.approx(st["chiT"], rel=1e-6 if prec == "mixed" else 1e-10)
+    dx = max(np.abs(x[i] - s[i]).max() for i in range(3))
+    print("pos", dx, flush=True)
+    assert dx < pos
+
+
+@pytest.mark.gpu
+def test_device_thermostat_graph_modes_agree(c0):
+    """The same NVT_H run replayed by step graphs, by one graph for the
+    whole run and by plain launches gives the same records bit for bit
+    (mixed: fixed-point forces, fixed-order sums)."""
+    import mdbas_amd as M
+    wb = c0[0]
+    out = []
+    for mode in (0, 1, 2):
+        g = M.Context(wb, precision=M.MDG_MIXED)
+        st = start_state(wb, 3)
+        g.set_bath(3, tauT=0.1, chiT=st["chiT"], kinTemp0=st["kinTemp0"])
+        g.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+        g.initial_energy()
+        out.append(M.records_to_dict(g.run(22, use_graph=mode)))
+        g.close()
+    for k in ("tot", "kin", "consv", "chiT"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+        assert np.array_equal(out[0][k], out[2][k]), k

@@ -129,16 +129,19 @@ def test_driver_plain_cutoff_deck_against_oracle(mode, c0_oracle, tmp_path):
     check(prop, rest, s, et, ep, 1e-10, 1e-9)
 
 
-ENSEMBLE_DECKS = [("nvt beren", 1), ("npt beren", 2), ("nvt hoover", 3), ("npt hoover", 4)]
+ENSEMBLE_DECKS = [("nvt beren", 1, "dropin"), ("npt beren", 2, "dropin"),
+                  ("nvt hoover", 3, "dropin"), ("npt hoover", 4, "dropin"),
+                  ("nvt beren", 1, "device"), ("nvt hoover", 3, "device")]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ensemble,ens", ENSEMBLE_DECKS, ids=[e[0].replace(" ", "_")
-                                                              for e in ENSEMBLE_DECKS])
-def test_driver_ensemble_deck_against_oracle(ensemble, ens, c0_oracle, tmp_path):
+@pytest.mark.parametrize("ensemble,ens,mode", ENSEMBLE_DECKS,
+                         ids=[e[0].replace(" ", "_") + "_" + e[2] for e in ENSEMBLE_DECKS])
+def test_driver_ensemble_deck_against_oracle(ensemble, ens, mode, c0_oracle, tmp_path):
     """The water deck with a thermostat or barostat ("ensemble nvt beren",
     "npt hoover", ..., taut 0.1 ps, taup 0.5 ps, pressure 1 bar), 15 FP64
-    drop-in steps (list rebuilt every 20 steps, so once at the start)
+    steps through the drop-in calls, and for the thermostats also
+    device-resident (list rebuilt every 20 steps, so once at the start)
     against the oracle's MD loop with oc_vv_ens: PROP total and potential
     energy, consv and volume to 1e-10 relative, final positions to 1e-8 A
     (the device sums the kinetic energy in another order, and the bath
@@ -156,7 +159,7 @@ def test_driver_ensemble_deck_against_oracle(ensemble, ens, c0_oracle, tmp_path)
     assert err == 0
     d = str(tmp_path)
     decks.write_deck(d, wb, cfg, 15, ensemble=ensemble, taut=0.1, taup=0.5, pressure=1.0)
-    prop, rest = run(d, "-m", "dropin", "-p", "fp64")
+    prop, rest = run(d, "-m", mode, "-p", "fp64")
     assert prop.shape[0] == 16
     scale = abs(rec[0, 0])
     for col, k, name in ((5, 0, "tot"), (7, 1, "pot"), (22, 3, "consv")):
