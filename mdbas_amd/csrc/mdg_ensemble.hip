@@ -509,8 +509,8 @@ struct ThermoArgs {
 };
 
 // One block: K = sum of the partials' m v^2 / 2 (fixed order), then the
-// bath's scalar update of vv_nvt_b (integrate.c:1721-1731) or vv_nvt_h
-// (integrate.c:2033-2047 at stage 1, 2110-2124 at stage 2), in the
+// bath's scalar update of vv_nvt_b (integrate.c:1723-1737) or vv_nvt_h
+// (integrate.c:2033-2062 at stage 1, 2116-2134 at stage 2), in the
 // reference's expression order; lambda goes to bath[B_LAMBDA].
 __global__ void __launch_bounds__(256) k_thermo(ThermoArgs T)
 {

@@ -385,6 +385,20 @@ class OracleSystem:
                               P(wb.nAtConst), C.byref(ek), C.byref(vs), C.byref(nc))
         return err, ek.value, vs.value, nc.value
 
+    def vv_conv(self, stage, x, y, z, vx, vy, vz, f, cap=None):
+        """vv with each constraint's convergence test at every SHAKE sweep:
+        returns (err, conv) with conv a bool array [sweeps run, nConst]."""
+        wb = self.wb
+        cap = self.cfg["maxCycle"] if cap is None else cap
+        conv = np.zeros((cap, len(wb.ca)), np.uint8)
+        nc = C.c_int(0)
+        err = lib().oc_vv_nve_conv(C.byref(self.p), C.byref(self.b), int(stage), len(wb.ca),
+                                   P(wb.ca), P(wb.cb), P(wb.rc2), P(x), P(y), P(z), P(vx), P(vy),
+                                   P(vz), P(f[0]), P(f[1]), P(f[2]), P(wb.mass), P(wb.rmass),
+                                   P(wb.nAtConst), conv.ctypes.data_as(C.c_void_p), int(cap),
+                                   C.byref(nc))
+        return err, conv[:min(nc.value, cap)].astype(bool)
+
     def vv_ens(self, st, stage, x, y, z, vx, vy, vz, f):
         """oc_vv_ens: one stage of a thermostat/barostat integrator; st
         (ens_state) is updated in place, box included."""

@@ -1031,6 +1031,11 @@ static double oc_dist2(const oc_box *b, double *dx, double *dy, double *dz)
 }
 
 /* shake.c:209-357 */
+/* Optional record of each constraint's convergence test at each sweep
+ * (conv[sweep * nConst + c]), for the tests of the stopping rule. */
+static unsigned char *g_conv;
+static int g_conv_cap;
+
 static int oc_shake_r(const oc_param *p, const oc_box *b, int nc, const int *ca,
                       const int *cb, const double *rc2, double *x, double *y,
                       double *z, double *vx, double *vy, double *vz,
@@ -1056,6 +1061,7 @@ static int oc_shake_r(const oc_param *p, const oc_box *b, int nc, const int *ca,
             rt2[i] = oc_dist2(b, &dtx[i], &dty[i], &dtz[i]);
             double dd = fabs(rt2[i] - rc2[i]) / sqrt(rc2[i]);
             maxdist = maxdist >= dd ? maxdist : dd;
+            if (g_conv && icycle < g_conv_cap) g_conv[(long)icycle * nc + i] = 0.5 * dd < p->tolShake;
         }
         maxdist = 0.5 * maxdist;
         if (maxdist < p->tolShake) conv = 1;
@@ -1112,6 +1118,7 @@ static int oc_shake_v(const oc_param *p, int nc, const int *ca, const int *cb,
                             ((trmi + trmj) * (SQ(ddx[i]) + SQ(ddy[i]) + SQ(ddz[i])));
             double al = fabs(lambda);
             maxdist = maxdist >= al ? maxdist : al;
+            if (g_conv && icycle < g_conv_cap) g_conv[(long)icycle * nc + i] = al < tolvel;
             double li = lambda * trmi;
             xt[ia] += ddx[i] * li; yt[ia] += ddy[i] * li; zt[ia] += ddz[i] * li;
             double lj = -lambda * trmj;
@@ -2571,5 +2578,24 @@ done:
     free(S); free(exOff); free(sxOff); free(exLst); free(sxLst);
     free(L.nl); free(L.np); free(L.no);
     free(xb); free(vb); free(vb2); free(fp); free(fbar); free(gr); free(tq); free(te); free(ts);
+    return err;
+}
+
+/* oc_vv_nve with each constraint's convergence test of every SHAKE sweep
+ * recorded in conv (cap sweeps of nConst bytes); *ncycle = sweeps run. */
+int oc_vv_nve_conv(const oc_param *p, const oc_box *b, int stage,
+                   int nConst, const int *ca, const int *cb, const double *rc2,
+                   double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                   const double *fx, const double *fy, const double *fz,
+                   const double *mass, const double *rmass, const int *nAtConst,
+                   unsigned char *conv, int cap, int *ncycle)
+{
+    g_conv = conv;
+    g_conv_cap = cap;
+    double ek, vs;
+    int err = oc_vv_nve(p, b, stage, nConst, ca, cb, rc2, x, y, z, vx, vy, vz, fx, fy, fz, mass,
+                        rmass, nAtConst, &ek, &vs, ncycle);
+    g_conv = NULL;
+    g_conv_cap = 0;
     return err;
 }

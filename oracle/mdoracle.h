@@ -197,6 +197,15 @@ int oc_vv_nve(const oc_param *p, const oc_box *b, int stage,
               const double *mass, const double *rmass, const int *nAtConst,
               double *ekin, double *virshake, int *ncycle);
 
+/* oc_vv_nve, recording each constraint's convergence test at every SHAKE
+ * sweep: conv[sweep * nConst + c] (cap sweeps); *ncycle = sweeps run. */
+int oc_vv_nve_conv(const oc_param *p, const oc_box *b, int stage,
+                   int nConst, const int *ca, const int *cb, const double *rc2,
+                   double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                   const double *fx, const double *fy, const double *fz,
+                   const double *mass, const double *rmass, const int *nAtConst,
+                   unsigned char *conv, int cap, int *ncycle);
+
 /* Thermostat / barostat state of one VV call: BATH (global.h:240-245),
  * the PARAM targets, and the ENERGY members the integrators read (virpot)
  * and write (kin, virshake, conint, consv); stress[9] accumulates into the
