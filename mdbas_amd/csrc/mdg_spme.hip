@@ -473,9 +473,10 @@ k_spme_bin(MdgParams Pm, MdgBox B, int n, int nby, int nbz, const double *__rest
     double s[3];
     scaled_coords(B, Pm.m1, Pm.m2, Pm.m3, x[l], y[l], z[l], s[0], s[1], s[2]);
     const int i1 = (int)s[0], i2 = (int)s[1], i3 = (int)s[2];
-    splI[l] = make_int4(i1, i2, i3, 0);
+    // the FP64 spread evaluates its splines in FP64 itself: bins only
+    if (splB) splI[l] = make_int4(i1, i2, i3, 0);
 #pragma unroll
-    for (int dim = 0; dim < 3; dim++) {
+    for (int dim = 0; dim < 3 && splB; dim++) {
         float b[P], d[P], bq[4 * NQ], dq[4 * NQ];
         bspline_fix<float, P>(s[dim], b, d);
 #pragma unroll
@@ -520,13 +521,21 @@ __device__ inline int spme_tile_bins(int X0, int m, int *bins)
     return nb;
 }
 
-template <int P>
+// F64: the FP64 mesh of the FP64 path.  Each bucketed atom's splines are
+// evaluated in FP64 from its position (as k_spme_spread_rows<false> does,
+// the same products in the same order) and added into the tile with LDS
+// FP64 atomics; every mesh point is written once, so the mesh needs no
+// clearing.  The order of the additions varies from run to run, as it does
+// with the global FP64 atomics this replaces.
+template <int P, bool F64>
 __global__ void __launch_bounds__(256)
-k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
-                    const double *__restrict__ q, const int *__restrict__ binCnt,
-                    const int *__restrict__ bucket, const int *__restrict__ binOut,
-                    const int *__restrict__ binNOut, const int4 *__restrict__ splI,
-                    const float4 *__restrict__ splB, float *__restrict__ grid32)
+k_spme_spread_tiles(MdgParams Pm, MdgBox B, int nty, int ntz, int nby, int nbz,
+                    const double *__restrict__ x, const double *__restrict__ y,
+                    const double *__restrict__ z, const double *__restrict__ q,
+                    const int *__restrict__ binCnt, const int *__restrict__ bucket,
+                    const int *__restrict__ binOut, const int *__restrict__ binNOut,
+                    const int4 *__restrict__ splI, const float4 *__restrict__ splB,
+                    float *__restrict__ grid32, double *__restrict__ grid64)
 {
     if (Pm.spmePrio == 1) __builtin_amdgcn_s_setprio(2);   // ahead of the pair kernel beside it
     constexpr int T = SPME_TILE, NBD = (SPME_TILE + P - 2) / SPME_BIN + 2;
@@ -589,25 +598,57 @@ k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
             } else {
                 l = binOut[c - ncand];
             }
-            SplineAtom<P> S;
-            spline_load<P>(splI, splB, nullptr, Pm.n, l, false, S);
-            const float ql = (float)q[l];
+            if (F64) {
+#pragma clang fp contract(off)
+                double s1, s2, s3;
+                scaled_coords(B, m1, m2, m3, x[l], y[l], z[l], s1, s2, s3);
+                double b1[P], b2[P], b3[P], d[P];
+                bspline_fix<double, P>(s1, b1, d);
+                bspline_fix<double, P>(s2, b2, d);
+                bspline_fix<double, P>(s3, b3, d);
+                const int i1 = (int)s1, i2 = (int)s2, i3 = (int)s3;
+                const double ql = q[l];
+                double *acc64 = reinterpret_cast<double *>(acc);
 #pragma unroll
-            for (int i = 0; i < P; i++) {
-                const int lx = wrap_index(S.base[0] - i, m1) - X0;
-                if (lx < 0 || lx >= T) continue;
-                const float qa = ql * S.b[0][i];
+                for (int i = 0; i < P; i++) {
+                    const int lx = wrap_index(i1 - i, m1) - X0;
+                    if (lx < 0 || lx >= T) continue;
+                    const double qa = ql * b1[i];
 #pragma unroll
-                for (int j = 0; j < P; j++) {
-                    const int ly = wrap_index(S.base[1] - j, m2) - Y0;
-                    if (ly < 0 || ly >= T) continue;
-                    const float qb = qa * S.b[1][j];
+                    for (int j = 0; j < P; j++) {
+                        const int ly = wrap_index(i2 - j, m2) - Y0;
+                        if (ly < 0 || ly >= T) continue;
+                        const double qb = qa * b2[j];
 #pragma unroll
-                    for (int k = 0; k < P; k++) {
-                        const int lz = wrap_index(S.base[2] - k, m3) - Z0;
-                        if (lz < 0 || lz >= T) continue;
-                        const long long v = __double2ll_rn((double)(qb * S.b[2][k]) * MDG_QSCALE);
-                        atomicAdd(&acc[(lx * T + ly) * T + lz], (unsigned long long)v);
+                        for (int k = 0; k < P; k++) {
+                            const int lz = wrap_index(i3 - k, m3) - Z0;
+                            if (lz < 0 || lz >= T) continue;
+                            atomicAdd(&acc64[(lx * T + ly) * T + lz], qb * b3[k]);
+                        }
+                    }
+                }
+            } else {
+                SplineAtom<P> S;
+                spline_load<P>(splI, splB, nullptr, Pm.n, l, false, S);
+                const float ql = (float)q[l];
+#pragma unroll
+                for (int i = 0; i < P; i++) {
+                    const int lx = wrap_index(S.base[0] - i, m1) - X0;
+                    if (lx < 0 || lx >= T) continue;
+                    const float qa = ql * S.b[0][i];
+#pragma unroll
+                    for (int j = 0; j < P; j++) {
+                        const int ly = wrap_index(S.base[1] - j, m2) - Y0;
+                        if (ly < 0 || ly >= T) continue;
+                        const float qb = qa * S.b[1][j];
+#pragma unroll
+                        for (int k = 0; k < P; k++) {
+                            const int lz = wrap_index(S.base[2] - k, m3) - Z0;
+                            if (lz < 0 || lz >= T) continue;
+                            const long long v =
+                                __double2ll_rn((double)(qb * S.b[2][k]) * MDG_QSCALE);
+                            atomicAdd(&acc[(lx * T + ly) * T + lz], (unsigned long long)v);
+                        }
                     }
                 }
             }
@@ -616,9 +657,11 @@ k_spme_spread_tiles(MdgParams Pm, int nty, int ntz, int nby, int nbz,
     __syncthreads();
     for (int i = threadIdx.x; i < T * T * T; i += blockDim.x) {
         const int gx = X0 + i / (T * T), gy = Y0 + (i / T) % T, gz = Z0 + i % T;
-        if (gx < m1 && gy < m2 && gz < m3)
-            grid32[gz + (long)m3 * (gy + (long)m2 * gx)] =
-                (float)((double)(long long)acc[i] * (1.0 / MDG_QSCALE));
+        if (gx < m1 && gy < m2 && gz < m3) {
+            const long o = gz + (long)m3 * (gy + (long)m2 * gx);
+            if (F64) grid64[o] = reinterpret_cast<const double *>(acc)[i];
+            else grid32[o] = (float)((double)(long long)acc[i] * (1.0 / MDG_QSCALE));
+        }
     }
 }
 
@@ -643,18 +686,32 @@ static int spme_bin_alloc(mdg_ctx *ctx)
     HIPCHK(hipMalloc(&ctx->binNOut, 2 * sizeof(int)));
     RETCHK(mdg_fill_i32(ctx, ctx->binCnt, nb, 0));
     RETCHK(mdg_fill_i32(ctx, ctx->binNOut, 2, 0));
-    const int nq = (ctx->P.nbsp + 3) / 4;
-    HIPCHK(hipMalloc(&ctx->splI, n * sizeof(int4)));
-    HIPCHK(hipMalloc(&ctx->splB, 3 * nq * (size_t)n * sizeof(float4)));
-    HIPCHK(hipMalloc(&ctx->splD, 3 * nq * (size_t)n * sizeof(float4)));
+    if (ctx->precision == MDG_MIXED) {
+        const int nq = (ctx->P.nbsp + 3) / 4;
+        HIPCHK(hipMalloc(&ctx->splI, n * sizeof(int4)));
+        HIPCHK(hipMalloc(&ctx->splB, 3 * nq * (size_t)n * sizeof(float4)));
+        HIPCHK(hipMalloc(&ctx->splD, 3 * nq * (size_t)n * sizeof(float4)));
+    }
     RETCHK(mdg_fill_i32(ctx, ctx->binCnt, nb, 0));
     RETCHK(mdg_fill_i32(ctx, ctx->binNOut, 2, 0));
     return MDG_OK;
 }
 
-// Binned mixed spreading into grid32 (replaces the atomic spread and the
-// fixed-point conversion); leaves the per-atom splines for the gather.
-template <int P>
+// Clears the bucket counts and the outlier count for the next step.
+__global__ void k_bin_clear(long nbins, int *__restrict__ binCnt, int *__restrict__ binNOut)
+{
+    for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < nbins;
+         b += (long)gridDim.x * blockDim.x)
+        binCnt[b] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) binNOut[0] = 0;
+}
+
+// Binned spreading.  Mixed: into grid32 (replaces the atomic spread and the
+// fixed-point conversion), leaving the per-atom splines for the gather.
+// FP64: into grid64 (replaces the mesh clear and the global FP64 atomics);
+// the bucket counts are cleared after the tiles have read them (the FP64
+// gather evaluates its own splines and does not touch the bins).
+template <int P, bool F64>
 static int spread_binned_p(mdg_ctx *ctx)
 {
     const int n = ctx->n;
@@ -662,21 +719,30 @@ static int spread_binned_p(mdg_ctx *ctx)
               ntz = (ctx->m3 + SPME_TILE - 1) / SPME_TILE;
     hipLaunchKernelGGL(k_spme_bin<P>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, ctx->P, ctx->B,
                        n, ctx->nby, ctx->nbz, ctx->x, ctx->y, ctx->z, ctx->binCnt, ctx->binBucket,
-                       ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB, ctx->splD);
+                       ctx->binOut, ctx->binNOut, F64 ? nullptr : ctx->splI,
+                       F64 ? nullptr : ctx->splB, F64 ? nullptr : ctx->splD);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_spme_spread_tiles<P>, dim3(ntx * nty * ntz), dim3(256), 0, ctx->stream,
-                       ctx->P, nty, ntz, ctx->nby, ctx->nbz, ctx->q, ctx->binCnt, ctx->binBucket,
-                       ctx->binOut, ctx->binNOut, ctx->splI, ctx->splB, ctx->grid32);
+    hipLaunchKernelGGL((k_spme_spread_tiles<P, F64>), dim3(ntx * nty * ntz), dim3(256), 0,
+                       ctx->stream, ctx->P, ctx->B, nty, ntz, ctx->nby, ctx->nbz, ctx->x, ctx->y,
+                       ctx->z, ctx->q, ctx->binCnt, ctx->binBucket, ctx->binOut, ctx->binNOut,
+                       ctx->splI, ctx->splB, ctx->grid32, ctx->grid64);
     LAUNCHCHK();
+    if (F64) {
+        const long nb = (long)ctx->nbx * ctx->nby * ctx->nbz;
+        hipLaunchKernelGGL(k_bin_clear, dim3((int)std::min<long>(nblk(nb, 256), 1024)), dim3(256),
+                           0, ctx->stream, nb, ctx->binCnt, ctx->binNOut);
+        LAUNCHCHK();
+    }
     return MDG_OK;
 }
 
+template <bool F64>
 static int spread_binned(mdg_ctx *ctx)
 {
     switch (ctx->P.nbsp) {
-    case 4: return spread_binned_p<4>(ctx);
-    case 6: return spread_binned_p<6>(ctx);
-    default: return spread_binned_p<8>(ctx);
+    case 4: return spread_binned_p<4, F64>(ctx);
+    case 6: return spread_binned_p<6, F64>(ctx);
+    default: return spread_binned_p<8, F64>(ctx);
     }
 }
 
@@ -1028,8 +1094,8 @@ int mdg_spme_setup(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&ctx->gridq, G * sizeof(long long)));
         RETCHK(mdg_fill_i64(ctx, ctx->gridq, G));
         RETCHK(make_plans(ctx, false, &ctx->fwd32, &ctx->bwd32));
-        RETCHK(spme_bin_alloc(ctx));
     }
+    RETCHK(spme_bin_alloc(ctx));
     return MDG_OK;
 }
 
@@ -1054,7 +1120,7 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
     const double fact1 = -2.0 * (TWOPI / ctx->B.vol) * ctx->P.chargeConst;
     if (mixed) {
         if (spme_binned(ctx)) {
-            RETCHK(spread_binned(ctx));
+            RETCHK(spread_binned<false>(ctx));
         } else {
             launch_spread<true>(ctx, nullptr, ctx->gridq);
             LAUNCHCHK();
@@ -1082,9 +1148,13 @@ static int spme_run(mdg_ctx *ctx, bool mixed, double *partInfl, double *partGath
                                 (float *)fry, (float *)frz, partGather);
         LAUNCHCHK();
     } else {
-        RETCHK(mdg_fill_f64(ctx, ctx->grid64, G, 0.0));
-        launch_spread<false>(ctx, ctx->grid64, nullptr);
-        LAUNCHCHK();
+        if (spme_binned(ctx)) {
+            RETCHK(spread_binned<true>(ctx));
+        } else {
+            RETCHK(mdg_fill_f64(ctx, ctx->grid64, G, 0.0));
+            launch_spread<false>(ctx, ctx->grid64, nullptr);
+            LAUNCHCHK();
+        }
         void *in[1] = {ctx->grid64}, *out[1] = {ctx->cgrid64};
         if (rocfft_execute(ctx->fwd, in, out, ctx->info) != rocfft_status_success) {
             ctx->err = "rocfft_execute (forward) failed";
@@ -1750,14 +1820,6 @@ k_b_gather_pre(MdgParams Pm, const MdgBox *__restrict__ boxes, const int *__rest
     }
 }
 
-// Clears the bucket counts and the outlier count for the next step.
-__global__ void k_bin_clear(long nbins, int *__restrict__ binCnt, int *__restrict__ binNOut)
-{
-    for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < nbins;
-         b += (long)gridDim.x * blockDim.x)
-        binCnt[b] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) binNOut[0] = 0;
-}
 
 // Whether every replica's mesh takes the binned path.
 static bool b_spme_binned(const mdg_ctx *ctx)
