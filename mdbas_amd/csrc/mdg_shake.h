@@ -13,20 +13,6 @@
 #define SHK_SLOTS (SHK_W + 2 + SHK_R)
 #define SHK_MASKW 4               // 256 sweep indices
 
-// a / b, correctly rounded, from rb = 1 / b correctly rounded (Markstein,
-// IBM J. Res. Dev. 34 (1990) 111-119): q = a rb is within an ulp of a / b,
-// r = a - b q is exact (one fma), and q + r rb rounds to a / b.  The
-// operands here are the velocity stage's multiplier numerators and
-// denominators, far from the underflow range; a zero a returns a * rb,
-// which carries the quotient's sign of zero.  Three dependent operations
-// in place of a division's eleven.
-__device__ inline double shake_div_rcp(double a, double b, double rb)
-{
-    const double q = a * rb;
-    const double r = __builtin_fma(-q, b, a);
-    return a == 0. ? q : __builtin_fma(r, rb, q);
-}
-
 struct ShakeArgs {
     MdgParams P;
     MdgBox B;

@@ -34,7 +34,6 @@ struct M3 {
     double ddx[3], ddy[3], ddz[3], ddsq[3];   // constraint vectors at the stage start
     double shx[3], shy[3], shz[3];            // image shifts (stage 1)
     double K[3];        // stage 1: dt (trmi + trmj); stage 2: (trmi + trmj) |dd|^2
-    double rK[3];       // stage 2: 1 / K
     double ti[3], tj[3];   // trmi, trmj of each constraint
 };
 
@@ -131,7 +130,6 @@ __device__ inline void m_setup_v(const ShakeArgs &A, M3 &M)
         M.tj[c] = 0.5 * dt * M.rm[M_B[c]];
         M.K[c] = (M.ti[c] + M.tj[c]) *
                  (M.ddx[c] * M.ddx[c] + M.ddy[c] * M.ddy[c] + M.ddz[c] * M.ddz[c]);
-        M.rK[c] = 1. / M.K[c];
     }
 }
 
@@ -308,8 +306,7 @@ __device__ inline bool m_lambda_v(const M3 &M, double tolvel, double (&xt)[3], d
     for (int c = 0; c < 3; c++) {
         const int a = M_A[c], b = M_B[c];
         const double dvx = M.vx[b] - M.vx[a], dvy = M.vy[b] - M.vy[a], dvz = M.vz[b] - M.vz[a];
-        const double lambda =
-            shake_div_rcp(M.ddx[c] * dvx + M.ddy[c] * dvy + M.ddz[c] * dvz, M.K[c], M.rK[c]);
+        const double lambda = (M.ddx[c] * dvx + M.ddy[c] * dvy + M.ddz[c] * dvz) / M.K[c];
         cv = cv && fabs(lambda) < tolvel;
         const double li = lambda * M.ti[c], lj = lambda * -M.tj[c];
         ax[c] = M.ddx[c] * li;

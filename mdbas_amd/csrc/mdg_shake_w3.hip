@@ -71,7 +71,6 @@ struct Q3 {
     double ddx, ddy, ddz, ddsq;   // constraint vector at the stage start, |dd|^2
     double shx, shy, shz;         // image shift of the constraint vector (stage 1)
     double K;           // stage 1: dt (trmi + trmj); stage 2: (trmi + trmj) |dd|^2
-    double rK;          // stage 2: 1 / K
     double fA, fB;      // the atom's factors in its constraints cA, cB
     double dAx, dAy, dAz, dBx, dBy, dBz;   // dd of cA and cB
     double rma, rmb;    // inverse masses of constraint k's atoms
@@ -88,7 +87,7 @@ __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
     Q.thr = 1.;
     Q.ddx = Q.ddy = Q.ddz = Q.ddsq = 0.;
     Q.shx = Q.shy = Q.shz = 0.;
-    Q.K = Q.rK = 1.;
+    Q.K = 1.;
     Q.fA = Q.fB = 0.;
     Q.dAx = Q.dAy = Q.dAz = Q.dBx = Q.dBy = Q.dBz = 0.;
     if (k < 3) {
@@ -149,7 +148,6 @@ __device__ inline void q_setup_v(const ShakeArgs &A, Q3 &Q)
     const double dt = A.P.timeStep;
     const double trmi = 0.5 * dt * Q.rma, trmj = 0.5 * dt * Q.rmb;
     Q.K = Q.k < 3 ? (trmi + trmj) * (Q.ddx * Q.ddx + Q.ddy * Q.ddy + Q.ddz * Q.ddz) : 1.;
-    Q.rK = 1. / Q.K;
     q_factors(Q, trmi, trmj);
 }
 
@@ -328,7 +326,7 @@ __device__ inline bool q_lambda_v(const Q3 &Q, double tolvel, double &xt, double
     const double dvx = qperm<QP_B>(Q.vx) - qperm<QP_A>(Q.vx);
     const double dvy = qperm<QP_B>(Q.vy) - qperm<QP_A>(Q.vy);
     const double dvz = qperm<QP_B>(Q.vz) - qperm<QP_A>(Q.vz);
-    double lambda = shake_div_rcp(Q.ddx * dvx + Q.ddy * dvy + Q.ddz * dvz, Q.K, Q.rK);
+    double lambda = (Q.ddx * dvx + Q.ddy * dvy + Q.ddz * dvz) / Q.K;
     lambda = Q.k < 3 ? lambda : 0.;
     q_exchange(Q, lambda, xt, yt, zt);
     return quad_all(fabs(lambda) < tolvel);
