@@ -420,6 +420,19 @@ double mdg_prune_skin(const mdg_ctx *ctx);   /* the current pruning distance */
  * [1] entries in the current pruned list. */
 int mdg_prune_stats(mdg_ctx *ctx, long out[2]);
 
+/* How the mixed pair kernel of one rectangular box forms its pair
+ * displacements.  0 (default): in FP32 from each slot's offset to its
+ * cluster's origin plus the entry's origin-to-origin offset, while every
+ * offset stays below (shortest edge / 2 - cutOff) / 2 less 0.05 A, with the
+ * cutoff test in FP64 from the positions for any entry that has a listed
+ * pair whose FP32 r^2 lies within a rounding bound of cutOff^2 (so the
+ * in-cutoff set is the reference's exactly); otherwise, and for the
+ * cutoff functors, as 1.  1: in FP64 from the slot positions (the
+ * reference's dist(), utils.c:38-80), converted to FP32.  2: as 0 with
+ * the FP64 cutoff test on every entry (a check of the bound: its results
+ * equal 0's bit for bit).  No reference counterpart. */
+int mdg_set_pair_path(mdg_ctx *ctx, int path);
+
 /* Counters of the device cluster-pair list: [0] cluster pairs,
  * [1] listed atom pairs (mask bits), [2] clusters. */
 int mdg_list_stats(mdg_ctx *ctx, long out[3]);

@@ -99,7 +99,7 @@ EXPORTS = ("mdg_create mdg_destroy mdg_last_error mdg_precision mdg_key_link mdg
            "mdg_vv_integrate mdg_upload_state mdg_download_state mdg_initial_energy "
            "mdg_run_steps mdg_set_profiling mdg_phase_times mdg_list_stats "
            "mdg_list_pairs mdg_set_nonbond_params mdg_backward mdg_backward_traj "
-           "mdg_batch_create mdg_batch_size mdg_autotune mdg_set_prune mdg_prune_stats "
+           "mdg_batch_create mdg_batch_size mdg_autotune mdg_set_prune mdg_prune_stats mdg_set_pair_path "
            "mdg_prune_skin mdg_get_launch mdg_set_launch mdg_set_bonded "
            "mdg_read_simu mdg_read_conf mdg_read_rest mdg_read_forf mdg_setup "
            "mdg_charge_const mdg_init_box mdg_get_kinfromtemp mdg_image_update mdg_kinetic "
@@ -531,6 +531,12 @@ class Context:
         """Pruning distance beyond the cutoff for the mixed pair kernel's
         shorter per-step list (0 switches it off)."""
         self._chk(self.L.mdg_set_prune(self.h, C.c_double(skin)), "mdg_set_prune")
+
+    def set_pair_path(self, path: int):
+        """Pair displacements of the mixed kernel: 0 relative coordinates
+        where the box allows (default), 1 FP64 from the positions, 2 relative
+        with every entry's cutoff test in FP64."""
+        self._chk(self.L.mdg_set_pair_path(self.h, int(path)), "mdg_set_pair_path")
 
     def prune_stats(self):
         out = (C.c_long * 2)()

@@ -542,7 +542,8 @@ extern "C" void mdg_destroy(mdg_ctx *ctx)
                     ctx->adjS, ctx->adjPot, ctx->binCnt, ctx->binBucket, ctx->binOut,
                     ctx->binNOut, ctx->splI, ctx->splB, ctx->splD,
                     ctx->cl.pJ, ctx->cl.pMask, ctx->cl.pCnt, ctx->dispPart, ctx->ensSave,
-                    ctx->gridU, ctx->hvpBuf, ctx->bathDev};
+                    ctx->gridU, ctx->hvpBuf, ctx->bathDev, ctx->cl.cO, ctx->cl.cpD,
+                    ctx->cl.pD, ctx->spr, ctx->relPart};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->bt) {

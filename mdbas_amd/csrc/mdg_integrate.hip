@@ -682,6 +682,10 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
     A.fix = nullptr;
     A.fstr = 0;
     A.dispPart = nullptr;
+    A.spr = nullptr;
+    A.cO = nullptr;
+    A.ncluO = 0;
+    A.relPart = nullptr;
     if (opt) {
         A.ddStored = opt->ddStored;
         A.noKick = opt->noKick;
@@ -707,6 +711,12 @@ int mdg_vv_stage(mdg_ctx *ctx, int stage, double *fx, double *fy, double *fz, do
         A.fix = ctx->fix;
         A.fstr = ctx->fiy - ctx->fix;
         A.dispPart = ctx->dispPart;
+        if (ctx->cl.cO && ctx->spr) {
+            A.spr = ctx->spr;
+            A.cO = ctx->cl.cO;
+            A.ncluO = ctx->cl.nclu;
+            A.relPart = ctx->relPart;
+        }
         ctx->slotsDone = true;
     }
     if (batch) {

@@ -99,6 +99,12 @@ struct ClusterList {
     int *pJ = nullptr;
     unsigned long long *pMask = nullptr;
     int *pCnt = nullptr;         // per chunk: entries kept
+    // relative coordinates (mixed pair kernel, one box): a fixed FP64
+    // origin per cluster for the life of the list (its build-time bounding
+    // box centre), and per entry the minimum image of O_J - O_I in FP32
+    // (pD: the same for the pruned list's entries)
+    double *cO = nullptr;        // [3][maxclu]
+    float4 *cpD = nullptr, *pD = nullptr;
 };
 
 // Replica batch (mdg_batch.hip): many independent boxes concatenated in
@@ -254,6 +260,13 @@ struct mdg_ctx {
     double *sx = nullptr, *sy = nullptr, *sz = nullptr;        // slot positions
     double *dispPart = nullptr;  // per k_slot_pos block: max squared move since the last prune
     int nDispPart = 0;
+    // mixed, one box: slot positions relative to their cluster's origin
+    // (FP32, minimum image), and per slot-writing block the largest
+    // |component| of them (relPart); pairPath 0 automatic, 1 FP64
+    // displacements, 2 relative with every entry's cutoff test in FP64
+    float4 *spr = nullptr;
+    double *relPart = nullptr;
+    int pairPath = 0;
     double pruneSkin = 0.;       // > 0: the mixed pair kernel may use the pruned list
     bool pruneFresh = true;      // the list was rebuilt since the last slot update
     // device steps without a rebuild (one box of three-site molecules,
@@ -500,6 +513,38 @@ __device__ inline void mdg_block_partials(double (&v)[NV], double *out)
         const int nw = (blockDim.x + 63) >> 6;
         for (int w = 0; w < nw; w++) s += red[w][threadIdx.x];
         out[(long)blockIdx.x * NV + threadIdx.x] = s;
+    }
+}
+
+// Relative coordinates of slot s (cluster c) for the mixed pair kernel:
+// the minimum image of the position minus the cluster's origin (a
+// rectangular box; cO is [3][nclu]), stored as FP32.  Returns the largest
+// |component| (+inf for a NaN position).
+__device__ inline double mdg_slot_rel(const MdgBox &B, const double *cO, int nclu, int c, double x,
+                                      double y, double z, float4 *spr, int s)
+{
+    const double L0 = B.a1, L1 = B.type == 1 ? B.a1 : B.b2, L2 = B.type == 1 ? B.a1 : B.c3;
+    const double i0 = B.u1, i1 = B.type == 1 ? B.u1 : B.v2, i2 = B.type == 1 ? B.u1 : B.w3;
+    double u = x - cO[c], v = y - cO[nclu + c], w = z - cO[2 * (long)nclu + c];
+    u -= L0 * rint(u * i0);
+    v -= L1 * rint(v * i1);
+    w -= L2 * rint(w * i2);
+    spr[s] = make_float4((float)u, (float)v, (float)w, 0.f);
+    const double r = fmax(fabs(u), fmax(fabs(v), fabs(w)));
+    return r <= 1e300 ? r : INFINITY;
+}
+
+// Largest value over the block into out[blockIdx.x] (every thread calls).
+__device__ inline void mdg_block_max(double v, double *out)
+{
+    __shared__ double wm[16];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wm[0];
+        for (int w = 1; w < (int)((blockDim.x + 63) >> 6); w++) m = fmax(m, wm[w]);
+        out[blockIdx.x] = m;
     }
 }
 

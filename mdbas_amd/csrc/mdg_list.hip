@@ -460,13 +460,37 @@ __global__ void k_col_slots(int ncol, const int *__restrict__ colStart,
 
 // Bounding box of each cluster from wrapped coordinates (centre, half
 // extent, both FP32, extents rounded outward).
+//
+// With cO given (rectangular boxes), also the cluster's origin for the
+// relative coordinates of the mixed pair kernel: its first atom plus the
+// mean minimum-image offset of its atoms from that atom (a centre that does
+// not depend on where the box's seams cut the cluster), fixed until the
+// next build.
 __global__ void k_cluster_bbox(MdgBox B, int nclu, const int *__restrict__ slotAtom,
                                const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, float4 *__restrict__ bb,
-                               float4 *__restrict__ bbh)
+                               float4 *__restrict__ bbh, double *__restrict__ cO)
 {
     int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nclu) return;
+    if (cO) {
+        const double L[3] = {B.a1, B.type == 1 ? B.a1 : B.b2, B.type == 1 ? B.a1 : B.c3};
+        const double iL[3] = {B.u1, B.type == 1 ? B.u1 : B.v2, B.type == 1 ? B.u1 : B.w3};
+        double o[3] = {0., 0., 0.}, m[3] = {0., 0., 0.};
+        int cnt = 0;
+        for (int k = 0; k < MDG_CL; k++) {
+            const int a = slotAtom[I * MDG_CL + k];
+            if (a < 0) continue;
+            const double p[3] = {x[a], y[a], z[a]};
+            for (int d = 0; d < 3; d++) {
+                if (cnt == 0) o[d] = p[d];
+                const double u = p[d] - o[d];
+                m[d] += u - L[d] * rint(u * iL[d]);
+            }
+            cnt++;
+        }
+        for (int d = 0; d < 3; d++) cO[(long)d * nclu + I] = cnt ? o[d] + m[d] / cnt : 0.;
+    }
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int k = 0; k < MDG_CL; k++) {
         int a = slotAtom[I * MDG_CL + k];
@@ -757,20 +781,34 @@ k_cluster_pairs(MdgParams P, MdgBox B, int keyLink, int nx, int ny, int nz, int 
     }
 }
 
+// With cpD given: each entry's minimum image of O_J - O_I (cluster
+// origins, rectangular box), rounded to FP32.
 __global__ void k_cluster_compact(int nclu, int kmax, const int *__restrict__ tmpCnt,
                                   const int *__restrict__ tmpJ,
                                   const unsigned long long *__restrict__ tmpMask,
                                   const int *__restrict__ cpOff, long cap, int *__restrict__ cpJ,
                                   unsigned long long *__restrict__ cpMask, int *__restrict__ cpI,
-                                  int *__restrict__ ctrl)
+                                  int *__restrict__ ctrl, MdgBox B, const double *__restrict__ cO,
+                                  float4 *__restrict__ cpD)
 {
     int I = blockIdx.x;
     int o = cpOff[I], c = tmpCnt[I];
     for (int t = threadIdx.x; t < c; t += blockDim.x) {
         if (o + t < cap) {
-            cpJ[o + t] = tmpJ[(long)I * kmax + t];
+            const int J = tmpJ[(long)I * kmax + t];
+            cpJ[o + t] = J;
             cpMask[o + t] = tmpMask[(long)I * kmax + t];
             cpI[o + t] = I;
+            if (cpD) {
+                const double L[3] = {B.a1, B.type == 1 ? B.a1 : B.b2, B.type == 1 ? B.a1 : B.c3};
+                const double iL[3] = {B.u1, B.type == 1 ? B.u1 : B.v2, B.type == 1 ? B.u1 : B.w3};
+                float v[3];
+                for (int d = 0; d < 3; d++) {
+                    const double u = cO[(long)d * nclu + J] - cO[(long)d * nclu + I];
+                    v[d] = (float)(u - L[d] * rint(u * iL[d]));
+                }
+                cpD[o + t] = make_float4(v[0], v[1], v[2], 0.f);
+            }
         } else if (t == 0) {
             atomicOr(&ctrl[CTRL_LIST_OVF], 2);
         }
@@ -844,6 +882,10 @@ static int compact_cluster_list(mdg_ctx *ctx)
             HIPCHK(hipMalloc(&C.pJ, C.cap * sizeof(int)));
             HIPCHK(hipMalloc(&C.pMask, C.cap * sizeof(unsigned long long)));
             HIPCHK(hipMalloc(&C.pCnt, (size_t)C.maxChunks * sizeof(int)));
+            if (C.cO) {
+                HIPCHK(hipMalloc(&C.cpD, C.cap * sizeof(float4)));
+                HIPCHK(hipMalloc(&C.pD, C.cap * sizeof(float4)));
+            }
         }
     }
     // the pruned list belongs to the previous list: the next slot update
@@ -851,7 +893,7 @@ static int compact_cluster_list(mdg_ctx *ctx)
     ctx->pruneFresh = true;
     hipLaunchKernelGGL(k_cluster_compact, dim3(nclu), dim3(64), 0, ctx->stream, nclu, C.kmax,
                        C.tmpCnt, C.tmpJ, C.tmpMask, C.cpOff, C.cap, C.cpJ, C.cpMask, C.cpI,
-                       ctx->ctrl);
+                       ctx->ctrl, ctx->B, C.cO, C.cpD);
     LAUNCHCHK();
     // work chunks of at most MDG_CHUNK entries for the pair kernel; the
     // per-I chunk counts reuse tmpCnt's space after the compaction
@@ -890,6 +932,8 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
         HIPCHK(hipMalloc(&C.atomSlot, n * sizeof(int)));
         HIPCHK(hipMalloc(&C.bb, maxclu * sizeof(float4)));
         HIPCHK(hipMalloc(&C.bbh, maxclu * sizeof(float4)));
+        if (ctx->precision == MDG_MIXED && ctx->B.type != 3 && !ctx->bt)
+            HIPCHK(hipMalloc(&C.cO, 3 * (size_t)maxclu * sizeof(double)));
         HIPCHK(hipMalloc(&C.cpOff, (maxclu + 1) * sizeof(int)));
         HIPCHK(hipMalloc(&C.tmpCnt, maxclu * sizeof(int)));
         HIPCHK(hipMalloc(&C.cluCol, maxclu * sizeof(int)));
@@ -918,7 +962,7 @@ int mdg_build_cluster_list(mdg_ctx *ctx)
     // maximum and empty clusters produce nothing.
     const int nclu = C.nclu;
     hipLaunchKernelGGL(k_cluster_bbox, dim3(nblk(nclu, 128)), dim3(128), 0, ctx->stream, ctx->B,
-                       nclu, C.slotAtom, ctx->x, ctx->y, ctx->z, C.bb, C.bbh);
+                       nclu, C.slotAtom, ctx->x, ctx->y, ctx->z, C.bb, C.bbh, C.cO);
     LAUNCHCHK();
     if (ctx->keyLink) {
         // the link-cell predicate needs scaled coordinates and cells

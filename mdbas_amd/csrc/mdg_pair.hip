@@ -221,6 +221,7 @@ int mdg_excl_corr64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double *eo
 struct PairF {
     float alpha, c2a, qc, cutOn2, cutOff2, switch2, threeOn2;
     float rcutOff, rcutOff2;         // 1/rc, 1/rc^2 (SHIFT1 / SHIFT2)
+    float rsDelta;                   // relative coordinates: exact test within this of cutOff2
     double cutOff2d, L0, L1, L2, iL0, iL1, iL2;
 };
 
@@ -285,6 +286,13 @@ struct ClusterArgs {
     int nDisp;
     double pruneLim;                 // pruned list usable while 4 max move^2 <= pruneLim
     int *ctrl;
+    // relative coordinates (mixed, one rectangular box; see mx_pair_rel)
+    const float4 *spr;               // per slot (null: FP64 displacements only)
+    const float4 *cpD, *pD;          // per entry of the full / pruned list
+    const double *relPart;           // per slot-writing block: max |relative component|
+    int nRel;
+    double relLim;
+    int relExact;                    // every entry's cutoff test in FP64 (a check mode)
 };
 
 // Sum over the 8 lanes of each aligned group of 8 with DPP moves (plain
@@ -439,7 +447,7 @@ struct MxJ {
 };
 
 struct MxSlots {
-    __amdgpu_buffer_rsrc_t x, y, z, p;
+    __amdgpu_buffer_rsrc_t x, y, z, p, r;
     int vo8, vo16;                   // per-lane byte offsets of slot b
 };
 
@@ -475,46 +483,40 @@ __device__ inline long long mdg_f32_fix(float v)
     return (long long)(((unsigned long long)(unsigned int)(int)fl << 32) | lo);
 }
 
-template <bool BATCH, int ELEC>
-__device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsigned long long m,
-                                        int J, double xi, double yi, double zi, float qi, float ei,
-                                        float gi, double qid, double L0, double L1, double L2,
-                                        double iL0, double iL1, double iL2, int a, int b,
-                                        MxAcc &acc)
+// The reference's minimum-image r^2 of one pair (utils.c:38-80, 76): the
+// shift L * n is exact for |n| <= 1 (slot positions are never more than a
+// box edge apart) and so is the difference when n != 0 (Sterbenz), so
+// rint's ties-to-even and the reference's nint (utils.c:606-609) give the
+// same displacement wherever |dx| < L / 2 - rounding, i.e. for every pair
+// that can be inside the cutoff; each product is rounded, so the cutoff
+// test decides as energy.c:500 does.
+__device__ inline double mx_ref_r2(double &dx, double &dy, double &dz, double L0, double L1,
+                                   double L2, double iL0, double iL1, double iL2)
 {
-    double dx, dy, dz, r2;
-    {
 #pragma clang fp contract(off)
-        dx = j.x - xi;
-        dy = j.y - yi;
-        dz = j.z - zi;
-        // minimum image: the shift L * n is exact for |n| <= 1 (slot
-        // positions are never more than a box edge apart) and so is the
-        // difference when n != 0 (Sterbenz), so rint's ties-to-even and the
-        // reference's nint (utils.c:606-609) give the same displacement
-        // wherever |dx| < L / 2 - rounding, i.e. for every pair that can be
-        // inside the cutoff
-        dx = dx - L0 * rint(dx * iL0);
-        dy = dy - L1 * rint(dy * iL1);
-        dz = dz - L2 * rint(dz * iL2);
-        // the reference's r^2 (utils.c:76), each product rounded, so the
-        // cutoff test decides as energy.c:500 does
-        r2 = (dx * dx + dy * dy) + dz * dz;
-    }
-    const unsigned long long inm = m & __builtin_amdgcn_ballot_w64(r2 <= A.F.cutOff2d);
-    // an entry with no pair inside the cutoff adds exact zeros everywhere
-    // (zero coefficients, no j-force atomics): skip it, wave-uniformly.
-    // The list keeps entries with a pair within cutOff + delr (or + the
-    // prune skin), so a sixth to a third of them have none.
-    if (inm == 0) return;
+    dx = dx - L0 * rint(dx * iL0);
+    dy = dy - L1 * rint(dy * iL1);
+    dz = dz - L2 * rint(dz * iL2);
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// The pair arithmetic of one entry once its in-cutoff lanes inm are known:
+// (dx, dy, dz) = x_j - x_i and r2f its square (FP32), r2 the reference's
+// FP64 r^2 (read only by the cutoff functors' FP64 energy).
+template <bool BATCH, int ELEC>
+__device__ __forceinline__ void mx_body(const ClusterArgs &A, const float4 &jp,
+                                        unsigned long long inm, int J, float dxf, float dyf,
+                                        float dzf, float r2in, double r2, float qi, float ei,
+                                        float gi, double qid, int a, int b, MxAcc &acc)
+{
     const bool in = __builtin_amdgcn_inverse_ballot_w64(inm);
     acc.nin += __builtin_popcountll(inm);
-    const float r2f = in ? (float)r2 : A.F.cutOff2;
-    const float qq = in ? qi * j.p.x : 0.f;
-    const float veps = (in && A.P.vdwOn) ? ei * j.p.y : 0.f;
+    const float r2f = in ? r2in : A.F.cutOff2;
+    const float qq = in ? qi * jp.x : 0.f;
+    const float veps = (in && A.P.vdwOn) ? ei * jp.y : 0.f;
     const float rinv = __builtin_amdgcn_rsqf(r2f);
     const float r = r2f * rinv;
-    const float s = (gi + j.p.z) * rinv;
+    const float s = (gi + jp.z) * rinv;
     const float s2 = s * s;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
@@ -588,7 +590,7 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
         }
     }
     const float fs = (del + dvdw) * rinv;
-    const float fjx = fs * (float)dx, fjy = fs * (float)dy, fjz = fs * (float)dz;
+    const float fjx = fs * dxf, fjy = fs * dyf, fjz = fs * dzf;
     if (ELEC == PAIR_EWALD || ELEC == PAIR_NONE) acc.ed += eel;
     else acc.ed += eel64;
     acc.ev += evdw;
@@ -607,6 +609,112 @@ __device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsi
     }
 }
 
+// One entry with the displacement in FP64 from the slot positions.
+template <bool BATCH, int ELEC>
+__device__ __forceinline__ void mx_pair(const ClusterArgs &A, const MxJ &j, unsigned long long m,
+                                        int J, double xi, double yi, double zi, float qi, float ei,
+                                        float gi, double qid, double L0, double L1, double L2,
+                                        double iL0, double iL1, double iL2, int a, int b,
+                                        MxAcc &acc)
+{
+    double dx, dy, dz, r2;
+    {
+#pragma clang fp contract(off)
+        dx = j.x - xi;
+        dy = j.y - yi;
+        dz = j.z - zi;
+    }
+    r2 = mx_ref_r2(dx, dy, dz, L0, L1, L2, iL0, iL1, iL2);
+    const unsigned long long inm = m & __builtin_amdgcn_ballot_w64(r2 <= A.F.cutOff2d);
+    // an entry with no pair inside the cutoff adds exact zeros everywhere
+    // (zero coefficients, no j-force atomics): skip it, wave-uniformly.
+    // The list keeps entries with a pair within cutOff + delr (or + the
+    // prune skin), so a sixth to a third of them have none.
+    if (inm == 0) return;
+    mx_body<BATCH, ELEC>(A, j.p, inm, J, (float)dx, (float)dy, (float)dz, (float)r2, r2, qi, ei,
+                         gi, qid, a, b, acc);
+}
+
+// One entry in relative coordinates (one rectangular box, Ewald or no
+// electrostatics): x_j - x_i = D + (r_j - r_i) in FP32, with D the
+// entry's minimum image of O_J - O_I and r the slots' minimum images from
+// their cluster origins.  While every |r| component stays below relLim
+// ((min edge / 2 - cutoff) / 2 less a margin), this is the minimum image
+// of every pair inside the cutoff (any other image is at least an edge
+// minus the cutoff away).  Its FP32 r^2 is within rsDelta / 2 of the
+// reference's FP64 r^2 there (the bound is formed on the host from the
+// FP32 roundings), so lanes farther than rsDelta from the cutoff decide as
+// the reference does; an entry with a listed lane closer than that (or
+// every entry under relExact) takes the FP64 test of mx_pair for its
+// mask.  The results are the reference's in-cutoff set exactly.
+struct MxR {
+    float4 r, p;
+};
+
+template <int ELEC>
+__device__ __forceinline__ void mx_pair_rel(const ClusterArgs &A, const MxR &j,
+                                            unsigned long long m, float4 D, int J, float4 ri,
+                                            int si, float qi, float ei, float gi, int a, int b,
+                                            MxAcc &acc)
+{
+    const float dx = D.x + (j.r.x - ri.x), dy = D.y + (j.r.y - ri.y), dz = D.z + (j.r.z - ri.z);
+    const float r2f = dx * dx + dy * dy + dz * dz;
+    const unsigned long long near =
+        m & __builtin_amdgcn_ballot_w64(fabsf(r2f - A.F.cutOff2) < A.F.rsDelta);
+    unsigned long long inm;
+    if (near == 0 && !A.relExact) {
+        inm = m & __builtin_amdgcn_ballot_w64(r2f <= A.F.cutOff2);
+    } else {
+        const int sj = J * MDG_CL + b;
+        double ex, ey, ez;
+        {
+#pragma clang fp contract(off)
+            ex = A.sx[sj] - A.sx[si];
+            ey = A.sy[sj] - A.sy[si];
+            ez = A.sz[sj] - A.sz[si];
+        }
+        const double r2 = mx_ref_r2(ex, ey, ez, A.F.L0, A.F.L1, A.F.L2, A.F.iL0, A.F.iL1, A.F.iL2);
+        inm = m & __builtin_amdgcn_ballot_w64(r2 <= A.F.cutOff2d);
+    }
+    if (inm == 0) return;
+    mx_body<false, ELEC>(A, j.p, inm, J, dx, dy, dz, r2f, 0., qi, ei, gi, 0., a, b, acc);
+}
+
+// A wave-uniform float4 through the scalar cache (two 64-bit loads).
+__device__ inline float4 mx_uload_f4(const float4 *p, long i)
+{
+    const unsigned long long *q = (const unsigned long long *)p;
+    const unsigned long long lo = mdg_uload(q, 2 * i), hi = mdg_uload(q, 2 * i + 1);
+    return make_float4(__uint_as_float((unsigned int)lo), __uint_as_float((unsigned int)(lo >> 32)),
+                       __uint_as_float((unsigned int)hi), __uint_as_float((unsigned int)(hi >> 32)));
+}
+
+__device__ __forceinline__ MxR mx_load_rel(const MxSlots &R, int J)
+{
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    MxR o;
+    const i4v u = __builtin_amdgcn_raw_buffer_load_b128(R.r, R.vo16, J * 128, 0);
+    o.r = make_float4(__int_as_float(u.x), __int_as_float(u.y), __int_as_float(u.z),
+                      __int_as_float(u.w));
+    const i4v v = __builtin_amdgcn_raw_buffer_load_b128(R.p, R.vo16, J * 128, 0);
+    o.p = make_float4(__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z),
+                      __int_as_float(v.w));
+    return o;
+}
+
+// Whether this launch runs in relative coordinates: the context has them,
+// the electrostatics allow it, and the slot writer's per-block largest
+// |component| stays within relLim.
+template <bool BATCH, int ELEC>
+__device__ inline bool mx_use_rel(const ClusterArgs &A, int lane)
+{
+    if (BATCH || !(ELEC == PAIR_EWALD || ELEC == PAIR_NONE) || !A.spr || A.nRel <= 0) return false;
+    double r = 0.;
+    for (int k = lane; k < A.nRel; k += 64) r = fmax(r, A.relPart[k]);
+    for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+    return __builtin_amdgcn_readfirstlane((int)(r <= A.relLim)) != 0;
+}
+
 template <bool BATCH, int ELEC>
 __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
 {
@@ -620,6 +728,8 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
     R.y = __builtin_amdgcn_make_buffer_rsrc((void *)A.sy, 0, (int)(nsb * 8), 0x00020000);
     R.z = __builtin_amdgcn_make_buffer_rsrc((void *)A.sz, 0, (int)(nsb * 8), 0x00020000);
     R.p = __builtin_amdgcn_make_buffer_rsrc((void *)A.sp32, 0, (int)(nsb * 16), 0x00020000);
+    R.r = __builtin_amdgcn_make_buffer_rsrc((void *)A.spr, 0, A.spr ? (int)(nsb * 16) : 0,
+                                            0x00020000);
     R.vo8 = b * 8;
     R.vo16 = b * 16;
     double e_acc[5] = {0, 0, 0, 0, 0};
@@ -641,6 +751,8 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
     }
     const int *__restrict__ eJ = pruned ? A.pJ : A.cpJ;
     const unsigned long long *__restrict__ eM = pruned ? A.pMask : A.cpMask;
+    const bool rel = mx_use_rel<BATCH, ELEC>(A, lane);
+    const float4 *__restrict__ eD = pruned ? A.pD : A.cpD;
     for (int W = w0; W < nch; W += gridDim.x * 4) {
         const long long ch = mdg_uload((const long long *)A.chunk, W);   // int2 (I, e0)
         const int I = (int)(ch & 0xffffffffll), e0 = (int)(ch >> 32);
@@ -661,29 +773,51 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
             iL2 = Bx.type == 1 ? Bx.u1 : Bx.w3;
         }
         const int si = I * MDG_CL + a;
-        const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
         const float4 pi = A.sp32[si];
         const float qi = A.F.qc * pi.x;
-        const double qid = ELEC >= PAIR_SHIFT1 ? A.P.chargeConst * A.sq[si] : 0.;
         MxAcc acc = {0.f, 0.f, 0.f, 0., 0., 0.f, 0.f, 0};
         // entries e (set 0) and e + 1 (set 1) alternate; indices past the
         // end are clamped to the last entry (loaded, never computed)
-        int J0 = mdg_uload(eJ, e0);
-        unsigned long long m0 = mdg_uload(eM, e0);
-        MxJ j0 = mx_load(R, J0);
-        for (int e = e0;;) {
-            const int J1 = mdg_uload(eJ, min(e + 1, e1 - 1));
-            const unsigned long long m1 = mdg_uload(eM, min(e + 1, e1 - 1));
-            const MxJ j1 = mx_load(R, J1);
-            mx_pair<BATCH, ELEC>(A, j0, m0, J0, xi, yi, zi, qi, pi.y, pi.z, qid, L0, L1, L2, iL0,
-                                 iL1, iL2, a, b, acc);
-            if (++e >= e1) break;
-            J0 = mdg_uload(eJ, min(e + 1, e1 - 1));
-            m0 = mdg_uload(eM, min(e + 1, e1 - 1));
-            j0 = mx_load(R, J0);
-            mx_pair<BATCH, ELEC>(A, j1, m1, J1, xi, yi, zi, qi, pi.y, pi.z, qid, L0, L1, L2, iL0,
-                                 iL1, iL2, a, b, acc);
-            if (++e >= e1) break;
+        if (rel) {
+            const float4 ri = A.spr[si];
+            int J0 = mdg_uload(eJ, e0);
+            unsigned long long m0 = mdg_uload(eM, e0);
+            float4 D0 = mx_uload_f4(eD, e0);
+            MxR j0 = mx_load_rel(R, J0);
+            for (int e = e0;;) {
+                const int J1 = mdg_uload(eJ, min(e + 1, e1 - 1));
+                const unsigned long long m1 = mdg_uload(eM, min(e + 1, e1 - 1));
+                const float4 D1 = mx_uload_f4(eD, min(e + 1, e1 - 1));
+                const MxR j1 = mx_load_rel(R, J1);
+                mx_pair_rel<ELEC>(A, j0, m0, D0, J0, ri, si, qi, pi.y, pi.z, a, b, acc);
+                if (++e >= e1) break;
+                J0 = mdg_uload(eJ, min(e + 1, e1 - 1));
+                m0 = mdg_uload(eM, min(e + 1, e1 - 1));
+                D0 = mx_uload_f4(eD, min(e + 1, e1 - 1));
+                j0 = mx_load_rel(R, J0);
+                mx_pair_rel<ELEC>(A, j1, m1, D1, J1, ri, si, qi, pi.y, pi.z, a, b, acc);
+                if (++e >= e1) break;
+            }
+        } else {
+            const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
+            const double qid = ELEC >= PAIR_SHIFT1 ? A.P.chargeConst * A.sq[si] : 0.;
+            int J0 = mdg_uload(eJ, e0);
+            unsigned long long m0 = mdg_uload(eM, e0);
+            MxJ j0 = mx_load(R, J0);
+            for (int e = e0;;) {
+                const int J1 = mdg_uload(eJ, min(e + 1, e1 - 1));
+                const unsigned long long m1 = mdg_uload(eM, min(e + 1, e1 - 1));
+                const MxJ j1 = mx_load(R, J1);
+                mx_pair<BATCH, ELEC>(A, j0, m0, J0, xi, yi, zi, qi, pi.y, pi.z, qid, L0, L1, L2,
+                                     iL0, iL1, iL2, a, b, acc);
+                if (++e >= e1) break;
+                J0 = mdg_uload(eJ, min(e + 1, e1 - 1));
+                m0 = mdg_uload(eM, min(e + 1, e1 - 1));
+                j0 = mx_load(R, J0);
+                mx_pair<BATCH, ELEC>(A, j1, m1, J1, xi, yi, zi, qi, pi.y, pi.z, qid, L0, L1, L2,
+                                     iL0, iL1, iL2, a, b, acc);
+                if (++e >= e1) break;
+            }
         }
         // i forces: sums over the 8 j lanes of each i atom; lanes b = 0, 1,
         // 2 of the group add x, y, z
@@ -725,6 +859,9 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
 // of this kernel, which rewrites the slots untracked, clears it too).  Each
 // block writes its largest squared move (minimum image), or +inf when the
 // pruned list does not belong to the previous positions.
+//
+// With spr given (mixed, one rectangular box) every variant also writes the
+// slots' relative coordinates and per block their largest |component|.
 struct SlotMove {
     int *ctrl;
     int fresh;                       // the list was rebuilt: slots changed atoms
@@ -733,6 +870,10 @@ struct SlotMove {
     const MdgBox *boxes;             // batch: box of each replica
     const int *crep;                 // batch: replica of each cluster
     double *part;                    // per block: max squared move
+    float4 *spr;                     // relative coordinates (or null)
+    const double *cO;                // cluster origins [3][nclu]
+    int nclu;
+    double *relPart;                 // per block: max |relative component|
 };
 
 template <bool MIXED>
@@ -743,6 +884,13 @@ k_slot_pos(int nslot, const int *__restrict__ slotAtom, const double *__restrict
            long long *fiz, double *fdx, double *fdy, double *fdz, SlotMove M)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double rel = 0.;
+    if (M.spr && s < nslot) {
+        const int a = slotAtom[s];
+        if (a >= 0) rel = mdg_slot_rel(M.B, M.cO, M.nclu, s / MDG_CL, x[a], y[a], z[a], M.spr, s);
+        else M.spr[s] = make_float4(1e6f, 1e6f, 1e6f, 0.f);   // never in a mask
+    }
+    if (M.spr) mdg_block_max(rel, M.relPart);
     if (!MIXED) {
         if (s == 0) M.ctrl[CTRL_PRUNE] = 0;
         if (s >= nslot) return;
@@ -875,6 +1023,41 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     // usable while 2 max move <= skin, with a relative margin for rounding
     A.pruneLim = ctx->pruneSkin > 0. && C.pJ ? ctx->pruneSkin * ctx->pruneSkin * (1. - 1e-6) : -1.;
     A.ctrl = ctx->ctrl;
+    // relative coordinates (the launch sets nRel from the slot writer)
+    A.spr = nullptr;
+    A.cpD = C.cpD;
+    A.pD = C.pD;
+    A.relPart = ctx->relPart;
+    A.nRel = 0;
+    A.relLim = -1.;
+    A.relExact = ctx->pairPath == 2 ? 1 : 0;
+    A.F.rsDelta = 0.f;
+    if (ctx->precision == MDG_MIXED && !ctx->bt && C.cO && C.cpD && ctx->spr &&
+        ctx->pairPath != 1) {
+        const double L[3] = {A.F.L0, A.F.L1, A.F.L2};
+        const double lmin = std::min(L[0], std::min(L[1], L[2]));
+        const double lmax = std::max(L[0], std::max(L[1], L[2]));
+        const double rc = ctx->P.cutOff, rc2 = ctx->P.cutOff2;
+        // every |relative component| below relLim keeps 2 relLim + rc below
+        // half the shortest edge, with 0.1 A to spare
+        const double lim = 0.5 * (0.5 * lmin - rc) - 0.05;
+        if (lim > 0.) {
+            // FP32 unit in the last place of v > 0
+            auto ulp = [](double v) { return std::ldexp(1.0, std::ilogb(v) - 23); };
+            // per component: two relative coordinates (one rounding each),
+            // D (|D| <= half an edge), their difference and the final sum
+            // near the cutoff, each rounded to nearest
+            const double eps = ulp(lim) + 0.5 * ulp(0.5 * lmax) + 0.5 * ulp(2. * lim) +
+                               0.5 * ulp(1.01 * rc);
+            // |r2f - r^2| near the cutoff: 2 |d|_1 eps + 3 eps^2, the three
+            // roundings of the sum of squares, the FP32 cutoff's own rounding
+            const double bound = 2. * std::sqrt(3.) * 1.01 * rc * eps + 3. * eps * eps +
+                                 1.5 * ulp(1.01 * rc2) + 0.5 * ulp(rc2) + 1e-12 * rc2;
+            A.spr = ctx->spr;
+            A.relLim = lim;
+            A.F.rsDelta = (float)(4. * bound);
+        }
+    }
     return A;
 }
 
@@ -891,6 +1074,12 @@ static int slot_pos(mdg_ctx *ctx, bool mixed)
     M.boxes = ctx->bt ? ctx->bt->boxes : nullptr;
     M.crep = ctx->bt ? ctx->bt->crep : nullptr;
     M.part = ctx->dispPart;
+    // relative coordinates whenever the list has cluster origins (every
+    // variant keeps them in step with the slot positions)
+    M.spr = C.cO && ctx->spr ? ctx->spr : nullptr;
+    M.cO = C.cO;
+    M.nclu = C.nclu;
+    M.relPart = ctx->relPart;
     if (mixed) {
         ctx->pruneFresh = false;
         hipLaunchKernelGGL(k_slot_pos<true>, dim3(nblk(nslot, 256)), dim3(256), 0, ctx->stream,
@@ -944,6 +1133,7 @@ int mdg_pair_cluster(mdg_ctx *ctx)
     ctx->slotsDone = false;
     ClusterArgs A = cluster_args(ctx);
     if (A.nDisp) A.nDisp = nDisp;
+    if (A.spr) A.nRel = nDisp;   // the slot writer just before wrote the relative coordinates
     const int nb = mdg_pair_blocks(ctx);
     if (ctx->profiling) HIPCHK(hipEventRecord(ctx->evCur[0], ctx->stream));
     launch_pair<false>(ctx, mixed, nb, A);
@@ -987,6 +1177,8 @@ PruneArgs mdg_prune_args(mdg_ctx *ctx)
     P.pJ = C.pJ;
     P.pMask32 = (unsigned int *)C.pMask;
     P.pCnt = C.pCnt;
+    P.cpD = C.cpD;
+    P.pD = C.cpD ? C.pD : nullptr;
     P.rp = ctx->P.cutOff + ctx->pruneSkin;
     P.rp2 = P.rp * P.rp;
     P.ctrl = ctx->ctrl;

@@ -80,6 +80,10 @@ static int ensure_slots(mdg_ctx *ctx)
         // molecule per wavefront, four wavefronts per block)
         const int cap = std::max(ctx->nDispPart, nblk(std::max(ctx->nGrp, 1), 4) + 1);
         HIPCHK(hipMalloc(&ctx->dispPart, cap * sizeof(double)));
+        if (!ctx->bt) {
+            HIPCHK(hipMalloc(&ctx->relPart, cap * sizeof(double)));
+            HIPCHK(hipMalloc(&ctx->spr, ns * sizeof(float4)));
+        }
     }
     return MDG_OK;
 }
@@ -635,6 +639,14 @@ extern "C" int mdg_set_prune(mdg_ctx *ctx, double skin)
 }
 
 extern "C" double mdg_prune_skin(const mdg_ctx *ctx) { return ctx ? ctx->pruneSkin : 0.; }
+
+extern "C" int mdg_set_pair_path(mdg_ctx *ctx, int path)
+{
+    if (!ctx || path < 0 || path > 2) return MDG_EARG;
+    ctx->pairPath = path;
+    mdg_invalidate_graphs(ctx);
+    return MDG_OK;
+}
 
 extern "C" int mdg_get_launch(const mdg_ctx *ctx, int out[4])
 {

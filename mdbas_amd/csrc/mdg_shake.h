@@ -57,6 +57,12 @@ struct ShakeArgs {
     long long *fix;
     long fstr;
     double *dispPart;
+    // and the slots' relative coordinates when spr is given (mdg_slot_rel),
+    // with per block their largest |component| into relPart
+    float4 *spr;
+    const double *cO;
+    int ncluO;
+    double *relPart;
     // thermostat and barostat integrators (generic-group kernels only):
     // ddStored takes the constraint vectors from dd, formed at the call's
     // entry, instead of from the positions the kernel loads; noKick skips

@@ -486,6 +486,7 @@ __global__ void __launch_bounds__(256) k_m3_vv1_b(ShakeArgs A)
     const int g = m_group(A);
     double vblk = 0.;   // this lane's molecule's SHAKE virial
     double d2 = 0.;     // largest squared move of this lane's atoms
+    double rel = 0.;    // largest |relative coordinate| of this lane's atoms
     const bool track = A.track && A.ctrl[CTRL_PRUNE] == 1;
     if (g < A.nGrp) {
         M3 M;
@@ -540,6 +541,9 @@ __global__ void __launch_bounds__(256) k_m3_vv1_b(ShakeArgs A)
                     if (!(e <= 1e300)) e = INFINITY;   // NaN positions: no pruning
                     d2 = fmax(d2, e);
                 }
+                if (A.spr)
+                    rel = fmax(rel, mdg_slot_rel(A.B, A.cO, A.ncluO, s / MDG_CL, nx, ny, nz,
+                                                 A.spr, s));
                 A.sx[s] = nx;
                 A.sy[s] = ny;
                 A.sz[s] = nz;
@@ -562,6 +566,7 @@ __global__ void __launch_bounds__(256) k_m3_vv1_b(ShakeArgs A)
         if (threadIdx.x == 0)
             A.dispPart[blockIdx.x] =
                 track ? fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3])) : INFINITY;
+        if (A.spr) mdg_block_max(rel, A.relPart);
     }
 }
 

@@ -539,11 +539,12 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
         }
     }
     // slot positions (the atom's own store above is visible to this lane)
-    double d2 = 0.;
+    double d2 = 0., rel = 0.;
     const bool track = A.track && A.ctrl[CTRL_PRUNE] == 1;
     if (!BATCH && A.slotWrite && myAtom >= 0) {
         const int s = A.atomSlot[myAtom];
         const double nx = A.x[myAtom], ny = A.y[myAtom], nz = A.z[myAtom];
+        if (A.spr) rel = mdg_slot_rel(A.B, A.cO, A.ncluO, s / MDG_CL, nx, ny, nz, A.spr, s);
         if (track) {
             const double L0 = A.B.a1, L1 = A.B.type == 1 ? A.B.a1 : A.B.b2,
                          L2 = A.B.type == 1 ? A.B.a1 : A.B.c3;
@@ -574,6 +575,7 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
         if (threadIdx.x == 0)
             A.dispPart[blockIdx.x] =
                 track ? fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3])) : INFINITY;
+        if (A.spr) mdg_block_max(rel, A.relPart);
     }
 }
 

@@ -320,6 +320,8 @@ struct PruneArgs {
     int *pJ;
     unsigned int *pMask32;           // the 64-bit masks as two 32-bit halves
     int *pCnt;
+    const float4 *cpD;               // entry origin offsets, copied with the
+    float4 *pD;                      // kept entries (relative coordinates)
     double rp2, rp;                  // (cutOff + skin)^2, cutOff + skin
     int *ctrl;
 };
@@ -426,6 +428,7 @@ __device__ inline void prune_waves(const PruneArgs &A, int w0, int nw)
         const int pos = __builtin_popcount(kept & ((1u << t) - 1u));
         if (live && ((kept >> t) & 1u)) {
             if (h == 0) A.pJ[e0 + pos] = J;
+            if (h == 0 && A.pD) A.pD[e0 + pos] = A.cpD[e0 + t];
             A.pMask32[2 * (long)(e0 + pos) + h] = keep;
         }
         if (lane == 0) A.pCnt[W] = __builtin_popcount(kept);

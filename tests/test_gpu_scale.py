@@ -170,6 +170,64 @@ def test_mixed_forces_against_oracle_at_scale(which, c1_box, c2_box, oracle_ener
     assert r.pot == pytest.approx(eo["pot"], rel=5e-6)
 
 
+def _mixed_run(wb, path, skin=0.0, nsteps=25):
+    ctx = M.Context(wb, precision=M.MDG_MIXED)
+    ctx.set_pair_path(path)
+    ctx.set_prune(skin)
+    ctx.upload(wb.x, wb.y, wb.z, wb.vx, wb.vy, wb.vz)
+    r0 = ctx.initial_energy()
+    f0 = np.stack(ctx.download()[6:9])
+    recs = M.records_to_dict(ctx.run(nsteps, use_graph=True))
+    out = (r0, f0, recs, ctx.download(), ctx.prune_stats())
+    ctx.close()
+    return out
+
+
+def test_mixed_relative_coordinates_c1(c1_box, oracle_energy):
+    """The mixed pair kernel in relative coordinates (path 0: FP32
+    displacements from cluster origins, the FP64 cutoff test only for
+    entries with a listed pair near the cutoff) against the same with the
+    FP64 test on every entry (path 2): bit-identical step-0 forces and a
+    bit-identical 25-step trajectory through the rebuild at step 20, so the
+    rounding bound never lets the FP32 test decide otherwise than the
+    reference's FP64 one.  Against FP64 displacements (path 1): the paths
+    differ (path 0 does run in relative coordinates), the in-cutoff counts
+    agree at step 0 (within 1e-6 over the 25 steps, as the trajectories part
+    at the FP32 level), step-0 forces agree to 1e-6, and both meet the mixed
+    bar against the oracle (RMS relative force error <= 1e-5, nInCut
+    exact).  With the pruned list (0.8 A) path 0 is the full list's bit
+    for bit."""
+    wb = c1_box
+    fo, eo = oracle_energy(wb, "c1")
+    a = _mixed_run(wb, 0)
+    b = _mixed_run(wb, 2)
+    c = _mixed_run(wb, 1)
+    say("rel vs exact", a[0].pot, b[0].pot, "rel vs fp64-disp pot", a[0].pot / c[0].pot - 1,
+        "forces", relerr(a[1], c[1]), "vs oracle", relerr(a[1], fo), relerr(c[1], fo))
+    assert np.array_equal(a[1], b[1])
+    assert a[0].pot == b[0].pot and a[0].nInCut == b[0].nInCut
+    for k in ("tot", "pot", "kin", "eDir", "virelec", "virvdw", "virshake", "nInCut"):
+        assert np.array_equal(a[2][k], b[2][k]), k
+    for u, v in zip(a[3][:6], b[3][:6]):
+        assert np.array_equal(u, v)
+    assert not np.array_equal(a[1], c[1])
+    # the two trajectories part at the FP32 level, and within a few steps
+    # a pair or two on the cutoff sphere falls on different sides
+    assert np.abs(a[2]["nInCut"] - c[2]["nInCut"]).max() <= 1e-6 * eo["nInCut"]
+    assert relerr(a[1], c[1]) < 1e-6
+    for r, f in ((a[0], a[1]), (c[0], c[1])):
+        assert r.nInCut == eo["nInCut"]
+        assert relerr(f, fo) <= 1e-5
+    p = _mixed_run(wb, 0, skin=0.8, nsteps=45)
+    q = _mixed_run(wb, 0, skin=0.0, nsteps=45)
+    say("pruned launches", p[4])
+    assert p[4]["pruned_launches"] >= 30
+    for k in ("tot", "pot", "eDir", "virelec", "virvdw", "nInCut"):
+        assert np.array_equal(p[2][k], q[2][k]), k
+    for u, v in zip(p[3][:6], q[3][:6]):
+        assert np.array_equal(u, v)
+
+
 def test_device_fp64_trajectory_c1(c1_box):
     """25 FP64 device steps of the 30,000-atom box (dt 2 fs, SHAKE, list
     rebuilt at step 20) against the oracle's MD loop."""
