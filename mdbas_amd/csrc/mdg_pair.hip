@@ -220,6 +220,7 @@ int mdg_excl_corr64(mdg_ctx *ctx, double *fx, double *fy, double *fz, double *eo
 // FP32 constants of the mixed pair kernel.
 struct PairF {
     float alpha, c2a, qc, cutOn2, cutOff2, switch2, threeOn2;
+    float swA, swB, swD;             // 2 switch2, (cutOff2 - 3 cutOn2) switch2, 12 switch2
     float rcutOff, rcutOff2;         // 1/rc, 1/rc^2 (SHIFT1 / SHIFT2)
     float rsDelta;                   // relative coordinates: exact test within this of cutOff2
     double cutOff2d, L0, L1, L2, iL0, iL1, iL2;
@@ -294,6 +295,33 @@ struct ClusterArgs {
     double relLim;
     int relExact;                    // every entry's cutoff test in FP64 (a check mode)
 };
+
+// The three j-force components of each aligned group of 8 lanes (the 8
+// i atoms of one j atom), summed and scattered in three DPP steps: the
+// group's lanes a = 0, 1 end with the x sum, a = 2, 3 with y, a = 4, 5
+// with z (a = 6, 7 with zero).  Step 1 pairs lane a with 7 - a
+// (row_half_mirror): lanes a < 4 keep x and y and receive the partner's,
+// lanes a >= 4 keep z; step 2 pairs a with a ^ 2 (quad_perm [2,3,0,1]),
+// each keeping one of its two values; step 3 adds lane a ^ 1's.
+template <int CTRL>
+__device__ inline float dpp_movf(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+__device__ inline float dpp_rs8(float fx, float fy, float fz, int a)
+{
+    const bool hi = a >= 4;
+    float k0 = hi ? fz : fx, k1 = hi ? 0.f : fy;
+    const float s0 = hi ? fx : fz, s1 = hi ? fy : 0.f;
+    k0 += dpp_movf<0x141>(s0);
+    k1 += dpp_movf<0x141>(s1);
+    const bool b1 = (a & 2) != 0;
+    float k = b1 ? k1 : k0;
+    k += dpp_movf<0x4E>(b1 ? k0 : k1);
+    k += dpp_movf<0xB1>(k);
+    return k;
+}
 
 // Sum over the 8 lanes of each aligned group of 8 with DPP moves (plain
 // VALU ops, no LDS): quad_perm [1,0,3,2], quad_perm [2,3,0,1], then
@@ -527,9 +555,11 @@ __device__ __forceinline__ void mx_body(const ClusterArgs &A, const float4 &jp,
     const bool sw = r2f > A.F.cutOn2;
     const float u = sw ? r2f : A.F.cutOn2;
     const float sw1 = A.F.cutOff2 - u;
-    const float Su = sw1 * sw1 * (A.F.cutOff2 + 2.f * u - A.F.threeOn2) * A.F.switch2;
+    // S = sw1^2 (cutOff2 + 2 u - 3 cutOn2) switch2 and
+    // dS = 12 u sw1 (cutOn2 - u) switch2, the constants folded on the host
+    const float Su = (sw1 * sw1) * __builtin_fmaf(u, A.F.swA, A.F.swB);
     const float S = sw ? Su : 1.f;
-    const float dS = 12.f * u * sw1 * (A.F.cutOn2 - u) * A.F.switch2;
+    const float dS = (u * sw1) * (A.F.cutOn2 - u) * A.F.swD;
     const float evdw = pv * S;
     const float dvdw = (pv * dS + dpv * S) * rinv;
     float eel, del;
@@ -599,12 +629,12 @@ __device__ __forceinline__ void mx_body(const ClusterArgs &A, const float4 &jp,
     acc.fx += fjx;
     acc.fy += fjy;
     acc.fz += fjz;
-    const float gx = dpp_sum8(fjx), gy = dpp_sum8(fjy), gz = dpp_sum8(fjz);
-    const float gc = a == 0 ? gx : (a == 1 ? gy : gz);
-    if (a < 3) {
+    // lanes a = 0, 2, 4 of each j group add x, y, z
+    const float gc = dpp_rs8(fjx, fjy, fjz, a);
+    if ((a & 1) == 0 && a < 6) {
         const long long iv = mdg_f32_fix(-gc);
         if (iv != 0)
-            atomicAdd((unsigned long long *)(A.fix + a * A.fstr + J * MDG_CL + b),
+            atomicAdd((unsigned long long *)(A.fix + (a >> 1) * A.fstr + J * MDG_CL + b),
                       (unsigned long long)iv);
     }
 }
@@ -715,7 +745,12 @@ __device__ inline bool mx_use_rel(const ClusterArgs &A, int lane)
     return __builtin_amdgcn_readfirstlane((int)(r <= A.relLim)) != 0;
 }
 
-template <bool BATCH, int ELEC>
+// REL (one box, Ewald or no electrostatics, a box the relative coordinates
+// fit): the relative-coordinate loop, with a plain FP64-displacement loop
+// for launches whose guard fails; otherwise the FP64-displacement loop with
+// its loads double-buffered.  The two variants differ in register use (the
+// relative loop alone fits six wavefronts per SIMD).
+template <bool BATCH, int ELEC, bool REL>
 __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
 {
     // ahead of the SPME kernels beside it when the launch shape asks for it
@@ -751,7 +786,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
     }
     const int *__restrict__ eJ = pruned ? A.pJ : A.cpJ;
     const unsigned long long *__restrict__ eM = pruned ? A.pMask : A.cpMask;
-    const bool rel = mx_use_rel<BATCH, ELEC>(A, lane);
+    const bool rel = REL && mx_use_rel<BATCH, ELEC>(A, lane);
     const float4 *__restrict__ eD = pruned ? A.pD : A.cpD;
     for (int W = w0; W < nch; W += gridDim.x * 4) {
         const long long ch = mdg_uload((const long long *)A.chunk, W);   // int2 (I, e0)
@@ -778,7 +813,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
         MxAcc acc = {0.f, 0.f, 0.f, 0., 0., 0.f, 0.f, 0};
         // entries e (set 0) and e + 1 (set 1) alternate; indices past the
         // end are clamped to the last entry (loaded, never computed)
-        if (rel) {
+        if (REL && rel) {
             const float4 ri = A.spr[si];
             int J0 = mdg_uload(eJ, e0);
             unsigned long long m0 = mdg_uload(eM, e0);
@@ -797,6 +832,14 @@ __global__ void __launch_bounds__(256) k_pair_mixed(ClusterArgs A)
                 j0 = mx_load_rel(R, J0);
                 mx_pair_rel<ELEC>(A, j1, m1, D1, J1, ri, si, qi, pi.y, pi.z, a, b, acc);
                 if (++e >= e1) break;
+            }
+        } else if (REL) {
+            // the guard failed: FP64 displacements, one entry at a time
+            const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
+            for (int e = e0; e < e1; e++) {
+                const int J = mdg_uload(eJ, e);
+                mx_pair<BATCH, ELEC>(A, mx_load(R, J), mdg_uload(eM, e), J, xi, yi, zi, qi, pi.y,
+                                     pi.z, 0., L0, L1, L2, iL0, iL1, iL2, a, b, acc);
             }
         } else {
             const double xi = A.sx[si], yi = A.sy[si], zi = A.sz[si];
@@ -978,6 +1021,9 @@ static ClusterArgs cluster_args(mdg_ctx *ctx)
     A.F.cutOff2 = (float)ctx->P.cutOff2;
     A.F.switch2 = (float)ctx->P.switch2;
     A.F.threeOn2 = (float)(3.0 * ctx->P.cutOn2);
+    A.F.swA = (float)(2.0 * ctx->P.switch2);
+    A.F.swB = (float)((ctx->P.cutOff2 - 3.0 * ctx->P.cutOn2) * ctx->P.switch2);
+    A.F.swD = (float)(12.0 * ctx->P.switch2);
     A.F.rcutOff = (float)ctx->P.rcutOff;
     A.F.rcutOff2 = (float)ctx->P.rcutOff2;
     A.F.cutOff2d = ctx->P.cutOff2;
@@ -1107,8 +1153,12 @@ static void launch_pair(mdg_ctx *ctx, bool mixed, int nb, const ClusterArgs &A)
 {
 #define MDG_PAIR_LAUNCH(E)                                                                         \
     do {                                                                                           \
-        if (mixed)                                                                                 \
-            hipLaunchKernelGGL((k_pair_mixed<BATCH, E>), dim3(nb), dim3(256), 0, ctx->stream, A);  \
+        if (mixed && !BATCH && A.spr && (E == PAIR_EWALD || E == PAIR_NONE))                       \
+            hipLaunchKernelGGL((k_pair_mixed<false, E, true>), dim3(nb), dim3(256), 0, ctx->stream, \
+                               A);                                                                 \
+        else if (mixed)                                                                            \
+            hipLaunchKernelGGL((k_pair_mixed<BATCH, E, false>), dim3(nb), dim3(256), 0,            \
+                               ctx->stream, A);                                                    \
         else                                                                                       \
             hipLaunchKernelGGL((k_pair_fp64<BATCH, E>), dim3(nb), dim3(256), 0, ctx->stream, A);   \
     } while (0)
