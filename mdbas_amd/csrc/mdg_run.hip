@@ -1045,9 +1045,14 @@ extern "C" int mdg_autotune(mdg_ctx *ctx, int nsteps, int out[2])
         *param = best;
     };
     if (ctx->shakeW3) sweep(&ctx->tune.shkSpread, {0, 1, 2, 4, 8});
-    sweep(&ctx->tune.pairBlocks, {512, 768, 1024, 2048, 4096, 8192, 16384});
-    sweep(&ctx->tune.spmeFirst, {0, 1});
-    sweep(&ctx->tune.spmePrio, {0, 1, 2});
+    // the pair grid, the fork order and the priorities interact (a full
+    // grid leaves the SPME branch no room beside it, which only matters
+    // when that branch starts early): two rounds of the three sweeps
+    for (int round = 0; round < 2; round++) {
+        sweep(&ctx->tune.pairBlocks, {512, 768, 1024, 2048, 4096, 8192, 16384});
+        sweep(&ctx->tune.spmeFirst, {0, 1});
+        sweep(&ctx->tune.spmePrio, {0, 1, 2});
+    }
     if (ctx->pruneSkin > 0. && st == MDG_OK) {
         // the pruned list, or the full list every step (no prune kernel)
         const double skin = ctx->pruneSkin;

@@ -241,9 +241,12 @@ k_spme_influence(MdgBox B, int m1, int m2, int m3, int mh, double rAlpha2,
     // grid-stride: at most mdg_infl_blocks blocks, so the record sums few rows
     for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < G;
          g += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(g % mh);
-        const int j = (int)((g / mh) % m2);
-        const int i = (int)(g / ((long)mh * m2));
+        // (i, j, k) in 32-bit arithmetic (a half spectrum has < 2^31 points;
+        // 64-bit division costs several times more)
+        const unsigned gu = (unsigned)g, row = gu / (unsigned)mh;
+        const int k = (int)(gu - row * (unsigned)mh);
+        const unsigned iu = row / (unsigned)m2;
+        const int i = (int)iu, j = (int)(row - iu * (unsigned)m2);
         const T th = theta[g];
         const T re = cg[2 * g], im = cg[2 * g + 1];
         if (th != (T)0) {
@@ -1494,9 +1497,12 @@ k_b_spread(MdgParams P, const MdgBox *__restrict__ boxes, const int *__restrict_
 }
 
 // Influence function of replica blockIdx.x over its half spectrum, and its
-// energy / stress sums (one block per replica, fixed order).
+// energy / stress sums (one block per replica, fixed order).  256 threads:
+// 1024 halved the kernel's time alone but quadrupled it beside the pair
+// kernel, which leaves few CUs with 16 free wavefront slots.
+#define B_INFL_THREADS 256
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(B_INFL_THREADS)
 k_b_influence(const MdgBox *__restrict__ boxes, const int *__restrict__ mesh,
               const long *__restrict__ hOff, double rAlpha2, const T *__restrict__ theta, T *cg,
               double *__restrict__ repInfl)
@@ -1507,9 +1513,11 @@ k_b_influence(const MdgBox *__restrict__ boxes, const int *__restrict__ mesh,
     const long H = (long)m * m * mh, h0 = hOff[r];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (long g = threadIdx.x; g < H; g += blockDim.x) {
-        const int k = (int)(g % mh);
-        const int j = (int)((g / mh) % m);
-        const int i = (int)(g / ((long)mh * m));
+        // (i, j, k) in 32-bit arithmetic (a replica's half spectrum is small)
+        const unsigned gu = (unsigned)g, row = gu / (unsigned)mh;
+        const int k = (int)(gu - row * (unsigned)mh);
+        const unsigned iu = row / (unsigned)m;
+        const int i = (int)iu, j = (int)(row - iu * (unsigned)m);
         const T th = theta[h0 + g];
         T *c = cg + 2 * (h0 + g);
         const T re = c[0], im = c[1];
@@ -2039,7 +2047,7 @@ int mdg_b_spme(mdg_ctx *ctx)
                 return MDG_EHIP;
             }
         }
-        hipLaunchKernelGGL(k_b_influence<float>, dim3(R), dim3(256), 0, ctx->stream, T.boxes,
+        hipLaunchKernelGGL(k_b_influence<float>, dim3(R), dim3(B_INFL_THREADS), 0, ctx->stream, T.boxes,
                            T.mesh, T.hOff, rAlpha2, ctx->theta32, ctx->cgrid32, T.repInfl);
         LAUNCHCHK();
         for (MdgFftGroup &F : T.fft) {
@@ -2073,7 +2081,7 @@ int mdg_b_spme(mdg_ctx *ctx)
                 return MDG_EHIP;
             }
         }
-        hipLaunchKernelGGL(k_b_influence<double>, dim3(R), dim3(256), 0, ctx->stream, T.boxes,
+        hipLaunchKernelGGL(k_b_influence<double>, dim3(R), dim3(B_INFL_THREADS), 0, ctx->stream, T.boxes,
                            T.mesh, T.hOff, rAlpha2, ctx->theta, ctx->cgrid64, T.repInfl);
         LAUNCHCHK();
         for (MdgFftGroup &F : T.fft) {

@@ -183,7 +183,8 @@ def _mixed_run(wb, path, skin=0.0, nsteps=25):
     return out
 
 
-def test_mixed_relative_coordinates_c1(c1_box, oracle_energy):
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_mixed_relative_coordinates(which, c1_box, c2_box, oracle_energy):
     """The mixed pair kernel in relative coordinates (path 0: FP32
     displacements from cluster origins, the FP64 cutoff test only for
     entries with a listed pair near the cutoff) against the same with the
@@ -196,12 +197,14 @@ def test_mixed_relative_coordinates_c1(c1_box, oracle_energy):
     at the FP32 level), step-0 forces agree to 1e-6, and both meet the mixed
     bar against the oracle (RMS relative force error <= 1e-5, nInCut
     exact).  With the pruned list (0.8 A) path 0 is the full list's bit
-    for bit."""
-    wb = c1_box
-    fo, eo = oracle_energy(wb, "c1")
-    a = _mixed_run(wb, 0)
-    b = _mixed_run(wb, 2)
-    c = _mixed_run(wb, 1)
+    for bit (c1).  c2's 124 A box has origin offsets up to 62 A, so its
+    rounding bound is the larger one."""
+    wb = _box(which, c1_box, c2_box)
+    fo, eo = oracle_energy(wb, which)
+    n = 25 if which == "c1" else 12
+    a = _mixed_run(wb, 0, nsteps=n)
+    b = _mixed_run(wb, 2, nsteps=n)
+    c = _mixed_run(wb, 1, nsteps=n)
     say("rel vs exact", a[0].pot, b[0].pot, "rel vs fp64-disp pot", a[0].pot / c[0].pot - 1,
         "forces", relerr(a[1], c[1]), "vs oracle", relerr(a[1], fo), relerr(c[1], fo))
     assert np.array_equal(a[1], b[1])
@@ -218,6 +221,8 @@ def test_mixed_relative_coordinates_c1(c1_box, oracle_energy):
     for r, f in ((a[0], a[1]), (c[0], c[1])):
         assert r.nInCut == eo["nInCut"]
         assert relerr(f, fo) <= 1e-5
+    if which != "c1":
+        return
     p = _mixed_run(wb, 0, skin=0.8, nsteps=45)
     q = _mixed_run(wb, 0, skin=0.0, nsteps=45)
     say("pruned launches", p[4])
