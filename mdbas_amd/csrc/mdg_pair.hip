@@ -326,14 +326,6 @@ __device__ inline float dpp_rs8(float fx, float fy, float fz, int a)
 // Sum over the 8 lanes of each aligned group of 8 with DPP moves (plain
 // VALU ops, no LDS): quad_perm [1,0,3,2], quad_perm [2,3,0,1], then
 // row_half_mirror (lane i <- lane 7-i within the half row).
-__device__ inline float dpp_sum8(float v)
-{
-    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
-    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
-    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));
-    return v;
-}
-
 __device__ inline double dpp_sum8(double v)
 {
     auto mv = [](double x, int ctrl) {
@@ -452,12 +444,13 @@ __global__ void __launch_bounds__(256) k_pair_fp64(ClusterArgs A)
     if (!BATCH) mdg_block_partials<5>(e_acc, A.part);
 }
 
-// Mixed precision over the same list and lane mapping: displacements and
-// the cutoff test in FP64, pair arithmetic in FP32, energies summed in
-// FP64 (each FP32 term, before any rounding of a partial sum: the total
-// is a sum of ~1e7 terms of +-1e4 with heavy cancellation), virials in
-// FP32 over a chunk and FP64 across chunks, forces added as int64 fixed
-// point (deterministic).
+// Mixed precision over the same list and lane mapping: displacements in
+// FP64 from the positions (mx_pair) or in FP32 relative coordinates
+// (mx_pair_rel), the cutoff test the reference's FP64 one either way, pair
+// arithmetic in FP32, energies summed in FP64 (each FP32 term, before any
+// rounding of a partial sum: the total is a sum of ~1e7 terms of +-1e4
+// with heavy cancellation), virials in FP32 over a chunk and FP64 across
+// chunks, forces added as int64 fixed point (deterministic).
 //
 // The loop runs on wave-uniform scalars: chunk, entry, j cluster and mask
 // come through the scalar cache, and the j slots are read with buffer loads
@@ -467,8 +460,8 @@ __global__ void __launch_bounds__(256) k_pair_fp64(ClusterArgs A)
 // pair: the list mask and the cutoff test combine as wave masks (the mask
 // bit of a pair is its lane), and lanes outside either evaluate a harmless
 // distance with zero coefficients.  The j-force sums of a cluster pair go
-// out as one atomic per component: lanes a = 0, 1, 2 of each j group add
-// x, y, z.
+// out as one atomic per component: after the reduce-scatter (dpp_rs8)
+// lanes a = 0, 2, 4 of each j group add x, y, z.
 struct MxJ {
     double x, y, z;
     float4 p;
