@@ -292,30 +292,67 @@ __device__ inline double q_sum3(double v, double t)
     return ((v + t0) + t1) + t2;
 }
 
+// The state after one sweep, formed before the sweep's stopping decision
+// and taken only when the sweep is applied: the multiplier's division and
+// the corrections then overlap the wave-wide vote instead of following it.
+struct QNext {
+    double lambda;
+    double x, y, z, vx, vy, vz;
+};
+
+// Keeps the values computed at this point of the sweep: an empty asm
+// statement that reads them, which the compiler neither moves nor deletes,
+// so it cannot sink their arithmetic past the branch that follows.
+__device__ inline void q_pin(const QNext &N)
+{
+    asm volatile("" ::"v"(N.x), "v"(N.y), "v"(N.z), "v"(N.vx), "v"(N.vy), "v"(N.vz));
+}
+
 // shake.c:262-332: one position sweep (lambda per constraint, corrections
-// summed per atom, applied once per constraint membership); vir collects
-// this lane's constraint's virial terms.
-__device__ inline void q_apply_r(const ShakeArgs &A, Q3 &Q, double dtx, double dty, double dtz,
-                                 double rt2, double &vir)
+// summed per atom, applied once per constraint membership).
+__device__ inline void q_next_r(const ShakeArgs &A, const Q3 &Q, double dtx, double dty,
+                                double dtz, double rt2, QNext &N)
 {
 #pragma clang fp contract(off)
     const double dt = A.P.timeStep;
     const double dot = Q.ddx * dtx + Q.ddy * dty + Q.ddz * dtz;
     double lambda = -(Q.rc2 - rt2) / (Q.K * dot);
     lambda = Q.k < 3 ? lambda : 0.;
-    vir += lambda * Q.ddsq;
+    N.lambda = lambda;
     double xt, yt, zt;
     q_exchange(Q, 0.5 * lambda, xt, yt, zt);
     // the correction once per constraint membership (two per atom); the
     // idle lane has nat = 0 and zero corrections
     const double ix = dt * xt * Q.nat, iy = dt * yt * Q.nat, iz = dt * zt * Q.nat;
-    Q.x = (Q.x + ix) + ix;
-    Q.y = (Q.y + iy) + iy;
-    Q.z = (Q.z + iz) + iz;
+    N.x = (Q.x + ix) + ix;
+    N.y = (Q.y + iy) + iy;
+    N.z = (Q.z + iz) + iz;
     const double jx = xt * Q.nat, jy = yt * Q.nat, jz = zt * Q.nat;
-    Q.vx = (Q.vx + jx) + jx;
-    Q.vy = (Q.vy + jy) + jy;
-    Q.vz = (Q.vz + jz) + jz;
+    N.vx = (Q.vx + jx) + jx;
+    N.vy = (Q.vy + jy) + jy;
+    N.vz = (Q.vz + jz) + jz;
+}
+
+// Applies a position sweep; vir collects this lane's constraint's virial
+// terms.
+__device__ inline void q_take_r(const QNext &N, Q3 &Q, double &vir)
+{
+#pragma clang fp contract(off)
+    vir += N.lambda * Q.ddsq;
+    Q.x = N.x;
+    Q.y = N.y;
+    Q.z = N.z;
+    Q.vx = N.vx;
+    Q.vy = N.vy;
+    Q.vz = N.vz;
+}
+
+__device__ inline void q_apply_r(const ShakeArgs &A, Q3 &Q, double dtx, double dty, double dtz,
+                                 double rt2, double &vir)
+{
+    QNext N;
+    q_next_r(A, Q, dtx, dty, dtz, rt2, N);
+    q_take_r(N, Q, vir);
 }
 
 // shake.c:378-413: one velocity sweep's lambda; converged when every
@@ -332,14 +369,28 @@ __device__ inline bool q_lambda_v(const Q3 &Q, double tolvel, double &xt, double
     return quad_all(fabs(lambda) < tolvel);
 }
 
-// shake.c:418-436: one sweep's corrections, once per constraint membership.
-__device__ inline void q_apply_v(Q3 &Q, double xt, double yt, double zt)
+// shake.c:418-436: one sweep's corrections, once per constraint membership
+// (positions are not moved in this stage: N keeps them).
+__device__ inline void q_next_v(const Q3 &Q, double xt, double yt, double zt, QNext &N)
 {
 #pragma clang fp contract(off)
     const double jx = xt * Q.nat, jy = yt * Q.nat, jz = zt * Q.nat;
-    Q.vx = (Q.vx + jx) + jx;
-    Q.vy = (Q.vy + jy) + jy;
-    Q.vz = (Q.vz + jz) + jz;
+    N.lambda = 0.;
+    N.x = Q.x;
+    N.y = Q.y;
+    N.z = Q.z;
+    N.vx = (Q.vx + jx) + jx;
+    N.vy = (Q.vy + jy) + jy;
+    N.vz = (Q.vz + jz) + jz;
+}
+
+__device__ inline void q_apply_v(Q3 &Q, double xt, double yt, double zt)
+{
+    QNext N;
+    q_next_v(Q, xt, yt, zt, N);
+    Q.vx = N.vx;
+    Q.vy = N.vy;
+    Q.vz = N.vz;
 }
 
 __device__ inline void q_kick(const ShakeArgs &A, Q3 &Q)
@@ -444,6 +495,9 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
         for (;;) {
             double dtx, dty, dtz, rt2;
             const bool cv = q_check(B, Q, dtx, dty, dtz, rt2);
+            QNext N;
+            q_next_r(A, Q, dtx, dty, dtz, rt2, N);
+            q_pin(N);
             if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) {
@@ -464,7 +518,7 @@ __global__ void __launch_bounds__(256) k_w3_vv1_a(ShakeArgs A, SideWork S)
             }
             prev = cv;
             if (it + 1 >= A.P.maxCycle) break;
-            q_apply_r(A, Q, dtx, dty, dtz, rt2, vir);
+            q_take_r(N, Q, vir);
             {
                 // inside the loop every live lane of the wavefront is at the
                 // same sweep, so the word is uniform
@@ -634,6 +688,9 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fu
             // with the ring of wavefront-converged states, as in stage 1.
             double xt, yt, zt;
             const bool cv = q_lambda_v(Q, tolvel, xt, yt, zt);
+            QNext N;
+            q_next_v(Q, xt, yt, zt, N);
+            q_pin(N);
             if (cv) bits.cur |= 1ull << (it & 63);
             if (cv && first < 0) first = it;
             if (first >= 0 && it - first < SHK_W) q_scr_store(A, Q, 1 + it - first, false);
@@ -650,7 +707,9 @@ __global__ void __launch_bounds__(256) k_w3_vv2_a(ShakeArgs A, FinArgs F, int fu
             }
             prev = cv;
             if (it + 1 >= A.P.maxCycle) break;
-            q_apply_v(Q, xt, yt, zt);
+            Q.vx = N.vx;
+            Q.vy = N.vy;
+            Q.vz = N.vz;
             {
                 // inside the loop every live lane of the wavefront is at the
                 // same sweep, so the word is uniform
