@@ -4,8 +4,8 @@
 // consth).  Same arithmetic, operation order and global stopping rule as
 // the generic kernels of mdg_integrate.hip.
 //
-// One molecule per quad of lanes: lane k < 3 holds atom k's state and
-// constraint k's data, lane 3 idles.  Molecules sit on every spread-th
+// One molecule per quad of lanes: lane k < 3 holds atom k's state and the
+// data of one constraint that contains atom k, lane 3 idles.  Molecules sit on every spread-th
 // quad, so a smaller spread packs more molecules per wavefront and a larger
 // one gives the SIMDs more wavefronts to interleave (the sweeps are
 // latency-bound chains); the autotuner picks it.  A sweep evaluates the three
@@ -19,8 +19,11 @@
 namespace {
 
 // quad_perm selectors: lane j of the quad reads lane sel_j
-constexpr int QP_A = 0xD0;     // [0,0,1,3]: atom a of constraint k
-constexpr int QP_B = 0xE9;     // [1,2,2,3]: atom b of constraint k
+constexpr int QP_R = 0xC9;     // [1,2,0,3]: the other atom of the lane's constraint
+constexpr int QP_A = 0xC4;     // [0,1,0,3]: atom a of the lane's constraint
+constexpr int QP_B = 0xE9;     // [1,2,2,3]: atom b of the lane's constraint
+constexpr int QP_CA = 0xE0;    // [0,0,2,3]: lane of atom k's first constraint
+constexpr int QP_CB = 0xD6;    // [2,1,1,3]: lane of atom k's second constraint
 constexpr int QP_L0 = 0xC0;    // [0,0,0,3]
 constexpr int QP_L1 = 0xD5;    // [1,1,1,3]
 constexpr int QP_L2 = 0xEA;    // [2,2,2,3]
@@ -50,22 +53,30 @@ __device__ inline bool quad_all(bool c)
     return v != 0;
 }
 
-// One lane's share of a molecule.  Lane k < 3 holds atom k and constraint
-// k = (a_k, b_k) = (0,1), (0,2), (1,2); lane 3 idles with zero data.
+// One lane's share of a molecule.  Lane k < 3 holds atom k and one
+// constraint that contains atom k: lane 0 constraint 0 = (0,1), lane 1
+// constraint 2 = (1,2), lane 2 constraint 1 = (0,2); lane 3 idles with zero
+// data.  A lane forms its constraint's vector as (the other atom) - (its
+// own atom) with one DPP read per component: x_b - x_a on lanes 0 and 1,
+// and -(x_b - x_a), exactly, on lane 2.  That sign cancels wherever the
+// vector enters: |d|^2, the image shift (nint is odd), the multiplier
+// (dd . dt with both vectors negated), and the corrections, whose factors
+// for constraint 1 are negated below, so every product is the reference's.
 //
 // Correction exchange: atom 0 receives the a-side terms of constraints 0
 // and 1, atom 1 the b-side of 0 and the a-side of 2, atom 2 the b-side of 1
-// and 2: in constraint order, its first constraint is cA = (0, 0, 1)[k] and
-// its second cB = (1, 2, 2)[k], the lanes that quad_perm QP_A and QP_B read.
+// and 2: in constraint order, its first constraint cA sits on lane
+// (0, 0, 2)[k] and its second cB on lane (2, 1, 1)[k] (QP_CA, QP_CB).
 // Each sweep only the constraints' multipliers move (two DPP reads); the
 // atom holds the vectors dd of cA and cB and its factors fA, fB (trmi, or
-// -trmj on the b side) from the stage's start, forms li = h fA and lj = h
-// fB as the constraint's lane would, and sums (0 + dd_cA l_A) + dd_cB l_B
-// as the reference accumulates xt[] (shake.c:300-310).
+// -trmj on the b side; negated for constraint 1) from the stage's start,
+// forms li = h fA and lj = h fB as the constraint's lane would, and sums
+// (0 + dd_cA l_A) + dd_cB l_B as the reference accumulates xt[]
+// (shake.c:300-310).
 struct Q3 {
     int k;              // 0..2 live, 3 idle
     int at;             // atom k
-    int cid;            // constraint k
+    int cid;            // the lane's constraint (0, 2, 1)[k]
     double x, y, z, vx, vy, vz, rm, nat;
     double rc2, thr;    // stage 1: converged while |r^2 - rc2| < thr
     double ddx, ddy, ddz, ddsq;   // constraint vector at the stage start, |dd|^2
@@ -73,7 +84,7 @@ struct Q3 {
     double K;           // stage 1: dt (trmi + trmj); stage 2: (trmi + trmj) |dd|^2
     double fA, fB;      // the atom's factors in its constraints cA, cB
     double dAx, dAy, dAz, dBx, dBy, dBz;   // dd of cA and cB
-    double rma, rmb;    // inverse masses of constraint k's atoms
+    double rma, rmb;    // inverse masses of the lane constraint's atoms a, b
 };
 
 __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
@@ -101,7 +112,7 @@ __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
         Q.vz = A.vz[i];
         Q.rm = A.rmass[i];
         Q.nat = 1.0 / (double)A.nAtConst[i];
-        Q.cid = A.cons[A.conOff[g] + k];
+        Q.cid = A.cons[A.conOff[g] + (k == 0 ? 0 : 3 - k)];
         Q.rc2 = A.rc2[Q.cid];
         Q.thr = A.thr[Q.cid];
     }
@@ -115,16 +126,18 @@ __device__ inline void q_load(const ShakeArgs &A, int g, int k, Q3 &Q)
 // 1 the b side of 0 and the a side of 2, atom 2 the b side of 1 and 2.
 __device__ inline void q_factors(Q3 &Q, double ti, double tj)
 {
-    const double aI = qperm<QP_A>(ti), aJ = qperm<QP_A>(tj);
-    const double bI = qperm<QP_B>(ti), bJ = qperm<QP_B>(tj);
-    Q.fA = Q.k == 0 ? aI : -aJ;
-    Q.fB = Q.k < 2 ? bI : -bJ;
-    Q.dAx = qperm<QP_A>(Q.ddx);
-    Q.dAy = qperm<QP_A>(Q.ddy);
-    Q.dAz = qperm<QP_A>(Q.ddz);
-    Q.dBx = qperm<QP_B>(Q.ddx);
-    Q.dBy = qperm<QP_B>(Q.ddy);
-    Q.dBz = qperm<QP_B>(Q.ddz);
+    const double aI = qperm<QP_CA>(ti), aJ = qperm<QP_CA>(tj);
+    const double bI = qperm<QP_CB>(ti), bJ = qperm<QP_CB>(tj);
+    // sides as above; constraint 1 (atom 0's cB, atom 2's cA) is held
+    // negated, so its factors are too
+    Q.fA = Q.k == 0 ? aI : Q.k == 1 ? -aJ : aJ;
+    Q.fB = Q.k == 0 ? -bI : Q.k == 1 ? bI : -bJ;
+    Q.dAx = qperm<QP_CA>(Q.ddx);
+    Q.dAy = qperm<QP_CA>(Q.ddy);
+    Q.dAz = qperm<QP_CA>(Q.ddz);
+    Q.dBx = qperm<QP_CB>(Q.ddx);
+    Q.dBy = qperm<QP_CB>(Q.ddy);
+    Q.dBz = qperm<QP_CB>(Q.ddz);
     if (Q.k >= 3) Q.fA = Q.fB = 0.;
 }
 
@@ -195,12 +208,13 @@ __device__ inline void q_scr_load(const ShakeArgs &A, Q3 &Q, int slot, bool pos)
     Q.vz = s[5 * n];
 }
 
-// x_b - x_a of constraint k (every lane takes part in the moves).
+// The lane constraint's vector, x_b - x_a (lane 2: its negation); every
+// lane takes part in the move.
 __device__ inline void q_bond(const Q3 &Q, double &dx, double &dy, double &dz)
 {
-    dx = qperm<QP_B>(Q.x) - qperm<QP_A>(Q.x);
-    dy = qperm<QP_B>(Q.y) - qperm<QP_A>(Q.y);
-    dz = qperm<QP_B>(Q.z) - qperm<QP_A>(Q.z);
+    dx = qperm<QP_R>(Q.x) - Q.x;
+    dy = qperm<QP_R>(Q.y) - Q.y;
+    dz = qperm<QP_R>(Q.z) - Q.z;
 }
 
 // The image shift utils.c:38-80 subtracts from a constraint vector, as a
@@ -278,7 +292,7 @@ __device__ inline bool q_check(const MdgBox &B, const Q3 &Q, double &dtx, double
 __device__ inline void q_exchange(const Q3 &Q, double h, double &xt, double &yt, double &zt)
 {
 #pragma clang fp contract(off)
-    const double lA = qperm<QP_A>(h) * Q.fA, lB = qperm<QP_B>(h) * Q.fB;
+    const double lA = qperm<QP_CA>(h) * Q.fA, lB = qperm<QP_CB>(h) * Q.fB;
     xt = (0. + Q.dAx * lA) + Q.dBx * lB;
     yt = (0. + Q.dAy * lA) + Q.dBy * lB;
     zt = (0. + Q.dAz * lA) + Q.dBz * lB;
@@ -289,6 +303,14 @@ __device__ inline double q_sum3(double v, double t)
 {
 #pragma clang fp contract(off)
     const double t0 = qperm<QP_L0>(t), t1 = qperm<QP_L1>(t), t2 = qperm<QP_L2>(t);
+    return ((v + t0) + t1) + t2;
+}
+
+// The same over the constraints in their order 0, 1, 2 (lanes 0, 2, 1).
+__device__ inline double q_sum3c(double v, double t)
+{
+#pragma clang fp contract(off)
+    const double t0 = qperm<QP_L0>(t), t1 = qperm<QP_L2>(t), t2 = qperm<QP_L1>(t);
     return ((v + t0) + t1) + t2;
 }
 
@@ -360,9 +382,9 @@ __device__ inline void q_apply_r(const ShakeArgs &A, Q3 &Q, double dtx, double d
 __device__ inline bool q_lambda_v(const Q3 &Q, double tolvel, double &xt, double &yt, double &zt)
 {
 #pragma clang fp contract(off)
-    const double dvx = qperm<QP_B>(Q.vx) - qperm<QP_A>(Q.vx);
-    const double dvy = qperm<QP_B>(Q.vy) - qperm<QP_A>(Q.vy);
-    const double dvz = qperm<QP_B>(Q.vz) - qperm<QP_A>(Q.vz);
+    const double dvx = qperm<QP_R>(Q.vx) - Q.vx;
+    const double dvy = qperm<QP_R>(Q.vy) - Q.vy;
+    const double dvz = qperm<QP_R>(Q.vz) - Q.vz;
     double lambda = (Q.ddx * dvx + Q.ddy * dvy + Q.ddz * dvz) / Q.K;
     lambda = Q.k < 3 ? lambda : 0.;
     q_exchange(Q, lambda, xt, yt, zt);
@@ -583,7 +605,7 @@ __global__ void __launch_bounds__(256) k_w3_vv1_b(ShakeArgs A)
                 if (!q_check(B, Q, dtx, dty, dtz, rt2) && k == 0)
                     atomicOr(&A.ctrl[CTRL_SHAKE_ORDER], 1);
             }
-            const double vm = q_sum3(0., vir);
+            const double vm = q_sum3c(0., vir);
             if (BATCH) {
                 if (k == 0) A.vir[g] = vm;   // per molecule, summed per replica
             } else if (k == 0) {
